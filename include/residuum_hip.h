@@ -1,0 +1,224 @@
+/*
+ * residuum_hip.h -- C-ABI of libresiduum_hip.so, the MI355X (gfx950) backend
+ * for Residuum's heat-equation hot path.
+ *
+ * Every entry point returns an int status: RSD_OK (0) or a negative error
+ * code; rsd_last_error_string() describes the last failure on the calling
+ * thread.  Sizes are int64_t; device pointers are plain pointers obtained
+ * from rsd_malloc (or any hipMalloc'd memory on the context's device).  All
+ * work is ordered on the context's HIP stream; functions that return host
+ * values synchronise that stream.  A context is not re-entrant: use one per
+ * host thread.  Results are deterministic for identical inputs.
+ *
+ * Each declaration names the reference interface it replaces
+ * (elicase4/residuum-solver, path:line).  The reference has no GPU code and no
+ * FFI; these functions are what its Backend-tag seam would bind
+ * (linalg/types/backend/CPU.hpp:15-40 and the CUDA tag scaffold in
+ * linalg/types/backend/CUDA.hpp:16-73).
+ */
+#ifndef RESIDUUM_HIP_H
+#define RESIDUUM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* status codes                                                            */
+/* ---------------------------------------------------------------------- */
+#define RSD_OK 0
+#define RSD_ERR_ARG (-1)        /* invalid argument or unsupported discretisation */
+#define RSD_ERR_HIP (-2)        /* HIP runtime error */
+#define RSD_ERR_INTERNAL (-3)
+#define RSD_ERR_NOT_CONVERGED (-4)
+
+/* copy kinds: linalg/types/backend/cuda/CopyKind.hpp:9-14 */
+#define RSD_COPY_H2H 0
+#define RSD_COPY_H2D 1
+#define RSD_COPY_D2H 2
+#define RSD_COPY_D2D 3
+
+/* form kinds (fem/form/FormRegistry.hpp:202-216); OR RSD_FORM_SCALED to wrap
+ * a form in fem/form/ScaledForm.hpp:14-57 with the matching coefficient */
+#define RSD_FORM_DIFFUSION 1        /* equation/heateq/form/DiffusionForm.hpp:11-52 */
+#define RSD_FORM_MASS 2             /* equation/heateq/form/MassForm.hpp:11-27 */
+#define RSD_FORM_TDIFFUSION 3       /* equation/heateq/form/TangentDiffusionForm.hpp:11-56 */
+#define RSD_FORM_TMASS 4            /* equation/heateq/form/TangentMassForm.hpp:11-27 */
+#define RSD_FORM_SCALED 0x100
+
+/* state buffer selectors: equation/heateq/evaluator/HeatStateVector.hpp:28-30 */
+#define RSD_SEL_VALUE 0
+#define RSD_SEL_RATE 1
+#define RSD_SEL_OPERAND 2
+
+/* preconditioners: linalg/solver/preconditioner/{Identity,Jacobi}.hpp */
+#define RSD_PC_IDENTITY 0
+#define RSD_PC_JACOBI 1
+
+typedef struct rsd_ctx rsd_ctx;
+typedef struct rsd_problem rsd_problem;
+typedef struct rsd_matrix rsd_matrix;
+
+/* ---------------------------------------------------------------------- */
+/* context                                                                 */
+/* ---------------------------------------------------------------------- */
+/* one HIP stream + reduction scratch on `device` */
+int rsd_ctx_create(int device, rsd_ctx** out);
+int rsd_ctx_destroy(rsd_ctx* ctx);
+int rsd_ctx_sync(rsd_ctx* ctx);
+/* the context's stream, as a hipStream_t, for callers that overlap work */
+int rsd_ctx_stream(rsd_ctx* ctx, void** stream);
+const char* rsd_last_error_string(void);
+const char* rsd_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Backend-tag primitives: CPU.hpp:15-40 alloc/copy/zero/set               */
+/* (CUDA.hpp:30-64 cudaMalloc / cudaMemcpy / cudaMemset / cudaFree)        */
+/* ---------------------------------------------------------------------- */
+int rsd_malloc(rsd_ctx* ctx, size_t bytes, void** out);
+int rsd_free(rsd_ctx* ctx, void* p);
+int rsd_memcpy(rsd_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
+int rsd_memset_zero(rsd_ctx* ctx, void* p, size_t bytes);
+/* Vector::set (linalg/types/Vector.hpp:37-39): a fill kernel, which memset
+ * cannot do for doubles (CUDA.hpp:66-71 leaves it commented out) */
+int rsd_fill_f64(rsd_ctx* ctx, double* p, int64_t n, double value);
+int rsd_fill_f32(rsd_ctx* ctx, float* p, int64_t n, float value);
+
+/* ---------------------------------------------------------------------- */
+/* BLAS-1: linalg/operations/backend/cpu/VectorOps.tpp:7-86                */
+/* dot/norm write the scalar to d_out (device, nullable) and h_out (host,  */
+/* nullable; synchronises).  Reductions use the fixed block tree described */
+/* in DESIGN.md, so results are bitwise reproducible.                      */
+/* ---------------------------------------------------------------------- */
+int rsd_dot_f64(rsd_ctx* ctx, int64_t n, const double* x, const double* y, double* d_out, double* h_out);
+int rsd_norm2_f64(rsd_ctx* ctx, int64_t n, const double* x, double* d_out, double* h_out);
+int rsd_axpy_f64(rsd_ctx* ctx, int64_t n, double alpha, const double* x, double* y);           /* y += a x */
+int rsd_axpby_f64(rsd_ctx* ctx, int64_t n, double alpha, double beta, const double* x, double* y); /* y = b y + a x */
+int rsd_scal_f64(rsd_ctx* ctx, int64_t n, double alpha, double* x);
+int rsd_copy_f64(rsd_ctx* ctx, int64_t n, const double* x, double* y);
+int rsd_mul_f64(rsd_ctx* ctx, int64_t n, const double* x, const double* y, double* z);        /* z = x .* y */
+
+/* ---------------------------------------------------------------------- */
+/* problem: mesh + topological DOFs + CSR pattern + element->nnz map +     */
+/* colouring + material model, resident in HBM                             */
+/* ---------------------------------------------------------------------- */
+/* BlockMesh2D/3D::generate (src/mesh/generator/BlockMesh{2D,3D}.cpp),
+ * TopologicalDOF::buildConstraints (topology/TopologicalDOF.tpp:111-190),
+ * Assembler<CPU>::createMatrix (fem/assembly/backend/cpu/Assembler.tpp:7-76)
+ * and the implicit element->nnz map of scatterElementMatrix
+ * (ElementMap.tpp:115-184).  dim 2 (quad) or 3 (hex); n[3], p[3] element
+ * counts and basis orders; ext[6] = x0 x1 y0 y1 z0 z1; nq Gauss points per
+ * direction; Dirichlet tags with value expressions g(t, x, y, z). */
+int rsd_problem_create_block(rsd_ctx* ctx, int32_t dim, const int64_t* n, const double* ext,
+                             const int32_t* p, int32_t nq, int32_t n_bc, const int32_t* bc_tags,
+                             const char* const* bc_exprs, rsd_problem** out);
+/* same from host mesh arrays (mesh/Data.hpp:13-38 layout: xyz [nn*dim],
+ * ien [ne*npe], rng [ne*2*dim]) */
+int rsd_problem_create(rsd_ctx* ctx, int32_t dim, const int32_t* p, int32_t nq, int64_t n_nodes,
+                       int64_t n_elem, const double* xyz, const int64_t* ien, const int32_t* rng,
+                       int32_t n_bc, const int32_t* bc_tags, const char* const* bc_exprs, rsd_problem** out);
+int rsd_problem_destroy(rsd_problem* pb);
+/* out[8] = n_nodes, n_elem, npe, n_free, nnz, n_colours, sell_slots, max_row */
+int rsd_problem_sizes(rsd_problem* pb, int64_t* out);
+int rsd_problem_get_mesh(rsd_problem* pb, double* xyz, int64_t* ien, int32_t* rng);
+/* topo->alg map (-1 constrained), alg->topo, constraint tag per node */
+int rsd_problem_get_dofs(rsd_problem* pb, int32_t* topo_to_alg, int32_t* alg_to_topo, int32_t* ctag);
+/* CSR pattern as the reference's size_t arrays, widened to int64 */
+int rsd_problem_get_pattern(rsd_problem* pb, int64_t* row_ptr, int64_t* col_idx);
+/* element -> nnz map in CSR positions (-1 constrained), [ne*npe*npe] */
+int rsd_problem_get_elem_map(rsd_problem* pb, int64_t* map);
+/* element colour and colour-major assembly order */
+int rsd_problem_get_colours(rsd_problem* pb, int32_t* colour, int64_t* order);
+
+/* conductivity K = kappa_e * k(T) * shape (ConductivityModel.hpp:60-96; the
+ * per-element factor kappa_e has no reference counterpart).  k_expr / dk_expr
+ * nullable (constant k_scalar, k' = 0); tensor nullable (isotropic), else
+ * dim*dim row-major; kappa nullable, else host [n_elem]. */
+int rsd_problem_set_conductivity(rsd_problem* pb, double k_scalar, const char* k_expr,
+                                 const char* dk_expr, const double* tensor, const double* kappa);
+/* DensityModel (DensityModel.hpp:13) and constant SpecificHeatModel */
+int rsd_problem_set_mass(rsd_problem* pb, double rho, double cp);
+/* per-element kappa from device memory [n_elem] (adjoint sweeps) */
+int rsd_problem_set_kappa_device(rsd_problem* pb, const double* d_kappa);
+
+/* ---------------------------------------------------------------------- */
+/* matrices on the problem's pattern: CSRMatrix<Real,Hip>                  */
+/* (linalg/types/CSRMatrix.hpp:14-88).  Values live in HBM in a sliced     */
+/* ELL layout (64-row slices, one wavefront each) for coalesced SpMV.      */
+/* ---------------------------------------------------------------------- */
+int rsd_matrix_create(rsd_problem* pb, rsd_matrix** out);
+int rsd_matrix_destroy(rsd_matrix* m);
+int rsd_matrix_zero(rsd_ctx* ctx, rsd_matrix* m);                        /* CSRMatrix::zero */
+/* values in CSR order, host [nnz] */
+int rsd_matrix_get_values(rsd_ctx* ctx, rsd_matrix* m, double* csr_values);
+int rsd_matrix_set_values(rsd_ctx* ctx, rsd_matrix* m, const double* csr_values);
+
+/* y = A x: operations::matvec (linalg/operations/backend/cpu/MatrixOps.tpp:5-23),
+ * row sums in the reference's left-to-right order (bitwise equal) */
+int rsd_spmv_f64(rsd_ctx* ctx, rsd_matrix* A, const double* x, double* y);
+/* Jacobi::update (linalg/solver/preconditioner/Jacobi.hpp:23-36) via
+ * CSROperator::diagonal (CSROperator.hpp:40-45): 1/d, or 1 if |d| <= 1e-300 */
+int rsd_jacobi_inv_diag(rsd_ctx* ctx, rsd_matrix* A, double* inv_diag);
+
+/* ---------------------------------------------------------------------- */
+/* solvers                                                                  */
+/* ---------------------------------------------------------------------- */
+/* SolverReport (linalg/solver/base/SolverReport.hpp:12-28) */
+typedef struct rsd_report {
+    int32_t converged;
+    int64_t iterations;
+    double initial_residual;
+    double final_residual;
+    double final_residual_rel;
+} rsd_report;
+
+/* cg::Solver::solve (linalg/solver/iterative/cg/Solver.tpp:15-119): PCG with
+ * the relative criterion ||r_k|| / (||r_0|| + 1e-50) < tol, warm start from
+ * x.  Fused kernels, scalars kept on the device.  hist (host, nullable,
+ * >= max_iters entries) receives ||r_k||. */
+int rsd_cg_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol,
+                     int64_t max_iters, int32_t precond, rsd_report* rep, double* hist);
+
+/* ---------------------------------------------------------------------- */
+/* assembly (fem/assembly/backend/cpu/Assembler.tpp:86-255,                */
+/* fem/boundary/backend/cpu/BoundaryApplicator.tpp:7-64).  Element kernels  */
+/* run colour by colour: elements of one colour share no node, so the       */
+/* scatter through the element->nnz map needs no atomics and the result is  */
+/* deterministic.  U / Udot / operand are free-DOF device vectors.         */
+/* ---------------------------------------------------------------------- */
+int rsd_assemble_matrix(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t n_forms,
+                        const int32_t* kinds, const double* coefs, const double* U,
+                        const double* Udot, rsd_matrix* K);
+int rsd_assemble_vector(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t n_forms,
+                        const int32_t* kinds, const double* coefs, const double* U,
+                        const double* Udot, const double* operand, int32_t selector, double* F);
+/* SourceForm (SourceForm.hpp:22-32) with expression f(t,x,y,z) (nullable = 0)
+ * or NodalSourceForm (NodalSourceForm.hpp:62-86) with device nodal [n_nodes] */
+int rsd_assemble_load(rsd_ctx* ctx, rsd_problem* pb, double time, const char* src_expr,
+                      const double* nodal, double* F);
+/* essential lift F -= A_fc g (BoundaryApplicator.tpp:7-64) */
+int rsd_apply_essential(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t n_forms,
+                        const int32_t* kinds, const double* coefs, double* F);
+
+/* ---------------------------------------------------------------------- */
+/* drivers                                                                  */
+/* ---------------------------------------------------------------------- */
+/* Linear Backward Euler on the problem: BackwardEulerRunner::step
+ * (solver/timestepper/BackwardEuler.hpp:32-72) over BackwardEulerStage
+ * (solver/stage/BackwardEulerStage.hpp:71-163) and Transient::solve
+ * (solver/driver/Transient.hpp:21-35), CSR operator, CG + preconditioner.
+ * d_U holds the initial free vector and receives the final one.  src_expr
+ * nullable.  iters (host, nullable, >= max_steps) receives CG iterations per
+ * step.  use_graph != 0 replays each step's CG chunks from a hipGraph.
+ * Returns the number of steps in *steps. */
+int rsd_be_run(rsd_ctx* ctx, rsd_problem* pb, double t0, double tf, double dt,
+               const char* src_expr, double cg_tol, int64_t cg_max_iters, int32_t precond,
+               int32_t use_graph, double* d_U, int64_t max_steps, int64_t* steps, int64_t* iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,574 @@
+// rsd_assembly.hip -- element kernels for Q1/Q2 quads and Q1 hexes.
+//
+// One thread per element, launched colour by colour: elements of one colour
+// share no node, so the scatter through the element->slot map is a plain
+// read-modify-write and the summation order per matrix / vector entry is
+// the colour order.  The element math follows the reference operand by
+// operand (EvalQuadraturePointVolume.hpp:64-85, JacobianTransform.tpp:3-206,
+// Lagrange1D.tpp:4-51, the heat-equation forms) with every product and sum
+// rounded separately, so a CPU loop visiting the elements in the same
+// colour order produces bitwise identical results.
+#include "rsd_assembly.hpp"
+#include "rsd_linalg.hpp"
+
+#include <algorithm>
+
+namespace rsd {
+
+template<int P>
+__device__ __forceinline__ void lag1d(double xi, double* N, double* dN) {
+    if constexpr (P == 1) {
+        N[0] = 0.5 * (1.0 - xi); N[1] = 0.5 * (1.0 + xi);
+        dN[0] = -0.5; dN[1] = 0.5;
+    } else {
+        N[0] = 0.5 * xi * (xi - 1.0); N[1] = (1.0 - xi) * (1.0 + xi); N[2] = 0.5 * xi * (xi + 1.0);
+        dN[0] = xi - 0.5; dN[1] = -2.0 * xi; dN[2] = xi + 0.5;
+    }
+}
+
+// Gauss-Legendre nodes/weights (GaussQuadrature1D.tpp:4-71)
+template<int NQ>
+__device__ __forceinline__ void gauss(int i, double& x, double& w) {
+    if constexpr (NQ == 1) { x = 0.0; w = 2.0; }
+    else if constexpr (NQ == 2) { x = i == 0 ? -0.5773502691896257 : 0.5773502691896257; w = 1.0; }
+    else if constexpr (NQ == 3) {
+        x = i == 0 ? -0.7745966692414834 : (i == 1 ? 0.0 : 0.7745966692414834);
+        w = i == 1 ? 0.8888888888888889 : 0.5555555555555556;
+    } else {
+        const double X[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+        const double W[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
+        x = X[i]; w = W[i];
+    }
+}
+
+template<int D, int P>
+struct Elem {
+    static constexpr int NPE = D == 2 ? (P + 1) * (P + 1) : (P + 1) * (P + 1) * (P + 1);
+};
+
+// quadrature-point data (EvalQuadraturePointVolume members)
+template<int D, int NPE>
+struct QP {
+    double N[NPE], dNdx[NPE * D], x[D], measure, w;
+    double T, Trate, gradT[D], K[D * D], dKdT[D * D], rho, cp, dcpdT;
+};
+
+template<int D, int P, int NQ>
+__device__ __forceinline__ void qp_eval(int q, const double (&X)[Elem<D, P>::NPE][D], QP<D, Elem<D, P>::NPE>& Q,
+                                        bool need_x) {
+    constexpr int NPE = Elem<D, P>::NPE;
+    const int qi = q % NQ, qj = (q / NQ) % NQ, qk = q / (NQ * NQ);
+    double xi[3], wq[3];
+    gauss<NQ>(qi, xi[0], wq[0]);
+    gauss<NQ>(qj, xi[1], wq[1]);
+    if constexpr (D == 3) gauss<NQ>(qk, xi[2], wq[2]);
+    Q.w = D == 2 ? wq[0] * wq[1] : wq[0] * wq[1] * wq[2];
+    double Nx[D][P + 1], dNx[D][P + 1];
+#pragma unroll
+    for (int k = 0; k < D; ++k) lag1d<P>(xi[k], Nx[k], dNx[k]);
+    double dNdxi[NPE * D];
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) {
+        const int i = a % (P + 1), j = (a / (P + 1)) % (P + 1), k = a / ((P + 1) * (P + 1));
+        if constexpr (D == 2) {
+            Q.N[a] = Nx[0][i] * Nx[1][j];
+            dNdxi[a * 2] = dNx[0][i] * Nx[1][j];
+            dNdxi[a * 2 + 1] = Nx[0][i] * dNx[1][j];
+        } else {
+            Q.N[a] = Nx[0][i] * Nx[1][j] * Nx[2][k];
+            dNdxi[a * 3] = dNx[0][i] * Nx[1][j] * Nx[2][k];
+            dNdxi[a * 3 + 1] = Nx[0][i] * dNx[1][j] * Nx[2][k];
+            dNdxi[a * 3 + 2] = Nx[0][i] * Nx[1][j] * dNx[2][k];
+        }
+    }
+    if (need_x) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) Q.x[i] = 0.0;
+#pragma unroll
+        for (int b = 0; b < NPE; ++b)
+#pragma unroll
+            for (int i = 0; i < D; ++i) Q.x[i] += Q.N[b] * X[b][i];
+    }
+    double J[D * D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int al = 0; al < D; ++al) {
+            double s = 0.0;
+#pragma unroll
+            for (int b = 0; b < NPE; ++b) s += dNdxi[b * D + al] * X[b][i];
+            J[i * D + al] = s;
+        }
+    double g[D * D];
+#pragma unroll
+    for (int be = 0; be < D; ++be)
+#pragma unroll
+        for (int al = 0; al < D; ++al) {
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) s += J[i * D + al] * J[i * D + be];
+            g[be * D + al] = s;
+            g[al * D + be] = s;
+        }
+    double inv[D * D];
+    if constexpr (D == 2) {
+        Q.measure = sqrt((g[0] * g[3]) - (g[1] * g[2]));
+        const double di = 1.0 / ((J[0] * J[3]) - (J[1] * J[2]));
+        inv[0] = di * J[3]; inv[1] = -di * J[1]; inv[2] = -di * J[2]; inv[3] = di * J[0];
+    } else {
+        const double dg = (g[0] * ((g[4] * g[8]) - (g[5] * g[7]))) - (g[1] * ((g[3] * g[8]) - (g[5] * g[6]))) +
+                          (g[2] * ((g[3] * g[7]) - (g[4] * g[6])));
+        Q.measure = sqrt(dg);
+        const double* A = J;
+        const double dj = (A[0] * ((A[4] * A[8]) - (A[5] * A[7]))) - (A[1] * ((A[3] * A[8]) - (A[5] * A[6]))) +
+                          (A[2] * ((A[3] * A[7]) - (A[4] * A[6])));
+        const double di = 1.0 / dj;
+        inv[0] = di * ((A[4] * A[8]) - (A[5] * A[7]));
+        inv[1] = -di * ((A[3] * A[8]) - (A[5] * A[6]));
+        inv[2] = di * ((A[3] * A[7]) - (A[4] * A[6]));
+        inv[3] = -di * ((A[1] * A[8]) - (A[2] * A[7]));
+        inv[4] = di * ((A[0] * A[8]) - (A[2] * A[6]));
+        inv[5] = -di * ((A[0] * A[7]) - (A[1] * A[6]));
+        inv[6] = di * ((A[1] * A[5]) - (A[2] * A[4]));
+        inv[7] = -di * ((A[0] * A[5]) - (A[2] * A[3]));
+        inv[8] = di * ((A[0] * A[4]) - (A[1] * A[3]));
+    }
+#pragma unroll
+    for (int b = 0; b < NPE; ++b)
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int al = 0; al < D; ++al) s += inv[i * D + al] * dNdxi[b * D + al];
+            Q.dNdx[b * D + i] = s;
+        }
+}
+
+template<int D, int NPE>
+__device__ __forceinline__ void qp_state(QP<D, NPE>& Q, const double* Uv, const double* Ur) {
+    double u = 0.0;
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) u += Q.N[a] * Uv[a];
+    Q.T = u;
+#pragma unroll
+    for (int i = 0; i < D; ++i) Q.gradT[i] = 0.0;
+#pragma unroll
+    for (int a = 0; a < NPE; ++a)
+#pragma unroll
+        for (int i = 0; i < D; ++i) Q.gradT[i] += Q.dNdx[a * D + i] * Uv[a];
+    double r = 0.0;
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) r += Q.N[a] * Ur[a];
+    Q.Trate = r;
+}
+
+template<int D, int NPE>
+__device__ __forceinline__ void qp_model(QP<D, NPE>& Q, const Model& m, double kap, bool has_kap) {
+    double k = m.has_k ? eval_expr(m.k, 0.0, 0.0, 0.0, 0.0, Q.T) : m.k_scalar;
+    double dk = m.has_dk ? eval_expr(m.dk, 0.0, 0.0, 0.0, 0.0, Q.T) : 0.0;
+    if (has_kap) { k = kap * k; dk = kap * dk; }
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            const double shape = m.aniso ? m.tensor[i * D + j] : ((i == j) ? 1.0 : 0.0);
+            Q.K[i * D + j] = k * shape;
+            Q.dKdT[i * D + j] = dk * shape;
+        }
+    Q.rho = m.rho;
+    Q.cp = m.cp;
+    Q.dcpdT = 0.0;
+}
+
+// one form's contribution at (a, b) for the matrix forms
+template<int D, int NPE>
+__device__ __forceinline__ double form_entry(int kind, const QP<D, NPE>& Q, int a, int b) {
+    if (kind == RSD_FORM_DIFFUSION || kind == RSD_FORM_TDIFFUSION) {
+        double integrand = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            double mv = 0.0;
+            if (kind == RSD_FORM_DIFFUSION) {
+#pragma unroll
+                for (int j = 0; j < D; ++j) mv += Q.K[i * D + j] * Q.dNdx[b * D + j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < D; ++j) mv += Q.dKdT[i * D + j] * Q.gradT[j];
+            }
+            integrand += Q.dNdx[a * D + i] * mv;
+        }
+        return kind == RSD_FORM_DIFFUSION ? integrand * Q.measure * Q.w : integrand * Q.N[b] * Q.measure * Q.w;
+    }
+    if (kind == RSD_FORM_MASS) return Q.rho * Q.cp * Q.N[a] * Q.N[b] * Q.measure * Q.w;
+    return Q.rho * Q.dcpdT * Q.Trate * Q.N[a] * Q.N[b] * Q.measure * Q.w;
+}
+
+// vector-form contribution at (a, b) with operand Ue[b]
+template<int D, int NPE>
+__device__ __forceinline__ double form_vec_entry(int kind, const QP<D, NPE>& Q, int a, int b, double ub) {
+    if (kind == RSD_FORM_DIFFUSION || kind == RSD_FORM_TDIFFUSION) {
+        double integrand = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            double mv = 0.0;
+            if (kind == RSD_FORM_DIFFUSION) {
+#pragma unroll
+                for (int j = 0; j < D; ++j) mv += Q.K[i * D + j] * Q.dNdx[b * D + j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < D; ++j) mv += Q.dKdT[i * D + j] * Q.gradT[j];
+            }
+            integrand += Q.dNdx[a * D + i] * mv;
+        }
+        return kind == RSD_FORM_DIFFUSION ? integrand * ub * Q.measure * Q.w : integrand * Q.N[b] * ub * Q.measure * Q.w;
+    }
+    if (kind == RSD_FORM_MASS) return Q.rho * Q.cp * Q.N[a] * Q.N[b] * ub * Q.measure * Q.w;
+    return Q.rho * Q.dcpdT * Q.Trate * Q.N[a] * Q.N[b] * ub * Q.measure * Q.w;
+}
+
+template<int D, int P>
+__device__ __forceinline__ void load_element(const AsmArgs& A, int64_t e, int32_t (&nodes)[Elem<D, P>::NPE],
+                                             double (&X)[Elem<D, P>::NPE][D]) {
+    constexpr int NPE = Elem<D, P>::NPE;
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) {
+        nodes[a] = A.ien[e * NPE + a];
+#pragma unroll
+        for (int i = 0; i < D; ++i) X[a][i] = A.xyz[(int64_t)nodes[a] * D + i];
+    }
+}
+
+// gather modes: 0 free, 1 full (free values + Dirichlet values), 2 constrained
+template<int NPE>
+__device__ __forceinline__ void gather(const AsmArgs& A, const int32_t (&nodes)[NPE], int mode, const double* U,
+                                       double (&Ue)[NPE]) {
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) {
+        const int32_t r = A.t2a[nodes[a]];
+        double v = 0.0;
+        if (r >= 0) { if (mode != 2 && U) v = U[r]; }
+        else if (mode != 0) v = A.gval[nodes[a]];
+        Ue[a] = v;
+    }
+}
+
+template<int D, int P, int NQ>
+__global__ __launch_bounds__(kBlock) void k_asm_matrix(AsmArgs A, Model m, int64_t c0, int64_t c1) {
+    constexpr int NPE = Elem<D, P>::NPE;
+    const int64_t k = c0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= c1) return;
+    const int64_t e = A.order[k];
+    int32_t nodes[NPE];
+    double X[NPE][D];
+    load_element<D, P>(A, e, nodes, X);
+    double Uv[NPE], Ur[NPE];
+    if (A.need_state) {
+        gather<NPE>(A, nodes, 1, A.U, Uv);
+        gather<NPE>(A, nodes, 0, A.Udot, Ur);
+    }
+    const bool has_kap = A.kappa != nullptr;
+    const double kap = has_kap ? A.kappa[e] : 1.0;
+    double Ke[NPE * NPE];
+#pragma unroll
+    for (int i = 0; i < NPE * NPE; ++i) Ke[i] = 0.0;
+    for (int q = 0; q < (D == 2 ? NQ * NQ : NQ * NQ * NQ); ++q) {
+        QP<D, NPE> Q;
+        qp_eval<D, P, NQ>(q, X, Q, false);
+        if (A.need_state) qp_state<D, NPE>(Q, Uv, Ur);
+        else { Q.T = 0.0; Q.Trate = 0.0; for (int i = 0; i < D; ++i) Q.gradT[i] = 0.0; }
+        qp_model<D, NPE>(Q, m, kap, has_kap);
+        for (int f = 0; f < A.nf; ++f) {
+            const int kind = A.kinds[f] & 0xff;
+            const bool scaled = (A.kinds[f] & RSD_FORM_SCALED) != 0;
+            const double coef = A.coefs[f];
+#pragma unroll
+            for (int a = 0; a < NPE; ++a)
+#pragma unroll
+                for (int b = 0; b < NPE; ++b) {
+                    const double c = form_entry<D, NPE>(kind, Q, a, b);
+                    Ke[a * NPE + b] += scaled ? coef * c : c;
+                }
+        }
+    }
+    const int32_t* mp = A.map + e * NPE * NPE;
+#pragma unroll
+    for (int i = 0; i < NPE * NPE; ++i) {
+        const int32_t s = mp[i];
+        if (s >= 0) A.val[s] += Ke[i];
+    }
+}
+
+// vector assembly, load vector and essential lift share one kernel:
+//   mode 0: forms applied to a selected state buffer (assembleVector)
+//   mode 1: load vector (SourceForm / NodalSourceForm)
+//   mode 2: essential lift, F -= forms(constrained values)
+template<int D, int P, int NQ>
+__global__ __launch_bounds__(kBlock) void k_asm_vector(AsmArgs A, Model m, Expr src, int mode, int64_t c0, int64_t c1) {
+    constexpr int NPE = Elem<D, P>::NPE;
+    const int64_t k = c0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= c1) return;
+    const int64_t e = A.order[k];
+    int32_t nodes[NPE];
+    double X[NPE][D];
+    load_element<D, P>(A, e, nodes, X);
+    double Uv[NPE], Ur[NPE], Ue[NPE];
+    if (mode == 0) {
+        gather<NPE>(A, nodes, 1, A.U, Uv);
+        gather<NPE>(A, nodes, 0, A.Udot, Ur);
+        if (A.selector == RSD_SEL_VALUE) {
+#pragma unroll
+            for (int a = 0; a < NPE; ++a) Ue[a] = Uv[a];
+        } else if (A.selector == RSD_SEL_RATE) {
+#pragma unroll
+            for (int a = 0; a < NPE; ++a) Ue[a] = Ur[a];
+        } else {
+            gather<NPE>(A, nodes, 0, A.operand, Ue);
+        }
+    } else if (mode == 1) {
+        if (A.nodal) {
+#pragma unroll
+            for (int a = 0; a < NPE; ++a) Ue[a] = A.nodal[nodes[a]];
+        }
+    } else {
+        gather<NPE>(A, nodes, 2, nullptr, Ue);
+    }
+    const bool has_kap = A.kappa != nullptr;
+    const double kap = has_kap ? A.kappa[e] : 1.0;
+    double Fe[NPE];
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) Fe[a] = 0.0;
+    for (int q = 0; q < (D == 2 ? NQ * NQ : NQ * NQ * NQ); ++q) {
+        QP<D, NPE> Q;
+        qp_eval<D, P, NQ>(q, X, Q, mode == 1);
+        if (mode == 1) {
+            double val = 0.0;
+            if (A.nodal) {
+#pragma unroll
+                for (int b = 0; b < NPE; ++b) val += Q.N[b] * Ue[b];
+            } else {
+                val = eval_expr(src, A.time, Q.x[0], Q.x[1], D == 3 ? Q.x[2] : 0.0, 0.0);
+            }
+#pragma unroll
+            for (int a = 0; a < NPE; ++a) Fe[a] += val * Q.N[a] * Q.measure * Q.w;
+            continue;
+        }
+        if (mode == 0) qp_state<D, NPE>(Q, Uv, Ur);
+        else { Q.T = 0.0; Q.Trate = 0.0; for (int i = 0; i < D; ++i) Q.gradT[i] = 0.0; }
+        qp_model<D, NPE>(Q, m, kap, has_kap);
+        for (int f = 0; f < A.nf; ++f) {
+            const int kind = A.kinds[f] & 0xff;
+            const bool scaled = (A.kinds[f] & RSD_FORM_SCALED) != 0;
+            const double coef = A.coefs[f];
+#pragma unroll
+            for (int a = 0; a < NPE; ++a) {
+                if (scaled) {
+                    double buf = 0.0;
+#pragma unroll
+                    for (int b = 0; b < NPE; ++b) buf += form_vec_entry<D, NPE>(kind, Q, a, b, Ue[b]);
+                    Fe[a] += coef * buf;
+                } else {
+#pragma unroll
+                    for (int b = 0; b < NPE; ++b) Fe[a] += form_vec_entry<D, NPE>(kind, Q, a, b, Ue[b]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) {
+        const int32_t r = A.t2a[nodes[a]];
+        if (r >= 0) A.F[r] += mode == 2 ? -1.0 * Fe[a] : Fe[a];
+    }
+}
+
+template<int D, int P, int NQ>
+static void run_matrix(rsd_problem* pb, const AsmArgs& A) {
+    hipStream_t st = pb->ctx->stream;
+    for (int c = 0; c < pb->n_colours(); ++c) {
+        const int64_t c0 = pb->colour_off[c], c1 = pb->colour_off[c + 1];
+        if (c1 <= c0) continue;
+        hipLaunchKernelGGL((k_asm_matrix<D, P, NQ>), dim3((unsigned)nblocks(c1 - c0)), dim3(kBlock), 0, st, A,
+                           pb->model, c0, c1);
+    }
+    RSD_HIP(hipGetLastError());
+}
+
+template<int D, int P, int NQ>
+static void run_vector(rsd_problem* pb, const AsmArgs& A, const Expr& src, int mode) {
+    hipStream_t st = pb->ctx->stream;
+    for (int c = 0; c < pb->n_colours(); ++c) {
+        const int64_t c0 = pb->colour_off[c], c1 = pb->colour_off[c + 1];
+        if (c1 <= c0) continue;
+        hipLaunchKernelGGL((k_asm_vector<D, P, NQ>), dim3((unsigned)nblocks(c1 - c0)), dim3(kBlock), 0, st, A,
+                           pb->model, src, mode, c0, c1);
+    }
+    RSD_HIP(hipGetLastError());
+}
+
+#define RSD_DISPATCH(D_, P_, NQ_, CALL)                                                      \
+    if (pb->dim == D_ && pb->p == P_ && pb->nq == NQ_) { CALL<D_, P_, NQ_>; return; }
+
+static void dispatch_matrix(rsd_problem* pb, const AsmArgs& A) {
+#define RSD_M(D_, P_, NQ_) RSD_DISPATCH(D_, P_, NQ_, run_matrix)
+    if (pb->dim == 2 && pb->p == 1 && pb->nq == 1) { run_matrix<2, 1, 1>(pb, A); return; }
+    if (pb->dim == 2 && pb->p == 1 && pb->nq == 2) { run_matrix<2, 1, 2>(pb, A); return; }
+    if (pb->dim == 2 && pb->p == 1 && pb->nq == 3) { run_matrix<2, 1, 3>(pb, A); return; }
+    if (pb->dim == 2 && pb->p == 2 && pb->nq == 2) { run_matrix<2, 2, 2>(pb, A); return; }
+    if (pb->dim == 2 && pb->p == 2 && pb->nq == 3) { run_matrix<2, 2, 3>(pb, A); return; }
+    if (pb->dim == 2 && pb->p == 2 && pb->nq == 4) { run_matrix<2, 2, 4>(pb, A); return; }
+    if (pb->dim == 3 && pb->p == 1 && pb->nq == 1) { run_matrix<3, 1, 1>(pb, A); return; }
+    if (pb->dim == 3 && pb->p == 1 && pb->nq == 2) { run_matrix<3, 1, 2>(pb, A); return; }
+    if (pb->dim == 3 && pb->p == 1 && pb->nq == 3) { run_matrix<3, 1, 3>(pb, A); return; }
+#undef RSD_M
+    throw Error(RSD_ERR_ARG, "discretisation not supported by the HIP element kernels (dim/p/nq)");
+}
+
+static void dispatch_vector(rsd_problem* pb, const AsmArgs& A, const Expr& src, int mode) {
+    if (pb->dim == 2 && pb->p == 1 && pb->nq == 1) { run_vector<2, 1, 1>(pb, A, src, mode); return; }
+    if (pb->dim == 2 && pb->p == 1 && pb->nq == 2) { run_vector<2, 1, 2>(pb, A, src, mode); return; }
+    if (pb->dim == 2 && pb->p == 1 && pb->nq == 3) { run_vector<2, 1, 3>(pb, A, src, mode); return; }
+    if (pb->dim == 2 && pb->p == 2 && pb->nq == 2) { run_vector<2, 2, 2>(pb, A, src, mode); return; }
+    if (pb->dim == 2 && pb->p == 2 && pb->nq == 3) { run_vector<2, 2, 3>(pb, A, src, mode); return; }
+    if (pb->dim == 2 && pb->p == 2 && pb->nq == 4) { run_vector<2, 2, 4>(pb, A, src, mode); return; }
+    if (pb->dim == 3 && pb->p == 1 && pb->nq == 1) { run_vector<3, 1, 1>(pb, A, src, mode); return; }
+    if (pb->dim == 3 && pb->p == 1 && pb->nq == 2) { run_vector<3, 1, 2>(pb, A, src, mode); return; }
+    if (pb->dim == 3 && pb->p == 1 && pb->nq == 3) { run_vector<3, 1, 3>(pb, A, src, mode); return; }
+    throw Error(RSD_ERR_ARG, "discretisation not supported by the HIP element kernels (dim/p/nq)");
+}
+
+AsmArgs base_args(rsd_problem* pb) {
+    AsmArgs A{};
+    A.xyz = pb->xyz.p;
+    A.ien = pb->ien.p;
+    A.t2a = pb->t2a.p;
+    A.gval = pb->gval.p;
+    A.order = pb->order.p;
+    A.map = pb->map_sell.p;
+    A.kappa = pb->kappa_ptr();
+    return A;
+}
+
+static void set_forms(AsmArgs& A, int32_t nf, const int32_t* kinds, const double* coefs) {
+    RSD_CHECK(nf >= 0 && nf <= 4, "between 0 and 4 forms");
+    A.nf = nf;
+    for (int f = 0; f < nf; ++f) {
+        const int k = kinds[f] & 0xff;
+        RSD_CHECK(k >= RSD_FORM_DIFFUSION && k <= RSD_FORM_TMASS, "unknown form kind");
+        A.kinds[f] = kinds[f];
+        A.coefs[f] = coefs ? coefs[f] : 1.0;
+    }
+}
+
+static bool forms_need_state(const rsd_problem* pb, const AsmArgs& A) {
+    bool need = pb->model.has_k || pb->model.has_dk;
+    for (int f = 0; f < A.nf; ++f) {
+        const int k = A.kinds[f] & 0xff;
+        if (k == RSD_FORM_TDIFFUSION || k == RSD_FORM_TMASS) need = true;
+    }
+    return need;
+}
+
+void assemble_matrix(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
+                     const double* U, const double* Udot, rsd_matrix* K) {
+    RSD_CHECK(K->pb == pb, "matrix belongs to another problem");
+    AsmArgs A = base_args(pb);
+    set_forms(A, nf, kinds, coefs);
+    A.U = U;
+    A.Udot = Udot;
+    A.val = K->val.p;
+    A.time = time;
+    A.need_state = forms_need_state(pb, A);
+    update_gval(pb, time);
+    launch_fill(pb->ctx, K->val.n, K->val.p, 0.0);
+    dispatch_matrix(pb, A);
+}
+
+void assemble_vector(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
+                     const double* U, const double* Udot, const double* operand, int32_t selector, double* F) {
+    RSD_CHECK(selector >= 0 && selector <= 2, "bad selector");
+    AsmArgs A = base_args(pb);
+    set_forms(A, nf, kinds, coefs);
+    A.U = U;
+    A.Udot = Udot;
+    A.operand = operand;
+    A.selector = selector;
+    A.F = F;
+    A.time = time;
+    update_gval(pb, time);
+    launch_fill(pb->ctx, pb->n_free, F, 0.0);
+    dispatch_vector(pb, A, Expr{}, 0);
+}
+
+void assemble_load(rsd_problem* pb, double time, const Expr* src, const double* nodal, double* F) {
+    launch_fill(pb->ctx, pb->n_free, F, 0.0);
+    if (!nodal && !src) return;  // a zero source adds exact zeros
+    AsmArgs A = base_args(pb);
+    A.F = F;
+    A.nodal = nodal;
+    A.time = time;
+    dispatch_vector(pb, A, src ? *src : Expr{}, 1);
+}
+
+void apply_essential(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs, double* F) {
+    AsmArgs A = base_args(pb);
+    set_forms(A, nf, kinds, coefs);
+    A.F = F;
+    A.time = time;
+    update_gval(pb, time);
+    dispatch_vector(pb, A, Expr{}, 2);
+}
+
+const Expr* cached_source(rsd_problem* pb, const char* text) {
+    if (!text) return nullptr;
+    if (pb->src_text != text) {
+        pb->src_expr = compile_expr(text);
+        pb->src_text = text;
+    }
+    // an expression that is the literal 0 contributes exact zeros
+    if (pb->src_expr.n == 1 && pb->src_expr.op[0] == E_CONST && pb->src_expr.c[0] == 0.0) return nullptr;
+    return &pb->src_expr;
+}
+
+}  // namespace rsd
+
+using namespace rsd;
+
+extern "C" {
+
+int rsd_assemble_matrix(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t nf, const int32_t* kinds,
+                        const double* coefs, const double* U, const double* Udot, rsd_matrix* K) {
+    return guard([&] {
+        RSD_CHECK(ctx && pb && K && (nf == 0 || kinds), "rsd_assemble_matrix: null argument");
+        RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
+        assemble_matrix(pb, time, nf, kinds, coefs, U, Udot, K);
+    });
+}
+
+int rsd_assemble_vector(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t nf, const int32_t* kinds,
+                        const double* coefs, const double* U, const double* Udot, const double* operand,
+                        int32_t selector, double* F) {
+    return guard([&] {
+        RSD_CHECK(ctx && pb && F && (nf == 0 || kinds), "rsd_assemble_vector: null argument");
+        RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
+        assemble_vector(pb, time, nf, kinds, coefs, U, Udot, operand, selector, F);
+    });
+}
+
+int rsd_assemble_load(rsd_ctx* ctx, rsd_problem* pb, double time, const char* src_expr, const double* nodal,
+                      double* F) {
+    return guard([&] {
+        RSD_CHECK(ctx && pb && F, "rsd_assemble_load: null argument");
+        RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
+        assemble_load(pb, time, nodal ? nullptr : cached_source(pb, src_expr), nodal, F);
+    });
+}
+
+int rsd_apply_essential(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t nf, const int32_t* kinds,
+                        const double* coefs, double* F) {
+    return guard([&] {
+        RSD_CHECK(ctx && pb && F && (nf == 0 || kinds), "rsd_apply_essential: null argument");
+        RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
+        apply_essential(pb, time, nf, kinds, coefs, F);
+    });
+}
+
+}  // extern "C"

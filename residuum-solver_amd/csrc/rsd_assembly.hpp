@@ -1,0 +1,39 @@
+// rsd_assembly.hpp -- element-kernel arguments and host entry points.
+#pragma once
+
+#include "rsd_problem.hpp"
+
+namespace rsd {
+
+struct AsmArgs {
+    const double* xyz;
+    const int32_t* ien;
+    const int32_t* t2a;
+    const double* gval;
+    const int32_t* order;
+    const int32_t* map;
+    const double* kappa;
+    const double* U;
+    const double* Udot;
+    const double* operand;
+    const double* nodal;
+    double* val;
+    double* F;
+    double time;
+    int32_t selector;
+    int32_t nf;
+    int32_t kinds[4];
+    double coefs[4];
+    int32_t need_state;
+};
+
+AsmArgs base_args(rsd_problem* pb);
+void assemble_matrix(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
+                     const double* U, const double* Udot, rsd_matrix* K);
+void assemble_vector(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
+                     const double* U, const double* Udot, const double* operand, int32_t selector, double* F);
+void assemble_load(rsd_problem* pb, double time, const Expr* src, const double* nodal, double* F);
+void apply_essential(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs, double* F);
+const Expr* cached_source(rsd_problem* pb, const char* text);
+
+}  // namespace rsd

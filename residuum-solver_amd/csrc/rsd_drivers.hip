@@ -1,0 +1,169 @@
+// rsd_drivers.hip -- solver drivers on the device: the linear Backward
+// Euler loop (BackwardEulerRunner::step over BackwardEulerStage).
+//
+// A step is assemble (K + M/dt, load, M U_prev / dt, essential lift) and a
+// warm-started PCG solve.  With use_graph the whole step -- assembly, right
+// hand side, CG set-up and the first CG chunk -- is captured once as a
+// hipGraph and replayed every step; further CG chunks replay a second graph
+// until the device-side stop flag is set.  Capture requires the boundary
+// values and the source not to depend on t (their kernels' time argument
+// would be frozen into the graph); otherwise the step runs eagerly.
+#include "rsd_assembly.hpp"
+#include "rsd_linalg.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace rsd {
+
+namespace {
+
+struct BEBuffers {
+    double *Uprev, *Udot, *F, *ms;
+};
+
+// BackwardEulerStage::assemble (BackwardEulerStage.hpp:221-246)
+void be_assemble(rsd_problem* pb, rsd_matrix* K, const BEBuffers& b, double* U, double ct, double dt,
+                 const Expr* src) {
+    rsd_ctx* ctx = pb->ctx;
+    const int64_t n = pb->n_free;
+    const int32_t kinds[2] = {RSD_FORM_DIFFUSION, RSD_FORM_MASS | RSD_FORM_SCALED};
+    const double coefs[2] = {1.0, 1.0 / dt};
+    const int32_t kmass[1] = {RSD_FORM_MASS};
+    const double one[1] = {1.0};
+    launch_copy(ctx, n, U, b.Udot);
+    launch_axpby(ctx, n, -1.0 / dt, 1.0 / dt, b.Uprev, b.Udot);
+    assemble_matrix(pb, ct, 2, kinds, coefs, U, b.Udot, K);
+    assemble_load(pb, ct, src, nullptr, b.F);
+    assemble_vector(pb, ct - dt, 1, kmass, one, b.Uprev, nullptr, nullptr, RSD_SEL_VALUE, b.ms);
+    launch_axpy(ctx, n, 1.0 / dt, b.ms, b.F);
+    apply_essential(pb, ct, 2, kinds, coefs, b.F);
+}
+
+struct Pinned {
+    CGState* p = nullptr;
+    Pinned() { RSD_HIP(hipHostMalloc((void**)&p, sizeof(CGState), hipHostMallocDefault)); }
+    ~Pinned() { if (p) (void)hipHostFree(p); }
+};
+
+struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    ~Graph() { reset(); }
+    void reset() {
+        if (e) (void)hipGraphExecDestroy(e);
+        if (g) (void)hipGraphDestroy(g);
+        e = nullptr;
+        g = nullptr;
+    }
+};
+
+template<typename F>
+void capture(hipStream_t st, Graph& G, F&& body) {
+    G.reset();
+    RSD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        body();
+    } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(st, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+        throw;
+    }
+    RSD_HIP(hipStreamEndCapture(st, &G.g));
+    RSD_HIP(hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0));
+}
+
+}  // namespace
+
+constexpr int64_t kChunk = 8;
+
+// returns steps taken; throws on a failed solve
+int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const char* src_text, double tol,
+               int64_t max_iters, int32_t precond, bool use_graph, double* U, int64_t max_steps, int64_t* iters) {
+    rsd_ctx* ctx = pb->ctx;
+    const int64_t n = pb->n_free;
+    hipStream_t st = ctx->stream;
+    const size_t vb = sizeof(double) * (size_t)std::max<int64_t>(n, 1);
+    BEBuffers b{(double*)ctx->ws("be.Uprev", vb), (double*)ctx->ws("be.Udot", vb), (double*)ctx->ws("be.F", vb),
+                (double*)ctx->ws("be.ms", vb)};
+    rsd_matrix K;
+    K.pb = pb;
+    K.val.alloc(pb->sell.slots);
+    const Expr* src = cached_source(pb, src_text);
+    const bool graphable = use_graph && !pb->bc_time_dependent && !(src && src->uses_t);
+    CGWork w = cg_workspace(ctx, n, max_iters, false);
+    ctx->partials(2 * std::max<int64_t>(1, nblocks(n)));
+    update_gval(pb, t0);
+    launch_copy(ctx, n, U, b.Uprev);
+    Pinned pin;
+    Graph step_graph, chunk_graph;
+    double graph_dt = -1.0;
+    double time = t0;
+    int64_t step = 0;
+    while (!(time >= tf - 1e-9)) {
+        RSD_CHECK(step < max_steps, "be_run: more steps than max_steps");
+        const double dt = (dt_policy < tf - time) ? dt_policy : (tf - time);
+        const double ct = time + dt;
+        if (graphable) {
+            if (!step_graph.e || graph_dt != dt) {
+                capture(st, step_graph, [&] {
+                    be_assemble(pb, &K, b, U, ct, dt, src);
+                    cg_launch_init(ctx, &K, b.F, U, tol, max_iters, precond, w);
+                    for (int64_t k = 0; k < kChunk; ++k) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);
+                });
+                graph_dt = dt;
+            }
+            if (!chunk_graph.e)
+                capture(st, chunk_graph, [&] {
+                    for (int64_t k = 0; k < kChunk; ++k) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);
+                });
+            RSD_HIP(hipGraphLaunch(step_graph.e, st));
+        } else {
+            be_assemble(pb, &K, b, U, ct, dt, src);
+            cg_launch_init(ctx, &K, b.F, U, tol, max_iters, precond, w);
+            for (int64_t k = 0; k < kChunk; ++k) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);
+        }
+        int64_t launched = kChunk;
+        for (;;) {
+            RSD_HIP(hipMemcpyAsync(pin.p, w.st, sizeof(CGState), hipMemcpyDeviceToHost, st));
+            RSD_HIP(hipStreamSynchronize(st));
+            if (pin.p->stop && pin.p->xdone) break;
+            RSD_CHECK(launched <= max_iters + kChunk, "be_run: CG did not stop");
+            if (graphable) RSD_HIP(hipGraphLaunch(chunk_graph.e, st));
+            else for (int64_t k = 0; k < kChunk; ++k) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);
+            launched += kChunk;
+        }
+        const CGState s = *pin.p;
+        if (iters) iters[step] = s.converged ? s.iter : max_iters;
+        if (!s.converged) throw Error(RSD_ERR_NOT_CONVERGED, "be_run: CG did not converge in step " + std::to_string(step));
+        launch_copy(ctx, n, U, b.Uprev);
+        time += dt;
+        ++step;
+    }
+    RSD_HIP(hipStreamSynchronize(st));
+    return step;
+}
+
+}  // namespace rsd
+
+using namespace rsd;
+
+extern "C" {
+
+int rsd_be_run(rsd_ctx* ctx, rsd_problem* pb, double t0, double tf, double dt, const char* src_expr, double cg_tol,
+               int64_t cg_max_iters, int32_t precond, int32_t use_graph, double* d_U, int64_t max_steps,
+               int64_t* steps, int64_t* iters) {
+    int64_t taken = 0;
+    const int rc = guard([&] {
+        RSD_CHECK(ctx && pb && d_U, "rsd_be_run: null argument");
+        RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
+        RSD_CHECK(dt > 0.0 && tf >= t0, "rsd_be_run: need dt > 0 and tf >= t0");
+        RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
+        taken = be_run(pb, t0, tf, dt, src_expr, cg_tol, cg_max_iters, precond, use_graph != 0, d_U, max_steps, iters);
+    });
+    if (steps) *steps = taken;
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,255 @@
+// rsd_internal.hpp -- shared internals of libresiduum_hip.so (gfx950).
+//
+// Context, error plumbing, the deterministic block-tree reduction every
+// reduction kernel uses, and the device-side expression interpreter.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "residuum_hip.h"
+
+namespace rsd {
+
+// ------------------------------------------------------------------------
+// errors: every C entry point catches and maps to an int status
+// ------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define RSD_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw ::rsd::Error(RSD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define RSD_CHECK(cond, msg)                                                               \
+    do {                                                                                   \
+        if (!(cond)) throw ::rsd::Error(RSD_ERR_ARG, msg);                                 \
+    } while (0)
+
+template<typename F>
+int guard(F&& f) {
+    try {
+        f();
+        return RSD_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return RSD_ERR_INTERNAL;
+    }
+}
+
+// ------------------------------------------------------------------------
+// launch geometry
+// ------------------------------------------------------------------------
+constexpr int kBlock = 256;       // threads per block, 4 wave64
+constexpr int kSlice = 64;        // SELL slice height = one wavefront
+constexpr int kXcds = 8;          // MI355X: 8 XCDs, blocks dispatched round-robin
+
+inline int64_t nblocks(int64_t n) { return (n + kBlock - 1) / kBlock; }
+
+// Blocks are handed to XCDs round-robin (block b -> XCD b % 8).  Remap so
+// each XCD walks one contiguous range of logical blocks: neighbouring rows
+// (and the x entries their stencils share) stay in one XCD's L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+    if (nb < kXcds) return b;
+    const int64_t q = nb / kXcds, r = nb % kXcds;
+    const int64_t x = b % kXcds, i = b / kXcds;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// ------------------------------------------------------------------------
+// deterministic reduction
+//   block: four 64-lane trees (lane l += lane l+off, off = 32..1), then the
+//   four wave sums added left to right by thread 0;
+//   grid: each block stores its partial; the last block to finish (atomic
+//   ticket) lets lane k add partials k, k+256, ... from 0.0 and reduces the
+//   256 lane sums with the same block tree.  The result never depends on
+//   which block finishes last.
+// ------------------------------------------------------------------------
+template<typename T>
+__device__ __forceinline__ T wave_tree(T v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_down(v, off, 64);
+    return v;
+}
+
+// returns the block sum in thread 0; sm must hold 4 entries per value
+template<int NV>
+__device__ __forceinline__ void block_tree(double (&v)[NV], double* sm) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double w = wave_tree(v[k]);
+        if (lane == 0) sm[k * 4 + wave] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double r = sm[k * 4 + 0];
+            r += sm[k * 4 + 1];
+            r += sm[k * 4 + 2];
+            r += sm[k * 4 + 3];
+            v[k] = r;
+        }
+    }
+    __syncthreads();
+}
+
+// Grid reduction of NV values per thread.  partials: [NV][nb] doubles,
+// ticket: one uint32 zero-initialised counter (reset by the last block).
+// Returns true in every thread of the last block, with out[] holding the
+// final sums (valid in all threads of that block).
+template<int NV>
+__device__ __forceinline__ bool grid_reduce(double (&v)[NV], int64_t lb, int64_t nb, double* partials,
+                                            unsigned* ticket, double (&out)[NV]) {
+    __shared__ double sm[NV * 4];
+    __shared__ bool last;
+    block_tree<NV>(v, sm);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) partials[k * nb + lb] = v[k];
+        __threadfence();
+        const unsigned t = atomicAdd(ticket, 1u);
+        last = (t == (unsigned)(nb - 1));
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        acc[k] = 0.0;
+        for (int64_t b = threadIdx.x; b < nb; b += kBlock) acc[k] += __hip_atomic_load(&partials[k * nb + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_tree<NV>(acc, sm);
+    __shared__ double res[NV];
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) res[k] = acc[k];
+        *ticket = 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = res[k];
+    return true;
+}
+
+// ------------------------------------------------------------------------
+// context
+// ------------------------------------------------------------------------
+struct Scratch {
+    double* partials = nullptr;   // reduction partials
+    int64_t partials_cap = 0;     // doubles
+    unsigned* tickets = nullptr;  // kTickets counters, zero when idle
+};
+constexpr int kTickets = 64;
+
+}  // namespace rsd
+
+struct rsd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rsd::Scratch scratch;
+    double* d_scalars = nullptr;  // small device scalar area for BLAS-1 results
+    // make sure the partials area can hold `need` doubles
+    double* partials(int64_t need);
+    // named device workspaces, grown on demand and kept for the context's life
+    struct Ws { void* p = nullptr; size_t bytes = 0; };
+    std::vector<std::pair<std::string, Ws>> ws_tab;
+    void* ws(const std::string& name, size_t bytes);
+    ~rsd_ctx();
+};
+
+namespace rsd {
+
+// ------------------------------------------------------------------------
+// expressions (compiled on the host, interpreted on the device)
+// ------------------------------------------------------------------------
+constexpr int kExprMax = 48;
+enum ExprOp : int32_t { E_CONST, E_VAR, E_ADD, E_SUB, E_MUL, E_DIV, E_POW, E_NEG,
+                        E_SIN, E_COS, E_TAN, E_EXP, E_LOG, E_SQRT, E_ABS, E_MIN, E_MAX };
+struct Expr {
+    int32_t n = 0;
+    int32_t uses_t = 0;
+    int32_t uses_T = 0;
+    int32_t op[kExprMax];
+    double c[kExprMax];
+};
+Expr compile_expr(const std::string& text);
+double eval_expr_host(const Expr& e, double t, double x, double y, double z, double T);
+
+__device__ inline double eval_expr(const Expr& e, double t, double x, double y, double z, double T) {
+    double st[16];
+    int sp = 0;
+    for (int i = 0; i < e.n; ++i) {
+        switch (e.op[i]) {
+        case E_CONST: st[sp++] = e.c[i]; break;
+        case E_VAR: {
+            const int v = (int)e.c[i];
+            st[sp++] = v == 0 ? t : v == 1 ? x : v == 2 ? y : v == 3 ? z : T;
+            break;
+        }
+        case E_ADD: sp--; st[sp - 1] = st[sp - 1] + st[sp]; break;
+        case E_SUB: sp--; st[sp - 1] = st[sp - 1] - st[sp]; break;
+        case E_MUL: sp--; st[sp - 1] = st[sp - 1] * st[sp]; break;
+        case E_DIV: sp--; st[sp - 1] = st[sp - 1] / st[sp]; break;
+        case E_POW: sp--; st[sp - 1] = pow(st[sp - 1], st[sp]); break;
+        case E_MIN: sp--; st[sp - 1] = (st[sp] < st[sp - 1]) ? st[sp] : st[sp - 1]; break;
+        case E_MAX: sp--; st[sp - 1] = (st[sp] > st[sp - 1]) ? st[sp] : st[sp - 1]; break;
+        case E_NEG: st[sp - 1] = -st[sp - 1]; break;
+        case E_SIN: st[sp - 1] = sin(st[sp - 1]); break;
+        case E_COS: st[sp - 1] = cos(st[sp - 1]); break;
+        case E_TAN: st[sp - 1] = tan(st[sp - 1]); break;
+        case E_EXP: st[sp - 1] = exp(st[sp - 1]); break;
+        case E_LOG: st[sp - 1] = log(st[sp - 1]); break;
+        case E_SQRT: st[sp - 1] = sqrt(st[sp - 1]); break;
+        default: st[sp - 1] = fabs(st[sp - 1]); break;
+        }
+    }
+    return st[0];
+}
+
+// ------------------------------------------------------------------------
+// device buffer helper
+// ------------------------------------------------------------------------
+template<typename T>
+struct DBuf {
+    T* p = nullptr;
+    int64_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(int64_t count) {
+        reset();
+        if (count > 0) RSD_HIP(hipMalloc(&p, sizeof(T) * (size_t)count));
+        n = count;
+    }
+    void upload(const T* h, int64_t count, hipStream_t s) {
+        alloc(count);
+        if (count > 0) RSD_HIP(hipMemcpyAsync(p, h, sizeof(T) * (size_t)count, hipMemcpyHostToDevice, s));
+    }
+};
+
+}  // namespace rsd

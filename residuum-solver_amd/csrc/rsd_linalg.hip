@@ -1,0 +1,449 @@
+// rsd_linalg.hip -- fills, BLAS-1, sliced-ELL SpMV, Jacobi and fused PCG.
+//
+// Products and sums are rounded separately (built with -ffp-contract=off):
+// the SpMV row sums reproduce the reference's left-to-right order
+// (linalg/operations/backend/cpu/MatrixOps.tpp:5-23) bit for bit, and every
+// reduction uses the fixed block tree of rsd_internal.hpp.
+#include "rsd_linalg.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace rsd {
+
+// ------------------------------------------------------------------------
+// elementwise kernels
+// ------------------------------------------------------------------------
+template<typename T>
+__global__ void k_fill(int64_t n, T* __restrict__ p, T v) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_axpy(int64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+
+__global__ void k_axpby(int64_t n, double a, double b, const double* __restrict__ x, double* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] = b * y[i] + a * x[i];
+}
+
+__global__ void k_scal(int64_t n, double a, double* __restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] *= a;
+}
+
+__global__ void k_copy(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] = x[i];
+}
+
+__global__ void k_mul(int64_t n, const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ z) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) z[i] = x[i] * y[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, int64_t nb, const double* __restrict__ x,
+                                                const double* __restrict__ y, double* partials, unsigned* ticket,
+                                                double* out, int32_t take_sqrt) {
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    double v[1] = {i < n ? x[i] * y[i] : 0.0};
+    double r[1];
+    if (grid_reduce<1>(v, lb, nb, partials, ticket, r) && threadIdx.x == 0) *out = take_sqrt ? sqrt(r[0]) : r[0];
+}
+
+// ------------------------------------------------------------------------
+// SpMV: one thread per row, one wavefront per 64-row slice
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double row_dot_f64(int64_t r, const int64_t* __restrict__ slice_ptr,
+                                              const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                              const double* __restrict__ val, const double* __restrict__ x) {
+    const int64_t base = slice_ptr[r >> 6] + (r & 63);
+    const int len = rp[r + 1] - rp[r];
+    double sum = 0.0;
+    int j = 0;
+    for (; j + 4 <= len; j += 4) {
+        const int64_t s0 = base + (int64_t)j * kSlice;
+        const int c0 = col[s0], c1 = col[s0 + kSlice], c2 = col[s0 + 2 * kSlice], c3 = col[s0 + 3 * kSlice];
+        const double v0 = val[s0], v1 = val[s0 + kSlice], v2 = val[s0 + 2 * kSlice], v3 = val[s0 + 3 * kSlice];
+        const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        sum += v0 * x0;
+        sum += v1 * x1;
+        sum += v2 * x2;
+        sum += v3 * x3;
+    }
+    for (; j < len; ++j) {
+        const int64_t s0 = base + (int64_t)j * kSlice;
+        sum += val[s0] * x[col[s0]];
+    }
+    return sum;
+}
+
+__global__ __launch_bounds__(kBlock) void k_spmv_f64(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
+                                                     const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                     const double* __restrict__ val, const double* __restrict__ x,
+                                                     double* __restrict__ y) {
+    const int64_t r = xcd_block(blockIdx.x, nb) * kBlock + threadIdx.x;
+    if (r < n) y[r] = row_dot_f64(r, slice_ptr, rp, col, val, x);
+}
+
+__global__ void k_inv_diag(int64_t n, const int32_t* __restrict__ diag, const double* __restrict__ val,
+                           double* __restrict__ inv) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int32_t s = diag[i];
+    const double d = s >= 0 ? val[s] : 0.0;
+    inv[i] = (fabs(d) > 1e-300) ? 1.0 / d : 1.0;
+}
+
+// ------------------------------------------------------------------------
+// fused PCG (cg/Solver.tpp:15-119), three kernels per iteration:
+//   spmv_pap:  Ap = A p, pAp = p.Ap, alpha = rz_old / pAp
+//   update_r:  r += (-alpha) Ap, z = M r, ||r||, r.z, convergence, beta
+//   update_xp: x += alpha p, p = beta p + z (skipped once stopped)
+// The state lives on the device; after `stop` the kernels exit at once,
+// so fixed-size chunks (or graph replays) reproduce the reference's
+// stopping iterate exactly.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_cg_init(int64_t n, int64_t nb, const double* __restrict__ b,
+                                                    const double* __restrict__ Ax, double* __restrict__ r,
+                                                    double* __restrict__ p, const double* __restrict__ invd,
+                                                    CGState* st, double tol, int64_t max_iters, double* hist,
+                                                    double* partials, unsigned* ticket) {
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < n) {
+        const double ri = b[i] + -1.0 * Ax[i];
+        const double zi = invd ? ri * invd[i] : ri;
+        r[i] = ri;
+        p[i] = zi;
+        v[0] = ri * ri;
+        v[1] = ri * zi;
+    }
+    double out[2];
+    if (grid_reduce<2>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) {
+        const double res0 = sqrt(out[0]);
+        st->res0 = res0;
+        st->res = res0;
+        st->rz_old = out[1];
+        st->iter = 0;
+        st->converged = 0;
+        st->stop = 0;
+        st->xdone = 0;
+        if (hist) hist[0] = res0;
+        if (res0 / (res0 + 1e-50) < tol) {
+            st->converged = 1;
+            st->stop = 1;
+            st->xdone = 1;
+        } else if (max_iters <= 1) {
+            st->stop = 1;
+            st->xdone = 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_pap(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
+                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                        const double* __restrict__ val, const double* __restrict__ p,
+                                                        double* __restrict__ Ap, CGState* st, double* partials,
+                                                        unsigned* ticket) {
+    if (st->stop) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t r = lb * kBlock + threadIdx.x;
+    double v[1] = {0.0};
+    if (r < n) {
+        const double y = row_dot_f64(r, slice_ptr, rp, col, val, p);
+        Ap[r] = y;
+        v[0] = p[r] * y;
+    }
+    double out[1];
+    if (grid_reduce<1>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) st->alpha = st->rz_old / out[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_update_r(int64_t n, int64_t nb, const double* __restrict__ Ap,
+                                                        double* __restrict__ r, const double* __restrict__ invd,
+                                                        CGState* st, double tol, int64_t max_iters, double* hist,
+                                                        double* partials, unsigned* ticket) {
+    if (st->stop) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    const double alpha = st->alpha;
+    double v[2] = {0.0, 0.0};
+    if (i < n) {
+        const double ri = r[i] + -alpha * Ap[i];
+        r[i] = ri;
+        const double zi = invd ? ri * invd[i] : ri;
+        v[0] = ri * ri;
+        v[1] = ri * zi;
+    }
+    double out[2];
+    if (grid_reduce<2>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) {
+        const int64_t k = st->iter + 1;
+        st->iter = k;
+        const double res = sqrt(out[0]);
+        st->res = res;
+        if (hist) hist[k] = res;
+        if (res / (st->res0 + 1e-50) < tol) {
+            st->converged = 1;
+            st->stop = 1;
+        } else {
+            st->beta = out[1] / st->rz_old;
+            st->rz_old = out[1];
+            if (k >= max_iters - 1) st->stop = 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_update_xp(int64_t n, int64_t nb, double* __restrict__ x,
+                                                         double* __restrict__ p, const double* __restrict__ r,
+                                                         const double* __restrict__ invd, CGState* st,
+                                                         unsigned* ticket) {
+    if (st->xdone) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    const double alpha = st->alpha, beta = st->beta;
+    const int32_t stop = st->stop;
+    if (i < n) {
+        const double pi = p[i];
+        x[i] += alpha * pi;
+        if (!stop) {
+            const double zi = invd ? r[i] * invd[i] : r[i];
+            p[i] = beta * pi + 1.0 * zi;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned t = atomicAdd(ticket, 1u);
+        if (t == (unsigned)(nb - 1)) {
+            st->xdone = stop;
+            *ticket = 0u;
+        }
+    }
+}
+
+}  // namespace rsd
+
+using namespace rsd;
+
+// ------------------------------------------------------------------------
+// host-side launchers used by the drivers
+// ------------------------------------------------------------------------
+namespace rsd {
+
+void launch_spmv_f64(rsd_ctx* ctx, const rsd_matrix* A, const double* x, double* y) {
+    const Sell& S = A->pb->sell;
+    const int64_t n = S.n_rows, nb = nblocks(n);
+    if (!n) return;
+    hipLaunchKernelGGL(k_spmv_f64, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, S.slice_ptr.p,
+                       S.row_ptr.p, S.col.p, A->val.p, x, y);
+}
+
+void launch_inv_diag(rsd_ctx* ctx, const rsd_matrix* A, double* inv) {
+    const Sell& S = A->pb->sell;
+    const int64_t n = S.n_rows;
+    if (!n) return;
+    hipLaunchKernelGGL(k_inv_diag, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, ctx->stream, n, S.diag.p, A->val.p, inv);
+}
+
+void launch_dot(rsd_ctx* ctx, int64_t n, const double* x, const double* y, double* d_out, bool take_sqrt) {
+    const int64_t nb = std::max<int64_t>(1, nblocks(n));
+    double* part = ctx->partials(nb);
+    hipLaunchKernelGGL(k_dot, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, x, y, part,
+                       ctx->scratch.tickets, d_out, (int32_t)take_sqrt);
+}
+
+void launch_axpy(rsd_ctx* ctx, int64_t n, double a, const double* x, double* y) {
+    if (n) hipLaunchKernelGGL(k_axpy, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, ctx->stream, n, a, x, y);
+}
+void launch_axpby(rsd_ctx* ctx, int64_t n, double a, double b, const double* x, double* y) {
+    if (n) hipLaunchKernelGGL(k_axpby, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, ctx->stream, n, a, b, x, y);
+}
+void launch_scal(rsd_ctx* ctx, int64_t n, double a, double* x) {
+    if (n) hipLaunchKernelGGL(k_scal, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, ctx->stream, n, a, x);
+}
+void launch_copy(rsd_ctx* ctx, int64_t n, const double* x, double* y) {
+    if (n) hipLaunchKernelGGL(k_copy, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, ctx->stream, n, x, y);
+}
+void launch_fill(rsd_ctx* ctx, int64_t n, double* p, double v) {
+    if (n) hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, ctx->stream, n, p, v);
+}
+
+CGWork cg_workspace(rsd_ctx* ctx, int64_t n, int64_t max_iters, bool want_hist) {
+    CGWork w;
+    w.r = (double*)ctx->ws("cg.r", sizeof(double) * (size_t)std::max<int64_t>(n, 1));
+    w.p = (double*)ctx->ws("cg.p", sizeof(double) * (size_t)std::max<int64_t>(n, 1));
+    w.Ap = (double*)ctx->ws("cg.Ap", sizeof(double) * (size_t)std::max<int64_t>(n, 1));
+    w.invd = (double*)ctx->ws("cg.invd", sizeof(double) * (size_t)std::max<int64_t>(n, 1));
+    w.st = (CGState*)ctx->ws("cg.state", sizeof(CGState));
+    w.hist = want_hist ? (double*)ctx->ws("cg.hist", sizeof(double) * (size_t)(max_iters + 1)) : nullptr;
+    return w;
+}
+
+void cg_launch_init(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
+                    int32_t precond, const CGWork& w) {
+    const int64_t n = A->pb->sell.n_rows, nb = std::max<int64_t>(1, nblocks(n));
+    double* part = ctx->partials(2 * nb);
+    if (precond == RSD_PC_JACOBI) launch_inv_diag(ctx, A, w.invd);
+    launch_spmv_f64(ctx, A, x, w.Ap);
+    hipLaunchKernelGGL(k_cg_init, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, b, w.Ap, w.r, w.p,
+                       precond == RSD_PC_JACOBI ? w.invd : nullptr, w.st, tol, max_iters, w.hist, part,
+                       ctx->scratch.tickets);
+}
+
+void cg_launch_iter(rsd_ctx* ctx, const rsd_matrix* A, double* x, double tol, int64_t max_iters, int32_t precond,
+                    const CGWork& w) {
+    const Sell& S = A->pb->sell;
+    const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n));
+    double* part = ctx->partials(2 * nb);
+    const double* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
+    hipLaunchKernelGGL(k_cg_spmv_pap, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, S.slice_ptr.p,
+                       S.row_ptr.p, S.col.p, A->val.p, w.p, w.Ap, w.st, part, ctx->scratch.tickets);
+    hipLaunchKernelGGL(k_cg_update_r, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, w.Ap, w.r, invd,
+                       w.st, tol, max_iters, w.hist, part, ctx->scratch.tickets);
+    hipLaunchKernelGGL(k_cg_update_xp, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, x, w.p, w.r, invd,
+                       w.st, ctx->scratch.tickets);
+}
+
+void cg_report(const CGState& s, int64_t max_iters, rsd_report* rep) {
+    if (!rep) return;
+    rep->converged = s.converged;
+    rep->initial_residual = s.res0;
+    rep->final_residual = s.converged || s.iter > 0 ? s.res : s.res0;
+    rep->iterations = s.converged ? s.iter : max_iters;
+    rep->final_residual_rel = s.converged && s.iter == 0 ? 1.0 : rep->final_residual / (s.res0 + 1e-50);
+}
+
+// Run a CG solve to completion with chunked launches; returns the final state.
+CGState cg_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
+               int32_t precond, const CGWork& w) {
+    cg_launch_init(ctx, A, b, x, tol, max_iters, precond, w);
+    CGState h{};
+    CGState* pinned = nullptr;
+    RSD_HIP(hipHostMalloc((void**)&pinned, sizeof(CGState), hipHostMallocDefault));
+    int64_t chunk = 4, done_iters = 0;
+    for (;;) {
+        RSD_HIP(hipMemcpyAsync(pinned, w.st, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
+        RSD_HIP(hipStreamSynchronize(ctx->stream));
+        h = *pinned;
+        if (h.stop && h.xdone) break;
+        RSD_CHECK(done_iters <= max_iters + 2, "CG did not stop within max_iters launches");
+        const int64_t c = std::min<int64_t>(chunk, std::max<int64_t>(1, max_iters - done_iters));
+        for (int64_t k = 0; k < c; ++k) cg_launch_iter(ctx, A, x, tol, max_iters, precond, w);
+        RSD_HIP(hipGetLastError());
+        done_iters += c;
+        chunk = std::min<int64_t>(chunk * 2, 64);
+    }
+    (void)hipHostFree(pinned);
+    return h;
+}
+
+}  // namespace rsd
+
+// ------------------------------------------------------------------------
+// C-ABI
+// ------------------------------------------------------------------------
+extern "C" {
+
+int rsd_fill_f64(rsd_ctx* ctx, double* p, int64_t n, double v) {
+    return guard([&] {
+        RSD_CHECK(ctx && (p || n == 0) && n >= 0, "rsd_fill_f64: bad argument");
+        launch_fill(ctx, n, p, v);
+        RSD_HIP(hipGetLastError());
+    });
+}
+
+int rsd_fill_f32(rsd_ctx* ctx, float* p, int64_t n, float v) {
+    return guard([&] {
+        RSD_CHECK(ctx && (p || n == 0) && n >= 0, "rsd_fill_f32: bad argument");
+        if (n) hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, ctx->stream, n, p, v);
+        RSD_HIP(hipGetLastError());
+    });
+}
+
+static void finish_scalar(rsd_ctx* ctx, double* d_out, double* h_out) {
+    if (h_out) {
+        RSD_HIP(hipMemcpyAsync(h_out, d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        RSD_HIP(hipStreamSynchronize(ctx->stream));
+    }
+}
+
+int rsd_dot_f64(rsd_ctx* ctx, int64_t n, const double* x, const double* y, double* d_out, double* h_out) {
+    return guard([&] {
+        RSD_CHECK(ctx && n >= 0 && (n == 0 || (x && y)), "rsd_dot_f64: bad argument");
+        double* o = d_out ? d_out : ctx->d_scalars;
+        launch_dot(ctx, n, x, y, o, false);
+        RSD_HIP(hipGetLastError());
+        finish_scalar(ctx, o, h_out);
+    });
+}
+
+int rsd_norm2_f64(rsd_ctx* ctx, int64_t n, const double* x, double* d_out, double* h_out) {
+    return guard([&] {
+        RSD_CHECK(ctx && n >= 0 && (n == 0 || x), "rsd_norm2_f64: bad argument");
+        double* o = d_out ? d_out : ctx->d_scalars;
+        launch_dot(ctx, n, x, x, o, true);
+        RSD_HIP(hipGetLastError());
+        finish_scalar(ctx, o, h_out);
+    });
+}
+
+int rsd_axpy_f64(rsd_ctx* ctx, int64_t n, double a, const double* x, double* y) {
+    return guard([&] { RSD_CHECK(ctx && n >= 0, "bad argument"); launch_axpy(ctx, n, a, x, y); RSD_HIP(hipGetLastError()); });
+}
+int rsd_axpby_f64(rsd_ctx* ctx, int64_t n, double a, double b, const double* x, double* y) {
+    return guard([&] { RSD_CHECK(ctx && n >= 0, "bad argument"); launch_axpby(ctx, n, a, b, x, y); RSD_HIP(hipGetLastError()); });
+}
+int rsd_scal_f64(rsd_ctx* ctx, int64_t n, double a, double* x) {
+    return guard([&] { RSD_CHECK(ctx && n >= 0, "bad argument"); launch_scal(ctx, n, a, x); RSD_HIP(hipGetLastError()); });
+}
+int rsd_copy_f64(rsd_ctx* ctx, int64_t n, const double* x, double* y) {
+    return guard([&] { RSD_CHECK(ctx && n >= 0, "bad argument"); launch_copy(ctx, n, x, y); RSD_HIP(hipGetLastError()); });
+}
+int rsd_mul_f64(rsd_ctx* ctx, int64_t n, const double* x, const double* y, double* z) {
+    return guard([&] {
+        RSD_CHECK(ctx && n >= 0, "bad argument");
+        if (n) hipLaunchKernelGGL(k_mul, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, ctx->stream, n, x, y, z);
+        RSD_HIP(hipGetLastError());
+    });
+}
+
+int rsd_spmv_f64(rsd_ctx* ctx, rsd_matrix* A, const double* x, double* y) {
+    return guard([&] {
+        RSD_CHECK(ctx && A && x && y, "rsd_spmv_f64: null argument");
+        launch_spmv_f64(ctx, A, x, y);
+        RSD_HIP(hipGetLastError());
+    });
+}
+
+int rsd_jacobi_inv_diag(rsd_ctx* ctx, rsd_matrix* A, double* inv) {
+    return guard([&] {
+        RSD_CHECK(ctx && A && inv, "rsd_jacobi_inv_diag: null argument");
+        launch_inv_diag(ctx, A, inv);
+        RSD_HIP(hipGetLastError());
+    });
+}
+
+int rsd_cg_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
+                     int32_t precond, rsd_report* rep, double* hist) {
+    return guard([&] {
+        RSD_CHECK(ctx && A && b && x, "rsd_cg_solve_f64: null argument");
+        RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
+        RSD_CHECK(max_iters >= 0, "max_iters must be >= 0");
+        const int64_t n = A->pb->sell.n_rows;
+        CGWork w = cg_workspace(ctx, n, max_iters, hist != nullptr);
+        CGState s = cg_run(ctx, A, b, x, tol, max_iters, precond, w);
+        cg_report(s, max_iters, rep);
+        if (hist) {
+            const int64_t cnt = std::min<int64_t>(s.iter + 1, max_iters + 1);
+            RSD_HIP(hipMemcpyAsync(hist, w.hist, sizeof(double) * (size_t)cnt, hipMemcpyDeviceToHost, ctx->stream));
+            RSD_HIP(hipStreamSynchronize(ctx->stream));
+        }
+    });
+}
+
+}  // extern "C"

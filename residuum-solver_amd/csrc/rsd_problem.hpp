@@ -1,0 +1,85 @@
+// rsd_problem.hpp -- the device-resident discretised heat problem and the
+// sliced-ELL matrix layout shared by assembly, SpMV and the solvers.
+#pragma once
+
+#include "rsd_internal.hpp"
+
+namespace rsd {
+
+// Material model passed by value to the element kernels.
+struct Model {
+    double k_scalar = 1.0;
+    double tensor[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    int32_t aniso = 0;
+    int32_t has_k = 0, has_dk = 0;
+    double rho = 1.0, cp = 1.0;
+    Expr k, dk;
+};
+
+// Sliced ELL ("SELL-64") layout of the problem's pattern.  Row r lives in
+// slice s = r / 64, lane l = r % 64; its j-th entry (CSR order) sits at slot
+// slice_ptr[s] + 64*j + l.  One wavefront walks one slice with perfectly
+// coalesced value / column loads, and for stencil-like FEM rows the gathered
+// x entries of neighbouring lanes are neighbours too.
+struct Sell {
+    int64_t n_rows = 0, n_slices = 0, slots = 0;
+    std::vector<int64_t> h_slice_ptr;
+    DBuf<int64_t> slice_ptr;   // [n_slices + 1]
+    DBuf<int32_t> row_ptr;     // CSR row pointer (int32), gives row lengths
+    DBuf<int32_t> col;         // [slots] absolute column (padding: own row)
+    DBuf<int16_t> off16;       // [slots] column - row, when every offset fits
+    DBuf<int32_t> diag;        // [n_rows] slot of the diagonal entry, -1 if absent
+    bool has_off16 = false;
+};
+
+}  // namespace rsd
+
+struct rsd_problem {
+    rsd_ctx* ctx = nullptr;
+    int32_t dim = 0, p = 1, nq = 2, npe = 0, nfaces = 0;
+    int64_t n_nodes = 0, n_elem = 0, n_free = 0, nnz = 0;
+    int32_t max_row = 0;
+    // host copies (setup products, served by the getters)
+    std::vector<double> h_xyz;
+    std::vector<int32_t> h_ien, h_rng, h_t2a, h_a2t, h_ctag, h_colour;
+    std::vector<int32_t> h_rp, h_ci;   // CSR pattern, int32
+    std::vector<int32_t> h_order;      // colour-major element order
+    std::vector<int64_t> colour_off;   // [n_colours + 1] into h_order
+    // Dirichlet registry
+    std::vector<int32_t> bc_tags;
+    std::vector<rsd::Expr> bc_exprs;
+    rsd::DBuf<int32_t> d_bc_tags;
+    rsd::DBuf<rsd::Expr> d_bc_exprs;
+    bool bc_time_dependent = false;
+    double gval_time = 0.0;
+    bool gval_valid = false;
+    // device arrays
+    rsd::DBuf<double> xyz;        // [n_nodes * dim]
+    rsd::DBuf<int32_t> ien;       // [n_elem * npe]
+    rsd::DBuf<int32_t> t2a;       // [n_nodes]
+    rsd::DBuf<int32_t> ctag;      // [n_nodes]
+    rsd::DBuf<int32_t> order;     // [n_elem] colour-major
+    rsd::DBuf<int32_t> map_sell;  // [n_elem * npe * npe] SELL slot or -1
+    rsd::DBuf<double> gval;       // [n_nodes] Dirichlet value (0 on free nodes)
+    rsd::DBuf<double> kappa;      // [n_elem] or empty
+    const double* kappa_ext = nullptr;  // device kappa supplied by the caller
+    rsd::Sell sell;
+    rsd::Model model;
+    // caches for compiled source expressions
+    std::string src_text;
+    rsd::Expr src_expr;
+
+    const double* kappa_ptr() const { return kappa_ext ? kappa_ext : (kappa.n ? kappa.p : nullptr); }
+    int n_colours() const { return (int)colour_off.size() - 1; }
+};
+
+struct rsd_matrix {
+    rsd_problem* pb = nullptr;
+    rsd::DBuf<double> val;    // [slots] SELL order
+    rsd::DBuf<float> val32;   // [slots] FP32 copy for the mixed-precision path
+};
+
+namespace rsd {
+// refresh the per-node Dirichlet values for `time` (no-op when cached)
+void update_gval(rsd_problem* pb, double time);
+}

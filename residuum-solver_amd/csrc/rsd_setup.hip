@@ -1,0 +1,794 @@
+// rsd_setup.hip -- context, memory primitives, expressions and problem setup.
+//
+// Setup follows the reference's host-side steps: BlockMesh2D/3D::generate,
+// TopologicalDOF::buildConstraints, Assembler<CPU>::createMatrix and the
+// element->nnz positions scatterElementMatrix finds by merging.  The
+// products are bitwise identical to the reference's (checked against the
+// oracle and golden vectors); the pattern build is node-centric and
+// multi-threaded rather than the reference's two-pass touch list.
+#include "rsd_problem.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdlib>
+#include <thread>
+
+namespace rsd {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+// ------------------------------------------------------------------------
+// expression compiler: recursive descent to a postfix program
+// grammar: sum := prod (('+'|'-') prod)*, prod := unary (('*'|'/') unary)*,
+// unary := '-' unary | '+' unary | pow, pow := atom ('^' unary)?
+// ------------------------------------------------------------------------
+namespace {
+struct Parser {
+    const char* s;
+    Expr e;
+    void emit(int op, double c = 0.0) {
+        if (e.n >= kExprMax) throw Error(RSD_ERR_ARG, "expression too long");
+        e.op[e.n] = op;
+        e.c[e.n] = c;
+        e.n++;
+    }
+    void ws() { while (*s && std::isspace((unsigned char)*s)) ++s; }
+    [[noreturn]] void fail() { throw Error(RSD_ERR_ARG, std::string("cannot parse expression near '") + s + "'"); }
+    void atom() {
+        ws();
+        if (std::isdigit((unsigned char)*s) || (*s == '.' && std::isdigit((unsigned char)s[1]))) {
+            char* end;
+            const double v = std::strtod(s, &end);
+            s = end;
+            emit(E_CONST, v);
+            return;
+        }
+        if (std::isalpha((unsigned char)*s) || *s == '_') {
+            std::string id;
+            while (std::isalnum((unsigned char)*s) || *s == '_') id += *s++;
+            ws();
+            if (*s == '(') {
+                ++s;
+                struct F { const char* name; int op; int arity; };
+                static const F fs[] = {{"sin", E_SIN, 1}, {"cos", E_COS, 1}, {"tan", E_TAN, 1},
+                                       {"exp", E_EXP, 1}, {"log", E_LOG, 1}, {"sqrt", E_SQRT, 1},
+                                       {"abs", E_ABS, 1}, {"pow", E_POW, 2}, {"min", E_MIN, 2},
+                                       {"max", E_MAX, 2}};
+                for (const F& f : fs) {
+                    if (id != f.name) continue;
+                    sum();
+                    if (f.arity == 2) {
+                        ws();
+                        if (*s != ',') fail();
+                        ++s;
+                        sum();
+                    }
+                    ws();
+                    if (*s != ')') fail();
+                    ++s;
+                    emit(f.op);
+                    return;
+                }
+                fail();
+            }
+            static const char* vars[] = {"t", "x", "y", "z", "T"};
+            for (int v = 0; v < 5; ++v)
+                if (id == vars[v]) {
+                    if (v == 0) e.uses_t = 1;
+                    if (v == 4) e.uses_T = 1;
+                    emit(E_VAR, (double)v);
+                    return;
+                }
+            if (id == "pi") { emit(E_CONST, 3.14159265358979323846); return; }
+            if (id == "e") { emit(E_CONST, 2.71828182845904523536); return; }
+            fail();
+        }
+        if (*s == '(') {
+            ++s;
+            sum();
+            ws();
+            if (*s != ')') fail();
+            ++s;
+            return;
+        }
+        fail();
+    }
+    void power() {
+        atom();
+        ws();
+        if (*s == '^') { ++s; unary(); emit(E_POW); }
+    }
+    void unary() {
+        ws();
+        if (*s == '-') { ++s; unary(); emit(E_NEG); return; }
+        if (*s == '+') { ++s; unary(); return; }
+        power();
+    }
+    void prod() {
+        unary();
+        for (;;) {
+            ws();
+            if (*s == '*') { ++s; unary(); emit(E_MUL); }
+            else if (*s == '/') { ++s; unary(); emit(E_DIV); }
+            else return;
+        }
+    }
+    void sum() {
+        prod();
+        for (;;) {
+            ws();
+            if (*s == '+') { ++s; prod(); emit(E_ADD); }
+            else if (*s == '-') { ++s; prod(); emit(E_SUB); }
+            else return;
+        }
+    }
+};
+}  // namespace
+
+Expr compile_expr(const std::string& text) {
+    Parser p{text.c_str(), Expr{}};
+    p.sum();
+    p.ws();
+    if (*p.s) p.fail();
+    // stack depth check (device interpreter holds 16)
+    int d = 0, dmax = 0;
+    for (int i = 0; i < p.e.n; ++i) {
+        const int op = p.e.op[i];
+        if (op == E_CONST || op == E_VAR) ++d;
+        else if (op == E_ADD || op == E_SUB || op == E_MUL || op == E_DIV || op == E_POW || op == E_MIN || op == E_MAX) --d;
+        dmax = std::max(dmax, d);
+    }
+    if (dmax > 16) throw Error(RSD_ERR_ARG, "expression nests too deeply");
+    return p.e;
+}
+
+double eval_expr_host(const Expr& e, double t, double x, double y, double z, double T) {
+    double st[16];
+    int sp = 0;
+    for (int i = 0; i < e.n; ++i) {
+        switch (e.op[i]) {
+        case E_CONST: st[sp++] = e.c[i]; break;
+        case E_VAR: { const int v = (int)e.c[i]; st[sp++] = v == 0 ? t : v == 1 ? x : v == 2 ? y : v == 3 ? z : T; break; }
+        case E_ADD: sp--; st[sp - 1] = st[sp - 1] + st[sp]; break;
+        case E_SUB: sp--; st[sp - 1] = st[sp - 1] - st[sp]; break;
+        case E_MUL: sp--; st[sp - 1] = st[sp - 1] * st[sp]; break;
+        case E_DIV: sp--; st[sp - 1] = st[sp - 1] / st[sp]; break;
+        case E_POW: sp--; st[sp - 1] = std::pow(st[sp - 1], st[sp]); break;
+        case E_MIN: sp--; st[sp - 1] = (st[sp] < st[sp - 1]) ? st[sp] : st[sp - 1]; break;
+        case E_MAX: sp--; st[sp - 1] = (st[sp] > st[sp - 1]) ? st[sp] : st[sp - 1]; break;
+        case E_NEG: st[sp - 1] = -st[sp - 1]; break;
+        case E_SIN: st[sp - 1] = std::sin(st[sp - 1]); break;
+        case E_COS: st[sp - 1] = std::cos(st[sp - 1]); break;
+        case E_TAN: st[sp - 1] = std::tan(st[sp - 1]); break;
+        case E_EXP: st[sp - 1] = std::exp(st[sp - 1]); break;
+        case E_LOG: st[sp - 1] = std::log(st[sp - 1]); break;
+        case E_SQRT: st[sp - 1] = std::sqrt(st[sp - 1]); break;
+        default: st[sp - 1] = std::fabs(st[sp - 1]); break;
+        }
+    }
+    return st[0];
+}
+
+// run f(lo, hi) over [0, n) on the host's cores
+template<typename F>
+static void parallel_for(int64_t n, F&& f) {
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 4096) nt = 1;
+    std::vector<std::thread> th;
+    const int64_t chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const int64_t lo = t * chunk, hi = std::min(n, lo + chunk);
+        if (lo >= hi) break;
+        th.emplace_back([&, lo, hi] { f(lo, hi); });
+    }
+    for (auto& x : th) x.join();
+}
+
+// ------------------------------------------------------------------------
+// Dirichlet values per node
+// ------------------------------------------------------------------------
+__global__ void k_gval(int64_t n_nodes, int32_t dim, const double* __restrict__ xyz, const int32_t* __restrict__ t2a,
+                       const int32_t* __restrict__ ctag, int32_t n_bc, const int32_t* __restrict__ tags,
+                       const Expr* __restrict__ exprs, double time, double* __restrict__ g) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_nodes) return;
+    double v = 0.0;
+    if (t2a[i] < 0) {
+        const double x = xyz[i * dim], y = xyz[i * dim + 1], z = dim == 3 ? xyz[i * dim + 2] : 0.0;
+        for (int k = 0; k < n_bc; ++k)
+            if (tags[k] == ctag[i]) v = eval_expr(exprs[k], time, x, y, z, 0.0);
+    }
+    g[i] = v;
+}
+
+}  // namespace rsd
+
+using namespace rsd;
+
+void rsd::update_gval(rsd_problem* pb, double time) {
+    if (pb->gval_valid && (!pb->bc_time_dependent || pb->gval_time == time)) return;
+    const int64_t nb = nblocks(pb->n_nodes);
+    hipLaunchKernelGGL(k_gval, dim3((unsigned)nb), dim3(kBlock), 0, pb->ctx->stream, pb->n_nodes, pb->dim,
+                       pb->xyz.p, pb->t2a.p, pb->ctag.p, (int32_t)pb->bc_tags.size(), pb->d_bc_tags.p, pb->d_bc_exprs.p, time,
+                       pb->gval.p);
+    RSD_HIP(hipGetLastError());
+    pb->gval_time = time;
+    pb->gval_valid = true;
+}
+
+// ------------------------------------------------------------------------
+// context
+// ------------------------------------------------------------------------
+double* rsd_ctx::partials(int64_t need) {
+    if (need > scratch.partials_cap) {
+        if (scratch.partials) RSD_HIP(hipFree(scratch.partials));
+        const int64_t cap = std::max<int64_t>(need, 1 << 16);
+        RSD_HIP(hipMalloc(&scratch.partials, sizeof(double) * (size_t)cap));
+        scratch.partials_cap = cap;
+    }
+    return scratch.partials;
+}
+
+void* rsd_ctx::ws(const std::string& name, size_t bytes) {
+    for (auto& e : ws_tab)
+        if (e.first == name) {
+            if (e.second.bytes < bytes) {
+                RSD_HIP(hipStreamSynchronize(stream));
+                RSD_HIP(hipFree(e.second.p));
+                RSD_HIP(hipMalloc(&e.second.p, bytes));
+                e.second.bytes = bytes;
+            }
+            return e.second.p;
+        }
+    Ws w;
+    RSD_HIP(hipMalloc(&w.p, std::max<size_t>(bytes, 256)));
+    w.bytes = std::max<size_t>(bytes, 256);
+    ws_tab.emplace_back(name, w);
+    return w.p;
+}
+
+rsd_ctx::~rsd_ctx() {
+    for (auto& e : ws_tab) (void)hipFree(e.second.p);
+}
+
+extern "C" {
+
+const char* rsd_last_error_string(void) { return g_last_error.c_str(); }
+const char* rsd_version(void) { return "residuum-hip 0.1 (gfx950)"; }
+
+int rsd_ctx_create(int device, rsd_ctx** out) {
+    return guard([&] {
+        RSD_CHECK(out, "rsd_ctx_create: null out");
+        auto c = std::make_unique<rsd_ctx>();
+        c->device = device;
+        RSD_HIP(hipSetDevice(device));
+        RSD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        RSD_HIP(hipMalloc(&c->scratch.tickets, sizeof(unsigned) * kTickets));
+        RSD_HIP(hipMemsetAsync(c->scratch.tickets, 0, sizeof(unsigned) * kTickets, c->stream));
+        RSD_HIP(hipMalloc(&c->d_scalars, sizeof(double) * 64));
+        c->partials(1 << 16);
+        RSD_HIP(hipStreamSynchronize(c->stream));
+        *out = c.release();
+    });
+}
+
+int rsd_ctx_destroy(rsd_ctx* ctx) {
+    return guard([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->scratch.partials) (void)hipFree(ctx->scratch.partials);
+        if (ctx->scratch.tickets) (void)hipFree(ctx->scratch.tickets);
+        if (ctx->d_scalars) (void)hipFree(ctx->d_scalars);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int rsd_ctx_sync(rsd_ctx* ctx) {
+    return guard([&] { RSD_CHECK(ctx, "null ctx"); RSD_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int rsd_ctx_stream(rsd_ctx* ctx, void** stream) {
+    return guard([&] { RSD_CHECK(ctx && stream, "null argument"); *stream = (void*)ctx->stream; });
+}
+
+int rsd_malloc(rsd_ctx* ctx, size_t bytes, void** out) {
+    return guard([&] {
+        RSD_CHECK(ctx && out, "rsd_malloc: null argument");
+        *out = nullptr;
+        if (bytes) RSD_HIP(hipMalloc(out, bytes));
+    });
+}
+
+int rsd_free(rsd_ctx* ctx, void* p) {
+    return guard([&] {
+        RSD_CHECK(ctx, "rsd_free: null ctx");
+        if (p) {
+            RSD_HIP(hipStreamSynchronize(ctx->stream));
+            RSD_HIP(hipFree(p));
+        }
+    });
+}
+
+int rsd_memcpy(rsd_ctx* ctx, void* dst, const void* src, size_t bytes, int kind) {
+    return guard([&] {
+        RSD_CHECK(ctx, "rsd_memcpy: null ctx");
+        RSD_CHECK(kind >= 0 && kind <= 3, "rsd_memcpy: bad copy kind");
+        if (!bytes) return;
+        RSD_CHECK(dst && src, "rsd_memcpy: null pointer");
+        static const hipMemcpyKind k[4] = {hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost,
+                                           hipMemcpyDeviceToDevice};
+        RSD_HIP(hipMemcpyAsync(dst, src, bytes, k[kind], ctx->stream));
+        if (kind == RSD_COPY_D2H || kind == RSD_COPY_H2H || kind == RSD_COPY_H2D) RSD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int rsd_memset_zero(rsd_ctx* ctx, void* p, size_t bytes) {
+    return guard([&] {
+        RSD_CHECK(ctx, "null ctx");
+        if (bytes) {
+            RSD_CHECK(p, "rsd_memset_zero: null pointer");
+            RSD_HIP(hipMemsetAsync(p, 0, bytes, ctx->stream));
+        }
+    });
+}
+
+// ------------------------------------------------------------------------
+// problem setup
+// ------------------------------------------------------------------------
+static void face_nodes(int dim, int p, int f, std::vector<int>& out) {
+    out.clear();
+    const int X = p, Y = p, Z = p;
+    if (dim == 2) {
+        if (f == 0) for (int i = 0; i <= Y; ++i) out.push_back(i * (X + 1));
+        if (f == 1) for (int i = 0; i <= Y; ++i) out.push_back(i * (X + 1) + X);
+        if (f == 2) for (int i = 0; i <= X; ++i) out.push_back(i);
+        if (f == 3) for (int i = 0; i <= X; ++i) out.push_back(i + (X + 1) * Y);
+        return;
+    }
+    const int XY = (X + 1) * (Y + 1);
+    for (int j = 0; j <= (f < 4 ? Z : Y); ++j)
+        for (int i = 0; i <= (f < 2 ? Y : X); ++i) {
+            switch (f) {
+            case 0: out.push_back(i * (X + 1) + j * XY); break;
+            case 1: out.push_back(i * (X + 1) + j * XY + X); break;
+            case 2: out.push_back(i + j * XY); break;
+            case 3: out.push_back(i + j * XY + (X + 1) * Y); break;
+            case 4: out.push_back(i + j * (X + 1)); break;
+            default: out.push_back(i + j * (X + 1) + XY * Z); break;
+            }
+        }
+}
+
+static void build_problem(rsd_problem* pb) {
+    const int dim = pb->dim, npe = pb->npe;
+    const int64_t ne = pb->n_elem, nn = pb->n_nodes;
+    hipStream_t st = pb->ctx->stream;
+    // constraints: TopologicalDOF::buildConstraints
+    std::vector<char> con(nn, 0);
+    pb->h_ctag.assign(nn, -1);
+    std::vector<int> fn;
+    for (int64_t e = 0; e < ne; ++e)
+        for (int f = 0; f < pb->nfaces; ++f) {
+            const int32_t tag = pb->h_rng[e * pb->nfaces + f];
+            if (tag < 0) continue;
+            if (std::find(pb->bc_tags.begin(), pb->bc_tags.end(), tag) == pb->bc_tags.end()) continue;
+            face_nodes(dim, pb->p, f, fn);
+            for (int a : fn) {
+                const int32_t g = pb->h_ien[e * npe + a];
+                con[g] = 1;
+                pb->h_ctag[g] = tag;
+            }
+        }
+    pb->h_t2a.assign(nn, -1);
+    pb->h_a2t.clear();
+    for (int64_t i = 0; i < nn; ++i)
+        if (!con[i]) {
+            pb->h_t2a[i] = (int32_t)pb->h_a2t.size();
+            pb->h_a2t.push_back((int32_t)i);
+        }
+    const int64_t n = (int64_t)pb->h_a2t.size();
+    pb->n_free = n;
+    // node -> elements
+    std::vector<int64_t> n2e_ptr(nn + 1, 0);
+    for (int64_t k = 0; k < ne * npe; ++k) n2e_ptr[pb->h_ien[k] + 1]++;
+    for (int64_t i = 0; i < nn; ++i) n2e_ptr[i + 1] += n2e_ptr[i];
+    std::vector<int32_t> n2e(n2e_ptr[nn]);
+    {
+        std::vector<int64_t> cur(n2e_ptr.begin(), n2e_ptr.end() - 1);
+        for (int64_t e = 0; e < ne; ++e)
+            for (int a = 0; a < npe; ++a) n2e[cur[pb->h_ien[e * npe + a]]++] = (int32_t)e;
+    }
+    // pattern: row r = sorted unique free columns of elements containing node(r)
+    std::vector<int32_t> rowlen(n);
+    auto row_cols = [&](int64_t r, std::vector<int32_t>& cols) {
+        cols.clear();
+        const int32_t node = pb->h_a2t[r];
+        for (int64_t k = n2e_ptr[node]; k < n2e_ptr[node + 1]; ++k) {
+            const int64_t e = n2e[k];
+            for (int b = 0; b < npe; ++b) {
+                const int32_t c = pb->h_t2a[pb->h_ien[e * npe + b]];
+                if (c >= 0) cols.push_back(c);
+            }
+        }
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    };
+    parallel_for(n, [&](int64_t lo, int64_t hi) {
+        std::vector<int32_t> cols;
+        for (int64_t r = lo; r < hi; ++r) { row_cols(r, cols); rowlen[r] = (int32_t)cols.size(); }
+    });
+    pb->h_rp.assign(n + 1, 0);
+    for (int64_t r = 0; r < n; ++r) {
+        RSD_CHECK((int64_t)pb->h_rp[r] + rowlen[r] < (int64_t)INT32_MAX, "pattern exceeds int32 nnz");
+        pb->h_rp[r + 1] = pb->h_rp[r] + rowlen[r];
+    }
+    pb->nnz = pb->h_rp[n];
+    pb->h_ci.resize(pb->nnz);
+    parallel_for(n, [&](int64_t lo, int64_t hi) {
+        std::vector<int32_t> cols;
+        for (int64_t r = lo; r < hi; ++r) {
+            row_cols(r, cols);
+            std::copy(cols.begin(), cols.end(), pb->h_ci.begin() + pb->h_rp[r]);
+        }
+    });
+    pb->max_row = n ? *std::max_element(rowlen.begin(), rowlen.end()) : 0;
+    // SELL layout
+    Sell& S = pb->sell;
+    S.n_rows = n;
+    S.n_slices = (n + kSlice - 1) / kSlice;
+    S.h_slice_ptr.assign(S.n_slices + 1, 0);
+    for (int64_t s = 0; s < S.n_slices; ++s) {
+        int32_t w = 0;
+        for (int64_t r = s * kSlice; r < std::min(n, (s + 1) * kSlice); ++r) w = std::max(w, rowlen[r]);
+        S.h_slice_ptr[s + 1] = S.h_slice_ptr[s] + (int64_t)w * kSlice;
+    }
+    S.slots = S.h_slice_ptr[S.n_slices];
+    RSD_CHECK(S.slots < (int64_t)INT32_MAX, "sliced layout exceeds int32 slots");
+    auto slot_of = [&](int64_t r, int64_t j) { return S.h_slice_ptr[r / kSlice] + j * kSlice + (r % kSlice); };
+    std::vector<int32_t> hcol(S.slots);
+    std::vector<int16_t> hoff(S.slots, 0);
+    std::vector<int32_t> hdiag(n, -1);
+    bool fits16 = true;
+    for (int64_t s = 0; s < S.n_slices; ++s)
+        for (int64_t r = s * kSlice; r < (s + 1) * kSlice; ++r) {
+            const int64_t w = (S.h_slice_ptr[s + 1] - S.h_slice_ptr[s]) / kSlice;
+            for (int64_t j = 0; j < w; ++j) {
+                const int64_t sl = S.h_slice_ptr[s] + j * kSlice + (r % kSlice);
+                int32_t c = r < n ? (int32_t)r : 0;
+                if (r < n && j < rowlen[r]) c = pb->h_ci[pb->h_rp[r] + j];
+                hcol[sl] = c;
+                const int64_t off = (int64_t)c - (r < n ? r : 0);
+                if (off < -32768 || off > 32767) fits16 = false;
+                hoff[sl] = (int16_t)off;
+                if (r < n && j < rowlen[r] && c == r) hdiag[r] = (int32_t)sl;
+            }
+        }
+    S.has_off16 = fits16;
+    S.slice_ptr.upload(S.h_slice_ptr.data(), S.n_slices + 1, st);
+    S.row_ptr.upload(pb->h_rp.data(), n + 1, st);
+    S.col.upload(hcol.data(), S.slots, st);
+    if (fits16) S.off16.upload(hoff.data(), S.slots, st);
+    S.diag.upload(hdiag.data(), n, st);
+    // element -> SELL slot map
+    std::vector<int32_t> hmap((size_t)ne * npe * npe);
+    parallel_for(ne, [&](int64_t lo, int64_t hi) {
+        for (int64_t e = lo; e < hi; ++e)
+            for (int a = 0; a < npe; ++a) {
+                const int32_t r = pb->h_t2a[pb->h_ien[e * npe + a]];
+                for (int b = 0; b < npe; ++b) {
+                    const int32_t c = pb->h_t2a[pb->h_ien[e * npe + b]];
+                    int32_t v = -1;
+                    if (r >= 0 && c >= 0) {
+                        const int32_t* beg = pb->h_ci.data() + pb->h_rp[r];
+                        const int32_t* end = pb->h_ci.data() + pb->h_rp[r + 1];
+                        const int32_t* it = std::lower_bound(beg, end, c);
+                        v = (int32_t)slot_of(r, it - beg);
+                    }
+                    hmap[((size_t)e * npe + a) * npe + b] = v;
+                }
+            }
+    });
+    pb->map_sell.upload(hmap.data(), (int64_t)hmap.size(), st);
+    // first-fit greedy colouring in element order, colour-major order
+    pb->h_colour.assign(ne, 0);
+    std::vector<uint64_t> mask(nn, 0);
+    int nc = 0;
+    for (int64_t e = 0; e < ne; ++e) {
+        uint64_t used = 0;
+        for (int a = 0; a < npe; ++a) used |= mask[pb->h_ien[e * npe + a]];
+        int c = 0;
+        while (used & (1ull << c)) ++c;
+        RSD_CHECK(c < 64, "colouring needs more than 64 colours");
+        pb->h_colour[e] = c;
+        nc = std::max(nc, c + 1);
+        for (int a = 0; a < npe; ++a) mask[pb->h_ien[e * npe + a]] |= (1ull << c);
+    }
+    pb->colour_off.assign(nc + 1, 0);
+    for (int64_t e = 0; e < ne; ++e) pb->colour_off[pb->h_colour[e] + 1]++;
+    for (int c = 0; c < nc; ++c) pb->colour_off[c + 1] += pb->colour_off[c];
+    pb->h_order.resize(ne);
+    {
+        std::vector<int64_t> cur(pb->colour_off.begin(), pb->colour_off.end() - 1);
+        for (int64_t e = 0; e < ne; ++e) pb->h_order[cur[pb->h_colour[e]]++] = (int32_t)e;
+    }
+    // device copies
+    pb->xyz.upload(pb->h_xyz.data(), nn * dim, st);
+    pb->ien.upload(pb->h_ien.data(), ne * npe, st);
+    pb->t2a.upload(pb->h_t2a.data(), nn, st);
+    pb->ctag.upload(pb->h_ctag.data(), nn, st);
+    pb->order.upload(pb->h_order.data(), ne, st);
+    pb->gval.alloc(nn);
+    pb->d_bc_tags.upload(pb->bc_tags.data(), (int64_t)pb->bc_tags.size(), st);
+    pb->d_bc_exprs.upload(pb->bc_exprs.data(), (int64_t)pb->bc_exprs.size(), st);
+    RSD_HIP(hipStreamSynchronize(st));
+}
+
+static void init_problem(rsd_ctx* ctx, rsd_problem* pb, int32_t dim, const int32_t* p, int32_t nq, int32_t n_bc,
+                         const int32_t* bc_tags, const char* const* bc_exprs) {
+    RSD_CHECK(dim == 2 || dim == 3, "dim must be 2 or 3");
+    for (int d = 1; d < dim; ++d) RSD_CHECK(p[d] == p[0], "the HIP backend needs equal basis orders per direction");
+    RSD_CHECK(p[0] >= 1 && p[0] <= 3, "basis order outside [1, 3]");
+    RSD_CHECK(nq >= 1 && nq <= 5, "quadrature points outside [1, 5]");
+    pb->ctx = ctx;
+    pb->dim = dim;
+    pb->p = p[0];
+    pb->nq = nq;
+    pb->npe = dim == 2 ? (p[0] + 1) * (p[0] + 1) : (p[0] + 1) * (p[0] + 1) * (p[0] + 1);
+    pb->nfaces = 2 * dim;
+    for (int k = 0; k < n_bc; ++k) {
+        pb->bc_tags.push_back(bc_tags[k]);
+        Expr e = compile_expr(bc_exprs[k] ? bc_exprs[k] : "0");
+        if (e.uses_t) pb->bc_time_dependent = true;
+        pb->bc_exprs.push_back(e);
+    }
+}
+
+int rsd_problem_create_block(rsd_ctx* ctx, int32_t dim, const int64_t* nel, const double* ext, const int32_t* p,
+                             int32_t nq, int32_t n_bc, const int32_t* bc_tags, const char* const* bc_exprs,
+                             rsd_problem** out) {
+    return guard([&] {
+        RSD_CHECK(ctx && nel && ext && p && out, "rsd_problem_create_block: null argument");
+        auto pb = std::make_unique<rsd_problem>();
+        init_problem(ctx, pb.get(), dim, p, nq, n_bc, bc_tags, bc_exprs);
+        const int64_t P = pb->p;
+        const int64_t nx = nel[0], ny = nel[1], nz = dim == 3 ? nel[2] : 1;
+        RSD_CHECK(nx > 0 && ny > 0 && nz > 0, "element counts must be positive");
+        const int64_t NX = nx * P + 1, NY = ny * P + 1, NZ = dim == 3 ? nz * P + 1 : 1;
+        pb->n_nodes = NX * NY * NZ;
+        pb->n_elem = nx * ny * nz;
+        RSD_CHECK(pb->n_nodes < INT32_MAX, "mesh too large for int32 node ids");
+        // nodes: x fastest (BlockMesh2D.cpp:50-71, BlockMesh3D.cpp:50-76)
+        pb->h_xyz.resize(pb->n_nodes * dim);
+        const double dx = (ext[1] - ext[0]) / (double)(nx * P);
+        const double dy = (ext[3] - ext[2]) / (double)(ny * P);
+        const double dz = dim == 3 ? (ext[5] - ext[4]) / (double)(nz * P) : 0.0;
+        int64_t node = 0;
+        for (int64_t k = 0; k < NZ; ++k)
+            for (int64_t j = 0; j < NY; ++j)
+                for (int64_t i = 0; i < NX; ++i, ++node) {
+                    pb->h_xyz[dim * node] = ext[0] + (double)i * dx;
+                    pb->h_xyz[dim * node + 1] = ext[2] + (double)j * dy;
+                    if (dim == 3) pb->h_xyz[dim * node + 2] = ext[4] + (double)k * dz;
+                }
+        // connectivity, local node a -> (a % (P+1), ...) (BlockMesh3D.cpp:78-108)
+        const int npe = pb->npe;
+        pb->h_ien.resize(pb->n_elem * npe);
+        pb->h_rng.resize(pb->n_elem * pb->nfaces);
+        for (int64_t ez = 0; ez < nz; ++ez)
+            for (int64_t ey = 0; ey < ny; ++ey)
+                for (int64_t ex = 0; ex < nx; ++ex) {
+                    const int64_t e = (ez * ny + ey) * nx + ex;
+                    for (int a = 0; a < npe; ++a) {
+                        const int64_t ax = a % (P + 1), ay = (a / (P + 1)) % (P + 1), az = a / ((P + 1) * (P + 1));
+                        pb->h_ien[e * npe + a] = (int32_t)(((ez * P + az) * NY + (ey * P + ay)) * NX + ex * P + ax);
+                    }
+                    int32_t* r = &pb->h_rng[e * pb->nfaces];
+                    r[0] = ex == 0 ? 0 : -1;
+                    r[1] = ex == nx - 1 ? 1 : -1;
+                    r[2] = ey == 0 ? 2 : -1;
+                    r[3] = ey == ny - 1 ? 3 : -1;
+                    if (dim == 3) {
+                        r[4] = ez == 0 ? 4 : -1;
+                        r[5] = ez == nz - 1 ? 5 : -1;
+                    }
+                }
+        build_problem(pb.get());
+        *out = pb.release();
+    });
+}
+
+int rsd_problem_create(rsd_ctx* ctx, int32_t dim, const int32_t* p, int32_t nq, int64_t n_nodes, int64_t n_elem,
+                       const double* xyz, const int64_t* ien, const int32_t* rng, int32_t n_bc, const int32_t* bc_tags,
+                       const char* const* bc_exprs, rsd_problem** out) {
+    return guard([&] {
+        RSD_CHECK(ctx && p && xyz && ien && rng && out, "rsd_problem_create: null argument");
+        auto pb = std::make_unique<rsd_problem>();
+        init_problem(ctx, pb.get(), dim, p, nq, n_bc, bc_tags, bc_exprs);
+        RSD_CHECK(n_nodes > 0 && n_nodes < INT32_MAX && n_elem > 0, "bad mesh sizes");
+        pb->n_nodes = n_nodes;
+        pb->n_elem = n_elem;
+        pb->h_xyz.assign(xyz, xyz + n_nodes * dim);
+        pb->h_ien.resize(n_elem * pb->npe);
+        for (int64_t k = 0; k < n_elem * pb->npe; ++k) {
+            RSD_CHECK(ien[k] >= 0 && ien[k] < n_nodes, "ien entry out of range");
+            pb->h_ien[k] = (int32_t)ien[k];
+        }
+        pb->h_rng.assign(rng, rng + n_elem * pb->nfaces);
+        build_problem(pb.get());
+        *out = pb.release();
+    });
+}
+
+int rsd_problem_destroy(rsd_problem* pb) {
+    return guard([&] {
+        if (!pb) return;
+        (void)hipStreamSynchronize(pb->ctx->stream);
+        delete pb;
+    });
+}
+
+int rsd_problem_sizes(rsd_problem* pb, int64_t* out) {
+    return guard([&] {
+        RSD_CHECK(pb && out, "null argument");
+        out[0] = pb->n_nodes; out[1] = pb->n_elem; out[2] = pb->npe; out[3] = pb->n_free;
+        out[4] = pb->nnz; out[5] = pb->n_colours(); out[6] = pb->sell.slots; out[7] = pb->max_row;
+    });
+}
+
+int rsd_problem_get_mesh(rsd_problem* pb, double* xyz, int64_t* ien, int32_t* rng) {
+    return guard([&] {
+        RSD_CHECK(pb, "null problem");
+        if (xyz) std::copy(pb->h_xyz.begin(), pb->h_xyz.end(), xyz);
+        if (ien) std::copy(pb->h_ien.begin(), pb->h_ien.end(), ien);
+        if (rng) std::copy(pb->h_rng.begin(), pb->h_rng.end(), rng);
+    });
+}
+
+int rsd_problem_get_dofs(rsd_problem* pb, int32_t* t2a, int32_t* a2t, int32_t* ctag) {
+    return guard([&] {
+        RSD_CHECK(pb, "null problem");
+        if (t2a) std::copy(pb->h_t2a.begin(), pb->h_t2a.end(), t2a);
+        if (a2t) std::copy(pb->h_a2t.begin(), pb->h_a2t.end(), a2t);
+        if (ctag) std::copy(pb->h_ctag.begin(), pb->h_ctag.end(), ctag);
+    });
+}
+
+int rsd_problem_get_pattern(rsd_problem* pb, int64_t* rp, int64_t* ci) {
+    return guard([&] {
+        RSD_CHECK(pb, "null problem");
+        if (rp) std::copy(pb->h_rp.begin(), pb->h_rp.end(), rp);
+        if (ci) std::copy(pb->h_ci.begin(), pb->h_ci.end(), ci);
+    });
+}
+
+int rsd_problem_get_elem_map(rsd_problem* pb, int64_t* map) {
+    return guard([&] {
+        RSD_CHECK(pb && map, "null argument");
+        const int npe = pb->npe;
+        parallel_for(pb->n_elem, [&](int64_t lo, int64_t hi) {
+            for (int64_t e = lo; e < hi; ++e)
+                for (int a = 0; a < npe; ++a) {
+                    const int32_t r = pb->h_t2a[pb->h_ien[e * npe + a]];
+                    for (int b = 0; b < npe; ++b) {
+                        const int32_t c = pb->h_t2a[pb->h_ien[e * npe + b]];
+                        int64_t v = -1;
+                        if (r >= 0 && c >= 0) {
+                            const int32_t* beg = pb->h_ci.data() + pb->h_rp[r];
+                            const int32_t* end = pb->h_ci.data() + pb->h_rp[r + 1];
+                            const int32_t* it = std::lower_bound(beg, end, c);
+                            if (it != end && *it == c) v = pb->h_rp[r] + (it - beg);
+                        }
+                        map[(e * npe + a) * npe + b] = v;
+                    }
+                }
+        });
+    });
+}
+
+int rsd_problem_get_colours(rsd_problem* pb, int32_t* colour, int64_t* order) {
+    return guard([&] {
+        RSD_CHECK(pb, "null problem");
+        if (colour) std::copy(pb->h_colour.begin(), pb->h_colour.end(), colour);
+        if (order) std::copy(pb->h_order.begin(), pb->h_order.end(), order);
+    });
+}
+
+int rsd_problem_set_conductivity(rsd_problem* pb, double k_scalar, const char* k_expr, const char* dk_expr,
+                                 const double* tensor, const double* kappa) {
+    return guard([&] {
+        RSD_CHECK(pb, "null problem");
+        Model& m = pb->model;
+        m.k_scalar = k_scalar;
+        m.has_k = k_expr != nullptr;
+        m.has_dk = dk_expr != nullptr;
+        if (k_expr) m.k = compile_expr(k_expr);
+        if (dk_expr) m.dk = compile_expr(dk_expr);
+        m.aniso = tensor != nullptr;
+        const int d = pb->dim;
+        for (int i = 0; i < 9; ++i) m.tensor[i] = 0.0;
+        for (int i = 0; i < d * d; ++i) m.tensor[i] = tensor ? tensor[i] : ((i % (d + 1)) == 0 ? 1.0 : 0.0);
+        if (kappa) pb->kappa.upload(kappa, pb->n_elem, pb->ctx->stream);
+        else pb->kappa.reset();
+        pb->kappa_ext = nullptr;
+        RSD_HIP(hipStreamSynchronize(pb->ctx->stream));
+    });
+}
+
+int rsd_problem_set_mass(rsd_problem* pb, double rho, double cp) {
+    return guard([&] {
+        RSD_CHECK(pb, "null problem");
+        pb->model.rho = rho;
+        pb->model.cp = cp;
+    });
+}
+
+int rsd_problem_set_kappa_device(rsd_problem* pb, const double* d_kappa) {
+    return guard([&] {
+        RSD_CHECK(pb, "null problem");
+        pb->kappa_ext = d_kappa;
+    });
+}
+
+// ------------------------------------------------------------------------
+// matrices
+// ------------------------------------------------------------------------
+int rsd_matrix_create(rsd_problem* pb, rsd_matrix** out) {
+    return guard([&] {
+        RSD_CHECK(pb && out, "null argument");
+        auto m = std::make_unique<rsd_matrix>();
+        m->pb = pb;
+        m->val.alloc(pb->sell.slots);
+        if (pb->sell.slots) RSD_HIP(hipMemsetAsync(m->val.p, 0, sizeof(double) * pb->sell.slots, pb->ctx->stream));
+        RSD_HIP(hipStreamSynchronize(pb->ctx->stream));
+        *out = m.release();
+    });
+}
+
+int rsd_matrix_destroy(rsd_matrix* m) {
+    return guard([&] {
+        if (!m) return;
+        (void)hipStreamSynchronize(m->pb->ctx->stream);
+        delete m;
+    });
+}
+
+int rsd_matrix_zero(rsd_ctx* ctx, rsd_matrix* m) {
+    return guard([&] {
+        RSD_CHECK(ctx && m, "null argument");
+        if (m->val.n) RSD_HIP(hipMemsetAsync(m->val.p, 0, sizeof(double) * m->val.n, ctx->stream));
+    });
+}
+
+static int64_t sell_slot(const rsd_problem* pb, int64_t r, int64_t j) {
+    return pb->sell.h_slice_ptr[r / kSlice] + j * kSlice + (r % kSlice);
+}
+
+int rsd_matrix_get_values(rsd_ctx* ctx, rsd_matrix* m, double* out) {
+    return guard([&] {
+        RSD_CHECK(ctx && m && out, "null argument");
+        const rsd_problem* pb = m->pb;
+        std::vector<double> h(m->val.n);
+        RSD_HIP(hipMemcpyAsync(h.data(), m->val.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+        RSD_HIP(hipStreamSynchronize(ctx->stream));
+        for (int64_t r = 0; r < pb->n_free; ++r)
+            for (int64_t j = 0; j < pb->h_rp[r + 1] - pb->h_rp[r]; ++j) out[pb->h_rp[r] + j] = h[sell_slot(pb, r, j)];
+    });
+}
+
+int rsd_matrix_set_values(rsd_ctx* ctx, rsd_matrix* m, const double* in) {
+    return guard([&] {
+        RSD_CHECK(ctx && m && in, "null argument");
+        const rsd_problem* pb = m->pb;
+        std::vector<double> h(m->val.n, 0.0);
+        for (int64_t r = 0; r < pb->n_free; ++r)
+            for (int64_t j = 0; j < pb->h_rp[r + 1] - pb->h_rp[r]; ++j) h[sell_slot(pb, r, j)] = in[pb->h_rp[r] + j];
+        RSD_HIP(hipMemcpyAsync(m->val.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+        RSD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,355 @@
+"""Python access to libresiduum_hip.so, the MI355X (gfx950) backend for
+Residuum's heat-equation hot path.
+
+A thin ctypes layer over the C-ABI declared in include/residuum_hip.h.
+There is no fallback: if the shared library is missing or no GPU is
+visible, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libresiduum_hip.so")
+
+OK = 0
+COPY_H2H, COPY_H2D, COPY_D2H, COPY_D2D = 0, 1, 2, 3
+DIFFUSION, MASS, TDIFFUSION, TMASS = 1, 2, 3, 4
+SCALED = 0x100
+SEL_VALUE, SEL_RATE, SEL_OPERAND = 0, 1, 2
+PC_IDENTITY, PC_JACOBI = 0, 1
+
+_d = C.POINTER(C.c_double)
+_f = C.POINTER(C.c_float)
+_i64 = C.POINTER(C.c_int64)
+_i32 = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+
+
+class RsdError(RuntimeError):
+    pass
+
+
+class Report(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("iterations", C.c_int64), ("initial_residual", C.c_double),
+                ("final_residual", C.c_double), ("final_residual_rel", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RsdError(f"{LIB_PATH} not built: run __graft_entry__.build() or make -C residuum-solver_amd")
+        L = C.CDLL(LIB_PATH)
+        L.rsd_last_error_string.restype = C.c_char_p
+        L.rsd_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != OK:
+        msg = lib().rsd_last_error_string().decode()
+        raise RsdError(f"residuum_hip error {rc}: {msg}")
+    return rc
+
+
+def dptr(a):
+    return None if a is None else _vp(a.ptr if isinstance(a, DeviceArray) else a)
+
+
+class Ctx:
+    """rsd_ctx: one HIP stream on a device."""
+
+    def __init__(self, device=0):
+        self.h = _vp()
+        check(lib().rsd_ctx_create(C.c_int(device), C.byref(self.h)))
+
+    def sync(self):
+        check(lib().rsd_ctx_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib().rsd_ctx_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- memory -----------------------------------------------------------
+    def empty(self, n, dtype=np.float64):
+        return DeviceArray(self, n, dtype)
+
+    def from_numpy(self, a):
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def zeros(self, n, dtype=np.float64):
+        d = DeviceArray(self, n, dtype)
+        if n:
+            check(lib().rsd_memset_zero(self.h, _vp(d.ptr), C.c_size_t(d.nbytes)))
+        return d
+
+    # -- BLAS-1 -------------------------------------------------------------
+    def dot(self, x, y):
+        out = C.c_double()
+        check(lib().rsd_dot_f64(self.h, C.c_int64(x.n), dptr(x), dptr(y), None, C.byref(out)))
+        return out.value
+
+    def norm2(self, x):
+        out = C.c_double()
+        check(lib().rsd_norm2_f64(self.h, C.c_int64(x.n), dptr(x), None, C.byref(out)))
+        return out.value
+
+    def axpy(self, a, x, y):
+        check(lib().rsd_axpy_f64(self.h, C.c_int64(x.n), C.c_double(a), dptr(x), dptr(y)))
+
+    def axpby(self, a, b, x, y):
+        check(lib().rsd_axpby_f64(self.h, C.c_int64(x.n), C.c_double(a), C.c_double(b), dptr(x), dptr(y)))
+
+    def scal(self, a, x):
+        check(lib().rsd_scal_f64(self.h, C.c_int64(x.n), C.c_double(a), dptr(x)))
+
+    def copy(self, x, y):
+        check(lib().rsd_copy_f64(self.h, C.c_int64(x.n), dptr(x), dptr(y)))
+
+    def mul(self, x, y, z):
+        check(lib().rsd_mul_f64(self.h, C.c_int64(x.n), dptr(x), dptr(y), dptr(z)))
+
+    def fill(self, x, v):
+        if x.dtype == np.float32:
+            check(lib().rsd_fill_f32(self.h, dptr(x), C.c_int64(x.n), C.c_float(v)))
+        else:
+            check(lib().rsd_fill_f64(self.h, dptr(x), C.c_int64(x.n), C.c_double(v)))
+
+
+class DeviceArray:
+    """A device buffer allocated through rsd_malloc."""
+
+    def __init__(self, ctx, n, dtype=np.float64):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = _vp()
+        check(lib().rsd_malloc(ctx.h, C.c_size_t(self.nbytes), C.byref(p)))
+        self.ptr = p.value or 0
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        if self.n:
+            check(lib().rsd_memcpy(self.ctx.h, _vp(self.ptr), a.ctypes.data_as(_vp), C.c_size_t(self.nbytes), COPY_H2D))
+
+    def numpy(self):
+        out = np.empty(self.n, self.dtype)
+        if self.n:
+            check(lib().rsd_memcpy(self.ctx.h, out.ctypes.data_as(_vp), _vp(self.ptr), C.c_size_t(self.nbytes), COPY_D2H))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().rsd_free(self.ctx.h, _vp(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _cstrs(items):
+    arr = (C.c_char_p * max(1, len(items)))()
+    for i, s in enumerate(items):
+        arr[i] = s.encode()
+    return arr
+
+
+def _forms(forms):
+    kinds = (C.c_int32 * max(1, len(forms)))(*[int(f[0]) for f in forms])
+    coefs = (C.c_double * max(1, len(forms)))(*[float(f[1]) for f in forms])
+    return len(forms), kinds, coefs
+
+
+class Problem:
+    """rsd_problem: mesh + DOFs + pattern + element map + colouring in HBM."""
+
+    def __init__(self, ctx, dim, n, p, nq, tags=(), bc_exprs=(), ext=None):
+        self.ctx = ctx
+        self.dim = dim
+        ext = ext if ext is not None else (0.0, 1.0) * dim
+        n3 = (C.c_int64 * 3)(*(list(n) + [1] * (3 - dim)))
+        e6 = (C.c_double * 6)(*(list(ext) + [0.0] * (6 - len(ext))))
+        p3 = (C.c_int32 * 3)(*(list(p) + [1] * (3 - dim)))
+        t = (C.c_int32 * max(1, len(tags)))(*tags)
+        self._exprs = _cstrs(list(bc_exprs))
+        self.h = _vp()
+        check(lib().rsd_problem_create_block(ctx.h, C.c_int32(dim), n3, e6, p3, C.c_int32(nq), C.c_int32(len(tags)),
+                                             t, self._exprs, C.byref(self.h)))
+        s = (C.c_int64 * 8)()
+        check(lib().rsd_problem_sizes(self.h, s))
+        (self.n_nodes, self.n_elem, self.npe, self.n_free, self.nnz, self.n_colours, self.slots,
+         self.max_row) = [int(v) for v in s]
+
+    def close(self):
+        if self.h:
+            lib().rsd_problem_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- getters ------------------------------------------------------------
+    def mesh(self):
+        xyz = np.zeros(self.n_nodes * self.dim)
+        ien = np.zeros(self.n_elem * self.npe, np.int64)
+        rng = np.zeros(self.n_elem * 2 * self.dim, np.int32)
+        check(lib().rsd_problem_get_mesh(self.h, xyz.ctypes.data_as(_d), ien.ctypes.data_as(_i64),
+                                         rng.ctypes.data_as(_i32)))
+        return xyz, ien, rng
+
+    def dofs(self):
+        t2a = np.zeros(self.n_nodes, np.int32)
+        a2t = np.zeros(self.n_free, np.int32)
+        ct = np.zeros(self.n_nodes, np.int32)
+        check(lib().rsd_problem_get_dofs(self.h, t2a.ctypes.data_as(_i32), a2t.ctypes.data_as(_i32),
+                                         ct.ctypes.data_as(_i32)))
+        return t2a, a2t, ct
+
+    def pattern(self):
+        rp = np.zeros(self.n_free + 1, np.int64)
+        ci = np.zeros(self.nnz, np.int64)
+        check(lib().rsd_problem_get_pattern(self.h, rp.ctypes.data_as(_i64), ci.ctypes.data_as(_i64)))
+        return rp, ci
+
+    def elem_map(self):
+        m = np.zeros(self.n_elem * self.npe * self.npe, np.int64)
+        check(lib().rsd_problem_get_elem_map(self.h, m.ctypes.data_as(_i64)))
+        return m
+
+    def colours(self):
+        c = np.zeros(self.n_elem, np.int32)
+        o = np.zeros(self.n_elem, np.int64)
+        check(lib().rsd_problem_get_colours(self.h, c.ctypes.data_as(_i32), o.ctypes.data_as(_i64)))
+        return c, o
+
+    # -- model --------------------------------------------------------------
+    def set_conductivity(self, k_scalar=1.0, k_expr=None, dk_expr=None, tensor=None, kappa=None):
+        tz = None if tensor is None else np.ascontiguousarray(tensor, np.float64)
+        kp = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+        check(lib().rsd_problem_set_conductivity(self.h, C.c_double(k_scalar),
+                                                 k_expr.encode() if k_expr else None,
+                                                 dk_expr.encode() if dk_expr else None,
+                                                 None if tz is None else tz.ctypes.data_as(_d),
+                                                 None if kp is None else kp.ctypes.data_as(_d)))
+
+    def set_mass(self, rho=1.0, cp=1.0):
+        check(lib().rsd_problem_set_mass(self.h, C.c_double(rho), C.c_double(cp)))
+
+    def set_kappa_device(self, d_kappa):
+        check(lib().rsd_problem_set_kappa_device(self.h, dptr(d_kappa)))
+
+    # -- assembly -------------------------------------------------------------
+    def matrix(self):
+        return Matrix(self)
+
+    def vector(self, fill=None):
+        v = self.ctx.empty(self.n_free)
+        if fill is not None:
+            self.ctx.fill(v, fill)
+        return v
+
+    def assemble_matrix(self, K, forms, U=None, Udot=None, time=0.0):
+        nf, kinds, coefs = _forms(forms)
+        check(lib().rsd_assemble_matrix(self.ctx.h, self.h, C.c_double(time), C.c_int32(nf), kinds, coefs,
+                                        dptr(U), dptr(Udot), K.h))
+
+    def assemble_vector(self, F, forms, U=None, Udot=None, operand=None, selector=SEL_VALUE, time=0.0):
+        nf, kinds, coefs = _forms(forms)
+        check(lib().rsd_assemble_vector(self.ctx.h, self.h, C.c_double(time), C.c_int32(nf), kinds, coefs,
+                                        dptr(U), dptr(Udot), dptr(operand), C.c_int32(selector), dptr(F)))
+
+    def assemble_load(self, F, src=None, nodal=None, time=0.0):
+        check(lib().rsd_assemble_load(self.ctx.h, self.h, C.c_double(time), src.encode() if src else None,
+                                      dptr(nodal), dptr(F)))
+
+    def apply_essential(self, F, forms, time=0.0):
+        nf, kinds, coefs = _forms(forms)
+        check(lib().rsd_apply_essential(self.ctx.h, self.h, C.c_double(time), C.c_int32(nf), kinds, coefs, dptr(F)))
+
+    # -- drivers ----------------------------------------------------------------
+    def be_run(self, U, t0, tf, dt, src=None, tol=1e-8, max_iters=1000, precond=PC_JACOBI, use_graph=True,
+               max_steps=None):
+        max_steps = max_steps or int(round((tf - t0) / dt)) + 2
+        iters = np.zeros(max_steps, np.int64)
+        steps = C.c_int64()
+        check(lib().rsd_be_run(self.ctx.h, self.h, C.c_double(t0), C.c_double(tf), C.c_double(dt),
+                               src.encode() if src else None, C.c_double(tol), C.c_int64(max_iters),
+                               C.c_int32(precond), C.c_int32(int(use_graph)), dptr(U), C.c_int64(max_steps),
+                               C.byref(steps), iters.ctypes.data_as(_i64)))
+        return steps.value, iters[:steps.value]
+
+
+class Matrix:
+    """rsd_matrix: values on the problem's pattern (sliced ELL in HBM)."""
+
+    def __init__(self, pb):
+        self.pb = pb
+        self.h = _vp()
+        check(lib().rsd_matrix_create(pb.h, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().rsd_matrix_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def values(self):
+        out = np.zeros(self.pb.nnz)
+        check(lib().rsd_matrix_get_values(self.pb.ctx.h, self.h, out.ctypes.data_as(_d)))
+        return out
+
+    def set_values(self, v):
+        v = np.ascontiguousarray(v, np.float64)
+        check(lib().rsd_matrix_set_values(self.pb.ctx.h, self.h, v.ctypes.data_as(_d)))
+
+    def spmv(self, x, y):
+        check(lib().rsd_spmv_f64(self.pb.ctx.h, self.h, dptr(x), dptr(y)))
+
+    def inv_diag(self, out):
+        check(lib().rsd_jacobi_inv_diag(self.pb.ctx.h, self.h, dptr(out)))
+
+    def cg(self, b, x, tol=1e-10, max_iters=1000, precond=PC_JACOBI, hist=False):
+        rep = Report()
+        h = np.zeros(max_iters + 1) if hist else None
+        check(lib().rsd_cg_solve_f64(self.pb.ctx.h, self.h, dptr(b), dptr(x), C.c_double(tol), C.c_int64(max_iters),
+                                     C.c_int32(precond), C.byref(rep), None if h is None else h.ctypes.data_as(_d)))
+        return (rep, h) if hist else rep
+
+
+def exported_symbols():
+    """Names declared in include/residuum_hip.h (for the load test)."""
+    import re
+    hdr = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include", "residuum_hip.h")
+    text = open(hdr).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(rsd_\w+)\s*\(", text, re.M)))

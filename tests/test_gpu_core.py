@@ -1,0 +1,231 @@
+"""GPU parity of the core path: setup products, SpMV, reductions, assembly,
+CG and the Backward Euler loop, checked against the CPU oracle.
+
+The oracle runs in GPU order (colour-major element loop, the kernels'
+block-tree reductions), so everything here must agree bit for bit; the
+reference-order oracle (itself bitwise equal to the reference's own code,
+see test_oracle_vs_reference.py) bounds the rounding difference.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # dim, n, p, nq, tags, exprs, ext
+    (2, (4, 4), (1, 1), 2, [0, 1, 3], ["x"] * 3, (0.0, 4.0, 0.0, 4.0)),
+    (2, (7, 5), (2, 2), 3, [0, 1, 2, 3], ["(1-x)*(1-y)"] * 4, None),
+    (3, (5, 4, 3), (1, 1, 1), 2, [0, 1], ["1", "0"], None),
+    (3, (6, 6, 6), (1, 1, 1), 2, [4], ["1"], None),
+]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def build(ctx, case, kappa_seed=None):
+    import residuum_hip as rh
+    dim, n, p, nq, tags, exprs, ext = case
+    say("case", case)
+    pb = rh.Problem(ctx, dim, n, p, nq, tags, exprs, ext=ext)
+    oc = O.Case(dim, n, p, nq, tags, exprs, ext=ext, order=O.ORDER_GPU)
+    if kappa_seed is not None:
+        kap = np.exp(0.5 * O.normal_field(kappa_seed, 0, oc.n_elem))
+        oc.kappa = kap
+        oc.set_order(O.ORDER_GPU)
+        pb.set_conductivity(1.0, kappa=kap)
+    return pb, oc
+
+
+def same(a, b):
+    return a.shape == b.shape and np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_setup_products_bitexact(ctx, case):
+    pb, oc = build(ctx, case)
+    xyz, ien, rng = pb.mesh()
+    assert same(xyz, oc.xyz) and same(ien, oc.ien) and same(rng, oc.rng)
+    t2a, a2t, ct = pb.dofs()
+    assert pb.n_free == oc.n_free
+    assert same(t2a, oc.t2a) and same(a2t, oc.a2t) and same(ct, oc.ctag)
+    rp, ci = pb.pattern()
+    assert same(rp, oc.rp) and same(ci, oc.ci)
+    assert same(pb.elem_map(), oc.map)
+    col, order = pb.colours()
+    assert same(col, oc.colour) and same(order, oc.elem_order)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_spmv_bitexact(ctx, case):
+    pb, oc = build(ctx, case)
+    rng = np.random.default_rng(1)
+    val = rng.standard_normal(oc.nnz)
+    x = rng.standard_normal(oc.n_free)
+    K = pb.matrix()
+    K.set_values(val)
+    assert same(K.values(), val)
+    dx, dy = ctx.from_numpy(x), pb.vector()
+    K.spmv(dx, dy)
+    assert same(dy.numpy(), oc.spmv(val, x))
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 256, 257, 5000, 300001])
+def test_dot_norm_bitexact(ctx, n):
+    say("dot n", n)
+    rng = np.random.default_rng(n)
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    dx, dy = ctx.from_numpy(x), ctx.from_numpy(y)
+    assert ctx.dot(dx, dy) == O.dot(x, y, O.ORDER_GPU)
+    assert ctx.norm2(dx) == np.sqrt(O.dot(x, x, O.ORDER_GPU))
+    if n:
+        assert abs(ctx.dot(dx, dy) - np.dot(x, y)) <= 1e-12 * np.abs(x * y).sum()
+
+
+def test_blas1(ctx):
+    rng = np.random.default_rng(3)
+    x, y = rng.standard_normal(1000), rng.standard_normal(1000)
+    dx, dy, dz = ctx.from_numpy(x), ctx.from_numpy(y), ctx.empty(1000)
+    ctx.axpy(0.3, dx, dy)
+    assert same(dy.numpy(), y + 0.3 * x)
+    ctx.axpby(0.7, -1.5, dx, dy)
+    assert same(dy.numpy(), -1.5 * (y + 0.3 * x) + 0.7 * x)
+    ctx.scal(2.5, dx)
+    assert same(dx.numpy(), x * 2.5)
+    ctx.mul(dx, dy, dz)
+    assert same(dz.numpy(), dx.numpy() * dy.numpy())
+    ctx.copy(dz, dx)
+    assert same(dx.numpy(), dz.numpy())
+    ctx.fill(dx, 4.25)
+    assert np.all(dx.numpy() == 4.25)
+
+
+FORMSETS = {
+    "diffusion": [(O.DIFFUSION, 1.0)],
+    "mass": [(O.MASS, 1.0)],
+    "be": [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 1.0 / 3e-3)],
+}
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("fs", sorted(FORMSETS))
+def test_assemble_matrix_bitexact(ctx, case, fs):
+    pb, oc = build(ctx, case, kappa_seed=7)
+    forms = FORMSETS[fs]
+    K = pb.matrix()
+    pb.assemble_matrix(K, forms)
+    got = K.values()
+    assert same(got, oc.assemble_matrix(forms))
+    oc.set_order(O.ORDER_REF)
+    ref = oc.assemble_matrix(forms)
+    assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("case", CASES[:2])
+def test_assemble_jacobian_tdep_bitexact(ctx, case):
+    import residuum_hip as rh
+    dim, n, p, nq, tags, exprs, ext = case
+    tensor = (1.0, 0.2, 0.2, 0.1) if dim == 2 else (1, 0.2, 0, 0.2, 0.1, 0, 0, 0, 1)
+    pb = rh.Problem(ctx, dim, n, p, nq, tags, exprs, ext=ext)
+    pb.set_conductivity(1.0, k_expr="1.0 + 0.5*T", dk_expr="0.5", tensor=tensor)
+    oc = O.Case(dim, n, p, nq, tags, exprs, ext=ext, order=O.ORDER_GPU, k_expr="1.0 + 0.5*T", dk_expr="0.5",
+                tensor=tensor)
+    U = np.random.default_rng(2).standard_normal(oc.n_free)
+    forms = [(O.DIFFUSION, 1.0), (O.TDIFFUSION, 1.0)]
+    K = pb.matrix()
+    pb.assemble_matrix(K, forms, U=ctx.from_numpy(U))
+    assert same(K.values(), oc.assemble_matrix(forms, U=U))
+    R = pb.vector()
+    pb.assemble_vector(R, [(O.DIFFUSION, 1.0)], U=ctx.from_numpy(U))
+    assert same(R.numpy(), oc.assemble_vector([(O.DIFFUSION, 1.0)], U=U))
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_assemble_vectors_bitexact(ctx, case):
+    pb, oc = build(ctx, case, kappa_seed=11)
+    rng = np.random.default_rng(5)
+    U = rng.standard_normal(oc.n_free)
+    dU = ctx.from_numpy(U)
+    F = pb.vector()
+    for forms in ([(O.DIFFUSION, 1.0)], [(O.MASS, 1.0)], [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 250.0)]):
+        say("vector forms", forms)
+        pb.assemble_vector(F, forms, U=dU)
+        assert same(F.numpy(), oc.assemble_vector(forms, U=U))
+        pb.assemble_vector(F, forms, U=dU, operand=dU, selector=O.SEL_OPERAND)
+        assert same(F.numpy(), oc.assemble_vector(forms, U=U, operand=U, selector=O.SEL_OPERAND))
+    src = "x*y*z" if case[0] == 3 else "x*y"
+    pb.assemble_load(F, src=src)
+    assert same(F.numpy(), oc.load(src))
+    nod = rng.random(oc.n_nodes)
+    pb.assemble_load(F, nodal=ctx.from_numpy(nod))
+    assert same(F.numpy(), oc.load(nodal=nod))
+    pb.assemble_load(F, src="10*sin(pi*x)*sin(pi*y)")
+    ref = oc.load("10*sin(pi*x)*sin(pi*y)")
+    assert np.max(np.abs(F.numpy() - ref)) <= 1e-14 * max(1.0, np.max(np.abs(ref)))
+    for forms in ([(O.DIFFUSION, 1.0)], [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 250.0)]):
+        F.upload(U)
+        pb.apply_essential(F, forms)
+        assert same(F.numpy(), oc.lift(forms, U))
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("precond", [0, 1])
+def test_cg_bitexact(ctx, case, precond):
+    pb, oc = build(ctx, case, kappa_seed=3)
+    forms = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 100.0)]
+    val = oc.assemble_matrix(forms)
+    K = pb.matrix()
+    pb.assemble_matrix(K, forms)
+    rng = np.random.default_rng(9)
+    b = rng.standard_normal(oc.n_free)
+    x0 = rng.standard_normal(oc.n_free) * 0.1
+    db, dx = ctx.from_numpy(b), ctx.from_numpy(x0)
+    rep, hist = K.cg(db, dx, tol=1e-10, max_iters=500, precond=precond, hist=True)
+    xo, ro, ho = oc.cg(val, b, x0=x0, tol=1e-10, max_iters=500, precond=precond, hist=True)
+    say("iters", rep.iterations, ro.iterations)
+    assert rep.converged == ro.converged == 1
+    assert rep.iterations == ro.iterations
+    assert same(dx.numpy(), xo)
+    assert same(hist[:rep.iterations + 1], ho[:ro.iterations + 1])
+    assert rep.final_residual == ro.final_residual and rep.initial_residual == ro.initial_residual
+    # against the reference-order oracle (tightly converged)
+    oc.set_order(O.ORDER_REF)
+    xr, rr = oc.cg(oc.assemble_matrix(forms), b, x0=x0, tol=1e-13, max_iters=2000, precond=precond)
+    dx.upload(x0)
+    K.cg(db, dx, tol=1e-13, max_iters=2000, precond=precond)
+    assert np.max(np.abs(dx.numpy() - xr)) <= 1e-12 * np.max(np.abs(xr))
+
+
+def test_cg_maxiter_and_zero_rhs(ctx):
+    pb, oc = build(ctx, CASES[2])
+    K = pb.matrix()
+    pb.assemble_matrix(K, [(O.DIFFUSION, 1.0)])
+    val = oc.assemble_matrix([(O.DIFFUSION, 1.0)])
+    b = np.ones(oc.n_free)
+    dx = pb.vector(0.0)
+    rep = K.cg(ctx.from_numpy(b), dx, tol=1e-14, max_iters=3)
+    xo, ro = oc.cg(val, b, tol=1e-14, max_iters=3)
+    assert rep.converged == 0 and rep.iterations == 3 == ro.iterations
+    assert same(dx.numpy(), xo) and rep.final_residual == ro.final_residual
+    dx = pb.vector(0.0)
+    rep = K.cg(pb.vector(0.0), dx, tol=1e-8)
+    assert rep.converged == 1 and rep.iterations == 0
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_be_run_bitexact(ctx, use_graph):
+    case = (3, (8, 7, 6), (1, 1, 1), 2, [0, 1], ["1", "0"], None)
+    pb, oc = build(ctx, case, kappa_seed=20261201)
+    U = pb.vector(0.0)
+    steps, iters = pb.be_run(U, 0.0, 0.01, 1e-3, tol=1e-8, use_graph=use_graph)
+    out = oc.be_run(0.0, 0.01, 1e-3, cg_tol=1e-8)
+    say("steps", steps, "iters", list(iters), list(out["iters"]))
+    assert steps == out["steps"] == 10
+    assert list(iters) == list(out["iters"])
+    assert same(U.numpy(), out["U"])

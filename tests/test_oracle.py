@@ -1,0 +1,278 @@
+"""The CPU oracle pinned against the reference.
+
+1. Golden vectors written by the reference's own headers
+   (tests/golden/make_golden.py): every product must match bit for bit.
+2. The closed-form values of the reference's fem-tier tests
+   (tests/integration/fem/heateq/cpu/HeatEquationMinimal{2D,3D}.cpp).
+3. Mathematics: exact bilinear / trilinear solutions, the Kirchhoff
+   closed form, 2x2 known systems, eigenmode decay, adjoint vs finite
+   differences.
+4. When oracle/_ref/libref.so is present: live comparisons on fresh cases.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+def case_from(g, **kw):
+    dim = int(g["dim"])
+    return O.Case(dim, tuple(int(v) for v in g["n"]), tuple(int(v) for v in g["p"]), int(g["nq"]),
+                  [int(t) for t in g["tags"]], [O.KIND_EXPR[int(k)] for k in g["kinds"]],
+                  ext=tuple(float(v) for v in g["ext"]), **kw)
+
+
+def same(a, b):
+    return np.asarray(a).shape == np.asarray(b).shape and np.array_equal(a, b)
+
+
+@pytest.fixture(params=GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def golden(request):
+    return dict(np.load(request.param, allow_pickle=False))
+
+
+def test_golden_setup(golden):
+    c = case_from(golden)
+    assert same(c.xyz, golden["xyz"]) and same(c.ien, golden["ien"]) and same(c.rng, golden["rng"])
+    assert same(c.t2a, golden["t2a"]) and same(c.ctag, golden["ctag"])
+    assert same(c.rp, golden["row_ptr"]) and same(c.ci, golden["col_idx"])
+    assert same(c.map, golden["elem_map"])
+
+
+def test_golden_assembly(golden):
+    c = case_from(golden)
+    dim = c.dim
+    assert same(c.assemble_matrix([(O.DIFFUSION, 1.0)]), golden["K"])
+    assert same(c.assemble_matrix([(O.MASS, 1.0)]), golden["M"])
+    be = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 250.0)]
+    assert same(c.assemble_matrix(be), golden["A_be"])
+    src = O.KIND_EXPR[int(golden["src_kind"])]
+    F = c.load(src)
+    assert same(F, golden["F_load"])
+    assert same(c.load(nodal=golden["nodal"]), golden["F_nodal"])
+    assert same(c.lift([(O.DIFFUSION, 1.0)], F), golden["F_lift"])
+    assert same(c.assemble_vector([(O.MASS, 1.0)], U=golden["U"]), golden["MU"])
+    ck = case_from(golden, kappa=golden["kappa"])
+    assert same(ck.assemble_matrix([(O.DIFFUSION, 1.0)]), golden["K_kappa"])
+    assert same(ck.assemble_matrix(be), golden["A_be_kappa"])
+    assert same(ck.assemble_vector([(O.DIFFUSION, 1.0)], U=golden["U"]), golden["KU_kappa"])
+    assert same(ck.lift(be, F), golden["F_lift_be_kappa"])
+    tensor = (1.0, 0.2, 0.0, 0.2, 0.1, 0.0, 0.0, 0.0, 1.0) if dim == 3 else (1.0, 0.2, 0.2, 0.1)
+    ct = case_from(golden, k_expr="1.0 + 0.5*T", dk_expr="0.5", tensor=tensor)
+    assert same(ct.assemble_matrix([(O.DIFFUSION, 1.0), (O.TDIFFUSION, 1.0)], U=golden["U"]), golden["J_tdep"])
+
+
+def test_golden_solvers(golden):
+    ck = case_from(golden, kappa=golden["kappa"])
+    A, b = golden["A_be_kappa"], golden["b"]
+    for pc in (0, 1):
+        x, rep = ck.cg(A, b, tol=1e-10, max_iters=2000, precond=pc)
+        assert rep.iterations == int(golden[f"cg_pc{pc}_iters"])
+        assert same(x, golden[f"cg_pc{pc}_x"])
+        x, rep = ck.gmres(A, b, tol=1e-10, max_iters=2000, krylov=7, precond=pc)
+        assert rep.iterations == int(golden[f"gmres_pc{pc}_iters"])
+        assert same(x, golden[f"gmres_pc{pc}_x"])
+    out = ck.be_run(0.0, 0.05, 0.01, cg_tol=1e-9)
+    assert out["steps"] == int(golden["be_steps"])
+    assert same(out["iters"], golden["be_iters"])
+    assert same(out["U"], golden["be_U"])
+    if "newton_U" in golden:
+        cz = O.Case(ck.dim, ck.n, ck.p, ck.nq, ck.tags, ["0"] * len(ck.tags), ext=ck.ext, k_expr="1.0 + 0.5*T",
+                    dk_expr="0.5", tensor=(1.0, 0.2, 0.2, 0.1))
+        U, its, _, lin = cz.newton(src=O.KIND_EXPR[3], abs_tol=1e-10, rel_tol=1e-10, lin_solver=1, lin_tol=1e-12,
+                                   krylov=30)
+        assert its == int(golden["newton_iters"])
+        assert same(lin, golden["newton_lin_iters"])
+        assert same(U, golden["newton_U"])
+
+
+# ---------------------------------------------------------------------------
+# closed forms from the reference's fem-tier tests
+# ---------------------------------------------------------------------------
+def dense(c, val):
+    A = np.zeros((c.n_free, c.n_free))
+    for i in range(c.n_free):
+        A[i, c.ci[c.rp[i]:c.rp[i + 1]]] = val[c.rp[i]:c.rp[i + 1]]
+    return A
+
+
+def test_fem2d_closed_forms():
+    """HeatEquationMinimal2D.cpp:154-398 (DOF maps, K entries, F, lift)."""
+    c = O.Case(2, (4, 4), (1, 1), 2, [0, 1, 3], ["x"] * 3, ext=(0.0, 4.0, 0.0, 4.0))
+    assert c.n_nodes == 25 and c.n_free == 12
+    expect = {1: 0, 2: 1, 3: 2, 6: 3, 7: 4, 8: 5, 11: 6, 12: 7, 13: 8, 16: 9, 17: 10, 18: 11}
+    for node in range(25):
+        assert c.t2a[node] == expect.get(node, -1)
+    assert c.ctag[10] == 0 and c.ctag[15] == 0 and c.ctag[21] == 3 and c.ctag[9] == 1
+    K = dense(c, c.assemble_matrix([(O.DIFFUSION, 1.0)]))
+    tol = 1e-10
+    assert np.allclose(np.diag(K)[:3], 4 / 3, atol=tol) and np.allclose(np.diag(K)[3:], 8 / 3, atol=tol)
+    for i in range(2):
+        assert abs(K[i, i + 1] + 1 / 6) < tol
+    for i in (3, 6, 7, 9, 10):
+        assert abs(K[i, i + 1] + 1 / 3) < tol
+    for i in range(9):
+        assert abs(K[i, i + 3] + 1 / 3) < tol
+    F = c.load("x*y")
+    assert np.allclose(F, [1 / 6, 1 / 3, 1 / 2, 1, 2, 3, 2, 4, 6, 3, 6, 9], atol=tol)
+    lift = c.lift([(O.DIFFUSION, 1.0)], F) - F
+    assert np.allclose(lift, [0, 0, 2, 0, 0, 4, 0, 0, 4, 1, 2, 17 / 3], atol=tol)
+
+
+def test_fem3d_closed_forms():
+    """HeatEquationMinimal3D.cpp:168-236 (constraints and K entries)."""
+    c = O.Case(3, (4, 4, 4), (1, 1, 1), 2, [0, 1, 2, 3], ["x"] * 4, ext=(0.0, 4.0) * 3)
+    assert c.n_nodes == 125 and c.n_free == 45
+
+    def nid(i, j, k):
+        return 25 * k + 5 * j + i
+    assert c.t2a[nid(2, 2, 2)] >= 0 and c.t2a[nid(0, 2, 2)] < 0 and c.t2a[nid(4, 2, 2)] < 0
+    assert c.ctag[nid(0, 2, 2)] == 0 and c.ctag[nid(4, 2, 2)] == 1
+    assert c.ctag[nid(2, 0, 2)] == 2 and c.ctag[nid(2, 4, 2)] == 3
+    K = dense(c, c.assemble_matrix([(O.DIFFUSION, 1.0)]))
+    a = c.t2a[nid(2, 2, 2)]
+    assert abs(K[a, a] - 8 / 3) < 1e-10
+    assert abs(K[a, c.t2a[nid(3, 2, 2)]]) < 1e-10
+    assert abs(K[a, c.t2a[nid(3, 3, 2)]] + 1 / 6) < 1e-10
+    assert abs(K[a, c.t2a[nid(3, 3, 3)]] + 1 / 12) < 1e-10
+
+
+# ---------------------------------------------------------------------------
+# mathematics
+# ---------------------------------------------------------------------------
+def test_config1_exact_bilinear():
+    """64x64 Q1 with g=(1-x)(1-y): the discrete solution is the nodal
+    interpolant (HeatEquationMinimal2D full-tier test, config 1)."""
+    c = O.Case(2, (64, 64), (1, 1), 2, [0, 1, 2, 3], ["(1-x)*(1-y)"] * 4)
+    U, rep = c.steady(tol=1e-10, max_iters=5000)
+    assert rep.converged
+    xy = c.xyz.reshape(-1, 2)[c.a2t]
+    exact = (1 - xy[:, 0]) * (1 - xy[:, 1])
+    assert np.max(np.abs(U - exact)) < 1e-8
+
+
+def test_trilinear_exact_hex():
+    c = O.Case(3, (8, 8, 8), (1, 1, 1), 2, list(range(6)), ["x+2*y+3*z"] * 6)
+    U, rep = c.steady(tol=1e-12, max_iters=5000)
+    xyz = c.xyz.reshape(-1, 3)[c.a2t]
+    assert np.max(np.abs(U - (xyz[:, 0] + 2 * xyz[:, 1] + 3 * xyz[:, 2]))) < 1e-9
+
+
+def test_kirchhoff_newton():
+    """HeatEquationSteadyNonlinearConductivity2D.cpp:29-64: k(T)=1+0.5T,
+    T=0 at x=0, T=1 at x=1; theta(x) = (-1 + sqrt(1 + 2*0.5*u(x)))/0.5."""
+    c = O.Case(2, (40, 4), (1, 1), 2, [0, 1], ["0.0", "1.0"], k_expr="1.0 + 0.5*T", dk_expr="0.5")
+    U, its, res, _ = c.newton(abs_tol=1e-10, rel_tol=1e-10, lin_solver=0, lin_tol=1e-12, lin_max=5000)
+    assert its > 0
+    x = c.xyz.reshape(-1, 2)[c.a2t][:, 0]
+    u = (1.0 + 0.25) * x  # kirchhoff(1) * s
+    theta = (-1.0 + np.sqrt(1.0 + 2 * 0.5 * u)) / 0.5
+    assert np.max(np.abs(U - theta)) < 1e-6
+
+
+def test_cg_gmres_2x2_known():
+    """tests/unit/linalg/solver/iterative/cg/Solver.cpp: a 2x2 SPD system."""
+    c = O.Case(2, (1, 1), (1, 1), 2, [], [])  # 4 free nodes, a full 4x4 pattern
+    A = np.array([[4.0, 1, 0, 0], [1, 3, 0, 0], [0, 0, 2, 0], [0, 0, 0, 5]])
+    val = np.array([A[i, j] for i in range(4) for j in c.ci[c.rp[i]:c.rp[i + 1]]])
+    b = np.array([1.0, 2.0, 3.0, 4.0])
+    for order in (O.ORDER_REF, O.ORDER_GPU):
+        x, rep = c.cg(val, b, tol=1e-14, order=order)
+        assert np.allclose(x, np.linalg.solve(A, b), atol=1e-13)
+        x, rep = c.gmres(val, b, tol=1e-14, krylov=2, order=order)
+        assert np.allclose(x, np.linalg.solve(A, b), atol=1e-13)
+
+
+def test_jacobi_inverse_diag():
+    c = O.Case(2, (3, 3), (1, 1), 2, [0], ["0"])
+    val = c.assemble_matrix([(O.DIFFUSION, 1.0)])
+    d = np.zeros(c.n_free)
+    O.lib().orc_inv_diag(O.C.c_int64(c.n_free), O.lp(c.rp), O.lp(c.ci), O.dp(val), O.dp(d))
+    assert np.allclose(d, 1.0 / np.diag(dense(c, val)))
+
+
+def test_gpu_order_reduction_is_a_sum():
+    rng = np.random.default_rng(0)
+    for n in (0, 1, 255, 256, 257, 70000):
+        x, y = rng.standard_normal(n), rng.standard_normal(n)
+        assert abs(O.dot(x, y, O.ORDER_GPU) - np.dot(x, y)) <= 1e-12 * (np.abs(x * y).sum() + 1e-300)
+        assert O.dot(x, y, O.ORDER_REF) == sum((x * y).tolist(), 0.0) or n == 0
+
+
+def test_be_eigenmode_decay():
+    """HeatEquationTransientBackwardEuler2D.cpp:38-46,194-251: the sin-sin
+    mode decays like exp(-2 pi^2 t) (within 5%), monotonically."""
+    c = O.Case(2, (32, 32), (1, 1), 2, [0, 1, 2, 3], ["0"] * 4)
+    xy = c.xyz.reshape(-1, 2)[c.a2t]
+    U0 = np.sin(np.pi * xy[:, 0]) * np.sin(np.pi * xy[:, 1])
+    out = c.be_run(0.0, 0.05, 1e-3, U0=U0, cg_tol=1e-12, hist=True)
+    amp = out["U_hist"].max(axis=1)
+    assert np.all(np.diff(amp) < 0)
+    assert abs(amp[-1] / np.exp(-2 * np.pi ** 2 * 0.05) - 1.0) < 0.05
+
+
+def test_be_linear_field_preserved():
+    """BackwardEulerStage.cpp:278-354: a linear steady field stays exact."""
+    c = O.Case(3, (4, 4, 4), (1, 1, 1), 2, list(range(6)), ["x+2*y+3*z"] * 6)
+    xyz = c.xyz.reshape(-1, 3)[c.a2t]
+    U0 = xyz[:, 0] + 2 * xyz[:, 1] + 3 * xyz[:, 2]
+    out = c.be_run(0.0, 0.03, 0.01, U0=U0, cg_tol=1e-12)
+    assert np.max(np.abs(out["U"] - U0)) < 1e-8
+
+
+def test_adjoint_matches_finite_differences():
+    """The discrete adjoint (no reference counterpart) against central
+    differences on a small hex problem."""
+    c = O.Case(3, (4, 4, 4), (1, 1, 1), 2, [4], ["1"])
+    theta = 0.5 * O.normal_field(7, 0, c.n_elem)
+    obs = np.arange(0, c.n_free, 7)
+    tobs = np.linspace(0.1, 0.5, obs.size)
+    c.kappa = np.exp(theta)
+    c.set_order(O.ORDER_REF)
+    loss, grad, _ = c.adjoint(0.0, 5e-3, 4, obs, tobs)
+    rng = np.random.default_rng(1)
+    h = 1e-6
+    for _ in range(3):
+        d = rng.standard_normal(c.n_elem)
+        ls = []
+        for s in (1, -1):
+            c.kappa = np.exp(theta + s * h * d)
+            c.set_order(O.ORDER_REF)
+            ls.append(c.adjoint(0.0, 5e-3, 4, obs, tobs)[0])
+        fd = (ls[0] - ls[1]) / (2 * h)
+        assert abs(fd - grad @ d) <= 1e-6 * max(abs(fd), 1e-12)
+
+
+# ---------------------------------------------------------------------------
+# live comparison with the reference harness (only where it was built)
+# ---------------------------------------------------------------------------
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref/libref.so not built (needs /root/reference)")
+@pytest.mark.parametrize("spec", [
+    (2, (6, 4), (1, 1), 2, [0, 2], [7, 7], None),
+    (2, (3, 4), (2, 2), 3, [1, 3], [2, 2], None),
+    (3, (3, 2, 4), (1, 1, 1), 2, [4, 5], [6, 0], None),
+    (3, (2, 3, 2), (2, 2, 2), 3, [0], [7], None),
+])
+def test_live_reference(spec):
+    dim, n, p, nq, tags, kinds, ext = spec
+    c = O.Case(dim, n, p, nq, tags, [O.KIND_EXPR[k] for k in kinds], ext=ext)
+    r = O.RefCase(dim, n, p, nq, tags, kinds, ext=ext)
+    xyz, ien, rng = r.mesh()
+    assert same(c.xyz, xyz) and same(c.ien, ien) and same(c.rng, rng)
+    t2a, ct = r.dofs()
+    assert same(c.t2a, t2a) and same(c.ctag, ct)
+    rp, ci = r.pattern()
+    assert same(c.rp, rp) and same(c.ci, ci) and same(c.map, r.map())
+    kap = np.exp(np.random.default_rng(2).standard_normal(c.n_elem))
+    c.kappa = kap
+    c.set_order(O.ORDER_REF)
+    A = c.assemble_matrix([(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 40.0)])
+    assert same(A, r.assemble_matrix(2, 40.0, kappa=kap))
+    out = c.be_run(0.0, 0.03, 0.01, cg_tol=1e-9)
+    Ur, st, its = r.be_run(0.0, 0.03, 0.01, tol=1e-9, kappa=kap)
+    assert out["steps"] == st and same(out["iters"], its) and same(out["U"], Ur)
