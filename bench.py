@@ -1,0 +1,212 @@
+"""Benchmark: CG DOF-iterations/s and SpMV HBM GB/s on MI355X, mixed
+precision vs FP64, relative residual 1e-8.
+
+Workload: 3D transient heat on a structured 128^3 Q1 hex mesh of the unit
+cube (2,146,689 nodes, 2,113,407 free DOFs), 2x2x2 Gauss, per-element
+conductivity lognormal(0, 0.5) from Philox seed 20261201, rho*c = 1, T0 = 0,
+T = 1 on x = 0 and T = 0 on x = 1, insulated elsewhere, Backward Euler with
+dt = 1e-3, CG + Jacobi to relative residual 1e-8 warm-started each step.
+A "step" is one Backward Euler step (assembly of K + M/dt, right-hand side,
+essential lift, CG solve), run once in FP64 and once in mixed precision.
+
+Prints one JSON line.  value = mixed-precision CG DOF-iterations per second
+(free DOFs x CG iterations / GPU solve time); the FP64 figure, the
+whole-step rates and the SpMV roofline sit beside it.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "residuum-solver_amd"))
+
+PEAK_HBM_GBS = 8000.0
+
+
+def hip_events():
+    """hipEvent helpers on an explicit stream (the rsd context's stream)."""
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+    hip.hipEventSynchronize.argtypes = [C.c_void_p]
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+
+    def new():
+        e = C.c_void_p()
+        assert hip.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def rec(e, stream):
+        assert hip.hipEventRecord(e, stream) == 0
+
+    def ms(a, b):
+        assert hip.hipEventSynchronize(b) == 0
+        t = C.c_float()
+        assert hip.hipEventElapsedTime(C.byref(t), a, b) == 0
+        return float(t.value)
+    return new, rec, ms
+
+
+def cpu_info():
+    model = platform.processor() or "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(pb, K, b_host, iters):
+    """The oracle's serial PCG (reference order) on the same assembled matrix:
+    a bounded sample of `iters` iterations, timed on one host core."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    rp, ci = pb.pattern()
+    val = K.values()
+    x = np.zeros(pb.n_free)
+    rep = O.OrcReport()
+    t0 = time.perf_counter()
+    O.lib().orc_cg(C.c_int64(pb.n_free), O.lp(rp), O.lp(ci), O.dp(val), O.dp(b_host), O.dp(x), C.c_double(1e-300),
+                   C.c_int64(iters + 1), 1, O.ORDER_REF, C.byref(rep), None)
+    dt = time.perf_counter() - t0
+    return pb.n_free * rep.iterations / dt, rep.iterations, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50, help="timed Backward Euler steps")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128, help="elements per edge")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import residuum_hip as rh
+    new_ev, rec_ev, ev_ms = hip_events()
+    ctx = rh.Ctx(0)
+    stream = C.c_void_p()
+    rh.check(rh.lib().rsd_ctx_stream(ctx.h, C.byref(stream)))
+    n = args.n
+    dt = 1e-3
+    t_setup = time.perf_counter()
+    pb = rh.Problem(ctx, 3, (n, n, n), (1, 1, 1), 2, [0, 1], ["1", "0"])
+    kappa = np.exp(0.5 * rh.philox_normal(20261201, 0, pb.n_elem))
+    pb.set_conductivity(1.0, kappa=kappa)
+    pb.set_mass(1.0, 1.0)
+    t_setup = time.perf_counter() - t_setup
+    nfree, nnz, slots = pb.n_free, pb.nnz, pb.slots
+    print(f"# setup {t_setup:.2f}s: nodes={pb.n_nodes} free={nfree} nnz={nnz} slots={slots} colours={pb.n_colours}",
+          file=sys.stderr, flush=True)
+
+    results = {}
+    for mode in ("fp64", "mixed"):
+        mixed = mode == "mixed"
+        U = pb.vector(0.0)
+        if args.warmup:
+            pb.be_run(U, 0.0, args.warmup * dt, dt, tol=1e-8, max_iters=5000, mixed=mixed)
+        U = pb.vector(0.0)
+        ctx.sync()
+        e0, e1 = new_ev(), new_ev()
+        rec_ev(e0, stream)
+        w0 = time.perf_counter()
+        steps, iters = pb.be_run(U, 0.0, args.steps * dt, dt, tol=1e-8, max_iters=5000, mixed=mixed)
+        rec_ev(e1, stream)
+        total_ms = ev_ms(e0, e1)
+        wall = time.perf_counter() - w0
+        tm = dict(pb.last_timing_ms)
+        its = int(np.sum(iters))
+        results[mode] = dict(steps=steps, iters=its, iters_per_step=[int(v) for v in iters], total_ms=total_ms,
+                             wall_s=wall, solve_ms=tm["solve"], assembly_ms=tm["assembly"],
+                             dof_iters_per_s=nfree * its / (tm["solve"] * 1e-3),
+                             dof_iters_per_s_step=nfree * its / (total_ms * 1e-3), U=U)
+        print(f"# {mode}: steps={steps} iters={its} solve={tm['solve']:.1f}ms assembly={tm['assembly']:.1f}ms "
+              f"total={total_ms:.1f}ms", file=sys.stderr, flush=True)
+    Ud = results["fp64"]["U"].numpy()
+    Um = results["mixed"]["U"].numpy()
+    mixed_vs_fp64_err = float(np.max(np.abs(Um - Ud)) / max(np.max(np.abs(Ud)), 1e-300))
+
+    # SpMV roofline: the dominant kernel of each path, timed alone on the same matrix
+    K = pb.matrix()
+    pb.assemble_matrix(K, [(rh.DIFFUSION, 1.0), (rh.MASS | rh.SCALED, 1.0 / dt)], time=dt)
+    K.to_f32()
+    x64, y64 = pb.vector(1.0), pb.vector()
+    x32, y32 = ctx.empty(nfree, np.float32), ctx.empty(nfree, np.float32)
+    ctx.fill(x32, 1.0)
+    spmv = {}
+    for name, fn, byts in (("spmv_f64", lambda: K.spmv(x64, y64), 12 * nnz + 4 * (nfree + 1) + 16 * nfree),
+                           ("spmv_f32", lambda: K.spmv_f32(x32, y32), 6 * nnz + 4 * (nfree + 1) + 8 * nfree)):
+        for _ in range(3):
+            fn()
+        ctx.sync()
+        a, b = new_ev(), new_ev()
+        rec_ev(a, stream)
+        for _ in range(args.spmv_reps):
+            fn()
+        rec_ev(b, stream)
+        avg_ms = ev_ms(a, b) / args.spmv_reps
+        spmv[name] = dict(avg_us=avg_ms * 1e3, bytes=byts, gbs=byts / (avg_ms * 1e-3) / 1e9)
+        print(f"# {name}: {avg_ms*1e3:.1f} us/launch, {spmv[name]['gbs']:.0f} GB/s algorithmic", file=sys.stderr,
+              flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        b_host = np.ones(nfree)
+        rate, cit, cs = cpu_baseline(pb, K, b_host, args.cpu_iters)
+        model, ncpu = cpu_info()
+        cpu = {"value": rate, "unit": "DOF-iters/s", "cores": 1, "kind": "port",
+               "sample": f"{cit} Jacobi-PCG iterations of the oracle's serial reference-order CG on the same "
+                         f"{n}^3 BE operator ({nfree} DOFs), {cs:.1f} s on 1 of {ncpu} cores ({model})"}
+        print(f"# cpu: {rate:.3e} DOF-iters/s ({cit} iters in {cs:.1f}s)", file=sys.stderr, flush=True)
+
+    mx, fp = results["mixed"], results["fp64"]
+    steps = args.steps
+    line = {
+        "metric": "CG DOF-iters/s + SpMV HBM GB/s, mixed vs FP64 GPU at rel resid 1e-8",
+        "value": mx["dof_iters_per_s"],
+        "unit": "DOF-iters/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": mx["total_ms"] / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 inner CG / fp64 refinement",
+        "data": "synthetic (seeded lognormal conductivity)",
+        "config": {"workload": f"BE heat {n}^3 Q1 hex, dt=1e-3, CG+Jacobi to rel resid 1e-8", "dofs": nfree,
+                   "nnz": nnz, "steps": steps},
+        "fp64": {"dof_iters_per_s": fp["dof_iters_per_s"], "iters": fp["iters"], "solve_ms": fp["solve_ms"],
+                 "assembly_ms": fp["assembly_ms"], "ms_per_step": fp["total_ms"] / steps},
+        "mixed": {"dof_iters_per_s": mx["dof_iters_per_s"], "iters": mx["iters"], "solve_ms": mx["solve_ms"],
+                  "assembly_ms": mx["assembly_ms"], "ms_per_step": mx["total_ms"] / steps},
+        "mixed_over_fp64": {"dof_iters_per_s": mx["dof_iters_per_s"] / fp["dof_iters_per_s"],
+                            "solve_time": fp["solve_ms"] / mx["solve_ms"],
+                            "step_time": fp["total_ms"] / mx["total_ms"]},
+        "mixed_vs_fp64_solution_max_rel_diff": mixed_vs_fp64_err,
+        "spmv": spmv,
+        "roofline": {"bound": "hbm", "kernel": "k_spmv_f32 (SELL-64, fp32 values, int16 offsets)",
+                     "achieved": spmv["spmv_f32"]["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": spmv["spmv_f32"]["gbs"] / PEAK_HBM_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

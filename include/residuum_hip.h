@@ -61,6 +61,7 @@ extern "C" {
 typedef struct rsd_ctx rsd_ctx;
 typedef struct rsd_problem rsd_problem;
 typedef struct rsd_matrix rsd_matrix;
+typedef struct rsd_batch rsd_batch;
 
 /* ---------------------------------------------------------------------- */
 /* context                                                                 */
@@ -90,8 +91,10 @@ int rsd_fill_f32(rsd_ctx* ctx, float* p, int64_t n, float value);
 /* ---------------------------------------------------------------------- */
 /* BLAS-1: linalg/operations/backend/cpu/VectorOps.tpp:7-86                */
 /* dot/norm write the scalar to d_out (device, nullable) and h_out (host,  */
-/* nullable; synchronises).  Reductions use the fixed block tree described */
-/* in DESIGN.md, so results are bitwise reproducible.                      */
+/* nullable; synchronises).  Reductions use a fixed tree: 256-index       */
+/* blocks, 64-lane trees per wavefront, wave sums left to right, block    */
+/* partials gathered per lane and reduced the same way, so results are    */
+/* bitwise reproducible run to run.                                        */
 /* ---------------------------------------------------------------------- */
 int rsd_dot_f64(rsd_ctx* ctx, int64_t n, const double* x, const double* y, double* d_out, double* h_out);
 int rsd_norm2_f64(rsd_ctx* ctx, int64_t n, const double* x, double* d_out, double* h_out);
@@ -182,6 +185,27 @@ typedef struct rsd_report {
 int rsd_cg_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol,
                      int64_t max_iters, int32_t precond, rsd_report* rep, double* hist);
 
+/* gmres::Solver::solve (linalg/solver/iterative/gmres/Solver.tpp:5-170):
+ * right-preconditioned restarted GMRES(krylov) with modified Gram-Schmidt,
+ * Givens rotations, breakdown at 1e-14 and the true residual at every
+ * restart; rep->iterations counts Arnoldi steps. */
+int rsd_gmres_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol,
+                        int64_t max_iters, int32_t krylov, int32_t precond, rsd_report* rep);
+
+/* Mixed precision (no reference counterpart; the reference is FP64 only).
+ * rsd_matrix_to_f32 refreshes the FP32 copy of A's values; rsd_spmv_f32 is
+ * y = A x with FP32 values, FP32 accumulation and int16 row-relative column
+ * offsets when they fit.  rsd_cg_solve_mixed: iterative refinement with an
+ * FP64 outer residual r = b - A x and an FP32 inner PCG whose dot products
+ * accumulate in FP64; stops on the FP64 relative residual
+ * ||r|| / (||r_0|| + 1e-50) < tol.  inner_floor bounds the inner relative
+ * tolerance from below (<= 0: 1e-5).  rep->iterations counts inner
+ * iterations. */
+int rsd_matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A);
+int rsd_spmv_f32(rsd_ctx* ctx, rsd_matrix* A, const float* x, float* y);
+int rsd_cg_solve_mixed(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
+                       int32_t precond, double inner_floor, rsd_report* rep, int64_t* outer_cycles);
+
 /* ---------------------------------------------------------------------- */
 /* assembly (fem/assembly/backend/cpu/Assembler.tpp:86-255,                */
 /* fem/boundary/backend/cpu/BoundaryApplicator.tpp:7-64).  Element kernels  */
@@ -212,11 +236,65 @@ int rsd_apply_essential(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t n_fo
  * (solver/driver/Transient.hpp:21-35), CSR operator, CG + preconditioner.
  * d_U holds the initial free vector and receives the final one.  src_expr
  * nullable.  iters (host, nullable, >= max_steps) receives CG iterations per
- * step.  use_graph != 0 replays each step's CG chunks from a hipGraph.
- * Returns the number of steps in *steps. */
+ * step.  use_graph != 0 replays each step's work from hipGraphs.  precision 0:
+ * FP64 PCG (bitwise reproducible); 1: mixed precision (rsd_cg_solve_mixed,
+ * iters counts inner iterations).  Returns the number of steps in *steps. */
+#define RSD_PREC_FP64 0
+#define RSD_PREC_MIXED 1
 int rsd_be_run(rsd_ctx* ctx, rsd_problem* pb, double t0, double tf, double dt,
                const char* src_expr, double cg_tol, int64_t cg_max_iters, int32_t precond,
-               int32_t use_graph, double* d_U, int64_t max_steps, int64_t* steps, int64_t* iters);
+               int32_t precision, int32_t use_graph, double* d_U, int64_t max_steps, int64_t* steps,
+               int64_t* iters, double* timing_ms);
+/* timing_ms (host, nullable, [2]): GPU time of the assembly phases and of
+ * the solve phases summed over the steps, from HIP events on the stream. */
+
+/* Steady nonlinear solve: NewtonRunner::solve (solver/nonlinear/Newton.hpp:
+ * 32-83) over SteadyStage in nonlinear mode (solver/stage/SteadyStage.hpp:
+ * 60-116): per iteration assemble the load, R = F - K(U) U with Dirichlet
+ * values gathered in, ||R||, stop if ||R|| < abs_tol or ||R||/||R_0|| <
+ * rel_tol, else reassemble J = K + K_T in place, dU = 0, solve J dU = R
+ * (lin_solver 0 CG, 1 GMRES(krylov)), U += dU.  newton_iters receives the
+ * iteration at which it converged (-1 not converged); res_hist (host,
+ * nullable, >= max_newton) receives ||R||, lin_iters (host, nullable) the
+ * inner iterations per Newton step. */
+int rsd_newton_run(rsd_ctx* ctx, rsd_problem* pb, const char* src_expr, double abs_tol, double rel_tol,
+                   int64_t max_newton, int32_t lin_solver, double lin_tol, int64_t lin_max_iters,
+                   int32_t krylov, int32_t precond, double* d_U, int64_t* newton_iters, double* res_hist,
+                   int64_t* lin_iters);
+
+/* Adjoint gradient (no reference counterpart).  Loss 1/2 sum_k (U_N[obs_k]
+ * - t_obs_k)^2 after n_steps linear Backward Euler steps of fixed dt from
+ * d_U0 (free vector) with per-element conductivity d_kappa = exp(theta)
+ * (device [n_elem]); forward and adjoint solves are Jacobi-PCG to cg_tol.
+ * Every U_k is checkpointed in HBM.  want_grad != 0 writes dL/dtheta to
+ * d_grad (device [n_elem]); d_UN (device, nullable) receives U_N.  obs and
+ * t_obs are host arrays of free-DOF indices and values. */
+int rsd_adjoint_grad_logk(rsd_ctx* ctx, rsd_problem* pb, const double* d_kappa, double t0, double dt,
+                          int64_t n_steps, double cg_tol, int64_t cg_max_iters, const double* d_U0,
+                          int64_t n_obs, const int64_t* obs, const double* t_obs, int32_t want_grad,
+                          double* loss, double* d_grad, double* d_UN);
+
+/* Batched PCG (no reference counterpart): many independent systems solved
+ * together.  The batch holds the matrices (not copies; they must outlive
+ * it); vectors are concatenated per problem at the offsets returned by
+ * rsd_batch_layout (each segment padded to a multiple of 256 rows; padding
+ * entries are ignored).  Each problem follows cg::Solver::solve
+ * (cg/Solver.tpp:15-119) and its result is bitwise identical to
+ * rsd_cg_solve_f64 on that problem alone.  reps: host [n_problems]. */
+int rsd_batch_create(rsd_ctx* ctx, int32_t n_problems, rsd_matrix* const* mats, rsd_batch** out);
+int rsd_batch_destroy(rsd_batch* b);
+int rsd_batch_layout(rsd_batch* b, int64_t* total_rows, int64_t* offsets);
+int rsd_cg_solve_batched(rsd_ctx* ctx, rsd_batch* b, const double* rhs, double* x, double tol, int64_t max_iters,
+                         int32_t precond, int32_t use_graph, rsd_report* reps);
+
+/* ---------------------------------------------------------------------- */
+/* seeded synthetic inputs (no reference counterpart)                       */
+/* ---------------------------------------------------------------------- */
+/* Philox4x32-10 (Salmon et al., SC'11), counter (i, stream), key seed;
+ * normal: Box-Muller on two 53-bit uniforms, uniform: the first of them.
+ * Host output [n]. */
+int rsd_philox_normal(uint64_t seed, uint32_t stream, int64_t n, double* out);
+int rsd_philox_uniform(uint64_t seed, uint32_t stream, int64_t n, double* out);
 
 #ifdef __cplusplus
 }

@@ -380,6 +380,55 @@ __global__ __launch_bounds__(kBlock) void k_asm_vector(AsmArgs A, Model m, Expr 
     }
 }
 
+// adjoint contribution per element (no colouring: one output per element):
+// grad[e] += -(kappa_e * lam_e . (Khat_e T_e)), Khat_e the diffusion matrix
+// at unit conductivity, T_e the full gather of A.U, lam_e the free gather
+// of A.operand.
+template<int D, int P, int NQ>
+__global__ __launch_bounds__(kBlock) void k_adj_grad(AsmArgs A, Model m, const double* __restrict__ kappa,
+                                                     double* __restrict__ grad, int64_t n_elem) {
+    constexpr int NPE = Elem<D, P>::NPE;
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n_elem) return;
+    int32_t nodes[NPE];
+    double X[NPE][D];
+    load_element<D, P>(A, e, nodes, X);
+    double Te[NPE], Le[NPE];
+    gather<NPE>(A, nodes, 1, A.U, Te);
+    gather<NPE>(A, nodes, 0, A.operand, Le);
+    double Ke[NPE * NPE];
+#pragma unroll
+    for (int i = 0; i < NPE * NPE; ++i) Ke[i] = 0.0;
+    for (int q = 0; q < (D == 2 ? NQ * NQ : NQ * NQ * NQ); ++q) {
+        QP<D, NPE> Q;
+        qp_eval<D, P, NQ>(q, X, Q, false);
+        Q.T = 0.0; Q.Trate = 0.0;
+        for (int i = 0; i < D; ++i) Q.gradT[i] = 0.0;
+        qp_model<D, NPE>(Q, m, 1.0, false);
+#pragma unroll
+        for (int a = 0; a < NPE; ++a)
+#pragma unroll
+            for (int b = 0; b < NPE; ++b) Ke[a * NPE + b] += form_entry<D, NPE>(RSD_FORM_DIFFUSION, Q, a, b);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) {
+        double v = 0.0;
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) v += Ke[a * NPE + b] * Te[b];
+        s += Le[a] * v;
+    }
+    grad[e] += -(kappa[e] * s);
+}
+
+template<int D, int P, int NQ>
+static void run_adj(rsd_problem* pb, const AsmArgs& A, const Model& unit, const double* kappa, double* grad) {
+    if (pb->n_elem)
+        hipLaunchKernelGGL((k_adj_grad<D, P, NQ>), dim3((unsigned)nblocks(pb->n_elem)), dim3(kBlock), 0,
+                           pb->ctx->stream, A, unit, kappa, grad, pb->n_elem);
+    RSD_HIP(hipGetLastError());
+}
+
 template<int D, int P, int NQ>
 static void run_matrix(rsd_problem* pb, const AsmArgs& A) {
     hipStream_t st = pb->ctx->stream;
@@ -515,6 +564,23 @@ void apply_essential(rsd_problem* pb, double time, int32_t nf, const int32_t* ki
     A.time = time;
     update_gval(pb, time);
     dispatch_vector(pb, A, Expr{}, 2);
+}
+
+void adjoint_grad_step(rsd_problem* pb, double time, const double* U, const double* lam, const double* kappa,
+                       double* grad) {
+    AsmArgs A = base_args(pb);
+    A.U = U;
+    A.operand = lam;
+    A.time = time;
+    update_gval(pb, time);
+    Model unit = pb->model;
+    unit.k_scalar = 1.0;
+    unit.has_k = 0;
+    unit.has_dk = 0;
+    if (pb->dim == 2 && pb->p == 1 && pb->nq == 2) { run_adj<2, 1, 2>(pb, A, unit, kappa, grad); return; }
+    if (pb->dim == 2 && pb->p == 2 && pb->nq == 3) { run_adj<2, 2, 3>(pb, A, unit, kappa, grad); return; }
+    if (pb->dim == 3 && pb->p == 1 && pb->nq == 2) { run_adj<3, 1, 2>(pb, A, unit, kappa, grad); return; }
+    throw Error(RSD_ERR_ARG, "adjoint gradient: discretisation not supported (Q1/Q2 quad, Q1 hex, nq = p + 1)");
 }
 
 const Expr* cached_source(rsd_problem* pb, const char* text) {

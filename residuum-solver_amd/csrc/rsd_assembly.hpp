@@ -35,5 +35,7 @@ void assemble_vector(rsd_problem* pb, double time, int32_t nf, const int32_t* ki
 void assemble_load(rsd_problem* pb, double time, const Expr* src, const double* nodal, double* F);
 void apply_essential(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs, double* F);
 const Expr* cached_source(rsd_problem* pb, const char* text);
+void adjoint_grad_step(rsd_problem* pb, double time, const double* U, const double* lam, const double* kappa,
+                       double* grad);
 
 }  // namespace rsd

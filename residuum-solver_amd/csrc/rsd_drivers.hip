@@ -10,6 +10,8 @@
 // would be frozen into the graph); otherwise the step runs eagerly.
 #include "rsd_assembly.hpp"
 #include "rsd_linalg.hpp"
+#include "rsd_mixed.hpp"
+#include "rsd_gmres.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -79,8 +81,43 @@ void capture(hipStream_t st, Graph& G, F&& body) {
 constexpr int64_t kChunk = 8;
 
 // returns steps taken; throws on a failed solve
+struct GraphChunk : ChunkRunner {
+    hipStream_t st;
+    Graph* g;
+    void run() override { RSD_HIP(hipGraphLaunch(g->e, st)); }
+};
+
+struct Events {
+    hipEvent_t a = nullptr, b = nullptr, c = nullptr;
+    bool on = false;
+    double asm_ms = 0.0, solve_ms = 0.0;
+    explicit Events(bool enable) : on(enable) {
+        if (!on) return;
+        RSD_HIP(hipEventCreate(&a));
+        RSD_HIP(hipEventCreate(&b));
+        RSD_HIP(hipEventCreate(&c));
+    }
+    ~Events() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        if (c) (void)hipEventDestroy(c);
+    }
+    void rec(hipEvent_t e, hipStream_t st) { if (on) RSD_HIP(hipEventRecord(e, st)); }
+    void accumulate() {
+        if (!on) return;
+        RSD_HIP(hipEventSynchronize(c));
+        float x = 0.f, y = 0.f;
+        RSD_HIP(hipEventElapsedTime(&x, a, b));
+        RSD_HIP(hipEventElapsedTime(&y, b, c));
+        asm_ms += x;
+        solve_ms += y;
+    }
+};
+
 int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const char* src_text, double tol,
-               int64_t max_iters, int32_t precond, bool use_graph, double* U, int64_t max_steps, int64_t* iters) {
+               int64_t max_iters, int32_t precond, int32_t precision, bool use_graph, double* U, int64_t max_steps,
+               int64_t* iters, double* timing) {
+    Events ev(timing != nullptr);
     rsd_ctx* ctx = pb->ctx;
     const int64_t n = pb->n_free;
     hipStream_t st = ctx->stream;
@@ -94,10 +131,16 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
     const bool graphable = use_graph && !pb->bc_time_dependent && !(src && src->uses_t);
     CGWork w = cg_workspace(ctx, n, max_iters, false);
     ctx->partials(2 * std::max<int64_t>(1, nblocks(n)));
+    const bool mixed = precision == RSD_PREC_MIXED;
+    MixedWork mw;
+    if (mixed) {
+        mw = mixed_workspace(ctx, n);
+        K.val32.alloc(pb->sell.slots);
+    }
     update_gval(pb, t0);
     launch_copy(ctx, n, U, b.Uprev);
     Pinned pin;
-    Graph step_graph, chunk_graph;
+    Graph step_graph, start_graph, chunk_graph;
     double graph_dt = -1.0;
     double time = t0;
     int64_t step = 0;
@@ -105,10 +148,42 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
         RSD_CHECK(step < max_steps, "be_run: more steps than max_steps");
         const double dt = (dt_policy < tf - time) ? dt_policy : (tf - time);
         const double ct = time + dt;
+        if (mixed) {
+            ev.rec(ev.a, st);
+            if (graphable) {
+                if (!step_graph.e || graph_dt != dt) {
+                    capture(st, step_graph, [&] { be_assemble(pb, &K, b, U, ct, dt, src); });
+                    graph_dt = dt;
+                }
+                if (!chunk_graph.e)
+                    capture(st, chunk_graph, [&] {
+                        for (int64_t k = 0; k < kChunk; ++k) mixed_launch_inner_iter(ctx, &K, precond, mw);
+                    });
+                RSD_HIP(hipGraphLaunch(step_graph.e, st));
+            } else {
+                be_assemble(pb, &K, b, U, ct, dt, src);
+            }
+            ev.rec(ev.b, st);
+            GraphChunk gc;
+            gc.st = st;
+            gc.g = &chunk_graph;
+            const MixedState ms = mixed_run(ctx, &K, b.F, U, tol, max_iters, precond, 1e-5, false, mw,
+                                            graphable ? &gc : nullptr);
+            ev.rec(ev.c, st);
+            ev.accumulate();
+            if (iters) iters[step] = ms.total_inner;
+            if (!ms.converged)
+                throw Error(RSD_ERR_NOT_CONVERGED, "be_run: mixed CG did not converge in step " + std::to_string(step));
+            launch_copy(ctx, n, U, b.Uprev);
+            time += dt;
+            ++step;
+            continue;
+        }
+        ev.rec(ev.a, st);
         if (graphable) {
             if (!step_graph.e || graph_dt != dt) {
-                capture(st, step_graph, [&] {
-                    be_assemble(pb, &K, b, U, ct, dt, src);
+                capture(st, step_graph, [&] { be_assemble(pb, &K, b, U, ct, dt, src); });
+                capture(st, start_graph, [&] {
                     cg_launch_init(ctx, &K, b.F, U, tol, max_iters, precond, w);
                     for (int64_t k = 0; k < kChunk; ++k) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);
                 });
@@ -119,8 +194,11 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
                     for (int64_t k = 0; k < kChunk; ++k) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);
                 });
             RSD_HIP(hipGraphLaunch(step_graph.e, st));
+            ev.rec(ev.b, st);
+            RSD_HIP(hipGraphLaunch(start_graph.e, st));
         } else {
             be_assemble(pb, &K, b, U, ct, dt, src);
+            ev.rec(ev.b, st);
             cg_launch_init(ctx, &K, b.F, U, tol, max_iters, precond, w);
             for (int64_t k = 0; k < kChunk; ++k) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);
         }
@@ -134,6 +212,8 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
             else for (int64_t k = 0; k < kChunk; ++k) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);
             launched += kChunk;
         }
+        ev.rec(ev.c, st);
+        ev.accumulate();
         const CGState s = *pin.p;
         if (iters) iters[step] = s.converged ? s.iter : max_iters;
         if (!s.converged) throw Error(RSD_ERR_NOT_CONVERGED, "be_run: CG did not converge in step " + std::to_string(step));
@@ -142,7 +222,65 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
         ++step;
     }
     RSD_HIP(hipStreamSynchronize(st));
+    if (timing) {
+        timing[0] = ev.asm_ms;
+        timing[1] = ev.solve_ms;
+    }
     return step;
+}
+
+// NewtonRunner::solve over SteadyStage (Newton.hpp:32-83, SteadyStage.hpp:60-116)
+int64_t newton_run(rsd_problem* pb, const char* src_text, double abs_tol, double rel_tol, int64_t max_newton,
+                   int32_t lin_solver, double lin_tol, int64_t lin_max, int32_t krylov, int32_t precond, double* U,
+                   double* res_hist, int64_t* lin_iters) {
+    rsd_ctx* ctx = pb->ctx;
+    const int64_t n = pb->n_free;
+    hipStream_t st = ctx->stream;
+    const size_t vb = sizeof(double) * (size_t)std::max<int64_t>(n, 1);
+    double* F = (double*)ctx->ws("nt.F", vb);
+    double* R = (double*)ctx->ws("nt.R", vb);
+    double* dU = (double*)ctx->ws("nt.dU", vb);
+    double* nrm = (double*)ctx->ws("nt.norm", sizeof(double));
+    rsd_matrix J;
+    J.pb = pb;
+    J.val.alloc(pb->sell.slots);
+    const Expr* src = cached_source(pb, src_text);
+    const int32_t kdiff[1] = {RSD_FORM_DIFFUSION};
+    const int32_t kjac[2] = {RSD_FORM_DIFFUSION, RSD_FORM_TDIFFUSION};
+    const double one[2] = {1.0, 1.0};
+    double res0 = 0.0;
+    for (int64_t it = 0; it < max_newton; ++it) {
+        assemble_load(pb, 0.0, src, nullptr, F);
+        assemble_vector(pb, 0.0, 1, kdiff, one, U, nullptr, nullptr, RSD_SEL_VALUE, R);
+        launch_axpby(ctx, n, 1.0, -1.0, F, R);
+        launch_dot(ctx, n, R, R, nrm, true);
+        double rn = 0.0;
+        RSD_HIP(hipMemcpyAsync(&rn, nrm, sizeof(double), hipMemcpyDeviceToHost, st));
+        RSD_HIP(hipStreamSynchronize(st));
+        if (res_hist) res_hist[it] = rn;
+        if (it == 0) res0 = rn;
+        const double rr = res0 > 0.0 ? rn / res0 : 0.0;
+        if (rn < abs_tol || rr < rel_tol) return it;
+        assemble_matrix(pb, 0.0, 2, kjac, one, U, nullptr, &J);
+        launch_fill(ctx, n, dU, 0.0);
+        int64_t li = 0;
+        bool ok;
+        if (lin_solver == 0) {
+            CGWork w = cg_workspace(ctx, n, lin_max, false);
+            const CGState s = cg_run(ctx, &J, R, dU, lin_tol, lin_max, precond, w);
+            ok = s.converged;
+            li = s.converged ? s.iter : lin_max;
+        } else {
+            const GMRESResult g = gmres_run(ctx, &J, R, dU, lin_tol, lin_max, krylov, precond);
+            ok = g.converged;
+            li = g.iterations;
+        }
+        if (lin_iters) lin_iters[it] = li;
+        if (!ok) return -2;
+        launch_axpy(ctx, n, 1.0, dU, U);
+    }
+    RSD_HIP(hipStreamSynchronize(st));
+    return -1;
 }
 
 }  // namespace rsd
@@ -152,17 +290,35 @@ using namespace rsd;
 extern "C" {
 
 int rsd_be_run(rsd_ctx* ctx, rsd_problem* pb, double t0, double tf, double dt, const char* src_expr, double cg_tol,
-               int64_t cg_max_iters, int32_t precond, int32_t use_graph, double* d_U, int64_t max_steps,
-               int64_t* steps, int64_t* iters) {
+               int64_t cg_max_iters, int32_t precond, int32_t precision, int32_t use_graph, double* d_U,
+               int64_t max_steps, int64_t* steps, int64_t* iters, double* timing_ms) {
     int64_t taken = 0;
     const int rc = guard([&] {
         RSD_CHECK(ctx && pb && d_U, "rsd_be_run: null argument");
         RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
         RSD_CHECK(dt > 0.0 && tf >= t0, "rsd_be_run: need dt > 0 and tf >= t0");
         RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
-        taken = be_run(pb, t0, tf, dt, src_expr, cg_tol, cg_max_iters, precond, use_graph != 0, d_U, max_steps, iters);
+        RSD_CHECK(precision == RSD_PREC_FP64 || precision == RSD_PREC_MIXED, "unknown precision");
+        taken = be_run(pb, t0, tf, dt, src_expr, cg_tol, cg_max_iters, precond, precision, use_graph != 0, d_U,
+                       max_steps, iters, timing_ms);
     });
     if (steps) *steps = taken;
+    return rc;
+}
+
+int rsd_newton_run(rsd_ctx* ctx, rsd_problem* pb, const char* src_expr, double abs_tol, double rel_tol,
+                   int64_t max_newton, int32_t lin_solver, double lin_tol, int64_t lin_max_iters, int32_t krylov,
+                   int32_t precond, double* d_U, int64_t* newton_iters, double* res_hist, int64_t* lin_iters) {
+    int64_t its = -1;
+    const int rc = guard([&] {
+        RSD_CHECK(ctx && pb && d_U, "rsd_newton_run: null argument");
+        RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
+        RSD_CHECK(lin_solver == 0 || lin_solver == 1, "lin_solver must be 0 (CG) or 1 (GMRES)");
+        RSD_CHECK(krylov >= 1 && krylov <= 512, "krylov dimension outside [1, 512]");
+        its = newton_run(pb, src_expr, abs_tol, rel_tol, max_newton, lin_solver, lin_tol, lin_max_iters, krylov,
+                         precond, d_U, res_hist, lin_iters);
+    });
+    if (newton_iters) *newton_iters = its;
     return rc;
 }
 

@@ -1,0 +1,359 @@
+// rsd_mixed.hip -- mixed-precision CG: FP32 bulk, FP64 refinement.
+//
+// Iterative refinement around an FP32 preconditioned CG:
+//   outer (FP64):  r = b - A x with the FP64 matrix, ||r|| in FP64;
+//                  stop when ||r|| / (||r_0|| + 1e-50) < tol, the same
+//                  relative criterion as cg/Solver.tpp:45-48;
+//   inner (FP32):  solve A d = r / ||r|| with FP32 values (and int16
+//                  row-relative column offsets when every offset fits),
+//                  FP32 vectors and FP32-accumulated SpMV rows; the dot
+//                  products behind alpha / beta are accumulated in FP64;
+//   update (FP64): x += ||r|| d.
+// Per inner iteration the matrix stream is 6 bytes per slot instead of 12.
+#include "rsd_mixed.hpp"
+#include "rsd_rowdot.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace rsd {
+
+template<bool OFF16>
+__device__ __forceinline__ float row_dot_f32(int64_t r, const int64_t* __restrict__ slice_ptr,
+                                             const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                             const int16_t* __restrict__ off, const float* __restrict__ val,
+                                             const float* __restrict__ x) {
+    const int64_t base = slice_ptr[r >> 6] + (r & 63);
+    const int len = rp[r + 1] - rp[r];
+    float sum = 0.0f;
+    int j = 0;
+    for (; j + 4 <= len; j += 4) {
+        const int64_t s0 = base + (int64_t)j * kSlice;
+        int c[4];
+        float v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            c[k] = OFF16 ? (int)r + (int)off[s0 + k * kSlice] : col[s0 + k * kSlice];
+            v[k] = val[s0 + k * kSlice];
+        }
+        float xv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) xv[k] = x[c[k]];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sum = __builtin_fmaf(v[k], xv[k], sum);
+    }
+    for (; j < len; ++j) {
+        const int64_t s0 = base + (int64_t)j * kSlice;
+        const int c = OFF16 ? (int)r + (int)off[s0] : col[s0];
+        sum = __builtin_fmaf(val[s0], x[c], sum);
+    }
+    return sum;
+}
+
+__global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+template<bool OFF16>
+__global__ __launch_bounds__(kBlock) void k_spmv_f32(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
+                                                     const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                     const int16_t* __restrict__ off, const float* __restrict__ val,
+                                                     const float* __restrict__ x, float* __restrict__ y) {
+    const int64_t r = xcd_block(blockIdx.x, nb) * kBlock + threadIdx.x;
+    if (r < n) y[r] = row_dot_f32<OFF16>(r, slice_ptr, rp, col, off, val, x);
+}
+
+// outer residual r = b - A x (FP64), ||r||, inner right-hand side r/||r||
+__global__ __launch_bounds__(kBlock) void k_mx_residual(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
+                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                        const double* __restrict__ val, const double* __restrict__ b,
+                                                        const double* __restrict__ x, double* __restrict__ r64,
+                                                        MixedState* st, double* partials, unsigned* ticket) {
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    double v[1] = {0.0};
+    if (i < n) {
+        const double ri = b[i] + -1.0 * row_dot_f64(i, slice_ptr, rp, col, val, x);
+        r64[i] = ri;
+        v[0] = ri * ri;
+    }
+    double out[1];
+    if (grid_reduce<1>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) {
+        const double res = sqrt(out[0]);
+        if (st->cycles == 0) st->res0 = res;
+        st->res = res;
+    }
+}
+
+// inner set-up: r32 = r64 / ||r||, d = 0, z = M r, p = z
+__global__ __launch_bounds__(kBlock) void k_mx_init(int64_t n, int64_t nb, const double* __restrict__ r64,
+                                                    float* __restrict__ r, float* __restrict__ p,
+                                                    float* __restrict__ d, const float* __restrict__ invd,
+                                                    MixedState* st, double* partials, unsigned* ticket) {
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    const double s = st->res > 0.0 ? 1.0 / st->res : 0.0;
+    double v[2] = {0.0, 0.0};
+    if (i < n) {
+        const float ri = (float)(r64[i] * s);
+        const float zi = invd ? ri * invd[i] : ri;
+        r[i] = ri;
+        p[i] = zi;
+        d[i] = 0.0f;
+        v[0] = (double)ri * (double)ri;
+        v[1] = (double)ri * (double)zi;
+    }
+    double out[2];
+    if (grid_reduce<2>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) {
+        st->in_res0 = sqrt(out[0]);
+        st->rz_old = out[1];
+        st->in_iter = 0;
+        st->stop = (st->in_res0 == 0.0) ? 1 : 0;
+        st->xdone = st->stop;
+    }
+}
+
+template<bool OFF16>
+__global__ __launch_bounds__(kBlock) void k_mx_spmv_pap(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
+                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                        const int16_t* __restrict__ off, const float* __restrict__ val,
+                                                        const float* __restrict__ p, float* __restrict__ Ap,
+                                                        MixedState* st, double* partials, unsigned* ticket) {
+    if (st->stop) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t r = lb * kBlock + threadIdx.x;
+    double v[1] = {0.0};
+    if (r < n) {
+        const float y = row_dot_f32<OFF16>(r, slice_ptr, rp, col, off, val, p);
+        Ap[r] = y;
+        v[0] = (double)p[r] * (double)y;
+    }
+    double out[1];
+    if (grid_reduce<1>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) st->alpha = st->rz_old / out[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_mx_update_r(int64_t n, int64_t nb, const float* __restrict__ Ap,
+                                                        float* __restrict__ r, const float* __restrict__ invd,
+                                                        MixedState* st, double* partials, unsigned* ticket) {
+    if (st->stop) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    const float alpha = (float)st->alpha;
+    double v[2] = {0.0, 0.0};
+    if (i < n) {
+        const float ri = __builtin_fmaf(-alpha, Ap[i], r[i]);
+        r[i] = ri;
+        const float zi = invd ? ri * invd[i] : ri;
+        v[0] = (double)ri * (double)ri;
+        v[1] = (double)ri * (double)zi;
+    }
+    double out[2];
+    if (grid_reduce<2>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) {
+        const int64_t k = st->in_iter + 1;
+        st->in_iter = k;
+        st->total_inner += 1;
+        const double res = sqrt(out[0]);
+        if (res / (st->in_res0 + 1e-50) < st->in_tol || !(res == res)) {
+            st->stop = 1;
+        } else {
+            st->beta = out[1] / st->rz_old;
+            st->rz_old = out[1];
+            if (k >= st->in_max) st->stop = 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mx_update_xp(int64_t n, int64_t nb, float* __restrict__ d,
+                                                         float* __restrict__ p, const float* __restrict__ r,
+                                                         const float* __restrict__ invd, MixedState* st,
+                                                         unsigned* ticket) {
+    if (st->xdone) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    const float alpha = (float)st->alpha, beta = (float)st->beta;
+    const int32_t stop = st->stop;
+    if (i < n) {
+        const float pi = p[i];
+        d[i] = __builtin_fmaf(alpha, pi, d[i]);
+        if (!stop) {
+            const float zi = invd ? r[i] * invd[i] : r[i];
+            p[i] = __builtin_fmaf(beta, pi, zi);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned t = atomicAdd(ticket, 1u);
+        if (t == (unsigned)(nb - 1)) {
+            st->xdone = stop;
+            *ticket = 0u;
+        }
+    }
+}
+
+__global__ void k_mx_update_x(int64_t n, double* __restrict__ x, const float* __restrict__ d, const MixedState* st) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] += st->res * (double)d[i];
+}
+
+__global__ void k_inv_diag_f32(int64_t n, const int32_t* __restrict__ diag, const double* __restrict__ val,
+                               float* __restrict__ inv) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int32_t s = diag[i];
+    const double dd = s >= 0 ? val[s] : 0.0;
+    inv[i] = (float)((fabs(dd) > 1e-300) ? 1.0 / dd : 1.0);
+}
+
+// ------------------------------------------------------------------------
+void matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A) {
+    if (A->val32.n != A->val.n) A->val32.alloc(A->val.n);
+    if (A->val.n)
+        hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nblocks(A->val.n)), dim3(kBlock), 0, ctx->stream, A->val.n,
+                           A->val.p, A->val32.p);
+}
+
+void launch_spmv_f32(rsd_ctx* ctx, const rsd_matrix* A, const float* x, float* y) {
+    const Sell& S = A->pb->sell;
+    const int64_t n = S.n_rows, nb = nblocks(n);
+    if (!n) return;
+    if (S.has_off16)
+        hipLaunchKernelGGL(k_spmv_f32<true>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, S.slice_ptr.p,
+                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, x, y);
+    else
+        hipLaunchKernelGGL(k_spmv_f32<false>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, S.slice_ptr.p,
+                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, x, y);
+}
+
+MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n) {
+    MixedWork w;
+    const size_t f = sizeof(float) * (size_t)std::max<int64_t>(n, 1), dd = sizeof(double) * (size_t)std::max<int64_t>(n, 1);
+    w.r64 = (double*)ctx->ws("mx.r64", dd);
+    w.r = (float*)ctx->ws("mx.r", f);
+    w.p = (float*)ctx->ws("mx.p", f);
+    w.Ap = (float*)ctx->ws("mx.Ap", f);
+    w.d = (float*)ctx->ws("mx.d", f);
+    w.invd = (float*)ctx->ws("mx.invd", f);
+    w.st = (MixedState*)ctx->ws("mx.state", sizeof(MixedState));
+    return w;
+}
+
+void mixed_launch_inner_iter(rsd_ctx* ctx, const rsd_matrix* A, int32_t precond, const MixedWork& w) {
+    const Sell& S = A->pb->sell;
+    const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n));
+    double* part = ctx->partials(2 * nb);
+    const float* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
+    if (S.has_off16)
+        hipLaunchKernelGGL(k_mx_spmv_pap<true>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb,
+                           S.slice_ptr.p, S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.p, w.Ap, w.st, part,
+                           ctx->scratch.tickets);
+    else
+        hipLaunchKernelGGL(k_mx_spmv_pap<false>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb,
+                           S.slice_ptr.p, S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.p, w.Ap, w.st, part,
+                           ctx->scratch.tickets);
+    hipLaunchKernelGGL(k_mx_update_r, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, w.Ap, w.r, invd, w.st,
+                       part, ctx->scratch.tickets);
+    hipLaunchKernelGGL(k_mx_update_xp, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, w.d, w.p, w.r, invd,
+                       w.st, ctx->scratch.tickets);
+}
+
+// Full mixed solve; A must hold current FP64 values (FP32 copy refreshed here
+// unless fresh_f32 is already true).  Returns the final state.
+MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
+                     int32_t precond, double inner_floor, bool fresh_f32, const MixedWork& w, ChunkRunner* runner) {
+    const Sell& S = A->pb->sell;
+    const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n));
+    hipStream_t st = ctx->stream;
+    if (!fresh_f32) matrix_to_f32(ctx, A);
+    double* part = ctx->partials(2 * nb);
+    if (precond == RSD_PC_JACOBI)
+        hipLaunchKernelGGL(k_inv_diag_f32, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, st, n, S.diag.p, A->val.p, w.invd);
+    MixedState* h = nullptr;
+    RSD_HIP(hipHostMalloc((void**)&h, sizeof(MixedState), hipHostMallocDefault));
+    std::memset(h, 0, sizeof(MixedState));
+    h->in_max = max_iters;
+    RSD_HIP(hipMemcpyAsync(w.st, h, sizeof(MixedState), hipMemcpyHostToDevice, st));
+    MixedState out{};
+    try {
+        for (int cycle = 0;; ++cycle) {
+            hipLaunchKernelGGL(k_mx_residual, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, S.slice_ptr.p, S.row_ptr.p,
+                               S.col.p, A->val.p, b, x, w.r64, w.st, part, ctx->scratch.tickets);
+            RSD_HIP(hipMemcpyAsync(h, w.st, sizeof(MixedState), hipMemcpyDeviceToHost, st));
+            RSD_HIP(hipStreamSynchronize(st));
+            const double rel = h->res / (h->res0 + 1e-50);
+            if (rel < tol) { h->converged = 1; break; }
+            if (cycle >= 40 || h->total_inner >= max_iters || !(rel == rel)) break;
+            // inner tolerance: what is still missing, never below the FP32 floor
+            double it = 0.5 * tol * (h->res0 + 1e-50) / h->res;
+            it = std::min(0.5, std::max(inner_floor, it));
+            h->in_tol = it;
+            h->in_max = std::max<int64_t>(1, max_iters - h->total_inner);
+            h->cycles = cycle + 1;
+            RSD_HIP(hipMemcpyAsync(w.st, h, sizeof(MixedState), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_mx_init, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w.r64, w.r, w.p, w.d,
+                               precond == RSD_PC_JACOBI ? w.invd : nullptr, w.st, part, ctx->scratch.tickets);
+            for (int64_t launched = 0;;) {
+                if (runner) runner->run();
+                else for (int k = 0; k < 8; ++k) mixed_launch_inner_iter(ctx, A, precond, w);
+                launched += 8;
+                RSD_HIP(hipMemcpyAsync(h, w.st, sizeof(MixedState), hipMemcpyDeviceToHost, st));
+                RSD_HIP(hipStreamSynchronize(st));
+                if (h->stop && h->xdone) break;
+                RSD_CHECK(launched <= max_iters + 16, "mixed CG: inner solve did not stop");
+            }
+            hipLaunchKernelGGL(k_mx_update_x, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, st, n, x, w.d, w.st);
+            RSD_HIP(hipGetLastError());
+        }
+        out = *h;
+    } catch (...) {
+        (void)hipHostFree(h);
+        throw;
+    }
+    (void)hipHostFree(h);
+    return out;
+}
+
+}  // namespace rsd
+
+using namespace rsd;
+
+extern "C" {
+
+int rsd_matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A) {
+    return guard([&] {
+        RSD_CHECK(ctx && A, "null argument");
+        matrix_to_f32(ctx, A);
+        RSD_HIP(hipGetLastError());
+    });
+}
+
+int rsd_spmv_f32(rsd_ctx* ctx, rsd_matrix* A, const float* x, float* y) {
+    return guard([&] {
+        RSD_CHECK(ctx && A && x && y, "null argument");
+        RSD_CHECK(A->val32.n == A->val.n, "rsd_spmv_f32: call rsd_matrix_to_f32 first");
+        launch_spmv_f32(ctx, A, x, y);
+        RSD_HIP(hipGetLastError());
+    });
+}
+
+int rsd_cg_solve_mixed(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
+                       int32_t precond, double inner_floor, rsd_report* rep, int64_t* outer_cycles) {
+    return guard([&] {
+        RSD_CHECK(ctx && A && b && x, "rsd_cg_solve_mixed: null argument");
+        RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
+        MixedWork w = mixed_workspace(ctx, A->pb->sell.n_rows);
+        MixedState s = mixed_run(ctx, A, b, x, tol, max_iters, precond, inner_floor > 0 ? inner_floor : 1e-5, false, w,
+                                 nullptr);
+        if (rep) {
+            rep->converged = s.converged;
+            rep->iterations = s.total_inner;
+            rep->initial_residual = s.res0;
+            rep->final_residual = s.res;
+            rep->final_residual_rel = s.res / (s.res0 + 1e-50);
+        }
+        if (outer_cycles) *outer_cycles = s.cycles;
+    });
+}
+
+}  // extern "C"

@@ -791,4 +791,57 @@ int rsd_matrix_set_values(rsd_ctx* ctx, rsd_matrix* m, const double* in) {
     });
 }
 
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): ten rounds of the
+// 4x32 bijection with multipliers 0xD2511F53 / 0xCD9E8D57 and Weyl key
+// increments 0x9E3779B9 / 0xBB67AE85.
+static void philox4x32(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int round = 0; round < 10; ++round) {
+        const uint64_t a = (uint64_t)0xD2511F53u * c[0], b = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(b >> 32) ^ c[1] ^ k0, n2 = (uint32_t)(a >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)b;
+        c[3] = (uint32_t)a;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+static void philox_pair(uint64_t seed, uint32_t stream, int64_t i, double& u1, double& u2) {
+    uint32_t c[4] = {(uint32_t)i, (uint32_t)((uint64_t)i >> 32), stream, 0u};
+    philox4x32(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    auto u53 = [](uint32_t hi, uint32_t lo) {
+        const uint64_t v = (((uint64_t)hi << 32) | lo) >> 11;
+        return ((double)v + 0.5) * (1.0 / 9007199254740992.0);
+    };
+    u1 = u53(c[0], c[1]);
+    u2 = u53(c[2], c[3]);
+}
+
+int rsd_philox_normal(uint64_t seed, uint32_t stream, int64_t n, double* out) {
+    return guard([&] {
+        RSD_CHECK(n >= 0 && (n == 0 || out), "rsd_philox_normal: bad argument");
+        parallel_for(n, [&](int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; ++i) {
+                double u1, u2;
+                philox_pair(seed, stream, i, u1, u2);
+                out[i] = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+            }
+        });
+    });
+}
+
+int rsd_philox_uniform(uint64_t seed, uint32_t stream, int64_t n, double* out) {
+    return guard([&] {
+        RSD_CHECK(n >= 0 && (n == 0 || out), "rsd_philox_uniform: bad argument");
+        parallel_for(n, [&](int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; ++i) {
+                double u1, u2;
+                philox_pair(seed, stream, i, u1, u2);
+                out[i] = u1;
+            }
+        });
+    });
+}
+
 }  // extern "C"

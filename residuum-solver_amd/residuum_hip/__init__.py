@@ -63,7 +63,13 @@ def check(rc):
 
 
 def dptr(a):
-    return None if a is None else _vp(a.ptr if isinstance(a, DeviceArray) else a)
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return _vp(a.ptr)
+    if hasattr(a, "data_ptr"):
+        return _vp(a.data_ptr())
+    return _vp(a)
 
 
 class Ctx:
@@ -294,15 +300,44 @@ class Problem:
 
     # -- drivers ----------------------------------------------------------------
     def be_run(self, U, t0, tf, dt, src=None, tol=1e-8, max_iters=1000, precond=PC_JACOBI, use_graph=True,
-               max_steps=None):
+               max_steps=None, mixed=False):
         max_steps = max_steps or int(round((tf - t0) / dt)) + 2
         iters = np.zeros(max_steps, np.int64)
         steps = C.c_int64()
+        timing = np.zeros(2)
         check(lib().rsd_be_run(self.ctx.h, self.h, C.c_double(t0), C.c_double(tf), C.c_double(dt),
                                src.encode() if src else None, C.c_double(tol), C.c_int64(max_iters),
-                               C.c_int32(precond), C.c_int32(int(use_graph)), dptr(U), C.c_int64(max_steps),
-                               C.byref(steps), iters.ctypes.data_as(_i64)))
+                               C.c_int32(precond), C.c_int32(int(mixed)), C.c_int32(int(use_graph)), dptr(U),
+                               C.c_int64(max_steps),
+                               C.byref(steps), iters.ctypes.data_as(_i64), timing.ctypes.data_as(_d)))
+        self.last_timing_ms = {"assembly": float(timing[0]), "solve": float(timing[1])}
         return steps.value, iters[:steps.value]
+
+
+    def newton(self, U, src=None, abs_tol=1e-10, rel_tol=1e-10, max_newton=50, lin_solver=1, lin_tol=1e-12,
+               lin_max=5000, krylov=30, precond=PC_JACOBI):
+        its = C.c_int64()
+        rh = np.zeros(max_newton)
+        li = np.zeros(max_newton, np.int64)
+        check(lib().rsd_newton_run(self.ctx.h, self.h, src.encode() if src else None, C.c_double(abs_tol),
+                                   C.c_double(rel_tol), C.c_int64(max_newton), C.c_int32(lin_solver),
+                                   C.c_double(lin_tol), C.c_int64(lin_max), C.c_int32(krylov), C.c_int32(precond),
+                                   dptr(U), C.byref(its), rh.ctypes.data_as(_d), li.ctypes.data_as(_i64)))
+        k = its.value
+        return k, rh[:max(k, 0) + 1], li[:max(k, 0)]
+
+
+    def adjoint(self, d_kappa, U0, t0, dt, n_steps, obs, t_obs, tol=1e-13, max_iters=10000, want_grad=True,
+                d_grad=None, d_UN=None):
+        """Loss and (optionally) dL/dtheta into d_grad; device pointers or DeviceArrays."""
+        obs = np.ascontiguousarray(obs, np.int64)
+        t_obs = np.ascontiguousarray(t_obs, np.float64)
+        loss = C.c_double()
+        check(lib().rsd_adjoint_grad_logk(self.ctx.h, self.h, dptr(d_kappa), C.c_double(t0), C.c_double(dt),
+                                          C.c_int64(n_steps), C.c_double(tol), C.c_int64(max_iters), dptr(U0),
+                                          C.c_int64(obs.size), obs.ctypes.data_as(_i64), t_obs.ctypes.data_as(_d),
+                                          C.c_int32(int(want_grad)), C.byref(loss), dptr(d_grad), dptr(d_UN)))
+        return loss.value
 
 
 class Matrix:
@@ -339,12 +374,83 @@ class Matrix:
     def inv_diag(self, out):
         check(lib().rsd_jacobi_inv_diag(self.pb.ctx.h, self.h, dptr(out)))
 
+    def gmres(self, b, x, tol=1e-10, max_iters=1000, krylov=30, precond=PC_JACOBI):
+        rep = Report()
+        check(lib().rsd_gmres_solve_f64(self.pb.ctx.h, self.h, dptr(b), dptr(x), C.c_double(tol), C.c_int64(max_iters),
+                                        C.c_int32(krylov), C.c_int32(precond), C.byref(rep)))
+        return rep
+
+    def to_f32(self):
+        check(lib().rsd_matrix_to_f32(self.pb.ctx.h, self.h))
+
+    def spmv_f32(self, x, y):
+        check(lib().rsd_spmv_f32(self.pb.ctx.h, self.h, dptr(x), dptr(y)))
+
+    def cg_mixed(self, b, x, tol=1e-8, max_iters=1000, precond=PC_JACOBI, inner_floor=0.0):
+        rep = Report()
+        cyc = C.c_int64()
+        check(lib().rsd_cg_solve_mixed(self.pb.ctx.h, self.h, dptr(b), dptr(x), C.c_double(tol), C.c_int64(max_iters),
+                                       C.c_int32(precond), C.c_double(inner_floor), C.byref(rep), C.byref(cyc)))
+        return rep, cyc.value
+
     def cg(self, b, x, tol=1e-10, max_iters=1000, precond=PC_JACOBI, hist=False):
         rep = Report()
         h = np.zeros(max_iters + 1) if hist else None
         check(lib().rsd_cg_solve_f64(self.pb.ctx.h, self.h, dptr(b), dptr(x), C.c_double(tol), C.c_int64(max_iters),
                                      C.c_int32(precond), C.byref(rep), None if h is None else h.ctypes.data_as(_d)))
         return (rep, h) if hist else rep
+
+
+class Batch:
+    """rsd_batch: independent systems solved by one set of launches."""
+
+    def __init__(self, ctx, mats):
+        self.ctx, self.mats = ctx, list(mats)
+        arr = (_vp * len(self.mats))(*[m.h.value for m in self.mats])
+        self.h = _vp()
+        check(lib().rsd_batch_create(ctx.h, C.c_int32(len(self.mats)), arr, C.byref(self.h)))
+        tot = C.c_int64()
+        self.offsets = np.zeros(len(self.mats) + 1, np.int64)
+        check(lib().rsd_batch_layout(self.h, C.byref(tot), self.offsets.ctypes.data_as(_i64)))
+        self.total_rows = tot.value
+
+    def close(self):
+        if self.h:
+            lib().rsd_batch_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pack(self, vecs):
+        out = np.zeros(self.total_rows)
+        for i, v in enumerate(vecs):
+            out[self.offsets[i]:self.offsets[i] + v.size] = v
+        return out
+
+    def unpack(self, packed):
+        return [packed[self.offsets[i]:self.offsets[i] + m.pb.n_free].copy() for i, m in enumerate(self.mats)]
+
+    def cg(self, b, x, tol=1e-8, max_iters=1000, precond=PC_JACOBI, use_graph=True):
+        reps = (Report * len(self.mats))()
+        check(lib().rsd_cg_solve_batched(self.ctx.h, self.h, dptr(b), dptr(x), C.c_double(tol), C.c_int64(max_iters),
+                                         C.c_int32(precond), C.c_int32(int(use_graph)), reps))
+        return list(reps)
+
+
+def philox_normal(seed, stream, n):
+    out = np.zeros(n)
+    check(lib().rsd_philox_normal(C.c_uint64(seed), C.c_uint32(stream), C.c_int64(n), out.ctypes.data_as(_d)))
+    return out
+
+
+def philox_uniform(seed, stream, n):
+    out = np.zeros(n)
+    check(lib().rsd_philox_uniform(C.c_uint64(seed), C.c_uint32(stream), C.c_int64(n), out.ctypes.data_as(_d)))
+    return out
 
 
 def exported_symbols():

@@ -4,7 +4,7 @@ CG and the Backward Euler loop, checked against the CPU oracle.
 The oracle runs in GPU order (colour-major element loop, the kernels'
 block-tree reductions), so everything here must agree bit for bit; the
 reference-order oracle (itself bitwise equal to the reference's own code,
-see test_oracle_vs_reference.py) bounds the rounding difference.
+see test_oracle.py) bounds the rounding difference.
 """
 import sys
 
@@ -229,3 +229,187 @@ def test_be_run_bitexact(ctx, use_graph):
     assert steps == out["steps"] == 10
     assert list(iters) == list(out["iters"])
     assert same(U.numpy(), out["U"])
+
+
+@pytest.mark.parametrize("case", CASES[:3])
+@pytest.mark.parametrize("precond", [0, 1])
+def test_gmres_bitexact(ctx, case, precond):
+    dim, n, p, nq, tags, exprs, ext = case
+    import residuum_hip as rh
+    tensor = (1.0, 0.2, 0.2, 0.1) if dim == 2 else (1, 0.2, 0, 0.2, 0.1, 0, 0, 0, 1)
+    pb = rh.Problem(ctx, dim, n, p, nq, tags, exprs, ext=ext)
+    pb.set_conductivity(1.0, k_expr="1.0 + 0.5*T", dk_expr="0.5", tensor=tensor)
+    oc = O.Case(dim, n, p, nq, tags, exprs, ext=ext, order=O.ORDER_GPU, k_expr="1.0 + 0.5*T", dk_expr="0.5",
+                tensor=tensor)
+    U = np.random.default_rng(4).random(oc.n_free)
+    forms = [(O.DIFFUSION, 1.0), (O.TDIFFUSION, 1.0)]
+    J = pb.matrix()
+    pb.assemble_matrix(J, forms, U=ctx.from_numpy(U))
+    val = oc.assemble_matrix(forms, U=U)
+    b = np.random.default_rng(5).standard_normal(oc.n_free)
+    for krylov in (5, 30):
+        dx = pb.vector(0.0)
+        rep = J.gmres(ctx.from_numpy(b), dx, tol=1e-10, max_iters=2000, krylov=krylov, precond=precond)
+        xo, ro = oc.gmres(val, b, tol=1e-10, max_iters=2000, krylov=krylov, precond=precond)
+        say("gmres", krylov, rep.iterations, ro.iterations)
+        assert rep.converged == ro.converged == 1 and rep.iterations == ro.iterations
+        assert rep.final_residual == ro.final_residual
+        assert same(dx.numpy(), xo)
+
+
+def test_newton_bitexact(ctx):
+    import residuum_hip as rh
+    dim, n, p, nq = 2, (12, 10), (2, 2), 3
+    tensor = (1.0, 0.0, 0.0, 0.1)
+    pb = rh.Problem(ctx, dim, n, p, nq, [0, 1, 2, 3], ["0"] * 4)
+    pb.set_conductivity(1.0, k_expr="1.0 + 0.5*T", dk_expr="0.5", tensor=tensor)
+    oc = O.Case(dim, n, p, nq, [0, 1, 2, 3], ["0"] * 4, order=O.ORDER_GPU, k_expr="1.0 + 0.5*T", dk_expr="0.5",
+                tensor=tensor)
+    src = "10*sin(pi*x)*sin(pi*y)"
+    dU = pb.vector(0.0)
+    its, res, lin = pb.newton(dU, src=src, abs_tol=1e-10, rel_tol=1e-10, lin_solver=1, lin_tol=1e-12, krylov=30)
+    Uo, its_o, res_o, lin_o = oc.newton(src=src, abs_tol=1e-10, rel_tol=1e-10, lin_solver=1, lin_tol=1e-12, krylov=30)
+    say("newton", its, its_o, list(lin), list(lin_o))
+    assert its == its_o and its > 0
+    # the source uses sin: device and host libm may differ in the last bit
+    assert np.max(np.abs(dU.numpy() - Uo)) <= 1e-12 * np.max(np.abs(Uo))
+    assert np.max(np.abs(res - res_o) / np.maximum(res_o, 1e-300)) < 1e-6
+
+
+def test_newton_kirchhoff_closed_form(ctx):
+    import residuum_hip as rh
+    pb = rh.Problem(ctx, 2, (40, 4), (1, 1), 2, [0, 1], ["0.0", "1.0"])
+    pb.set_conductivity(1.0, k_expr="1.0 + 0.5*T", dk_expr="0.5")
+    dU = pb.vector(0.0)
+    its, res, lin = pb.newton(dU, abs_tol=1e-10, rel_tol=1e-10, lin_solver=0, lin_tol=1e-12)
+    assert its > 0
+    oc = O.Case(2, (40, 4), (1, 1), 2, [0, 1], ["0.0", "1.0"])
+    x = oc.xyz.reshape(-1, 2)[oc.a2t][:, 0]
+    theta = (-1.0 + np.sqrt(1.0 + 2 * 0.5 * 1.25 * x)) / 0.5
+    assert np.max(np.abs(dU.numpy() - theta)) < 1e-6
+
+
+@pytest.mark.parametrize("case", [CASES[2], CASES[3]])
+def test_mixed_cg_reaches_fp64_residual(ctx, case):
+    pb, oc = build(ctx, case, kappa_seed=17)
+    forms = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 1.0 / 1e-3)]
+    K = pb.matrix()
+    pb.assemble_matrix(K, forms)
+    val = oc.assemble_matrix(forms)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(oc.n_free)
+    db = ctx.from_numpy(b)
+    dx = pb.vector(0.0)
+    rep, cycles = K.cg_mixed(db, dx, tol=1e-8)
+    x = dx.numpy()
+    r = b - oc.spmv(val, x)
+    say("mixed", rep.iterations, cycles, rep.final_residual_rel)
+    assert rep.converged == 1 and cycles >= 1
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-8
+    xo, ro = oc.cg(val, b, tol=1e-12, max_iters=5000)
+    assert np.max(np.abs(x - xo)) <= 1e-6 * np.max(np.abs(xo))
+    # FP32 SpMV against the FP64 product
+    K.to_f32()
+    xv = rng.standard_normal(oc.n_free)
+    d32, y32 = ctx.from_numpy(xv.astype(np.float32)), ctx.empty(oc.n_free, np.float32)
+    K.spmv_f32(d32, y32)
+    ref = oc.spmv(val.astype(np.float32).astype(np.float64), xv.astype(np.float32).astype(np.float64))
+    assert np.max(np.abs(y32.numpy() - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
+def test_be_run_mixed(ctx):
+    case = (3, (10, 9, 8), (1, 1, 1), 2, [0, 1], ["1", "0"], None)
+    pb, oc = build(ctx, case, kappa_seed=20261201)
+    Um = pb.vector(0.0)
+    steps_m, iters_m = pb.be_run(Um, 0.0, 0.01, 1e-3, tol=1e-8, mixed=True, use_graph=True)
+    Ue = pb.vector(0.0)
+    steps_e, _ = pb.be_run(Ue, 0.0, 0.01, 1e-3, tol=1e-8, mixed=True, use_graph=False)
+    Ud = pb.vector(0.0)
+    steps_d, iters_d = pb.be_run(Ud, 0.0, 0.01, 1e-3, tol=1e-8)
+    say("mixed BE", list(iters_m), "fp64", list(iters_d))
+    assert steps_m == steps_e == steps_d == 10
+    assert same(Um.numpy(), Ue.numpy())
+    ud = Ud.numpy()
+    assert np.max(np.abs(Um.numpy() - ud)) <= 1e-6 * np.max(np.abs(ud))
+
+
+def test_adjoint_bitexact_and_fd(ctx):
+    import residuum_hip as rh
+    case = (3, (5, 5, 4), (1, 1, 1), 2, [4], ["1"], None)
+    dim, n, p, nq, tags, exprs, ext = case
+    pb = rh.Problem(ctx, dim, n, p, nq, tags, exprs)
+    oc = O.Case(dim, n, p, nq, tags, exprs, order=O.ORDER_GPU)
+    theta = 0.5 * O.normal_field(20261203, 0, oc.n_elem)
+    kappa = np.exp(theta)
+    oc.kappa = kappa
+    oc.set_order(O.ORDER_GPU)
+    obs = np.arange(3, oc.n_free, 11)
+    tobs = 0.2 + 0.01 * np.arange(obs.size) % 0.3
+    dk, dU0, dg, dUN = ctx.from_numpy(kappa), pb.vector(0.0), ctx.empty(oc.n_elem), pb.vector()
+    loss = pb.adjoint(dk, dU0, 0.0, 5e-3, 5, obs, tobs, tol=1e-13, d_grad=dg, d_UN=dUN)
+    lo, go, UNo = oc.adjoint(0.0, 5e-3, 5, obs, tobs, cg_tol=1e-13)
+    say("adjoint loss", loss, lo)
+    assert loss == lo
+    assert same(dUN.numpy(), UNo)
+    assert same(dg.numpy(), go)
+    # central differences along random directions (h = 1e-6)
+    g = dg.numpy()
+    rng = np.random.default_rng(20261206)
+    for _ in range(4):
+        d = rng.standard_normal(oc.n_elem)
+        lp = pb.adjoint(ctx.from_numpy(np.exp(theta + 1e-6 * d)), dU0, 0.0, 5e-3, 5, obs, tobs, tol=1e-13,
+                        want_grad=False)
+        lm = pb.adjoint(ctx.from_numpy(np.exp(theta - 1e-6 * d)), dU0, 0.0, 5e-3, 5, obs, tobs, tol=1e-13,
+                        want_grad=False)
+        fd = (lp - lm) / 2e-6
+        say("fd", fd, g @ d)
+        assert abs(fd - g @ d) <= 1e-6 * abs(fd)
+
+
+def test_adjoint_torch_autograd(ctx):
+    torch = pytest.importorskip("torch")
+    import residuum_hip as rh
+    from residuum_hip.autograd import LogConductivityLoss
+    pb = rh.Problem(ctx, 3, (4, 4, 4), (1, 1, 1), 2, [4], ["1"])
+    theta = torch.tensor(0.3 * O.normal_field(5, 0, pb.n_elem), dtype=torch.float64, device="cuda",
+                         requires_grad=True)
+    obs = np.arange(0, pb.n_free, 5)
+    tobs = np.full(obs.size, 0.3)
+    U0 = pb.vector(0.0)
+    loss = LogConductivityLoss.apply(theta, pb, U0, 0.0, 5e-3, 3, obs, tobs, 1e-13)
+    loss.backward()
+    g = theta.grad.detach().cpu().numpy()
+    oc = O.Case(3, (4, 4, 4), (1, 1, 1), 2, [4], ["1"], order=O.ORDER_GPU)
+    oc.kappa = np.exp(theta.detach().cpu().numpy())
+    oc.set_order(O.ORDER_GPU)
+    lo, go, _ = oc.adjoint(0.0, 5e-3, 3, obs, tobs, cg_tol=1e-13)
+    assert abs(loss.item() - lo) <= 1e-12 * abs(lo)
+    assert np.max(np.abs(g - go)) <= 1e-9 * np.max(np.abs(go))
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batched_cg_equals_standalone(ctx, use_graph):
+    import residuum_hip as rh
+    sides = [3, 7, 5, 9, 4, 6]
+    pbs, mats, bs = [], [], []
+    for i, s in enumerate(sides):
+        pb = rh.Problem(ctx, 3, (s, s, s), (1, 1, 1), 2, list(range(6)), ["0"] * 6)
+        kap = np.exp(rh.philox_normal(20261202, i, pb.n_elem))
+        pb.set_conductivity(1.0, kappa=kap)
+        K = pb.matrix()
+        pb.assemble_matrix(K, [(O.DIFFUSION, 1.0)])
+        F = pb.vector()
+        nod = ctx.from_numpy(rh.philox_uniform(20261202, 1000 + i, pb.n_nodes))
+        pb.assemble_load(F, nodal=nod)
+        pbs.append(pb); mats.append(K); bs.append(F.numpy())
+    batch = rh.Batch(ctx, mats)
+    db = ctx.from_numpy(batch.pack(bs))
+    dx = ctx.from_numpy(np.zeros(batch.total_rows))
+    reps = batch.cg(db, dx, tol=1e-8, max_iters=1000, use_graph=use_graph)
+    xs = batch.unpack(dx.numpy())
+    for i, (pb, K, b) in enumerate(zip(pbs, mats, bs)):
+        x1 = pb.vector(0.0)
+        r1 = K.cg(ctx.from_numpy(b), x1, tol=1e-8, max_iters=1000)
+        say("problem", i, reps[i].iterations, r1.iterations)
+        assert reps[i].converged == 1 and reps[i].iterations == r1.iterations
+        assert same(xs[i], x1.numpy())
