@@ -66,7 +66,9 @@ typedef struct rsd_batch rsd_batch;
 /* ---------------------------------------------------------------------- */
 /* context                                                                 */
 /* ---------------------------------------------------------------------- */
-/* one HIP stream + reduction scratch on `device` */
+/* one HIP stream + reduction scratch on `device`.  Problems, matrices,
+ * batches and rsd_malloc buffers made on a context are destroyed before it:
+ * their destroy calls synchronise on the context's stream. */
 int rsd_ctx_create(int device, rsd_ctx** out);
 int rsd_ctx_destroy(rsd_ctx* ctx);
 int rsd_ctx_sync(rsd_ctx* ctx);

@@ -7,6 +7,7 @@ visible, the calls raise.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -77,13 +78,22 @@ class Ctx:
 
     def __init__(self, device=0):
         self.h = _vp()
+        self._children = weakref.WeakSet()
         check(lib().rsd_ctx_create(C.c_int(device), C.byref(self.h)))
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def sync(self):
         check(lib().rsd_ctx_sync(self.h))
 
     def close(self):
+        """Release every object created on this context, then the context."""
         if self.h:
+            kids = list(self._children)
+            order = {"Batch": 0, "Matrix": 1, "DeviceArray": 2, "Problem": 3}
+            for k in sorted(kids, key=lambda o: order.get(type(o).__name__, 9)):
+                k.close()
             lib().rsd_ctx_destroy(self.h)
             self.h = _vp()
 
@@ -151,6 +161,7 @@ class DeviceArray:
         p = _vp()
         check(lib().rsd_malloc(ctx.h, C.c_size_t(self.nbytes), C.byref(p)))
         self.ptr = p.value or 0
+        ctx._adopt(self)
 
     def upload(self, a):
         a = np.ascontiguousarray(a, dtype=self.dtype)
@@ -165,9 +176,11 @@ class DeviceArray:
         return out
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             lib().rsd_free(self.ctx.h, _vp(self.ptr))
-            self.ptr = 0
+        self.ptr = 0
+
+    close = free
 
     def __del__(self):
         try:
@@ -204,6 +217,7 @@ class Problem:
         self.h = _vp()
         check(lib().rsd_problem_create_block(ctx.h, C.c_int32(dim), n3, e6, p3, C.c_int32(nq), C.c_int32(len(tags)),
                                              t, self._exprs, C.byref(self.h)))
+        ctx._adopt(self)
         s = (C.c_int64 * 8)()
         check(lib().rsd_problem_sizes(self.h, s))
         (self.n_nodes, self.n_elem, self.npe, self.n_free, self.nnz, self.n_colours, self.slots,
@@ -347,6 +361,7 @@ class Matrix:
         self.pb = pb
         self.h = _vp()
         check(lib().rsd_matrix_create(pb.h, C.byref(self.h)))
+        pb.ctx._adopt(self)
 
     def close(self):
         if self.h:
@@ -409,6 +424,7 @@ class Batch:
         arr = (_vp * len(self.mats))(*[m.h.value for m in self.mats])
         self.h = _vp()
         check(lib().rsd_batch_create(ctx.h, C.c_int32(len(self.mats)), arr, C.byref(self.h)))
+        ctx._adopt(self)
         tot = C.c_int64()
         self.offsets = np.zeros(len(self.mats) + 1, np.int64)
         check(lib().rsd_batch_layout(self.h, C.byref(tot), self.offsets.ctypes.data_as(_i64)))

@@ -1,26 +1,27 @@
 """torch.autograd.Function around the adjoint entry point, for gradient
-checks: loss(theta) with theta the per-element log-conductivity on the GPU.
+checks: loss(theta) with theta the per-element log-conductivity.
 
-forward runs the checkpointed Backward Euler solve and the adjoint sweep
-through rsd_adjoint_grad_logk (one call returns the loss and dL/dtheta);
-backward scales the stored gradient by the incoming grad_output.
+theta is a float64 host tensor.  forward uploads kappa = exp(theta) to the
+library's device memory, runs the checkpointed Backward Euler solve and the
+adjoint sweep through rsd_adjoint_grad_logk (one call returns the loss and
+dL/dtheta) and keeps the gradient; backward scales it by grad_output.  The
+GPU work runs on the library's own HIP stream, so torch's device runtime is
+not involved.
 """
+import numpy as np
 import torch
-
-from . import dptr
 
 
 class LogConductivityLoss(torch.autograd.Function):
     @staticmethod
     def forward(fctx, theta, problem, U0, t0, dt, n_steps, obs, t_obs, tol):
-        assert theta.is_cuda and theta.dtype == torch.float64 and theta.is_contiguous()
-        kappa = torch.exp(theta).contiguous()
-        grad = torch.empty_like(theta)
-        torch.cuda.synchronize()
-        loss = problem.adjoint(kappa.data_ptr(), U0, t0, dt, n_steps, obs, t_obs, tol=tol, want_grad=True,
-                               d_grad=grad.data_ptr())
-        fctx.save_for_backward(grad)
-        return torch.tensor(loss, dtype=torch.float64, device=theta.device)
+        th = theta.detach().cpu().numpy().astype(np.float64)
+        ctx = problem.ctx
+        d_kappa = ctx.from_numpy(np.exp(th))
+        d_grad = ctx.empty(th.size)
+        loss = problem.adjoint(d_kappa, U0, t0, dt, n_steps, obs, t_obs, tol=tol, want_grad=True, d_grad=d_grad)
+        fctx.save_for_backward(torch.from_numpy(d_grad.numpy()))
+        return torch.tensor(loss, dtype=torch.float64)
 
     @staticmethod
     def backward(fctx, g):
@@ -30,6 +31,6 @@ class LogConductivityLoss(torch.autograd.Function):
 
 def loss_only(theta, problem, U0, t0, dt, n_steps, obs, t_obs, tol):
     """Loss without the adjoint sweep (for finite differences)."""
-    kappa = torch.exp(theta).contiguous()
-    torch.cuda.synchronize()
-    return problem.adjoint(kappa.data_ptr(), U0, t0, dt, n_steps, obs, t_obs, tol=tol, want_grad=False)
+    th = theta.detach().cpu().numpy().astype(np.float64)
+    d_kappa = problem.ctx.from_numpy(np.exp(th))
+    return problem.adjoint(d_kappa, U0, t0, dt, n_steps, obs, t_obs, tol=tol, want_grad=False)

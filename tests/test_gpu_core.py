@@ -273,7 +273,7 @@ def test_newton_bitexact(ctx):
     assert its == its_o and its > 0
     # the source uses sin: device and host libm may differ in the last bit
     assert np.max(np.abs(dU.numpy() - Uo)) <= 1e-12 * np.max(np.abs(Uo))
-    assert np.max(np.abs(res - res_o) / np.maximum(res_o, 1e-300)) < 1e-6
+    assert np.max(np.abs(res - res_o)) <= 1e-12 * res_o[0]
 
 
 def test_newton_kirchhoff_closed_form(ctx):
@@ -371,20 +371,19 @@ def test_adjoint_torch_autograd(ctx):
     import residuum_hip as rh
     from residuum_hip.autograd import LogConductivityLoss
     pb = rh.Problem(ctx, 3, (4, 4, 4), (1, 1, 1), 2, [4], ["1"])
-    theta = torch.tensor(0.3 * O.normal_field(5, 0, pb.n_elem), dtype=torch.float64, device="cuda",
-                         requires_grad=True)
+    theta = torch.tensor(0.3 * O.normal_field(5, 0, pb.n_elem), dtype=torch.float64, requires_grad=True)
     obs = np.arange(0, pb.n_free, 5)
     tobs = np.full(obs.size, 0.3)
     U0 = pb.vector(0.0)
     loss = LogConductivityLoss.apply(theta, pb, U0, 0.0, 5e-3, 3, obs, tobs, 1e-13)
     loss.backward()
-    g = theta.grad.detach().cpu().numpy()
+    g = theta.grad.detach().numpy()
     oc = O.Case(3, (4, 4, 4), (1, 1, 1), 2, [4], ["1"], order=O.ORDER_GPU)
-    oc.kappa = np.exp(theta.detach().cpu().numpy())
+    oc.kappa = np.exp(theta.detach().numpy())
     oc.set_order(O.ORDER_GPU)
     lo, go, _ = oc.adjoint(0.0, 5e-3, 3, obs, tobs, cg_tol=1e-13)
-    assert abs(loss.item() - lo) <= 1e-12 * abs(lo)
-    assert np.max(np.abs(g - go)) <= 1e-9 * np.max(np.abs(go))
+    assert loss.item() == lo
+    assert np.array_equal(g, go)
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
