@@ -1,9 +1,8 @@
 // rsd_assembly.hip -- element kernels for Q1/Q2 quads and Q1 hexes.
 //
-// One thread per element, launched colour by colour: elements of one colour
-// share no node, so the scatter through the element->slot map is a plain
-// read-modify-write and the summation order per matrix / vector entry is
-// the colour order.  The element math follows the reference operand by
+// Launched colour by colour: elements of one colour share no node, so the
+// scatter through the element->slot map is a plain read-modify-write and the
+// summation order per matrix / vector entry is the colour order.  The element math follows the reference operand by
 // operand (EvalQuadraturePointVolume.hpp:64-85, JacobianTransform.tpp:3-206,
 // Lagrange1D.tpp:4-51, the heat-equation forms) with every product and sum
 // rounded separately, so a CPU loop visiting the elements in the same
@@ -162,70 +161,6 @@ __device__ __forceinline__ void qp_state(QP<D, NPE>& Q, const double* Uv, const 
     Q.Trate = r;
 }
 
-template<int D, int NPE>
-__device__ __forceinline__ void qp_model(QP<D, NPE>& Q, const Model& m, double kap, bool has_kap) {
-    double k = m.has_k ? eval_expr(m.k, 0.0, 0.0, 0.0, 0.0, Q.T) : m.k_scalar;
-    double dk = m.has_dk ? eval_expr(m.dk, 0.0, 0.0, 0.0, 0.0, Q.T) : 0.0;
-    if (has_kap) { k = kap * k; dk = kap * dk; }
-#pragma unroll
-    for (int i = 0; i < D; ++i)
-#pragma unroll
-        for (int j = 0; j < D; ++j) {
-            const double shape = m.aniso ? m.tensor[i * D + j] : ((i == j) ? 1.0 : 0.0);
-            Q.K[i * D + j] = k * shape;
-            Q.dKdT[i * D + j] = dk * shape;
-        }
-    Q.rho = m.rho;
-    Q.cp = m.cp;
-    Q.dcpdT = 0.0;
-}
-
-// one form's contribution at (a, b) for the matrix forms
-template<int D, int NPE>
-__device__ __forceinline__ double form_entry(int kind, const QP<D, NPE>& Q, int a, int b) {
-    if (kind == RSD_FORM_DIFFUSION || kind == RSD_FORM_TDIFFUSION) {
-        double integrand = 0.0;
-#pragma unroll
-        for (int i = 0; i < D; ++i) {
-            double mv = 0.0;
-            if (kind == RSD_FORM_DIFFUSION) {
-#pragma unroll
-                for (int j = 0; j < D; ++j) mv += Q.K[i * D + j] * Q.dNdx[b * D + j];
-            } else {
-#pragma unroll
-                for (int j = 0; j < D; ++j) mv += Q.dKdT[i * D + j] * Q.gradT[j];
-            }
-            integrand += Q.dNdx[a * D + i] * mv;
-        }
-        return kind == RSD_FORM_DIFFUSION ? integrand * Q.measure * Q.w : integrand * Q.N[b] * Q.measure * Q.w;
-    }
-    if (kind == RSD_FORM_MASS) return Q.rho * Q.cp * Q.N[a] * Q.N[b] * Q.measure * Q.w;
-    return Q.rho * Q.dcpdT * Q.Trate * Q.N[a] * Q.N[b] * Q.measure * Q.w;
-}
-
-// vector-form contribution at (a, b) with operand Ue[b]
-template<int D, int NPE>
-__device__ __forceinline__ double form_vec_entry(int kind, const QP<D, NPE>& Q, int a, int b, double ub) {
-    if (kind == RSD_FORM_DIFFUSION || kind == RSD_FORM_TDIFFUSION) {
-        double integrand = 0.0;
-#pragma unroll
-        for (int i = 0; i < D; ++i) {
-            double mv = 0.0;
-            if (kind == RSD_FORM_DIFFUSION) {
-#pragma unroll
-                for (int j = 0; j < D; ++j) mv += Q.K[i * D + j] * Q.dNdx[b * D + j];
-            } else {
-#pragma unroll
-                for (int j = 0; j < D; ++j) mv += Q.dKdT[i * D + j] * Q.gradT[j];
-            }
-            integrand += Q.dNdx[a * D + i] * mv;
-        }
-        return kind == RSD_FORM_DIFFUSION ? integrand * ub * Q.measure * Q.w : integrand * Q.N[b] * ub * Q.measure * Q.w;
-    }
-    if (kind == RSD_FORM_MASS) return Q.rho * Q.cp * Q.N[a] * Q.N[b] * ub * Q.measure * Q.w;
-    return Q.rho * Q.dcpdT * Q.Trate * Q.N[a] * Q.N[b] * ub * Q.measure * Q.w;
-}
-
 template<int D, int P>
 __device__ __forceinline__ void load_element(const AsmArgs& A, int64_t e, int32_t (&nodes)[Elem<D, P>::NPE],
                                              double (&X)[Elem<D, P>::NPE][D]) {
@@ -252,49 +187,229 @@ __device__ __forceinline__ void gather(const AsmArgs& A, const int32_t (&nodes)[
     }
 }
 
-template<int D, int P, int NQ>
-__global__ __launch_bounds__(kBlock) void k_asm_matrix(AsmArgs A, Model m, int64_t c0, int64_t c1) {
+// ------------------------------------------------------------------------
+// Element kernels: NPE threads per element.
+//   phase 1  the element's threads split its quadrature points; each
+//            evaluates the geometry (and, for state-dependent models, the
+//            state and the conductivity) of its points into LDS;
+//   phase 2  thread a accumulates row a of the element matrix (entry a of
+//            the element vector) over points, forms and columns in the
+//            reference's order, then scatters it.
+// Each entry's accumulation chain is the one a single thread per element
+// would run, so results are bitwise those of the serial loops.
+// ------------------------------------------------------------------------
+template<int D, int P, int NQ, bool STATE>
+struct ECfg {
+    static constexpr int NPE = Elem<D, P>::NPE;
+    static constexpr int NQP = D == 2 ? NQ * NQ : NQ * NQ * NQ;
+    // per-point record: dNdx[NPE*D], measure, source value, then (state)
+    // k, dk, Trate, gradT[D]
+    static constexpr int R_DN = 0, R_MEAS = NPE * D, R_SRC = R_MEAS + 1;
+    static constexpr int R_K = R_SRC + 1, R_DK = R_K + 1, R_TR = R_DK + 1, R_GT = R_TR + 1;
+    static constexpr int REC = STATE ? R_GT + D : R_SRC + 1;
+    static constexpr int ELEM_DOUBLES = NQP * REC;
+    static constexpr int LDS_BUDGET = 40 * 1024;
+    static constexpr int EPB_RAW = LDS_BUDGET / (ELEM_DOUBLES * 8);
+    static constexpr int EPB_MAX = 256 / NPE;
+    static constexpr int EPB_CAP = EPB_RAW < EPB_MAX ? EPB_RAW : EPB_MAX;
+    // keep whole wavefronts when the element size allows it
+    static constexpr int WAVE_EL = 64 % NPE == 0 ? 64 / NPE : 1;
+    static constexpr int EPB_W = EPB_CAP >= WAVE_EL ? (EPB_CAP / WAVE_EL) * WAVE_EL : EPB_CAP;
+    static constexpr int EPB = EPB_W > 0 ? EPB_W : 1;
+    static constexpr int BLOCK = EPB * NPE;
+};
+
+template<int D, int P>
+__device__ __forceinline__ void shape_values(const double (&xi)[3], double (&N)[Elem<D, P>::NPE]) {
     constexpr int NPE = Elem<D, P>::NPE;
-    const int64_t k = c0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (k >= c1) return;
-    const int64_t e = A.order[k];
-    int32_t nodes[NPE];
-    double X[NPE][D];
-    load_element<D, P>(A, e, nodes, X);
-    double Uv[NPE], Ur[NPE];
-    if (A.need_state) {
-        gather<NPE>(A, nodes, 1, A.U, Uv);
-        gather<NPE>(A, nodes, 0, A.Udot, Ur);
-    }
-    const bool has_kap = A.kappa != nullptr;
-    const double kap = has_kap ? A.kappa[e] : 1.0;
-    double Ke[NPE * NPE];
+    double Nx[D][P + 1], dNx[D][P + 1];
 #pragma unroll
-    for (int i = 0; i < NPE * NPE; ++i) Ke[i] = 0.0;
-    for (int q = 0; q < (D == 2 ? NQ * NQ : NQ * NQ * NQ); ++q) {
+    for (int k = 0; k < D; ++k) lag1d<P>(xi[k], Nx[k], dNx[k]);
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) {
+        const int i = a % (P + 1), j = (a / (P + 1)) % (P + 1), k = a / ((P + 1) * (P + 1));
+        if constexpr (D == 2) N[a] = Nx[0][i] * Nx[1][j];
+        else N[a] = Nx[0][i] * Nx[1][j] * Nx[2][k];
+    }
+}
+
+template<int D, int NQ>
+__device__ __forceinline__ double point_coords(int q, double (&xi)[3]) {
+    const int qi = q % NQ, qj = (q / NQ) % NQ, qk = q / (NQ * NQ);
+    double wq[3];
+    gauss<NQ>(qi, xi[0], wq[0]);
+    gauss<NQ>(qj, xi[1], wq[1]);
+    xi[2] = 0.0;
+    if constexpr (D == 3) {
+        gauss<NQ>(qk, xi[2], wq[2]);
+        return wq[0] * wq[1] * wq[2];
+    } else {
+        return wq[0] * wq[1];
+    }
+}
+
+// phase 1: geometry (and state) of the points q = a, a + NPE, ... of one element
+//   load: 0 none, 1 source expression at x_q, 2 nodal source interpolated
+template<int D, int P, int NQ, bool STATE>
+__device__ __forceinline__ void phase1(const AsmArgs& A, const Model& m, const Expr* src, int load, int64_t e, int a,
+                                       const double (&X)[Elem<D, P>::NPE][D], const double* Uv, const double* Ur,
+                                       const double* Un, double kap, bool has_kap, double* rec) {
+    using C = ECfg<D, P, NQ, STATE>;
+    constexpr int NPE = C::NPE;
+    for (int q = a; q < C::NQP; q += NPE) {
         QP<D, NPE> Q;
-        qp_eval<D, P, NQ>(q, X, Q, false);
-        if (A.need_state) qp_state<D, NPE>(Q, Uv, Ur);
-        else { Q.T = 0.0; Q.Trate = 0.0; for (int i = 0; i < D; ++i) Q.gradT[i] = 0.0; }
-        qp_model<D, NPE>(Q, m, kap, has_kap);
+        qp_eval<D, P, NQ>(q, X, Q, load == 1);
+        double* r = rec + q * C::REC;
+#pragma unroll
+        for (int i = 0; i < NPE * D; ++i) r[C::R_DN + i] = Q.dNdx[i];
+        r[C::R_MEAS] = Q.measure;
+        if (load == 1) {
+            r[C::R_SRC] = eval_expr(*src, A.time, Q.x[0], Q.x[1], D == 3 ? Q.x[2] : 0.0, 0.0);
+        } else if (load == 2) {
+            double val = 0.0;
+#pragma unroll
+            for (int b = 0; b < NPE; ++b) val += Q.N[b] * Un[b];
+            r[C::R_SRC] = val;
+        }
+        if constexpr (STATE) {
+            double Uvl[NPE], Url[NPE];
+#pragma unroll
+            for (int b = 0; b < NPE; ++b) { Uvl[b] = Uv[b]; Url[b] = Ur[b]; }
+            qp_state<D, NPE>(Q, Uvl, Url);
+            double k = m.has_k ? eval_expr(m.k, 0.0, 0.0, 0.0, 0.0, Q.T) : m.k_scalar;
+            double dk = m.has_dk ? eval_expr(m.dk, 0.0, 0.0, 0.0, 0.0, Q.T) : 0.0;
+            if (has_kap) { k = kap * k; dk = kap * dk; }
+            r[C::R_K] = k;
+            r[C::R_DK] = dk;
+            r[C::R_TR] = Q.Trate;
+#pragma unroll
+            for (int i = 0; i < D; ++i) r[C::R_GT + i] = Q.gradT[i];
+        }
+    }
+}
+
+// conductivity tensor entry (qp_model: K = k * shape)
+__device__ __forceinline__ double kshape(const Model& m, int D, int i, int j) {
+    return m.aniso ? m.tensor[i * D + j] : ((i == j) ? 1.0 : 0.0);
+}
+
+// diffusion integrand dNdx_a . (K dNdx_b), K = k * shape; for an isotropic
+// model the zero off-diagonal products are left out (they only add zeros)
+template<int D>
+__device__ __forceinline__ double diff_integrand(const Model& m, double k, const double* da, const double* db) {
+    double integrand = 0.0;
+    if (!m.aniso) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) integrand += da[i] * (k * db[i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            double mv = 0.0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) mv += (k * m.tensor[i * D + j]) * db[j];
+            integrand += da[i] * mv;
+        }
+    }
+    return integrand;
+}
+
+// dNdx_a . (dKdT gradT), dKdT = dk * shape
+template<int D>
+__device__ __forceinline__ double tdiff_integrand(const Model& m, double dk, const double* da, const double* gt) {
+    double integrand = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        double mv = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) mv += (dk * kshape(m, D, i, j)) * gt[j];
+        integrand += da[i] * mv;
+    }
+    return integrand;
+}
+
+template<int D, int P, int NQ, bool STATE>
+__global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(AsmArgs A, Model m, int64_t c0,
+                                                                             int64_t c1) {
+    using C = ECfg<D, P, NQ, STATE>;
+    constexpr int NPE = C::NPE;
+    __shared__ double lds[C::EPB * C::ELEM_DOUBLES];
+    const int el = threadIdx.x / NPE, a = threadIdx.x % NPE;
+    const int64_t k = c0 + (int64_t)blockIdx.x * C::EPB + el;
+    const bool valid = k < c1;
+    double* rec = lds + el * C::ELEM_DOUBLES;
+    int64_t e = 0;
+    double kap = 1.0;
+    const bool has_kap = A.kappa != nullptr;
+    if (valid) {
+        e = A.order[k];
+        int32_t nodes[NPE];
+        double X[NPE][D];
+        load_element<D, P>(A, e, nodes, X);
+        if (has_kap) kap = A.kappa[e];
+        double Uv[NPE], Ur[NPE];
+        if constexpr (STATE) {
+            gather<NPE>(A, nodes, 1, A.U, Uv);
+            gather<NPE>(A, nodes, 0, A.Udot, Ur);
+        }
+        phase1<D, P, NQ, STATE>(A, m, nullptr, 0, e, a, X, Uv, Ur, nullptr, kap, has_kap, rec);
+    }
+    __syncthreads();
+    if (!valid) return;
+    double kel = m.k_scalar;
+    if (has_kap) kel = kap * kel;
+    double Ke[NPE];
+#pragma unroll
+    for (int b = 0; b < NPE; ++b) Ke[b] = 0.0;
+    for (int q = 0; q < C::NQP; ++q) {
+        const double* r = rec + q * C::REC;
+        double xi[3];
+        const double w = point_coords<D, NQ>(q, xi);
+        double N[NPE];
+        shape_values<D, P>(xi, N);
+        const double meas = r[C::R_MEAS];
+        const double* da = r + C::R_DN + a * D;
+        const double kq = STATE ? r[C::R_K] : kel;
         for (int f = 0; f < A.nf; ++f) {
             const int kind = A.kinds[f] & 0xff;
             const bool scaled = (A.kinds[f] & RSD_FORM_SCALED) != 0;
             const double coef = A.coefs[f];
-#pragma unroll
-            for (int a = 0; a < NPE; ++a)
+            if (kind == RSD_FORM_DIFFUSION) {
 #pragma unroll
                 for (int b = 0; b < NPE; ++b) {
-                    const double c = form_entry<D, NPE>(kind, Q, a, b);
-                    Ke[a * NPE + b] += scaled ? coef * c : c;
+                    const double c = diff_integrand<D>(m, kq, da, r + C::R_DN + b * D) * meas * w;
+                    Ke[b] += scaled ? coef * c : c;
                 }
+            } else if (kind == RSD_FORM_MASS) {
+                const double ra = m.rho * m.cp * N[a];
+#pragma unroll
+                for (int b = 0; b < NPE; ++b) {
+                    const double c = ra * N[b] * meas * w;
+                    Ke[b] += scaled ? coef * c : c;
+                }
+            } else if constexpr (STATE) {
+                if (kind == RSD_FORM_TDIFFUSION) {
+                    const double it = tdiff_integrand<D>(m, r[C::R_DK], da, r + C::R_GT);
+#pragma unroll
+                    for (int b = 0; b < NPE; ++b) {
+                        const double c = it * N[b] * meas * w;
+                        Ke[b] += scaled ? coef * c : c;
+                    }
+                } else {  // TMASS: rho * dcp/dT * Trate * N_a * N_b, dcp/dT = 0 for a constant cp
+                    const double ra = m.rho * 0.0 * r[C::R_TR] * N[a];
+#pragma unroll
+                    for (int b = 0; b < NPE; ++b) {
+                        const double c = ra * N[b] * meas * w;
+                        Ke[b] += scaled ? coef * c : c;
+                    }
+                }
+            }
         }
     }
-    const int32_t* mp = A.map + e * NPE * NPE;
+    const int32_t* mp = A.map + (e * NPE + a) * NPE;
 #pragma unroll
-    for (int i = 0; i < NPE * NPE; ++i) {
-        const int32_t s = mp[i];
-        if (s >= 0) A.val[s] += Ke[i];
+    for (int b = 0; b < NPE; ++b) {
+        const int32_t s = mp[b];
+        if (s >= 0) A.val[s] += Ke[b];
     }
 }
 
@@ -302,155 +417,207 @@ __global__ __launch_bounds__(kBlock) void k_asm_matrix(AsmArgs A, Model m, int64
 //   mode 0: forms applied to a selected state buffer (assembleVector)
 //   mode 1: load vector (SourceForm / NodalSourceForm)
 //   mode 2: essential lift, F -= forms(constrained values)
-template<int D, int P, int NQ>
-__global__ __launch_bounds__(kBlock) void k_asm_vector(AsmArgs A, Model m, Expr src, int mode, int64_t c0, int64_t c1) {
-    constexpr int NPE = Elem<D, P>::NPE;
-    const int64_t k = c0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (k >= c1) return;
-    const int64_t e = A.order[k];
-    int32_t nodes[NPE];
-    double X[NPE][D];
-    load_element<D, P>(A, e, nodes, X);
-    double Uv[NPE], Ur[NPE], Ue[NPE];
-    if (mode == 0) {
-        gather<NPE>(A, nodes, 1, A.U, Uv);
-        gather<NPE>(A, nodes, 0, A.Udot, Ur);
-        if (A.selector == RSD_SEL_VALUE) {
-#pragma unroll
-            for (int a = 0; a < NPE; ++a) Ue[a] = Uv[a];
-        } else if (A.selector == RSD_SEL_RATE) {
-#pragma unroll
-            for (int a = 0; a < NPE; ++a) Ue[a] = Ur[a];
-        } else {
-            gather<NPE>(A, nodes, 0, A.operand, Ue);
-        }
-    } else if (mode == 1) {
-        if (A.nodal) {
-#pragma unroll
-            for (int a = 0; a < NPE; ++a) Ue[a] = A.nodal[nodes[a]];
-        }
-    } else {
-        gather<NPE>(A, nodes, 2, nullptr, Ue);
-    }
+template<int D, int P, int NQ, bool STATE>
+__global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_vector(AsmArgs A, Model m, Expr src, int mode,
+                                                                             int64_t c0, int64_t c1) {
+    using C = ECfg<D, P, NQ, STATE>;
+    constexpr int NPE = C::NPE;
+    __shared__ double lds[C::EPB * C::ELEM_DOUBLES];
+    const int el = threadIdx.x / NPE, a = threadIdx.x % NPE;
+    const int64_t k = c0 + (int64_t)blockIdx.x * C::EPB + el;
+    const bool valid = k < c1;
+    double* rec = lds + el * C::ELEM_DOUBLES;
+    int64_t e = 0;
+    double kap = 1.0;
     const bool has_kap = A.kappa != nullptr;
-    const double kap = has_kap ? A.kappa[e] : 1.0;
-    double Fe[NPE];
+    int32_t nodes[NPE];
+    double Ue[NPE];
+    if (valid) {
+        e = A.order[k];
+        double X[NPE][D];
+        load_element<D, P>(A, e, nodes, X);
+        if (has_kap) kap = A.kappa[e];
+        double Uv[NPE], Ur[NPE];
+        if (mode == 0) {
+            gather<NPE>(A, nodes, 1, A.U, Uv);
+            gather<NPE>(A, nodes, 0, A.Udot, Ur);
+            if (A.selector == RSD_SEL_VALUE) {
 #pragma unroll
-    for (int a = 0; a < NPE; ++a) Fe[a] = 0.0;
-    for (int q = 0; q < (D == 2 ? NQ * NQ : NQ * NQ * NQ); ++q) {
-        QP<D, NPE> Q;
-        qp_eval<D, P, NQ>(q, X, Q, mode == 1);
-        if (mode == 1) {
-            double val = 0.0;
+                for (int b = 0; b < NPE; ++b) Ue[b] = Uv[b];
+            } else if (A.selector == RSD_SEL_RATE) {
+#pragma unroll
+                for (int b = 0; b < NPE; ++b) Ue[b] = Ur[b];
+            } else {
+                gather<NPE>(A, nodes, 0, A.operand, Ue);
+            }
+        } else if (mode == 1) {
             if (A.nodal) {
 #pragma unroll
-                for (int b = 0; b < NPE; ++b) val += Q.N[b] * Ue[b];
-            } else {
-                val = eval_expr(src, A.time, Q.x[0], Q.x[1], D == 3 ? Q.x[2] : 0.0, 0.0);
+                for (int b = 0; b < NPE; ++b) Ue[b] = A.nodal[nodes[b]];
             }
+        } else {
+            gather<NPE>(A, nodes, 2, nullptr, Ue);
+            if constexpr (STATE) {
 #pragma unroll
-            for (int a = 0; a < NPE; ++a) Fe[a] += val * Q.N[a] * Q.measure * Q.w;
+                for (int b = 0; b < NPE; ++b) { Uv[b] = 0.0; Ur[b] = 0.0; }
+            }
+        }
+        const int load = mode == 1 ? (A.nodal ? 2 : 1) : 0;
+        phase1<D, P, NQ, STATE>(A, m, &src, load, e, a, X, Uv, Ur, Ue, kap, has_kap, rec);
+    }
+    __syncthreads();
+    if (!valid) return;
+    double kel = m.k_scalar;
+    if (has_kap) kel = kap * kel;
+    double Fe = 0.0;
+    for (int q = 0; q < C::NQP; ++q) {
+        const double* r = rec + q * C::REC;
+        double xi[3];
+        const double w = point_coords<D, NQ>(q, xi);
+        double N[NPE];
+        shape_values<D, P>(xi, N);
+        const double meas = r[C::R_MEAS];
+        if (mode == 1) {
+            Fe += r[C::R_SRC] * N[a] * meas * w;
             continue;
         }
-        if (mode == 0) qp_state<D, NPE>(Q, Uv, Ur);
-        else { Q.T = 0.0; Q.Trate = 0.0; for (int i = 0; i < D; ++i) Q.gradT[i] = 0.0; }
-        qp_model<D, NPE>(Q, m, kap, has_kap);
+        const double* da = r + C::R_DN + a * D;
+        const double kq = STATE ? r[C::R_K] : kel;
         for (int f = 0; f < A.nf; ++f) {
             const int kind = A.kinds[f] & 0xff;
             const bool scaled = (A.kinds[f] & RSD_FORM_SCALED) != 0;
             const double coef = A.coefs[f];
+            double buf = 0.0;
+            double& acc = scaled ? buf : Fe;
+            if (kind == RSD_FORM_DIFFUSION) {
 #pragma unroll
-            for (int a = 0; a < NPE; ++a) {
-                if (scaled) {
-                    double buf = 0.0;
+                for (int b = 0; b < NPE; ++b)
+                    acc += diff_integrand<D>(m, kq, da, r + C::R_DN + b * D) * Ue[b] * meas * w;
+            } else if (kind == RSD_FORM_MASS) {
+                const double ra = m.rho * m.cp * N[a];
 #pragma unroll
-                    for (int b = 0; b < NPE; ++b) buf += form_vec_entry<D, NPE>(kind, Q, a, b, Ue[b]);
-                    Fe[a] += coef * buf;
+                for (int b = 0; b < NPE; ++b) acc += ra * N[b] * Ue[b] * meas * w;
+            } else if constexpr (STATE) {
+                if (kind == RSD_FORM_TDIFFUSION) {
+                    const double it = tdiff_integrand<D>(m, r[C::R_DK], da, r + C::R_GT);
+#pragma unroll
+                    for (int b = 0; b < NPE; ++b) acc += it * N[b] * Ue[b] * meas * w;
                 } else {
+                    const double ra = m.rho * 0.0 * r[C::R_TR] * N[a];
 #pragma unroll
-                    for (int b = 0; b < NPE; ++b) Fe[a] += form_vec_entry<D, NPE>(kind, Q, a, b, Ue[b]);
+                    for (int b = 0; b < NPE; ++b) acc += ra * N[b] * Ue[b] * meas * w;
                 }
             }
+            if (scaled) Fe += coef * buf;
         }
     }
-#pragma unroll
-    for (int a = 0; a < NPE; ++a) {
-        const int32_t r = A.t2a[nodes[a]];
-        if (r >= 0) A.F[r] += mode == 2 ? -1.0 * Fe[a] : Fe[a];
-    }
+    const int32_t row = A.t2a[nodes[a]];
+    if (row >= 0) A.F[row] += mode == 2 ? -1.0 * Fe : Fe;
 }
 
 // adjoint contribution per element (no colouring: one output per element):
 // grad[e] += -(kappa_e * lam_e . (Khat_e T_e)), Khat_e the diffusion matrix
 // at unit conductivity, T_e the full gather of A.U, lam_e the free gather
-// of A.operand.
+// of A.operand.  Thread a forms row a of Khat_e T_e; the element's first
+// thread adds the rows in order.
 template<int D, int P, int NQ>
-__global__ __launch_bounds__(kBlock) void k_adj_grad(AsmArgs A, Model m, const double* __restrict__ kappa,
-                                                     double* __restrict__ grad, int64_t n_elem) {
-    constexpr int NPE = Elem<D, P>::NPE;
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= n_elem) return;
+__global__ __launch_bounds__((ECfg<D, P, NQ, false>::BLOCK)) void k_adj_grad(AsmArgs A, Model m,
+                                                                           const double* __restrict__ kappa,
+                                                                           double* __restrict__ grad, int64_t n_elem) {
+    using C = ECfg<D, P, NQ, false>;
+    constexpr int NPE = C::NPE;
+    __shared__ double lds[C::EPB * C::ELEM_DOUBLES];
+    __shared__ double rows[C::EPB * NPE];
+    const int el = threadIdx.x / NPE, a = threadIdx.x % NPE;
+    const int64_t e = (int64_t)blockIdx.x * C::EPB + el;
+    const bool valid = e < n_elem;
+    double* rec = lds + el * C::ELEM_DOUBLES;
     int32_t nodes[NPE];
-    double X[NPE][D];
-    load_element<D, P>(A, e, nodes, X);
-    double Te[NPE], Le[NPE];
-    gather<NPE>(A, nodes, 1, A.U, Te);
-    gather<NPE>(A, nodes, 0, A.operand, Le);
-    double Ke[NPE * NPE];
-#pragma unroll
-    for (int i = 0; i < NPE * NPE; ++i) Ke[i] = 0.0;
-    for (int q = 0; q < (D == 2 ? NQ * NQ : NQ * NQ * NQ); ++q) {
-        QP<D, NPE> Q;
-        qp_eval<D, P, NQ>(q, X, Q, false);
-        Q.T = 0.0; Q.Trate = 0.0;
-        for (int i = 0; i < D; ++i) Q.gradT[i] = 0.0;
-        qp_model<D, NPE>(Q, m, 1.0, false);
-#pragma unroll
-        for (int a = 0; a < NPE; ++a)
-#pragma unroll
-            for (int b = 0; b < NPE; ++b) Ke[a * NPE + b] += form_entry<D, NPE>(RSD_FORM_DIFFUSION, Q, a, b);
+    double Te[NPE];
+    if (valid) {
+        double X[NPE][D];
+        load_element<D, P>(A, e, nodes, X);
+        gather<NPE>(A, nodes, 1, A.U, Te);
+        phase1<D, P, NQ, false>(A, m, nullptr, 0, e, a, X, nullptr, nullptr, nullptr, 1.0, false, rec);
     }
-    double s = 0.0;
+    __syncthreads();
+    if (valid) {
+        double Ke[NPE];
 #pragma unroll
-    for (int a = 0; a < NPE; ++a) {
+        for (int b = 0; b < NPE; ++b) Ke[b] = 0.0;
+        for (int q = 0; q < C::NQP; ++q) {
+            const double* r = rec + q * C::REC;
+            double xi[3];
+            const double w = point_coords<D, NQ>(q, xi);
+            const double meas = r[C::R_MEAS];
+            const double* da = r + C::R_DN + a * D;
+#pragma unroll
+            for (int b = 0; b < NPE; ++b) Ke[b] += diff_integrand<D>(m, m.k_scalar, da, r + C::R_DN + b * D) * meas * w;
+        }
         double v = 0.0;
 #pragma unroll
-        for (int b = 0; b < NPE; ++b) v += Ke[a * NPE + b] * Te[b];
-        s += Le[a] * v;
+        for (int b = 0; b < NPE; ++b) v += Ke[b] * Te[b];
+        const int32_t ra = A.t2a[nodes[a]];
+        const double la = (ra >= 0 && A.operand) ? A.operand[ra] : 0.0;
+        rows[el * NPE + a] = la * v;
     }
-    grad[e] += -(kappa[e] * s);
+    __syncthreads();
+    if (valid && a == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) s += rows[el * NPE + b];
+        grad[e] += -(kappa[e] * s);
+    }
 }
 
 template<int D, int P, int NQ>
 static void run_adj(rsd_problem* pb, const AsmArgs& A, const Model& unit, const double* kappa, double* grad) {
+    using C = ECfg<D, P, NQ, false>;
     if (pb->n_elem)
-        hipLaunchKernelGGL((k_adj_grad<D, P, NQ>), dim3((unsigned)nblocks(pb->n_elem)), dim3(kBlock), 0,
-                           pb->ctx->stream, A, unit, kappa, grad, pb->n_elem);
+        hipLaunchKernelGGL((k_adj_grad<D, P, NQ>), dim3((unsigned)((pb->n_elem + C::EPB - 1) / C::EPB)),
+                           dim3(C::BLOCK), 0, pb->ctx->stream, A, unit, kappa, grad, pb->n_elem);
+    RSD_HIP(hipGetLastError());
+}
+
+template<int D, int P, int NQ, bool STATE>
+static void run_matrix_t(rsd_problem* pb, const AsmArgs& A) {
+    using C = ECfg<D, P, NQ, STATE>;
+    hipStream_t st = pb->ctx->stream;
+    for (int c = 0; c < pb->n_colours(); ++c) {
+        const int64_t c0 = pb->colour_off[c], c1 = pb->colour_off[c + 1];
+        if (c1 <= c0) continue;
+        hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, STATE>), dim3((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB)),
+                           dim3(C::BLOCK), 0, st, A, pb->model, c0, c1);
+    }
     RSD_HIP(hipGetLastError());
 }
 
 template<int D, int P, int NQ>
 static void run_matrix(rsd_problem* pb, const AsmArgs& A) {
+    if (A.need_state) run_matrix_t<D, P, NQ, true>(pb, A);
+    else run_matrix_t<D, P, NQ, false>(pb, A);
+}
+
+template<int D, int P, int NQ, bool STATE>
+static void run_vector_t(rsd_problem* pb, AsmArgs A, const Expr& src, int mode) {
+    using C = ECfg<D, P, NQ, STATE>;
     hipStream_t st = pb->ctx->stream;
+    const bool lift = mode == 2;
+    if (lift) A.order = pb->border.p;
+    const std::vector<int64_t>& off = lift ? pb->bcolour_off : pb->colour_off;
     for (int c = 0; c < pb->n_colours(); ++c) {
-        const int64_t c0 = pb->colour_off[c], c1 = pb->colour_off[c + 1];
+        const int64_t c0 = off[c], c1 = off[c + 1];
         if (c1 <= c0) continue;
-        hipLaunchKernelGGL((k_asm_matrix<D, P, NQ>), dim3((unsigned)nblocks(c1 - c0)), dim3(kBlock), 0, st, A,
-                           pb->model, c0, c1);
+        hipLaunchKernelGGL((k_asm_vector<D, P, NQ, STATE>), dim3((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB)),
+                           dim3(C::BLOCK), 0, st, A, pb->model, src, mode, c0, c1);
     }
     RSD_HIP(hipGetLastError());
 }
 
 template<int D, int P, int NQ>
 static void run_vector(rsd_problem* pb, const AsmArgs& A, const Expr& src, int mode) {
-    hipStream_t st = pb->ctx->stream;
-    for (int c = 0; c < pb->n_colours(); ++c) {
-        const int64_t c0 = pb->colour_off[c], c1 = pb->colour_off[c + 1];
-        if (c1 <= c0) continue;
-        hipLaunchKernelGGL((k_asm_vector<D, P, NQ>), dim3((unsigned)nblocks(c1 - c0)), dim3(kBlock), 0, st, A,
-                           pb->model, src, mode, c0, c1);
-    }
-    RSD_HIP(hipGetLastError());
+    if (A.need_state) run_vector_t<D, P, NQ, true>(pb, A, src, mode);
+    else run_vector_t<D, P, NQ, false>(pb, A, src, mode);
 }
 
 #define RSD_DISPATCH(D_, P_, NQ_, CALL)                                                      \
@@ -542,6 +709,7 @@ void assemble_vector(rsd_problem* pb, double time, int32_t nf, const int32_t* ki
     A.selector = selector;
     A.F = F;
     A.time = time;
+    A.need_state = forms_need_state(pb, A);
     update_gval(pb, time);
     launch_fill(pb->ctx, pb->n_free, F, 0.0);
     dispatch_vector(pb, A, Expr{}, 0);
@@ -562,6 +730,7 @@ void apply_essential(rsd_problem* pb, double time, int32_t nf, const int32_t* ki
     set_forms(A, nf, kinds, coefs);
     A.F = F;
     A.time = time;
+    A.need_state = forms_need_state(pb, A);
     update_gval(pb, time);
     dispatch_vector(pb, A, Expr{}, 2);
 }

@@ -3,10 +3,10 @@
 // The batch concatenates the problems' free-DOF vectors, each segment padded
 // to a multiple of 256 rows, so every 256-row block belongs to exactly one
 // problem.  A block finds its problem in a block table, walks that problem's
-// own sliced-ELL arrays, and reduces into that problem's partials with that
-// problem's ticket: the reduction tree is the one a standalone solve of the
-// problem uses, so each batched solve is bitwise identical to rsd_cg_solve_f64
-// on its own.  Converged problems drop out block by block while the others
+// own sliced-ELL arrays and stores its block partial in that problem's
+// partial range; one fold block per problem then finishes the sums.  The
+// reduction tree is the one a standalone solve of the problem uses, so each
+// batched solve is bitwise identical to rsd_cg_solve_f64 on its own.  Converged problems drop out block by block while the others
 // continue; one chunk of iterations for the whole batch is a single graph
 // launch.
 #include "rsd_batched.hpp"
@@ -60,8 +60,7 @@ __global__ void k_b_spmv(BatchDev B, int64_t total_blocks, const double* __restr
 __global__ __launch_bounds__(kBlock) void k_b_init(BatchDev B, int64_t total_blocks, const double* __restrict__ b,
                                                    const double* __restrict__ Ax, double* __restrict__ r,
                                                    double* __restrict__ p, const double* __restrict__ invd,
-                                                   CGState* st, double tol, int64_t max_iters, double* partials,
-                                                   unsigned* tickets) {
+                                                   double* partials) {
     const int64_t gb = xcd_block(blockIdx.x, total_blocks);
     int pid; int64_t lb, nb;
     locate(B, gb, pid, lb, nb);
@@ -76,30 +75,35 @@ __global__ __launch_bounds__(kBlock) void k_b_init(BatchDev B, int64_t total_blo
         v[0] = ri * ri;
         v[1] = ri * zi;
     }
+    block_partials<2>(v, lb, nb, partials + 2 * B.first_block[pid]);
+}
+
+// one block per problem
+__global__ __launch_bounds__(kBlock) void k_b_init_fold(BatchDev B, const double* __restrict__ partials, CGState* st,
+                                                        double tol, int64_t max_iters) {
+    const int pid = blockIdx.x;
+    const int64_t nb = B.first_block[pid + 1] - B.first_block[pid];
     double out[2];
-    if (grid_reduce<2>(v, lb, nb, partials + 2 * B.first_block[pid], tickets + pid, out) && threadIdx.x == 0) {
-        CGState* s = st + pid;
-        const double res0 = sqrt(out[0]);
-        s->res0 = res0;
-        s->res = res0;
-        s->rz_old = out[1];
-        s->iter = 0;
-        s->converged = 0;
-        s->stop = 0;
-        s->xdone = 0;
-        if (res0 / (res0 + 1e-50) < tol) { s->converged = 1; s->stop = 1; s->xdone = 1; }
-        else if (max_iters <= 1) { s->stop = 1; s->xdone = 1; }
-    }
+    fold_partials<2>(nb, partials + 2 * B.first_block[pid], out);
+    if (threadIdx.x != 0) return;
+    CGState* s = st + pid;
+    const double res0 = sqrt(out[0]);
+    s->res0 = res0;
+    s->res = res0;
+    s->rz_old = out[1];
+    s->iter = 0;
+    s->converged = 0;
+    s->stop = 0;
+    if (res0 / (res0 + 1e-50) < tol) { s->converged = 1; s->stop = 1; }
+    else if (max_iters <= 1) s->stop = 1;
 }
 
 __global__ __launch_bounds__(kBlock) void k_b_spmv_pap(BatchDev B, int64_t total_blocks, const double* __restrict__ p,
-                                                       double* __restrict__ Ap, CGState* st, double* partials,
-                                                       unsigned* tickets) {
+                                                       double* __restrict__ Ap, const CGState* st, double* partials) {
     const int64_t gb = xcd_block(blockIdx.x, total_blocks);
     int pid; int64_t lb, nb;
     locate(B, gb, pid, lb, nb);
-    CGState* s = st + pid;
-    if (s->stop) return;
+    if (st[pid].stop) return;
     const int64_t rl = lb * kBlock + threadIdx.x;
     const int64_t off = B.voff[pid];
     double v[1] = {0.0};
@@ -108,75 +112,78 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv_pap(BatchDev B, int64_t total
         Ap[off + rl] = y;
         v[0] = p[off + rl] * y;
     }
-    double out[1];
-    if (grid_reduce<1>(v, lb, nb, partials + 2 * B.first_block[pid], tickets + pid, out) && threadIdx.x == 0)
-        s->alpha = s->rz_old / out[0];
+    block_partials<1>(v, lb, nb, partials + 2 * B.first_block[pid]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_b_update_r(BatchDev B, int64_t total_blocks, const double* __restrict__ Ap,
+__global__ __launch_bounds__(kBlock) void k_b_alpha(BatchDev B, const double* __restrict__ partials, CGState* st) {
+    const int pid = blockIdx.x;
+    CGState* s = st + pid;
+    if (s->stop) return;
+    const int64_t nb = B.first_block[pid + 1] - B.first_block[pid];
+    double out[1];
+    fold_partials<1>(nb, partials + 2 * B.first_block[pid], out);
+    if (threadIdx.x == 0) s->alpha = s->rz_old / out[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_update_r(BatchDev B, int64_t total_blocks, double* __restrict__ x,
+                                                       const double* __restrict__ p, const double* __restrict__ Ap,
                                                        double* __restrict__ r, const double* __restrict__ invd,
-                                                       CGState* st, double tol, int64_t max_iters, double* partials,
-                                                       unsigned* tickets) {
+                                                       const CGState* st, double* partials) {
     const int64_t gb = xcd_block(blockIdx.x, total_blocks);
     int pid; int64_t lb, nb;
     locate(B, gb, pid, lb, nb);
-    CGState* s = st + pid;
+    const CGState* s = st + pid;
     if (s->stop) return;
     const int64_t rl = lb * kBlock + threadIdx.x;
     const int64_t i = B.voff[pid] + rl;
     const double alpha = s->alpha;
     double v[2] = {0.0, 0.0};
     if (rl < B.nrows[pid]) {
+        x[i] += alpha * p[i];
         const double ri = r[i] + -alpha * Ap[i];
         r[i] = ri;
         const double zi = invd ? ri * invd[i] : ri;
         v[0] = ri * ri;
         v[1] = ri * zi;
     }
+    block_partials<2>(v, lb, nb, partials + 2 * B.first_block[pid]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_beta(BatchDev B, const double* __restrict__ partials, CGState* st,
+                                                   double tol, int64_t max_iters) {
+    const int pid = blockIdx.x;
+    CGState* s = st + pid;
+    if (s->stop) return;
+    const int64_t nb = B.first_block[pid + 1] - B.first_block[pid];
     double out[2];
-    if (grid_reduce<2>(v, lb, nb, partials + 2 * B.first_block[pid], tickets + pid, out) && threadIdx.x == 0) {
-        const int64_t k = s->iter + 1;
-        s->iter = k;
-        const double res = sqrt(out[0]);
-        s->res = res;
-        if (res / (s->res0 + 1e-50) < tol) { s->converged = 1; s->stop = 1; }
-        else {
-            s->beta = out[1] / s->rz_old;
-            s->rz_old = out[1];
-            if (k >= max_iters - 1) s->stop = 1;
-        }
+    fold_partials<2>(nb, partials + 2 * B.first_block[pid], out);
+    if (threadIdx.x != 0) return;
+    const int64_t k = s->iter + 1;
+    s->iter = k;
+    const double res = sqrt(out[0]);
+    s->res = res;
+    if (res / (s->res0 + 1e-50) < tol) { s->converged = 1; s->stop = 1; }
+    else {
+        s->beta = out[1] / s->rz_old;
+        s->rz_old = out[1];
+        if (k >= max_iters - 1) s->stop = 1;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_b_update_xp(BatchDev B, int64_t total_blocks, double* __restrict__ x,
-                                                        double* __restrict__ p, const double* __restrict__ r,
-                                                        const double* __restrict__ invd, CGState* st,
-                                                        unsigned* tickets) {
+__global__ __launch_bounds__(kBlock) void k_b_update_p(BatchDev B, int64_t total_blocks, double* __restrict__ p,
+                                                       const double* __restrict__ r, const double* __restrict__ invd,
+                                                       const CGState* st) {
     const int64_t gb = xcd_block(blockIdx.x, total_blocks);
     int pid; int64_t lb, nb;
     locate(B, gb, pid, lb, nb);
-    CGState* s = st + pid;
-    if (s->xdone) return;
+    const CGState* s = st + pid;
+    if (s->stop) return;
     const int64_t rl = lb * kBlock + threadIdx.x;
     const int64_t i = B.voff[pid] + rl;
-    const double alpha = s->alpha, beta = s->beta;
-    const int32_t stop = s->stop;
+    const double beta = s->beta;
     if (rl < B.nrows[pid]) {
-        const double pi = p[i];
-        x[i] += alpha * pi;
-        if (!stop) {
-            const double zi = invd ? r[i] * invd[i] : r[i];
-            p[i] = beta * pi + 1.0 * zi;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned t = atomicAdd(tickets + pid, 1u);
-        if (t == (unsigned)(nb - 1)) {
-            s->xdone = stop;
-            tickets[pid] = 0u;
-        }
+        const double zi = invd ? r[i] * invd[i] : r[i];
+        p[i] = beta * p[i] + 1.0 * zi;
     }
 }
 
@@ -196,7 +203,6 @@ struct rsd_batch {
     DBuf<int64_t> first_block, nrows, voff;
     DBuf<double> r, p, Ap, invd;
     DBuf<CGState> st;
-    DBuf<unsigned> tickets;
     DBuf<double> partials;
     BatchDev dev() const {
         BatchDev B;
@@ -220,12 +226,14 @@ static void b_launch_iter(rsd_batch* bt, double* x, double tol, int64_t max_iter
     const BatchDev B = bt->dev();
     const int64_t nb = bt->total_blocks;
     const double* invd = precond == RSD_PC_JACOBI ? bt->invd.p : nullptr;
+    const unsigned np = (unsigned)bt->np;
     hipLaunchKernelGGL(k_b_spmv_pap, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, bt->p.p, bt->Ap.p, bt->st.p,
-                       bt->partials.p, bt->tickets.p);
-    hipLaunchKernelGGL(k_b_update_r, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, bt->Ap.p, bt->r.p, invd, bt->st.p,
-                       tol, max_iters, bt->partials.p, bt->tickets.p);
-    hipLaunchKernelGGL(k_b_update_xp, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, x, bt->p.p, bt->r.p, invd,
-                       bt->st.p, bt->tickets.p);
+                       bt->partials.p);
+    hipLaunchKernelGGL(k_b_alpha, dim3(np), dim3(kBlock), 0, st, B, bt->partials.p, bt->st.p);
+    hipLaunchKernelGGL(k_b_update_r, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, x, bt->p.p, bt->Ap.p, bt->r.p,
+                       invd, bt->st.p, bt->partials.p);
+    hipLaunchKernelGGL(k_b_beta, dim3(np), dim3(kBlock), 0, st, B, bt->partials.p, bt->st.p, tol, max_iters);
+    hipLaunchKernelGGL(k_b_update_p, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, bt->p.p, bt->r.p, invd, bt->st.p);
 }
 
 void batched_cg(rsd_batch* bt, const double* b, double* x, double tol, int64_t max_iters, int32_t precond,
@@ -237,8 +245,9 @@ void batched_cg(rsd_batch* bt, const double* b, double* x, double tol, int64_t m
         hipLaunchKernelGGL(k_b_inv_diag, dim3((unsigned)nb), dim3(kBlock), 0, st, B, bt->total_rows, bt->invd.p);
     hipLaunchKernelGGL(k_b_spmv, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, x, bt->Ap.p);
     hipLaunchKernelGGL(k_b_init, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, b, bt->Ap.p, bt->r.p, bt->p.p,
-                       precond == RSD_PC_JACOBI ? bt->invd.p : nullptr, bt->st.p, tol, max_iters, bt->partials.p,
-                       bt->tickets.p);
+                       precond == RSD_PC_JACOBI ? bt->invd.p : nullptr, bt->partials.p);
+    hipLaunchKernelGGL(k_b_init_fold, dim3((unsigned)bt->np), dim3(kBlock), 0, st, B, bt->partials.p, bt->st.p, tol,
+                       max_iters);
     RSD_HIP(hipGetLastError());
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
@@ -256,7 +265,7 @@ void batched_cg(rsd_batch* bt, const double* b, double* x, double tol, int64_t m
             RSD_HIP(hipMemcpyAsync(h, bt->st.p, sizeof(CGState) * (size_t)bt->np, hipMemcpyDeviceToHost, st));
             RSD_HIP(hipStreamSynchronize(st));
             bool all = true;
-            for (int i = 0; i < bt->np; ++i) all = all && h[i].stop && h[i].xdone;
+            for (int i = 0; i < bt->np; ++i) all = all && h[i].stop;
             if (all) break;
             RSD_CHECK(launched <= max_iters + chunk, "batched CG did not stop");
             if (ge) RSD_HIP(hipGraphLaunch(ge, st));
@@ -325,9 +334,7 @@ int rsd_batch_create(rsd_ctx* ctx, int32_t n_problems, rsd_matrix* const* mats, 
         bt->Ap.alloc(bt->total_rows);
         bt->invd.alloc(bt->total_rows);
         bt->st.alloc(n_problems);
-        bt->tickets.alloc(n_problems);
         bt->partials.alloc(2 * bt->total_blocks);
-        RSD_HIP(hipMemsetAsync(bt->tickets.p, 0, sizeof(unsigned) * n_problems, st));
         RSD_HIP(hipMemsetAsync(bt->r.p, 0, sizeof(double) * bt->total_rows, st));
         RSD_HIP(hipMemsetAsync(bt->p.p, 0, sizeof(double) * bt->total_rows, st));
         RSD_HIP(hipMemsetAsync(bt->Ap.p, 0, sizeof(double) * bt->total_rows, st));

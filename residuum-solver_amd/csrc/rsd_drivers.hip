@@ -206,7 +206,7 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
         for (;;) {
             RSD_HIP(hipMemcpyAsync(pin.p, w.st, sizeof(CGState), hipMemcpyDeviceToHost, st));
             RSD_HIP(hipStreamSynchronize(st));
-            if (pin.p->stop && pin.p->xdone) break;
+            if (pin.p->stop) break;
             RSD_CHECK(launched <= max_iters + kChunk, "be_run: CG did not stop");
             if (graphable) RSD_HIP(hipGraphLaunch(chunk_graph.e, st));
             else for (int64_t k = 0; k < kChunk; ++k) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);

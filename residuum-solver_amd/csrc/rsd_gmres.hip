@@ -6,8 +6,9 @@
 // and accumulates q_i . w for the next coefficient, which is exactly the
 // reference's dot-then-axpy sequence.  The Hessenberg column, the Givens
 // rotations and the back-substitution are scalar work done by one device
-// thread, so no scalar crosses to the host inside a cycle except the stop
-// flag.
+// thread.  A whole cycle is launched at once: after the stop flag is set
+// the remaining kernels of the cycle return immediately, and the host reads
+// the state once per cycle.
 #include "rsd_gmres.hpp"
 #include "rsd_linalg.hpp"
 #include "rsd_rowdot.hpp"
@@ -39,11 +40,12 @@ __global__ __launch_bounds__(kBlock) void k_gm_apply(int64_t n, int64_t nb, cons
     w[r] = sum;
 }
 
-// MGS step i of Arnoldi column j
+// MGS step i of Arnoldi column j: block partials of q_i . w (of w . w for
+// the norm step, Qi == nullptr)
 __global__ __launch_bounds__(kBlock) void k_gm_mgs(int64_t n, int64_t nb, double* __restrict__ w,
                                                    const double* __restrict__ Qprev, const double* __restrict__ Qi,
-                                                   int32_t i, int32_t j, int32_t m, double* H, GMState* st,
-                                                   double* partials, unsigned* ticket) {
+                                                   int32_t i, int32_t j, int32_t m, const double* H, const GMState* st,
+                                                   double* partials) {
     if (st->stop) return;
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t k = lb * kBlock + threadIdx.x;
@@ -57,11 +59,17 @@ __global__ __launch_bounds__(kBlock) void k_gm_mgs(int64_t n, int64_t nb, double
         }
         v[0] = Qi ? Qi[k] * wk : wk * wk;
     }
+    block_partials<1>(v, lb, nb, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gm_mgs_fold(int64_t nb, const double* __restrict__ partials, int32_t i,
+                                                        int32_t j, int32_t m, int32_t norm, double* H, GMState* st) {
+    if (st->stop) return;
     double out[1];
-    if (grid_reduce<1>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) {
-        if (Qi) H[(int64_t)j * (m + 1) + i] = out[0];
-        else st->hnext = sqrt(out[0]);
-    }
+    fold_partials<1>(nb, partials, out);
+    if (threadIdx.x != 0) return;
+    if (norm) st->hnext = sqrt(out[0]);
+    else H[(int64_t)j * (m + 1) + i] = out[0];
 }
 
 // q_{j+1} = w * (1 / hnext) unless the column broke down
@@ -129,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_gm_residual(int64_t n, int64_t nb, c
                                                         const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                         const double* __restrict__ val, const double* __restrict__ b,
                                                         const double* __restrict__ x, double* __restrict__ q0,
-                                                        GMState* st, int32_t first, double* partials, unsigned* ticket) {
+                                                        double* partials) {
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t k = lb * kBlock + threadIdx.x;
     double v[1] = {0.0};
@@ -138,8 +146,14 @@ __global__ __launch_bounds__(kBlock) void k_gm_residual(int64_t n, int64_t nb, c
         q0[k] = qk;
         v[0] = qk * qk;
     }
+    block_partials<1>(v, lb, nb, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gm_residual_fold(int64_t nb, const double* __restrict__ partials,
+                                                             GMState* st, int32_t first) {
     double out[1];
-    if (grid_reduce<1>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) {
+    fold_partials<1>(nb, partials, out);
+    if (threadIdx.x == 0) {
         const double beta = sqrt(out[0]);
         st->beta = beta;
         if (first) st->res0 = beta;
@@ -181,7 +195,6 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
     double** dQ = (double**)ctx->ws("gm.Qptr", sizeof(double*) * (size_t)(m + 1));
     GMState* dst = (GMState*)ctx->ws("gm.state", sizeof(GMState));
     double* part = ctx->partials(nb);
-    unsigned* tk = ctx->scratch.tickets;
     RSD_HIP(hipMemcpyAsync(dQ, hq.data(), sizeof(double*) * (size_t)(m + 1), hipMemcpyHostToDevice, st));
     RSD_HIP(hipMemsetAsync(dst, 0, sizeof(GMState), st));
     if (invd) launch_inv_diag(ctx, A, invd);
@@ -194,7 +207,8 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
     GMRESResult res{};
     try {
         hipLaunchKernelGGL(k_gm_residual, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, S.slice_ptr.p, S.row_ptr.p,
-                           S.col.p, A->val.p, b, x, hq[0], dst, 1, part, tk);
+                           S.col.p, A->val.p, b, x, hq[0], part);
+        hipLaunchKernelGGL(k_gm_residual_fold, dim3(1), dim3(kBlock), 0, st, nb, part, dst, 1);
         read();
         const double res0 = h->res0;
         res.initial_residual = res0;
@@ -211,25 +225,26 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
                 for (int j = 0; j < m; ++j) {
                     hipLaunchKernelGGL(k_gm_apply, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, S.slice_ptr.p,
                                        S.row_ptr.p, S.col.p, A->val.p, hq[j], invd, w, dst);
-                    for (int i = 0; i <= j; ++i)
+                    for (int i = 0; i <= j + 1; ++i) {
+                        const bool norm = i == j + 1;
                         hipLaunchKernelGGL(k_gm_mgs, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
-                                           i > 0 ? hq[i - 1] : nullptr, hq[i], i, j, m, H, dst, part, tk);
-                    hipLaunchKernelGGL(k_gm_mgs, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w, hq[j], nullptr,
-                                       j + 1, j, m, H, dst, part, tk);
+                                           i > 0 ? hq[i - 1] : nullptr, norm ? nullptr : hq[i], i, j, m, H, dst, part);
+                        hipLaunchKernelGGL(k_gm_mgs_fold, dim3(1), dim3(kBlock), 0, st, nb, part, i, j, m,
+                                           (int32_t)norm, H, dst);
+                    }
                     hipLaunchKernelGGL(k_gm_newq, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, w, hq[j + 1], dst);
                     hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, st, j, m, H, cs, sn, g, dst, tol, max_iters);
                     RSD_HIP(hipGetLastError());
-                    read();
-                    if (h->stop) break;
                 }
-                total = h->total;
                 hipLaunchKernelGGL(k_gm_backsub, dim3(1), dim3(1), 0, st, m, H, g, y, dst);
                 hipLaunchKernelGGL(k_gm_update_x, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, (const double* const*)dQ,
                                    y, invd, x, dst);
                 hipLaunchKernelGGL(k_gm_residual, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, S.slice_ptr.p,
-                                   S.row_ptr.p, S.col.p, A->val.p, b, x, hq[0], dst, 0, part, tk);
+                                   S.row_ptr.p, S.col.p, A->val.p, b, x, hq[0], part);
+                hipLaunchKernelGGL(k_gm_residual_fold, dim3(1), dim3(kBlock), 0, st, nb, part, dst, 0);
                 RSD_HIP(hipGetLastError());
                 read();
+                total = h->total;
                 const double beta = h->beta;
                 const bool conv = beta / (res0 + 1e-50) < tol;
                 if (conv || h->breakdown || total >= max_iters) {

@@ -75,11 +75,16 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
 // ------------------------------------------------------------------------
 // deterministic reduction
 //   block: four 64-lane trees (lane l += lane l+off, off = 32..1), then the
-//   four wave sums added left to right by thread 0;
-//   grid: each block stores its partial; the last block to finish (atomic
-//   ticket) lets lane k add partials k, k+256, ... from 0.0 and reduces the
-//   256 lane sums with the same block tree.  The result never depends on
-//   which block finishes last.
+//   four wave sums added left to right by thread 0; the block stores that
+//   partial with a plain store;
+//   fold: after the kernel boundary one 256-thread block lets lane k add
+//   partials k, k+256, ... from 0.0 and reduces the 256 lane sums with the
+//   same block tree.
+// No block waits on or signals another: on the 8-XCD part a device-scope
+// release/acquire pair per block writes back and invalidates the XCD's L2
+// (buffer_wbl2 / buffer_inv), which costs more than a second launch.  The
+// result depends only on the partition into 256-index chunks, never on the
+// launch geometry or on which block finishes first.
 // ------------------------------------------------------------------------
 template<typename T>
 __device__ __forceinline__ T wave_tree(T v) {
@@ -111,43 +116,49 @@ __device__ __forceinline__ void block_tree(double (&v)[NV], double* sm) {
     __syncthreads();
 }
 
-// Grid reduction of NV values per thread.  partials: [NV][nb] doubles,
-// ticket: one uint32 zero-initialised counter (reset by the last block).
-// Returns true in every thread of the last block, with out[] holding the
-// final sums (valid in all threads of that block).
+// Block sums of NV values per thread, stored at partials[k * nb + lb].
 template<int NV>
-__device__ __forceinline__ bool grid_reduce(double (&v)[NV], int64_t lb, int64_t nb, double* partials,
-                                            unsigned* ticket, double (&out)[NV]) {
+__device__ __forceinline__ void block_partials(double (&v)[NV], int64_t lb, int64_t nb, double* partials) {
     __shared__ double sm[NV * 4];
-    __shared__ bool last;
     block_tree<NV>(v, sm);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) partials[k * nb + lb] = v[k];
-        __threadfence();
-        const unsigned t = atomicAdd(ticket, 1u);
-        last = (t == (unsigned)(nb - 1));
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
+}
+
+// Totals of the nb partials of NV sums; call from every thread of one
+// 256-thread block, result valid in all threads.
+template<int NV>
+__device__ __forceinline__ void fold_partials(int64_t nb, const double* __restrict__ partials, double (&out)[NV]) {
+    __shared__ double sm[NV * 4];
+    __shared__ double res[NV];
+    // loads issued 8 at a time, added in index order (the fold is latency
+    // bound: a few thousand partials, one block)
+    constexpr int U = 8;
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         acc[k] = 0.0;
-        for (int64_t b = threadIdx.x; b < nb; b += kBlock) acc[k] += __hip_atomic_load(&partials[k * nb + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double* pk = partials + k * nb;
+        int64_t b = threadIdx.x;
+        for (; b + (U - 1) * kBlock < nb; b += U * kBlock) {
+            double t[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) t[u] = pk[b + u * kBlock];
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc[k] += t[u];
+        }
+        for (; b < nb; b += kBlock) acc[k] += pk[b];
     }
     block_tree<NV>(acc, sm);
-    __shared__ double res[NV];
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) res[k] = acc[k];
-        *ticket = 0u;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NV; ++k) out[k] = res[k];
-    return true;
 }
 
 // ------------------------------------------------------------------------
@@ -156,9 +167,7 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[NV], int64_t lb, int64_t
 struct Scratch {
     double* partials = nullptr;   // reduction partials
     int64_t partials_cap = 0;     // doubles
-    unsigned* tickets = nullptr;  // kTickets counters, zero when idle
 };
-constexpr int kTickets = 64;
 
 }  // namespace rsd
 

@@ -47,13 +47,18 @@ __global__ void k_mul(int64_t n, const double* __restrict__ x, const double* __r
 }
 
 __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, int64_t nb, const double* __restrict__ x,
-                                                const double* __restrict__ y, double* partials, unsigned* ticket,
-                                                double* out, int32_t take_sqrt) {
+                                                const double* __restrict__ y, double* partials) {
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t i = lb * kBlock + threadIdx.x;
     double v[1] = {i < n ? x[i] * y[i] : 0.0};
+    block_partials<1>(v, lb, nb, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_fold(int64_t nb, const double* __restrict__ partials, double* out,
+                                                     int32_t take_sqrt) {
     double r[1];
-    if (grid_reduce<1>(v, lb, nb, partials, ticket, r) && threadIdx.x == 0) *out = take_sqrt ? sqrt(r[0]) : r[0];
+    fold_partials<1>(nb, partials, r);
+    if (threadIdx.x == 0) *out = take_sqrt ? sqrt(r[0]) : r[0];
 }
 
 // ------------------------------------------------------------------------
@@ -77,19 +82,21 @@ __global__ void k_inv_diag(int64_t n, const int32_t* __restrict__ diag, const do
 }
 
 // ------------------------------------------------------------------------
-// fused PCG (cg/Solver.tpp:15-119), three kernels per iteration:
-//   spmv_pap:  Ap = A p, pAp = p.Ap, alpha = rz_old / pAp
-//   update_r:  r += (-alpha) Ap, z = M r, ||r||, r.z, convergence, beta
-//   update_xp: x += alpha p, p = beta p + z (skipped once stopped)
-// The state lives on the device; after `stop` the kernels exit at once,
-// so fixed-size chunks (or graph replays) reproduce the reference's
-// stopping iterate exactly.
+// fused PCG (cg/Solver.tpp:15-119), five launches per iteration:
+//   spmv_pap:  Ap = A p, block partials of p.Ap
+//   alpha:     (one block) alpha = rz_old / pAp
+//   update_r:  x += alpha p, r += (-alpha) Ap, z = M r, partials of r.r, r.z
+//   beta:      (one block) ||r||, convergence, beta, rz_old
+//   update_p:  p = beta p + z
+// The state lives on the device; once `stop` is set every kernel returns at
+// once, so fixed-size chunks (or graph replays) reproduce the reference's
+// stopping iterate exactly.  x is updated in update_r, before the
+// convergence test of the same iteration, as cg/Solver.tpp:40-44 does.
 // ------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_cg_init(int64_t n, int64_t nb, const double* __restrict__ b,
                                                     const double* __restrict__ Ax, double* __restrict__ r,
                                                     double* __restrict__ p, const double* __restrict__ invd,
-                                                    CGState* st, double tol, int64_t max_iters, double* hist,
-                                                    double* partials, unsigned* ticket) {
+                                                    double* partials) {
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t i = lb * kBlock + threadIdx.x;
     double v[2] = {0.0, 0.0};
@@ -101,33 +108,34 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int64_t n, int64_t nb, const
         v[0] = ri * ri;
         v[1] = ri * zi;
     }
+    block_partials<2>(v, lb, nb, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_init_fold(int64_t nb, const double* __restrict__ partials, CGState* st,
+                                                         double tol, int64_t max_iters, double* hist) {
     double out[2];
-    if (grid_reduce<2>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) {
-        const double res0 = sqrt(out[0]);
-        st->res0 = res0;
-        st->res = res0;
-        st->rz_old = out[1];
-        st->iter = 0;
-        st->converged = 0;
-        st->stop = 0;
-        st->xdone = 0;
-        if (hist) hist[0] = res0;
-        if (res0 / (res0 + 1e-50) < tol) {
-            st->converged = 1;
-            st->stop = 1;
-            st->xdone = 1;
-        } else if (max_iters <= 1) {
-            st->stop = 1;
-            st->xdone = 1;
-        }
+    fold_partials<2>(nb, partials, out);
+    if (threadIdx.x != 0) return;
+    const double res0 = sqrt(out[0]);
+    st->res0 = res0;
+    st->res = res0;
+    st->rz_old = out[1];
+    st->iter = 0;
+    st->converged = 0;
+    st->stop = 0;
+    if (hist) hist[0] = res0;
+    if (res0 / (res0 + 1e-50) < tol) {
+        st->converged = 1;
+        st->stop = 1;
+    } else if (max_iters <= 1) {
+        st->stop = 1;
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_pap(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
                                                         const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                         const double* __restrict__ val, const double* __restrict__ p,
-                                                        double* __restrict__ Ap, CGState* st, double* partials,
-                                                        unsigned* ticket) {
+                                                        double* __restrict__ Ap, const CGState* st, double* partials) {
     if (st->stop) return;
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t r = lb * kBlock + threadIdx.x;
@@ -137,69 +145,66 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_pap(int64_t n, int64_t nb, c
         Ap[r] = y;
         v[0] = p[r] * y;
     }
-    double out[1];
-    if (grid_reduce<1>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) st->alpha = st->rz_old / out[0];
+    block_partials<1>(v, lb, nb, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_update_r(int64_t n, int64_t nb, const double* __restrict__ Ap,
+__global__ __launch_bounds__(kBlock) void k_cg_alpha(int64_t nb, const double* __restrict__ partials, CGState* st) {
+    if (st->stop) return;
+    double out[1];
+    fold_partials<1>(nb, partials, out);
+    if (threadIdx.x == 0) st->alpha = st->rz_old / out[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_update_r(int64_t n, int64_t nb, double* __restrict__ x,
+                                                        const double* __restrict__ p, const double* __restrict__ Ap,
                                                         double* __restrict__ r, const double* __restrict__ invd,
-                                                        CGState* st, double tol, int64_t max_iters, double* hist,
-                                                        double* partials, unsigned* ticket) {
+                                                        const CGState* st, double* partials) {
     if (st->stop) return;
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t i = lb * kBlock + threadIdx.x;
     const double alpha = st->alpha;
     double v[2] = {0.0, 0.0};
     if (i < n) {
+        x[i] += alpha * p[i];
         const double ri = r[i] + -alpha * Ap[i];
         r[i] = ri;
         const double zi = invd ? ri * invd[i] : ri;
         v[0] = ri * ri;
         v[1] = ri * zi;
     }
+    block_partials<2>(v, lb, nb, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_beta(int64_t nb, const double* __restrict__ partials, CGState* st,
+                                                    double tol, int64_t max_iters, double* hist) {
+    if (st->stop) return;
     double out[2];
-    if (grid_reduce<2>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) {
-        const int64_t k = st->iter + 1;
-        st->iter = k;
-        const double res = sqrt(out[0]);
-        st->res = res;
-        if (hist) hist[k] = res;
-        if (res / (st->res0 + 1e-50) < tol) {
-            st->converged = 1;
-            st->stop = 1;
-        } else {
-            st->beta = out[1] / st->rz_old;
-            st->rz_old = out[1];
-            if (k >= max_iters - 1) st->stop = 1;
-        }
+    fold_partials<2>(nb, partials, out);
+    if (threadIdx.x != 0) return;
+    const int64_t k = st->iter + 1;
+    st->iter = k;
+    const double res = sqrt(out[0]);
+    st->res = res;
+    if (hist) hist[k] = res;
+    if (res / (st->res0 + 1e-50) < tol) {
+        st->converged = 1;
+        st->stop = 1;
+    } else {
+        st->beta = out[1] / st->rz_old;
+        st->rz_old = out[1];
+        if (k >= max_iters - 1) st->stop = 1;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_update_xp(int64_t n, int64_t nb, double* __restrict__ x,
-                                                         double* __restrict__ p, const double* __restrict__ r,
-                                                         const double* __restrict__ invd, CGState* st,
-                                                         unsigned* ticket) {
-    if (st->xdone) return;
-    const int64_t lb = xcd_block(blockIdx.x, nb);
-    const int64_t i = lb * kBlock + threadIdx.x;
-    const double alpha = st->alpha, beta = st->beta;
-    const int32_t stop = st->stop;
+__global__ __launch_bounds__(kBlock) void k_cg_update_p(int64_t n, int64_t nb, double* __restrict__ p,
+                                                        const double* __restrict__ r, const double* __restrict__ invd,
+                                                        const CGState* st) {
+    if (st->stop) return;
+    const int64_t i = xcd_block(blockIdx.x, nb) * kBlock + threadIdx.x;
+    const double beta = st->beta;
     if (i < n) {
-        const double pi = p[i];
-        x[i] += alpha * pi;
-        if (!stop) {
-            const double zi = invd ? r[i] * invd[i] : r[i];
-            p[i] = beta * pi + 1.0 * zi;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned t = atomicAdd(ticket, 1u);
-        if (t == (unsigned)(nb - 1)) {
-            st->xdone = stop;
-            *ticket = 0u;
-        }
+        const double zi = invd ? r[i] * invd[i] : r[i];
+        p[i] = beta * p[i] + 1.0 * zi;
     }
 }
 
@@ -230,8 +235,8 @@ void launch_inv_diag(rsd_ctx* ctx, const rsd_matrix* A, double* inv) {
 void launch_dot(rsd_ctx* ctx, int64_t n, const double* x, const double* y, double* d_out, bool take_sqrt) {
     const int64_t nb = std::max<int64_t>(1, nblocks(n));
     double* part = ctx->partials(nb);
-    hipLaunchKernelGGL(k_dot, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, x, y, part,
-                       ctx->scratch.tickets, d_out, (int32_t)take_sqrt);
+    hipLaunchKernelGGL(k_dot, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, x, y, part);
+    hipLaunchKernelGGL(k_dot_fold, dim3(1), dim3(kBlock), 0, ctx->stream, nb, part, d_out, (int32_t)take_sqrt);
 }
 
 void launch_axpy(rsd_ctx* ctx, int64_t n, double a, const double* x, double* y) {
@@ -268,8 +273,8 @@ void cg_launch_init(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* 
     if (precond == RSD_PC_JACOBI) launch_inv_diag(ctx, A, w.invd);
     launch_spmv_f64(ctx, A, x, w.Ap);
     hipLaunchKernelGGL(k_cg_init, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, b, w.Ap, w.r, w.p,
-                       precond == RSD_PC_JACOBI ? w.invd : nullptr, w.st, tol, max_iters, w.hist, part,
-                       ctx->scratch.tickets);
+                       precond == RSD_PC_JACOBI ? w.invd : nullptr, part);
+    hipLaunchKernelGGL(k_cg_init_fold, dim3(1), dim3(kBlock), 0, ctx->stream, nb, part, w.st, tol, max_iters, w.hist);
 }
 
 void cg_launch_iter(rsd_ctx* ctx, const rsd_matrix* A, double* x, double tol, int64_t max_iters, int32_t precond,
@@ -278,12 +283,14 @@ void cg_launch_iter(rsd_ctx* ctx, const rsd_matrix* A, double* x, double tol, in
     const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n));
     double* part = ctx->partials(2 * nb);
     const double* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
-    hipLaunchKernelGGL(k_cg_spmv_pap, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, S.slice_ptr.p,
-                       S.row_ptr.p, S.col.p, A->val.p, w.p, w.Ap, w.st, part, ctx->scratch.tickets);
-    hipLaunchKernelGGL(k_cg_update_r, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, w.Ap, w.r, invd,
-                       w.st, tol, max_iters, w.hist, part, ctx->scratch.tickets);
-    hipLaunchKernelGGL(k_cg_update_xp, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, x, w.p, w.r, invd,
-                       w.st, ctx->scratch.tickets);
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_cg_spmv_pap, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p, S.row_ptr.p,
+                       S.col.p, A->val.p, w.p, w.Ap, w.st, part);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, s, nb, part, w.st);
+    hipLaunchKernelGGL(k_cg_update_r, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, x, w.p, w.Ap, w.r, invd, w.st,
+                       part);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kBlock), 0, s, nb, part, w.st, tol, max_iters, w.hist);
+    hipLaunchKernelGGL(k_cg_update_p, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.p, w.r, invd, w.st);
 }
 
 void cg_report(const CGState& s, int64_t max_iters, rsd_report* rep) {
@@ -307,7 +314,7 @@ CGState cg_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* x, do
         RSD_HIP(hipMemcpyAsync(pinned, w.st, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
         RSD_HIP(hipStreamSynchronize(ctx->stream));
         h = *pinned;
-        if (h.stop && h.xdone) break;
+        if (h.stop) break;
         RSD_CHECK(done_iters <= max_iters + 2, "CG did not stop within max_iters launches");
         const int64_t c = std::min<int64_t>(chunk, std::max<int64_t>(1, max_iters - done_iters));
         for (int64_t k = 0; k < c; ++k) cg_launch_iter(ctx, A, x, tol, max_iters, precond, w);

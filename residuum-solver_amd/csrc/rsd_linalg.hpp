@@ -9,7 +9,7 @@ namespace rsd {
 struct CGState {
     double rz_old, alpha, beta, res0, res;
     int64_t iter;
-    int32_t converged, stop, xdone, pad;
+    int32_t converged, stop;
 };
 
 struct CGWork {

@@ -64,12 +64,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv_f32(int64_t n, int64_t nb, cons
     if (r < n) y[r] = row_dot_f32<OFF16>(r, slice_ptr, rp, col, off, val, x);
 }
 
-// outer residual r = b - A x (FP64), ||r||, inner right-hand side r/||r||
+// outer residual r = b - A x (FP64), partials of ||r||^2
 __global__ __launch_bounds__(kBlock) void k_mx_residual(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
                                                         const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                         const double* __restrict__ val, const double* __restrict__ b,
                                                         const double* __restrict__ x, double* __restrict__ r64,
-                                                        MixedState* st, double* partials, unsigned* ticket) {
+                                                        double* partials) {
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t i = lb * kBlock + threadIdx.x;
     double v[1] = {0.0};
@@ -78,8 +78,14 @@ __global__ __launch_bounds__(kBlock) void k_mx_residual(int64_t n, int64_t nb, c
         r64[i] = ri;
         v[0] = ri * ri;
     }
+    block_partials<1>(v, lb, nb, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mx_residual_fold(int64_t nb, const double* __restrict__ partials,
+                                                             MixedState* st) {
     double out[1];
-    if (grid_reduce<1>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) {
+    fold_partials<1>(nb, partials, out);
+    if (threadIdx.x == 0) {
         const double res = sqrt(out[0]);
         if (st->cycles == 0) st->res0 = res;
         st->res = res;
@@ -90,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_residual(int64_t n, int64_t nb, c
 __global__ __launch_bounds__(kBlock) void k_mx_init(int64_t n, int64_t nb, const double* __restrict__ r64,
                                                     float* __restrict__ r, float* __restrict__ p,
                                                     float* __restrict__ d, const float* __restrict__ invd,
-                                                    MixedState* st, double* partials, unsigned* ticket) {
+                                                    const MixedState* st, double* partials) {
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t i = lb * kBlock + threadIdx.x;
     const double s = st->res > 0.0 ? 1.0 / st->res : 0.0;
@@ -104,13 +110,18 @@ __global__ __launch_bounds__(kBlock) void k_mx_init(int64_t n, int64_t nb, const
         v[0] = (double)ri * (double)ri;
         v[1] = (double)ri * (double)zi;
     }
+    block_partials<2>(v, lb, nb, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mx_init_fold(int64_t nb, const double* __restrict__ partials,
+                                                         MixedState* st) {
     double out[2];
-    if (grid_reduce<2>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) {
+    fold_partials<2>(nb, partials, out);
+    if (threadIdx.x == 0) {
         st->in_res0 = sqrt(out[0]);
         st->rz_old = out[1];
         st->in_iter = 0;
         st->stop = (st->in_res0 == 0.0) ? 1 : 0;
-        st->xdone = st->stop;
     }
 }
 
@@ -119,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_spmv_pap(int64_t n, int64_t nb, c
                                                         const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                         const int16_t* __restrict__ off, const float* __restrict__ val,
                                                         const float* __restrict__ p, float* __restrict__ Ap,
-                                                        MixedState* st, double* partials, unsigned* ticket) {
+                                                        const MixedState* st, double* partials) {
     if (st->stop) return;
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t r = lb * kBlock + threadIdx.x;
@@ -129,66 +140,63 @@ __global__ __launch_bounds__(kBlock) void k_mx_spmv_pap(int64_t n, int64_t nb, c
         Ap[r] = y;
         v[0] = (double)p[r] * (double)y;
     }
-    double out[1];
-    if (grid_reduce<1>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) st->alpha = st->rz_old / out[0];
+    block_partials<1>(v, lb, nb, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_mx_update_r(int64_t n, int64_t nb, const float* __restrict__ Ap,
+__global__ __launch_bounds__(kBlock) void k_mx_alpha(int64_t nb, const double* __restrict__ partials, MixedState* st) {
+    if (st->stop) return;
+    double out[1];
+    fold_partials<1>(nb, partials, out);
+    if (threadIdx.x == 0) st->alpha = st->rz_old / out[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_mx_update_r(int64_t n, int64_t nb, float* __restrict__ d,
+                                                        const float* __restrict__ p, const float* __restrict__ Ap,
                                                         float* __restrict__ r, const float* __restrict__ invd,
-                                                        MixedState* st, double* partials, unsigned* ticket) {
+                                                        const MixedState* st, double* partials) {
     if (st->stop) return;
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t i = lb * kBlock + threadIdx.x;
     const float alpha = (float)st->alpha;
     double v[2] = {0.0, 0.0};
     if (i < n) {
+        d[i] = __builtin_fmaf(alpha, p[i], d[i]);
         const float ri = __builtin_fmaf(-alpha, Ap[i], r[i]);
         r[i] = ri;
         const float zi = invd ? ri * invd[i] : ri;
         v[0] = (double)ri * (double)ri;
         v[1] = (double)ri * (double)zi;
     }
+    block_partials<2>(v, lb, nb, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mx_beta(int64_t nb, const double* __restrict__ partials, MixedState* st) {
+    if (st->stop) return;
     double out[2];
-    if (grid_reduce<2>(v, lb, nb, partials, ticket, out) && threadIdx.x == 0) {
-        const int64_t k = st->in_iter + 1;
-        st->in_iter = k;
-        st->total_inner += 1;
-        const double res = sqrt(out[0]);
-        if (res / (st->in_res0 + 1e-50) < st->in_tol || !(res == res)) {
-            st->stop = 1;
-        } else {
-            st->beta = out[1] / st->rz_old;
-            st->rz_old = out[1];
-            if (k >= st->in_max) st->stop = 1;
-        }
+    fold_partials<2>(nb, partials, out);
+    if (threadIdx.x != 0) return;
+    const int64_t k = st->in_iter + 1;
+    st->in_iter = k;
+    st->total_inner += 1;
+    const double res = sqrt(out[0]);
+    if (res / (st->in_res0 + 1e-50) < st->in_tol || !(res == res)) {
+        st->stop = 1;
+    } else {
+        st->beta = out[1] / st->rz_old;
+        st->rz_old = out[1];
+        if (k >= st->in_max) st->stop = 1;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_mx_update_xp(int64_t n, int64_t nb, float* __restrict__ d,
-                                                         float* __restrict__ p, const float* __restrict__ r,
-                                                         const float* __restrict__ invd, MixedState* st,
-                                                         unsigned* ticket) {
-    if (st->xdone) return;
-    const int64_t lb = xcd_block(blockIdx.x, nb);
-    const int64_t i = lb * kBlock + threadIdx.x;
-    const float alpha = (float)st->alpha, beta = (float)st->beta;
-    const int32_t stop = st->stop;
+__global__ __launch_bounds__(kBlock) void k_mx_update_p(int64_t n, int64_t nb, float* __restrict__ p,
+                                                        const float* __restrict__ r, const float* __restrict__ invd,
+                                                        const MixedState* st) {
+    if (st->stop) return;
+    const int64_t i = xcd_block(blockIdx.x, nb) * kBlock + threadIdx.x;
+    const float beta = (float)st->beta;
     if (i < n) {
-        const float pi = p[i];
-        d[i] = __builtin_fmaf(alpha, pi, d[i]);
-        if (!stop) {
-            const float zi = invd ? r[i] * invd[i] : r[i];
-            p[i] = __builtin_fmaf(beta, pi, zi);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned t = atomicAdd(ticket, 1u);
-        if (t == (unsigned)(nb - 1)) {
-            st->xdone = stop;
-            *ticket = 0u;
-        }
+        const float zi = invd ? r[i] * invd[i] : r[i];
+        p[i] = __builtin_fmaf(beta, p[i], zi);
     }
 }
 
@@ -244,18 +252,18 @@ void mixed_launch_inner_iter(rsd_ctx* ctx, const rsd_matrix* A, int32_t precond,
     const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n));
     double* part = ctx->partials(2 * nb);
     const float* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
+    hipStream_t s = ctx->stream;
     if (S.has_off16)
-        hipLaunchKernelGGL(k_mx_spmv_pap<true>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb,
-                           S.slice_ptr.p, S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.p, w.Ap, w.st, part,
-                           ctx->scratch.tickets);
+        hipLaunchKernelGGL(k_mx_spmv_pap<true>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p,
+                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.p, w.Ap, w.st, part);
     else
-        hipLaunchKernelGGL(k_mx_spmv_pap<false>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb,
-                           S.slice_ptr.p, S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.p, w.Ap, w.st, part,
-                           ctx->scratch.tickets);
-    hipLaunchKernelGGL(k_mx_update_r, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, w.Ap, w.r, invd, w.st,
-                       part, ctx->scratch.tickets);
-    hipLaunchKernelGGL(k_mx_update_xp, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, w.d, w.p, w.r, invd,
-                       w.st, ctx->scratch.tickets);
+        hipLaunchKernelGGL(k_mx_spmv_pap<false>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p,
+                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.p, w.Ap, w.st, part);
+    hipLaunchKernelGGL(k_mx_alpha, dim3(1), dim3(kBlock), 0, s, nb, part, w.st);
+    hipLaunchKernelGGL(k_mx_update_r, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.d, w.p, w.Ap, w.r, invd, w.st,
+                       part);
+    hipLaunchKernelGGL(k_mx_beta, dim3(1), dim3(kBlock), 0, s, nb, part, w.st);
+    hipLaunchKernelGGL(k_mx_update_p, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.p, w.r, invd, w.st);
 }
 
 // Full mixed solve; A must hold current FP64 values (FP32 copy refreshed here
@@ -278,7 +286,8 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
     try {
         for (int cycle = 0;; ++cycle) {
             hipLaunchKernelGGL(k_mx_residual, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, S.slice_ptr.p, S.row_ptr.p,
-                               S.col.p, A->val.p, b, x, w.r64, w.st, part, ctx->scratch.tickets);
+                               S.col.p, A->val.p, b, x, w.r64, part);
+            hipLaunchKernelGGL(k_mx_residual_fold, dim3(1), dim3(kBlock), 0, st, nb, part, w.st);
             RSD_HIP(hipMemcpyAsync(h, w.st, sizeof(MixedState), hipMemcpyDeviceToHost, st));
             RSD_HIP(hipStreamSynchronize(st));
             const double rel = h->res / (h->res0 + 1e-50);
@@ -292,14 +301,15 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
             h->cycles = cycle + 1;
             RSD_HIP(hipMemcpyAsync(w.st, h, sizeof(MixedState), hipMemcpyHostToDevice, st));
             hipLaunchKernelGGL(k_mx_init, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w.r64, w.r, w.p, w.d,
-                               precond == RSD_PC_JACOBI ? w.invd : nullptr, w.st, part, ctx->scratch.tickets);
+                               precond == RSD_PC_JACOBI ? w.invd : nullptr, w.st, part);
+            hipLaunchKernelGGL(k_mx_init_fold, dim3(1), dim3(kBlock), 0, st, nb, part, w.st);
             for (int64_t launched = 0;;) {
                 if (runner) runner->run();
                 else for (int k = 0; k < 8; ++k) mixed_launch_inner_iter(ctx, A, precond, w);
                 launched += 8;
                 RSD_HIP(hipMemcpyAsync(h, w.st, sizeof(MixedState), hipMemcpyDeviceToHost, st));
                 RSD_HIP(hipStreamSynchronize(st));
-                if (h->stop && h->xdone) break;
+                if (h->stop) break;
                 RSD_CHECK(launched <= max_iters + 16, "mixed CG: inner solve did not stop");
             }
             hipLaunchKernelGGL(k_mx_update_x, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, st, n, x, w.d, w.st);

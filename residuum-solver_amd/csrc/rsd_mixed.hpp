@@ -10,7 +10,7 @@ struct MixedState {
     double in_res0, in_tol;    // inner solve
     double rz_old, alpha, beta;
     int64_t in_iter, in_max, total_inner;
-    int32_t cycles, converged, stop, xdone;
+    int32_t cycles, converged, stop, pad;
 };
 
 struct MixedWork {

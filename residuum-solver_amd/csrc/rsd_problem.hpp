@@ -45,6 +45,7 @@ struct rsd_problem {
     std::vector<int32_t> h_rp, h_ci;   // CSR pattern, int32
     std::vector<int32_t> h_order;      // colour-major element order
     std::vector<int64_t> colour_off;   // [n_colours + 1] into h_order
+    std::vector<int64_t> bcolour_off;  // [n_colours + 1] into border
     // Dirichlet registry
     std::vector<int32_t> bc_tags;
     std::vector<rsd::Expr> bc_exprs;
@@ -59,6 +60,7 @@ struct rsd_problem {
     rsd::DBuf<int32_t> t2a;       // [n_nodes]
     rsd::DBuf<int32_t> ctag;      // [n_nodes]
     rsd::DBuf<int32_t> order;     // [n_elem] colour-major
+    rsd::DBuf<int32_t> border;    // elements with a constrained node, colour-major
     rsd::DBuf<int32_t> map_sell;  // [n_elem * npe * npe] SELL slot or -1
     rsd::DBuf<double> gval;       // [n_nodes] Dirichlet value (0 on free nodes)
     rsd::DBuf<double> kappa;      // [n_elem] or empty

@@ -265,8 +265,6 @@ int rsd_ctx_create(int device, rsd_ctx** out) {
         c->device = device;
         RSD_HIP(hipSetDevice(device));
         RSD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        RSD_HIP(hipMalloc(&c->scratch.tickets, sizeof(unsigned) * kTickets));
-        RSD_HIP(hipMemsetAsync(c->scratch.tickets, 0, sizeof(unsigned) * kTickets, c->stream));
         RSD_HIP(hipMalloc(&c->d_scalars, sizeof(double) * 64));
         c->partials(1 << 16);
         RSD_HIP(hipStreamSynchronize(c->stream));
@@ -280,7 +278,6 @@ int rsd_ctx_destroy(rsd_ctx* ctx) {
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
         if (ctx->scratch.partials) (void)hipFree(ctx->scratch.partials);
-        if (ctx->scratch.tickets) (void)hipFree(ctx->scratch.tickets);
         if (ctx->d_scalars) (void)hipFree(ctx->d_scalars);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
@@ -514,6 +511,22 @@ static void build_problem(rsd_problem* pb) {
     {
         std::vector<int64_t> cur(pb->colour_off.begin(), pb->colour_off.end() - 1);
         for (int64_t e = 0; e < ne; ++e) pb->h_order[cur[pb->h_colour[e]]++] = (int32_t)e;
+    }
+    // the elements touching a constrained node, in the same colour-major
+    // order: the essential lift only adds exact zeros on the others
+    {
+        std::vector<int32_t> border;
+        pb->bcolour_off.assign(nc + 1, 0);
+        for (int c = 0; c < nc; ++c) {
+            for (int64_t k = pb->colour_off[c]; k < pb->colour_off[c + 1]; ++k) {
+                const int32_t e = pb->h_order[k];
+                bool touches = false;
+                for (int a = 0; a < npe; ++a) touches = touches || pb->h_t2a[pb->h_ien[(int64_t)e * npe + a]] < 0;
+                if (touches) border.push_back(e);
+            }
+            pb->bcolour_off[c + 1] = (int64_t)border.size();
+        }
+        pb->border.upload(border.data(), (int64_t)border.size(), st);
     }
     // device copies
     pb->xyz.upload(pb->h_xyz.data(), nn * dim, st);
