@@ -7,8 +7,9 @@
  *   - terms t_i, blocks of 256 consecutive indices, zero padded;
  *   - in a block: four 64-lane trees (lane l += lane l+off, off = 32..1),
  *     then the four wave sums added left to right;
- *   - across B blocks: lane k (k < 256) adds partials k, k+256, ... left to
- *     right from 0.0, and the 256 lane sums go through the block tree.
+ *   - across B blocks: lane k (k < 1024) adds block sums k, k+1024, ...
+ *     left to right from 0.0; the 1024 lane sums go through sixteen 64-lane
+ *     trees and the sixteen wave sums are added left to right.
  */
 #include "oracle.h"
 
@@ -23,16 +24,15 @@ void orc__element_matrix(const orc_problem* P, int64_t e, double time, int32_t n
 /* ------------------------------------------------------------------ */
 /* reductions                                                           */
 /* ------------------------------------------------------------------ */
-static double block_tree(double* v) {
-    double w[4];
-    for (int k = 0; k < 4; ++k) {
+/* nw 64-lane trees over v[64 * nw], wave sums added left to right */
+static double wave_trees(double* v, int nw) {
+    double r = 0.0;
+    for (int k = 0; k < nw; ++k) {
         double* s = v + 64 * k;
         for (int off = 32; off >= 1; off >>= 1)
             for (int l = 0; l < off; ++l) s[l] = s[l] + s[l + off];
-        w[k] = s[0];
+        r = k == 0 ? s[0] : r + s[0];
     }
-    double r = w[0];
-    r += w[1]; r += w[2]; r += w[3];
     return r;
 }
 
@@ -40,17 +40,17 @@ typedef double (*term_fn)(int64_t i, const void* ctx);
 
 static double gpu_reduce(int64_t n, term_fn f, const void* ctx) {
     const int64_t B = (n + 255) / 256;
-    double lane[256];
-    for (int k = 0; k < 256; ++k) lane[k] = 0.0;
+    double lane[1024];
     double v[256];
+    for (int k = 0; k < 1024; ++k) lane[k] = 0.0;
     for (int64_t b = 0; b < B; ++b) {
         for (int t = 0; t < 256; ++t) {
             const int64_t i = b * 256 + t;
             v[t] = (i < n) ? f(i, ctx) : 0.0;
         }
-        lane[b % 256] += block_tree(v);
+        lane[b % 1024] += wave_trees(v, 4);
     }
-    return block_tree(lane);
+    return wave_trees(lane, 16);
 }
 
 typedef struct { const double* x; const double* y; } xy_t;

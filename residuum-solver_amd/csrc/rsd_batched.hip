@@ -18,6 +18,13 @@
 
 namespace rsd {
 
+// per-problem PCG state (cg/Solver.tpp:15-119 scalars)
+struct BState {
+    double rz_old, alpha, beta, res0, res;
+    int64_t iter;
+    int32_t converged, stop;
+};
+
 struct BatchDev {
     const int64_t* const* slice_ptr;  // per problem
     const int32_t* const* rp;
@@ -79,14 +86,14 @@ __global__ __launch_bounds__(kBlock) void k_b_init(BatchDev B, int64_t total_blo
 }
 
 // one block per problem
-__global__ __launch_bounds__(kBlock) void k_b_init_fold(BatchDev B, const double* __restrict__ partials, CGState* st,
+__global__ __launch_bounds__(kFold) void k_b_init_fold(BatchDev B, const double* __restrict__ partials, BState* st,
                                                         double tol, int64_t max_iters) {
     const int pid = blockIdx.x;
     const int64_t nb = B.first_block[pid + 1] - B.first_block[pid];
     double out[2];
     fold_partials<2>(nb, partials + 2 * B.first_block[pid], out);
     if (threadIdx.x != 0) return;
-    CGState* s = st + pid;
+    BState* s = st + pid;
     const double res0 = sqrt(out[0]);
     s->res0 = res0;
     s->res = res0;
@@ -99,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_b_init_fold(BatchDev B, const double
 }
 
 __global__ __launch_bounds__(kBlock) void k_b_spmv_pap(BatchDev B, int64_t total_blocks, const double* __restrict__ p,
-                                                       double* __restrict__ Ap, const CGState* st, double* partials) {
+                                                       double* __restrict__ Ap, const BState* st, double* partials) {
     const int64_t gb = xcd_block(blockIdx.x, total_blocks);
     int pid; int64_t lb, nb;
     locate(B, gb, pid, lb, nb);
@@ -115,9 +122,9 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv_pap(BatchDev B, int64_t total
     block_partials<1>(v, lb, nb, partials + 2 * B.first_block[pid]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_b_alpha(BatchDev B, const double* __restrict__ partials, CGState* st) {
+__global__ __launch_bounds__(kFold) void k_b_alpha(BatchDev B, const double* __restrict__ partials, BState* st) {
     const int pid = blockIdx.x;
-    CGState* s = st + pid;
+    BState* s = st + pid;
     if (s->stop) return;
     const int64_t nb = B.first_block[pid + 1] - B.first_block[pid];
     double out[1];
@@ -128,11 +135,11 @@ __global__ __launch_bounds__(kBlock) void k_b_alpha(BatchDev B, const double* __
 __global__ __launch_bounds__(kBlock) void k_b_update_r(BatchDev B, int64_t total_blocks, double* __restrict__ x,
                                                        const double* __restrict__ p, const double* __restrict__ Ap,
                                                        double* __restrict__ r, const double* __restrict__ invd,
-                                                       const CGState* st, double* partials) {
+                                                       const BState* st, double* partials) {
     const int64_t gb = xcd_block(blockIdx.x, total_blocks);
     int pid; int64_t lb, nb;
     locate(B, gb, pid, lb, nb);
-    const CGState* s = st + pid;
+    const BState* s = st + pid;
     if (s->stop) return;
     const int64_t rl = lb * kBlock + threadIdx.x;
     const int64_t i = B.voff[pid] + rl;
@@ -149,10 +156,10 @@ __global__ __launch_bounds__(kBlock) void k_b_update_r(BatchDev B, int64_t total
     block_partials<2>(v, lb, nb, partials + 2 * B.first_block[pid]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_b_beta(BatchDev B, const double* __restrict__ partials, CGState* st,
+__global__ __launch_bounds__(kFold) void k_b_beta(BatchDev B, const double* __restrict__ partials, BState* st,
                                                    double tol, int64_t max_iters) {
     const int pid = blockIdx.x;
-    CGState* s = st + pid;
+    BState* s = st + pid;
     if (s->stop) return;
     const int64_t nb = B.first_block[pid + 1] - B.first_block[pid];
     double out[2];
@@ -172,11 +179,11 @@ __global__ __launch_bounds__(kBlock) void k_b_beta(BatchDev B, const double* __r
 
 __global__ __launch_bounds__(kBlock) void k_b_update_p(BatchDev B, int64_t total_blocks, double* __restrict__ p,
                                                        const double* __restrict__ r, const double* __restrict__ invd,
-                                                       const CGState* st) {
+                                                       const BState* st) {
     const int64_t gb = xcd_block(blockIdx.x, total_blocks);
     int pid; int64_t lb, nb;
     locate(B, gb, pid, lb, nb);
-    const CGState* s = st + pid;
+    const BState* s = st + pid;
     if (s->stop) return;
     const int64_t rl = lb * kBlock + threadIdx.x;
     const int64_t i = B.voff[pid] + rl;
@@ -202,7 +209,7 @@ struct rsd_batch {
     DBuf<int32_t> blk2prob;
     DBuf<int64_t> first_block, nrows, voff;
     DBuf<double> r, p, Ap, invd;
-    DBuf<CGState> st;
+    DBuf<BState> st;
     DBuf<double> partials;
     BatchDev dev() const {
         BatchDev B;
@@ -229,15 +236,15 @@ static void b_launch_iter(rsd_batch* bt, double* x, double tol, int64_t max_iter
     const unsigned np = (unsigned)bt->np;
     hipLaunchKernelGGL(k_b_spmv_pap, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, bt->p.p, bt->Ap.p, bt->st.p,
                        bt->partials.p);
-    hipLaunchKernelGGL(k_b_alpha, dim3(np), dim3(kBlock), 0, st, B, bt->partials.p, bt->st.p);
+    hipLaunchKernelGGL(k_b_alpha, dim3(np), dim3(kFold), 0, st, B, bt->partials.p, bt->st.p);
     hipLaunchKernelGGL(k_b_update_r, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, x, bt->p.p, bt->Ap.p, bt->r.p,
                        invd, bt->st.p, bt->partials.p);
-    hipLaunchKernelGGL(k_b_beta, dim3(np), dim3(kBlock), 0, st, B, bt->partials.p, bt->st.p, tol, max_iters);
+    hipLaunchKernelGGL(k_b_beta, dim3(np), dim3(kFold), 0, st, B, bt->partials.p, bt->st.p, tol, max_iters);
     hipLaunchKernelGGL(k_b_update_p, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, bt->p.p, bt->r.p, invd, bt->st.p);
 }
 
 void batched_cg(rsd_batch* bt, const double* b, double* x, double tol, int64_t max_iters, int32_t precond,
-                bool use_graph, std::vector<CGState>& out, int64_t chunk) {
+                bool use_graph, std::vector<BState>& out, int64_t chunk) {
     hipStream_t st = bt->ctx->stream;
     const BatchDev B = bt->dev();
     const int64_t nb = bt->total_blocks;
@@ -246,7 +253,7 @@ void batched_cg(rsd_batch* bt, const double* b, double* x, double tol, int64_t m
     hipLaunchKernelGGL(k_b_spmv, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, x, bt->Ap.p);
     hipLaunchKernelGGL(k_b_init, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, b, bt->Ap.p, bt->r.p, bt->p.p,
                        precond == RSD_PC_JACOBI ? bt->invd.p : nullptr, bt->partials.p);
-    hipLaunchKernelGGL(k_b_init_fold, dim3((unsigned)bt->np), dim3(kBlock), 0, st, B, bt->partials.p, bt->st.p, tol,
+    hipLaunchKernelGGL(k_b_init_fold, dim3((unsigned)bt->np), dim3(kFold), 0, st, B, bt->partials.p, bt->st.p, tol,
                        max_iters);
     RSD_HIP(hipGetLastError());
     hipGraph_t g = nullptr;
@@ -257,12 +264,12 @@ void batched_cg(rsd_batch* bt, const double* b, double* x, double tol, int64_t m
         RSD_HIP(hipStreamEndCapture(st, &g));
         RSD_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     }
-    CGState* h = nullptr;
-    RSD_HIP(hipHostMalloc((void**)&h, sizeof(CGState) * (size_t)bt->np, hipHostMallocDefault));
+    BState* h = nullptr;
+    RSD_HIP(hipHostMalloc((void**)&h, sizeof(BState) * (size_t)bt->np, hipHostMallocDefault));
     int64_t launched = 0;
     try {
         for (;;) {
-            RSD_HIP(hipMemcpyAsync(h, bt->st.p, sizeof(CGState) * (size_t)bt->np, hipMemcpyDeviceToHost, st));
+            RSD_HIP(hipMemcpyAsync(h, bt->st.p, sizeof(BState) * (size_t)bt->np, hipMemcpyDeviceToHost, st));
             RSD_HIP(hipStreamSynchronize(st));
             bool all = true;
             for (int i = 0; i < bt->np; ++i) all = all && h[i].stop;
@@ -364,10 +371,18 @@ int rsd_cg_solve_batched(rsd_ctx* ctx, rsd_batch* bt, const double* b, double* x
     return guard([&] {
         RSD_CHECK(ctx && bt && b && x && ctx == bt->ctx, "rsd_cg_solve_batched: bad argument");
         RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
-        std::vector<CGState> s;
+        std::vector<BState> s;
         batched_cg(bt, b, x, tol, max_iters, precond, use_graph != 0, s, 8);
         if (reps)
-            for (int i = 0; i < bt->np; ++i) cg_report(s[i], max_iters, &reps[i]);
+            for (int i = 0; i < bt->np; ++i) {
+                CGState c{};
+                c.res0 = s[i].res0;
+                c.res = s[i].res;
+                c.iter = s[i].iter;
+                c.converged = s[i].converged;
+                c.stop = s[i].stop;
+                cg_report(c, max_iters, &reps[i]);
+            }
     });
 }
 

@@ -142,6 +142,8 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
     Pinned pin;
     Graph step_graph, start_graph, chunk_graph;
     double graph_dt = -1.0;
+    int64_t last_iters = 0;
+    int64_t mixed_hint[kMixedMaxCycles + 1] = {};
     double time = t0;
     int64_t step = 0;
     while (!(time >= tf - 1e-9)) {
@@ -168,7 +170,7 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
             gc.st = st;
             gc.g = &chunk_graph;
             const MixedState ms = mixed_run(ctx, &K, b.F, U, tol, max_iters, precond, 1e-5, false, mw,
-                                            graphable ? &gc : nullptr);
+                                            graphable ? &gc : nullptr, mixed_hint);
             ev.rec(ev.c, st);
             ev.accumulate();
             if (iters) iters[step] = ms.total_inner;
@@ -203,18 +205,26 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
             for (int64_t k = 0; k < kChunk; ++k) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);
         }
         int64_t launched = kChunk;
+        auto launch_chunk = [&] {
+            if (graphable) RSD_HIP(hipGraphLaunch(chunk_graph.e, st));
+            else for (int64_t k = 0; k < kChunk; ++k) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);
+            launched += kChunk;
+        };
+        // queue the chunks the previous step needed (minus a margin) before
+        // the first look at the stop flag: one host round trip per step
+        // instead of one per chunk when the iteration count is steady
+        for (int64_t ahead = (last_iters - 2) / kChunk - 1; ahead > 0 && launched < max_iters; --ahead) launch_chunk();
         for (;;) {
             RSD_HIP(hipMemcpyAsync(pin.p, w.st, sizeof(CGState), hipMemcpyDeviceToHost, st));
             RSD_HIP(hipStreamSynchronize(st));
             if (pin.p->stop) break;
             RSD_CHECK(launched <= max_iters + kChunk, "be_run: CG did not stop");
-            if (graphable) RSD_HIP(hipGraphLaunch(chunk_graph.e, st));
-            else for (int64_t k = 0; k < kChunk; ++k) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);
-            launched += kChunk;
+            launch_chunk();
         }
         ev.rec(ev.c, st);
         ev.accumulate();
         const CGState s = *pin.p;
+        last_iters = s.iter;
         if (iters) iters[step] = s.converged ? s.iter : max_iters;
         if (!s.converged) throw Error(RSD_ERR_NOT_CONVERGED, "be_run: CG did not converge in step " + std::to_string(step));
         launch_copy(ctx, n, U, b.Uprev);

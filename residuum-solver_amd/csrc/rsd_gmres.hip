@@ -33,9 +33,9 @@ __global__ __launch_bounds__(kBlock) void k_gm_apply(int64_t n, int64_t nb, cons
     double sum = 0.0;
     for (int j = 0; j < len; ++j) {
         const int64_t s = base + (int64_t)j * kSlice;
-        const int c = col[s];
+        const int c = stream_load(col + s);
         const double z = invd ? q[c] * invd[c] : q[c];
-        sum += val[s] * z;
+        sum += stream_load(val + s) * z;
     }
     w[r] = sum;
 }
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kBlock) void k_gm_mgs(int64_t n, int64_t nb, double
     block_partials<1>(v, lb, nb, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_gm_mgs_fold(int64_t nb, const double* __restrict__ partials, int32_t i,
+__global__ __launch_bounds__(kFold) void k_gm_mgs_fold(int64_t nb, const double* __restrict__ partials, int32_t i,
                                                         int32_t j, int32_t m, int32_t norm, double* H, GMState* st) {
     if (st->stop) return;
     double out[1];
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_gm_residual(int64_t n, int64_t nb, c
     block_partials<1>(v, lb, nb, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_gm_residual_fold(int64_t nb, const double* __restrict__ partials,
+__global__ __launch_bounds__(kFold) void k_gm_residual_fold(int64_t nb, const double* __restrict__ partials,
                                                              GMState* st, int32_t first) {
     double out[1];
     fold_partials<1>(nb, partials, out);
@@ -208,7 +208,7 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
     try {
         hipLaunchKernelGGL(k_gm_residual, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, S.slice_ptr.p, S.row_ptr.p,
                            S.col.p, A->val.p, b, x, hq[0], part);
-        hipLaunchKernelGGL(k_gm_residual_fold, dim3(1), dim3(kBlock), 0, st, nb, part, dst, 1);
+        hipLaunchKernelGGL(k_gm_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part, dst, 1);
         read();
         const double res0 = h->res0;
         res.initial_residual = res0;
@@ -229,7 +229,7 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
                         const bool norm = i == j + 1;
                         hipLaunchKernelGGL(k_gm_mgs, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
                                            i > 0 ? hq[i - 1] : nullptr, norm ? nullptr : hq[i], i, j, m, H, dst, part);
-                        hipLaunchKernelGGL(k_gm_mgs_fold, dim3(1), dim3(kBlock), 0, st, nb, part, i, j, m,
+                        hipLaunchKernelGGL(k_gm_mgs_fold, dim3(1), dim3(kFold), 0, st, nb, part, i, j, m,
                                            (int32_t)norm, H, dst);
                     }
                     hipLaunchKernelGGL(k_gm_newq, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, w, hq[j + 1], dst);
@@ -241,7 +241,7 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
                                    y, invd, x, dst);
                 hipLaunchKernelGGL(k_gm_residual, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, S.slice_ptr.p,
                                    S.row_ptr.p, S.col.p, A->val.p, b, x, hq[0], part);
-                hipLaunchKernelGGL(k_gm_residual_fold, dim3(1), dim3(kBlock), 0, st, nb, part, dst, 0);
+                hipLaunchKernelGGL(k_gm_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part, dst, 0);
                 RSD_HIP(hipGetLastError());
                 read();
                 total = h->total;

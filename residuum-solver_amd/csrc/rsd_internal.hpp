@@ -74,18 +74,23 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
 
 // ------------------------------------------------------------------------
 // deterministic reduction
-//   block: four 64-lane trees (lane l += lane l+off, off = 32..1), then the
-//   four wave sums added left to right by thread 0; the block stores that
-//   partial with a plain store;
-//   fold: after the kernel boundary one 256-thread block lets lane k add
-//   partials k, k+256, ... from 0.0 and reduces the 256 lane sums with the
-//   same block tree.
+//   block: the terms of 256 consecutive indices (one term per thread of a
+//   256-thread block) go through four 64-lane trees (lane l += lane l+off,
+//   off = 32..1); thread 0 adds the four wave sums left to right and stores
+//   the block sum with a plain store;
+//   fold: after the kernel boundary one kFold-thread block lets lane k
+//   (k < 1024) add block sums k, k+1024, ... from 0.0; the 1024 lane sums
+//   go through sixteen 64-lane trees whose sums are added left to right.
 // No block waits on or signals another: on the 8-XCD part a device-scope
 // release/acquire pair per block writes back and invalidates the XCD's L2
-// (buffer_wbl2 / buffer_inv), which costs more than a second launch.  The
-// result depends only on the partition into 256-index chunks, never on the
-// launch geometry or on which block finishes first.
+// (buffer_wbl2 / buffer_inv), and having every consumer block fold the
+// partials itself makes thousands of blocks read the same lines; a one-block
+// fold kernel costs less than either.  The result depends only on the
+// partition into 256-index blocks, never on the launch order or on which
+// block finishes first.
 // ------------------------------------------------------------------------
+constexpr int kFold = 1024;  // threads of a fold kernel
+
 template<typename T>
 __device__ __forceinline__ T wave_tree(T v) {
 #pragma unroll
@@ -93,9 +98,11 @@ __device__ __forceinline__ T wave_tree(T v) {
     return v;
 }
 
-// returns the block sum in thread 0; sm must hold 4 entries per value
+// Block sums of NV values per thread (256-thread block), stored at
+// partials[k * nb + lb].
 template<int NV>
-__device__ __forceinline__ void block_tree(double (&v)[NV], double* sm) {
+__device__ __forceinline__ void block_partials(double (&v)[NV], int64_t lb, int64_t nb, double* partials) {
+    __shared__ double sm[NV * 4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -110,51 +117,52 @@ __device__ __forceinline__ void block_tree(double (&v)[NV], double* sm) {
             r += sm[k * 4 + 1];
             r += sm[k * 4 + 2];
             r += sm[k * 4 + 3];
-            v[k] = r;
+            partials[k * nb + lb] = r;
         }
     }
-    __syncthreads();
 }
 
-// Block sums of NV values per thread, stored at partials[k * nb + lb].
-template<int NV>
-__device__ __forceinline__ void block_partials(double (&v)[NV], int64_t lb, int64_t nb, double* partials) {
-    __shared__ double sm[NV * 4];
-    block_tree<NV>(v, sm);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) partials[k * nb + lb] = v[k];
-    }
-}
-
-// Totals of the nb partials of NV sums; call from every thread of one
-// 256-thread block, result valid in all threads.
+// Totals of the nb block sums of NV values; call from every thread of one
+// kFold-thread block, result valid in all threads.  A lane's loads are all
+// issued before its adds: the fold is latency bound.
 template<int NV>
 __device__ __forceinline__ void fold_partials(int64_t nb, const double* __restrict__ partials, double (&out)[NV]) {
-    __shared__ double sm[NV * 4];
+    __shared__ double sm[NV * 16];
     __shared__ double res[NV];
-    // loads issued 8 at a time, added in index order (the fold is latency
-    // bound: a few thousand partials, one block)
-    constexpr int U = 8;
+    constexpr int U = 16;
+    const int t = threadIdx.x;
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         acc[k] = 0.0;
         const double* pk = partials + k * nb;
-        int64_t b = threadIdx.x;
-        for (; b + (U - 1) * kBlock < nb; b += U * kBlock) {
-            double t[U];
+        for (int64_t b0 = t; b0 < nb; b0 += (int64_t)U * kFold) {
+            double v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) t[u] = pk[b + u * kBlock];
+            for (int u = 0; u < U; ++u) {
+                const int64_t b = b0 + (int64_t)u * kFold;
+                v[u] = b < nb ? pk[b] : 0.0;
+            }
 #pragma unroll
-            for (int u = 0; u < U; ++u) acc[k] += t[u];
+            for (int u = 0; u < U; ++u)
+                if (b0 + (int64_t)u * kFold < nb) acc[k] += v[u];
         }
-        for (; b < nb; b += kBlock) acc[k] += pk[b];
     }
-    block_tree<NV>(acc, sm);
-    if (threadIdx.x == 0) {
+    const int lane = t & 63, wave = t >> 6;
 #pragma unroll
-        for (int k = 0; k < NV; ++k) res[k] = acc[k];
+    for (int k = 0; k < NV; ++k) {
+        const double w = wave_tree(acc[k]);
+        if (lane == 0) sm[k * 16 + wave] = w;
+    }
+    __syncthreads();
+    if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double r = sm[k * 16];
+#pragma unroll
+            for (int i = 1; i < 16; ++i) r += sm[k * 16 + i];
+            res[k] = r;
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -173,6 +181,7 @@ struct Scratch {
 
 struct rsd_ctx {
     int device = 0;
+    int n_cu = 256;               // compute units of the device
     hipStream_t stream = nullptr;
     rsd::Scratch scratch;
     double* d_scalars = nullptr;  // small device scalar area for BLAS-1 results

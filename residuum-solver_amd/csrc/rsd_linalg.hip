@@ -54,8 +54,8 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, int64_t nb, const dou
     block_partials<1>(v, lb, nb, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_dot_fold(int64_t nb, const double* __restrict__ partials, double* out,
-                                                     int32_t take_sqrt) {
+__global__ __launch_bounds__(kFold) void k_dot_fold(int64_t nb, const double* __restrict__ partials, double* out,
+                                                    int32_t take_sqrt) {
     double r[1];
     fold_partials<1>(nb, partials, r);
     if (threadIdx.x == 0) *out = take_sqrt ? sqrt(r[0]) : r[0];
@@ -83,9 +83,9 @@ __global__ void k_inv_diag(int64_t n, const int32_t* __restrict__ diag, const do
 
 // ------------------------------------------------------------------------
 // fused PCG (cg/Solver.tpp:15-119), five launches per iteration:
-//   spmv_pap:  Ap = A p, block partials of p.Ap
+//   spmv_pap:  Ap = A p, block sums of p.Ap
 //   alpha:     (one block) alpha = rz_old / pAp
-//   update_r:  x += alpha p, r += (-alpha) Ap, z = M r, partials of r.r, r.z
+//   update_r:  x += alpha p, r += (-alpha) Ap, z = M r, block sums of r.r, r.z
 //   beta:      (one block) ||r||, convergence, beta, rz_old
 //   update_p:  p = beta p + z
 // The state lives on the device; once `stop` is set every kernel returns at
@@ -111,8 +111,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int64_t n, int64_t nb, const
     block_partials<2>(v, lb, nb, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_init_fold(int64_t nb, const double* __restrict__ partials, CGState* st,
-                                                         double tol, int64_t max_iters, double* hist) {
+__global__ __launch_bounds__(kFold) void k_cg_init_fold(int64_t nb, const double* __restrict__ partials, CGState* st,
+                                                        double tol, int64_t max_iters, double* hist) {
     double out[2];
     fold_partials<2>(nb, partials, out);
     if (threadIdx.x != 0) return;
@@ -148,11 +148,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_pap(int64_t n, int64_t nb, c
     block_partials<1>(v, lb, nb, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_alpha(int64_t nb, const double* __restrict__ partials, CGState* st) {
+__global__ __launch_bounds__(kFold) void k_cg_alpha(int64_t nb, const double* __restrict__ partials, CGState* st) {
     if (st->stop) return;
+    const double rz = st->rz_old;
     double out[1];
     fold_partials<1>(nb, partials, out);
-    if (threadIdx.x == 0) st->alpha = st->rz_old / out[0];
+    if (threadIdx.x == 0) st->alpha = rz / out[0];
 }
 
 __global__ __launch_bounds__(kBlock) void k_cg_update_r(int64_t n, int64_t nb, double* __restrict__ x,
@@ -175,22 +176,23 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_r(int64_t n, int64_t nb, d
     block_partials<2>(v, lb, nb, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_beta(int64_t nb, const double* __restrict__ partials, CGState* st,
-                                                    double tol, int64_t max_iters, double* hist) {
+__global__ __launch_bounds__(kFold) void k_cg_beta(int64_t nb, const double* __restrict__ partials, CGState* st,
+                                                   double tol, int64_t max_iters, double* hist) {
     if (st->stop) return;
+    const int64_t k = st->iter + 1;
+    const double res0 = st->res0, rz_old = st->rz_old;
     double out[2];
     fold_partials<2>(nb, partials, out);
     if (threadIdx.x != 0) return;
-    const int64_t k = st->iter + 1;
     st->iter = k;
     const double res = sqrt(out[0]);
     st->res = res;
     if (hist) hist[k] = res;
-    if (res / (st->res0 + 1e-50) < tol) {
+    if (res / (res0 + 1e-50) < tol) {
         st->converged = 1;
         st->stop = 1;
     } else {
-        st->beta = out[1] / st->rz_old;
+        st->beta = out[1] / rz_old;
         st->rz_old = out[1];
         if (k >= max_iters - 1) st->stop = 1;
     }
@@ -236,7 +238,7 @@ void launch_dot(rsd_ctx* ctx, int64_t n, const double* x, const double* y, doubl
     const int64_t nb = std::max<int64_t>(1, nblocks(n));
     double* part = ctx->partials(nb);
     hipLaunchKernelGGL(k_dot, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, x, y, part);
-    hipLaunchKernelGGL(k_dot_fold, dim3(1), dim3(kBlock), 0, ctx->stream, nb, part, d_out, (int32_t)take_sqrt);
+    hipLaunchKernelGGL(k_dot_fold, dim3(1), dim3(kFold), 0, ctx->stream, nb, part, d_out, (int32_t)take_sqrt);
 }
 
 void launch_axpy(rsd_ctx* ctx, int64_t n, double a, const double* x, double* y) {
@@ -269,27 +271,27 @@ CGWork cg_workspace(rsd_ctx* ctx, int64_t n, int64_t max_iters, bool want_hist) 
 void cg_launch_init(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
                     int32_t precond, const CGWork& w) {
     const int64_t n = A->pb->sell.n_rows, nb = std::max<int64_t>(1, nblocks(n));
-    double* part = ctx->partials(2 * nb);
+    double* part = ctx->partials(3 * nb);
     if (precond == RSD_PC_JACOBI) launch_inv_diag(ctx, A, w.invd);
     launch_spmv_f64(ctx, A, x, w.Ap);
     hipLaunchKernelGGL(k_cg_init, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, b, w.Ap, w.r, w.p,
                        precond == RSD_PC_JACOBI ? w.invd : nullptr, part);
-    hipLaunchKernelGGL(k_cg_init_fold, dim3(1), dim3(kBlock), 0, ctx->stream, nb, part, w.st, tol, max_iters, w.hist);
+    hipLaunchKernelGGL(k_cg_init_fold, dim3(1), dim3(kFold), 0, ctx->stream, nb, part, w.st, tol, max_iters, w.hist);
 }
 
 void cg_launch_iter(rsd_ctx* ctx, const rsd_matrix* A, double* x, double tol, int64_t max_iters, int32_t precond,
                     const CGWork& w) {
     const Sell& S = A->pb->sell;
     const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n));
-    double* part = ctx->partials(2 * nb);
+    double* part = ctx->partials(3 * nb);
     const double* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     hipStream_t s = ctx->stream;
     hipLaunchKernelGGL(k_cg_spmv_pap, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p, S.row_ptr.p,
                        S.col.p, A->val.p, w.p, w.Ap, w.st, part);
-    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, s, nb, part, w.st);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kFold), 0, s, nb, part, w.st);
     hipLaunchKernelGGL(k_cg_update_r, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, x, w.p, w.Ap, w.r, invd, w.st,
-                       part);
-    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kBlock), 0, s, nb, part, w.st, tol, max_iters, w.hist);
+                       part + nb);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kFold), 0, s, nb, part + nb, w.st, tol, max_iters, w.hist);
     hipLaunchKernelGGL(k_cg_update_p, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.p, w.r, invd, w.st);
 }
 

@@ -5,7 +5,8 @@
 
 namespace rsd {
 
-// device-resident PCG state (cg/Solver.tpp:15-119 scalars)
+// device-resident PCG state (cg/Solver.tpp:15-119 scalars); only the
+// one-block fold kernels write it
 struct CGState {
     double rz_old, alpha, beta, res0, res;
     int64_t iter;

@@ -6,10 +6,17 @@
 //                  relative criterion as cg/Solver.tpp:45-48;
 //   inner (FP32):  solve A d = r / ||r|| with FP32 values (and int16
 //                  row-relative column offsets when every offset fits),
-//                  FP32 vectors and FP32-accumulated SpMV rows; the dot
-//                  products behind alpha / beta are accumulated in FP64;
+//                  FP32 vectors and FP32-accumulated SpMV rows, by the
+//                  Chronopoulos-Gear form of PCG (one fused reduction of
+//                  three dot products per iteration, accumulated in FP64):
+//                    u = M r, w = A u, gamma = r.u, delta = w.u,
+//                    beta = gamma / gamma', alpha = gamma / (delta - beta gamma / alpha'),
+//                    p = u + beta p, s = w + beta s, d += alpha p, r -= alpha s;
 //   update (FP64): x += ||r|| d.
-// Per inner iteration the matrix stream is 6 bytes per slot instead of 12.
+// Per inner iteration the matrix stream is 6 bytes per slot instead of 12,
+// and an iteration is three launches: the SpMV with the dot products, the
+// one-block fold of the dot products, and the vector update.
+#include "rsd_linalg.hpp"
 #include "rsd_mixed.hpp"
 #include "rsd_rowdot.hpp"
 
@@ -23,29 +30,30 @@ __device__ __forceinline__ float row_dot_f32(int64_t r, const int64_t* __restric
                                              const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                              const int16_t* __restrict__ off, const float* __restrict__ val,
                                              const float* __restrict__ x) {
+    constexpr int U = 8;
     const int64_t base = slice_ptr[r >> 6] + (r & 63);
     const int len = rp[r + 1] - rp[r];
     float sum = 0.0f;
     int j = 0;
-    for (; j + 4 <= len; j += 4) {
+    for (; j + U <= len; j += U) {
         const int64_t s0 = base + (int64_t)j * kSlice;
-        int c[4];
-        float v[4];
+        int c[U];
+        float v[U];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            c[k] = OFF16 ? (int)r + (int)off[s0 + k * kSlice] : col[s0 + k * kSlice];
-            v[k] = val[s0 + k * kSlice];
+        for (int k = 0; k < U; ++k) {
+            c[k] = OFF16 ? (int)r + (int)stream_load(off + s0 + k * kSlice) : stream_load(col + s0 + k * kSlice);
+            v[k] = stream_load(val + s0 + k * kSlice);
         }
-        float xv[4];
+        float xv[U];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) xv[k] = x[c[k]];
+        for (int k = 0; k < U; ++k) xv[k] = x[c[k]];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) sum = __builtin_fmaf(v[k], xv[k], sum);
+        for (int k = 0; k < U; ++k) sum = __builtin_fmaf(v[k], xv[k], sum);
     }
     for (; j < len; ++j) {
         const int64_t s0 = base + (int64_t)j * kSlice;
-        const int c = OFF16 ? (int)r + (int)off[s0] : col[s0];
-        sum = __builtin_fmaf(val[s0], x[c], sum);
+        const int c = OFF16 ? (int)r + (int)stream_load(off + s0) : stream_load(col + s0);
+        sum = __builtin_fmaf(stream_load(val + s0), x[c], sum);
     }
     return sum;
 }
@@ -64,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_f32(int64_t n, int64_t nb, cons
     if (r < n) y[r] = row_dot_f32<OFF16>(r, slice_ptr, rp, col, off, val, x);
 }
 
-// outer residual r = b - A x (FP64), partials of ||r||^2
+// outer residual r = b - A x (FP64), block sums of ||r||^2
 __global__ __launch_bounds__(kBlock) void k_mx_residual(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
                                                         const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                                         const double* __restrict__ val, const double* __restrict__ b,
@@ -81,8 +89,8 @@ __global__ __launch_bounds__(kBlock) void k_mx_residual(int64_t n, int64_t nb, c
     block_partials<1>(v, lb, nb, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_mx_residual_fold(int64_t nb, const double* __restrict__ partials,
-                                                             MixedState* st) {
+__global__ __launch_bounds__(kFold) void k_mx_residual_fold(int64_t nb, const double* __restrict__ partials,
+                                                            MixedState* st) {
     double out[1];
     fold_partials<1>(nb, partials, out);
     if (threadIdx.x == 0) {
@@ -92,112 +100,93 @@ __global__ __launch_bounds__(kBlock) void k_mx_residual_fold(int64_t nb, const d
     }
 }
 
-// inner set-up: r32 = r64 / ||r||, d = 0, z = M r, p = z
-__global__ __launch_bounds__(kBlock) void k_mx_init(int64_t n, int64_t nb, const double* __restrict__ r64,
-                                                    float* __restrict__ r, float* __restrict__ p,
-                                                    float* __restrict__ d, const float* __restrict__ invd,
-                                                    const MixedState* st, double* partials) {
-    const int64_t lb = xcd_block(blockIdx.x, nb);
-    const int64_t i = lb * kBlock + threadIdx.x;
-    const double s = st->res > 0.0 ? 1.0 / st->res : 0.0;
-    double v[2] = {0.0, 0.0};
-    if (i < n) {
-        const float ri = (float)(r64[i] * s);
-        const float zi = invd ? ri * invd[i] : ri;
-        r[i] = ri;
-        p[i] = zi;
-        d[i] = 0.0f;
-        v[0] = (double)ri * (double)ri;
-        v[1] = (double)ri * (double)zi;
-    }
-    block_partials<2>(v, lb, nb, partials);
+// inner set-up: r = r64 / ||r||, u = M r, p = s = d = 0
+__global__ void k_mx_init(int64_t n, const double* __restrict__ r64, float* __restrict__ r, float* __restrict__ u,
+                          float* __restrict__ p, float* __restrict__ s, float* __restrict__ d,
+                          const float* __restrict__ invd, const MixedState* st) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double sc = st->res > 0.0 ? 1.0 / st->res : 0.0;
+    const float ri = (float)(r64[i] * sc);
+    r[i] = ri;
+    u[i] = invd ? ri * invd[i] : ri;
+    p[i] = 0.0f;
+    s[i] = 0.0f;
+    d[i] = 0.0f;
 }
 
-__global__ __launch_bounds__(kBlock) void k_mx_init_fold(int64_t nb, const double* __restrict__ partials,
-                                                         MixedState* st) {
-    double out[2];
-    fold_partials<2>(nb, partials, out);
-    if (threadIdx.x == 0) {
-        st->in_res0 = sqrt(out[0]);
-        st->rz_old = out[1];
-        st->in_iter = 0;
-        st->stop = (st->in_res0 == 0.0) ? 1 : 0;
-    }
-}
-
+// w = A u, block sums of r.u, w.u, r.r
 template<bool OFF16>
-__global__ __launch_bounds__(kBlock) void k_mx_spmv_pap(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
-                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                        const int16_t* __restrict__ off, const float* __restrict__ val,
-                                                        const float* __restrict__ p, float* __restrict__ Ap,
-                                                        const MixedState* st, double* partials) {
-    if (st->stop) return;
-    const int64_t lb = xcd_block(blockIdx.x, nb);
-    const int64_t r = lb * kBlock + threadIdx.x;
-    double v[1] = {0.0};
-    if (r < n) {
-        const float y = row_dot_f32<OFF16>(r, slice_ptr, rp, col, off, val, p);
-        Ap[r] = y;
-        v[0] = (double)p[r] * (double)y;
-    }
-    block_partials<1>(v, lb, nb, partials);
-}
-
-__global__ __launch_bounds__(kBlock) void k_mx_alpha(int64_t nb, const double* __restrict__ partials, MixedState* st) {
-    if (st->stop) return;
-    double out[1];
-    fold_partials<1>(nb, partials, out);
-    if (threadIdx.x == 0) st->alpha = st->rz_old / out[0];
-}
-
-__global__ __launch_bounds__(kBlock) void k_mx_update_r(int64_t n, int64_t nb, float* __restrict__ d,
-                                                        const float* __restrict__ p, const float* __restrict__ Ap,
-                                                        float* __restrict__ r, const float* __restrict__ invd,
-                                                        const MixedState* st, double* partials) {
+__global__ __launch_bounds__(kBlock) void k_mx_spmv_dots(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
+                                                         const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                         const int16_t* __restrict__ off, const float* __restrict__ val,
+                                                         const float* __restrict__ u, const float* __restrict__ r,
+                                                         float* __restrict__ w, const MixedState* st, double* P) {
     if (st->stop) return;
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t i = lb * kBlock + threadIdx.x;
-    const float alpha = (float)st->alpha;
-    double v[2] = {0.0, 0.0};
+    double v[3] = {0.0, 0.0, 0.0};
     if (i < n) {
-        d[i] = __builtin_fmaf(alpha, p[i], d[i]);
-        const float ri = __builtin_fmaf(-alpha, Ap[i], r[i]);
-        r[i] = ri;
-        const float zi = invd ? ri * invd[i] : ri;
-        v[0] = (double)ri * (double)ri;
-        v[1] = (double)ri * (double)zi;
+        const float wi = row_dot_f32<OFF16>(i, slice_ptr, rp, col, off, val, u);
+        w[i] = wi;
+        const double ui = (double)u[i], ri = (double)r[i];
+        v[0] = ri * ui;
+        v[1] = (double)wi * ui;
+        v[2] = ri * ri;
     }
-    block_partials<2>(v, lb, nb, partials);
+    block_partials<3>(v, lb, nb, P);
 }
 
-__global__ __launch_bounds__(kBlock) void k_mx_beta(int64_t nb, const double* __restrict__ partials, MixedState* st) {
+// fold gamma, delta, ||r||^2: convergence, alpha, beta
+__global__ __launch_bounds__(kFold) void k_mx_scalars(int64_t nb, const double* __restrict__ P, MixedState* st) {
     if (st->stop) return;
-    double out[2];
-    fold_partials<2>(nb, partials, out);
+    const int64_t k = st->it;
+    const double gam_prev = st->gam, alp_prev = st->alp, in_res0 = st->in_res0, in_tol = st->in_tol;
+    const int64_t in_max = st->in_max;
+    double o[3];
+    fold_partials<3>(nb, P, o);
     if (threadIdx.x != 0) return;
-    const int64_t k = st->in_iter + 1;
-    st->in_iter = k;
+    const double gam = o[0], del = o[1], res = sqrt(o[2]);
+    const double r0 = k == 0 ? res : in_res0;
     st->total_inner += 1;
-    const double res = sqrt(out[0]);
-    if (res / (st->in_res0 + 1e-50) < st->in_tol || !(res == res)) {
+    if (k == 0) st->in_res0 = res;
+    if (res / (r0 + 1e-50) < in_tol || !(res == res) || k >= in_max) {
         st->stop = 1;
-    } else {
-        st->beta = out[1] / st->rz_old;
-        st->rz_old = out[1];
-        if (k >= st->in_max) st->stop = 1;
+        return;
     }
+    double alpha, beta;
+    if (k == 0) {
+        beta = 0.0;
+        alpha = gam / del;
+    } else {
+        beta = gam / gam_prev;
+        alpha = gam / (del - beta * gam / alp_prev);
+    }
+    st->alpha = alpha;
+    st->beta = beta;
+    st->gam = gam;
+    st->alp = alpha;
+    st->it = k + 1;
 }
 
-__global__ __launch_bounds__(kBlock) void k_mx_update_p(int64_t n, int64_t nb, float* __restrict__ p,
-                                                        const float* __restrict__ r, const float* __restrict__ invd,
-                                                        const MixedState* st) {
+// p = u + beta p, s = w + beta s, d += alpha p, r -= alpha s, u = M r
+__global__ __launch_bounds__(kBlock) void k_mx_update(int64_t n, int64_t nb, float* __restrict__ p,
+                                                      float* __restrict__ s, float* __restrict__ d,
+                                                      float* __restrict__ r, float* __restrict__ u,
+                                                      const float* __restrict__ w, const float* __restrict__ invd,
+                                                      const MixedState* st) {
     if (st->stop) return;
     const int64_t i = xcd_block(blockIdx.x, nb) * kBlock + threadIdx.x;
-    const float beta = (float)st->beta;
-    if (i < n) {
-        const float zi = invd ? r[i] * invd[i] : r[i];
-        p[i] = __builtin_fmaf(beta, p[i], zi);
-    }
+    if (i >= n) return;
+    const float a = (float)st->alpha, bt = (float)st->beta;
+    const float pi = __builtin_fmaf(bt, p[i], u[i]);
+    const float si = __builtin_fmaf(bt, s[i], w[i]);
+    p[i] = pi;
+    s[i] = si;
+    d[i] = __builtin_fmaf(a, pi, d[i]);
+    const float ri = __builtin_fmaf(-a, si, r[i]);
+    r[i] = ri;
+    u[i] = invd ? ri * invd[i] : ri;
 }
 
 __global__ void k_mx_update_x(int64_t n, double* __restrict__ x, const float* __restrict__ d, const MixedState* st) {
@@ -239,8 +228,10 @@ MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n) {
     const size_t f = sizeof(float) * (size_t)std::max<int64_t>(n, 1), dd = sizeof(double) * (size_t)std::max<int64_t>(n, 1);
     w.r64 = (double*)ctx->ws("mx.r64", dd);
     w.r = (float*)ctx->ws("mx.r", f);
+    w.u = (float*)ctx->ws("mx.u", f);
+    w.w = (float*)ctx->ws("mx.w", f);
     w.p = (float*)ctx->ws("mx.p", f);
-    w.Ap = (float*)ctx->ws("mx.Ap", f);
+    w.s = (float*)ctx->ws("mx.s", f);
     w.d = (float*)ctx->ws("mx.d", f);
     w.invd = (float*)ctx->ws("mx.invd", f);
     w.st = (MixedState*)ctx->ws("mx.state", sizeof(MixedState));
@@ -250,31 +241,33 @@ MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n) {
 void mixed_launch_inner_iter(rsd_ctx* ctx, const rsd_matrix* A, int32_t precond, const MixedWork& w) {
     const Sell& S = A->pb->sell;
     const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n));
-    double* part = ctx->partials(2 * nb);
+    double* P = ctx->partials(3 * nb);
     const float* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     hipStream_t s = ctx->stream;
     if (S.has_off16)
-        hipLaunchKernelGGL(k_mx_spmv_pap<true>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p,
-                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.p, w.Ap, w.st, part);
+        hipLaunchKernelGGL(k_mx_spmv_dots<true>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p,
+                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.u, w.r, w.w, w.st, P);
     else
-        hipLaunchKernelGGL(k_mx_spmv_pap<false>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p,
-                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.p, w.Ap, w.st, part);
-    hipLaunchKernelGGL(k_mx_alpha, dim3(1), dim3(kBlock), 0, s, nb, part, w.st);
-    hipLaunchKernelGGL(k_mx_update_r, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.d, w.p, w.Ap, w.r, invd, w.st,
-                       part);
-    hipLaunchKernelGGL(k_mx_beta, dim3(1), dim3(kBlock), 0, s, nb, part, w.st);
-    hipLaunchKernelGGL(k_mx_update_p, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.p, w.r, invd, w.st);
+        hipLaunchKernelGGL(k_mx_spmv_dots<false>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p,
+                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.u, w.r, w.w, w.st, P);
+    hipLaunchKernelGGL(k_mx_scalars, dim3(1), dim3(kFold), 0, s, nb, P, w.st);
+    hipLaunchKernelGGL(k_mx_update, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.p, w.s, w.d, w.r, w.u, w.w, invd,
+                       w.st);
 }
+
+constexpr int64_t kInnerChunk = 8;
 
 // Full mixed solve; A must hold current FP64 values (FP32 copy refreshed here
 // unless fresh_f32 is already true).  Returns the final state.
 MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
-                     int32_t precond, double inner_floor, bool fresh_f32, const MixedWork& w, ChunkRunner* runner) {
+                     int32_t precond, double inner_floor, bool fresh_f32, const MixedWork& w, ChunkRunner* runner,
+                     int64_t* hint) {
     const Sell& S = A->pb->sell;
     const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n));
     hipStream_t st = ctx->stream;
     if (!fresh_f32) matrix_to_f32(ctx, A);
-    double* part = ctx->partials(2 * nb);
+    double* part = ctx->partials(3 * nb);
+    const float* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     if (precond == RSD_PC_JACOBI)
         hipLaunchKernelGGL(k_inv_diag_f32, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, st, n, S.diag.p, A->val.p, w.invd);
     MixedState* h = nullptr;
@@ -283,35 +276,46 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
     h->in_max = max_iters;
     RSD_HIP(hipMemcpyAsync(w.st, h, sizeof(MixedState), hipMemcpyHostToDevice, st));
     MixedState out{};
+    auto chunk = [&] {
+        if (runner) runner->run();
+        else for (int64_t k = 0; k < kInnerChunk; ++k) mixed_launch_inner_iter(ctx, A, precond, w);
+    };
     try {
         for (int cycle = 0;; ++cycle) {
             hipLaunchKernelGGL(k_mx_residual, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, S.slice_ptr.p, S.row_ptr.p,
                                S.col.p, A->val.p, b, x, w.r64, part);
-            hipLaunchKernelGGL(k_mx_residual_fold, dim3(1), dim3(kBlock), 0, st, nb, part, w.st);
+            hipLaunchKernelGGL(k_mx_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part, w.st);
             RSD_HIP(hipMemcpyAsync(h, w.st, sizeof(MixedState), hipMemcpyDeviceToHost, st));
             RSD_HIP(hipStreamSynchronize(st));
             const double rel = h->res / (h->res0 + 1e-50);
             if (rel < tol) { h->converged = 1; break; }
-            if (cycle >= 40 || h->total_inner >= max_iters || !(rel == rel)) break;
+            if (cycle >= kMixedMaxCycles || h->total_inner >= max_iters || !(rel == rel)) break;
             // inner tolerance: what is still missing, never below the FP32 floor
             double it = 0.5 * tol * (h->res0 + 1e-50) / h->res;
             it = std::min(0.5, std::max(inner_floor, it));
             h->in_tol = it;
             h->in_max = std::max<int64_t>(1, max_iters - h->total_inner);
             h->cycles = cycle + 1;
+            h->it = 0;
+            h->stop = 0;
             RSD_HIP(hipMemcpyAsync(w.st, h, sizeof(MixedState), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_mx_init, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w.r64, w.r, w.p, w.d,
-                               precond == RSD_PC_JACOBI ? w.invd : nullptr, w.st, part);
-            hipLaunchKernelGGL(k_mx_init_fold, dim3(1), dim3(kBlock), 0, st, nb, part, w.st);
-            for (int64_t launched = 0;;) {
-                if (runner) runner->run();
-                else for (int k = 0; k < 8; ++k) mixed_launch_inner_iter(ctx, A, precond, w);
-                launched += 8;
+            hipLaunchKernelGGL(k_mx_init, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, st, n, w.r64, w.r, w.u, w.p, w.s,
+                               w.d, invd, w.st);
+            const int64_t before = h->total_inner;
+            // queue what the same cycle of the previous solve needed (minus a
+            // margin) before the first look at the stop flag
+            const int64_t ahead = (hint && hint[cycle] > 2) ? std::max<int64_t>(1, (hint[cycle] - 2) / kInnerChunk) : 1;
+            int64_t launched = 0;
+            for (int64_t k = 0; k < ahead; ++k) { chunk(); launched += kInnerChunk; }
+            for (;;) {
                 RSD_HIP(hipMemcpyAsync(h, w.st, sizeof(MixedState), hipMemcpyDeviceToHost, st));
                 RSD_HIP(hipStreamSynchronize(st));
                 if (h->stop) break;
-                RSD_CHECK(launched <= max_iters + 16, "mixed CG: inner solve did not stop");
+                RSD_CHECK(launched <= max_iters + 2 * kInnerChunk, "mixed CG: inner solve did not stop");
+                chunk();
+                launched += kInnerChunk;
             }
+            if (hint) hint[cycle] = h->total_inner - before;
             hipLaunchKernelGGL(k_mx_update_x, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, st, n, x, w.d, w.st);
             RSD_HIP(hipGetLastError());
         }

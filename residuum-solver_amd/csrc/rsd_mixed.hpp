@@ -5,17 +5,20 @@
 
 namespace rsd {
 
+// Device state of the mixed solve; only one-block kernels write it.
 struct MixedState {
     double res0, res;          // outer FP64 residual norms
-    double in_res0, in_tol;    // inner solve
-    double rz_old, alpha, beta;
-    int64_t in_iter, in_max, total_inner;
+    double in_res0, in_tol;    // inner solve: first residual, relative tolerance
+    double gam, alp;           // gamma and alpha of the previous inner iteration
+    double alpha, beta;        // scalars of the current inner iteration
+    int64_t it;                // inner iteration index
+    int64_t in_max, total_inner;  // inner iteration cap; SpMVs of all inner solves
     int32_t cycles, converged, stop, pad;
 };
 
 struct MixedWork {
     double* r64 = nullptr;
-    float *r = nullptr, *p = nullptr, *Ap = nullptr, *d = nullptr, *invd = nullptr;
+    float *r = nullptr, *u = nullptr, *w = nullptr, *p = nullptr, *s = nullptr, *d = nullptr, *invd = nullptr;
     MixedState* st = nullptr;
 };
 
@@ -29,7 +32,11 @@ void matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A);
 void launch_spmv_f32(rsd_ctx* ctx, const rsd_matrix* A, const float* x, float* y);
 MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n);
 void mixed_launch_inner_iter(rsd_ctx* ctx, const rsd_matrix* A, int32_t precond, const MixedWork& w);
+constexpr int kMixedMaxCycles = 40;
+// hint: optional [kMixedMaxCycles] inner iteration counts of a previous solve,
+// used to queue chunks before the first look at the stop flag (updated)
 MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
-                     int32_t precond, double inner_floor, bool fresh_f32, const MixedWork& w, ChunkRunner* runner);
+                     int32_t precond, double inner_floor, bool fresh_f32, const MixedWork& w, ChunkRunner* runner,
+                     int64_t* hint = nullptr);
 
 }  // namespace rsd

@@ -264,6 +264,7 @@ int rsd_ctx_create(int device, rsd_ctx** out) {
         auto c = std::make_unique<rsd_ctx>();
         c->device = device;
         RSD_HIP(hipSetDevice(device));
+        RSD_HIP(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
         RSD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         RSD_HIP(hipMalloc(&c->d_scalars, sizeof(double) * 64));
         c->partials(1 << 16);

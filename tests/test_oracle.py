@@ -198,7 +198,8 @@ def test_jacobi_inverse_diag():
 
 def test_gpu_order_reduction_is_a_sum():
     rng = np.random.default_rng(0)
-    for n in (0, 1, 255, 256, 257, 70000):
+    # block edges and a size with more than 1024 blocks (fold lane wrap-around)
+    for n in (0, 1, 255, 256, 257, 70000, 300001):
         x, y = rng.standard_normal(n), rng.standard_normal(n)
         assert abs(O.dot(x, y, O.ORDER_GPU) - np.dot(x, y)) <= 1e-12 * (np.abs(x * y).sum() + 1e-300)
         assert O.dot(x, y, O.ORDER_REF) == sum((x * y).tolist(), 0.0) or n == 0
