@@ -82,6 +82,18 @@ def cpu_baseline(pb, K, b_host, iters):
     return pb.n_free * rep.iterations / dt, rep.iterations, dt
 
 
+def measured_traffic(n):
+    """HBM bytes per k_spmv_f32 launch at this size, from the committed
+    FETCH_SIZE / WRITE_SIZE counter profile (scaled by the calibration copy's
+    counters); None when no profile for this size is committed."""
+    path = os.path.join(ROOT, "profiles", "spmv_f32_%dcube_hbm_counters.json" % n)
+    try:
+        with open(path) as f:
+            return float(json.load(f)["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +176,16 @@ def main():
         print(f"# {name}: {avg_ms*1e3:.1f} us/launch, {spmv[name]['gbs']:.0f} GB/s algorithmic", file=sys.stderr,
               flush=True)
 
+    # calibration copy of known size (2^26 doubles = 512 MiB each way): the
+    # FETCH_SIZE / WRITE_SIZE counters of this dispatch scale the SpMV's
+    calib_n = 1 << 26
+    ca, cb = ctx.empty(calib_n), ctx.empty(calib_n)
+    ctx.fill(ca, 1.0)
+    ctx.copy(ca, cb)
+    ctx.sync()
+    ca.free()
+    cb.free()
+
     cpu = None
     if not args.no_cpu:
         b_host = np.ones(nfree)
@@ -202,7 +224,7 @@ def main():
         "spmv": spmv,
         "roofline": {"bound": "hbm", "kernel": "k_spmv_f32 (SELL-64, fp32 values, int16 offsets)",
                      "achieved": spmv["spmv_f32"]["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": spmv["spmv_f32"]["gbs"] / PEAK_HBM_GBS, "traffic": None},
+                     "frac": spmv["spmv_f32"]["gbs"] / PEAK_HBM_GBS, "traffic": measured_traffic(n)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

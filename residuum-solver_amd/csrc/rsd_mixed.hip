@@ -9,13 +9,14 @@
 //                  FP32 vectors and FP32-accumulated SpMV rows, by the
 //                  Chronopoulos-Gear form of PCG (one fused reduction of
 //                  three dot products per iteration, accumulated in FP64):
-//                    u = M r, w = A u, gamma = r.u, delta = w.u,
+//                    u = M r, w = A u, delta = w.u, gamma = r.u,
 //                    beta = gamma / gamma', alpha = gamma / (delta - beta gamma / alpha'),
 //                    p = u + beta p, s = w + beta s, d += alpha p, r -= alpha s;
 //   update (FP64): x += ||r|| d.
 // Per inner iteration the matrix stream is 6 bytes per slot instead of 12,
-// and an iteration is three launches: the SpMV with the dot products, the
-// one-block fold of the dot products, and the vector update.
+// and an iteration is three launches: the SpMV with w.u, the one-block fold
+// of the three dot products, and the vector update, which also forms r.u
+// and r.r of the new residual for the next iteration.
 #include "rsd_linalg.hpp"
 #include "rsd_mixed.hpp"
 #include "rsd_rowdot.hpp"
@@ -100,44 +101,51 @@ __global__ __launch_bounds__(kFold) void k_mx_residual_fold(int64_t nb, const do
     }
 }
 
-// inner set-up: r = r64 / ||r||, u = M r, p = s = d = 0
-__global__ void k_mx_init(int64_t n, const double* __restrict__ r64, float* __restrict__ r, float* __restrict__ u,
-                          float* __restrict__ p, float* __restrict__ s, float* __restrict__ d,
-                          const float* __restrict__ invd, const MixedState* st) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const double sc = st->res > 0.0 ? 1.0 / st->res : 0.0;
-    const float ri = (float)(r64[i] * sc);
-    r[i] = ri;
-    u[i] = invd ? ri * invd[i] : ri;
-    p[i] = 0.0f;
-    s[i] = 0.0f;
-    d[i] = 0.0f;
+// inner set-up: r = r64 / ||r||, u = M r, p = s = d = 0; block sums of r.u, r.r
+__global__ __launch_bounds__(kBlock) void k_mx_init(int64_t n, int64_t nb, const double* __restrict__ r64,
+                                                    float* __restrict__ r, float* __restrict__ u,
+                                                    float* __restrict__ p, float* __restrict__ s,
+                                                    float* __restrict__ d, const float* __restrict__ invd,
+                                                    const MixedState* st, double* P12) {
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < n) {
+        const double sc = st->res > 0.0 ? 1.0 / st->res : 0.0;
+        const float ri = (float)(r64[i] * sc);
+        const float ui = invd ? ri * invd[i] : ri;
+        r[i] = ri;
+        u[i] = ui;
+        p[i] = 0.0f;
+        s[i] = 0.0f;
+        d[i] = 0.0f;
+        v[0] = (double)ri * (double)ui;
+        v[1] = (double)ri * (double)ri;
+    }
+    block_partials<2>(v, lb, nb, P12);
 }
 
-// w = A u, block sums of r.u, w.u, r.r
+// w = A u, block sums of w.u
 template<bool OFF16>
-__global__ __launch_bounds__(kBlock) void k_mx_spmv_dots(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
-                                                         const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                         const int16_t* __restrict__ off, const float* __restrict__ val,
-                                                         const float* __restrict__ u, const float* __restrict__ r,
-                                                         float* __restrict__ w, const MixedState* st, double* P) {
+__global__ __launch_bounds__(kBlock) void k_mx_spmv_wu(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
+                                                       const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                       const int16_t* __restrict__ off, const float* __restrict__ val,
+                                                       const float* __restrict__ u, float* __restrict__ w,
+                                                       const MixedState* st, double* P0) {
     if (st->stop) return;
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t i = lb * kBlock + threadIdx.x;
-    double v[3] = {0.0, 0.0, 0.0};
+    double v[1] = {0.0};
     if (i < n) {
         const float wi = row_dot_f32<OFF16>(i, slice_ptr, rp, col, off, val, u);
         w[i] = wi;
-        const double ui = (double)u[i], ri = (double)r[i];
-        v[0] = ri * ui;
-        v[1] = (double)wi * ui;
-        v[2] = ri * ri;
+        v[0] = (double)wi * (double)u[i];
     }
-    block_partials<3>(v, lb, nb, P);
+    block_partials<1>(v, lb, nb, P0);
 }
 
-// fold gamma, delta, ||r||^2: convergence, alpha, beta
+// fold delta = w.u (P0), gamma = r.u and ||r||^2 (P12 = P0 + nb):
+// convergence, alpha, beta
 __global__ __launch_bounds__(kFold) void k_mx_scalars(int64_t nb, const double* __restrict__ P, MixedState* st) {
     if (st->stop) return;
     const int64_t k = st->it;
@@ -146,7 +154,7 @@ __global__ __launch_bounds__(kFold) void k_mx_scalars(int64_t nb, const double* 
     double o[3];
     fold_partials<3>(nb, P, o);
     if (threadIdx.x != 0) return;
-    const double gam = o[0], del = o[1], res = sqrt(o[2]);
+    const double del = o[0], gam = o[1], res = sqrt(o[2]);
     const double r0 = k == 0 ? res : in_res0;
     st->total_inner += 1;
     if (k == 0) st->in_res0 = res;
@@ -169,24 +177,32 @@ __global__ __launch_bounds__(kFold) void k_mx_scalars(int64_t nb, const double* 
     st->it = k + 1;
 }
 
-// p = u + beta p, s = w + beta s, d += alpha p, r -= alpha s, u = M r
+// p = u + beta p, s = w + beta s, d += alpha p, r -= alpha s, u = M r;
+// block sums of the new r.u, r.r
 __global__ __launch_bounds__(kBlock) void k_mx_update(int64_t n, int64_t nb, float* __restrict__ p,
                                                       float* __restrict__ s, float* __restrict__ d,
                                                       float* __restrict__ r, float* __restrict__ u,
                                                       const float* __restrict__ w, const float* __restrict__ invd,
-                                                      const MixedState* st) {
+                                                      const MixedState* st, double* P12) {
     if (st->stop) return;
-    const int64_t i = xcd_block(blockIdx.x, nb) * kBlock + threadIdx.x;
-    if (i >= n) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t i = lb * kBlock + threadIdx.x;
     const float a = (float)st->alpha, bt = (float)st->beta;
-    const float pi = __builtin_fmaf(bt, p[i], u[i]);
-    const float si = __builtin_fmaf(bt, s[i], w[i]);
-    p[i] = pi;
-    s[i] = si;
-    d[i] = __builtin_fmaf(a, pi, d[i]);
-    const float ri = __builtin_fmaf(-a, si, r[i]);
-    r[i] = ri;
-    u[i] = invd ? ri * invd[i] : ri;
+    double v[2] = {0.0, 0.0};
+    if (i < n) {
+        const float pi = __builtin_fmaf(bt, p[i], u[i]);
+        const float si = __builtin_fmaf(bt, s[i], w[i]);
+        p[i] = pi;
+        s[i] = si;
+        d[i] = __builtin_fmaf(a, pi, d[i]);
+        const float ri = __builtin_fmaf(-a, si, r[i]);
+        const float ui = invd ? ri * invd[i] : ri;
+        r[i] = ri;
+        u[i] = ui;
+        v[0] = (double)ri * (double)ui;
+        v[1] = (double)ri * (double)ri;
+    }
+    block_partials<2>(v, lb, nb, P12);
 }
 
 __global__ void k_mx_update_x(int64_t n, double* __restrict__ x, const float* __restrict__ d, const MixedState* st) {
@@ -245,14 +261,14 @@ void mixed_launch_inner_iter(rsd_ctx* ctx, const rsd_matrix* A, int32_t precond,
     const float* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     hipStream_t s = ctx->stream;
     if (S.has_off16)
-        hipLaunchKernelGGL(k_mx_spmv_dots<true>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p,
-                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.u, w.r, w.w, w.st, P);
+        hipLaunchKernelGGL(k_mx_spmv_wu<true>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p,
+                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.u, w.w, w.st, P);
     else
-        hipLaunchKernelGGL(k_mx_spmv_dots<false>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p,
-                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.u, w.r, w.w, w.st, P);
+        hipLaunchKernelGGL(k_mx_spmv_wu<false>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p,
+                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.u, w.w, w.st, P);
     hipLaunchKernelGGL(k_mx_scalars, dim3(1), dim3(kFold), 0, s, nb, P, w.st);
     hipLaunchKernelGGL(k_mx_update, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.p, w.s, w.d, w.r, w.u, w.w, invd,
-                       w.st);
+                       w.st, P + nb);
 }
 
 constexpr int64_t kInnerChunk = 8;
@@ -299,8 +315,8 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
             h->it = 0;
             h->stop = 0;
             RSD_HIP(hipMemcpyAsync(w.st, h, sizeof(MixedState), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_mx_init, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, st, n, w.r64, w.r, w.u, w.p, w.s,
-                               w.d, invd, w.st);
+            hipLaunchKernelGGL(k_mx_init, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w.r64, w.r, w.u, w.p, w.s,
+                               w.d, invd, w.st, part + nb);
             const int64_t before = h->total_inner;
             // queue what the same cycle of the previous solve needed (minus a
             // margin) before the first look at the stop flag
