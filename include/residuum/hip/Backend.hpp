@@ -1,37 +1,49 @@
-// residuum/hip/Backend.hpp -- the Hip backend tag and device containers.
+// residuum/hip/Backend.hpp -- the Hip backend tag.
 //
-// C++20 counterpart of the reference's Backend-tag seam
-// (linalg/types/backend/CPU.hpp:15-40, the CUDA scaffold in
-// linalg/types/backend/CUDA.hpp:16-73) and of the containers that consume
-// it (linalg/types/Vector.hpp:10-45, linalg/types/CSRMatrix.hpp:14-88).
-// Everything calls libresiduum_hip.so through include/residuum_hip.h.
+// The reference's containers are templated on a backend tag with the duck
+// type alloc / copy / zero / set / Ptr (linalg/types/backend/CPU.hpp:15-40;
+// the unused CUDA scaffold in linalg/types/backend/CUDA.hpp:16-73).
+// backend::Hip satisfies it with device memory from libresiduum_hip.so, so
+// the reference's own Vector<T, Hip> and CSRMatrix<T, Hip>
+// (linalg/types/Vector.hpp:10-45, CSRMatrix.hpp:14-88) hold their data in HBM
+// unchanged.  Include this header with the reference's include directory on
+// the include path; link libresiduum_hip.so.
 #pragma once
 
 #include <cstddef>
 #include <memory>
 #include <stdexcept>
 #include <string>
-#include <vector>
+#include <type_traits>
 
+#include "core/Types.hpp"
+#include "linalg/types/backend/cuda/CopyKind.hpp"
 #include "residuum_hip.h"
 
-namespace residuum::hip {
+namespace residuum {
+namespace hip {
 
-using Real = double;
-using Index = std::size_t;
-
-// status -> exception, as the reference throws std::runtime_error (CUDA.hpp:32-35)
+// C-ABI status -> exception; the reference reports backend failures with
+// std::runtime_error (CUDA.hpp:32-35) and a missing diagonal with
+// std::out_of_range (CSRMatrix.hpp:70)
 inline void check(int rc) {
-    if (rc != RSD_OK) throw std::runtime_error(std::string("residuum_hip: ") + rsd_last_error_string());
+    if (rc == RSD_OK) return;
+    throw std::runtime_error(std::string("residuum_hip: ") + rsd_last_error_string());
 }
 
-// One rsd_ctx (one HIP stream) per host thread, created on first use.
+// The HIP context (one stream plus scratch) used by every operation on the
+// calling host thread: all work is stream-ordered on it, which keeps the
+// reference's synchronous call semantics for the host code around it.
 class Context {
 public:
-    static rsd_ctx* get(int device = 0) {
-        thread_local Context c(device);
-        return c.ctx_;
+    static rsd_ctx* get() { return instance().ctx_; }
+    // Select the device for this thread's context; only before first use.
+    static void select_device(int device) {
+        if (device_slot() != device && created()) throw std::runtime_error("residuum_hip: context already created");
+        device_slot() = device;
     }
+    static void synchronize() { check(rsd_ctx_sync(get())); }
+
     Context(const Context&) = delete;
     Context& operator=(const Context&) = delete;
     ~Context() {
@@ -39,81 +51,74 @@ public:
     }
 
 private:
-    explicit Context(int device) { check(rsd_ctx_create(device, &ctx_)); }
+    Context() { check(rsd_ctx_create(device_slot(), &ctx_)); created() = true; }
+    static Context& instance() {
+        thread_local Context c;
+        return c;
+    }
+    static int& device_slot() {
+        thread_local int d = 0;
+        return d;
+    }
+    static bool& created() {
+        thread_local bool c = false;
+        return c;
+    }
     rsd_ctx* ctx_ = nullptr;
 };
 
-// mirrors linalg/types/backend/cuda/CopyKind.hpp:9-14
-enum class CopyKind { HostToHost = RSD_COPY_H2H, HostToDevice = RSD_COPY_H2D, DeviceToHost = RSD_COPY_D2H,
-                      DeviceToDevice = RSD_COPY_D2D };
+}  // namespace hip
+
+namespace linalg {
+namespace types {
+namespace backend {
 
 template<typename T>
 struct HipDeleter {
     void operator()(T* p) const noexcept {
-        if (p) rsd_free(Context::get(), p);
+        if (p) rsd_free(hip::Context::get(), p);
     }
 };
 
-// The backend tag: alloc / copy / zero / set, same duck type as CPU.hpp:15-40.
 class Hip {
 public:
     template<typename T>
-    using Ptr = std::unique_ptr<T, HipDeleter<T>>;
+    using Ptr = std::unique_ptr<T[], HipDeleter<T>>;
 
     template<typename T>
     static Ptr<T> alloc(Index n) {
         void* p = nullptr;
-        check(rsd_malloc(Context::get(), n * sizeof(T), &p));
+        if (n) hip::check(rsd_malloc(hip::Context::get(), sizeof(T) * n, &p));
         return Ptr<T>(static_cast<T*>(p));
     }
+
     template<typename T>
     static void copy(T* dst, const T* src, Index n, CopyKind kind) {
-        check(rsd_memcpy(Context::get(), dst, src, n * sizeof(T), static_cast<int>(kind)));
+        if (!n) return;
+        int k = RSD_COPY_D2D;
+        switch (kind) {
+        case CopyKind::HostToHost: k = RSD_COPY_H2H; break;
+        case CopyKind::HostToDevice: k = RSD_COPY_H2D; break;
+        case CopyKind::DeviceToHost: k = RSD_COPY_D2H; break;
+        case CopyKind::DeviceToDevice: k = RSD_COPY_D2D; break;
+        }
+        hip::check(rsd_memcpy(hip::Context::get(), dst, src, sizeof(T) * n, k));
     }
+
     template<typename T>
-    static void zero(T* p, Index n) {
-        check(rsd_memset_zero(Context::get(), p, n * sizeof(T)));
+    static void zero(T* ptr, Index n) {
+        if (n) hip::check(rsd_memset_zero(hip::Context::get(), ptr, sizeof(T) * n));
     }
+
     template<typename T>
-    static void set(T* p, Index n, T v) {
-        if constexpr (sizeof(T) == 8) check(rsd_fill_f64(Context::get(), reinterpret_cast<double*>(p), (int64_t)n, (double)v));
-        else check(rsd_fill_f32(Context::get(), reinterpret_cast<float*>(p), (int64_t)n, (float)v));
+    static void set(T* ptr, Index n, T value) {
+        if constexpr (std::is_same_v<T, double>) hip::check(rsd_fill_f64(hip::Context::get(), ptr, (int64_t)n, value));
+        else if constexpr (std::is_same_v<T, float>) hip::check(rsd_fill_f32(hip::Context::get(), ptr, (int64_t)n, value));
+        else static_assert(sizeof(T) == 0, "Hip::set: double and float only");
     }
 };
 
-// Vector<Real, Hip> (linalg/types/Vector.hpp:10-45): move-only, device data.
-template<typename T>
-class Vector {
-public:
-    using value_type = T;
-    using backend_type = Hip;
-
-    explicit Vector(Index size) : size_(size), data_(Hip::alloc<T>(size)) {}
-    Vector(const Vector&) = delete;
-    Vector& operator=(const Vector&) = delete;
-    Vector(Vector&&) = default;
-    Vector& operator=(Vector&&) = default;
-
-    Index size() const { return size_; }
-    T* data() { return data_.get(); }
-    const T* data() const { return data_.get(); }
-    void zero() { Hip::zero(data_.get(), size_); }
-    void set(T v) { Hip::set(data_.get(), size_, v); }
-
-    // host transfers (the reference reads data() on the host; here it is device memory)
-    void upload(const std::vector<T>& h) {
-        if (h.size() != size_) throw std::runtime_error("Vector::upload: size mismatch");
-        Hip::copy(data_.get(), h.data(), size_, CopyKind::HostToDevice);
-    }
-    std::vector<T> download() const {
-        std::vector<T> h(size_);
-        Hip::copy(h.data(), data_.get(), size_, CopyKind::DeviceToHost);
-        return h;
-    }
-
-private:
-    Index size_;
-    typename Hip::Ptr<T> data_;
-};
-
-}  // namespace residuum::hip
+}  // namespace backend
+}  // namespace types
+}  // namespace linalg
+}  // namespace residuum

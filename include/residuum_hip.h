@@ -180,19 +180,49 @@ typedef struct rsd_report {
     double final_residual_rel;
 } rsd_report;
 
+/* tolerance type (cg/Config.hpp:12, gmres/Config.hpp:12 ToleranceType) */
+#define RSD_TOL_RELATIVE 0 /* ||r_k|| / (||r_0|| + 1e-50) < tol */
+#define RSD_TOL_ABSOLUTE 1 /* ||r_k|| < tol */
+
 /* cg::Solver::solve (linalg/solver/iterative/cg/Solver.tpp:15-119): PCG with
- * the relative criterion ||r_k|| / (||r_0|| + 1e-50) < tol, warm start from
- * x.  Fused kernels, scalars kept on the device.  hist (host, nullable,
+ * the reference's stopping test (tol_type), warm start from x.  Fused
+ * kernels, scalars kept on the device.  hist (host, nullable,
  * >= max_iters entries) receives ||r_k||. */
-int rsd_cg_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol,
+int rsd_cg_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int32_t tol_type,
                      int64_t max_iters, int32_t precond, rsd_report* rep, double* hist);
 
 /* gmres::Solver::solve (linalg/solver/iterative/gmres/Solver.tpp:5-170):
  * right-preconditioned restarted GMRES(krylov) with modified Gram-Schmidt,
  * Givens rotations, breakdown at 1e-14 and the true residual at every
  * restart; rep->iterations counts Arnoldi steps. */
-int rsd_gmres_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol,
+int rsd_gmres_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int32_t tol_type,
                         int64_t max_iters, int32_t krylov, int32_t precond, rsd_report* rep);
+
+/* ---------------------------------------------------------------------- */
+/* raw device CSR in the reference's layout (linalg/types/CSRMatrix.hpp:14-88:
+ * size_t row pointer [n+1], size_t columns [nnz], values [nnz]), all
+ * pointers device memory.  Row sums run left to right in CSR order.       */
+/* ---------------------------------------------------------------------- */
+/* operations::matvec (linalg/operations/backend/cpu/MatrixOps.tpp:5-23) */
+int rsd_csr_spmv_f64(rsd_ctx* ctx, int64_t n_rows, const uint64_t* row_ptr, const uint64_t* col_idx,
+                     const double* val, const double* x, double* y);
+/* operations::lump (MatrixOps.tpp:26-44): d_i = sum of row i */
+int rsd_csr_lump_f64(rsd_ctx* ctx, int64_t n_rows, const uint64_t* row_ptr, const double* val, double* d);
+/* CSROperator::diagonal (linalg/operator/CSROperator.hpp:40-45) with
+ * CSRMatrix::getDataIndex (CSRMatrix.hpp:49-73): a row without a diagonal
+ * entry fails with RSD_ERR_ARG, as the reference throws std::out_of_range */
+int rsd_csr_diag_f64(rsd_ctx* ctx, int64_t n_rows, const uint64_t* row_ptr, const uint64_t* col_idx,
+                     const double* val, double* d);
+/* Jacobi::update (linalg/solver/preconditioner/Jacobi.hpp:33-36):
+ * inv_i = 1/d_i where |d_i| > 1e-300, else 1; d == inv is allowed */
+int rsd_inv_diag_f64(rsd_ctx* ctx, int64_t n, const double* d, double* inv);
+/* a solver matrix (sliced ELL) on the pattern of a square device CSR with
+ * sorted unique columns, for the fused solvers above; values are loaded in
+ * CSR order with rsd_matrix_load_csr.  Replaces the CSRMatrix the
+ * reference's cg / gmres Solver templates read through CSROperator. */
+int rsd_matrix_from_csr(rsd_ctx* ctx, int64_t n_rows, const uint64_t* row_ptr, const uint64_t* col_idx,
+                        rsd_matrix** out);
+int rsd_matrix_load_csr(rsd_ctx* ctx, rsd_matrix* A, const double* val);
 
 /* Mixed precision (no reference counterpart; the reference is FP64 only).
  * rsd_matrix_to_f32 refreshes the FP32 copy of A's values; rsd_spmv_f32 is

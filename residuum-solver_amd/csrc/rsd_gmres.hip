@@ -84,7 +84,7 @@ __global__ void k_gm_newq(int64_t n, const double* __restrict__ w, double* __res
 
 // Givens rotations for column j (Solver.tpp:78-122)
 __global__ void k_gm_givens(int32_t j, int32_t m, double* H, double* cs, double* sn, double* g, GMState* st,
-                            double tol, int64_t max_iters) {
+                            double tol, int32_t abs_tol, int64_t max_iters) {
     if (st->stop) return;
     auto hat = [&](int i, int jj) -> double& { return H[(int64_t)jj * (m + 1) + i]; };
     const double hnext = st->hnext;
@@ -107,7 +107,7 @@ __global__ void k_gm_givens(int32_t j, int32_t m, double* H, double* cs, double*
     st->est = est;
     st->mloc = j + 1;
     st->breakdown = breakdown;
-    if (est / (st->res0 + 1e-50) < tol || breakdown || st->total >= max_iters) st->stop = 1;
+    if ((abs_tol ? est : est / (st->res0 + 1e-50)) < tol || breakdown || st->total >= max_iters) st->stop = 1;
 }
 
 // back-substitution H y = g (Solver.tpp:126-133)
@@ -178,7 +178,8 @@ __global__ void k_gm_reset_flags(GMState* st) {
 }
 
 GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
-                      int32_t m, int32_t precond) {
+                      int32_t m, int32_t precond, bool abs_tol) {
+    auto crit = [&](double res, double res0) { return abs_tol ? res : res / (res0 + 1e-50); };
     const Sell& S = A->pb->sell;
     const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n)), nbe = nblocks(std::max<int64_t>(n, 1));
     hipStream_t st = ctx->stream;
@@ -212,7 +213,7 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
         read();
         const double res0 = h->res0;
         res.initial_residual = res0;
-        if (res0 / (res0 + 1e-50) < tol) {
+        if (crit(res0, res0) < tol) {
             res.converged = 1;
             res.final_residual = res0;
             res.final_residual_rel = 1.0;
@@ -233,7 +234,8 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
                                            (int32_t)norm, H, dst);
                     }
                     hipLaunchKernelGGL(k_gm_newq, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, w, hq[j + 1], dst);
-                    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, st, j, m, H, cs, sn, g, dst, tol, max_iters);
+                    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, st, j, m, H, cs, sn, g, dst, tol,
+                                       (int32_t)abs_tol, max_iters);
                     RSD_HIP(hipGetLastError());
                 }
                 hipLaunchKernelGGL(k_gm_backsub, dim3(1), dim3(1), 0, st, m, H, g, y, dst);
@@ -246,7 +248,7 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
                 read();
                 total = h->total;
                 const double beta = h->beta;
-                const bool conv = beta / (res0 + 1e-50) < tol;
+                const bool conv = crit(beta, res0) < tol;
                 if (conv || h->breakdown || total >= max_iters) {
                     res.converged = conv;
                     res.final_residual = beta;
@@ -269,12 +271,14 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
 using namespace rsd;
 
 extern "C" int rsd_gmres_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol,
-                                   int64_t max_iters, int32_t krylov, int32_t precond, rsd_report* rep) {
+                                   int32_t tol_type, int64_t max_iters, int32_t krylov, int32_t precond,
+                                   rsd_report* rep) {
     return guard([&] {
         RSD_CHECK(ctx && A && b && x, "rsd_gmres_solve_f64: null argument");
         RSD_CHECK(krylov >= 1 && krylov <= 512, "krylov dimension outside [1, 512]");
         RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
-        GMRESResult r = gmres_run(ctx, A, b, x, tol, max_iters, krylov, precond);
+        RSD_CHECK(tol_type == RSD_TOL_RELATIVE || tol_type == RSD_TOL_ABSOLUTE, "unknown tolerance type");
+        GMRESResult r = gmres_run(ctx, A, b, x, tol, max_iters, krylov, precond, tol_type == RSD_TOL_ABSOLUTE);
         if (rep) {
             rep->converged = r.converged;
             rep->iterations = r.iterations;

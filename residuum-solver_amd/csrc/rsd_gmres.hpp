@@ -18,6 +18,6 @@ struct GMRESResult {
 };
 
 GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
-                      int32_t m, int32_t precond);
+                      int32_t m, int32_t precond, bool abs_tol = false);
 
 }  // namespace rsd

@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int64_t n, int64_t nb, const
 }
 
 __global__ __launch_bounds__(kFold) void k_cg_init_fold(int64_t nb, const double* __restrict__ partials, CGState* st,
-                                                        double tol, int64_t max_iters, double* hist) {
+                                                        double tol, int32_t abs_tol, int64_t max_iters, double* hist) {
     double out[2];
     fold_partials<2>(nb, partials, out);
     if (threadIdx.x != 0) return;
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kFold) void k_cg_init_fold(int64_t nb, const double
     st->converged = 0;
     st->stop = 0;
     if (hist) hist[0] = res0;
-    if (res0 / (res0 + 1e-50) < tol) {
+    if ((abs_tol ? res0 : res0 / (res0 + 1e-50)) < tol) {
         st->converged = 1;
         st->stop = 1;
     } else if (max_iters <= 1) {
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_r(int64_t n, int64_t nb, d
 }
 
 __global__ __launch_bounds__(kFold) void k_cg_beta(int64_t nb, const double* __restrict__ partials, CGState* st,
-                                                   double tol, int64_t max_iters, double* hist) {
+                                                   double tol, int32_t abs_tol, int64_t max_iters, double* hist) {
     if (st->stop) return;
     const int64_t k = st->iter + 1;
     const double res0 = st->res0, rz_old = st->rz_old;
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kFold) void k_cg_beta(int64_t nb, const double* __r
     const double res = sqrt(out[0]);
     st->res = res;
     if (hist) hist[k] = res;
-    if (res / (res0 + 1e-50) < tol) {
+    if ((abs_tol ? res : res / (res0 + 1e-50)) < tol) {
         st->converged = 1;
         st->stop = 1;
     } else {
@@ -276,7 +276,8 @@ void cg_launch_init(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* 
     launch_spmv_f64(ctx, A, x, w.Ap);
     hipLaunchKernelGGL(k_cg_init, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, b, w.Ap, w.r, w.p,
                        precond == RSD_PC_JACOBI ? w.invd : nullptr, part);
-    hipLaunchKernelGGL(k_cg_init_fold, dim3(1), dim3(kFold), 0, ctx->stream, nb, part, w.st, tol, max_iters, w.hist);
+    hipLaunchKernelGGL(k_cg_init_fold, dim3(1), dim3(kFold), 0, ctx->stream, nb, part, w.st, tol, w.abs_tol, max_iters,
+                       w.hist);
 }
 
 void cg_launch_iter(rsd_ctx* ctx, const rsd_matrix* A, double* x, double tol, int64_t max_iters, int32_t precond,
@@ -291,7 +292,7 @@ void cg_launch_iter(rsd_ctx* ctx, const rsd_matrix* A, double* x, double tol, in
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kFold), 0, s, nb, part, w.st);
     hipLaunchKernelGGL(k_cg_update_r, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, x, w.p, w.Ap, w.r, invd, w.st,
                        part + nb);
-    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kFold), 0, s, nb, part + nb, w.st, tol, max_iters, w.hist);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kFold), 0, s, nb, part + nb, w.st, tol, w.abs_tol, max_iters, w.hist);
     hipLaunchKernelGGL(k_cg_update_p, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.p, w.r, invd, w.st);
 }
 
@@ -414,14 +415,16 @@ int rsd_jacobi_inv_diag(rsd_ctx* ctx, rsd_matrix* A, double* inv) {
     });
 }
 
-int rsd_cg_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
-                     int32_t precond, rsd_report* rep, double* hist) {
+int rsd_cg_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int32_t tol_type,
+                     int64_t max_iters, int32_t precond, rsd_report* rep, double* hist) {
     return guard([&] {
         RSD_CHECK(ctx && A && b && x, "rsd_cg_solve_f64: null argument");
         RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
+        RSD_CHECK(tol_type == RSD_TOL_RELATIVE || tol_type == RSD_TOL_ABSOLUTE, "unknown tolerance type");
         RSD_CHECK(max_iters >= 0, "max_iters must be >= 0");
         const int64_t n = A->pb->sell.n_rows;
         CGWork w = cg_workspace(ctx, n, max_iters, hist != nullptr);
+        w.abs_tol = tol_type == RSD_TOL_ABSOLUTE;
         CGState s = cg_run(ctx, A, b, x, tol, max_iters, precond, w);
         cg_report(s, max_iters, rep);
         if (hist) {

@@ -16,6 +16,7 @@ struct CGState {
 struct CGWork {
     double *r = nullptr, *p = nullptr, *Ap = nullptr, *invd = nullptr, *hist = nullptr;
     CGState* st = nullptr;
+    int32_t abs_tol = 0;  // cg::ToleranceType::Absolute: stop on ||r|| < tol instead of the relative test
 };
 
 void launch_spmv_f64(rsd_ctx* ctx, const rsd_matrix* A, const double* x, double* y);

@@ -62,6 +62,7 @@ struct rsd_problem {
     rsd::DBuf<int32_t> order;     // [n_elem] colour-major
     rsd::DBuf<int32_t> border;    // elements with a constrained node, colour-major
     rsd::DBuf<int32_t> map_sell;  // [n_elem * npe * npe] SELL slot or -1
+    rsd::DBuf<int32_t> csr_slot;  // [nnz] SELL slot of each CSR entry (matrices made from a CSR)
     rsd::DBuf<double> gval;       // [n_nodes] Dirichlet value (0 on free nodes)
     rsd::DBuf<double> kappa;      // [n_elem] or empty
     const double* kappa_ext = nullptr;  // device kappa supplied by the caller
@@ -77,6 +78,7 @@ struct rsd_problem {
 
 struct rsd_matrix {
     rsd_problem* pb = nullptr;
+    std::unique_ptr<rsd_problem> own_pb;  // pattern-only problem of a matrix made from a CSR
     rsd::DBuf<double> val;    // [slots] SELL order
     rsd::DBuf<float> val32;   // [slots] FP32 copy for the mixed-precision path
 };
@@ -84,4 +86,6 @@ struct rsd_matrix {
 namespace rsd {
 // refresh the per-node Dirichlet values for `time` (no-op when cached)
 void update_gval(rsd_problem* pb, double time);
+// sliced ELL layout of pb->h_rp / h_ci, uploaded
+void build_sell(rsd_problem* pb, const std::vector<int32_t>& rowlen);
 }

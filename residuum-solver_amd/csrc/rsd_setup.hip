@@ -253,6 +253,49 @@ rsd_ctx::~rsd_ctx() {
     for (auto& e : ws_tab) (void)hipFree(e.second.p);
 }
 
+// Sliced ELL layout of the CSR pattern in pb->h_rp / h_ci (rowlen[r] = row
+// lengths), uploaded to the device; pb->n_free rows.
+void rsd::build_sell(rsd_problem* pb, const std::vector<int32_t>& rowlen) {
+    hipStream_t st = pb->ctx->stream;
+    const int64_t n = pb->n_free;
+    pb->max_row = n ? *std::max_element(rowlen.begin(), rowlen.end()) : 0;
+    Sell& S = pb->sell;
+    S.n_rows = n;
+    S.n_slices = (n + kSlice - 1) / kSlice;
+    S.h_slice_ptr.assign(S.n_slices + 1, 0);
+    for (int64_t s = 0; s < S.n_slices; ++s) {
+        int32_t w = 0;
+        for (int64_t r = s * kSlice; r < std::min(n, (s + 1) * kSlice); ++r) w = std::max(w, rowlen[r]);
+        S.h_slice_ptr[s + 1] = S.h_slice_ptr[s] + (int64_t)w * kSlice;
+    }
+    S.slots = S.h_slice_ptr[S.n_slices];
+    RSD_CHECK(S.slots < (int64_t)INT32_MAX, "sliced layout exceeds int32 slots");
+    std::vector<int32_t> hcol(S.slots);
+    std::vector<int16_t> hoff(S.slots, 0);
+    std::vector<int32_t> hdiag(n, -1);
+    bool fits16 = true;
+    for (int64_t s = 0; s < S.n_slices; ++s)
+        for (int64_t r = s * kSlice; r < (s + 1) * kSlice; ++r) {
+            const int64_t w = (S.h_slice_ptr[s + 1] - S.h_slice_ptr[s]) / kSlice;
+            for (int64_t j = 0; j < w; ++j) {
+                const int64_t sl = S.h_slice_ptr[s] + j * kSlice + (r % kSlice);
+                int32_t c = r < n ? (int32_t)r : 0;
+                if (r < n && j < rowlen[r]) c = pb->h_ci[pb->h_rp[r] + j];
+                hcol[sl] = c;
+                const int64_t off = (int64_t)c - (r < n ? r : 0);
+                if (off < -32768 || off > 32767) fits16 = false;
+                hoff[sl] = (int16_t)off;
+                if (r < n && j < rowlen[r] && c == r) hdiag[r] = (int32_t)sl;
+            }
+        }
+    S.has_off16 = fits16;
+    S.slice_ptr.upload(S.h_slice_ptr.data(), S.n_slices + 1, st);
+    S.row_ptr.upload(pb->h_rp.data(), n + 1, st);
+    S.col.upload(hcol.data(), S.slots, st);
+    if (fits16) S.off16.upload(hoff.data(), S.slots, st);
+    S.diag.upload(hdiag.data(), n, st);
+}
+
 extern "C" {
 
 const char* rsd_last_error_string(void) { return g_last_error.c_str(); }
@@ -433,44 +476,9 @@ static void build_problem(rsd_problem* pb) {
             std::copy(cols.begin(), cols.end(), pb->h_ci.begin() + pb->h_rp[r]);
         }
     });
-    pb->max_row = n ? *std::max_element(rowlen.begin(), rowlen.end()) : 0;
-    // SELL layout
+    build_sell(pb, rowlen);
     Sell& S = pb->sell;
-    S.n_rows = n;
-    S.n_slices = (n + kSlice - 1) / kSlice;
-    S.h_slice_ptr.assign(S.n_slices + 1, 0);
-    for (int64_t s = 0; s < S.n_slices; ++s) {
-        int32_t w = 0;
-        for (int64_t r = s * kSlice; r < std::min(n, (s + 1) * kSlice); ++r) w = std::max(w, rowlen[r]);
-        S.h_slice_ptr[s + 1] = S.h_slice_ptr[s] + (int64_t)w * kSlice;
-    }
-    S.slots = S.h_slice_ptr[S.n_slices];
-    RSD_CHECK(S.slots < (int64_t)INT32_MAX, "sliced layout exceeds int32 slots");
     auto slot_of = [&](int64_t r, int64_t j) { return S.h_slice_ptr[r / kSlice] + j * kSlice + (r % kSlice); };
-    std::vector<int32_t> hcol(S.slots);
-    std::vector<int16_t> hoff(S.slots, 0);
-    std::vector<int32_t> hdiag(n, -1);
-    bool fits16 = true;
-    for (int64_t s = 0; s < S.n_slices; ++s)
-        for (int64_t r = s * kSlice; r < (s + 1) * kSlice; ++r) {
-            const int64_t w = (S.h_slice_ptr[s + 1] - S.h_slice_ptr[s]) / kSlice;
-            for (int64_t j = 0; j < w; ++j) {
-                const int64_t sl = S.h_slice_ptr[s] + j * kSlice + (r % kSlice);
-                int32_t c = r < n ? (int32_t)r : 0;
-                if (r < n && j < rowlen[r]) c = pb->h_ci[pb->h_rp[r] + j];
-                hcol[sl] = c;
-                const int64_t off = (int64_t)c - (r < n ? r : 0);
-                if (off < -32768 || off > 32767) fits16 = false;
-                hoff[sl] = (int16_t)off;
-                if (r < n && j < rowlen[r] && c == r) hdiag[r] = (int32_t)sl;
-            }
-        }
-    S.has_off16 = fits16;
-    S.slice_ptr.upload(S.h_slice_ptr.data(), S.n_slices + 1, st);
-    S.row_ptr.upload(pb->h_rp.data(), n + 1, st);
-    S.col.upload(hcol.data(), S.slots, st);
-    if (fits16) S.off16.upload(hoff.data(), S.slots, st);
-    S.diag.upload(hdiag.data(), n, st);
     // element -> SELL slot map
     std::vector<int32_t> hmap((size_t)ne * npe * npe);
     parallel_for(ne, [&](int64_t lo, int64_t hi) {

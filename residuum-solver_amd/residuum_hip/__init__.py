@@ -20,6 +20,7 @@ DIFFUSION, MASS, TDIFFUSION, TMASS = 1, 2, 3, 4
 SCALED = 0x100
 SEL_VALUE, SEL_RATE, SEL_OPERAND = 0, 1, 2
 PC_IDENTITY, PC_JACOBI = 0, 1
+TOL_RELATIVE, TOL_ABSOLUTE = 0, 1
 
 _d = C.POINTER(C.c_double)
 _f = C.POINTER(C.c_float)
@@ -389,10 +390,10 @@ class Matrix:
     def inv_diag(self, out):
         check(lib().rsd_jacobi_inv_diag(self.pb.ctx.h, self.h, dptr(out)))
 
-    def gmres(self, b, x, tol=1e-10, max_iters=1000, krylov=30, precond=PC_JACOBI):
+    def gmres(self, b, x, tol=1e-10, max_iters=1000, krylov=30, precond=PC_JACOBI, tol_type=TOL_RELATIVE):
         rep = Report()
-        check(lib().rsd_gmres_solve_f64(self.pb.ctx.h, self.h, dptr(b), dptr(x), C.c_double(tol), C.c_int64(max_iters),
-                                        C.c_int32(krylov), C.c_int32(precond), C.byref(rep)))
+        check(lib().rsd_gmres_solve_f64(self.pb.ctx.h, self.h, dptr(b), dptr(x), C.c_double(tol), C.c_int32(tol_type),
+                                        C.c_int64(max_iters), C.c_int32(krylov), C.c_int32(precond), C.byref(rep)))
         return rep
 
     def to_f32(self):
@@ -408,11 +409,12 @@ class Matrix:
                                        C.c_int32(precond), C.c_double(inner_floor), C.byref(rep), C.byref(cyc)))
         return rep, cyc.value
 
-    def cg(self, b, x, tol=1e-10, max_iters=1000, precond=PC_JACOBI, hist=False):
+    def cg(self, b, x, tol=1e-10, max_iters=1000, precond=PC_JACOBI, hist=False, tol_type=TOL_RELATIVE):
         rep = Report()
         h = np.zeros(max_iters + 1) if hist else None
-        check(lib().rsd_cg_solve_f64(self.pb.ctx.h, self.h, dptr(b), dptr(x), C.c_double(tol), C.c_int64(max_iters),
-                                     C.c_int32(precond), C.byref(rep), None if h is None else h.ctypes.data_as(_d)))
+        check(lib().rsd_cg_solve_f64(self.pb.ctx.h, self.h, dptr(b), dptr(x), C.c_double(tol), C.c_int32(tol_type),
+                                     C.c_int64(max_iters), C.c_int32(precond), C.byref(rep),
+                                     None if h is None else h.ctypes.data_as(_d)))
         return (rep, h) if hist else rep
 
 

@@ -24,7 +24,7 @@ def test_null_arguments_map_to_status_codes():
     assert b"null" in L.rsd_last_error_string()
     assert L.rsd_memcpy(None, None, None, C.c_size_t(8), 1) == -1
     assert L.rsd_problem_sizes(None, None) == -1
-    assert L.rsd_cg_solve_f64(None, None, None, None, C.c_double(1e-8), C.c_int64(10), 1, None, None) == -1
+    assert L.rsd_cg_solve_f64(None, None, None, None, C.c_double(1e-8), 0, C.c_int64(10), 1, None, None) == -1
     assert L.rsd_philox_normal(C.c_uint64(1), C.c_uint32(0), C.c_int64(-1), None) == -1
 
 

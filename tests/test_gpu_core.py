@@ -202,6 +202,34 @@ def test_cg_bitexact(ctx, case, precond):
     assert np.max(np.abs(dx.numpy() - xr)) <= 1e-12 * np.max(np.abs(xr))
 
 
+def test_cg_absolute_tolerance(ctx):
+    """cg/Solver.tpp:45-48 with ToleranceType::Absolute: stop at the first k
+    with ||r_k|| < tol, leaving exactly the iterate of k steps."""
+    import residuum_hip as rh
+    pb, oc = build(ctx, CASES[2], kappa_seed=5)
+    forms = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 10.0)]
+    K = pb.matrix()
+    pb.assemble_matrix(K, forms)
+    b = np.random.default_rng(4).standard_normal(oc.n_free)
+    db, dx = ctx.from_numpy(b), ctx.empty(oc.n_free)
+    ctx.fill(dx, 0.0)
+    rep, hist = K.cg(db, dx, tol=1e-300, max_iters=60, hist=True)
+    assert not rep.converged
+    tau = 0.5 * (hist[12] + hist[13])
+    want = next(k for k in range(len(hist)) if hist[k] < tau)
+    ctx.fill(dx, 0.0)
+    rep_a = K.cg(db, dx, tol=tau, max_iters=60, tol_type=rh.TOL_ABSOLUTE)
+    xa = dx.numpy()
+    assert rep_a.converged and rep_a.iterations == want and rep_a.final_residual < tau
+    ctx.fill(dx, 0.0)
+    K.cg(db, dx, tol=1e-300, max_iters=want + 1)  # stops on the iteration cap after `want` steps
+    assert same(dx.numpy(), xa)
+    # GMRES takes the same switch
+    ctx.fill(dx, 0.0)
+    rg = K.gmres(db, dx, tol=1e-6 * hist[0], max_iters=400, krylov=20, tol_type=rh.TOL_ABSOLUTE)
+    assert rg.converged and rg.final_residual < 1e-6 * hist[0]
+
+
 def test_cg_maxiter_and_zero_rhs(ctx):
     pb, oc = build(ctx, CASES[2])
     K = pb.matrix()

@@ -1,0 +1,24 @@
+"""The reference's own linear-algebra templates (Vector, CSRMatrix,
+CSROperator, Jacobi, cg / gmres Solver) instantiated on the Hip backend tag of
+include/residuum/hip, checked case by case against the same templates on the
+CPU tag (tests/cpp/hip_seam_test.cpp).  The executable is compiled where the
+reference headers are available and runs wherever the GPU is."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "oracle", "_ref", "hip_seam_test")
+
+
+@pytest.mark.gpu
+def test_reference_templates_on_hip_tag():
+    if not os.path.exists(EXE):
+        pytest.skip("hip_seam_test is built only where the reference headers are present")
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    print(r.stderr)
+    fails = [ln for ln in r.stdout.splitlines() if ln.startswith("FAIL")]
+    assert r.returncode == 0 and not fails, "\n".join(fails) or r.stderr
+    assert sum(ln.startswith("PASS") for ln in r.stdout.splitlines()) >= 25
