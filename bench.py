@@ -123,7 +123,9 @@ def main():
     pb.set_mass(1.0, 1.0)
     t_setup = time.perf_counter() - t_setup
     nfree, nnz, slots = pb.n_free, pb.nnz, pb.slots
-    print(f"# setup {t_setup:.2f}s: nodes={pb.n_nodes} free={nfree} nnz={nnz} slots={slots} colours={pb.n_colours}",
+    col_mode, npat, _, _ = pb.layout()
+    print(f"# setup {t_setup:.2f}s: nodes={pb.n_nodes} free={nfree} nnz={nnz} slots={slots} colours={pb.n_colours} "
+          f"column mode={['int32 columns', 'int16 offsets', 'shared offset patterns'][col_mode]} ({npat} patterns)",
           file=sys.stderr, flush=True)
 
     results = {}
@@ -161,8 +163,17 @@ def main():
     x32, y32 = ctx.empty(nfree, np.float32), ctx.empty(nfree, np.float32)
     ctx.fill(x32, 1.0)
     spmv = {}
-    for name, fn, byts in (("spmv_f64", lambda: K.spmv(x64, y64), 12 * nnz + 4 * (nfree + 1) + 16 * nfree),
-                           ("spmv_f32", lambda: K.spmv_f32(x32, y32), 6 * nnz + 4 * (nfree + 1) + 8 * nfree)):
+    # algorithmic bytes of one product in the layout in use: values, the
+    # column information (int32 columns, int16 offsets, or a 1-byte pattern
+    # id per row), the row lengths where they are streamed, x once, y once
+    if col_mode == 2:
+        b64, b32 = 8 * nnz + nfree + 16 * nfree, 4 * nnz + nfree + 8 * nfree
+    elif col_mode == 1:
+        b64, b32 = 10 * nnz + 4 * (nfree + 1) + 16 * nfree, 6 * nnz + 4 * (nfree + 1) + 8 * nfree
+    else:
+        b64, b32 = 12 * nnz + 4 * (nfree + 1) + 16 * nfree, 8 * nnz + 4 * (nfree + 1) + 8 * nfree
+    for name, fn, byts in (("spmv_f64", lambda: K.spmv(x64, y64), b64),
+                           ("spmv_f32", lambda: K.spmv_f32(x32, y32), b32)):
         for _ in range(3):
             fn()
         ctx.sync()
@@ -222,7 +233,8 @@ def main():
                             "step_time": fp["total_ms"] / mx["total_ms"]},
         "mixed_vs_fp64_solution_max_rel_diff": mixed_vs_fp64_err,
         "spmv": spmv,
-        "roofline": {"bound": "hbm", "kernel": "k_spmv_f32 (SELL-64, fp32 values, int16 offsets)",
+        "roofline": {"bound": "hbm", "kernel": "k_spmv_f32 (SELL-64, fp32 values, "
+                                               + ["int32 columns", "int16 offsets", "shared offset patterns"][col_mode] + ")",
                      "achieved": spmv["spmv_f32"]["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": spmv["spmv_f32"]["gbs"] / PEAK_HBM_GBS, "traffic": measured_traffic(n)},
         "cpu_baseline": cpu,

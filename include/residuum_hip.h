@@ -129,6 +129,11 @@ int rsd_problem_destroy(rsd_problem* pb);
 /* out[8] = n_nodes, n_elem, npe, n_free, nnz, n_colours, sell_slots, max_row */
 int rsd_problem_sizes(rsd_problem* pb, int64_t* out);
 int rsd_problem_get_mesh(rsd_problem* pb, double* xyz, int64_t* ien, int32_t* rng);
+/* matrix layout (no reference counterpart): out[4] = column mode (0 absolute
+ * int32 columns, 1 int16 row-relative offsets, 2 offset patterns shared by
+ * rows: a pattern id per row, no index streamed per slot), number of
+ * patterns, whether int16 offsets fit, sliced-ELL slots */
+int rsd_problem_layout(rsd_problem* pb, int64_t* out);
 /* topo->alg map (-1 constrained), alg->topo, constraint tag per node */
 int rsd_problem_get_dofs(rsd_problem* pb, int32_t* topo_to_alg, int32_t* alg_to_topo, int32_t* ctag);
 /* CSR pattern as the reference's size_t arrays, widened to int64 */

@@ -35,6 +35,13 @@ struct BatchDev {
     const int64_t* first_block;       // [np + 1]
     const int64_t* nrows;             // [np]
     const int64_t* voff;              // [np + 1] vector segment offsets
+    __device__ __forceinline__ Cols cols(int pid) const {
+        Cols c;
+        c.slice_ptr = slice_ptr[pid];
+        c.rp = rp[pid];
+        c.col = col[pid];
+        return c;
+    }
 };
 
 __device__ __forceinline__ void locate(const BatchDev& B, int64_t gb, int& pid, int64_t& lb, int64_t& nb) {
@@ -61,7 +68,7 @@ __global__ void k_b_spmv(BatchDev B, int64_t total_blocks, const double* __restr
     locate(B, gb, pid, lb, nb);
     const int64_t r = lb * kBlock + threadIdx.x;
     const int64_t off = B.voff[pid];
-    if (r < B.nrows[pid]) y[off + r] = row_dot_f64(r, B.slice_ptr[pid], B.rp[pid], B.col[pid], B.val[pid], x + off);
+    if (r < B.nrows[pid]) y[off + r] = row_dot_f64<COL_ABS>(r, B.cols(pid), B.val[pid], x + off);
 }
 
 __global__ __launch_bounds__(kBlock) void k_b_init(BatchDev B, int64_t total_blocks, const double* __restrict__ b,
@@ -115,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv_pap(BatchDev B, int64_t total
     const int64_t off = B.voff[pid];
     double v[1] = {0.0};
     if (rl < B.nrows[pid]) {
-        const double y = row_dot_f64(rl, B.slice_ptr[pid], B.rp[pid], B.col[pid], B.val[pid], p + off);
+        const double y = row_dot_f64<COL_ABS>(rl, B.cols(pid), B.val[pid], p + off);
         Ap[off + rl] = y;
         v[0] = p[off + rl] * y;
     }

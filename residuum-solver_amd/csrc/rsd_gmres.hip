@@ -20,20 +20,19 @@ namespace rsd {
 
 // w = A (M q): the Jacobi product q_c * invd_c is recomputed at every
 // gathered column, bitwise the vector z = M q the reference forms first
-__global__ __launch_bounds__(kBlock) void k_gm_apply(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
-                                                     const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                     const double* __restrict__ val, const double* __restrict__ q,
-                                                     const double* __restrict__ invd, double* __restrict__ w,
-                                                     const GMState* st) {
+template<int MODE>
+__global__ __launch_bounds__(kBlock) void k_gm_apply(int64_t n, int64_t nb, Cols C, const double* __restrict__ val,
+                                                     const double* __restrict__ q, const double* __restrict__ invd,
+                                                     double* __restrict__ w, const GMState* st) {
     if (st->stop) return;
     const int64_t r = xcd_block(blockIdx.x, nb) * kBlock + threadIdx.x;
     if (r >= n) return;
-    const int64_t base = slice_ptr[r >> 6] + (r & 63);
-    const int len = rp[r + 1] - rp[r];
+    const int64_t base = C.slice_ptr[r >> 6] + (r & 63);
+    const RowCols<MODE> rc(r, C);
     double sum = 0.0;
-    for (int j = 0; j < len; ++j) {
+    for (int j = 0; j < rc.len; ++j) {
         const int64_t s = base + (int64_t)j * kSlice;
-        const int c = stream_load(col + s);
+        const int c = rc.col(C, r, s, j);
         const double z = invd ? q[c] * invd[c] : q[c];
         sum += stream_load(val + s) * z;
     }
@@ -133,16 +132,15 @@ __global__ void k_gm_update_x(int64_t n, const double* const* __restrict__ Q, co
 }
 
 // q0 = b - A x, beta = ||q0|| (Solver.tpp:22-27, 141-147)
-__global__ __launch_bounds__(kBlock) void k_gm_residual(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
-                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                        const double* __restrict__ val, const double* __restrict__ b,
-                                                        const double* __restrict__ x, double* __restrict__ q0,
-                                                        double* partials) {
+template<int MODE>
+__global__ __launch_bounds__(kBlock) void k_gm_residual(int64_t n, int64_t nb, Cols C, const double* __restrict__ val,
+                                                        const double* __restrict__ b, const double* __restrict__ x,
+                                                        double* __restrict__ q0, double* partials) {
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t k = lb * kBlock + threadIdx.x;
     double v[1] = {0.0};
     if (k < n) {
-        const double qk = b[k] + -1.0 * row_dot_f64(k, slice_ptr, rp, col, val, x);
+        const double qk = b[k] + -1.0 * row_dot_f64<MODE>(k, C, val, x);
         q0[k] = qk;
         v[0] = qk * qk;
     }
@@ -207,8 +205,8 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
     };
     GMRESResult res{};
     try {
-        hipLaunchKernelGGL(k_gm_residual, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, S.slice_ptr.p, S.row_ptr.p,
-                           S.col.p, A->val.p, b, x, hq[0], part);
+        RSD_LAUNCH_COLS(S.mode(), k_gm_residual, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(), A->val.p, b, x,
+                        hq[0], part);
         hipLaunchKernelGGL(k_gm_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part, dst, 1);
         read();
         const double res0 = h->res0;
@@ -224,8 +222,8 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
                 hipLaunchKernelGGL(k_gm_reset_flags, dim3(1), dim3(1), 0, st, dst);
                 hipLaunchKernelGGL(k_gm_cycle_start, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, hq[0], g, m, dst);
                 for (int j = 0; j < m; ++j) {
-                    hipLaunchKernelGGL(k_gm_apply, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, S.slice_ptr.p,
-                                       S.row_ptr.p, S.col.p, A->val.p, hq[j], invd, w, dst);
+                    RSD_LAUNCH_COLS(S.mode(), k_gm_apply, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(),
+                                    A->val.p, hq[j], invd, w, dst);
                     for (int i = 0; i <= j + 1; ++i) {
                         const bool norm = i == j + 1;
                         hipLaunchKernelGGL(k_gm_mgs, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
@@ -241,8 +239,8 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
                 hipLaunchKernelGGL(k_gm_backsub, dim3(1), dim3(1), 0, st, m, H, g, y, dst);
                 hipLaunchKernelGGL(k_gm_update_x, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, (const double* const*)dQ,
                                    y, invd, x, dst);
-                hipLaunchKernelGGL(k_gm_residual, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, S.slice_ptr.p,
-                                   S.row_ptr.p, S.col.p, A->val.p, b, x, hq[0], part);
+                RSD_LAUNCH_COLS(S.mode(), k_gm_residual, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(),
+                                A->val.p, b, x, hq[0], part);
                 hipLaunchKernelGGL(k_gm_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part, dst, 0);
                 RSD_HIP(hipGetLastError());
                 read();

@@ -64,12 +64,11 @@ __global__ __launch_bounds__(kFold) void k_dot_fold(int64_t nb, const double* __
 // ------------------------------------------------------------------------
 // SpMV: one thread per row, one wavefront per 64-row slice
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_spmv_f64(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
-                                                     const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                     const double* __restrict__ val, const double* __restrict__ x,
-                                                     double* __restrict__ y) {
+template<int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv_f64(int64_t n, int64_t nb, Cols C, const double* __restrict__ val,
+                                                     const double* __restrict__ x, double* __restrict__ y) {
     const int64_t r = xcd_block(blockIdx.x, nb) * kBlock + threadIdx.x;
-    if (r < n) y[r] = row_dot_f64(r, slice_ptr, rp, col, val, x);
+    if (r < n) y[r] = row_dot_f64<MODE>(r, C, val, x);
 }
 
 __global__ void k_inv_diag(int64_t n, const int32_t* __restrict__ diag, const double* __restrict__ val,
@@ -132,16 +131,16 @@ __global__ __launch_bounds__(kFold) void k_cg_init_fold(int64_t nb, const double
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_pap(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
-                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                        const double* __restrict__ val, const double* __restrict__ p,
-                                                        double* __restrict__ Ap, const CGState* st, double* partials) {
+template<int MODE>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_pap(int64_t n, int64_t nb, Cols C, const double* __restrict__ val,
+                                                        const double* __restrict__ p, double* __restrict__ Ap,
+                                                        const CGState* st, double* partials) {
     if (st->stop) return;
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t r = lb * kBlock + threadIdx.x;
     double v[1] = {0.0};
     if (r < n) {
-        const double y = row_dot_f64(r, slice_ptr, rp, col, val, p);
+        const double y = row_dot_f64<MODE>(r, C, val, p);
         Ap[r] = y;
         v[0] = p[r] * y;
     }
@@ -223,8 +222,8 @@ void launch_spmv_f64(rsd_ctx* ctx, const rsd_matrix* A, const double* x, double*
     const Sell& S = A->pb->sell;
     const int64_t n = S.n_rows, nb = nblocks(n);
     if (!n) return;
-    hipLaunchKernelGGL(k_spmv_f64, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, S.slice_ptr.p,
-                       S.row_ptr.p, S.col.p, A->val.p, x, y);
+    RSD_LAUNCH_COLS(S.mode(), k_spmv_f64, dim3((unsigned)nb), dim3(kBlock), ctx->stream, n, nb, S.cols(), A->val.p,
+                    x, y);
 }
 
 void launch_inv_diag(rsd_ctx* ctx, const rsd_matrix* A, double* inv) {
@@ -287,8 +286,8 @@ void cg_launch_iter(rsd_ctx* ctx, const rsd_matrix* A, double* x, double tol, in
     double* part = ctx->partials(3 * nb);
     const double* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     hipStream_t s = ctx->stream;
-    hipLaunchKernelGGL(k_cg_spmv_pap, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p, S.row_ptr.p,
-                       S.col.p, A->val.p, w.p, w.Ap, w.st, part);
+    RSD_LAUNCH_COLS(S.mode(), k_cg_spmv_pap, dim3((unsigned)nb), dim3(kBlock), s, n, nb, S.cols(), A->val.p, w.p, w.Ap,
+                    w.st, part);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kFold), 0, s, nb, part, w.st);
     hipLaunchKernelGGL(k_cg_update_r, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, x, w.p, w.Ap, w.r, invd, w.st,
                        part + nb);

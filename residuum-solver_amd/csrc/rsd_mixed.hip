@@ -4,8 +4,8 @@
 //   outer (FP64):  r = b - A x with the FP64 matrix, ||r|| in FP64;
 //                  stop when ||r|| / (||r_0|| + 1e-50) < tol, the same
 //                  relative criterion as cg/Solver.tpp:45-48;
-//   inner (FP32):  solve A d = r / ||r|| with FP32 values (and int16
-//                  row-relative column offsets when every offset fits),
+//   inner (FP32):  solve A d = r / ||r|| with FP32 values (shared offset
+//                  patterns or int16 row-relative offsets when they apply),
 //                  FP32 vectors and FP32-accumulated SpMV rows, by the
 //                  Chronopoulos-Gear form of PCG (one fused reduction of
 //                  three dot products per iteration, accumulated in FP64):
@@ -13,7 +13,8 @@
 //                    beta = gamma / gamma', alpha = gamma / (delta - beta gamma / alpha'),
 //                    p = u + beta p, s = w + beta s, d += alpha p, r -= alpha s;
 //   update (FP64): x += ||r|| d.
-// Per inner iteration the matrix stream is 6 bytes per slot instead of 12,
+// Per inner iteration the matrix stream is 4-6 bytes per slot instead of
+// 8-12 (no index stream when the rows share offset patterns),
 // and an iteration is three launches: the SpMV with w.u, the one-block fold
 // of the three dot products, and the vector update, which also forms r.u
 // and r.r of the new residual for the next iteration.
@@ -26,64 +27,28 @@
 
 namespace rsd {
 
-template<bool OFF16>
-__device__ __forceinline__ float row_dot_f32(int64_t r, const int64_t* __restrict__ slice_ptr,
-                                             const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                             const int16_t* __restrict__ off, const float* __restrict__ val,
-                                             const float* __restrict__ x) {
-    constexpr int U = 8;
-    const int64_t base = slice_ptr[r >> 6] + (r & 63);
-    const int len = rp[r + 1] - rp[r];
-    float sum = 0.0f;
-    int j = 0;
-    for (; j + U <= len; j += U) {
-        const int64_t s0 = base + (int64_t)j * kSlice;
-        int c[U];
-        float v[U];
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-            c[k] = OFF16 ? (int)r + (int)stream_load(off + s0 + k * kSlice) : stream_load(col + s0 + k * kSlice);
-            v[k] = stream_load(val + s0 + k * kSlice);
-        }
-        float xv[U];
-#pragma unroll
-        for (int k = 0; k < U; ++k) xv[k] = x[c[k]];
-#pragma unroll
-        for (int k = 0; k < U; ++k) sum = __builtin_fmaf(v[k], xv[k], sum);
-    }
-    for (; j < len; ++j) {
-        const int64_t s0 = base + (int64_t)j * kSlice;
-        const int c = OFF16 ? (int)r + (int)stream_load(off + s0) : stream_load(col + s0);
-        sum = __builtin_fmaf(stream_load(val + s0), x[c], sum);
-    }
-    return sum;
-}
-
 __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
 }
 
-template<bool OFF16>
-__global__ __launch_bounds__(kBlock) void k_spmv_f32(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
-                                                     const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                     const int16_t* __restrict__ off, const float* __restrict__ val,
+template<int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv_f32(int64_t n, int64_t nb, Cols C, const float* __restrict__ val,
                                                      const float* __restrict__ x, float* __restrict__ y) {
     const int64_t r = xcd_block(blockIdx.x, nb) * kBlock + threadIdx.x;
-    if (r < n) y[r] = row_dot_f32<OFF16>(r, slice_ptr, rp, col, off, val, x);
+    if (r < n) y[r] = row_dot_f32<MODE>(r, C, val, x);
 }
 
 // outer residual r = b - A x (FP64), block sums of ||r||^2
-__global__ __launch_bounds__(kBlock) void k_mx_residual(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
-                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                        const double* __restrict__ val, const double* __restrict__ b,
-                                                        const double* __restrict__ x, double* __restrict__ r64,
-                                                        double* partials) {
+template<int MODE>
+__global__ __launch_bounds__(kBlock) void k_mx_residual(int64_t n, int64_t nb, Cols C, const double* __restrict__ val,
+                                                        const double* __restrict__ b, const double* __restrict__ x,
+                                                        double* __restrict__ r64, double* partials) {
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t i = lb * kBlock + threadIdx.x;
     double v[1] = {0.0};
     if (i < n) {
-        const double ri = b[i] + -1.0 * row_dot_f64(i, slice_ptr, rp, col, val, x);
+        const double ri = b[i] + -1.0 * row_dot_f64<MODE>(i, C, val, x);
         r64[i] = ri;
         v[0] = ri * ri;
     }
@@ -126,10 +91,8 @@ __global__ __launch_bounds__(kBlock) void k_mx_init(int64_t n, int64_t nb, const
 }
 
 // w = A u, block sums of w.u
-template<bool OFF16>
-__global__ __launch_bounds__(kBlock) void k_mx_spmv_wu(int64_t n, int64_t nb, const int64_t* __restrict__ slice_ptr,
-                                                       const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                                       const int16_t* __restrict__ off, const float* __restrict__ val,
+template<int MODE>
+__global__ __launch_bounds__(kBlock) void k_mx_spmv_wu(int64_t n, int64_t nb, Cols C, const float* __restrict__ val,
                                                        const float* __restrict__ u, float* __restrict__ w,
                                                        const MixedState* st, double* P0) {
     if (st->stop) return;
@@ -137,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_spmv_wu(int64_t n, int64_t nb, co
     const int64_t i = lb * kBlock + threadIdx.x;
     double v[1] = {0.0};
     if (i < n) {
-        const float wi = row_dot_f32<OFF16>(i, slice_ptr, rp, col, off, val, u);
+        const float wi = row_dot_f32<MODE>(i, C, val, u);
         w[i] = wi;
         v[0] = (double)wi * (double)u[i];
     }
@@ -231,12 +194,8 @@ void launch_spmv_f32(rsd_ctx* ctx, const rsd_matrix* A, const float* x, float* y
     const Sell& S = A->pb->sell;
     const int64_t n = S.n_rows, nb = nblocks(n);
     if (!n) return;
-    if (S.has_off16)
-        hipLaunchKernelGGL(k_spmv_f32<true>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, S.slice_ptr.p,
-                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, x, y);
-    else
-        hipLaunchKernelGGL(k_spmv_f32<false>, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, S.slice_ptr.p,
-                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, x, y);
+    RSD_LAUNCH_COLS(S.mode(), k_spmv_f32, dim3((unsigned)nb), dim3(kBlock), ctx->stream, n, nb, S.cols(), A->val32.p,
+                    x, y);
 }
 
 MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n) {
@@ -260,12 +219,8 @@ void mixed_launch_inner_iter(rsd_ctx* ctx, const rsd_matrix* A, int32_t precond,
     double* P = ctx->partials(3 * nb);
     const float* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     hipStream_t s = ctx->stream;
-    if (S.has_off16)
-        hipLaunchKernelGGL(k_mx_spmv_wu<true>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p,
-                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.u, w.w, w.st, P);
-    else
-        hipLaunchKernelGGL(k_mx_spmv_wu<false>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.slice_ptr.p,
-                           S.row_ptr.p, S.col.p, S.off16.p, A->val32.p, w.u, w.w, w.st, P);
+    RSD_LAUNCH_COLS(S.mode(), k_mx_spmv_wu, dim3((unsigned)nb), dim3(kBlock), s, n, nb, S.cols(), A->val32.p, w.u,
+                    w.w, w.st, P);
     hipLaunchKernelGGL(k_mx_scalars, dim3(1), dim3(kFold), 0, s, nb, P, w.st);
     hipLaunchKernelGGL(k_mx_update, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.p, w.s, w.d, w.r, w.u, w.w, invd,
                        w.st, P + nb);
@@ -298,8 +253,8 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
     };
     try {
         for (int cycle = 0;; ++cycle) {
-            hipLaunchKernelGGL(k_mx_residual, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, S.slice_ptr.p, S.row_ptr.p,
-                               S.col.p, A->val.p, b, x, w.r64, part);
+            RSD_LAUNCH_COLS(S.mode(), k_mx_residual, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(), A->val.p, b,
+                            x, w.r64, part);
             hipLaunchKernelGGL(k_mx_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part, w.st);
             RSD_HIP(hipMemcpyAsync(h, w.st, sizeof(MixedState), hipMemcpyDeviceToHost, st));
             RSD_HIP(hipStreamSynchronize(st));

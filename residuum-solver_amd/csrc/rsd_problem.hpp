@@ -3,6 +3,7 @@
 #pragma once
 
 #include "rsd_internal.hpp"
+#include "rsd_rowdot.hpp"
 
 namespace rsd {
 
@@ -30,7 +31,39 @@ struct Sell {
     DBuf<int16_t> off16;       // [slots] column - row, when every offset fits
     DBuf<int32_t> diag;        // [n_rows] slot of the diagonal entry, -1 if absent
     bool has_off16 = false;
+    // shared column-offset patterns (COL_PAT): pattern id per row and the
+    // offset lists, when at most 255 distinct lists cover every row
+    DBuf<uint8_t> pat_id;      // [n_rows]
+    DBuf<int32_t> pat_off;     // [n_pat * pat_stride] column - row, CSR order
+    DBuf<int32_t> pat_len;     // [n_pat]
+    int32_t n_pat = 0, pat_stride = 0;
+    bool has_pat = false;
+
+    Cols cols() const {
+        Cols c;
+        c.slice_ptr = slice_ptr.p;
+        c.rp = row_ptr.p;
+        c.col = col.p;
+        c.off16 = off16.p;
+        c.pid = pat_id.p;
+        c.poff = pat_off.p;
+        c.plen = pat_len.p;
+        c.stride = pat_stride;
+        return c;
+    }
+    // the cheapest column mode this layout supports
+    int mode() const { return has_pat ? COL_PAT : has_off16 ? COL_OFF16 : COL_ABS; }
 };
+
+// launch kern<MODE> for the layout's column mode
+#define RSD_LAUNCH_COLS(mode, kern, grid, block, stream, ...)                                        \
+    do {                                                                                            \
+        switch (mode) {                                                                             \
+        case ::rsd::COL_PAT: hipLaunchKernelGGL(kern<::rsd::COL_PAT>, grid, block, 0, stream, __VA_ARGS__); break;     \
+        case ::rsd::COL_OFF16: hipLaunchKernelGGL(kern<::rsd::COL_OFF16>, grid, block, 0, stream, __VA_ARGS__); break; \
+        default: hipLaunchKernelGGL(kern<::rsd::COL_ABS>, grid, block, 0, stream, __VA_ARGS__); break;                 \
+        }                                                                                           \
+    } while (0)
 
 }  // namespace rsd
 

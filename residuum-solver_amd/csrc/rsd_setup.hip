@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <thread>
+#include <unordered_map>
+#include <vector>
 
 namespace rsd {
 
@@ -289,6 +291,51 @@ void rsd::build_sell(rsd_problem* pb, const std::vector<int32_t>& rowlen) {
             }
         }
     S.has_off16 = fits16;
+    // shared offset patterns: FNV-1a hash of (length, offsets), every hit
+    // verified against the stored list; given up past 255 patterns, rows
+    // longer than 64 or a table beyond 16 Ki entries
+    {
+        std::vector<std::vector<int32_t>> pats;
+        std::unordered_map<uint64_t, int> by_hash;
+        std::vector<uint8_t> pid(n);
+        bool ok = pb->max_row <= 64;
+        std::vector<int32_t> cur;
+        for (int64_t r = 0; r < n && ok; ++r) {
+            cur.clear();
+            uint64_t h = 1469598103934665603ull ^ (uint64_t)rowlen[r];
+            for (int32_t k = pb->h_rp[r]; k < pb->h_rp[r + 1]; ++k) {
+                const int32_t o = pb->h_ci[k] - (int32_t)r;
+                cur.push_back(o);
+                h = (h ^ (uint64_t)(uint32_t)o) * 1099511628211ull;
+            }
+            auto it = by_hash.find(h);
+            if (it != by_hash.end()) {
+                if (pats[it->second] != cur) ok = false;
+                else pid[r] = (uint8_t)it->second;
+            } else if (pats.size() >= 255) {
+                ok = false;
+            } else {
+                by_hash.emplace(h, (int)pats.size());
+                pid[r] = (uint8_t)pats.size();
+                pats.push_back(cur);
+            }
+        }
+        const int32_t stride = std::max<int32_t>(pb->max_row, 1);
+        ok = ok && n > 0 && (int64_t)pats.size() * stride <= 16384;
+        S.has_pat = ok;
+        if (ok) {
+            std::vector<int32_t> off((size_t)pats.size() * stride, 0), len(pats.size());
+            for (size_t q = 0; q < pats.size(); ++q) {
+                len[q] = (int32_t)pats[q].size();
+                std::copy(pats[q].begin(), pats[q].end(), off.begin() + (int64_t)q * stride);
+            }
+            S.n_pat = (int32_t)pats.size();
+            S.pat_stride = stride;
+            S.pat_id.upload(pid.data(), n, st);
+            S.pat_off.upload(off.data(), (int64_t)off.size(), st);
+            S.pat_len.upload(len.data(), (int64_t)len.size(), st);
+        }
+    }
     S.slice_ptr.upload(S.h_slice_ptr.data(), S.n_slices + 1, st);
     S.row_ptr.upload(pb->h_rp.data(), n + 1, st);
     S.col.upload(hcol.data(), S.slots, st);
@@ -658,6 +705,16 @@ int rsd_problem_sizes(rsd_problem* pb, int64_t* out) {
         RSD_CHECK(pb && out, "null argument");
         out[0] = pb->n_nodes; out[1] = pb->n_elem; out[2] = pb->npe; out[3] = pb->n_free;
         out[4] = pb->nnz; out[5] = pb->n_colours(); out[6] = pb->sell.slots; out[7] = pb->max_row;
+    });
+}
+
+int rsd_problem_layout(rsd_problem* pb, int64_t* out) {
+    return guard([&] {
+        RSD_CHECK(pb && out, "null argument");
+        out[0] = pb->sell.mode();
+        out[1] = pb->sell.n_pat;
+        out[2] = pb->sell.has_off16;
+        out[3] = pb->sell.slots;
     });
 }
 

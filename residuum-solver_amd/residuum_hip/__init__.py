@@ -5,6 +5,7 @@ A thin ctypes layer over the C-ABI declared in include/residuum_hip.h.
 There is no fallback: if the shared library is missing or no GPU is
 visible, the calls raise.
 """
+import atexit
 import ctypes as C
 import os
 import weakref
@@ -74,6 +75,21 @@ def dptr(a):
     return _vp(a)
 
 
+# Contexts still open at interpreter exit are closed by an atexit hook, while
+# the HIP runtime is still up: objects left to module teardown would
+# otherwise free device memory after the runtime's own destructors ran.
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts():
+    for c in list(_live_contexts):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
 class Ctx:
     """rsd_ctx: one HIP stream on a device."""
 
@@ -81,6 +97,7 @@ class Ctx:
         self.h = _vp()
         self._children = weakref.WeakSet()
         check(lib().rsd_ctx_create(C.c_int(device), C.byref(self.h)))
+        _live_contexts.add(self)
 
     def _adopt(self, obj):
         self._children.add(obj)
@@ -243,6 +260,13 @@ class Problem:
         check(lib().rsd_problem_get_mesh(self.h, xyz.ctypes.data_as(_d), ien.ctypes.data_as(_i64),
                                          rng.ctypes.data_as(_i32)))
         return xyz, ien, rng
+
+    def layout(self):
+        """(column mode, patterns, int16 offsets fit, slots); mode 0 absolute
+        columns, 1 int16 offsets, 2 shared offset patterns."""
+        out = (C.c_int64 * 4)()
+        check(lib().rsd_problem_layout(self.h, out))
+        return tuple(int(v) for v in out)
 
     def dofs(self):
         t2a = np.zeros(self.n_nodes, np.int32)
