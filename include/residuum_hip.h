@@ -231,13 +231,13 @@ int rsd_matrix_load_csr(rsd_ctx* ctx, rsd_matrix* A, const double* val);
 
 /* Mixed precision (no reference counterpart; the reference is FP64 only).
  * rsd_matrix_to_f32 refreshes the FP32 copy of A's values; rsd_spmv_f32 is
- * y = A x with FP32 values, FP32 accumulation and int16 row-relative column
- * offsets when they fit.  rsd_cg_solve_mixed: iterative refinement with an
+ * y = A x with FP32 values and FP32 accumulation, on the same column
+ * information as the FP64 matrix.  rsd_cg_solve_mixed: iterative refinement with an
  * FP64 outer residual r = b - A x and an FP32 inner PCG whose dot products
  * accumulate in FP64; stops on the FP64 relative residual
  * ||r|| / (||r_0|| + 1e-50) < tol.  inner_floor bounds the inner relative
- * tolerance from below (<= 0: 1e-5).  rep->iterations counts inner
- * iterations. */
+ * tolerance from below (<= 0: 1e-4).  rep->iterations counts inner
+ * iterations (SpMVs). */
 int rsd_matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A);
 int rsd_spmv_f32(rsd_ctx* ctx, rsd_matrix* A, const float* x, float* y);
 int rsd_cg_solve_mixed(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,

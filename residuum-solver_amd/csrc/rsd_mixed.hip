@@ -328,7 +328,8 @@ int rsd_cg_solve_mixed(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, 
         RSD_CHECK(ctx && A && b && x, "rsd_cg_solve_mixed: null argument");
         RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
         MixedWork w = mixed_workspace(ctx, A->pb->sell.n_rows);
-        MixedState s = mixed_run(ctx, A, b, x, tol, max_iters, precond, inner_floor > 0 ? inner_floor : 1e-5, false, w,
+        MixedState s = mixed_run(ctx, A, b, x, tol, max_iters, precond,
+                                 inner_floor > 0 ? inner_floor : kMixedDefaultInnerFloor, false, w,
                                  nullptr);
         if (rep) {
             rep->converged = s.converged;

@@ -33,6 +33,7 @@ void launch_spmv_f32(rsd_ctx* ctx, const rsd_matrix* A, const float* x, float* y
 MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n);
 void mixed_launch_inner_iter(rsd_ctx* ctx, const rsd_matrix* A, int32_t precond, const MixedWork& w);
 constexpr int kMixedMaxCycles = 40;
+constexpr double kMixedDefaultInnerFloor = 1e-4;
 // hint: optional [kMixedMaxCycles] inner iteration counts of a previous solve,
 // used to queue chunks before the first look at the stop flag (updated)
 MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
