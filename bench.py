@@ -82,14 +82,21 @@ def cpu_baseline(pb, K, b_host, iters):
     return pb.n_free * rep.iterations / dt, rep.iterations, dt
 
 
-def measured_traffic(n):
+LAYOUTS = ["int32 columns", "int16 offsets", "shared offset patterns", "symmetric half storage"]
+
+
+def measured_traffic(n, layout):
     """HBM bytes per k_spmv_f32 launch at this size, from the committed
     FETCH_SIZE / WRITE_SIZE counter profile (scaled by the calibration copy's
-    counters); None when no profile for this size is committed."""
+    counters); None when no profile for this size and FP32 layout is
+    committed."""
     path = os.path.join(ROOT, "profiles", "spmv_f32_%dcube_hbm_counters.json" % n)
     try:
         with open(path) as f:
-            return float(json.load(f)["traffic_bytes_per_launch"])
+            prof = json.load(f)
+        if prof.get("layout") != layout:
+            return None
+        return float(prof["traffic_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
         return None
 
@@ -124,9 +131,11 @@ def main():
     t_setup = time.perf_counter() - t_setup
     nfree, nnz, slots = pb.n_free, pb.nnz, pb.slots
     col_mode, npat, _, _ = pb.layout()
+    mode32, npat32, width32, slots32 = pb.layout_f32()
+    layout32 = LAYOUTS[mode32]
     print(f"# setup {t_setup:.2f}s: nodes={pb.n_nodes} free={nfree} nnz={nnz} slots={slots} colours={pb.n_colours} "
-          f"column mode={['int32 columns', 'int16 offsets', 'shared offset patterns'][col_mode]} ({npat} patterns)",
-          file=sys.stderr, flush=True)
+          f"column mode={LAYOUTS[col_mode]} ({npat} patterns); fp32 copy: {layout32} ({npat32} patterns, "
+          f"{slots32} slots)", file=sys.stderr, flush=True)
 
     results = {}
     for mode in ("fp64", "mixed"):
@@ -165,13 +174,19 @@ def main():
     spmv = {}
     # algorithmic bytes of one product in the layout in use: values, the
     # column information (int32 columns, int16 offsets, or a 1-byte pattern
-    # id per row), the row lengths where they are streamed, x once, y once
-    if col_mode == 2:
-        b64, b32 = 8 * nnz + nfree + 16 * nfree, 4 * nnz + nfree + 8 * nfree
-    elif col_mode == 1:
-        b64, b32 = 10 * nnz + 4 * (nfree + 1) + 16 * nfree, 6 * nnz + 4 * (nfree + 1) + 8 * nfree
-    else:
-        b64, b32 = 12 * nnz + 4 * (nfree + 1) + 16 * nfree, 8 * nnz + 4 * (nfree + 1) + 8 * nfree
+    # id per row), the row lengths where they are streamed, x once, y once;
+    # the symmetric half storage holds (nnz + n) / 2 values, each read once
+    # from HBM (the second reader of a pair finds it in L2 or not: the
+    # measured traffic tells)
+    def bytes_of(mode, vb):
+        if mode == 3:
+            return vb * ((nnz + nfree) // 2) + nfree + 2 * vb * nfree
+        if mode == 2:
+            return vb * nnz + nfree + 2 * vb * nfree
+        if mode == 1:
+            return (vb + 2) * nnz + 4 * (nfree + 1) + 2 * vb * nfree
+        return (vb + 4) * nnz + 4 * (nfree + 1) + 2 * vb * nfree
+    b64, b32 = bytes_of(col_mode, 8), bytes_of(mode32, 4)
     for name, fn, byts in (("spmv_f64", lambda: K.spmv(x64, y64), b64),
                            ("spmv_f32", lambda: K.spmv_f32(x32, y32), b32)):
         for _ in range(3):
@@ -233,10 +248,9 @@ def main():
                             "step_time": fp["total_ms"] / mx["total_ms"]},
         "mixed_vs_fp64_solution_max_rel_diff": mixed_vs_fp64_err,
         "spmv": spmv,
-        "roofline": {"bound": "hbm", "kernel": "k_spmv_f32 (SELL-64, fp32 values, "
-                                               + ["int32 columns", "int16 offsets", "shared offset patterns"][col_mode] + ")",
+        "roofline": {"bound": "hbm", "kernel": "k_spmv_f32 (SELL-64, fp32 values, " + layout32 + ")",
                      "achieved": spmv["spmv_f32"]["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": spmv["spmv_f32"]["gbs"] / PEAK_HBM_GBS, "traffic": measured_traffic(n)},
+                     "frac": spmv["spmv_f32"]["gbs"] / PEAK_HBM_GBS, "traffic": measured_traffic(n, layout32)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -134,6 +134,13 @@ int rsd_problem_get_mesh(rsd_problem* pb, double* xyz, int64_t* ien, int32_t* rn
  * rows: a pattern id per row, no index streamed per slot), number of
  * patterns, whether int16 offsets fit, sliced-ELL slots */
 int rsd_problem_layout(rsd_problem* pb, int64_t* out);
+/* layout of the FP32 copy used by rsd_spmv_f32 and the mixed CG (no reference
+ * counterpart; decided here on first call if no FP32 work came first):
+ * out[4] = column mode (3: symmetric half storage, the upper value of each
+ * symmetric pair kept once, below-diagonal entries read from the row above;
+ * else the FP64 layout's mode), number of patterns, upper entries per row
+ * (half storage; 0 otherwise), FP32 slots */
+int rsd_problem_layout_f32(rsd_problem* pb, int64_t* out);
 /* topo->alg map (-1 constrained), alg->topo, constraint tag per node */
 int rsd_problem_get_dofs(rsd_problem* pb, int32_t* topo_to_alg, int32_t* alg_to_topo, int32_t* ctag);
 /* CSR pattern as the reference's size_t arrays, widened to int64 */
@@ -231,10 +238,16 @@ int rsd_matrix_load_csr(rsd_ctx* ctx, rsd_matrix* A, const double* val);
 
 /* Mixed precision (no reference counterpart; the reference is FP64 only).
  * rsd_matrix_to_f32 refreshes the FP32 copy of A's values; rsd_spmv_f32 is
- * y = A x with FP32 values and FP32 accumulation, on the same column
- * information as the FP64 matrix.  rsd_cg_solve_mixed: iterative refinement with an
- * FP64 outer residual r = b - A x and an FP32 inner PCG whose dot products
- * accumulate in FP64; stops on the FP64 relative residual
+ * y = A x with FP32 values and FP32 accumulation.  When the pattern is
+ * structurally symmetric and its rows share offset patterns (block meshes),
+ * the copy keeps each symmetric pair once, the upper value, and the product
+ * uses it for both entries: the symmetric operator the mixed CG needs (an
+ * assembled heat matrix is symmetric to FP64 rounding; the conversion flags a
+ * pair that differs by more than 1e-10 of its row's largest entry, and
+ * rsd_cg_solve_mixed then fails with RSD_ERR_ARG).  rsd_problem_layout_f32
+ * tells which layout is in use.  rsd_cg_solve_mixed: iterative refinement
+ * with an FP64 outer residual r = b - A x and an FP32 inner PCG whose dot
+ * products accumulate in FP64; stops on the FP64 relative residual
  * ||r|| / (||r_0|| + 1e-50) < tol.  inner_floor bounds the inner relative
  * tolerance from below (<= 0: 1e-4).  rep->iterations counts inner
  * iterations (SpMVs). */

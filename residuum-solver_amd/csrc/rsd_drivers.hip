@@ -140,7 +140,8 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
     MixedWork mw;
     if (mixed) {
         mw = mixed_workspace(ctx, n);
-        K.val32.alloc(pb->sell.slots);
+        build_half(pb);
+        K.val32.alloc(pb->sell.slots32());
     }
     update_gval(pb, t0);
     launch_copy(ctx, n, U, b.Uprev);

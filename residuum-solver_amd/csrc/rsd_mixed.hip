@@ -4,20 +4,23 @@
 //   outer (FP64):  r = b - A x with the FP64 matrix, ||r|| in FP64;
 //                  stop when ||r|| / (||r_0|| + 1e-50) < tol, the same
 //                  relative criterion as cg/Solver.tpp:45-48;
-//   inner (FP32):  solve A d = r / ||r|| with FP32 values (shared offset
-//                  patterns or int16 row-relative offsets when they apply),
-//                  FP32 vectors and FP32-accumulated SpMV rows, by the
-//                  Chronopoulos-Gear form of PCG (one fused reduction of
-//                  three dot products per iteration, accumulated in FP64):
+//   inner (FP32):  solve A d = r / ||r|| with FP32 values, FP32 vectors and
+//                  FP32-accumulated SpMV rows, by the Chronopoulos-Gear
+//                  form of PCG (one fused reduction of three dot products
+//                  per iteration, accumulated in FP64):
 //                    u = M r, w = A u, delta = w.u, gamma = r.u,
 //                    beta = gamma / gamma', alpha = gamma / (delta - beta gamma / alpha'),
 //                    p = u + beta p, s = w + beta s, d += alpha p, r -= alpha s;
 //   update (FP64): x += ||r|| d.
-// Per inner iteration the matrix stream is 4-6 bytes per slot instead of
-// 8-12 (no index stream when the rows share offset patterns),
-// and an iteration is three launches: the SpMV with w.u, the one-block fold
-// of the three dot products, and the vector update, which also forms r.u
-// and r.r of the new residual for the next iteration.
+// The FP32 copy of a structurally symmetric matrix whose rows share offset
+// patterns keeps each symmetric pair's value once (the upper one; the
+// conversion checks that the FP64 matrix is symmetric to 1e-10 of each row's
+// largest entry, and the solve refuses one that is not): per inner iteration
+// the matrix stream is about 2 bytes per nonzero instead of the FP64
+// solver's 8, with no index stream.  An iteration is three launches: the
+// SpMV with w.u, the one-block fold of the three dot products, and the
+// vector update, which also forms r.u and r.r of the new residual for the
+// next iteration.
 #include "rsd_linalg.hpp"
 #include "rsd_mixed.hpp"
 #include "rsd_rowdot.hpp"
@@ -30,6 +33,43 @@ namespace rsd {
 __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void k_set_flag(int32_t* f, int32_t v) {
+    if (threadIdx.x == 0) *f = v;
+}
+
+// The half layout's FP32 upper values from the FP64 SELL values (one thread
+// per row): the m-th value right of the diagonal part of row r goes to
+// m * pad + r.  The row's values left of the diagonal are compared with the
+// FP64 values stored for the mirrored entries; *asym is set when one differs
+// by more than 1e-10 of the row's largest magnitude (an assembled heat
+// matrix is symmetric to rounding).
+__global__ __launch_bounds__(kBlock) void k_to_f32_half(int64_t n, Cols H, const int32_t* __restrict__ mir,
+                                                        const int32_t* __restrict__ low, int64_t pad,
+                                                        const double* __restrict__ val, float* __restrict__ vh,
+                                                        int32_t* __restrict__ asym) {
+    const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const int q = H.pid[r];
+    const int len = H.plen[q], lo = low[q];
+    const int32_t* po = H.poff + q * H.stride;
+    const int32_t* pm = mir + q * H.stride;
+    const int64_t base = H.slice_ptr[r >> 6] + (r & 63);
+    double amax = 0.0, dmax = 0.0;
+    for (int j = 0; j < len; ++j) {
+        const double a = val[base + (int64_t)j * kSlice];
+        amax = fmax(amax, fabs(a));
+        if (j >= lo) {
+            vh[(int64_t)(j - lo) * pad + r] = (float)a;
+        } else {
+            const int64_t c = r + po[j];
+            const int lc = low[H.pid[c]];
+            const double b = val[H.slice_ptr[c >> 6] + (int64_t)(lc + pm[j]) * kSlice + (c & 63)];
+            dmax = fmax(dmax, fabs(a - b));
+        }
+    }
+    if (dmax > 1e-10 * amax) *asym = 1;
 }
 
 template<int MODE>
@@ -56,13 +96,14 @@ __global__ __launch_bounds__(kBlock) void k_mx_residual(int64_t n, int64_t nb, C
 }
 
 __global__ __launch_bounds__(kFold) void k_mx_residual_fold(int64_t nb, const double* __restrict__ partials,
-                                                            MixedState* st) {
+                                                            const int32_t* __restrict__ asym, MixedState* st) {
     double out[1];
     fold_partials<1>(nb, partials, out);
     if (threadIdx.x == 0) {
         const double res = sqrt(out[0]);
         if (st->cycles == 0) st->res0 = res;
         st->res = res;
+        st->asym = asym ? *asym : 0;
     }
 }
 
@@ -184,18 +225,28 @@ __global__ void k_inv_diag_f32(int64_t n, const int32_t* __restrict__ diag, cons
 
 // ------------------------------------------------------------------------
 void matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A) {
-    if (A->val32.n != A->val.n) A->val32.alloc(A->val.n);
-    if (A->val.n)
+    build_half(A->pb);
+    const Sell& S = A->pb->sell;
+    if (A->val32.n != S.slots32()) A->val32.alloc(S.slots32());
+    if (S.has_half) {
+        if (!A->asym.n) A->asym.alloc(1);
+        hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, ctx->stream, A->asym.p, 0);
+        if (S.n_rows)
+            hipLaunchKernelGGL(k_to_f32_half, dim3((unsigned)nblocks(S.n_rows)), dim3(kBlock), 0, ctx->stream,
+                               S.n_rows, S.cols32(), S.half_mir.p, S.half_low.p, S.half_pad, A->val.p, A->val32.p,
+                               A->asym.p);
+    } else if (A->val.n) {
         hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nblocks(A->val.n)), dim3(kBlock), 0, ctx->stream, A->val.n,
                            A->val.p, A->val32.p);
+    }
 }
 
 void launch_spmv_f32(rsd_ctx* ctx, const rsd_matrix* A, const float* x, float* y) {
     const Sell& S = A->pb->sell;
     const int64_t n = S.n_rows, nb = nblocks(n);
     if (!n) return;
-    RSD_LAUNCH_COLS(S.mode(), k_spmv_f32, dim3((unsigned)nb), dim3(kBlock), ctx->stream, n, nb, S.cols(), A->val32.p,
-                    x, y);
+    RSD_LAUNCH_COLS32(S.mode32(), k_spmv_f32, dim3((unsigned)nb), dim3(kBlock), ctx->stream, n, nb, S.cols32(),
+                      A->val32.p, x, y);
 }
 
 MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n) {
@@ -219,8 +270,8 @@ void mixed_launch_inner_iter(rsd_ctx* ctx, const rsd_matrix* A, int32_t precond,
     double* P = ctx->partials(3 * nb);
     const float* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     hipStream_t s = ctx->stream;
-    RSD_LAUNCH_COLS(S.mode(), k_mx_spmv_wu, dim3((unsigned)nb), dim3(kBlock), s, n, nb, S.cols(), A->val32.p, w.u,
-                    w.w, w.st, P);
+    RSD_LAUNCH_COLS32(S.mode32(), k_mx_spmv_wu, dim3((unsigned)nb), dim3(kBlock), s, n, nb, S.cols32(), A->val32.p,
+                      w.u, w.w, w.st, P);
     hipLaunchKernelGGL(k_mx_scalars, dim3(1), dim3(kFold), 0, s, nb, P, w.st);
     hipLaunchKernelGGL(k_mx_update, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.p, w.s, w.d, w.r, w.u, w.w, invd,
                        w.st, P + nb);
@@ -255,9 +306,13 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
         for (int cycle = 0;; ++cycle) {
             RSD_LAUNCH_COLS(S.mode(), k_mx_residual, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(), A->val.p, b,
                             x, w.r64, part);
-            hipLaunchKernelGGL(k_mx_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part, w.st);
+            hipLaunchKernelGGL(k_mx_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part,
+                               S.has_half ? A->asym.p : nullptr, w.st);
             RSD_HIP(hipMemcpyAsync(h, w.st, sizeof(MixedState), hipMemcpyDeviceToHost, st));
             RSD_HIP(hipStreamSynchronize(st));
+            if (h->asym)
+                throw Error(RSD_ERR_ARG, "mixed CG: the matrix is not symmetric (a value differs from its mirror by "
+                                         "more than 1e-10 of its row's largest entry)");
             const double rel = h->res / (h->res0 + 1e-50);
             if (rel < tol) { h->converged = 1; break; }
             if (cycle >= kMixedMaxCycles || h->total_inner >= max_iters || !(rel == rel)) break;
@@ -316,7 +371,7 @@ int rsd_matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A) {
 int rsd_spmv_f32(rsd_ctx* ctx, rsd_matrix* A, const float* x, float* y) {
     return guard([&] {
         RSD_CHECK(ctx && A && x && y, "null argument");
-        RSD_CHECK(A->val32.n == A->val.n, "rsd_spmv_f32: call rsd_matrix_to_f32 first");
+        RSD_CHECK(A->val32.n == A->pb->sell.slots32(), "rsd_spmv_f32: call rsd_matrix_to_f32 first");
         launch_spmv_f32(ctx, A, x, y);
         RSD_HIP(hipGetLastError());
     });

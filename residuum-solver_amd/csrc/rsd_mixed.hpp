@@ -13,7 +13,7 @@ struct MixedState {
     double alpha, beta;        // scalars of the current inner iteration
     int64_t it;                // inner iteration index
     int64_t in_max, total_inner;  // inner iteration cap; SpMVs of all inner solves
-    int32_t cycles, converged, stop, pad;
+    int32_t cycles, converged, stop, asym;  // asym: the FP32 copy's symmetry check failed
 };
 
 struct MixedWork {

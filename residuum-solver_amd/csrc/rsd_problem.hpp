@@ -38,6 +38,18 @@ struct Sell {
     DBuf<int32_t> pat_len;     // [n_pat]
     int32_t n_pat = 0, pat_stride = 0;
     bool has_pat = false;
+    // symmetric half storage of the FP32 copy (COL_HALF), built on first
+    // use when the rows share offset patterns, the pattern is structurally
+    // symmetric and at most 255 (offset, upper index) lists cover every row
+    DBuf<uint8_t> half_id;     // [n_rows]
+    DBuf<int32_t> half_off;    // [n_half * pat_stride] column - row, CSR order
+    DBuf<int32_t> half_mir;    // [n_half * pat_stride] index m in the upper list of row min(r, c)
+    DBuf<int32_t> half_val;    // [n_half * pat_stride] m * half_pad + min(c - r, 0)
+    DBuf<int32_t> half_len;    // [n_half] entries
+    DBuf<int32_t> half_low;    // [n_half] entries left of the diagonal
+    int32_t n_half = 0, half_width = 0;
+    int64_t half_pad = 0;      // rows rounded up to 64: the stride between upper-value planes
+    bool has_half = false, half_tried = false;
 
     Cols cols() const {
         Cols c;
@@ -53,6 +65,19 @@ struct Sell {
     }
     // the cheapest column mode this layout supports
     int mode() const { return has_pat ? COL_PAT : has_off16 ? COL_OFF16 : COL_ABS; }
+    // the FP32 copy: symmetric half storage when built, else the FP64 layout
+    int mode32() const { return has_half ? COL_HALF : mode(); }
+    int64_t slots32() const { return has_half ? half_pad * half_width : slots; }
+    Cols cols32() const {
+        if (!has_half) return cols();
+        Cols c = cols();
+        c.pid = half_id.p;
+        c.poff = half_off.p;
+        c.plen = half_len.p;
+        c.pval = half_val.p;
+        c.stride = pat_stride;
+        return c;
+    }
 };
 
 // launch kern<MODE> for the layout's column mode
@@ -63,6 +88,15 @@ struct Sell {
         case ::rsd::COL_OFF16: hipLaunchKernelGGL(kern<::rsd::COL_OFF16>, grid, block, 0, stream, __VA_ARGS__); break; \
         default: hipLaunchKernelGGL(kern<::rsd::COL_ABS>, grid, block, 0, stream, __VA_ARGS__); break;                 \
         }                                                                                           \
+    } while (0)
+
+// the same for kernels on the FP32 copy, which may also use COL_HALF
+#define RSD_LAUNCH_COLS32(mode, kern, grid, block, stream, ...)                                      \
+    do {                                                                                            \
+        if ((mode) == ::rsd::COL_HALF)                                                              \
+            hipLaunchKernelGGL(kern<::rsd::COL_HALF>, grid, block, 0, stream, __VA_ARGS__);         \
+        else                                                                                        \
+            RSD_LAUNCH_COLS(mode, kern, grid, block, stream, __VA_ARGS__);                          \
     } while (0)
 
 }  // namespace rsd
@@ -113,7 +147,8 @@ struct rsd_matrix {
     rsd_problem* pb = nullptr;
     std::unique_ptr<rsd_problem> own_pb;  // pattern-only problem of a matrix made from a CSR
     rsd::DBuf<double> val;    // [slots] SELL order
-    rsd::DBuf<float> val32;   // [slots] FP32 copy for the mixed-precision path
+    rsd::DBuf<float> val32;   // [sell.slots32()] FP32 copy for the mixed-precision path
+    rsd::DBuf<int32_t> asym;  // [1] set by the FP32 conversion when the half layout's symmetry check fails
 };
 
 namespace rsd {
@@ -121,4 +156,7 @@ namespace rsd {
 void update_gval(rsd_problem* pb, double time);
 // sliced ELL layout of pb->h_rp / h_ci, uploaded
 void build_sell(rsd_problem* pb, const std::vector<int32_t>& rowlen);
+// the symmetric half layout of the FP32 copy, tried once per problem (host
+// work and uploads: call outside stream capture)
+void build_half(rsd_problem* pb);
 }

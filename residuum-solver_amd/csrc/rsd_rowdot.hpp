@@ -12,7 +12,16 @@ namespace rsd {
 //   COL_PAT    r + poff[pid[r] * stride + j]: the row's offset list is one
 //              of a few hundred patterns shared by many rows (the stencils
 //              of a structured mesh), so no index is streamed per slot.
-enum ColMode : int { COL_ABS = 0, COL_OFF16 = 1, COL_PAT = 2 };
+// and, for FP32 values of a structurally symmetric matrix only:
+//   COL_HALF   the values are stored once per symmetric pair, the upper one
+//              (column >= row), in an ELL layout: the m-th upper value of
+//              row c at m * n_pad + c.  Row r's entry j has column
+//              r + poff[q * stride + j] (q = pid[r], a pattern of the half
+//              layout) and its value at r + pval[q * stride + j]: for a
+//              column c >= r that is the row's own m-th upper value
+//              (pval = m n_pad), for c < r the value row c keeps for
+//              column r (pval = m' n_pad + c - r).
+enum ColMode : int { COL_ABS = 0, COL_OFF16 = 1, COL_PAT = 2, COL_HALF = 3 };
 
 struct Cols {
     const int64_t* slice_ptr = nullptr;
@@ -22,6 +31,7 @@ struct Cols {
     const uint8_t* pid = nullptr;    // [rows]
     const int32_t* poff = nullptr;   // [n_patterns * stride]
     const int32_t* plen = nullptr;   // [n_patterns]
+    const int32_t* pval = nullptr;   // [n_patterns * stride] value index - row (COL_HALF)
     int32_t stride = 0;
 };
 
@@ -119,10 +129,46 @@ __device__ __forceinline__ double row_dot_f64(int64_t r, const Cols& C, const do
     }
 }
 
+// COL_HALF row product, CSR order, FP32 fused multiply-adds.  Indices are
+// 32-bit (the layout is built only below 2^31 slots): one add per entry for
+// the column and one for the value.  The values of the lower entries are
+// read a second time by this row after the row above read them for itself:
+// plain (temporal) loads keep them in L2 in between.
+__device__ __forceinline__ float row_dot_half(int64_t r, const Cols& C, const float* __restrict__ val,
+                                              const float* __restrict__ x) {
+    constexpr int U = kRowUnroll;
+    const uint32_t ru = (uint32_t)r;
+    auto run = [&](const int32_t* po, const int32_t* pv, int len) {
+        float sum = 0.0f;
+        int j = 0;
+        for (; j + U <= len; j += U) {
+            uint32_t c[U];
+            float v[U];
+#pragma unroll
+            for (int k = 0; k < U; ++k) {
+                c[k] = ru + (uint32_t)po[j + k];
+                v[k] = val[ru + (uint32_t)pv[j + k]];
+            }
+            float xv[U];
+#pragma unroll
+            for (int k = 0; k < U; ++k) xv[k] = x[c[k]];
+#pragma unroll
+            for (int k = 0; k < U; ++k) sum = __builtin_fmaf(v[k], xv[k], sum);
+        }
+        for (; j < len; ++j) sum = __builtin_fmaf(val[ru + (uint32_t)pv[j]], x[ru + (uint32_t)po[j]], sum);
+        return sum;
+    };
+    const int p = C.pid[r];
+    const int p0 = __builtin_amdgcn_readfirstlane(p);
+    if (__all(p == p0)) return run(C.poff + p0 * C.stride, C.pval + p0 * C.stride, C.plen[p0]);
+    return run(C.poff + p * C.stride, C.pval + p * C.stride, C.plen[p]);
+}
+
 // FP32 values and vectors, FP32 accumulation (fused multiply-adds)
 template<int MODE>
 __device__ __forceinline__ float row_dot_f32(int64_t r, const Cols& C, const float* __restrict__ val,
                                              const float* __restrict__ x) {
+    if constexpr (MODE == COL_HALF) return row_dot_half(r, C, val, x);
     constexpr int U = kRowUnroll;
     const int64_t base = C.slice_ptr[r >> 6] + (r & 63);
     auto run = [&](auto colf, int len) {

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <thread>
+#include <atomic>
 #include <unordered_map>
 #include <vector>
 
@@ -341,6 +342,123 @@ void rsd::build_sell(rsd_problem* pb, const std::vector<int32_t>& rowlen) {
     S.col.upload(hcol.data(), S.slots, st);
     if (fits16) S.off16.upload(hoff.data(), S.slots, st);
     S.diag.upload(hdiag.data(), n, st);
+}
+
+// Symmetric half layout of the FP32 copy.  Row r's key is its list of
+// (column - r, m) pairs in CSR order, m being the index of the pair's value
+// in the upper list (columns >= row) of row min(r, c); rows with equal keys
+// share one table entry.  Needs the shared offset patterns (rows of at most
+// 64 entries) and a structurally symmetric pattern; given up past 255 keys.
+void rsd::build_half(rsd_problem* pb) {
+    Sell& S = pb->sell;
+    if (S.half_tried) return;
+    S.half_tried = true;
+    if (!S.has_pat) return;
+    const int64_t n = pb->n_free;
+    const int32_t stride = S.pat_stride;
+    const int32_t* rp = pb->h_rp.data();
+    const int32_t* ci = pb->h_ci.data();
+    // upper index of column r in row c (c < r), -1 when row c lacks it
+    auto mirror = [&](int32_t c, int32_t r) -> int32_t {
+        const int32_t* beg = ci + rp[c];
+        const int32_t* end = ci + rp[c + 1];
+        const int32_t* up = std::lower_bound(beg, end, c);
+        const int32_t* it = std::lower_bound(up, end, r);
+        return (it != end && *it == r) ? (int32_t)(it - up) : -1;
+    };
+    auto key_of = [&](int64_t r, std::vector<int32_t>& off, std::vector<int32_t>& mir) {
+        off.clear();
+        mir.clear();
+        int32_t m_up = 0;
+        for (int32_t k = rp[r]; k < rp[r + 1]; ++k) {
+            const int32_t c = ci[k];
+            off.push_back(c - (int32_t)r);
+            mir.push_back(c < r ? mirror(c, (int32_t)r) : m_up++);
+        }
+    };
+    auto hash_of = [](const std::vector<int32_t>& off, const std::vector<int32_t>& mir) {
+        uint64_t h = 1469598103934665603ull ^ (uint64_t)off.size();
+        for (size_t k = 0; k < off.size(); ++k) {
+            h = (h ^ (uint64_t)(uint32_t)off[k]) * 1099511628211ull;
+            h = (h ^ (uint64_t)(uint32_t)mir[k]) * 1099511628211ull;
+        }
+        return h;
+    };
+    // per-row hashes and upper counts in parallel; a missing mirror fails
+    std::vector<uint64_t> hs(n);
+    std::vector<int32_t> nup(n);
+    std::atomic<bool> sym{true};
+    parallel_for(n, [&](int64_t lo, int64_t hi) {
+        std::vector<int32_t> off, mir;
+        for (int64_t r = lo; r < hi && sym.load(std::memory_order_relaxed); ++r) {
+            key_of(r, off, mir);
+            int32_t u = 0;
+            for (size_t k = 0; k < off.size(); ++k) {
+                if (mir[k] < 0) sym = false;
+                u += off[k] >= 0;
+            }
+            nup[r] = u;
+            hs[r] = hash_of(off, mir);
+        }
+    });
+    if (!sym) return;
+    // ids in first-occurrence order
+    std::unordered_map<uint64_t, int> by_hash;
+    std::vector<uint8_t> id(n);
+    std::vector<int64_t> first;
+    for (int64_t r = 0; r < n; ++r) {
+        auto it = by_hash.find(hs[r]);
+        if (it != by_hash.end()) { id[r] = (uint8_t)it->second; continue; }
+        if (first.size() >= 255) return;
+        by_hash.emplace(hs[r], (int)first.size());
+        id[r] = (uint8_t)first.size();
+        first.push_back(r);
+    }
+    const int64_t np = (int64_t)first.size();
+    if (np * stride > 16384) return;
+    std::vector<int32_t> toff(np * stride, 0), tmir(np * stride, 0), tlen(np), tlow(np);
+    std::vector<int32_t> off, mir;
+    for (int64_t q = 0; q < np; ++q) {
+        key_of(first[q], off, mir);
+        tlen[q] = (int32_t)off.size();
+        tlow[q] = (int32_t)std::count_if(off.begin(), off.end(), [](int32_t o) { return o < 0; });
+        std::copy(off.begin(), off.end(), toff.begin() + q * stride);
+        std::copy(mir.begin(), mir.end(), tmir.begin() + q * stride);
+    }
+    // every row against its table entry (a hash collision gives up)
+    std::atomic<bool> same{true};
+    parallel_for(n, [&](int64_t lo, int64_t hi) {
+        std::vector<int32_t> o2, m2;
+        for (int64_t r = lo; r < hi && same.load(std::memory_order_relaxed); ++r) {
+            key_of(r, o2, m2);
+            const int64_t q = id[r];
+            if ((int32_t)o2.size() != tlen[q] || !std::equal(o2.begin(), o2.end(), toff.begin() + q * stride) ||
+                !std::equal(m2.begin(), m2.end(), tmir.begin() + q * stride))
+                same = false;
+        }
+    });
+    if (!same) return;
+    // ELL planes of n_pad values; every index below 2^31
+    const int32_t width = n ? *std::max_element(nup.begin(), nup.end()) : 0;
+    const int64_t pad = (n + kSlice - 1) / kSlice * kSlice;
+    if (width == 0 || pad * (int64_t)width >= (int64_t)INT32_MAX) return;
+    std::vector<int32_t> tval(np * stride, 0);
+    for (int64_t q = 0; q < np; ++q)
+        for (int32_t j = 0; j < tlen[q]; ++j) {
+            const int32_t o = toff[q * stride + j];
+            tval[q * stride + j] = (int32_t)(tmir[q * stride + j] * pad + std::min(o, 0));
+        }
+    hipStream_t st = pb->ctx->stream;
+    S.half_id.upload(id.data(), n, st);
+    S.half_off.upload(toff.data(), (int64_t)toff.size(), st);
+    S.half_mir.upload(tmir.data(), (int64_t)tmir.size(), st);
+    S.half_val.upload(tval.data(), (int64_t)tval.size(), st);
+    S.half_len.upload(tlen.data(), np, st);
+    S.half_low.upload(tlow.data(), np, st);
+    S.n_half = (int32_t)np;
+    S.half_width = width;
+    S.half_pad = pad;
+    S.has_half = true;
 }
 
 extern "C" {
@@ -715,6 +833,19 @@ int rsd_problem_layout(rsd_problem* pb, int64_t* out) {
         out[1] = pb->sell.n_pat;
         out[2] = pb->sell.has_off16;
         out[3] = pb->sell.slots;
+    });
+}
+
+int rsd_problem_layout_f32(rsd_problem* pb, int64_t* out) {
+    return guard([&] {
+        RSD_CHECK(pb && out, "null argument");
+        build_half(pb);
+        const Sell& S = pb->sell;
+        out[0] = S.mode32();
+        out[1] = S.has_half ? S.n_half : S.n_pat;
+        out[2] = S.half_width;
+        out[3] = S.slots32();
+        RSD_HIP(hipStreamSynchronize(pb->ctx->stream));
     });
 }
 

@@ -235,7 +235,32 @@ class Problem:
         self.h = _vp()
         check(lib().rsd_problem_create_block(ctx.h, C.c_int32(dim), n3, e6, p3, C.c_int32(nq), C.c_int32(len(tags)),
                                              t, self._exprs, C.byref(self.h)))
-        ctx._adopt(self)
+        self._finish()
+
+    @classmethod
+    def from_mesh(cls, ctx, dim, p, nq, xyz, ien, rng, tags=(), bc_exprs=()):
+        """A problem on host mesh arrays in the reference's mesh/Data layout:
+        xyz [n_nodes*dim], ien [n_elem*npe] (int64), rng [n_elem*2*dim]."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.dim = dim
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        ien = np.ascontiguousarray(ien, np.int64)
+        rng = np.ascontiguousarray(rng, np.int32)
+        npe = int(np.prod([q + 1 for q in p]))
+        p3 = (C.c_int32 * 3)(*(list(p) + [1] * (3 - dim)))
+        t = (C.c_int32 * max(1, len(tags)))(*tags)
+        self._exprs = _cstrs(list(bc_exprs))
+        self.h = _vp()
+        check(lib().rsd_problem_create(ctx.h, C.c_int32(dim), p3, C.c_int32(nq), C.c_int64(xyz.size // dim),
+                                       C.c_int64(ien.size // npe), xyz.ctypes.data_as(_d), ien.ctypes.data_as(_i64),
+                                       rng.ctypes.data_as(_i32), C.c_int32(len(tags)), t, self._exprs,
+                                       C.byref(self.h)))
+        self._finish()
+        return self
+
+    def _finish(self):
+        self.ctx._adopt(self)
         s = (C.c_int64 * 8)()
         check(lib().rsd_problem_sizes(self.h, s))
         (self.n_nodes, self.n_elem, self.npe, self.n_free, self.nnz, self.n_colours, self.slots,
@@ -266,6 +291,14 @@ class Problem:
         columns, 1 int16 offsets, 2 shared offset patterns."""
         out = (C.c_int64 * 4)()
         check(lib().rsd_problem_layout(self.h, out))
+        return tuple(int(v) for v in out)
+
+    def layout_f32(self):
+        """Layout of the FP32 copy: (column mode, patterns, upper entries per
+        row, slots); mode 3 is the symmetric half storage, else the FP64
+        layout's mode."""
+        out = (C.c_int64 * 4)()
+        check(lib().rsd_problem_layout_f32(self.h, out))
         return tuple(int(v) for v in out)
 
     def dofs(self):

@@ -345,6 +345,107 @@ def test_mixed_cg_reaches_fp64_residual(ctx, case):
     assert np.max(np.abs(y32.numpy() - ref)) <= 1e-5 * np.max(np.abs(ref))
 
 
+def permuted_problem(ctx, seed):
+    """A 10x8x6 Q1 hex block mesh with randomly renumbered nodes: the rows
+    no longer share at most 255 offset patterns, so every layout falls back
+    to explicit columns."""
+    import residuum_hip as rh
+    xyz, ien, rng = O.block_mesh(3, (10, 8, 6), (0.0, 1.0, 0.0, 1.0, 0.0, 1.0), (1, 1, 1))
+    nn = xyz.size // 3
+    perm = np.random.default_rng(seed).permutation(nn)
+    xyz2 = np.empty((nn, 3))
+    xyz2[perm] = xyz.reshape(-1, 3)
+    pb = rh.Problem.from_mesh(ctx, 3, (1, 1, 1), 2, xyz2.ravel(), perm[ien], rng, [0, 1], ["1", "0"])
+    kap = np.exp(0.5 * O.normal_field(5, 0, pb.n_elem))
+    pb.set_conductivity(1.0, kappa=kap)
+    return pb
+
+
+def symmetric_upper(A):
+    """The operator of the FP32 half storage: every entry below the diagonal
+    replaced by its mirror above it."""
+    import scipy.sparse as sp
+    return (sp.triu(A) + sp.triu(A, 1).T).tocsr()
+
+
+@pytest.mark.parametrize("case", CASES + ["permuted"])
+def test_fp32_half_storage(ctx, case):
+    """The FP32 copy keeps each symmetric pair once when the rows share offset
+    patterns: a product with a unit vector returns one column of the
+    symmetrised operator, rounded to FP32, bit for bit (a single product per
+    row, so no FP32 sum rounds).  A renumbered mesh keeps the full FP32 copy
+    on explicit columns."""
+    import scipy.sparse as sp
+    forms = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 1.0 / 1e-3)]
+    if case == "permuted":
+        pb = permuted_problem(ctx, 4)
+    else:
+        pb, oc = build(ctx, case, kappa_seed=5)
+    K = pb.matrix()
+    pb.assemble_matrix(K, forms)
+    rp, ci = pb.pattern()
+    n = pb.n_free
+    A = sp.csr_matrix((K.values(), ci, rp), shape=(n, n))
+    if case != "permuted":
+        assert np.array_equal(K.values(), oc.assemble_matrix(forms))
+    mode32, npat, width, slots = pb.layout_f32()
+    say("fp32 layout", mode32, npat, width, slots, "fp64 layout", pb.layout())
+    if case == "permuted":
+        assert pb.layout()[0] != 2 and mode32 == pb.layout()[0] and width == 0 and slots == pb.slots
+        As = A
+    else:
+        upper = np.array([np.count_nonzero(ci[rp[i]:rp[i + 1]] >= i) for i in range(n)])
+        assert mode32 == 3 and width == upper.max()
+        assert slots == ((n + 63) // 64) * 64 * width
+        As = symmetric_upper(A)
+    As32 = As.astype(np.float32)
+    K.to_f32()
+    y32 = ctx.empty(n, np.float32)
+    for k in sorted({0, 1, n // 3, n // 2, n - 2, n - 1}):
+        e = np.zeros(n, np.float32)
+        e[k] = 1.0
+        K.spmv_f32(ctx.from_numpy(e), y32)
+        got = y32.numpy()
+        want = np.asarray(As32[:, k].todense()).ravel()
+        assert np.array_equal(got, want), (k, np.flatnonzero(got != want)[:8])
+    xv = np.random.default_rng(11).standard_normal(n).astype(np.float32)
+    K.spmv_f32(ctx.from_numpy(xv), y32)
+    ref = As32.astype(np.float64) @ xv.astype(np.float64)
+    assert np.max(np.abs(y32.numpy() - ref)) <= 1e-5 * np.max(np.abs(ref))
+    # the mixed solve on either layout reaches the FP64 residual
+    b = np.random.default_rng(12).standard_normal(n)
+    x = pb.vector(0.0)
+    rep, cycles = K.cg_mixed(ctx.from_numpy(b), x, tol=1e-8)
+    r = b - A @ x.numpy()
+    assert rep.converged == 1 and np.linalg.norm(r) / np.linalg.norm(b) < 1e-8
+
+
+def test_mixed_rejects_asymmetric_matrix(ctx):
+    """A matrix whose mirrored pairs differ beyond rounding cannot go through
+    the symmetric FP32 copy: the mixed solve fails with an argument error, and
+    the same matrix made symmetric again solves."""
+    import residuum_hip as rh
+    pb, oc = build(ctx, CASES[2], kappa_seed=9)
+    assert pb.layout_f32()[0] == 3
+    forms = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 1.0 / 1e-3)]
+    val = oc.assemble_matrix(forms)
+    K = pb.matrix()
+    i = oc.n_free // 2
+    lo = oc.rp[i]  # the first entry of row i, left of the diagonal
+    assert oc.ci[lo] < i
+    bad = val.copy()
+    bad[lo] *= 1.0 + 1e-6
+    K.set_values(bad)
+    b = ctx.from_numpy(np.random.default_rng(2).standard_normal(oc.n_free))
+    x = pb.vector(0.0)
+    with pytest.raises(rh.RsdError, match="not symmetric"):
+        K.cg_mixed(b, x, tol=1e-8)
+    K.set_values(val)
+    x = pb.vector(0.0)
+    rep, cycles = K.cg_mixed(b, x, tol=1e-8)
+    assert rep.converged == 1
+
+
 def test_be_run_mixed(ctx):
     case = (3, (10, 9, 8), (1, 1, 1), 2, [0, 1], ["1", "0"], None)
     pb, oc = build(ctx, case, kappa_seed=20261201)
