@@ -135,8 +135,13 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
     const Expr* src = cached_source(pb, src_text);
     const bool graphable = use_graph && !pb->bc_time_dependent && !(src && src->uses_t);
     CGWork w = cg_workspace(ctx, n, max_iters, false);
-    ctx->partials(2 * std::max<int64_t>(1, nblocks(n)));
+    // every reduction area the captured launches use, reserved before capture
+    ctx->partials(std::max(3 * std::max<int64_t>(1, nblocks(n)), mixed_partials(n)));
     const bool mixed = precision == RSD_PREC_MIXED;
+    bool sym_model = true;
+    if (pb->model.aniso)
+        for (int i = 0; i < pb->dim; ++i)
+            for (int j = 0; j < i; ++j) sym_model = sym_model && pb->model.tensor[i * pb->dim + j] == pb->model.tensor[j * pb->dim + i];
     MixedWork mw;
     if (mixed) {
         mw = mixed_workspace(ctx, n);
@@ -175,7 +180,10 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
             GraphChunk gc;
             gc.st = st;
             gc.g = &chunk_graph;
-            const MixedState ms = mixed_run(ctx, &K, b.F, U, tol, max_iters, precond, kMixedInnerFloor, false, mw,
+            // DIFFUSION + MASS/dt with an isotropic or symmetric conductivity is
+            // symmetric to rounding: the FP32 conversion skips the pair check
+            matrix_to_f32(ctx, &K, !sym_model);
+            const MixedState ms = mixed_run(ctx, &K, b.F, U, tol, max_iters, precond, kMixedInnerFloor, true, mw,
                                             graphable ? &gc : nullptr, mixed_hint);
             ev.rec(ev.c, st);
             ev.accumulate();

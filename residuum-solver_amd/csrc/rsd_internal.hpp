@@ -122,31 +122,39 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], int64_t lb, int6
     }
 }
 
-// Totals of the nb block sums of NV values; call from every thread of one
-// kFold-thread block, result valid in all threads.  A lane's loads are all
-// issued before its adds: the fold is latency bound.
+// Totals of NV arrays of block sums (array k: nbk[k] values at pk[k]); call
+// from every thread of one kFold-thread block, result valid in all threads.
+// Lane t adds values t, t + 1024, ... of each array from 0.0 in increasing
+// order; the loads of all arrays are issued before the adds, so the fold
+// costs one memory round trip per 16 Ki values.
 template<int NV>
-__device__ __forceinline__ void fold_partials(int64_t nb, const double* __restrict__ partials, double (&out)[NV]) {
+__device__ __forceinline__ void fold_partials(const double* const (&pk)[NV], const int64_t (&nbk)[NV],
+                                              double (&out)[NV]) {
     __shared__ double sm[NV * 16];
     __shared__ double res[NV];
     constexpr int U = 16;
     const int t = threadIdx.x;
     double acc[NV];
+    int64_t nmax = 0;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         acc[k] = 0.0;
-        const double* pk = partials + k * nb;
-        for (int64_t b0 = t; b0 < nb; b0 += (int64_t)U * kFold) {
-            double v[U];
+        nmax = nbk[k] > nmax ? nbk[k] : nmax;
+    }
+    for (int64_t b0 = t; b0 < nmax; b0 += (int64_t)U * kFold) {
+        double v[NV][U];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int64_t b = b0 + (int64_t)u * kFold;
-                v[u] = b < nb ? pk[b] : 0.0;
+                v[k][u] = b < nbk[k] ? pk[k][b] : 0.0;
             }
 #pragma unroll
+        for (int k = 0; k < NV; ++k)
+#pragma unroll
             for (int u = 0; u < U; ++u)
-                if (b0 + (int64_t)u * kFold < nb) acc[k] += v[u];
-        }
+                if (b0 + (int64_t)u * kFold < nbk[k]) acc[k] += v[k][u];
     }
     const int lane = t & 63, wave = t >> 6;
 #pragma unroll
@@ -167,6 +175,19 @@ __device__ __forceinline__ void fold_partials(int64_t nb, const double* __restri
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NV; ++k) out[k] = res[k];
+}
+
+// NV arrays of nb block sums each, array k at partials + k nb
+template<int NV>
+__device__ __forceinline__ void fold_partials(int64_t nb, const double* __restrict__ partials, double (&out)[NV]) {
+    const double* pk[NV];
+    int64_t nbk[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        pk[k] = partials + k * nb;
+        nbk[k] = nb;
+    }
+    fold_partials<NV>(pk, nbk, out);
 }
 
 // ------------------------------------------------------------------------

@@ -10,7 +10,8 @@
 //                  per iteration, accumulated in FP64):
 //                    u = M r, w = A u, delta = w.u, gamma = r.u,
 //                    beta = gamma / gamma', alpha = gamma / (delta - beta gamma / alpha'),
-//                    p = u + beta p, s = w + beta s, d += alpha p, r -= alpha s;
+//                    p = u + beta p, s = w + beta s, d += alpha p, r -= alpha s
+//                  (r kept implicit as u / invd);
 //   update (FP64): x += ||r|| d.
 // The FP32 copy of a structurally symmetric matrix whose rows share offset
 // patterns keeps each symmetric pair's value once (the upper one; the
@@ -41,10 +42,11 @@ __global__ void k_set_flag(int32_t* f, int32_t v) {
 
 // The half layout's FP32 upper values from the FP64 SELL values (one thread
 // per row): the m-th value right of the diagonal part of row r goes to
-// m * pad + r.  The row's values left of the diagonal are compared with the
-// FP64 values stored for the mirrored entries; *asym is set when one differs
-// by more than 1e-10 of the row's largest magnitude (an assembled heat
-// matrix is symmetric to rounding).
+// m * pad + r.  With CHECK, the row's values left of the diagonal are also
+// compared with the FP64 values stored for the mirrored entries, and *asym
+// is set when one differs by more than 1e-10 of the row's largest magnitude
+// (an assembled heat matrix is symmetric to rounding).
+template<bool CHECK>
 __global__ __launch_bounds__(kBlock) void k_to_f32_half(int64_t n, Cols H, const int32_t* __restrict__ mir,
                                                         const int32_t* __restrict__ low, int64_t pad,
                                                         const double* __restrict__ val, float* __restrict__ vh,
@@ -53,23 +55,22 @@ __global__ __launch_bounds__(kBlock) void k_to_f32_half(int64_t n, Cols H, const
     if (r >= n) return;
     const int q = H.pid[r];
     const int len = H.plen[q], lo = low[q];
-    const int32_t* po = H.poff + q * H.stride;
-    const int32_t* pm = mir + q * H.stride;
     const int64_t base = H.slice_ptr[r >> 6] + (r & 63);
-    double amax = 0.0, dmax = 0.0;
-    for (int j = 0; j < len; ++j) {
-        const double a = val[base + (int64_t)j * kSlice];
-        amax = fmax(amax, fabs(a));
-        if (j >= lo) {
-            vh[(int64_t)(j - lo) * pad + r] = (float)a;
-        } else {
+    for (int j = lo; j < len; ++j) vh[(int64_t)(j - lo) * pad + r] = (float)val[base + (int64_t)j * kSlice];
+    if constexpr (CHECK) {
+        const int32_t* po = H.poff + q * H.stride;
+        const int32_t* pm = mir + q * H.stride;
+        double amax = 0.0, dmax = 0.0;
+        for (int j = 0; j < len; ++j) amax = fmax(amax, fabs(val[base + (int64_t)j * kSlice]));
+        for (int j = 0; j < lo; ++j) {
+            const double a = val[base + (int64_t)j * kSlice];
             const int64_t c = r + po[j];
             const int lc = low[H.pid[c]];
             const double b = val[H.slice_ptr[c >> 6] + (int64_t)(lc + pm[j]) * kSlice + (c & 63)];
             dmax = fmax(dmax, fabs(a - b));
         }
+        if (dmax > 1e-10 * amax) *asym = 1;
     }
-    if (dmax > 1e-10 * amax) *asym = 1;
 }
 
 template<int MODE>
@@ -107,31 +108,34 @@ __global__ __launch_bounds__(kFold) void k_mx_residual_fold(int64_t nb, const do
     }
 }
 
-// inner set-up: r = r64 / ||r||, u = M r, p = s = d = 0; block sums of r.u, r.r
+// inner set-up: r = r64 / ||r||, u = M r, p = s = d = 0; block sums of r.u,
+// r.r.  Only u is stored: the residual of row i is u_i / invd_i.
 __global__ __launch_bounds__(kBlock) void k_mx_init(int64_t n, int64_t nb, const double* __restrict__ r64,
-                                                    float* __restrict__ r, float* __restrict__ u,
-                                                    float* __restrict__ p, float* __restrict__ s,
-                                                    float* __restrict__ d, const float* __restrict__ invd,
-                                                    const MixedState* st, double* P12) {
+                                                    float* __restrict__ u, float* __restrict__ p,
+                                                    float* __restrict__ s, float* __restrict__ d,
+                                                    const float* __restrict__ invd, const MixedState* st,
+                                                    double* P12) {
     const int64_t lb = xcd_block(blockIdx.x, nb);
-    const int64_t i = lb * kBlock + threadIdx.x;
+    const double sc = st->res > 0.0 ? 1.0 / st->res : 0.0;
     double v[2] = {0.0, 0.0};
-    if (i < n) {
-        const double sc = st->res > 0.0 ? 1.0 / st->res : 0.0;
-        const float ri = (float)(r64[i] * sc);
-        const float ui = invd ? ri * invd[i] : ri;
-        r[i] = ri;
-        u[i] = ui;
-        p[i] = 0.0f;
-        s[i] = 0.0f;
-        d[i] = 0.0f;
-        v[0] = (double)ri * (double)ui;
-        v[1] = (double)ri * (double)ri;
+#pragma unroll
+    for (int t = 0; t < kMxTiles; ++t) {
+        const int64_t i = (lb * kMxTiles + t) * kBlock + threadIdx.x;
+        if (i < n) {
+            const float ri = (float)(r64[i] * sc);
+            const float ui = invd ? ri * invd[i] : ri;
+            u[i] = ui;
+            p[i] = 0.0f;
+            s[i] = 0.0f;
+            d[i] = 0.0f;
+            v[0] += (double)ri * (double)ui;
+            v[1] += (double)ri * (double)ri;
+        }
     }
     block_partials<2>(v, lb, nb, P12);
 }
 
-// w = A u, block sums of w.u
+// w = A u, block sums of w.u (one row per thread)
 template<int MODE>
 __global__ __launch_bounds__(kBlock) void k_mx_spmv_wu(int64_t n, int64_t nb, Cols C, const float* __restrict__ val,
                                                        const float* __restrict__ u, float* __restrict__ w,
@@ -148,15 +152,18 @@ __global__ __launch_bounds__(kBlock) void k_mx_spmv_wu(int64_t n, int64_t nb, Co
     block_partials<1>(v, lb, nb, P0);
 }
 
-// fold delta = w.u (P0), gamma = r.u and ||r||^2 (P12 = P0 + nb):
-// convergence, alpha, beta
-__global__ __launch_bounds__(kFold) void k_mx_scalars(int64_t nb, const double* __restrict__ P, MixedState* st) {
+// fold delta = w.u (nb block sums at P0), gamma = r.u and ||r||^2 (nbu
+// block sums each, at P12 and P12 + nbu): convergence, alpha, beta
+__global__ __launch_bounds__(kFold) void k_mx_scalars(int64_t nb, const double* __restrict__ P0, int64_t nbu,
+                                                      const double* __restrict__ P12, MixedState* st) {
     if (st->stop) return;
     const int64_t k = st->it;
     const double gam_prev = st->gam, alp_prev = st->alp, in_res0 = st->in_res0, in_tol = st->in_tol;
     const int64_t in_max = st->in_max;
     double o[3];
-    fold_partials<3>(nb, P, o);
+    const double* pk[3] = {P0, P12, P12 + nbu};
+    const int64_t nbk[3] = {nb, nbu, nbu};
+    fold_partials<3>(pk, nbk, o);
     if (threadIdx.x != 0) return;
     const double del = o[0], gam = o[1], res = sqrt(o[2]);
     const double r0 = k == 0 ? res : in_res0;
@@ -181,30 +188,54 @@ __global__ __launch_bounds__(kFold) void k_mx_scalars(int64_t nb, const double* 
     st->it = k + 1;
 }
 
-// p = u + beta p, s = w + beta s, d += alpha p, r -= alpha s, u = M r;
-// block sums of the new r.u, r.r
+// p = u + beta p, s = w + beta s, d += alpha p, and the preconditioned
+// residual u = M r of r -= alpha s as u -= alpha (M s), the residual itself
+// being u / invd (Jacobi) or u; block sums of the new r.u, r.r.  Keeping r
+// implicit saves one read and one write of a vector per iteration.  A
+// thread's loads for all its tiles are issued before the arithmetic.
 __global__ __launch_bounds__(kBlock) void k_mx_update(int64_t n, int64_t nb, float* __restrict__ p,
                                                       float* __restrict__ s, float* __restrict__ d,
-                                                      float* __restrict__ r, float* __restrict__ u,
-                                                      const float* __restrict__ w, const float* __restrict__ invd,
-                                                      const MixedState* st, double* P12) {
+                                                      float* __restrict__ u, const float* __restrict__ w,
+                                                      const float* __restrict__ invd, const MixedState* st,
+                                                      double* P12) {
     if (st->stop) return;
     const int64_t lb = xcd_block(blockIdx.x, nb);
-    const int64_t i = lb * kBlock + threadIdx.x;
     const float a = (float)st->alpha, bt = (float)st->beta;
+    constexpr int T = kMxTiles;
+    int64_t idx[T];
+    float pv[T], sv[T], dv[T], uv[T], wv[T], mv[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        idx[t] = (lb * T + t) * kBlock + threadIdx.x;
+        const bool in = idx[t] < n;
+        pv[t] = in ? p[idx[t]] : 0.0f;
+        sv[t] = in ? s[idx[t]] : 0.0f;
+        dv[t] = in ? d[idx[t]] : 0.0f;
+        uv[t] = in ? u[idx[t]] : 0.0f;
+        wv[t] = in ? w[idx[t]] : 0.0f;
+        mv[t] = (in && invd) ? invd[idx[t]] : 1.0f;
+    }
     double v[2] = {0.0, 0.0};
-    if (i < n) {
-        const float pi = __builtin_fmaf(bt, p[i], u[i]);
-        const float si = __builtin_fmaf(bt, s[i], w[i]);
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        if (idx[t] >= n) continue;
+        const int64_t i = idx[t];
+        const float pi = __builtin_fmaf(bt, pv[t], uv[t]);
+        const float si = __builtin_fmaf(bt, sv[t], wv[t]);
         p[i] = pi;
         s[i] = si;
-        d[i] = __builtin_fmaf(a, pi, d[i]);
-        const float ri = __builtin_fmaf(-a, si, r[i]);
-        const float ui = invd ? ri * invd[i] : ri;
-        r[i] = ri;
+        d[i] = __builtin_fmaf(a, pi, dv[t]);
+        float ui, ri;
+        if (invd) {
+            ui = __builtin_fmaf(-a, mv[t] * si, uv[t]);
+            ri = ui / mv[t];
+        } else {
+            ui = __builtin_fmaf(-a, si, uv[t]);
+            ri = ui;
+        }
         u[i] = ui;
-        v[0] = (double)ri * (double)ui;
-        v[1] = (double)ri * (double)ri;
+        v[0] += (double)ri * (double)ui;
+        v[1] += (double)ri * (double)ri;
     }
     block_partials<2>(v, lb, nb, P12);
 }
@@ -224,17 +255,22 @@ __global__ void k_inv_diag_f32(int64_t n, const int32_t* __restrict__ diag, cons
 }
 
 // ------------------------------------------------------------------------
-void matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A) {
+void matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A, bool check_symmetry) {
     build_half(A->pb);
     const Sell& S = A->pb->sell;
     if (A->val32.n != S.slots32()) A->val32.alloc(S.slots32());
     if (S.has_half) {
         if (!A->asym.n) A->asym.alloc(1);
         hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, ctx->stream, A->asym.p, 0);
-        if (S.n_rows)
-            hipLaunchKernelGGL(k_to_f32_half, dim3((unsigned)nblocks(S.n_rows)), dim3(kBlock), 0, ctx->stream,
-                               S.n_rows, S.cols32(), S.half_mir.p, S.half_low.p, S.half_pad, A->val.p, A->val32.p,
-                               A->asym.p);
+        if (S.n_rows) {
+            const dim3 grid((unsigned)nblocks(S.n_rows));
+            if (check_symmetry)
+                hipLaunchKernelGGL(k_to_f32_half<true>, grid, dim3(kBlock), 0, ctx->stream, S.n_rows, S.cols32(),
+                                   S.half_mir.p, S.half_low.p, S.half_pad, A->val.p, A->val32.p, A->asym.p);
+            else
+                hipLaunchKernelGGL(k_to_f32_half<false>, grid, dim3(kBlock), 0, ctx->stream, S.n_rows, S.cols32(),
+                                   S.half_mir.p, S.half_low.p, S.half_pad, A->val.p, A->val32.p, A->asym.p);
+        }
     } else if (A->val.n) {
         hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nblocks(A->val.n)), dim3(kBlock), 0, ctx->stream, A->val.n,
                            A->val.p, A->val32.p);
@@ -253,7 +289,6 @@ MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n) {
     MixedWork w;
     const size_t f = sizeof(float) * (size_t)std::max<int64_t>(n, 1), dd = sizeof(double) * (size_t)std::max<int64_t>(n, 1);
     w.r64 = (double*)ctx->ws("mx.r64", dd);
-    w.r = (float*)ctx->ws("mx.r", f);
     w.u = (float*)ctx->ws("mx.u", f);
     w.w = (float*)ctx->ws("mx.w", f);
     w.p = (float*)ctx->ws("mx.p", f);
@@ -264,16 +299,23 @@ MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n) {
     return w;
 }
 
+int64_t mixed_blocks(int64_t n) { return std::max<int64_t>(1, (n + kMxTiles * kBlock - 1) / (kMxTiles * kBlock)); }
+
+int64_t mixed_partials(int64_t n) {
+    const int64_t nb = std::max<int64_t>(1, nblocks(n));
+    return nb + 2 * mixed_blocks(n);
+}
+
 void mixed_launch_inner_iter(rsd_ctx* ctx, const rsd_matrix* A, int32_t precond, const MixedWork& w) {
     const Sell& S = A->pb->sell;
-    const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n));
-    double* P = ctx->partials(3 * nb);
+    const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n)), nbu = mixed_blocks(n);
+    double* P = ctx->partials(mixed_partials(n));
     const float* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     hipStream_t s = ctx->stream;
     RSD_LAUNCH_COLS32(S.mode32(), k_mx_spmv_wu, dim3((unsigned)nb), dim3(kBlock), s, n, nb, S.cols32(), A->val32.p,
                       w.u, w.w, w.st, P);
-    hipLaunchKernelGGL(k_mx_scalars, dim3(1), dim3(kFold), 0, s, nb, P, w.st);
-    hipLaunchKernelGGL(k_mx_update, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.p, w.s, w.d, w.r, w.u, w.w, invd,
+    hipLaunchKernelGGL(k_mx_scalars, dim3(1), dim3(kFold), 0, s, nb, P, nbu, P + nb, w.st);
+    hipLaunchKernelGGL(k_mx_update, dim3((unsigned)nbu), dim3(kBlock), 0, s, n, nbu, w.p, w.s, w.d, w.u, w.w, invd,
                        w.st, P + nb);
 }
 
@@ -285,10 +327,10 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
                      int32_t precond, double inner_floor, bool fresh_f32, const MixedWork& w, ChunkRunner* runner,
                      int64_t* hint) {
     const Sell& S = A->pb->sell;
-    const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n));
+    const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n)), nbt = mixed_blocks(n);
     hipStream_t st = ctx->stream;
-    if (!fresh_f32) matrix_to_f32(ctx, A);
-    double* part = ctx->partials(3 * nb);
+    if (!fresh_f32) matrix_to_f32(ctx, A, true);
+    double* part = ctx->partials(mixed_partials(n));
     const float* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     if (precond == RSD_PC_JACOBI)
         hipLaunchKernelGGL(k_inv_diag_f32, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, st, n, S.diag.p, A->val.p, w.invd);
@@ -325,7 +367,7 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
             h->it = 0;
             h->stop = 0;
             RSD_HIP(hipMemcpyAsync(w.st, h, sizeof(MixedState), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_mx_init, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w.r64, w.r, w.u, w.p, w.s,
+            hipLaunchKernelGGL(k_mx_init, dim3((unsigned)nbt), dim3(kBlock), 0, st, n, nbt, w.r64, w.u, w.p, w.s,
                                w.d, invd, w.st, part + nb);
             const int64_t before = h->total_inner;
             // queue what the same cycle of the previous solve needed (minus a
@@ -363,7 +405,7 @@ extern "C" {
 int rsd_matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A) {
     return guard([&] {
         RSD_CHECK(ctx && A, "null argument");
-        matrix_to_f32(ctx, A);
+        matrix_to_f32(ctx, A, true);
         RSD_HIP(hipGetLastError());
     });
 }

@@ -18,7 +18,7 @@ struct MixedState {
 
 struct MixedWork {
     double* r64 = nullptr;
-    float *r = nullptr, *u = nullptr, *w = nullptr, *p = nullptr, *s = nullptr, *d = nullptr, *invd = nullptr;
+    float *u = nullptr, *w = nullptr, *p = nullptr, *s = nullptr, *d = nullptr, *invd = nullptr;
     MixedState* st = nullptr;
 };
 
@@ -28,9 +28,21 @@ struct ChunkRunner {
     virtual void run() = 0;
 };
 
-void matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A);
+// the FP32 copy of A's values; check_symmetry: compare every mirrored pair of
+// the half layout (callers whose matrix is symmetric by construction skip it)
+void matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A, bool check_symmetry);
 void launch_spmv_f32(rsd_ctx* ctx, const rsd_matrix* A, const float* x, float* y);
 MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n);
+// row tiles of 256 per block in the inner vector kernels (set-up and
+// update; the SpMV keeps one row per thread): a quarter of the block sums
+// for the one-block fold, and more loads in flight per thread
+#ifndef RSD_MX_TILES
+#define RSD_MX_TILES 4
+#endif
+constexpr int kMxTiles = RSD_MX_TILES;
+int64_t mixed_blocks(int64_t n);
+// reduction partials (doubles) a mixed solve on n rows uses
+int64_t mixed_partials(int64_t n);
 void mixed_launch_inner_iter(rsd_ctx* ctx, const rsd_matrix* A, int32_t precond, const MixedWork& w);
 constexpr int kMixedMaxCycles = 40;
 constexpr double kMixedDefaultInnerFloor = 1e-4;
