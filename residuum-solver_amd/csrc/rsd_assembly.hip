@@ -52,7 +52,9 @@ struct QP {
     double T, Trate, gradT[D], K[D * D], dKdT[D * D], rho, cp, dcpdT;
 };
 
-template<int D, int P, int NQ>
+// GRAD = false leaves out J^-1 and dN/dx (forms without gradients); the
+// values computed are the same either way
+template<int D, int P, int NQ, bool GRAD = true>
 __device__ __forceinline__ void qp_eval(int q, const double (&X)[Elem<D, P>::NPE][D], QP<D, Elem<D, P>::NPE>& Q,
                                         bool need_x) {
     constexpr int NPE = Elem<D, P>::NPE;
@@ -112,12 +114,14 @@ __device__ __forceinline__ void qp_eval(int q, const double (&X)[Elem<D, P>::NPE
     double inv[D * D];
     if constexpr (D == 2) {
         Q.measure = sqrt((g[0] * g[3]) - (g[1] * g[2]));
+        if constexpr (!GRAD) return;
         const double di = 1.0 / ((J[0] * J[3]) - (J[1] * J[2]));
         inv[0] = di * J[3]; inv[1] = -di * J[1]; inv[2] = -di * J[2]; inv[3] = di * J[0];
     } else {
         const double dg = (g[0] * ((g[4] * g[8]) - (g[5] * g[7]))) - (g[1] * ((g[3] * g[8]) - (g[5] * g[6]))) +
                           (g[2] * ((g[3] * g[7]) - (g[4] * g[6])));
         Q.measure = sqrt(dg);
+        if constexpr (!GRAD) return;
         const double* A = J;
         const double dj = (A[0] * ((A[4] * A[8]) - (A[5] * A[7]))) - (A[1] * ((A[3] * A[8]) - (A[5] * A[6]))) +
                           (A[2] * ((A[3] * A[7]) - (A[4] * A[6])));
@@ -198,13 +202,14 @@ __device__ __forceinline__ void gather(const AsmArgs& A, const int32_t (&nodes)[
 // Each entry's accumulation chain is the one a single thread per element
 // would run, so results are bitwise those of the serial loops.
 // ------------------------------------------------------------------------
-template<int D, int P, int NQ, bool STATE>
+template<int D, int P, int NQ, bool STATE, bool GRAD = true>
 struct ECfg {
+    static_assert(GRAD || !STATE, "state-dependent models need the gradients");
     static constexpr int NPE = Elem<D, P>::NPE;
     static constexpr int NQP = D == 2 ? NQ * NQ : NQ * NQ * NQ;
-    // per-point record: dNdx[NPE*D], measure, source value, then (state)
-    // k, dk, Trate, gradT[D]
-    static constexpr int R_DN = 0, R_MEAS = NPE * D, R_SRC = R_MEAS + 1;
+    // per-point record: dNdx[NPE*D] (GRAD), measure, source value, then
+    // (state) k, dk, Trate, gradT[D]
+    static constexpr int R_DN = 0, R_MEAS = GRAD ? NPE * D : 0, R_SRC = R_MEAS + 1;
     static constexpr int R_K = R_SRC + 1, R_DK = R_K + 1, R_TR = R_DK + 1, R_GT = R_TR + 1;
     static constexpr int REC = STATE ? R_GT + D : R_SRC + 1;
     static constexpr int ELEM_DOUBLES = NQP * REC;
@@ -250,18 +255,20 @@ __device__ __forceinline__ double point_coords(int q, double (&xi)[3]) {
 
 // phase 1: geometry (and state) of the points q = a, a + NPE, ... of one element
 //   load: 0 none, 1 source expression at x_q, 2 nodal source interpolated
-template<int D, int P, int NQ, bool STATE>
+template<int D, int P, int NQ, bool STATE, bool GRAD = true>
 __device__ __forceinline__ void phase1(const AsmArgs& A, const Model& m, const Expr* src, int load, int64_t e, int a,
                                        const double (&X)[Elem<D, P>::NPE][D], const double* Uv, const double* Ur,
                                        const double* Un, double kap, bool has_kap, double* rec) {
-    using C = ECfg<D, P, NQ, STATE>;
+    using C = ECfg<D, P, NQ, STATE, GRAD>;
     constexpr int NPE = C::NPE;
     for (int q = a; q < C::NQP; q += NPE) {
         QP<D, NPE> Q;
-        qp_eval<D, P, NQ>(q, X, Q, load == 1);
+        qp_eval<D, P, NQ, GRAD>(q, X, Q, load == 1);
         double* r = rec + q * C::REC;
+        if constexpr (GRAD) {
 #pragma unroll
-        for (int i = 0; i < NPE * D; ++i) r[C::R_DN + i] = Q.dNdx[i];
+            for (int i = 0; i < NPE * D; ++i) r[C::R_DN + i] = Q.dNdx[i];
+        }
         r[C::R_MEAS] = Q.measure;
         if (load == 1) {
             r[C::R_SRC] = eval_expr(*src, A.time, Q.x[0], Q.x[1], D == 3 ? Q.x[2] : 0.0, 0.0);
@@ -417,10 +424,11 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
 //   mode 0: forms applied to a selected state buffer (assembleVector)
 //   mode 1: load vector (SourceForm / NodalSourceForm)
 //   mode 2: essential lift, F -= forms(constrained values)
-template<int D, int P, int NQ, bool STATE>
-__global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_vector(AsmArgs A, Model m, Expr src, int mode,
-                                                                             int64_t c0, int64_t c1) {
-    using C = ECfg<D, P, NQ, STATE>;
+template<int D, int P, int NQ, bool STATE, bool GRAD, int MODE>
+__global__ __launch_bounds__((ECfg<D, P, NQ, STATE, GRAD>::BLOCK)) void k_asm_vector(AsmArgs A, Model m, Expr src,
+                                                                                   int64_t c0, int64_t c1) {
+    using C = ECfg<D, P, NQ, STATE, GRAD>;
+    constexpr int mode = MODE;
     constexpr int NPE = C::NPE;
     __shared__ double lds[C::EPB * C::ELEM_DOUBLES];
     const int el = threadIdx.x / NPE, a = threadIdx.x % NPE;
@@ -463,7 +471,7 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_vector(A
             }
         }
         const int load = mode == 1 ? (A.nodal ? 2 : 1) : 0;
-        phase1<D, P, NQ, STATE>(A, m, &src, load, e, a, X, Uv, Ur, Ue, kap, has_kap, rec);
+        phase1<D, P, NQ, STATE, GRAD>(A, m, &src, load, e, a, X, Uv, Ur, Ue, kap, has_kap, rec);
     }
     __syncthreads();
     if (!valid) return;
@@ -490,9 +498,11 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_vector(A
             double buf = 0.0;
             double& acc = scaled ? buf : Fe;
             if (kind == RSD_FORM_DIFFUSION) {
+                if constexpr (GRAD) {
 #pragma unroll
-                for (int b = 0; b < NPE; ++b)
-                    acc += diff_integrand<D>(m, kq, da, r + C::R_DN + b * D) * Ue[b] * meas * w;
+                    for (int b = 0; b < NPE; ++b)
+                        acc += diff_integrand<D>(m, kq, da, r + C::R_DN + b * D) * Ue[b] * meas * w;
+                }
             } else if (kind == RSD_FORM_MASS) {
                 const double ra = m.rho * m.cp * N[a];
 #pragma unroll
@@ -598,26 +608,43 @@ static void run_matrix(rsd_problem* pb, const AsmArgs& A) {
     else run_matrix_t<D, P, NQ, false>(pb, A);
 }
 
-template<int D, int P, int NQ, bool STATE>
-static void run_vector_t(rsd_problem* pb, AsmArgs A, const Expr& src, int mode) {
-    using C = ECfg<D, P, NQ, STATE>;
+template<int D, int P, int NQ, bool STATE, bool GRAD, int MODE>
+static void run_vector_t(rsd_problem* pb, AsmArgs A, const Expr& src) {
+    using C = ECfg<D, P, NQ, STATE, GRAD>;
     hipStream_t st = pb->ctx->stream;
-    const bool lift = mode == 2;
+    const bool lift = MODE == 2;
     if (lift) A.order = pb->border.p;
     const std::vector<int64_t>& off = lift ? pb->bcolour_off : pb->colour_off;
     for (int c = 0; c < pb->n_colours(); ++c) {
         const int64_t c0 = off[c], c1 = off[c + 1];
         if (c1 <= c0) continue;
-        hipLaunchKernelGGL((k_asm_vector<D, P, NQ, STATE>), dim3((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB)),
-                           dim3(C::BLOCK), 0, st, A, pb->model, src, mode, c0, c1);
+        hipLaunchKernelGGL((k_asm_vector<D, P, NQ, STATE, GRAD, MODE>),
+                           dim3((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB)), dim3(C::BLOCK), 0, st, A, pb->model,
+                           src, c0, c1);
     }
     RSD_HIP(hipGetLastError());
 }
 
 template<int D, int P, int NQ>
 static void run_vector(rsd_problem* pb, const AsmArgs& A, const Expr& src, int mode) {
-    if (A.need_state) run_vector_t<D, P, NQ, true>(pb, A, src, mode);
-    else run_vector_t<D, P, NQ, false>(pb, A, src, mode);
+    bool grad = false;  // a load vector or mass forms only: no J^-1, no dN/dx
+    if (mode != 1)
+        for (int f = 0; f < A.nf; ++f) {
+            const int k = A.kinds[f] & 0xff;
+            if (k == RSD_FORM_DIFFUSION || k == RSD_FORM_TDIFFUSION) grad = true;
+        }
+    // one kernel per mode, so that the load vector's expression interpreter
+    // does not set the register budget of the others
+    if (mode == 1) run_vector_t<D, P, NQ, false, false, 1>(pb, A, src);
+    else if (mode == 0) {
+        if (A.need_state) run_vector_t<D, P, NQ, true, true, 0>(pb, A, src);
+        else if (grad) run_vector_t<D, P, NQ, false, true, 0>(pb, A, src);
+        else run_vector_t<D, P, NQ, false, false, 0>(pb, A, src);
+    } else {
+        if (A.need_state) run_vector_t<D, P, NQ, true, true, 2>(pb, A, src);
+        else if (grad) run_vector_t<D, P, NQ, false, true, 2>(pb, A, src);
+        else run_vector_t<D, P, NQ, false, false, 2>(pb, A, src);
+    }
 }
 
 #define RSD_DISPATCH(D_, P_, NQ_, CALL)                                                      \
