@@ -276,6 +276,23 @@ int rsd_assemble_load(rsd_ctx* ctx, rsd_problem* pb, double time, const char* sr
 /* essential lift F -= A_fc g (BoundaryApplicator.tpp:7-64) */
 int rsd_apply_essential(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t n_forms,
                         const int32_t* kinds, const double* coefs, double* F);
+/* Assembler::assembleDiagonal (Assembler.tpp:142-197): diag [n_free] zeroed,
+ * then each element's Ke[a][a] added in colour order; bitwise the diagonal
+ * of rsd_assemble_matrix with the same forms.  FEMOperator::diagonal. */
+int rsd_assemble_diagonal(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t n_forms,
+                          const int32_t* kinds, const double* coefs, const double* U, double* diag);
+
+/* Matrix-free operator (linalg/operator/FEMOperator.hpp:27-71):
+ * FEMOperator::apply is rsd_assemble_vector with selector RSD_SEL_OPERAND
+ * (y = forms applied to the free vector x, element by element).
+ * rsd_cg_solve_fem is cg::Solver::solve (cg/Solver.tpp:15-119) over that
+ * operator, Jacobi from rsd_assemble_diagonal: no matrix is formed.  U / Udot
+ * are the state the forms see (nullable); arguments otherwise as
+ * rsd_cg_solve_f64. */
+int rsd_cg_solve_fem(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t n_forms,
+                     const int32_t* kinds, const double* coefs, const double* U, const double* Udot,
+                     const double* b, double* x, double tol, int32_t tol_type, int64_t max_iters,
+                     int32_t precond, rsd_report* rep, double* hist);
 
 /* ---------------------------------------------------------------------- */
 /* drivers                                                                  */

@@ -177,6 +177,13 @@ int orc_cg(int64_t n, const int64_t* row_ptr, const int64_t* col_idx, const doub
            const double* b, double* x, double tol, int64_t max_iters, int32_t precond,
            int32_t order, orc_report* rep, double* hist);
 
+/* cg::Solver::solve over the matrix-free FEMOperator
+ * (linalg/operator/FEMOperator.hpp:38-56): apply = assembleVector with the
+ * operand selector, Jacobi from assembleDiagonal (Assembler.tpp:142-197). */
+int orc_cg_fem(const orc_problem* P, double time, int32_t n_forms, const int32_t* kinds, const double* coefs,
+               const double* U, const double* Udot, const double* b, double* x, double tol, int64_t max_iters,
+               int32_t precond, int32_t order, orc_report* rep, double* hist);
+
 /* gmres::Solver::solve (linalg/solver/iterative/gmres/Solver.tpp:5-170) */
 int orc_gmres(int64_t n, const int64_t* row_ptr, const int64_t* col_idx, const double* val,
               const double* b, double* x, double tol, int64_t max_iters, int64_t krylov,

@@ -96,16 +96,18 @@ static void precond_apply(int32_t pc, int64_t n, const double* invd, const doubl
 /* ------------------------------------------------------------------ */
 /* CG: linalg/solver/iterative/cg/Solver.tpp:15-119                     */
 /* ------------------------------------------------------------------ */
-int orc_cg(int64_t n, const int64_t* rp, const int64_t* ci, const double* val, const double* b,
-           double* x, double tol, int64_t max_iters, int32_t pc, int32_t order, orc_report* rep,
-           double* hist) {
+/* The solver is written against an operator (apply) and a precomputed
+ * Jacobi inverse diagonal, as cg::Solver is against its LinearOperator and
+ * preconditioner concepts (linalg/operator/Operator.hpp:10-18). */
+typedef void (*cg_apply_fn)(void* ctx, const double* x, double* y);
+
+static int cg_core(int64_t n, cg_apply_fn apply, void* actx, const double* invd, const double* b, double* x,
+                   double tol, int64_t max_iters, int32_t pc, int32_t order, orc_report* rep, double* hist) {
     double* r = (double*)malloc(sizeof(double) * (size_t)(n + 1));
     double* p = (double*)malloc(sizeof(double) * (size_t)(n + 1));
     double* Ap = (double*)malloc(sizeof(double) * (size_t)(n + 1));
     double* z = (double*)malloc(sizeof(double) * (size_t)(n + 1));
-    double* invd = (double*)malloc(sizeof(double) * (size_t)(n + 1));
-    if (pc == 1) orc_inv_diag(n, rp, ci, val, invd);
-    orc_spmv(n, rp, ci, val, x, Ap);
+    apply(actx, x, Ap);
     for (int64_t i = 0; i < n; ++i) r[i] = b[i];
     for (int64_t i = 0; i < n; ++i) r[i] += -1.0 * Ap[i];
     const double res0 = sqrt(orc_dot(n, r, r, order));
@@ -120,7 +122,7 @@ int orc_cg(int64_t n, const int64_t* rp, const int64_t* ci, const double* val, c
     for (int64_t i = 0; i < n; ++i) p[i] = z[i];
     double rz_old = orc_dot(n, r, z, order);
     for (int64_t k = 1; k < max_iters; ++k) {
-        orc_spmv(n, rp, ci, val, p, Ap);
+        apply(actx, p, Ap);
         const double pAp = orc_dot(n, p, Ap, order);
         const double alpha = rz_old / pAp;
         for (int64_t i = 0; i < n; ++i) x[i] += alpha * p[i];
@@ -147,7 +149,54 @@ int orc_cg(int64_t n, const int64_t* rp, const int64_t* ci, const double* val, c
     }
 done:
     if (hist) hist[0] = res0;
-    free(r); free(p); free(Ap); free(z); free(invd);
+    free(r); free(p); free(Ap); free(z);
+    return conv;
+}
+
+typedef struct { int64_t n; const int64_t* rp; const int64_t* ci; const double* val; } csr_op_t;
+
+/* CSROperator::apply (CSROperator.hpp:26-29) */
+static void csr_apply(void* c, const double* x, double* y) {
+    const csr_op_t* A = (const csr_op_t*)c;
+    orc_spmv(A->n, A->rp, A->ci, A->val, x, y);
+}
+
+int orc_cg(int64_t n, const int64_t* rp, const int64_t* ci, const double* val, const double* b,
+           double* x, double tol, int64_t max_iters, int32_t pc, int32_t order, orc_report* rep,
+           double* hist) {
+    double* invd = (double*)malloc(sizeof(double) * (size_t)(n + 1));
+    if (pc == 1) orc_inv_diag(n, rp, ci, val, invd);
+    csr_op_t A = {n, rp, ci, val};
+    const int conv = cg_core(n, csr_apply, &A, invd, b, x, tol, max_iters, pc, order, rep, hist);
+    free(invd);
+    return conv;
+}
+
+typedef struct {
+    const orc_problem* P; double time; int32_t nf; const int32_t* kinds; const double* coefs;
+    const double* U; const double* Udot;
+} fem_op_t;
+
+/* FEMOperator::apply (FEMOperator.hpp:38-43): assembleVector with the
+ * operand selector, the Krylov vector gathered from free DOFs only */
+static void fem_apply(void* c, const double* x, double* y) {
+    const fem_op_t* F = (const fem_op_t*)c;
+    orc_assemble_vector(F->P, F->time, F->nf, F->kinds, F->coefs, F->U, F->Udot, x, ORC_SEL_OPERAND, y);
+}
+
+int orc_cg_fem(const orc_problem* P, double time, int32_t n_forms, const int32_t* kinds, const double* coefs,
+               const double* U, const double* Udot, const double* b, double* x, double tol, int64_t max_iters,
+               int32_t pc, int32_t order, orc_report* rep, double* hist) {
+    const int64_t n = P->n_free;
+    double* invd = (double*)malloc(sizeof(double) * (size_t)(n + 1));
+    if (pc == 1) {
+        /* FEMOperator::diagonal (FEMOperator.hpp:49-51) + Jacobi::update (Jacobi.hpp:33-36) */
+        orc_assemble_diagonal(P, time, n_forms, kinds, coefs, U, invd);
+        for (int64_t i = 0; i < n; ++i) invd[i] = (fabs(invd[i]) > 1e-300) ? 1.0 / invd[i] : 1.0;
+    }
+    fem_op_t F = {P, time, n_forms, kinds, coefs, U, Udot};
+    const int conv = cg_core(n, fem_apply, &F, invd, b, x, tol, max_iters, pc, order, rep, hist);
+    free(invd);
     return conv;
 }
 

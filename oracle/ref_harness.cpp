@@ -53,6 +53,7 @@
 #include "linalg/types/Vector.hpp"
 #include "linalg/types/CSRMatrix.hpp"
 #include "linalg/operator/CSROperator.hpp"
+#include "linalg/operator/FEMOperator.hpp"
 #include "linalg/solver/iterative/cg/Solver.hpp"
 #include "linalg/solver/iterative/gmres/Solver.hpp"
 #include "linalg/solver/preconditioner/Jacobi.hpp"
@@ -156,6 +157,9 @@ struct Base {
     virtual void assemble_vector(int set, Real coef, const Real* U, Real* F) = 0;
     virtual void load(int kind, const Real* nodal, Real* F) = 0;
     virtual void lift(int set, Real coef, Real* F) = 0;
+    virtual void assemble_diagonal(int set, Real coef, const Real* U, Real* d) = 0;
+    virtual int cg_fem(int set, Real coef, const Real* U, int pc, const Real* rhs, Real* x, Real tol, int64_t maxit,
+                       int64_t* iters) = 0;
     virtual int be_run(Real t0, Real tf, Real dt, Real tol, int64_t maxit, Real* U, int64_t* iters) = 0;
     virtual int newton(Real abs_tol, Real rel_tol, int64_t max_newton, int src, Real lin_tol,
                        int64_t lin_max, int64_t krylov, Real* U, int64_t* lin_iters) = 0;
@@ -298,6 +302,58 @@ struct Ctx : Base {
             app.template applyEssentialBCs<1, EvalEle, QP, OpModel, BEForms, Quad>(mesh, *topo, bcs, 0.0, om, f, ele, quad, out);
         }
         for (Index i = 0; i < out.size(); ++i) F[i] = out.data()[i];
+    }
+
+    // Assembler::assembleDiagonal (Assembler.tpp:142-197); set as assemble_matrix (0 diffusion, 1 mass, 2 BE)
+    void assemble_diagonal(int set, Real coef, const Real* U, Real* d) override {
+        VecT u = vec_from(U);
+        VecT out(topo->numFreeDOFs());
+        State st; st.valueSource = &u;
+        Cond c = cond();
+        if (set == 0) asmb.template assembleDiagonal<1, EvalEle, QP, Cond, StiffForms, Quad, State>(mesh, *topo, 0.0, c, StiffForms{}, ele, quad, st, out, &bcs);
+        else if (set == 1) asmb.template assembleDiagonal<1, EvalEle, QP, MassModel, MassForms, Quad, State>(mesh, *topo, 0.0, mass_model(), MassForms{}, ele, quad, st, out, &bcs);
+        else {
+            BEForms f(StiffForms{}, fem::form::ScaledForm<1, MassForms>(coef));
+            OpModel om(c, mass_model());
+            asmb.template assembleDiagonal<1, EvalEle, QP, OpModel, BEForms, Quad, State>(mesh, *topo, 0.0, om, f, ele, quad, st, out, &bcs);
+        }
+        for (Index i = 0; i < out.size(); ++i) d[i] = out.data()[i];
+    }
+
+    // cg::Solver over the matrix-free FEMOperator (FEMOperator.hpp:27-71); set 0 diffusion, 2 BE
+    template<typename ModelT, typename FormsT>
+    int cg_fem_t(const ModelT& model, const FormsT& forms, const Real* U, int pc, const Real* rhs, Real* x, Real tol,
+                 int64_t maxit, int64_t* iters) {
+        const Index n = topo->numFreeDOFs();
+        VecT u = vec_from(U), B = vec_from(rhs), X = vec_from(x);
+        State st; st.valueSource = &u;
+        Real time = 0.0;
+        using Op = linalg::op::FEMOperator<fem::assembly::Assembler<Backend>, topology::TopologicalDOF<1>, EvalEle, QP,
+                                           ModelT, FormsT, Quad, State, VecT>;
+        Op op(asmb, mesh, *topo, &time, model, forms, ele, quad, &bcs, st);
+        utils::logging::NullLogger lg;
+        linalg::solver::SolverReport<VecT> rep;
+        auto run = [&](auto pcT) {
+            using PC = decltype(pcT);
+            PC M;
+            using S = linalg::solver::iterative::cg::Solver<Op, VecT, PC, utils::logging::NullLogger>;
+            typename S::Config cfg; cfg.tol = tol; cfg.maxIters = (Index)maxit;
+            S s(cfg); typename S::Workspace W(n);
+            return s.solve(rep, lg, W, M, op, B, X);
+        };
+        const bool ok = pc == 1 ? run(linalg::solver::preconditioner::Jacobi<VecT>{})
+                                : run(linalg::solver::preconditioner::Identity<VecT>{});
+        for (Index i = 0; i < n; ++i) x[i] = X.data()[i];
+        if (iters) *iters = (int64_t)rep.iterations;
+        return ok ? 1 : 0;
+    }
+    int cg_fem(int set, Real coef, const Real* U, int pc, const Real* rhs, Real* x, Real tol, int64_t maxit,
+               int64_t* iters) override {
+        Cond c = cond();
+        if (set == 0) return cg_fem_t(c, StiffForms{}, U, pc, rhs, x, tol, maxit, iters);
+        BEForms f(StiffForms{}, fem::form::ScaledForm<1, MassForms>(coef));
+        OpModel om(c, mass_model());
+        return cg_fem_t(om, f, U, pc, rhs, x, tol, maxit, iters);
     }
 
     template<typename PC>
@@ -522,6 +578,22 @@ int ref_solve(void* h, int solver, int pc, const double* val, const double* rhs,
     for (Index i = 0; i < n; ++i) x[i] = X.data()[i];
     if (iters) *iters = (int64_t)rep.iterations;
     return ok ? 1 : 0;
+}
+
+void ref_assemble_diagonal(void* h, int set, double coef, const double* U, const double* kappa, double* d) {
+    Base* b = static_cast<Base*>(h);
+    with_kappa(b, kappa);
+    b->assemble_diagonal(set, coef, U, d);
+    g_kappa = nullptr;
+}
+
+int ref_cg_fem(void* h, int set, double coef, const double* U, const double* kappa, int pc, const double* rhs,
+               double* x, double tol, int64_t maxit, int64_t* iters) {
+    Base* b = static_cast<Base*>(h);
+    with_kappa(b, kappa);
+    const int r = b->cg_fem(set, coef, U, pc, rhs, x, tol, maxit, iters);
+    g_kappa = nullptr;
+    return r;
 }
 
 int ref_be_run(void* h, double t0, double tf, double dt, double tol, int64_t maxit, const double* kappa,

@@ -334,17 +334,23 @@ __device__ __forceinline__ double tdiff_integrand(const Model& m, double dk, con
     return integrand;
 }
 
-template<int D, int P, int NQ, bool STATE>
+// Matrix assembly (DIAG = false): thread a scatters row a of Ke through the
+// element->slot map.  DIAG = true (Assembler::assembleDiagonal,
+// Assembler.tpp:142-197): thread a forms only Ke[a][a], the same chain of
+// products and sums as in the matrix, and adds it to diag[row(a)].
+template<int D, int P, int NQ, bool STATE, bool DIAG = false>
 __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(AsmArgs A, Model m, int64_t c0,
                                                                              int64_t c1) {
     using C = ECfg<D, P, NQ, STATE>;
     constexpr int NPE = C::NPE;
+    constexpr int NB = DIAG ? 1 : NPE;  // entries of row a this thread forms
     __shared__ double lds[C::EPB * C::ELEM_DOUBLES];
     const int el = threadIdx.x / NPE, a = threadIdx.x % NPE;
     const int64_t k = c0 + (int64_t)blockIdx.x * C::EPB + el;
     const bool valid = k < c1;
     double* rec = lds + el * C::ELEM_DOUBLES;
     int64_t e = 0;
+    int32_t node_a = 0;
     double kap = 1.0;
     const bool has_kap = A.kappa != nullptr;
     if (valid) {
@@ -352,6 +358,7 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
         int32_t nodes[NPE];
         double X[NPE][D];
         load_element<D, P>(A, e, nodes, X);
+        node_a = nodes[a];
         if (has_kap) kap = A.kappa[e];
         double Uv[NPE], Ur[NPE];
         if constexpr (STATE) {
@@ -364,9 +371,9 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
     if (!valid) return;
     double kel = m.k_scalar;
     if (has_kap) kel = kap * kel;
-    double Ke[NPE];
+    double Ke[NB];
 #pragma unroll
-    for (int b = 0; b < NPE; ++b) Ke[b] = 0.0;
+    for (int i = 0; i < NB; ++i) Ke[i] = 0.0;
     for (int q = 0; q < C::NQP; ++q) {
         const double* r = rec + q * C::REC;
         double xi[3];
@@ -382,41 +389,50 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
             const double coef = A.coefs[f];
             if (kind == RSD_FORM_DIFFUSION) {
 #pragma unroll
-                for (int b = 0; b < NPE; ++b) {
+                for (int i = 0; i < NB; ++i) {
+                    const int b = DIAG ? a : i;
                     const double c = diff_integrand<D>(m, kq, da, r + C::R_DN + b * D) * meas * w;
-                    Ke[b] += scaled ? coef * c : c;
+                    Ke[i] += scaled ? coef * c : c;
                 }
             } else if (kind == RSD_FORM_MASS) {
                 const double ra = m.rho * m.cp * N[a];
 #pragma unroll
-                for (int b = 0; b < NPE; ++b) {
+                for (int i = 0; i < NB; ++i) {
+                    const int b = DIAG ? a : i;
                     const double c = ra * N[b] * meas * w;
-                    Ke[b] += scaled ? coef * c : c;
+                    Ke[i] += scaled ? coef * c : c;
                 }
             } else if constexpr (STATE) {
                 if (kind == RSD_FORM_TDIFFUSION) {
                     const double it = tdiff_integrand<D>(m, r[C::R_DK], da, r + C::R_GT);
 #pragma unroll
-                    for (int b = 0; b < NPE; ++b) {
+                    for (int i = 0; i < NB; ++i) {
+                        const int b = DIAG ? a : i;
                         const double c = it * N[b] * meas * w;
-                        Ke[b] += scaled ? coef * c : c;
+                        Ke[i] += scaled ? coef * c : c;
                     }
                 } else {  // TMASS: rho * dcp/dT * Trate * N_a * N_b, dcp/dT = 0 for a constant cp
                     const double ra = m.rho * 0.0 * r[C::R_TR] * N[a];
 #pragma unroll
-                    for (int b = 0; b < NPE; ++b) {
+                    for (int i = 0; i < NB; ++i) {
+                        const int b = DIAG ? a : i;
                         const double c = ra * N[b] * meas * w;
-                        Ke[b] += scaled ? coef * c : c;
+                        Ke[i] += scaled ? coef * c : c;
                     }
                 }
             }
         }
     }
-    const int32_t* mp = A.map + (e * NPE + a) * NPE;
+    if constexpr (DIAG) {
+        const int32_t row = A.t2a[node_a];
+        if (row >= 0) A.F[row] += Ke[0];
+    } else {
+        const int32_t* mp = A.map + (e * NPE + a) * NPE;
 #pragma unroll
-    for (int b = 0; b < NPE; ++b) {
-        const int32_t s = mp[b];
-        if (s >= 0) A.val[s] += Ke[b];
+        for (int b = 0; b < NPE; ++b) {
+            const int32_t s = mp[b];
+            if (s >= 0) A.val[s] += Ke[b];
+        }
     }
 }
 
@@ -429,6 +445,7 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE, GRAD>::BLOCK)) void k_asm_ve
                                                                                    int64_t c0, int64_t c1) {
     using C = ECfg<D, P, NQ, STATE, GRAD>;
     constexpr int mode = MODE;
+    if (A.stop && *A.stop) return;
     constexpr int NPE = C::NPE;
     __shared__ double lds[C::EPB * C::ELEM_DOUBLES];
     const int el = threadIdx.x / NPE, a = threadIdx.x % NPE;
@@ -589,23 +606,30 @@ static void run_adj(rsd_problem* pb, const AsmArgs& A, const Model& unit, const 
     RSD_HIP(hipGetLastError());
 }
 
-template<int D, int P, int NQ, bool STATE>
+template<int D, int P, int NQ, bool STATE, bool DIAG>
 static void run_matrix_t(rsd_problem* pb, const AsmArgs& A) {
     using C = ECfg<D, P, NQ, STATE>;
     hipStream_t st = pb->ctx->stream;
     for (int c = 0; c < pb->n_colours(); ++c) {
         const int64_t c0 = pb->colour_off[c], c1 = pb->colour_off[c + 1];
         if (c1 <= c0) continue;
-        hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, STATE>), dim3((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB)),
+        hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, STATE, DIAG>), dim3((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB)),
                            dim3(C::BLOCK), 0, st, A, pb->model, c0, c1);
     }
     RSD_HIP(hipGetLastError());
 }
 
+// A.F set: the diagonal (assembleDiagonal) into A.F; else the matrix into A.val
 template<int D, int P, int NQ>
 static void run_matrix(rsd_problem* pb, const AsmArgs& A) {
-    if (A.need_state) run_matrix_t<D, P, NQ, true>(pb, A);
-    else run_matrix_t<D, P, NQ, false>(pb, A);
+    const bool diag = A.F != nullptr;
+    if (A.need_state) {
+        if (diag) run_matrix_t<D, P, NQ, true, true>(pb, A);
+        else run_matrix_t<D, P, NQ, true, false>(pb, A);
+    } else {
+        if (diag) run_matrix_t<D, P, NQ, false, true>(pb, A);
+        else run_matrix_t<D, P, NQ, false, false>(pb, A);
+    }
 }
 
 template<int D, int P, int NQ, bool STATE, bool GRAD, int MODE>
@@ -725,10 +749,32 @@ void assemble_matrix(rsd_problem* pb, double time, int32_t nf, const int32_t* ki
     dispatch_matrix(pb, A);
 }
 
+void assemble_diagonal(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
+                       const double* U, double* diag) {
+    AsmArgs A = base_args(pb);
+    set_forms(A, nf, kinds, coefs);
+    A.U = U;
+    A.Udot = nullptr;  // the rate enters no diagonal term of the heat forms
+    A.F = diag;
+    A.time = time;
+    A.need_state = forms_need_state(pb, A);
+    update_gval(pb, time);
+    launch_fill(pb->ctx, pb->n_free, diag, 0.0);
+    dispatch_matrix(pb, A);
+}
+
+__global__ void k_zero_unless(int64_t n, double* __restrict__ p, const int32_t* __restrict__ stop) {
+    if (*stop) return;
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
 void assemble_vector(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
-                     const double* U, const double* Udot, const double* operand, int32_t selector, double* F) {
+                     const double* U, const double* Udot, const double* operand, int32_t selector, double* F,
+                     const int32_t* stop) {
     RSD_CHECK(selector >= 0 && selector <= 2, "bad selector");
     AsmArgs A = base_args(pb);
+    A.stop = stop;
     set_forms(A, nf, kinds, coefs);
     A.U = U;
     A.Udot = Udot;
@@ -738,7 +784,13 @@ void assemble_vector(rsd_problem* pb, double time, int32_t nf, const int32_t* ki
     A.time = time;
     A.need_state = forms_need_state(pb, A);
     update_gval(pb, time);
-    launch_fill(pb->ctx, pb->n_free, F, 0.0);
+    if (stop) {
+        if (pb->n_free)
+            hipLaunchKernelGGL(k_zero_unless, dim3((unsigned)nblocks(pb->n_free)), dim3(kBlock), 0, pb->ctx->stream,
+                               pb->n_free, F, stop);
+    } else {
+        launch_fill(pb->ctx, pb->n_free, F, 0.0);
+    }
     dispatch_vector(pb, A, Expr{}, 0);
 }
 
@@ -802,6 +854,15 @@ int rsd_assemble_matrix(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t nf, 
         RSD_CHECK(ctx && pb && K && (nf == 0 || kinds), "rsd_assemble_matrix: null argument");
         RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
         assemble_matrix(pb, time, nf, kinds, coefs, U, Udot, K);
+    });
+}
+
+int rsd_assemble_diagonal(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t nf, const int32_t* kinds,
+                          const double* coefs, const double* U, double* diag) {
+    return guard([&] {
+        RSD_CHECK(ctx && pb && diag && (nf == 0 || kinds), "rsd_assemble_diagonal: null argument");
+        RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
+        assemble_diagonal(pb, time, nf, kinds, coefs, U, diag);
     });
 }
 

@@ -25,13 +25,19 @@ struct AsmArgs {
     int32_t kinds[4];
     double coefs[4];
     int32_t need_state;
+    const int32_t* stop;  // nullable: vector kernels return at once when *stop != 0
 };
 
 AsmArgs base_args(rsd_problem* pb);
 void assemble_matrix(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
                      const double* U, const double* Udot, rsd_matrix* K);
 void assemble_vector(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
-                     const double* U, const double* Udot, const double* operand, int32_t selector, double* F);
+                     const double* U, const double* Udot, const double* operand, int32_t selector, double* F,
+                     const int32_t* stop = nullptr);
+// Assembler::assembleDiagonal (Assembler.tpp:142-197): diag zeroed, then
+// each element's Ke[a][a] added in colour order
+void assemble_diagonal(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
+                       const double* U, double* diag);
 void assemble_load(rsd_problem* pb, double time, const Expr* src, const double* nodal, double* F);
 void apply_essential(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs, double* F);
 const Expr* cached_source(rsd_problem* pb, const char* text);

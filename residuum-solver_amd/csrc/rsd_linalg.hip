@@ -147,6 +147,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_pap(int64_t n, int64_t nb, C
     block_partials<1>(v, lb, nb, partials);
 }
 
+// block sums of p.Ap for an Ap formed by another kernel (a matrix-free apply)
+__global__ __launch_bounds__(kBlock) void k_cg_pap(int64_t n, int64_t nb, const double* __restrict__ p,
+                                                   const double* __restrict__ Ap, const CGState* st,
+                                                   double* partials) {
+    if (st->stop) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t r = lb * kBlock + threadIdx.x;
+    double v[1] = {0.0};
+    if (r < n) v[0] = p[r] * Ap[r];
+    block_partials<1>(v, lb, nb, partials);
+}
+
 __global__ __launch_bounds__(kFold) void k_cg_alpha(int64_t nb, const double* __restrict__ partials, CGState* st) {
     if (st->stop) return;
     const double rz = st->rz_old;
@@ -288,6 +300,30 @@ void cg_launch_iter(rsd_ctx* ctx, const rsd_matrix* A, double* x, double tol, in
     hipStream_t s = ctx->stream;
     RSD_LAUNCH_COLS(S.mode(), k_cg_spmv_pap, dim3((unsigned)nb), dim3(kBlock), s, n, nb, S.cols(), A->val.p, w.p, w.Ap,
                     w.st, part);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kFold), 0, s, nb, part, w.st);
+    hipLaunchKernelGGL(k_cg_update_r, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, x, w.p, w.Ap, w.r, invd, w.st,
+                       part + nb);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kFold), 0, s, nb, part + nb, w.st, tol, w.abs_tol, max_iters, w.hist);
+    hipLaunchKernelGGL(k_cg_update_p, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.p, w.r, invd, w.st);
+}
+
+void cg_launch_init_ax(rsd_ctx* ctx, int64_t n, const double* b, double tol, int64_t max_iters, int32_t precond,
+                       const CGWork& w) {
+    const int64_t nb = std::max<int64_t>(1, nblocks(n));
+    double* part = ctx->partials(3 * nb);
+    hipLaunchKernelGGL(k_cg_init, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, b, w.Ap, w.r, w.p,
+                       precond == RSD_PC_JACOBI ? w.invd : nullptr, part);
+    hipLaunchKernelGGL(k_cg_init_fold, dim3(1), dim3(kFold), 0, ctx->stream, nb, part, w.st, tol, w.abs_tol, max_iters,
+                       w.hist);
+}
+
+void cg_launch_iter_ap(rsd_ctx* ctx, int64_t n, double* x, double tol, int64_t max_iters, int32_t precond,
+                       const CGWork& w) {
+    const int64_t nb = std::max<int64_t>(1, nblocks(n));
+    double* part = ctx->partials(3 * nb);
+    const double* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_cg_pap, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, w.p, w.Ap, w.st, part);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kFold), 0, s, nb, part, w.st);
     hipLaunchKernelGGL(k_cg_update_r, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, x, w.p, w.Ap, w.r, invd, w.st,
                        part + nb);

@@ -35,6 +35,13 @@ void cg_launch_iter(rsd_ctx* ctx, const rsd_matrix* A, double* x, double tol, in
                     const CGWork& w);
 CGState cg_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
                int32_t precond, const CGWork& w);
+// the same iteration around an operator applied by other kernels: init
+// with w.Ap = A x (and w.invd set for Jacobi); each iteration with
+// w.Ap = A p already formed
+void cg_launch_init_ax(rsd_ctx* ctx, int64_t n, const double* b, double tol, int64_t max_iters, int32_t precond,
+                       const CGWork& w);
+void cg_launch_iter_ap(rsd_ctx* ctx, int64_t n, double* x, double tol, int64_t max_iters, int32_t precond,
+                       const CGWork& w);
 void cg_report(const CGState& s, int64_t max_iters, rsd_report* rep);
 
 }  // namespace rsd

@@ -370,6 +370,29 @@ class Problem:
         nf, kinds, coefs = _forms(forms)
         check(lib().rsd_apply_essential(self.ctx.h, self.h, C.c_double(time), C.c_int32(nf), kinds, coefs, dptr(F)))
 
+    def assemble_diagonal(self, diag, forms, U=None, time=0.0):
+        """Assembler::assembleDiagonal: the diagonal of the forms' matrix."""
+        nf, kinds, coefs = _forms(forms)
+        check(lib().rsd_assemble_diagonal(self.ctx.h, self.h, C.c_double(time), C.c_int32(nf), kinds, coefs,
+                                          dptr(U), dptr(diag)))
+
+    def fem_apply(self, y, forms, x, U=None, Udot=None, time=0.0):
+        """FEMOperator::apply: y = forms applied to the free vector x."""
+        self.assemble_vector(y, forms, U=U, Udot=Udot, operand=x, selector=SEL_OPERAND, time=time)
+
+    def cg_fem(self, b, x, forms, tol=1e-10, max_iters=1000, precond=PC_JACOBI, U=None, Udot=None, time=0.0,
+               tol_type=TOL_RELATIVE, hist=False):
+        """cg::Solver over the matrix-free FEMOperator; returns the Report
+        (and the residual history when hist)."""
+        rep = Report()
+        h = np.zeros(max_iters + 1) if hist else None
+        nf, kinds, coefs = _forms(forms)
+        check(lib().rsd_cg_solve_fem(self.ctx.h, self.h, C.c_double(time), C.c_int32(nf), kinds, coefs, dptr(U),
+                                     dptr(Udot), dptr(b), dptr(x), C.c_double(tol), C.c_int32(tol_type),
+                                     C.c_int64(max_iters), C.c_int32(precond), C.byref(rep),
+                                     h.ctypes.data_as(_d) if hist else None))
+        return (rep, h) if hist else rep
+
     # -- drivers ----------------------------------------------------------------
     def be_run(self, U, t0, tf, dt, src=None, tol=1e-8, max_iters=1000, precond=PC_JACOBI, use_graph=True,
                max_steps=None, mixed=False):

@@ -249,7 +249,26 @@ class Case:
         lib().orc_apply_essential(C.byref(self.P), C.c_double(time), len(forms), ip(kinds), dp(coefs), dp(F))
         return F
 
+    def assemble_diagonal(self, forms, U=None, time=0.0):
+        kinds = np.array([f[0] for f in forms], np.int32)
+        coefs = np.array([f[1] for f in forms], np.float64)
+        d = np.zeros(self.n_free)
+        lib().orc_assemble_diagonal(C.byref(self.P), C.c_double(time), len(forms), ip(kinds), dp(coefs), dp(U), dp(d))
+        return d
+
     # -- solvers --------------------------------------------------------
+    def cg_fem(self, forms, b, x0=None, tol=1e-10, max_iters=1000, precond=1, U=None, Udot=None, time=0.0,
+               order=None, hist=False):
+        kinds = np.array([f[0] for f in forms], np.int32)
+        coefs = np.array([f[1] for f in forms], np.float64)
+        x = np.zeros(self.n_free) if x0 is None else x0.astype(np.float64).copy()
+        rep = OrcReport()
+        h = np.zeros(max_iters + 1) if hist else None
+        lib().orc_cg_fem(C.byref(self.P), C.c_double(time), len(forms), ip(kinds), dp(coefs), dp(U), dp(Udot),
+                         dp(b), dp(x), C.c_double(tol), C.c_int64(max_iters), precond,
+                         self.order if order is None else order, C.byref(rep), dp(h))
+        return (x, rep, h) if hist else (x, rep)
+
     def spmv(self, val, x):
         y = np.zeros(self.n_free)
         lib().orc_spmv(C.c_int64(self.n_free), lp(self.rp), lp(self.ci), dp(val), dp(x), dp(y))
@@ -421,6 +440,18 @@ class RefCase:
         F = F.copy()
         ref().ref_lift(self.h, C.c_int(kind_set), C.c_double(coef), dp(kappa), dp(F))
         return F
+
+    def assemble_diagonal(self, kind_set, coef=1.0, U=None, kappa=None):
+        d = np.zeros(self.n_free)
+        ref().ref_assemble_diagonal(self.h, C.c_int(kind_set), C.c_double(coef), dp(U), dp(kappa), dp(d))
+        return d
+
+    def cg_fem(self, kind_set, b, coef=1.0, U=None, kappa=None, precond=1, x0=None, tol=1e-10, max_iters=1000):
+        x = np.zeros(self.n_free) if x0 is None else x0.astype(np.float64).copy()
+        it = np.zeros(1, np.int64)
+        ok = ref().ref_cg_fem(self.h, C.c_int(kind_set), C.c_double(coef), dp(U), dp(kappa), C.c_int(precond), dp(b),
+                              dp(x), C.c_double(tol), C.c_int64(max_iters), lp(it))
+        return x, bool(ok), int(it[0])
 
     def solve(self, solver, precond, val, b, x0=None, tol=1e-10, max_iters=1000, krylov=30):
         x = np.zeros(self.n_free) if x0 is None else x0.astype(np.float64).copy()

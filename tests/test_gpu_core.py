@@ -202,6 +202,71 @@ def test_cg_bitexact(ctx, case, precond):
     assert np.max(np.abs(dx.numpy() - xr)) <= 1e-12 * np.max(np.abs(xr))
 
 
+@pytest.mark.parametrize("case", CASES)
+def test_assemble_diagonal_bitexact(ctx, case):
+    """assembleDiagonal (Assembler.tpp:142-197) on the device: bitwise the
+    oracle in GPU order and bitwise the diagonal of the assembled matrix."""
+    pb, oc = build(ctx, case, kappa_seed=6)
+    rp, ci = pb.pattern()
+    ds = np.array([rp[i] + int(np.searchsorted(ci[rp[i]:rp[i + 1]], i)) for i in range(oc.n_free)])
+    K = pb.matrix()
+    d = pb.vector()
+    for forms in ([(O.DIFFUSION, 1.0)], [(O.MASS, 1.0)], [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 250.0)]):
+        pb.assemble_diagonal(d, forms)
+        pb.assemble_matrix(K, forms)
+        got = d.numpy()
+        assert same(got, oc.assemble_diagonal(forms))
+        assert same(got, K.values()[ds])
+
+
+@pytest.mark.parametrize("case", [CASES[1], CASES[2], CASES[3]])
+@pytest.mark.parametrize("precond", [0, 1])
+def test_cg_fem_bitexact(ctx, case, precond):
+    """CG over the matrix-free FEMOperator: apply by element assembly, Jacobi
+    from assembleDiagonal.  Iterates, iteration count and residual history
+    bitwise the oracle's FEM CG in GPU order; the solution equals the CSR
+    solve's to the solver tolerance."""
+    pb, oc = build(ctx, case, kappa_seed=8)
+    forms = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 100.0)]
+    rng = np.random.default_rng(10)
+    b = rng.standard_normal(oc.n_free)
+    x0 = rng.standard_normal(oc.n_free) * 0.1
+    db, dx = ctx.from_numpy(b), ctx.from_numpy(x0)
+    rep, hist = pb.cg_fem(db, dx, forms, tol=1e-10, max_iters=500, precond=precond, hist=True)
+    xo, ro, ho = oc.cg_fem(forms, b, x0=x0, tol=1e-10, max_iters=500, precond=precond, hist=True)
+    say("fem iters", rep.iterations, ro.iterations)
+    assert rep.converged == ro.converged == 1 and rep.iterations == ro.iterations
+    assert same(dx.numpy(), xo)
+    assert same(hist[:rep.iterations + 1], ho[:ro.iterations + 1])
+    K = pb.matrix()
+    pb.assemble_matrix(K, forms)
+    dx.upload(x0)
+    K.cg(db, dx, tol=1e-13, max_iters=2000, precond=precond)
+    xc = dx.numpy()
+    dx.upload(x0)
+    pb.cg_fem(db, dx, forms, tol=1e-13, max_iters=2000, precond=precond)
+    assert np.max(np.abs(dx.numpy() - xc)) <= 1e-11 * np.max(np.abs(xc))
+
+
+def test_matrix_free_cg_exact_bilinear(ctx):
+    """The reference's matrix-free CG test on the device
+    (tests/integration/full/heateq/cpu/HeatEquationMinimal2D.cpp:249-320):
+    24x24 Q1 on [-1,1]^2, g = (1-x)(1-y), CG + Identity to 1e-12."""
+    import residuum_hip as rh
+    g = "(1-x)*(1-y)"
+    case = (2, (24, 24), (1, 1), 2, [0, 1, 2, 3], [g] * 4, (-1.0, 1.0, -1.0, 1.0))
+    pb, oc = build(ctx, case)
+    assert pb.n_free == 529
+    F = pb.vector()
+    pb.assemble_load(F)
+    pb.apply_essential(F, [(O.DIFFUSION, 1.0)])
+    x = pb.vector(0.0)
+    rep = pb.cg_fem(F, x, [(O.DIFFUSION, 1.0)], tol=1e-12, max_iters=10000, precond=rh.PC_IDENTITY)
+    assert rep.converged == 1 and abs(rep.final_residual - 1e-12) <= 1e-11
+    xy = oc.xyz.reshape(-1, 2)[oc.a2t]
+    assert np.max(np.abs(x.numpy() - (1 - xy[:, 0]) * (1 - xy[:, 1]))) <= 1e-10
+
+
 def test_cg_absolute_tolerance(ctx):
     """cg/Solver.tpp:45-48 with ToleranceType::Absolute: stop at the first k
     with ||r_k|| < tol, leaving exactly the iterate of k steps."""

@@ -277,3 +277,92 @@ def test_live_reference(spec):
     out = c.be_run(0.0, 0.03, 0.01, cg_tol=1e-9)
     Ur, st, its = r.be_run(0.0, 0.03, 0.01, tol=1e-9, kappa=kap)
     assert out["steps"] == st and same(out["iters"], its) and same(out["U"], Ur)
+
+
+# ---------------------------------------------------------------------------
+# the matrix-free operator (FEMOperator.hpp:27-71) and assembleDiagonal
+# ---------------------------------------------------------------------------
+DIAG_CASES = [
+    (2, (5, 4), (1, 1), 2, [0, 2], ["x", "x"]),
+    (2, (3, 3), (2, 2), 3, [1], ["1"]),
+    (3, (3, 2, 3), (1, 1, 1), 2, [4], ["1"]),
+]
+BE_FORMS = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 40.0)]
+
+
+def diag_slots(c):
+    return np.array([c.rp[i] + int(np.searchsorted(c.ci[c.rp[i]:c.rp[i + 1]], i)) for i in range(c.n_free)])
+
+
+@pytest.mark.parametrize("order", [O.ORDER_REF, O.ORDER_GPU])
+@pytest.mark.parametrize("spec", DIAG_CASES)
+def test_assemble_diagonal_is_matrix_diagonal(spec, order):
+    """assembleDiagonal adds each element's Ke[a][a] in element order, which
+    is how assembleMatrix builds the diagonal entries: the two agree bit for
+    bit, in either element order."""
+    dim, n, p, nq, tags, exprs = spec
+    c = O.Case(dim, n, p, nq, tags, exprs)
+    c.kappa = np.exp(0.5 * np.random.default_rng(4).standard_normal(c.n_elem))
+    c.set_order(order)
+    ds = diag_slots(c)
+    for forms in ([(O.DIFFUSION, 1.0)], [(O.MASS, 1.0)], BE_FORMS):
+        A = c.assemble_matrix(forms)
+        assert same(c.assemble_diagonal(forms), A[ds])
+
+
+@pytest.mark.parametrize("spec", DIAG_CASES)
+def test_fem_apply_matches_csr(spec):
+    """FEMOperator::apply (assembleVector on the operand) is A x summed
+    element by element instead of row by row: equal to rounding."""
+    dim, n, p, nq, tags, exprs = spec
+    c = O.Case(dim, n, p, nq, tags, exprs)
+    c.kappa = np.exp(0.5 * np.random.default_rng(5).standard_normal(c.n_elem))
+    c.set_order(O.ORDER_REF)
+    A = c.assemble_matrix(BE_FORMS)
+    x = np.random.default_rng(6).standard_normal(c.n_free)
+    y_fem = c.assemble_vector(BE_FORMS, operand=x, selector=O.SEL_OPERAND)
+    y_csr = c.spmv(A, x)
+    scale = c.spmv(np.abs(A), np.abs(x))
+    assert np.max(np.abs(y_fem - y_csr) / scale) < 1e-14
+
+
+def test_matrix_free_cg_exact_bilinear():
+    """The reference's matrix-free CG test (tests/integration/full/heateq/cpu/
+    HeatEquationMinimal2D.cpp:249-320): 24x24 Q1 on [-1,1]^2, g = (1-x)(1-y)
+    on all sides, no source, CG + Identity to 1e-12: the nodal solution is
+    (1-x)(1-y) to 1e-10 and the final residual is 1e-12 +- 1e-11."""
+    g = "(1-x)*(1-y)"
+    c = O.Case(2, (24, 24), (1, 1), 2, [0, 1, 2, 3], [g] * 4, ext=(-1.0, 1.0, -1.0, 1.0))
+    assert c.n_free == 529
+    F = c.lift([(O.DIFFUSION, 1.0)], c.load())
+    x, rep = c.cg_fem([(O.DIFFUSION, 1.0)], F, tol=1e-12, max_iters=10000, precond=0)
+    assert rep.converged == 1 and abs(rep.final_residual - 1e-12) <= 1e-11
+    xy = c.xyz.reshape(-1, 2)[c.a2t]
+    assert np.max(np.abs(x - (1 - xy[:, 0]) * (1 - xy[:, 1]))) <= 1e-10
+    xc, repc = c.cg(c.assemble_matrix([(O.DIFFUSION, 1.0)]), F, tol=1e-12, max_iters=10000, precond=0)
+    assert repc.converged == 1 and np.max(np.abs(x - xc)) <= 1e-10
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref/libref.so not built (needs /root/reference)")
+@pytest.mark.parametrize("spec", [
+    (2, (6, 4), (1, 1), 2, [0, 2], [7, 7], None),
+    (2, (3, 4), (2, 2), 3, [1, 3], [2, 2], None),
+    (3, (3, 2, 4), (1, 1, 1), 2, [4, 5], [6, 0], None),
+])
+def test_live_reference_matrix_free(spec):
+    """assembleDiagonal and cg::Solver over FEMOperator, oracle against the
+    reference's own templates: bit for bit, iteration counts equal."""
+    dim, n, p, nq, tags, kinds, ext = spec
+    c = O.Case(dim, n, p, nq, tags, [O.KIND_EXPR[k] for k in kinds], ext=ext)
+    r = O.RefCase(dim, n, p, nq, tags, kinds, ext=ext)
+    kap = np.exp(np.random.default_rng(2).standard_normal(c.n_elem))
+    c.kappa = kap
+    c.set_order(O.ORDER_REF)
+    assert same(c.assemble_diagonal([(O.DIFFUSION, 1.0)]), r.assemble_diagonal(0, kappa=kap))
+    assert same(c.assemble_diagonal([(O.MASS, 1.0)]), r.assemble_diagonal(1, kappa=kap))
+    assert same(c.assemble_diagonal(BE_FORMS), r.assemble_diagonal(2, 40.0, kappa=kap))
+    b = np.random.default_rng(3).standard_normal(c.n_free)
+    for pc in (0, 1):
+        x, rep = c.cg_fem(BE_FORMS, b, tol=1e-10, max_iters=500, precond=pc)
+        xr, ok, it = r.cg_fem(2, b, coef=40.0, kappa=kap, precond=pc, tol=1e-10, max_iters=500)
+        assert ok and rep.converged == 1 and rep.iterations == it and same(x, xr)
