@@ -110,6 +110,7 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-matrix-free", action="store_true", help="skip the matrix-free operator solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -202,6 +203,36 @@ def main():
         print(f"# {name}: {avg_ms*1e3:.1f} us/launch, {spmv[name]['gbs']:.0f} GB/s algorithmic", file=sys.stderr,
               flush=True)
 
+    # matrix-free operator (FEMOperator): the same BE system solved with no
+    # matrix, apply = element assembly on the operand, Jacobi from the
+    # assembled diagonal; a bounded solve (one step's system)
+    matrix_free = None
+    if not args.no_matrix_free:
+        be_forms = [(rh.DIFFUSION, 1.0), (rh.MASS | rh.SCALED, 1.0 / dt)]
+        bx, yx = pb.vector(1.0), pb.vector()
+        for _ in range(2):
+            pb.fem_apply(yx, be_forms, bx)
+        ctx.sync()
+        a, b_ = new_ev(), new_ev()
+        rec_ev(a, stream)
+        reps = 5
+        for _ in range(reps):
+            pb.fem_apply(yx, be_forms, bx)
+        rec_ev(b_, stream)
+        apply_ms = ev_ms(a, b_) / reps
+        xs = pb.vector(0.0)
+        ctx.sync()
+        a, b_ = new_ev(), new_ev()
+        rec_ev(a, stream)
+        rep = pb.cg_fem(bx, xs, be_forms, tol=1e-8, max_iters=5000)
+        rec_ev(b_, stream)
+        solve_ms = ev_ms(a, b_)
+        matrix_free = {"apply_ms": apply_ms, "apply_over_spmv_f64": apply_ms * 1e3 / spmv["spmv_f64"]["avg_us"],
+                       "cg_iters": int(rep.iterations), "solve_ms": solve_ms,
+                       "dof_iters_per_s": nfree * rep.iterations / (solve_ms * 1e-3)}
+        print(f"# matrix-free: apply {apply_ms:.2f} ms, CG+Jacobi {rep.iterations} iterations in {solve_ms:.1f} ms",
+              file=sys.stderr, flush=True)
+
     # calibration copy of known size (2^26 doubles = 512 MiB each way): the
     # FETCH_SIZE / WRITE_SIZE counters of this dispatch scale the SpMV's
     calib_n = 1 << 26
@@ -248,6 +279,7 @@ def main():
                             "step_time": fp["total_ms"] / mx["total_ms"]},
         "mixed_vs_fp64_solution_max_rel_diff": mixed_vs_fp64_err,
         "spmv": spmv,
+        "matrix_free": matrix_free,
         "roofline": {"bound": "hbm", "kernel": "k_spmv_f32 (SELL-64, fp32 values, " + layout32 + ")",
                      "achieved": spmv["spmv_f32"]["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": spmv["spmv_f32"]["gbs"] / PEAK_HBM_GBS, "traffic": measured_traffic(n, layout32)},

@@ -120,6 +120,11 @@ int rsd_mul_f64(rsd_ctx* ctx, int64_t n, const double* x, const double* y, doubl
 int rsd_problem_create_block(rsd_ctx* ctx, int32_t dim, const int64_t* n, const double* ext,
                              const int32_t* p, int32_t nq, int32_t n_bc, const int32_t* bc_tags,
                              const char* const* bc_exprs, rsd_problem** out);
+/* BlockMesh2D/3D::generate on the host (no device work): xyz
+ * [n_nodes*dim], ien [n_elem*npe], rng [n_elem*2*dim] with n_nodes =
+ * prod(n_i*p+1), n_elem = prod(n_i); one basis order p[0] == p[1] (== p[2]) */
+int rsd_block_mesh(int32_t dim, const int64_t* n, const double* ext, const int32_t* p, double* xyz, int64_t* ien,
+                   int32_t* rng);
 /* same from host mesh arrays (mesh/Data.hpp:13-38 layout: xyz [nn*dim],
  * ien [ne*npe], rng [ne*2*dim]) */
 int rsd_problem_create(rsd_ctx* ctx, int32_t dim, const int32_t* p, int32_t nq, int64_t n_nodes,
@@ -141,6 +146,38 @@ int rsd_problem_layout(rsd_problem* pb, int64_t* out);
  * else the FP64 layout's mode), number of patterns, upper entries per row
  * (half storage; 0 otherwise), FP32 slots */
 int rsd_problem_layout_f32(rsd_problem* pb, int64_t* out);
+/* Binary mesh files, PMSH v1 (src/io/MeshIO.cpp:31-140; magic 0x504D5348,
+ * io/MeshIO.hpp:49-50).  Host-side, no device work.  rsd_pmsh_info:
+ * out[12] = parametric dim, spatial dim, element family (0 quad, 2 hex),
+ * basis type (0 Lagrange), n_nodes, n_elem, nodes per element, faces per
+ * element, number of basis orders, orders[0..2].  rsd_pmsh_read fills host
+ * xyz [n_nodes*sdim], ien [n_elem*npe], rng [n_elem*faces] (MeshIO::readBinary);
+ * rsd_pmsh_write writes them (MeshIO::writeBinary, no extraction operators). */
+int rsd_pmsh_info(const char* path, int64_t* out);
+int rsd_pmsh_read(const char* path, double* xyz, int64_t* ien, int32_t* rng);
+int rsd_pmsh_write(const char* path, int32_t pdim, int32_t sdim, int32_t family, int32_t basis_type,
+                   int64_t n_nodes, int64_t n_elem, int32_t npe, int32_t faces, int32_t n_orders,
+                   const int32_t* orders, const double* xyz, const int64_t* ien, const int32_t* rng);
+/* a problem on the mesh of a PMSH file (Lagrange quadrilaterals in 2D,
+ * hexahedra in 3D), as rsd_problem_create; HeatDispatcher.cpp:48 reads the
+ * mesh file this way before building the problem */
+int rsd_problem_create_pmsh(rsd_ctx* ctx, const char* path, int32_t nq, int32_t n_bc, const int32_t* bc_tags,
+                            const char* const* bc_exprs, rsd_problem** out);
+/* Nodal field files, PNDF v1 (io/fieldio/FieldIO.hpp:19-26,
+ * FieldIO.tpp:126-231).  rsd_problem_nodal_field gives the host nodal field
+ * [n_nodes] of the free device vector d_U, with the Dirichlet values at
+ * `time` on constrained nodes (FieldIO::reconstructNodalField);
+ * rsd_pndf_write writes it (FieldIO::writeBinary); rsd_pndf_read reads host
+ * nodal [n_nodes] and the time, refusing a file whose mesh tag (FNV-1a over
+ * xyz and ien, FieldIO::computeMeshTag) is not the problem's. */
+int rsd_problem_nodal_field(rsd_ctx* ctx, rsd_problem* pb, double time, const double* d_U, double* nodal);
+int rsd_pndf_write(rsd_ctx* ctx, rsd_problem* pb, double time, const double* d_U, const char* path);
+int rsd_pndf_read(rsd_problem* pb, const char* path, double* nodal, double* time);
+int rsd_problem_mesh_tag(rsd_problem* pb, uint64_t* out);
+/* d_out [n_free] = f(t, x, y, z) at each free DOF's node: an initial
+ * condition given by an expression (InitialConditionConfigParser.cpp with
+ * read mode "expression"), same grammar as the boundary expressions */
+int rsd_problem_eval_expr(rsd_ctx* ctx, rsd_problem* pb, const char* expr, double time, double* d_out);
 /* topo->alg map (-1 constrained), alg->topo, constraint tag per node */
 int rsd_problem_get_dofs(rsd_problem* pb, int32_t* topo_to_alg, int32_t* alg_to_topo, int32_t* ctag);
 /* CSR pattern as the reference's size_t arrays, widened to int64 */

@@ -206,6 +206,17 @@ __global__ void k_gval(int64_t n_nodes, int32_t dim, const double* __restrict__ 
     g[i] = v;
 }
 
+// f(t, x, y, z) at the node of every free DOF, into the free vector
+__global__ void k_eval_free(int64_t n_nodes, int32_t dim, const double* __restrict__ xyz,
+                            const int32_t* __restrict__ t2a, Expr e, double time, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_nodes) return;
+    const int32_t r = t2a[i];
+    if (r < 0) return;
+    const double x = xyz[i * dim], y = xyz[i * dim + 1], z = dim == 3 ? xyz[i * dim + 2] : 0.0;
+    out[r] = eval_expr(e, time, x, y, z, 0.0);
+}
+
 }  // namespace rsd
 
 using namespace rsd;
@@ -459,6 +470,55 @@ void rsd::build_half(rsd_problem* pb) {
     S.half_width = width;
     S.half_pad = pad;
     S.has_half = true;
+}
+
+// BlockMesh2D/3D::generate (src/mesh/generator/BlockMesh2D.cpp:6-121,
+// BlockMesh3D.cpp:6-139) with one basis order P in every direction
+static void block_mesh(int32_t dim, const int64_t* nel, const double* ext, int64_t P, std::vector<double>& xyz,
+                       std::vector<int32_t>& ien, std::vector<int32_t>& rng, int64_t* n_nodes, int64_t* n_elem) {
+    const int64_t nx = nel[0], ny = nel[1], nz = dim == 3 ? nel[2] : 1;
+    RSD_CHECK(nx > 0 && ny > 0 && nz > 0, "element counts must be positive");
+    const int64_t NX = nx * P + 1, NY = ny * P + 1, NZ = dim == 3 ? nz * P + 1 : 1;
+    const int64_t nn = NX * NY * NZ, ne = nx * ny * nz;
+    RSD_CHECK(nn < INT32_MAX, "mesh too large for int32 node ids");
+    *n_nodes = nn;
+    *n_elem = ne;
+    // nodes: x fastest (BlockMesh2D.cpp:50-71, BlockMesh3D.cpp:50-76)
+    xyz.resize(nn * dim);
+    const double dx = (ext[1] - ext[0]) / (double)(nx * P);
+    const double dy = (ext[3] - ext[2]) / (double)(ny * P);
+    const double dz = dim == 3 ? (ext[5] - ext[4]) / (double)(nz * P) : 0.0;
+    int64_t node = 0;
+    for (int64_t k = 0; k < NZ; ++k)
+        for (int64_t j = 0; j < NY; ++j)
+            for (int64_t i = 0; i < NX; ++i, ++node) {
+                xyz[dim * node] = ext[0] + (double)i * dx;
+                xyz[dim * node + 1] = ext[2] + (double)j * dy;
+                if (dim == 3) xyz[dim * node + 2] = ext[4] + (double)k * dz;
+            }
+    // connectivity, local node a -> (a % (P+1), ...) (BlockMesh3D.cpp:78-108)
+    const int npe = (int)(dim == 2 ? (P + 1) * (P + 1) : (P + 1) * (P + 1) * (P + 1));
+    const int nfaces = 2 * dim;
+    ien.resize(ne * npe);
+    rng.resize(ne * nfaces);
+    for (int64_t ez = 0; ez < nz; ++ez)
+        for (int64_t ey = 0; ey < ny; ++ey)
+            for (int64_t ex = 0; ex < nx; ++ex) {
+                const int64_t e = (ez * ny + ey) * nx + ex;
+                for (int a = 0; a < npe; ++a) {
+                    const int64_t ax = a % (P + 1), ay = (a / (P + 1)) % (P + 1), az = a / ((P + 1) * (P + 1));
+                    ien[e * npe + a] = (int32_t)(((ez * P + az) * NY + (ey * P + ay)) * NX + ex * P + ax);
+                }
+                int32_t* r = &rng[e * nfaces];
+                r[0] = ex == 0 ? 0 : -1;
+                r[1] = ex == nx - 1 ? 1 : -1;
+                r[2] = ey == 0 ? 2 : -1;
+                r[3] = ey == ny - 1 ? 3 : -1;
+                if (dim == 3) {
+                    r[4] = ez == 0 ? 4 : -1;
+                    r[5] = ez == nz - 1 ? 5 : -1;
+                }
+            }
 }
 
 extern "C" {
@@ -741,50 +801,25 @@ int rsd_problem_create_block(rsd_ctx* ctx, int32_t dim, const int64_t* nel, cons
         RSD_CHECK(ctx && nel && ext && p && out, "rsd_problem_create_block: null argument");
         auto pb = std::make_unique<rsd_problem>();
         init_problem(ctx, pb.get(), dim, p, nq, n_bc, bc_tags, bc_exprs);
-        const int64_t P = pb->p;
-        const int64_t nx = nel[0], ny = nel[1], nz = dim == 3 ? nel[2] : 1;
-        RSD_CHECK(nx > 0 && ny > 0 && nz > 0, "element counts must be positive");
-        const int64_t NX = nx * P + 1, NY = ny * P + 1, NZ = dim == 3 ? nz * P + 1 : 1;
-        pb->n_nodes = NX * NY * NZ;
-        pb->n_elem = nx * ny * nz;
-        RSD_CHECK(pb->n_nodes < INT32_MAX, "mesh too large for int32 node ids");
-        // nodes: x fastest (BlockMesh2D.cpp:50-71, BlockMesh3D.cpp:50-76)
-        pb->h_xyz.resize(pb->n_nodes * dim);
-        const double dx = (ext[1] - ext[0]) / (double)(nx * P);
-        const double dy = (ext[3] - ext[2]) / (double)(ny * P);
-        const double dz = dim == 3 ? (ext[5] - ext[4]) / (double)(nz * P) : 0.0;
-        int64_t node = 0;
-        for (int64_t k = 0; k < NZ; ++k)
-            for (int64_t j = 0; j < NY; ++j)
-                for (int64_t i = 0; i < NX; ++i, ++node) {
-                    pb->h_xyz[dim * node] = ext[0] + (double)i * dx;
-                    pb->h_xyz[dim * node + 1] = ext[2] + (double)j * dy;
-                    if (dim == 3) pb->h_xyz[dim * node + 2] = ext[4] + (double)k * dz;
-                }
-        // connectivity, local node a -> (a % (P+1), ...) (BlockMesh3D.cpp:78-108)
-        const int npe = pb->npe;
-        pb->h_ien.resize(pb->n_elem * npe);
-        pb->h_rng.resize(pb->n_elem * pb->nfaces);
-        for (int64_t ez = 0; ez < nz; ++ez)
-            for (int64_t ey = 0; ey < ny; ++ey)
-                for (int64_t ex = 0; ex < nx; ++ex) {
-                    const int64_t e = (ez * ny + ey) * nx + ex;
-                    for (int a = 0; a < npe; ++a) {
-                        const int64_t ax = a % (P + 1), ay = (a / (P + 1)) % (P + 1), az = a / ((P + 1) * (P + 1));
-                        pb->h_ien[e * npe + a] = (int32_t)(((ez * P + az) * NY + (ey * P + ay)) * NX + ex * P + ax);
-                    }
-                    int32_t* r = &pb->h_rng[e * pb->nfaces];
-                    r[0] = ex == 0 ? 0 : -1;
-                    r[1] = ex == nx - 1 ? 1 : -1;
-                    r[2] = ey == 0 ? 2 : -1;
-                    r[3] = ey == ny - 1 ? 3 : -1;
-                    if (dim == 3) {
-                        r[4] = ez == 0 ? 4 : -1;
-                        r[5] = ez == nz - 1 ? 5 : -1;
-                    }
-                }
+        block_mesh(dim, nel, ext, pb->p, pb->h_xyz, pb->h_ien, pb->h_rng, &pb->n_nodes, &pb->n_elem);
         build_problem(pb.get());
         *out = pb.release();
+    });
+}
+
+int rsd_block_mesh(int32_t dim, const int64_t* nel, const double* ext, const int32_t* p, double* xyz, int64_t* ien,
+                   int32_t* rng) {
+    return guard([&] {
+        RSD_CHECK(nel && ext && p && xyz && ien && rng, "rsd_block_mesh: null argument");
+        RSD_CHECK(dim == 2 || dim == 3, "rsd_block_mesh: dim 2 or 3");
+        for (int i = 1; i < dim; ++i) RSD_CHECK(p[i] == p[0], "rsd_block_mesh: equal basis orders per direction");
+        std::vector<double> X;
+        std::vector<int32_t> I, R;
+        int64_t nn = 0, ne = 0;
+        block_mesh(dim, nel, ext, p[0], X, I, R, &nn, &ne);
+        std::copy(X.begin(), X.end(), xyz);
+        for (size_t k = 0; k < I.size(); ++k) ien[k] = I[k];
+        std::copy(R.begin(), R.end(), rng);
     });
 }
 
@@ -807,6 +842,18 @@ int rsd_problem_create(rsd_ctx* ctx, int32_t dim, const int32_t* p, int32_t nq, 
         pb->h_rng.assign(rng, rng + n_elem * pb->nfaces);
         build_problem(pb.get());
         *out = pb.release();
+    });
+}
+
+int rsd_problem_eval_expr(rsd_ctx* ctx, rsd_problem* pb, const char* expr, double time, double* d_out) {
+    return guard([&] {
+        RSD_CHECK(ctx && pb && expr && d_out, "rsd_problem_eval_expr: null argument");
+        RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
+        const Expr e = compile_expr(expr);
+        if (pb->n_nodes)
+            hipLaunchKernelGGL(k_eval_free, dim3((unsigned)nblocks(pb->n_nodes)), dim3(kBlock), 0, ctx->stream,
+                               pb->n_nodes, pb->dim, pb->xyz.p, pb->t2a.p, e, time, d_out);
+        RSD_HIP(hipGetLastError());
     });
 }
 

@@ -220,6 +220,55 @@ def _forms(forms):
     return len(forms), kinds, coefs
 
 
+PMSH_QUAD, PMSH_HEX = 0, 2  # mesh/ElementFamily.hpp
+PMSH_LAGRANGE = 0           # mesh/BasisType.hpp
+
+
+def block_mesh(dim, n, ext, p):
+    """BlockMesh2D/3D::generate on the host: (xyz, ien int64, rng int32)."""
+    n = list(n) + [1] * (3 - dim)
+    p = list(p) + [p[0]] * (3 - dim)
+    P = p[0]
+    nn = int(np.prod([n[i] * P + 1 for i in range(dim)]))
+    ne = int(np.prod(n[:dim]))
+    npe = (P + 1) ** dim
+    xyz = np.zeros(nn * dim)
+    ien = np.zeros(ne * npe, np.int64)
+    rng = np.zeros(ne * 2 * dim, np.int32)
+    e6 = (C.c_double * 6)(*(list(ext) + [0.0] * (6 - len(ext))))
+    check(lib().rsd_block_mesh(C.c_int32(dim), (C.c_int64 * 3)(*n), e6, (C.c_int32 * 3)(*p),
+                               xyz.ctypes.data_as(_d), ien.ctypes.data_as(_i64), rng.ctypes.data_as(_i32)))
+    return xyz, ien, rng
+
+
+def read_pmsh(path):
+    """MeshIO::readBinary: a PMSH v1 file as a dict of header fields and the
+    xyz / ien / rng arrays."""
+    info = (C.c_int64 * 12)()
+    check(lib().rsd_pmsh_info(os.fsencode(path), info))
+    pdim, sdim, fam, basis, nn, ne, npe, faces, nord = [int(v) for v in info[:9]]
+    xyz = np.zeros(nn * sdim)
+    ien = np.zeros(ne * npe, np.int64)
+    rng = np.zeros(ne * faces, np.int32)
+    check(lib().rsd_pmsh_read(os.fsencode(path), xyz.ctypes.data_as(_d), ien.ctypes.data_as(_i64),
+                              rng.ctypes.data_as(_i32)))
+    return dict(pdim=pdim, sdim=sdim, family=fam, basis=basis, n_nodes=nn, n_elem=ne, npe=npe, faces=faces,
+                orders=[int(v) for v in info[9:9 + min(nord, 3)]], xyz=xyz, ien=ien, rng=rng)
+
+
+def write_pmsh(path, mesh):
+    """MeshIO::writeBinary of a dict shaped as read_pmsh returns."""
+    xyz = np.ascontiguousarray(mesh["xyz"], np.float64)
+    ien = np.ascontiguousarray(mesh["ien"], np.int64)
+    rng = np.ascontiguousarray(mesh["rng"], np.int32)
+    orders = (C.c_int32 * max(1, len(mesh["orders"])))(*mesh["orders"])
+    check(lib().rsd_pmsh_write(os.fsencode(path), C.c_int32(mesh["pdim"]), C.c_int32(mesh["sdim"]),
+                               C.c_int32(mesh["family"]), C.c_int32(mesh.get("basis", PMSH_LAGRANGE)),
+                               C.c_int64(mesh["n_nodes"]), C.c_int64(mesh["n_elem"]), C.c_int32(mesh["npe"]),
+                               C.c_int32(mesh["faces"]), C.c_int32(len(mesh["orders"])), orders,
+                               xyz.ctypes.data_as(_d), ien.ctypes.data_as(_i64), rng.ctypes.data_as(_i32)))
+
+
 class Problem:
     """rsd_problem: mesh + DOFs + pattern + element map + colouring in HBM."""
 
@@ -258,6 +307,51 @@ class Problem:
                                        C.byref(self.h)))
         self._finish()
         return self
+
+    @classmethod
+    def from_pmsh(cls, ctx, path, nq, tags=(), bc_exprs=()):
+        """A problem on the mesh of a PMSH file (HeatDispatcher's mesh file)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.dim = read_pmsh(path)["pdim"]
+        t = (C.c_int32 * max(1, len(tags)))(*tags)
+        self._exprs = _cstrs(list(bc_exprs))
+        self.h = _vp()
+        check(lib().rsd_problem_create_pmsh(ctx.h, os.fsencode(path), C.c_int32(nq), C.c_int32(len(tags)), t,
+                                            self._exprs, C.byref(self.h)))
+        self._finish()
+        return self
+
+    def write_pndf(self, path, U, time=0.0):
+        """FieldIO::writeBinary: the nodal field of the free vector U with
+        the Dirichlet values at `time`."""
+        check(lib().rsd_pndf_write(self.ctx.h, self.h, C.c_double(time), dptr(U), os.fsencode(path)))
+
+    def nodal_field(self, U, time=0.0):
+        """FieldIO::reconstructNodalField: host values at every node, the
+        Dirichlet values at `time` on constrained ones."""
+        out = np.zeros(self.n_nodes)
+        check(lib().rsd_problem_nodal_field(self.ctx.h, self.h, C.c_double(time), dptr(U), out.ctypes.data_as(_d)))
+        return out
+
+    def read_pndf(self, path):
+        """FieldIO::readBinary: (nodal values, time); the file's mesh tag must
+        be this problem's."""
+        nodal = np.zeros(self.n_nodes)
+        t = C.c_double()
+        check(lib().rsd_pndf_read(self.h, os.fsencode(path), nodal.ctypes.data_as(_d), C.byref(t)))
+        return nodal, t.value
+
+    def eval_expr(self, expr, time=0.0, out=None):
+        """The free vector of f(t, x, y, z) at the free DOFs' nodes."""
+        out = out if out is not None else self.vector()
+        check(lib().rsd_problem_eval_expr(self.ctx.h, self.h, expr.encode(), C.c_double(time), dptr(out)))
+        return out
+
+    def mesh_tag(self):
+        out = C.c_uint64()
+        check(lib().rsd_problem_mesh_tag(self.h, C.byref(out)))
+        return out.value
 
     def _finish(self):
         self.ctx._adopt(self)

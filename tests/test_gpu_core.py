@@ -6,6 +6,7 @@ block-tree reductions), so everything here must agree bit for bit; the
 reference-order oracle (itself bitwise equal to the reference's own code,
 see test_oracle.py) bounds the rounding difference.
 """
+import os
 import sys
 
 import numpy as np
@@ -606,3 +607,71 @@ def test_batched_cg_equals_standalone(ctx, use_graph):
         say("problem", i, reps[i].iterations, r1.iterations)
         assert reps[i].converged == 1 and reps[i].iterations == r1.iterations
         assert same(xs[i], x1.numpy())
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_problem_from_pmsh(ctx, tmp_path):
+    """A problem on the reference's mesh file (HeatDispatcher.cpp:48 reads
+    the mesh this way) equals the same block problem built on the device,
+    and solves the bilinear case exactly; a hex mesh written by the library
+    reads back into the same problem."""
+    import residuum_hip as rh
+    g = "(1-x)*(1-y)"
+    pf = rh.Problem.from_pmsh(ctx, os.path.join(GOLDEN, "square_n16.pmsh"), 2, [0, 1, 2, 3], [g] * 4)
+    pbk = rh.Problem(ctx, 2, (16, 16), (1, 1), 2, [0, 1, 2, 3], [g] * 4, ext=(-1.0, 1.0, -1.0, 1.0))
+    assert (pf.n_nodes, pf.n_elem, pf.n_free, pf.nnz) == (pbk.n_nodes, pbk.n_elem, pbk.n_free, pbk.nnz)
+    for a, b in zip(pf.pattern() + pf.dofs(), pbk.pattern() + pbk.dofs()):
+        assert same(a, b)
+    Kf, Kb = pf.matrix(), pbk.matrix()
+    pf.assemble_matrix(Kf, [(O.DIFFUSION, 1.0)])
+    pbk.assemble_matrix(Kb, [(O.DIFFUSION, 1.0)])
+    assert same(Kf.values(), Kb.values())
+    F = pf.vector()
+    pf.assemble_load(F)
+    pf.apply_essential(F, [(O.DIFFUSION, 1.0)])
+    x = pf.vector(0.0)
+    rep = Kf.cg(F, x, tol=1e-12, max_iters=5000)
+    xyz, _, _ = pf.mesh()
+    _, a2t, _ = pf.dofs()
+    xy = xyz.reshape(-1, 2)[a2t]
+    assert rep.converged and np.max(np.abs(x.numpy() - (1 - xy[:, 0]) * (1 - xy[:, 1]))) <= 1e-10
+    # hex mesh through the writer
+    xyz3, ien3, rng3 = O.block_mesh(3, (3, 2, 4), (0.0, 1.0, 0.0, 2.0, 0.0, 1.0), (1, 1, 1))
+    path = str(tmp_path / "cube.pmsh")
+    rh.write_pmsh(path, dict(pdim=3, sdim=3, family=rh.PMSH_HEX, n_nodes=xyz3.size // 3, n_elem=ien3.size // 8,
+                             npe=8, faces=6, orders=[1, 1, 1], xyz=xyz3, ien=ien3, rng=rng3))
+    ph = rh.Problem.from_pmsh(ctx, path, 2, [4], ["1"])
+    ph2 = rh.Problem(ctx, 3, (3, 2, 4), (1, 1, 1), 2, [4], ["1"], ext=(0.0, 1.0, 0.0, 2.0, 0.0, 1.0))
+    for a, b in zip(ph.pattern() + ph.dofs(), ph2.pattern() + ph2.dofs()):
+        assert same(a, b)
+
+
+def test_pndf_write_read(ctx, tmp_path):
+    """FieldIO::writeBinary / readBinary.  The reference's 24x24 [-1,1]^2 Q1
+    mesh carries the mesh tag 17859353126579886777 in the headers of its
+    .pndf files (examples/heateq/steady/file_mode); the values are the free
+    vector on free nodes and g(x, t) on constrained ones."""
+    import struct
+    import residuum_hip as rh
+    pb = rh.Problem(ctx, 2, (24, 24), (1, 1), 2, [0, 1, 2, 3], ["x+2*y+t"] * 4, ext=(-1.0, 1.0, -1.0, 1.0))
+    assert pb.mesh_tag() == 17859353126579886777
+    U = np.random.default_rng(1).standard_normal(pb.n_free)
+    path = str(tmp_path / "u.pndf")
+    pb.write_pndf(path, ctx.from_numpy(U), time=0.5)
+    raw = open(path, "rb").read()
+    magic, ver, nn, nd, t, tag = struct.unpack("<IIQIdQ", raw[:36])
+    assert (magic, ver, nn, nd, t, tag) == (0x504E4446, 1, pb.n_nodes, 1, 0.5, 17859353126579886777)
+    vals = np.frombuffer(raw[36:], "<f8")
+    xyz, _, _ = pb.mesh()
+    t2a, a2t, _ = pb.dofs()
+    xy = xyz.reshape(-1, 2)
+    assert same(vals[a2t], U)
+    c = t2a < 0
+    assert same(vals[c], xy[c, 0] + 2 * xy[c, 1] + 0.5)
+    nodal, tt = pb.read_pndf(path)
+    assert same(nodal, vals) and tt == 0.5
+    other = rh.Problem(ctx, 2, (24, 24), (1, 1), 2, [0], ["0"])
+    with pytest.raises(rh.RsdError, match="mesh tag"):
+        other.read_pndf(path)
