@@ -153,6 +153,17 @@ int orc_apply_essential(const orc_problem* P, double time, int32_t n_forms, cons
 int orc_assemble_diagonal(const orc_problem* P, double time, int32_t n_forms, const int32_t* kinds,
                           const double* coefs, const double* U, double* diag);
 
+/* QuantityEvaluator<CPU>::evaluateBoundaryReaction
+ * (fem/quantity/backend/cpu/QuantityEvaluator.tpp:95-199) with the weak
+ * heat-flux pairs of HeatProblem.tpp:385-410: per registered tag t, the sum
+ * of the element vectors (source - K(U) U [- M Udot when Udot != NULL]) at
+ * the nodes of the element's faces tagged t (each node once per element and
+ * tag), in ascending element order, and the faces' measure.  U is the free
+ * vector (Dirichlet values from the BCs at `time`); rng [n_elem * 2 dim]. */
+int orc_boundary_reaction(const orc_problem* P, double time, const void* src_expr, const double* U,
+                          const double* Udot, const int32_t* rng, int32_t n_tags, const int32_t* tags,
+                          double* result, double* measure);
+
 /* ------------------------------------------------------------------ */
 /* linear algebra                                                       */
 /* ------------------------------------------------------------------ */

@@ -677,3 +677,119 @@ void orc__element_matrix(const orc_problem* P, int64_t e, double time, int32_t n
         for (int f = 0; f < n_forms; ++f) form_matrix(P, kinds[f], coefs[f], &Q, Ke);
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* weak boundary reaction (QuantityEvaluator<CPU>::evaluateBoundaryReaction,
+ * fem/quantity/backend/cpu/QuantityEvaluator.tpp:95-199) with the heat
+ * problem's pairs (HeatProblem.tpp:385-410): source on the value, -1 x
+ * stiffness on the value, -1 x mass on the rate (transient only)          */
+/* ------------------------------------------------------------------ */
+/* face quadrature (GaussQuadrature1D / GaussQuadratureQuad, x fastest),
+ * mapFaceToElement and getFaceTopology (LagrangeQuad.tpp:95-181,
+ * LagrangeHex.tpp:120-267) */
+static int face_points(const orc_problem* P, int f, double* xi_elem, double* w, double* nref) {
+    double x[MAXQ1D], ww[MAXQ1D];
+    const int d = P->dim, nq = P->nq[0];
+    gauss1d(nq, x, ww);
+    for (int i = 0; i < d; ++i) nref[i] = 0.0;
+    nref[f / 2] = (f % 2) ? 1.0 : -1.0;
+    const double side = (f % 2) ? 1.0 : -1.0;
+    int q = 0;
+    if (d == 2) {
+        for (int i = 0; i < nq; ++i, ++q) {
+            const double s = x[i];
+            xi_elem[2 * q] = f < 2 ? side : s;
+            xi_elem[2 * q + 1] = f < 2 ? s : side;
+            w[q] = ww[i];
+        }
+        return q;
+    }
+    for (int j = 0; j < nq; ++j)
+        for (int i = 0; i < nq; ++i, ++q) {
+            const double s = x[i], t = x[j];
+            double* e = xi_elem + 3 * q;
+            if (f < 2) { e[0] = side; e[1] = s; e[2] = t; }
+            else if (f < 4) { e[0] = s; e[1] = side; e[2] = t; }
+            else { e[0] = s; e[1] = t; e[2] = side; }
+            w[q] = ww[i] * ww[j];
+        }
+    return q;
+}
+
+/* JacobianTransform::computeBoundaryNormal for ParametricDim == SpatialDim
+ * (JacobianTransform.tpp:110-123) with computeMatrixCofactor (:209-233) */
+static void boundary_normal(int d, const double* A, const double* nref, double* n) {
+    double c[9];
+    if (d == 2) {
+        c[0] = A[3]; c[1] = -A[2]; c[2] = -A[1]; c[3] = A[0];
+    } else {
+        c[0] = (A[4] * A[8] - A[5] * A[7]);
+        c[1] = -(A[3] * A[8] - A[5] * A[6]);
+        c[2] = (A[3] * A[7] - A[4] * A[6]);
+        c[3] = -(A[1] * A[8] - A[2] * A[7]);
+        c[4] = (A[0] * A[8] - A[2] * A[6]);
+        c[5] = -(A[0] * A[7] - A[1] * A[6]);
+        c[6] = (A[1] * A[5] - A[2] * A[4]);
+        c[7] = -(A[0] * A[5] - A[2] * A[3]);
+        c[8] = (A[0] * A[4] - A[1] * A[3]);
+    }
+    for (int i = 0; i < d; ++i) {
+        n[i] = 0.0;
+        for (int j = 0; j < d; ++j) n[i] += c[i * d + j] * nref[j];
+    }
+}
+
+int orc_boundary_reaction(const orc_problem* P, double time, const void* src_expr, const double* U,
+                          const double* Udot, const int32_t* rng, int32_t n_tags, const int32_t* tags,
+                          double* result, double* measure) {
+    double xi[3 * 125], w[125], coords[3 * MAXNPE], Oe[MAXNPE], Uv[MAXNPE], Ur[MAXNPE];
+    double fxi[3 * 25], fw[25], nref[3];
+    int64_t fids[MAXNPE];
+    int claimed[MAXNPE];
+    const int nq = quad_points(P, xi, w), n = P->npe, d = P->dim, nf = 2 * d;
+    for (int t = 0; t < n_tags; ++t) { result[t] = 0.0; measure[t] = 0.0; }
+    for (int64_t e = 0; e < P->n_elem; ++e) {
+        const int32_t* rt = rng + e * nf;
+        int touches = 0;
+        for (int f = 0; f < nf; ++f)
+            for (int t = 0; t < n_tags; ++t) touches |= rt[f] == tags[t];
+        if (!touches) continue;
+        const int64_t* nodes = P->ien + e * n;
+        for (int a = 0; a < n; ++a) { Oe[a] = 0.0; claimed[a] = -1; }
+        element_coords(P, nodes, coords);
+        gather(P, nodes, coords, 1, time, U, Uv);
+        gather(P, nodes, coords, 0, time, Udot, Ur);
+        for (int q = 0; q < nq; ++q) {
+            qp_t Q;
+            qp_evaluate(P, coords, &xi[d * q], w[q], &Q);
+            qp_interpolate(P, Uv, Ur, &Q);
+            qp_models(P, e, &Q);
+            /* SourceForm (SourceForm.hpp:22-32) */
+            const double fval = src_expr ? orc_expr_eval(src_expr, time, Q.x[0], Q.x[1], d == 3 ? Q.x[2] : 0.0, 0.0) : 0.0;
+            for (int a = 0; a < n; ++a) Oe[a] += fval * Q.N[a] * Q.measure * Q.w;
+            form_vector(P, ORC_FORM_DIFFUSION | ORC_SCALED, -1.0, &Q, Uv, Oe);
+            if (Udot) form_vector(P, ORC_FORM_MASS | ORC_SCALED, -1.0, &Q, Ur, Oe);
+        }
+        for (int f = 0; f < nf; ++f) {
+            int t = -1;
+            for (int k = 0; k < n_tags; ++k) if (rt[f] == tags[k]) t = k;
+            if (t < 0) continue;
+            const int fq = face_points(P, f, fxi, fw, nref);
+            for (int q = 0; q < fq; ++q) {
+                qp_t Q;
+                qp_evaluate(P, coords, &fxi[d * q], fw[q], &Q);
+                double nv[3], mag = 0.0;
+                boundary_normal(d, Q.J, nref, nv);
+                for (int s = 0; s < d; ++s) mag += nv[s] * nv[s];
+                measure[t] += sqrt(mag) * fw[q];
+            }
+            const int nfn = face_nodes(d, P->p, f, fids);
+            for (int i = 0; i < nfn; ++i)
+                if (claimed[fids[i]] != rt[f]) {
+                    result[t] += Oe[fids[i]];
+                    claimed[fids[i]] = rt[f];
+                }
+        }
+    }
+    return 0;
+}

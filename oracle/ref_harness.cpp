@@ -54,6 +54,10 @@
 #include "linalg/types/CSRMatrix.hpp"
 #include "linalg/operator/CSROperator.hpp"
 #include "linalg/operator/FEMOperator.hpp"
+#include "fem/quadrature/GaussQuadrature1D.hpp"
+#include "fem/quantity/BoundaryQuantityRegistry.hpp"
+#include "fem/quantity/QuantityEvaluator.hpp"
+#include "equation/heateq/evaluator/EvalQuadraturePointBoundary.hpp"
 #include "linalg/solver/iterative/cg/Solver.hpp"
 #include "linalg/solver/iterative/gmres/Solver.hpp"
 #include "linalg/solver/preconditioner/Jacobi.hpp"
@@ -158,6 +162,8 @@ struct Base {
     virtual void load(int kind, const Real* nodal, Real* F) = 0;
     virtual void lift(int set, Real coef, Real* F) = 0;
     virtual void assemble_diagonal(int set, Real coef, const Real* U, Real* d) = 0;
+    virtual void boundary_reaction(Real time, int src_kind, const Real* U, const Real* Udot, int n_tags,
+                                   const int32_t* tags, Real* result, Real* measure) = 0;
     virtual int cg_fem(int set, Real coef, const Real* U, int pc, const Real* rhs, Real* x, Real tol, int64_t maxit,
                        int64_t* iters) = 0;
     virtual int be_run(Real t0, Real tf, Real dt, Real tol, int64_t maxit, Real* U, int64_t* iters) = 0;
@@ -318,6 +324,43 @@ struct Ctx : Base {
             asmb.template assembleDiagonal<1, EvalEle, QP, OpModel, BEForms, Quad, State>(mesh, *topo, 0.0, om, f, ele, quad, st, out, &bcs);
         }
         for (Index i = 0; i < out.size(); ++i) d[i] = out.data()[i];
+    }
+
+    // QuantityEvaluator<CPU>::evaluateBoundaryReaction (QuantityEvaluator.tpp:95-199) with the
+    // weak heat-flux pairs of HeatProblem.tpp:385-410 (source, -stiffness, -mass on the rate)
+    void boundary_reaction(Real time, int src_kind, const Real* U, const Real* Udot, int n_tags, const int32_t* tags,
+                           Real* result, Real* measure) override {
+        using QuadBdy = std::conditional_t<D == 2, fem::quadrature::GaussQuadrature1D, fem::quadrature::GaussQuadratureQuad>;
+        using QPB = equation::heateq::evaluator::EvalQuadraturePointBoundary<EvalEle, Basis, fem::geometry::JacobianTransform<D, D>>;
+        using SF = equation::heateq::SourceForm<QP, equation::heateq::evaluator::SourceFunction<D, 1, KindFn<D>>>;
+        using SFR = fem::form::FormRegistry<SF>;
+        using NegStiff = fem::form::ScaledForm<1, StiffForms>;
+        using NegMass = fem::form::ScaledForm<1, MassForms>;
+        VecT u = vec_from(U), ud = vec_from(Udot);
+        State st; st.valueSource = &u; st.rateSource = &ud;
+        auto selV = [](State& s) -> Real* { return s.valueBuffer(); };
+        auto selR = [](State& s) -> Real* { return s.rateBuffer(); };
+        using SP = fem::assembly::FormsWithOperand<SFR, decltype(selV)>;
+        using KP = fem::assembly::FormsWithOperand<NegStiff, decltype(selV)>;
+        using MP = fem::assembly::FormsWithOperand<NegMass, decltype(selR)>;
+        KindFn<D> fn; fn.kind = src_kind;
+        SFR sf{SF{fn}};
+        Cond c = cond();
+        OpModel om(c, mass_model());
+        const Index nb = quad.numPointsTotal() == 0 ? 1 : (Index)std::llround(std::pow((double)quad.numPointsTotal(), 1.0 / D));
+        QuadBdy qb = [&] { if constexpr (D == 2) return QuadBdy(nb); else return QuadBdy(nb, nb); }();
+        fem::quantity::BoundaryQuantityRegistry<1> reg;
+        for (int k = 0; k < n_tags; ++k) reg.registerTag(tags[k]);
+        if (Udot)
+            fem::quantity::QuantityEvaluator<Backend>::template evaluateBoundaryReaction<1, EvalEle, QP, QPB, OpModel, Quad, QuadBdy, State, SP, KP, MP>(
+                mesh, *topo, bcs, time, om, ele, quad, qb, st, reg, SP{sf, selV}, KP{NegStiff(Real(-1))}, MP{NegMass(Real(-1))});
+        else
+            fem::quantity::QuantityEvaluator<Backend>::template evaluateBoundaryReaction<1, EvalEle, QP, QPB, OpModel, Quad, QuadBdy, State, SP, KP>(
+                mesh, *topo, bcs, time, om, ele, quad, qb, st, reg, SP{sf, selV}, KP{NegStiff(Real(-1))});
+        for (int k = 0; k < n_tags; ++k) {
+            result[k] = reg.result(tags[k])[0];
+            measure[k] = reg.entries().at(tags[k]).measureSum;
+        }
     }
 
     // cg::Solver over the matrix-free FEMOperator (FEMOperator.hpp:27-71); set 0 diffusion, 2 BE
@@ -594,6 +637,14 @@ int ref_cg_fem(void* h, int set, double coef, const double* U, const double* kap
     const int r = b->cg_fem(set, coef, U, pc, rhs, x, tol, maxit, iters);
     g_kappa = nullptr;
     return r;
+}
+
+void ref_boundary_reaction(void* h, double time, int src_kind, const double* U, const double* Udot,
+                           const double* kappa, int n_tags, const int32_t* tags, double* result, double* measure) {
+    Base* b = static_cast<Base*>(h);
+    with_kappa(b, kappa);
+    b->boundary_reaction(time, src_kind, U, Udot, n_tags, tags, result, measure);
+    g_kappa = nullptr;
 }
 
 int ref_be_run(void* h, double t0, double tf, double dt, double tol, int64_t maxit, const double* kappa,

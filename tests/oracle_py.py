@@ -249,6 +249,16 @@ class Case:
         lib().orc_apply_essential(C.byref(self.P), C.c_double(time), len(forms), ip(kinds), dp(coefs), dp(F))
         return F
 
+    def boundary_reaction(self, tags, U, Udot=None, src=None, time=0.0):
+        """Weak boundary reaction per tag (sum, face measure), ascending
+        element order."""
+        t = np.array(tags, np.int32)
+        res, meas = np.zeros(len(t)), np.zeros(len(t))
+        e = Expr(src) if src else None
+        lib().orc_boundary_reaction(C.byref(self.P), C.c_double(time), C.c_void_p(e.ptr) if e else None, dp(U),
+                                    dp(Udot), ip(self.rng), len(t), ip(t), dp(res), dp(meas))
+        return res, meas
+
     def assemble_diagonal(self, forms, U=None, time=0.0):
         kinds = np.array([f[0] for f in forms], np.int32)
         coefs = np.array([f[1] for f in forms], np.float64)
@@ -440,6 +450,13 @@ class RefCase:
         F = F.copy()
         ref().ref_lift(self.h, C.c_int(kind_set), C.c_double(coef), dp(kappa), dp(F))
         return F
+
+    def boundary_reaction(self, tags, U, Udot=None, src_kind=0, time=0.0):
+        t = np.array(tags, np.int32)
+        res, meas = np.zeros(len(t)), np.zeros(len(t))
+        ref().ref_boundary_reaction(self.h, C.c_double(time), C.c_int(src_kind), dp(U), dp(Udot), None,
+                                    C.c_int(len(t)), ip(t), dp(res), dp(meas))
+        return res, meas
 
     def assemble_diagonal(self, kind_set, coef=1.0, U=None, kappa=None):
         d = np.zeros(self.n_free)

@@ -366,3 +366,52 @@ def test_live_reference_matrix_free(spec):
         x, rep = c.cg_fem(BE_FORMS, b, tol=1e-10, max_iters=500, precond=pc)
         xr, ok, it = r.cg_fem(2, b, coef=40.0, kappa=kap, precond=pc, tol=1e-10, max_iters=500)
         assert ok and rep.converged == 1 and rep.iterations == it and same(x, xr)
+
+
+# ---------------------------------------------------------------------------
+# weak boundary reaction (QuantityEvaluator.tpp:95-199)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dim,n,p,ext,a_axis", [
+    (2, (4, 4), (1, 1), (-1.0, 1.0, -1.0, 1.0), 0),
+    (2, (3, 5), (2, 2), (0.0, 2.0, 0.0, 1.0), 1),
+    (3, (3, 2, 2), (1, 1, 1), (0.0, 1.0, 0.0, 2.0, 0.0, 3.0), 2),
+])
+def test_boundary_reaction_linear_field(dim, n, p, ext, a_axis):
+    """The reference's heat-flux fixture (tests/unit/equation/heateq/
+    QuantityEvaluator.cpp:20-100: k = 2, T = a x + b y (+ c z), no
+    essential BCs): with no source the weak reaction on the face x_i = min
+    is k * dT/dx_i times the face's measure (the corner terms of the
+    adjacent faces cancel), and the measure is the face's length/area."""
+    c = O.Case(dim, n, p, p[0] + 1, [], [], ext=ext, k_scalar=2.0)
+    xyz = c.xyz.reshape(-1, dim)
+    grad = [1.0, 2.0, -0.5][:dim]
+    U = (xyz[c.a2t] @ np.array(grad))
+    tag = 2 * a_axis  # the face at the low end of that axis
+    res, meas = c.boundary_reaction([tag], U)
+    lens = [ext[2 * i + 1] - ext[2 * i] for i in range(dim)]
+    area = float(np.prod([lens[i] for i in range(dim) if i != a_axis]))
+    assert abs(meas[0] - area) < 1e-12 * area
+    assert abs(res[0] - 2.0 * grad[a_axis] * area) < 1e-12 * max(1.0, abs(2.0 * grad[a_axis] * area))
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref/libref.so not built (needs /root/reference)")
+@pytest.mark.parametrize("spec", [
+    (2, (6, 4), (1, 1), 2, [0, 2], [7, 7], None),
+    (2, (3, 4), (2, 2), 3, [1, 3], [2, 2], None),
+    (3, (3, 2, 4), (1, 1, 1), 2, [4, 5], [6, 0], None),
+])
+def test_live_reference_boundary_reaction(spec):
+    """evaluateBoundaryReaction with the weak heat-flux pairs, oracle against
+    the reference's own QuantityEvaluator: bit for bit, steady and
+    transient, Dirichlet and free tags together."""
+    dim, n, p, nq, tags, kinds, ext = spec
+    c = O.Case(dim, n, p, nq, tags, [O.KIND_EXPR[k] for k in kinds], ext=ext)
+    r = O.RefCase(dim, n, p, nq, tags, kinds, ext=ext)
+    rng = np.random.default_rng(11)
+    U, Ud = rng.standard_normal(c.n_free), rng.standard_normal(c.n_free)
+    all_tags = list(range(2 * dim))
+    for src_kind in (0, 5 if dim == 3 else 4):
+        for Udot in (None, Ud):
+            got = c.boundary_reaction(all_tags, U, Udot=Udot, src=O.KIND_EXPR[src_kind], time=0.25)
+            want = r.boundary_reaction(all_tags, U, Udot=Udot, src_kind=src_kind, time=0.25)
+            assert same(got[0], want[0]) and same(got[1], want[1])
