@@ -331,6 +331,33 @@ int rsd_cg_solve_fem(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t n_forms
                      const double* b, double* x, double tol, int32_t tol_type, int64_t max_iters,
                      int32_t precond, rsd_report* rep, double* hist);
 
+/* Weak boundary reaction per tag: QuantityEvaluator<CPU>::
+ * evaluateBoundaryReaction (fem/quantity/backend/cpu/QuantityEvaluator.tpp:
+ * 95-199) with the heat problem's pairs (HeatProblem.tpp:385-410): source
+ * src_expr (nullable) on the value, -1 x stiffness on the value, -1 x mass on
+ * d_Udot (nullable: steady).  d_U / d_Udot are device free vectors [n_free];
+ * Dirichlet nodes take their values at `time`.  result[t] (host) receives
+ * the sum of the element residual over the nodes of the faces tagged
+ * tags[t], measure[t] (host) the faces' measure, both bitwise the reference's
+ * serial sums (BoundaryQuantityRegistry::result / measureSum,
+ * BoundaryQuantityRegistry.hpp:17-50).  The average a monitor reports is
+ * result / measure.  The plan for a tag list is built on first use and
+ * cached on the problem. */
+int rsd_boundary_reaction(rsd_ctx* ctx, rsd_problem* pb, double time, const char* src_expr,
+                          const double* d_U, const double* d_Udot, int32_t n_tags,
+                          const int32_t* tags, double* result, double* measure);
+/* Strong (pointwise) boundary heat flux per tag: QuantityEvaluator<CPU>::
+ * evaluateBoundaryRegistry (fem/quantity/backend/cpu/QuantityEvaluator.tpp:
+ * 14-93) with HeatFluxIntegrand (equation/heateq/quantity/
+ * HeatFluxIntegrand.hpp:19-38), Integral reduction: result[t] (host) = the
+ * sum over the points of the faces tagged tags[t] of -(K gradT).n w (n the
+ * unnormalised cofactor normal, T the full gather of d_U at `time`, K =
+ * k(T) x shape), measure[t] (host) the faces' measure, both bitwise the
+ * reference's serial sums.  The Average reduction (QuantityForms::finalize)
+ * is result / measure. */
+int rsd_boundary_flux(rsd_ctx* ctx, rsd_problem* pb, double time, const double* d_U, int32_t n_tags,
+                      const int32_t* tags, double* result, double* measure);
+
 /* ---------------------------------------------------------------------- */
 /* drivers                                                                  */
 /* ---------------------------------------------------------------------- */

@@ -163,6 +163,13 @@ int orc_assemble_diagonal(const orc_problem* P, double time, int32_t n_forms, co
 int orc_boundary_reaction(const orc_problem* P, double time, const void* src_expr, const double* U,
                           const double* Udot, const int32_t* rng, int32_t n_tags, const int32_t* tags,
                           double* result, double* measure);
+/* QuantityEvaluator<CPU>::evaluateBoundaryRegistry (QuantityEvaluator.tpp:
+ * 14-93) with HeatFluxIntegrand (HeatFluxIntegrand.hpp:19-38), Integral
+ * reduction: per tag the sum over its faces' points of -(K gradT).n w, n the
+ * unnormalised cofactor normal, in (element, face, point) order, and the
+ * faces' measure.  T is the full gather of U at `time`. */
+int orc_boundary_flux(const orc_problem* P, double time, const double* U, const int32_t* rng, int32_t n_tags,
+                      const int32_t* tags, double* result, double* measure);
 
 /* ------------------------------------------------------------------ */
 /* linear algebra                                                       */

@@ -793,3 +793,56 @@ int orc_boundary_reaction(const orc_problem* P, double time, const void* src_exp
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* strong (pointwise) boundary heat flux: QuantityEvaluator<CPU>::
+ * evaluateBoundaryRegistry (fem/quantity/backend/cpu/QuantityEvaluator.tpp:
+ * 14-93) with HeatFluxIntegrand (equation/heateq/quantity/
+ * HeatFluxIntegrand.hpp:19-38) at the boundary points of
+ * EvalQuadraturePointBoundary::evaluate (equation/heateq/evaluator/
+ * EvalQuadraturePointBoundary.hpp:73-101): per face point the full-gathered
+ * T gives gradT and k(T), and the term -(K gradT).n_cof * w; the measure
+ * terms as in the weak reaction.  Integral reduction (the Average division
+ * by the measure is the caller's).                                        */
+/* ------------------------------------------------------------------ */
+int orc_boundary_flux(const orc_problem* P, double time, const double* U, const int32_t* rng, int32_t n_tags,
+                      const int32_t* tags, double* result, double* measure) {
+    double coords[3 * MAXNPE], Uv[MAXNPE], zero[MAXNPE];
+    double fxi[3 * 25], fw[25], nref[3];
+    const int n = P->npe, d = P->dim, nf = 2 * d;
+    for (int a = 0; a < n; ++a) zero[a] = 0.0;
+    for (int t = 0; t < n_tags; ++t) { result[t] = 0.0; measure[t] = 0.0; }
+    for (int64_t e = 0; e < P->n_elem; ++e) {
+        const int32_t* rt = rng + e * nf;
+        int touches = 0;
+        for (int f = 0; f < nf; ++f)
+            for (int t = 0; t < n_tags; ++t) touches |= rt[f] == tags[t];
+        if (!touches) continue;
+        const int64_t* nodes = P->ien + e * n;
+        element_coords(P, nodes, coords);
+        gather(P, nodes, coords, 1, time, U, Uv);
+        for (int f = 0; f < nf; ++f) {
+            int t = -1;
+            for (int k = 0; k < n_tags; ++k) if (rt[f] == tags[k]) t = k;
+            if (t < 0) continue;
+            const int fq = face_points(P, f, fxi, fw, nref);
+            for (int q = 0; q < fq; ++q) {
+                qp_t Q;
+                qp_evaluate(P, coords, &fxi[d * q], fw[q], &Q);
+                qp_interpolate(P, Uv, zero, &Q);
+                qp_models(P, e, &Q);
+                double nv[3], mag = 0.0, flux = 0.0;
+                boundary_normal(d, Q.J, nref, nv);
+                for (int i = 0; i < d; ++i) {
+                    double kg = 0.0;
+                    for (int j = 0; j < d; ++j) kg += Q.K[i * d + j] * Q.gradT[j];
+                    flux -= kg * nv[i];
+                }
+                result[t] += flux * fw[q];
+                for (int s = 0; s < d; ++s) mag += nv[s] * nv[s];
+                measure[t] += sqrt(mag) * fw[q];
+            }
+        }
+    }
+    return 0;
+}

@@ -58,6 +58,9 @@
 #include "fem/quantity/BoundaryQuantityRegistry.hpp"
 #include "fem/quantity/QuantityEvaluator.hpp"
 #include "equation/heateq/evaluator/EvalQuadraturePointBoundary.hpp"
+#include "equation/heateq/evaluator/HeatBoundaryStateVector.hpp"
+#include "equation/heateq/quantity/HeatFluxIntegrand.hpp"
+#include "fem/quantity/QuantityForms.hpp"
 #include "linalg/solver/iterative/cg/Solver.hpp"
 #include "linalg/solver/iterative/gmres/Solver.hpp"
 #include "linalg/solver/preconditioner/Jacobi.hpp"
@@ -164,6 +167,8 @@ struct Base {
     virtual void assemble_diagonal(int set, Real coef, const Real* U, Real* d) = 0;
     virtual void boundary_reaction(Real time, int src_kind, const Real* U, const Real* Udot, int n_tags,
                                    const int32_t* tags, Real* result, Real* measure) = 0;
+    virtual void boundary_flux(Real time, const Real* U, int n_tags, const int32_t* tags, Real* result,
+                               Real* measure) = 0;
     virtual int cg_fem(int set, Real coef, const Real* U, int pc, const Real* rhs, Real* x, Real tol, int64_t maxit,
                        int64_t* iters) = 0;
     virtual int be_run(Real t0, Real tf, Real dt, Real tol, int64_t maxit, Real* U, int64_t* iters) = 0;
@@ -357,6 +362,33 @@ struct Ctx : Base {
         else
             fem::quantity::QuantityEvaluator<Backend>::template evaluateBoundaryReaction<1, EvalEle, QP, QPB, OpModel, Quad, QuadBdy, State, SP, KP>(
                 mesh, *topo, bcs, time, om, ele, quad, qb, st, reg, SP{sf, selV}, KP{NegStiff(Real(-1))});
+        for (int k = 0; k < n_tags; ++k) {
+            result[k] = reg.result(tags[k])[0];
+            measure[k] = reg.entries().at(tags[k]).measureSum;
+        }
+    }
+
+    // QuantityEvaluator<CPU>::evaluateBoundaryRegistry (QuantityEvaluator.tpp:14-93) with
+    // HeatFluxIntegrand, Integral reduction, on the boundary state HeatBoundaryStateVector
+    void boundary_flux(Real time, const Real* U, int n_tags, const int32_t* tags, Real* result,
+                       Real* measure) override {
+        using QuadBdy = std::conditional_t<D == 2, fem::quadrature::GaussQuadrature1D, fem::quadrature::GaussQuadratureQuad>;
+        using QPB = equation::heateq::evaluator::EvalQuadraturePointBoundary<EvalEle, Basis, fem::geometry::JacobianTransform<D, D>>;
+        using BState = equation::heateq::evaluator::HeatBoundaryStateVector<1, D, D, VecT>;
+        using CondB = HarnessConductivity<QPB>;
+        using Flux = equation::heateq::quantity::HeatFluxIntegrand<QPB>;
+        using Qs = fem::quantity::QuantityForms<fem::quantity::ReducedQuantity<Flux, fem::quantity::Reduction::Integral>>;
+        VecT u = vec_from(U);
+        BState st; st.valueSource = &u;
+        CondB c; c.tdep = tdep; c.ka = ka; c.kb = kb; c.aniso = aniso;
+        for (int i = 0; i < 9; ++i) c.tensor[i] = tensor[i];
+        Qs qf{Flux{}};
+        const Index nb = quad.numPointsTotal() == 0 ? 1 : (Index)std::llround(std::pow((double)quad.numPointsTotal(), 1.0 / D));
+        QuadBdy qb = [&] { if constexpr (D == 2) return QuadBdy(nb); else return QuadBdy(nb, nb); }();
+        fem::quantity::BoundaryQuantityRegistry<1> reg;
+        for (int k = 0; k < n_tags; ++k) reg.registerTag(tags[k]);
+        fem::quantity::QuantityEvaluator<Backend>::template evaluateBoundaryRegistry<1, EvalEle, QPB, CondB, Qs, QuadBdy, BState>(
+            mesh, *topo, bcs, time, c, qf, ele, qb, st, reg);
         for (int k = 0; k < n_tags; ++k) {
             result[k] = reg.result(tags[k])[0];
             measure[k] = reg.entries().at(tags[k]).measureSum;
@@ -645,6 +677,13 @@ void ref_boundary_reaction(void* h, double time, int src_kind, const double* U, 
     with_kappa(b, kappa);
     b->boundary_reaction(time, src_kind, U, Udot, n_tags, tags, result, measure);
     g_kappa = nullptr;
+}
+
+void ref_boundary_flux(void* h, double time, const double* U, int n_tags, const int32_t* tags, double* result,
+                       double* measure) {
+    Base* b = static_cast<Base*>(h);
+    with_kappa(b, nullptr);
+    b->boundary_flux(time, U, n_tags, tags, result, measure);
 }
 
 int ref_be_run(void* h, double t0, double tf, double dt, double tol, int64_t maxit, const double* kappa,

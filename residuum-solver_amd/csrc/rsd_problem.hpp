@@ -101,6 +101,10 @@ struct Sell {
 
 }  // namespace rsd
 
+namespace rsd {
+struct ReactionPlan;  // boundary-reaction plan, built on first use per tag list
+}
+
 struct rsd_problem {
     rsd_ctx* ctx = nullptr;
     int32_t dim = 0, p = 1, nq = 2, npe = 0, nfaces = 0;
@@ -138,6 +142,7 @@ struct rsd_problem {
     // caches for compiled source expressions
     std::string src_text;
     rsd::Expr src_expr;
+    std::shared_ptr<rsd::ReactionPlan> reaction;
 
     const double* kappa_ptr() const { return kappa_ext ? kappa_ext : (kappa.n ? kappa.p : nullptr); }
     int n_colours() const { return (int)colour_off.size() - 1; }

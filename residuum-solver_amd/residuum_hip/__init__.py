@@ -470,6 +470,27 @@ class Problem:
         check(lib().rsd_assemble_diagonal(self.ctx.h, self.h, C.c_double(time), C.c_int32(nf), kinds, coefs,
                                           dptr(U), dptr(diag)))
 
+    def boundary_reaction(self, tags, U, Udot=None, src=None, time=0.0):
+        """QuantityEvaluator::evaluateBoundaryReaction with the heat pairs
+        (source on U, -stiffness on U, -mass on Udot when given): returns
+        (result, measure) per tag; the monitor's average is result / measure."""
+        t = np.ascontiguousarray(tags, np.int32)
+        res, meas = np.zeros(len(t)), np.zeros(len(t))
+        check(lib().rsd_boundary_reaction(self.ctx.h, self.h, C.c_double(time), src.encode() if src else None,
+                                          dptr(U), dptr(Udot), C.c_int32(len(t)), t.ctypes.data_as(_i32),
+                                          res.ctypes.data_as(_d), meas.ctypes.data_as(_d)))
+        return res, meas
+
+    def boundary_flux(self, tags, U, time=0.0):
+        """QuantityEvaluator::evaluateBoundaryRegistry with HeatFluxIntegrand
+        (strong flux): returns (integral, measure) per tag; the Average
+        reduction is integral / measure."""
+        t = np.ascontiguousarray(tags, np.int32)
+        res, meas = np.zeros(len(t)), np.zeros(len(t))
+        check(lib().rsd_boundary_flux(self.ctx.h, self.h, C.c_double(time), dptr(U), C.c_int32(len(t)),
+                                      t.ctypes.data_as(_i32), res.ctypes.data_as(_d), meas.ctypes.data_as(_d)))
+        return res, meas
+
     def fem_apply(self, y, forms, x, U=None, Udot=None, time=0.0):
         """FEMOperator::apply: y = forms applied to the free vector x."""
         self.assemble_vector(y, forms, U=U, Udot=Udot, operand=x, selector=SEL_OPERAND, time=time)

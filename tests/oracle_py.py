@@ -249,6 +249,14 @@ class Case:
         lib().orc_apply_essential(C.byref(self.P), C.c_double(time), len(forms), ip(kinds), dp(coefs), dp(F))
         return F
 
+    def boundary_flux(self, tags, U, time=0.0):
+        """Strong boundary heat flux per tag (integral, face measure)."""
+        t = np.array(tags, np.int32)
+        res, meas = np.zeros(len(t)), np.zeros(len(t))
+        lib().orc_boundary_flux(C.byref(self.P), C.c_double(time), dp(U), ip(self.rng), len(t), ip(t), dp(res),
+                                dp(meas))
+        return res, meas
+
     def boundary_reaction(self, tags, U, Udot=None, src=None, time=0.0):
         """Weak boundary reaction per tag (sum, face measure), ascending
         element order."""
@@ -450,6 +458,12 @@ class RefCase:
         F = F.copy()
         ref().ref_lift(self.h, C.c_int(kind_set), C.c_double(coef), dp(kappa), dp(F))
         return F
+
+    def boundary_flux(self, tags, U, time=0.0):
+        t = np.array(tags, np.int32)
+        res, meas = np.zeros(len(t)), np.zeros(len(t))
+        ref().ref_boundary_flux(self.h, C.c_double(time), dp(U), C.c_int(len(t)), ip(t), dp(res), dp(meas))
+        return res, meas
 
     def boundary_reaction(self, tags, U, Udot=None, src_kind=0, time=0.0):
         t = np.array(tags, np.int32)

@@ -268,6 +268,107 @@ def test_matrix_free_cg_exact_bilinear(ctx):
     assert np.max(np.abs(x.numpy() - (1 - xy[:, 0]) * (1 - xy[:, 1]))) <= 1e-10
 
 
+@pytest.mark.parametrize("case", CASES)
+def test_boundary_reaction_bitexact(ctx, case):
+    """Weak boundary reaction (QuantityEvaluator.tpp:95-199, the heat pairs)
+    on the device: result and face measure per tag bitwise the oracle's
+    serial sums, steady and transient, with and without a source; the tag
+    list holds every face tag, a repeated tag (the last entry takes it) and
+    one no face carries."""
+    pb, oc = build(ctx, case, kappa_seed=9)
+    dim = case[0]
+    tags = list(range(2 * dim)) + [1, 17]
+    rng = np.random.default_rng(12)
+    U, Ud = rng.standard_normal(oc.n_free), rng.standard_normal(oc.n_free)
+    dU, dUd = ctx.from_numpy(U), ctx.from_numpy(Ud)
+    for src in (None, "x*y + 2*t"):
+        for rate in (False, True):
+            got = pb.boundary_reaction(tags, dU, Udot=dUd if rate else None, src=src, time=0.5)
+            want = oc.boundary_reaction(tags, U, Udot=Ud if rate else None, src=src, time=0.5)
+            say("reaction", src, rate, got[0][:3], want[0][:3])
+            assert same(got[0], want[0]) and same(got[1], want[1])
+    assert got[0][1] == 0.0 and got[1][1] == 0.0 and got[1][-1] == 0.0
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_boundary_flux_bitexact(ctx, case):
+    """Strong boundary flux (QuantityEvaluator.tpp:14-93, HeatFluxIntegrand)
+    on the device: integral and face measure per tag bitwise the oracle's
+    serial sums (per-element conductivity, Dirichlet values at `time`)."""
+    pb, oc = build(ctx, case, kappa_seed=4)
+    tags = list(range(2 * case[0])) + [1, 17]
+    U = np.random.default_rng(14).standard_normal(oc.n_free)
+    got = pb.boundary_flux(tags, ctx.from_numpy(U), time=0.25)
+    want = oc.boundary_flux(tags, U, time=0.25)
+    say("flux", got[0][:3], want[0][:3])
+    assert same(got[0], want[0]) and same(got[1], want[1])
+    # the measure is the same sum as the weak reaction's
+    assert same(got[1], pb.boundary_reaction(tags, ctx.from_numpy(U), time=0.25)[1])
+
+
+def test_boundary_flux_nonlinear_anisotropic_bitexact(ctx):
+    """k(T) times an anisotropic tensor: K = k(T) shape at each face point."""
+    import residuum_hip as rh
+    dim, n, p, nq = 2, (5, 6), (2, 2), 3
+    tensor = (2.0, 0.5, 0.5, 1.0)
+    pb = rh.Problem(ctx, dim, n, p, nq, [1, 2], ["x*y", "1 - x"])
+    pb.set_conductivity(1.0, k_expr="1.0 + 0.5*T", dk_expr="0.5", tensor=tensor)
+    oc = O.Case(dim, n, p, nq, [1, 2], ["x*y", "1 - x"], order=O.ORDER_GPU, k_expr="1.0 + 0.5*T", dk_expr="0.5",
+                tensor=tensor)
+    U = np.random.default_rng(15).standard_normal(oc.n_free)
+    got = pb.boundary_flux([0, 1, 2, 3], ctx.from_numpy(U), time=1.0)
+    want = oc.boundary_flux([0, 1, 2, 3], U, time=1.0)
+    assert same(got[0], want[0]) and same(got[1], want[1])
+
+
+def test_boundary_flux_reference_fixture(ctx):
+    """tests/unit/equation/heateq/QuantityEvaluator.cpp:101-238 on the
+    device: 4x4 Q1 on [-1,1]^2, k = 2, T = x + 2y; the left side's average
+    k a, integral and length; the four sides' integrals and combinations;
+    the same with T as a Dirichlet value on every side."""
+    import residuum_hip as rh
+    for tags in ([], [0, 1, 2, 3]):
+        pb = rh.Problem(ctx, 2, (4, 4), (1, 1), 2, tags, ["x + 2*y"] * len(tags), ext=(-1.0, 1.0, -1.0, 1.0))
+        pb.set_conductivity(2.0)
+        oc = O.Case(2, (4, 4), (1, 1), 2, tags, ["x + 2*y"] * len(tags), ext=(-1.0, 1.0, -1.0, 1.0))
+        xy = oc.xyz.reshape(-1, 2)[oc.a2t]
+        dU = ctx.from_numpy(xy[:, 0] + 2 * xy[:, 1]) if oc.n_free else ctx.zeros(1)
+        res, meas = pb.boundary_flux([0, 1, 2, 3], dU)
+        assert np.max(np.abs(res - np.array([4.0, -4.0, 8.0, -8.0]))) <= 1e-10
+        assert np.max(np.abs(meas - 2.0)) <= 1e-10
+        assert abs(res[0] / meas[0] - 2.0) <= 1e-10
+        assert abs(res[1] + res[2] - 4.0) <= 1e-10 and abs(res[1] - res[3] - 4.0) <= 1e-10
+        assert abs(res.sum()) <= 1e-9
+
+
+def test_boundary_reaction_nonlinear_bitexact(ctx):
+    """k(T) conductivity: the stiffness pair uses k at each point's T."""
+    import residuum_hip as rh
+    dim, n, p, nq = 2, (6, 5), (2, 2), 3
+    pb = rh.Problem(ctx, dim, n, p, nq, [0, 3], ["x", "1"])
+    pb.set_conductivity(1.0, k_expr="1.0 + 0.5*T", dk_expr="0.5")
+    oc = O.Case(dim, n, p, nq, [0, 3], ["x", "1"], order=O.ORDER_GPU, k_expr="1.0 + 0.5*T", dk_expr="0.5")
+    U = np.random.default_rng(13).standard_normal(oc.n_free)
+    got = pb.boundary_reaction([0, 1, 2, 3], ctx.from_numpy(U), src="3*x")
+    want = oc.boundary_reaction([0, 1, 2, 3], U, src="3*x")
+    assert same(got[0], want[0]) and same(got[1], want[1])
+
+
+def test_boundary_reaction_heat_flux(ctx):
+    """The reference's heat-flux fixture (tests/unit/equation/heateq/
+    QuantityEvaluator.cpp:20-100) on the device: k = 2, T = x + 2y, no
+    source; the reaction on x = -1 is k dT/dx times the edge length and the
+    measure is the edge length."""
+    import residuum_hip as rh
+    pb = rh.Problem(ctx, 2, (8, 8), (1, 1), 2, [], [], ext=(-1.0, 1.0, -1.0, 1.0))
+    pb.set_conductivity(2.0)
+    oc = O.Case(2, (8, 8), (1, 1), 2, [], [], ext=(-1.0, 1.0, -1.0, 1.0))
+    xy = oc.xyz.reshape(-1, 2)[oc.a2t]
+    res, meas = pb.boundary_reaction([0, 2], ctx.from_numpy(xy[:, 0] + 2 * xy[:, 1]))
+    assert abs(meas[0] - 2.0) < 1e-14 and abs(meas[1] - 2.0) < 1e-14
+    assert abs(res[0] - 2.0 * 1.0 * 2.0) < 1e-12 and abs(res[1] - 2.0 * 2.0 * 2.0) < 1e-12
+
+
 def test_cg_absolute_tolerance(ctx):
     """cg/Solver.tpp:45-48 with ToleranceType::Absolute: stop at the first k
     with ||r_k|| < tol, leaving exactly the iterate of k steps."""

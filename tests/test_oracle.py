@@ -415,3 +415,93 @@ def test_live_reference_boundary_reaction(spec):
             got = c.boundary_reaction(all_tags, U, Udot=Udot, src=O.KIND_EXPR[src_kind], time=0.25)
             want = r.boundary_reaction(all_tags, U, Udot=Udot, src_kind=src_kind, time=0.25)
             assert same(got[0], want[0]) and same(got[1], want[1])
+
+
+# ---------------------------------------------------------------------------
+# strong boundary heat flux (QuantityEvaluator.tpp:14-93, HeatFluxIntegrand)
+# ---------------------------------------------------------------------------
+def _flux_fixture(dirichlet=False):
+    """tests/unit/equation/heateq/QuantityEvaluator.cpp:33-95: 4x4 Q1 on
+    [-1,1]^2, k = 2, T = x + 2y, two-point rules; no essential BCs, or (the
+    constrained case, :139-183) T as a Dirichlet value on all four sides."""
+    tags = [0, 1, 2, 3] if dirichlet else []
+    c = O.Case(2, (4, 4), (1, 1), 2, tags, ["x + 2*y"] * len(tags), ext=(-1.0, 1.0, -1.0, 1.0), k_scalar=2.0)
+    xy = c.xyz.reshape(-1, 2)[c.a2t]
+    return c, xy[:, 0] + 2.0 * xy[:, 1]
+
+
+def test_flux_average_left_boundary():
+    """HeatFluxAverageOnLeftBoundaryMatchesAnalytic (:101-121): k a."""
+    c, U = _flux_fixture()
+    res, meas = c.boundary_flux([0], U)
+    assert abs(res[0] / meas[0] - 2.0) <= 1e-10
+
+
+def test_flux_integral_and_length():
+    """MultipleQuantitiesInOnePassAreIndependentlyCorrect (:123-145)."""
+    c, U = _flux_fixture()
+    res, meas = c.boundary_flux([0], U)
+    assert abs(res[0] - 4.0) <= 1e-10 and abs(meas[0] - 2.0) <= 1e-10
+
+
+def test_flux_target_dirichlet_constrained():
+    """HeatFluxIntegralIsCorrectWhenTargetBoundaryIsAlsoDirichletConstrained
+    (:147-190): the corner values come from the constrained gather."""
+    c, U = _flux_fixture(dirichlet=True)
+    res, _ = c.boundary_flux([0], U)
+    assert abs(res[0] - 4.0) <= 1e-10
+
+
+def test_flux_registry_and_combinations():
+    """BoundaryQuantityRegistryAndCombinationMatchAnalytic (:192-238): all
+    four sides in one pass, combinations sum coefficient x result, and the
+    harmonic field's net flux is zero."""
+    c, U = _flux_fixture()
+    res, _ = c.boundary_flux([0, 1, 2, 3], U)
+    want = np.array([4.0, -4.0, 8.0, -8.0])
+    assert np.max(np.abs(res - want)) <= 1e-10
+    comb = lambda terms: sum(coef * res[t] for t, coef in terms)
+    assert abs(comb([(1, 1.0), (2, 1.0)]) - 4.0) <= 1e-10
+    assert abs(comb([(1, 1.0), (3, -1.0)]) - 4.0) <= 1e-10
+    assert abs(comb([(t, 1.0) for t in range(4)])) <= 1e-9
+
+
+def test_weak_reaction_on_the_flux_fixture():
+    """The weak reaction of the same harmonic field, no source: equal to the
+    strong flux per side (the linear field is reproduced exactly), and the
+    four sides sum to zero."""
+    c, U = _flux_fixture()
+    res, meas = c.boundary_reaction([0, 1, 2, 3], U)
+    assert np.max(np.abs(res - np.array([4.0, -4.0, 8.0, -8.0]))) <= 1e-10
+    assert np.max(np.abs(meas - 2.0)) <= 1e-14
+    assert abs(res.sum()) <= 1e-9
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref/libref.so not built (needs /root/reference)")
+@pytest.mark.parametrize("spec", [
+    (2, (6, 4), (1, 1), 2, [0, 2], [7, 7], {}),
+    (2, (3, 4), (2, 2), 3, [1, 3], [2, 2], {"tensor": (2.0, 0.5, 0.5, 1.0)}),
+    (2, (5, 3), (1, 1), 3, [0], [1], {"tdep": True}),
+    (3, (3, 2, 4), (1, 1, 1), 2, [4, 5], [6, 0], {}),
+])
+def test_live_reference_boundary_flux(spec):
+    """evaluateBoundaryRegistry with HeatFluxIntegrand: oracle against the
+    reference's own code bit for bit (isotropic, anisotropic tensor,
+    k(T) = 1 + 0.5 T; free and Dirichlet tags)."""
+    dim, n, p, nq, tags, kinds, model = spec
+    kw = {}
+    if model.get("tensor"):
+        kw["tensor"] = model["tensor"]
+    if model.get("tdep"):
+        kw.update(k_expr="1 + 0.5*T", dk_expr="0.5")
+    c = O.Case(dim, n, p, nq, tags, [O.KIND_EXPR[k] for k in kinds], **kw)
+    r = O.RefCase(dim, n, p, nq, tags, kinds)
+    if model.get("tensor"):
+        r.set_model(tensor=list(model["tensor"]) + [0.0] * (9 - len(model["tensor"])))
+    if model.get("tdep"):
+        r.set_model(tdep=1, ka=1.0, kb=0.5)
+    U = np.random.default_rng(21).standard_normal(c.n_free)
+    all_tags = list(range(2 * dim))
+    got = c.boundary_flux(all_tags, U, time=0.75)
+    want = r.boundary_flux(all_tags, U, time=0.75)
+    assert same(got[0], want[0]) and same(got[1], want[1])
