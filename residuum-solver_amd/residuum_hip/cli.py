@@ -12,8 +12,17 @@ must be `hip` (the reference's parser knows cpu and cuda,
 BackendConfigParser.cpp:6-14).  What the device path does not run is refused
 with a message, never run on the host: natural (flux) boundary conditions,
 boundary values or sources other than expressions or (sources) nodal files,
-monitors, non-Lagrange meshes, the pseudo-transient driver, and time steppers
-other than Backward Euler with a constant step.
+weak flux monitors together with a nodal-file source, non-Lagrange meshes,
+the pseudo-transient driver, and time steppers other than Backward Euler
+with a constant step.
+
+Boundary heat-flux monitors (MonitorConfigParser.cpp, HeatProblem.tpp:
+133-167, 446-460) run on the device: monitors with the same (method,
+reduction) share one evaluation over the union of their tags; each reports
+sum(coefficient x result(tag)), the result divided by the tags' face measure
+for the average reduction; console lines and CSV rows (tick, time, value)
+after the steady solve (tick 0, time 0) or at the initial condition and
+after every step of a transient run.
 
 The flow is HeatDispatcher::run (src/application/heateq/HeatDispatcher.cpp):
 read the mesh, build the problem, then the steady stage (linear:
@@ -82,6 +91,33 @@ def _linear(node, where):
     return cfg
 
 
+def _monitors(root, base):
+    """MonitorConfigParser.cpp (with MonitorTermConfigParser,
+    MonitorOutputConfigParser, EvaluationMethodParser, ReductionParser)."""
+    mons = []
+    for k, m in enumerate(_opt(root, "monitors", None) or []):
+        where = f"monitors[{k}]"
+        name = str(_req(m, "name", where))
+        quantity = _req(m, "quantity", where)
+        if quantity != "heat_flux":
+            raise ConfigError(f"Unknown monitor quantity: {quantity}")
+        method = _opt(m, "flux_method", "weak")
+        if method not in ("strong", "weak"):
+            raise ConfigError(f"EvaluationMethodParser: unknown method '{method}'. Valid options: strong, weak")
+        reduction = _req(m, "reduction", where)
+        if reduction not in ("integral", "average"):
+            raise ConfigError(f"ReductionParser: unknown reduction '{reduction}'. Valid options: integral, average")
+        if "terms" not in m:
+            raise ConfigError(f"MonitorConfigParser: monitor '{name}': missing required 'terms' list")
+        terms = [(int(_req(t, "boundary", where + ".terms")), float(_opt(t, "coefficient", 1.0)))
+                 for t in (m["terms"] or [])]
+        out = _opt(m, "output", None) or {}
+        f = str(_opt(out, "file", ""))
+        mons.append(dict(name=name, method=method, reduction=reduction, terms=terms,
+                         console=bool(_opt(out, "console", True)), file=os.path.join(base, f) if f else ""))
+    return mons
+
+
 def parse_heat_config(path):
     import yaml
     with open(path) as f:
@@ -141,8 +177,7 @@ def parse_heat_config(path):
     cfg["bc_tags"], cfg["bc_exprs"] = tags, exprs
     ic = root.get("initial_condition")
     cfg["initial"] = _read_field(ic, "initial_condition", allow_file=True) if ic else ("expression", "0")
-    if root.get("monitors"):
-        raise ConfigError("monitors: boundary monitors are not run on the device")
+    cfg["monitors"] = _monitors(root, base)
     # solver (SolverConfigParser, DriverConfigParser, TimeStepperConfigParser, NonlinearSolverConfigParser)
     solver = _req(root, "solver", "config")
     drv = _req(_req(solver, "driver", "solver"), "type", "solver.driver")
@@ -234,6 +269,65 @@ def write_vtk_field(path, xyz, ien, npe, dim, nodal, name="T", fmt="vtu"):
 
 
 # ---------------------------------------------------------------------------
+# boundary monitors
+# ---------------------------------------------------------------------------
+class Monitors:
+    """The configured monitors of one run (HeatProblem.tpp:133-167): groups
+    keyed by (method, reduction) in order of first use, each evaluated once
+    per tick over its tags, then every monitor of the group reported."""
+
+    def __init__(self, pb, mons, src_text, transient, quiet):
+        self.pb, self.src, self.transient, self.quiet = pb, src_text, transient, quiet
+        self.groups = []
+        keyed = {}
+        bdim = pb.dim - 1
+        for m in mons:
+            key = (m["method"], m["reduction"])
+            if key not in keyed:
+                keyed[key] = dict(method=m["method"], reduction=m["reduction"], tags=[], members=[])
+                self.groups.append(keyed[key])
+            g = keyed[key]
+            for tag, _ in m["terms"]:
+                if tag not in g["tags"]:
+                    g["tags"].append(tag)
+            unit = "W" if m["reduction"] == "integral" else f"W/m^{bdim}"  # QuantityUnits.hpp:13-22
+            csv = None
+            if m["file"]:
+                os.makedirs(os.path.dirname(m["file"]) or ".", exist_ok=True)
+                csv = open(m["file"], "w")
+                csv.write(f"tick,time,{m['name']} [{unit}]\n")
+                csv.flush()
+            g["members"].append(dict(m, unit=unit, csv=csv))
+        self.values = {m["name"]: [] for m in mons}
+
+    def evaluate(self, tick, time, U, Udot=None):
+        for g in self.groups:
+            if g["method"] == "weak":
+                res, meas = self.pb.boundary_reaction(g["tags"], U, Udot=Udot if self.transient else None,
+                                                      src=self.src, time=time)
+            else:
+                res, meas = self.pb.boundary_flux(g["tags"], U, time=time)
+            if g["reduction"] == "average":
+                res = res / meas
+            for m in g["members"]:
+                v = 0.0
+                for tag, coef in m["terms"]:  # BoundaryQuantityCombination::evaluate
+                    v += coef * res[g["tags"].index(tag)]
+                self.values[m["name"]].append((tick, time, v))
+                if m["console"]:
+                    _log(f"monitor '{m['name']}' = {v:.6f} {m['unit']}", self.quiet)
+                if m["csv"]:
+                    m["csv"].write(f"{float(tick):g},{time:g},{v:g}\n")
+                    m["csv"].flush()
+
+    def close(self):
+        for g in self.groups:
+            for m in g["members"]:
+                if m["csv"]:
+                    m["csv"].close()
+
+
+# ---------------------------------------------------------------------------
 # the run
 # ---------------------------------------------------------------------------
 def _log(msg, quiet):
@@ -277,8 +371,17 @@ def run_heat(config_path, quiet=False, ctx=None):
             else:
                 nodal, _ = pb.read_pndf(os.path.join(cfg["base"], sval))
                 src_nodal = ctx.from_numpy(nodal)
-        summary = _steady(pb, cfg, U, src_text, src_nodal, quiet) if cfg["driver"] == "steady" else \
-            _transient(pb, cfg, U, src_text, src_nodal, mesh, quiet)
+        if src_nodal is not None and any(m["method"] == "weak" for m in cfg["monitors"]):
+            raise ConfigError("weak flux monitors with a nodal-file source are not run on the device")
+        mon = Monitors(pb, cfg["monitors"], src_text, cfg["driver"] == "transient", quiet) if cfg["monitors"] else None
+        try:
+            summary = _steady(pb, cfg, U, src_text, src_nodal, quiet, mon) if cfg["driver"] == "steady" else \
+                _transient(pb, cfg, U, src_text, src_nodal, mesh, quiet, mon)
+        finally:
+            if mon:
+                mon.close()
+        if mon:
+            summary["monitors"] = mon.values
         summary["U"] = U.numpy()
         summary["problem"] = dict(n_free=pb.n_free, n_nodes=pb.n_nodes, n_elem=pb.n_elem)
         _log("converged" if summary["converged"] else "did not converge", quiet)
@@ -318,7 +421,7 @@ def _write(pb, cfg, mesh, U, step, t, quiet):
     _log(f"output: {stem}", quiet)
 
 
-def _steady(pb, cfg, U, src_text, src_nodal, quiet):
+def _steady(pb, cfg, U, src_text, src_nodal, quiet, mon=None):
     mesh = read_pmsh(cfg["mesh"])
     nonlinear = cfg["conductivity"]["k_expr"] is not None
     if nonlinear or cfg["nonlinear"] is not None:
@@ -336,6 +439,8 @@ def _steady(pb, cfg, U, src_text, src_nodal, quiet):
         conv = its >= 0
         _log(f"newton: {its} iterations, residuals {list(res)} ({_time.perf_counter() - t0:.3f} s)", quiet)
         _write(pb, cfg, mesh, U, 0, 0.0, quiet)
+        if mon:
+            mon.evaluate(0, 0.0, U)
         return dict(converged=conv, newton_iterations=its, linear_iterations=[int(v) for v in lin_its])
     lin = cfg["linear"]
     forms = [(DIFFUSION, 1.0)]
@@ -351,17 +456,24 @@ def _steady(pb, cfg, U, src_text, src_nodal, quiet):
     _log(f"linear: {lin['type']} ({lin['operator']}, {lin['preconditioner']}) {rep.iterations} iterations, "
          f"residual {rep.final_residual:.3e} ({_time.perf_counter() - t0:.3f} s)", quiet)
     _write(pb, cfg, mesh, U, 0, 0.0, quiet)
+    if mon:
+        mon.evaluate(0, 0.0, U)
     return dict(converged=bool(rep.converged), iterations=int(rep.iterations), final_residual=rep.final_residual)
 
 
-def _transient(pb, cfg, U, src_text, src_nodal, mesh, quiet):
+def _transient(pb, cfg, U, src_text, src_nodal, mesh, quiet, mon=None):
     lin = cfg["linear"]
     t0, tf, dt = cfg["t0"], cfg["tf"], cfg["dt"]
     if cfg["conductivity"]["k_expr"] is not None:
         raise ConfigError("a temperature-dependent conductivity in a transient run is not run on the device")
     _write(pb, cfg, mesh, U, 0, t0, quiet)
+    # the rate the weak monitors see: zero at the initial condition, then the
+    # one each step's assemble() forms, BackwardEulerStage.hpp:73-74
+    Udot = pb.vector(0.0) if mon else None
+    if mon:
+        mon.evaluate(0, t0, U, Udot)
     start = _time.perf_counter()
-    if lin["operator"] == "csr" and lin["type"] == "cg" and src_nodal is None and cfg["output"] is None:
+    if lin["operator"] == "csr" and lin["type"] == "cg" and src_nodal is None and cfg["output"] is None and not mon:
         steps, iters = pb.be_run(U, t0, tf, dt, src=src_text, tol=lin["tolerance"], max_iters=lin["max_iterations"],
                                  precond=_precond(lin))
         _log(f"backward euler: {steps} steps on the device, {int(np.sum(iters))} CG iterations "
@@ -377,6 +489,9 @@ def _transient(pb, cfg, U, src_text, src_nodal, mesh, quiet):
         ct = t + h
         forms[1] = (MASS | SCALED, 1.0 / h)
         pb.ctx.copy(U, Uprev)
+        if mon:
+            pb.ctx.copy(U, Udot)
+            pb.ctx.axpby(-1.0 / h, 1.0 / h, Uprev, Udot)
         pb.assemble_load(F, src=src_text, nodal=src_nodal, time=ct)
         pb.assemble_vector(ms, [(MASS, 1.0)], U=Uprev, selector=SEL_VALUE, time=ct - h)
         pb.ctx.axpy(1.0 / h, ms, F)
@@ -389,6 +504,8 @@ def _transient(pb, cfg, U, src_text, src_nodal, mesh, quiet):
         iters.append(int(rep.iterations))
         t, step = ct, step + 1
         _write(pb, cfg, mesh, U, step, t, quiet)
+        if mon:
+            mon.evaluate(step, t, U, Udot)
     _log(f"backward euler: {step} steps, {sum(iters)} {lin['type']} iterations "
          f"({_time.perf_counter() - start:.3f} s)", quiet)
     return dict(converged=True, steps=step, iterations=iters)

@@ -114,3 +114,62 @@ def test_steady_nonlinear_kirchhoff(ctx, tmp_path):
     x = nodes_xy(ctx, str(tmp_path / "strip.pmsh"), tags=(0, 1), exprs=("0", "0"))[:, 0]
     theta = (-1.0 + np.sqrt(1.0 + 2 * 0.5 * (1.0 + 0.25) * x)) / 0.5
     assert np.max(np.abs(s["U"] - theta)) < 1e-6
+
+
+def test_monitors_steady_and_transient(ctx, tmp_path):
+    """Monitors from a configuration (HeatProblem.tpp:133-167, 446-460): the
+    strong flux of the bilinear solution (1-x)(1-y) on [-1,1]^2 is -2, 2,
+    -2, 2 on the four sides (average: the integral over the length 2); the
+    weak values equal the device reaction of the final state, combined
+    with the coefficients; CSV rows (tick, time, value).  A transient run
+    reports the initial condition and every step."""
+    import residuum_hip as rh
+    from residuum_hip import cli
+    d = bench_config(tmp_path)
+    d["solver"]["linear"]["operator"]["type"] = "csr"
+    d["monitors"] = [
+        {"name": "left_strong", "quantity": "heat_flux", "flux_method": "strong", "reduction": "integral",
+         "terms": [{"boundary": 0}], "output": {"console": False, "file": "mon/left_strong.csv"}},
+        {"name": "net_strong", "quantity": "heat_flux", "flux_method": "strong", "reduction": "integral",
+         "terms": [{"boundary": b} for b in range(4)]},
+        {"name": "top_avg", "quantity": "heat_flux", "flux_method": "strong", "reduction": "average",
+         "terms": [{"boundary": 3}]},
+        {"name": "net_influx", "quantity": "heat_flux", "reduction": "integral",
+         "terms": [{"boundary": b, "coefficient": -1.0} for b in range(4)],
+         "output": {"file": "mon/net_influx.csv"}},
+        {"name": "right_weak_avg", "quantity": "heat_flux", "flux_method": "weak", "reduction": "average",
+         "terms": [{"boundary": 1, "coefficient": 2.0}]},
+    ]
+    s = cli.run_heat(write_cfg(tmp_path, d), quiet=True, ctx=ctx)
+    mv = {k: v[0][2] for k, v in s["monitors"].items()}
+    say("monitors", mv)
+    assert abs(mv["left_strong"] + 2.0) <= 1e-10 and abs(mv["net_strong"]) <= 1e-10
+    assert abs(mv["top_avg"] - 1.0) <= 1e-10
+    pb = rh.Problem.from_pmsh(ctx, str(tmp_path / "square_n16.pmsh"), 2, [0, 1, 2, 3], ["(1-x)*(1-y)"] * 4)
+    dU = ctx.from_numpy(s["U"])
+    res, meas = pb.boundary_reaction([0, 1, 2, 3], dU, src="0.0")
+    want = 0.0
+    for b in range(4):
+        want += -1.0 * res[b]
+    assert mv["net_influx"] == want
+    assert mv["right_weak_avg"] == 2.0 * (res[1] / meas[1])
+    rows = open(tmp_path / "mon" / "left_strong.csv").read().splitlines()
+    assert rows[0] == "tick,time,left_strong [W]" and rows[1].startswith("0,0,")
+    assert abs(float(rows[1].split(",")[2]) + 2.0) < 1e-5
+    # transient: ticks 0..5 at t0 + k dt, the weak group with the step's rate
+    d["solver"]["driver"]["type"] = "transient"
+    d["solver"]["timestepper"] = {"type": "backward_euler", "t0": 0.0, "tf": 0.1,
+                                  "step_size": {"mode": "constant", "dt": 0.02}}
+    d["solver"]["linear"] = {"type": "cg", "operator": {"type": "csr"}, "preconditioner": {"type": "jacobi"},
+                             "tolerance": 1e-12, "max_iterations": 5000}
+    st = cli.run_heat(write_cfg(tmp_path, d, "tr.yaml"), quiet=True, ctx=ctx)
+    ticks = st["monitors"]["net_influx"]
+    assert [t for t, _, _ in ticks] == list(range(6))
+    assert np.allclose([t for _, t, _ in ticks], [0.02 * k for k in range(6)], atol=1e-15, rtol=0)
+    res_t, _ = pb.boundary_reaction([0, 1, 2, 3], ctx.from_numpy(st["U"]), Udot=pb.vector(0.0), src="0.0", time=0.1)
+    want = 0.0
+    for b in range(4):
+        want += -1.0 * res_t[b]
+    assert ticks[-1][2] == want
+    rows = open(tmp_path / "mon" / "net_influx.csv").read().splitlines()
+    assert len(rows) == 7 and rows[0] == "tick,time,net_influx [W]"

@@ -116,3 +116,53 @@ def test_heat_config_refusals(edit, msg, tmp_path):
     p.write_text(yaml.safe_dump(d))
     with pytest.raises(cli.ConfigError, match=msg):
         cli.parse_heat_config(str(p))
+
+
+MONITORS = [  # the monitor blocks of the reference's examples/heateq configurations
+    {"name": "net_influx", "quantity": "heat_flux", "flux_method": "weak", "reduction": "integral",
+     "terms": [{"boundary": b, "coefficient": -1.0} for b in range(4)],
+     "output": {"console": True, "file": "output/monitors/net_influx.csv"}},
+    {"name": "wall_flux", "quantity": "heat_flux", "flux_method": "strong", "reduction": "integral",
+     "terms": [{"boundary": 3}], "output": {"console": True, "file": "output/monitors/wall_flux.csv"}},
+    {"name": "inlet_flux_avg", "quantity": "heat_flux", "reduction": "average", "terms": [{"boundary": 1}]},
+]
+
+
+def test_heat_config_monitors(tmp_path):
+    """MonitorConfigParser.cpp: flux_method defaults to weak, coefficient to
+    1, console to true, file to none."""
+    import yaml
+    from residuum_hip import cli
+    d = yaml.safe_load(open(os.path.join(GOLDEN, "heateq_square_n16.yaml")))
+    d["monitors"] = MONITORS
+    p = tmp_path / "c.yaml"
+    p.write_text(yaml.safe_dump(d))
+    mons = cli.parse_heat_config(str(p))["monitors"]
+    assert [m["name"] for m in mons] == ["net_influx", "wall_flux", "inlet_flux_avg"]
+    assert mons[0]["terms"] == [(b, -1.0) for b in range(4)] and mons[0]["method"] == "weak"
+    assert mons[0]["file"] == str(tmp_path / "output/monitors/net_influx.csv")
+    assert mons[1]["method"] == "strong" and mons[1]["terms"] == [(3, 1.0)]
+    assert mons[2] == dict(name="inlet_flux_avg", method="weak", reduction="average", terms=[(1, 1.0)],
+                           console=True, file="")
+
+
+@pytest.mark.parametrize("edit,msg", [
+    (lambda m: m.update(flux_method="pointwise"), "unknown method"),
+    (lambda m: m.update(reduction="max"), "unknown reduction"),
+    (lambda m: m.pop("reduction"), "reduction"),
+    (lambda m: m.update(quantity="temperature"), "Unknown monitor quantity"),
+    (lambda m: m.pop("terms"), "missing required 'terms'"),
+    (lambda m: m["terms"][0].pop("boundary"), "boundary"),
+])
+def test_heat_config_monitor_errors(edit, msg, tmp_path):
+    import copy
+    import yaml
+    from residuum_hip import cli
+    d = yaml.safe_load(open(os.path.join(GOLDEN, "heateq_square_n16.yaml")))
+    m = copy.deepcopy(MONITORS[0])
+    edit(m)
+    d["monitors"] = [m]
+    p = tmp_path / "c.yaml"
+    p.write_text(yaml.safe_dump(d))
+    with pytest.raises(cli.ConfigError, match=msg):
+        cli.parse_heat_config(str(p))
