@@ -18,8 +18,12 @@ namespace rsd {
 // Matrix assembly (DIAG = false): thread a scatters row a of Ke through the
 // element->slot map.  DIAG = true (Assembler::assembleDiagonal,
 // Assembler.tpp:142-197): thread a forms only Ke[a][a], the same chain of
-// products and sums as in the matrix, and adds it to diag[row(a)].
-template<int D, int P, int NQ, bool STATE, bool DIAG = false>
+// products and sums as in the matrix, and adds it to diag[row(a)].  MV =
+// true: thread a also forms entry a of the unit mass form's element vector
+// on the full gather of A.mvU, with k_asm_vector's chain of products and
+// sums, and adds it to A.mvF[row(a)] (the same colour order, so bitwise the
+// separate vector assembly).
+template<int D, int P, int NQ, bool STATE, bool DIAG = false, bool MV = false>
 __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(AsmArgs A, Model m, int64_t c0,
                                                                              int64_t c1) {
     using C = ECfg<D, P, NQ, STATE>;
@@ -34,12 +38,14 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
     int32_t node_a = 0;
     double kap = 1.0;
     const bool has_kap = A.kappa != nullptr;
+    double Um[MV ? NPE : 1];
     if (valid) {
         e = A.order[k];
         int32_t nodes[NPE];
         double X[NPE][D];
         load_element<D, P>(A, e, nodes, X);
         node_a = nodes[a];
+        if constexpr (MV) gather<NPE>(A, nodes, 1, A.mvU, Um);
         if (has_kap) kap = A.kappa[e];
         double Uv[NPE], Ur[NPE];
         if constexpr (STATE) {
@@ -55,6 +61,7 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
     double Ke[NB];
 #pragma unroll
     for (int i = 0; i < NB; ++i) Ke[i] = 0.0;
+    double Fm = 0.0;
     for (int q = 0; q < C::NQP; ++q) {
         const double* r = rec + q * C::REC;
         double xi[3];
@@ -62,6 +69,11 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
         double N[NPE];
         shape_values<D, P>(xi, N);
         const double meas = r[C::R_MEAS];
+        if constexpr (MV) {
+            const double ra = m.rho * m.cp * N[a];
+#pragma unroll
+            for (int b = 0; b < NPE; ++b) Fm += ra * N[b] * Um[b] * meas * w;
+        }
         const double* da = r + C::R_DN + a * D;
         const double kq = STATE ? r[C::R_K] : kel;
         for (int f = 0; f < A.nf; ++f) {
@@ -103,6 +115,10 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
                 }
             }
         }
+    }
+    if constexpr (MV) {
+        const int32_t row = A.t2a[node_a];
+        if (row >= 0) A.mvF[row] += Fm;
     }
     if constexpr (DIAG) {
         const int32_t row = A.t2a[node_a];
@@ -287,28 +303,32 @@ static void run_adj(rsd_problem* pb, const AsmArgs& A, const Model& unit, const 
     RSD_HIP(hipGetLastError());
 }
 
-template<int D, int P, int NQ, bool STATE, bool DIAG>
+template<int D, int P, int NQ, bool STATE, bool DIAG, bool MV = false>
 static void run_matrix_t(rsd_problem* pb, const AsmArgs& A) {
     using C = ECfg<D, P, NQ, STATE>;
     hipStream_t st = pb->ctx->stream;
     for (int c = 0; c < pb->n_colours(); ++c) {
         const int64_t c0 = pb->colour_off[c], c1 = pb->colour_off[c + 1];
         if (c1 <= c0) continue;
-        hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, STATE, DIAG>), dim3((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB)),
-                           dim3(C::BLOCK), 0, st, A, pb->model, c0, c1);
+        hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, STATE, DIAG, MV>),
+                           dim3((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB)), dim3(C::BLOCK), 0, st, A, pb->model,
+                           c0, c1);
     }
     RSD_HIP(hipGetLastError());
 }
 
-// A.F set: the diagonal (assembleDiagonal) into A.F; else the matrix into A.val
+// A.F set: the diagonal (assembleDiagonal) into A.F; else the matrix into
+// A.val, with the mass vector of A.mvU into A.mvF when A.mvF is set
 template<int D, int P, int NQ>
 static void run_matrix(rsd_problem* pb, const AsmArgs& A) {
-    const bool diag = A.F != nullptr;
+    const bool diag = A.F != nullptr, mv = A.mvF != nullptr;
     if (A.need_state) {
         if (diag) run_matrix_t<D, P, NQ, true, true>(pb, A);
+        else if (mv) run_matrix_t<D, P, NQ, true, false, true>(pb, A);
         else run_matrix_t<D, P, NQ, true, false>(pb, A);
     } else {
         if (diag) run_matrix_t<D, P, NQ, false, true>(pb, A);
+        else if (mv) run_matrix_t<D, P, NQ, false, false, true>(pb, A);
         else run_matrix_t<D, P, NQ, false, false>(pb, A);
     }
 }
@@ -441,6 +461,33 @@ void assemble_diagonal(rsd_problem* pb, double time, int32_t nf, const int32_t* 
     A.need_state = forms_need_state(pb, A);
     update_gval(pb, time);
     launch_fill(pb->ctx, pb->n_free, diag, 0.0);
+    dispatch_matrix(pb, A);
+}
+
+bool matrix_mass_fusable(const rsd_problem* pb, int32_t nf, const int32_t* kinds) {
+    AsmArgs A{};
+    set_forms(A, nf, kinds, nullptr);
+    return !forms_need_state(pb, A) || !pb->bc_time_dependent;
+}
+
+void assemble_matrix_mass(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
+                          const double* U, const double* Udot, rsd_matrix* K, const double* Um, double time_m,
+                          double* Fm) {
+    RSD_CHECK(K->pb == pb, "matrix belongs to another problem");
+    RSD_CHECK(matrix_mass_fusable(pb, nf, kinds), "matrix and mass vector need Dirichlet values at two times");
+    AsmArgs A = base_args(pb);
+    set_forms(A, nf, kinds, coefs);
+    A.U = U;
+    A.Udot = Udot;
+    A.val = K->val.p;
+    A.time = time;
+    A.need_state = forms_need_state(pb, A);
+    A.mvU = Um;
+    A.mvF = Fm;
+    // the state gather (if any) and the mass gather read the same values
+    update_gval(pb, A.need_state ? time : time_m);
+    launch_fill(pb->ctx, K->val.n, K->val.p, 0.0);
+    launch_fill(pb->ctx, pb->n_free, Fm, 0.0);
     dispatch_matrix(pb, A);
 }
 

@@ -26,6 +26,8 @@ struct AsmArgs {
     double coefs[4];
     int32_t need_state;
     const int32_t* stop;  // nullable: vector kernels return at once when *stop != 0
+    const double* mvU;    // matrix kernels, nullable: also scatter (M_e Ue)[a] of mvU into mvF
+    double* mvF;
 };
 
 AsmArgs base_args(rsd_problem* pb);
@@ -38,6 +40,17 @@ void assemble_vector(rsd_problem* pb, double time, int32_t nf, const int32_t* ki
 // each element's Ke[a][a] added in colour order
 void assemble_diagonal(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
                        const double* U, double* diag);
+// assemble_matrix plus, from the same element pass, the unit mass form's
+// vector on Um (assemble_vector with one MASS form and RSD_SEL_VALUE) into
+// Fm, bitwise both separate assemblies.  The Dirichlet values of Um's
+// gather are those at time_m; the matrix's state (when its forms need one)
+// uses those at `time`, so the two times must give the same values.
+void assemble_matrix_mass(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
+                          const double* U, const double* Udot, rsd_matrix* K, const double* Um, double time_m,
+                          double* Fm);
+// whether assemble_matrix_mass applies: one set of Dirichlet values serves
+// both gathers
+bool matrix_mass_fusable(const rsd_problem* pb, int32_t nf, const int32_t* kinds);
 void assemble_load(rsd_problem* pb, double time, const Expr* src, const double* nodal, double* F);
 void apply_essential(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs, double* F);
 const Expr* cached_source(rsd_problem* pb, const char* text);

@@ -35,9 +35,15 @@ void be_assemble(rsd_problem* pb, rsd_matrix* K, const BEBuffers& b, double* U, 
     const double one[1] = {1.0};
     launch_copy(ctx, n, U, b.Udot);
     launch_axpby(ctx, n, -1.0 / dt, 1.0 / dt, b.Uprev, b.Udot);
-    assemble_matrix(pb, ct, 2, kinds, coefs, U, b.Udot, K);
-    assemble_load(pb, ct, src, nullptr, b.F);
-    assemble_vector(pb, ct - dt, 1, kmass, one, b.Uprev, nullptr, nullptr, RSD_SEL_VALUE, b.ms);
+    if (matrix_mass_fusable(pb, 2, kinds)) {
+        // K + M/dt and M U_prev from one element pass
+        assemble_matrix_mass(pb, ct, 2, kinds, coefs, U, b.Udot, K, b.Uprev, ct - dt, b.ms);
+        assemble_load(pb, ct, src, nullptr, b.F);
+    } else {
+        assemble_matrix(pb, ct, 2, kinds, coefs, U, b.Udot, K);
+        assemble_load(pb, ct, src, nullptr, b.F);
+        assemble_vector(pb, ct - dt, 1, kmass, one, b.Uprev, nullptr, nullptr, RSD_SEL_VALUE, b.ms);
+    }
     launch_axpy(ctx, n, 1.0 / dt, b.ms, b.F);
     apply_essential(pb, ct, 2, kinds, coefs, b.F);
 }
