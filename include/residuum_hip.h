@@ -143,9 +143,16 @@ int rsd_problem_layout(rsd_problem* pb, int64_t* out);
  * counterpart; decided here on first call if no FP32 work came first):
  * out[4] = column mode (3: symmetric half storage, the upper value of each
  * symmetric pair kept once, below-diagonal entries read from the row above;
- * else the FP64 layout's mode), number of patterns, upper entries per row
- * (half storage; 0 otherwise), FP32 slots */
+ * 4: the same in uniform stencil storage, one dense plane per non-negative
+ * column offset, used when every row's offsets lie in one 27- (hex) or
+ * 9-point (quad) stencil; else the FP64 layout's mode), number of patterns,
+ * upper entries per row (half storage) or planes (uniform stencil; 0
+ * otherwise), FP32 slots */
 int rsd_problem_layout_f32(rsd_problem* pb, int64_t* out);
+/* padding the uniform stencil kernel needs around its x vector (no reference
+ * counterpart): rsd_spmv_f32_padded reads x[-lo .. n + hi - 1], which must
+ * be finite (zeros); 0 and 0 when the layout is not the uniform stencil */
+int rsd_problem_padding_f32(rsd_problem* pb, int64_t* lo, int64_t* hi);
 /* Binary mesh files, PMSH v1 (src/io/MeshIO.cpp:31-140; magic 0x504D5348,
  * io/MeshIO.hpp:49-50).  Host-side, no device work.  rsd_pmsh_info:
  * out[12] = parametric dim, spatial dim, element family (0 quad, 2 hex),
@@ -290,6 +297,12 @@ int rsd_matrix_load_csr(rsd_ctx* ctx, rsd_matrix* A, const double* val);
  * iterations (SpMVs). */
 int rsd_matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A);
 int rsd_spmv_f32(rsd_ctx* ctx, rsd_matrix* A, const float* x, float* y);
+/* rsd_spmv_f32 by the mixed CG's own kernel: with the uniform stencil
+ * storage every row reads the same 27 (9) entries, x three at a time, with no
+ * per-row pattern, reading x at rsd_problem_padding_f32's padding (which the
+ * caller provides, zeroed); bitwise rsd_spmv_f32's result.  Other layouts:
+ * rsd_spmv_f32. */
+int rsd_spmv_f32_padded(rsd_ctx* ctx, rsd_matrix* A, const float* x, float* y);
 int rsd_cg_solve_mixed(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
                        int32_t precond, double inner_floor, rsd_report* rep, int64_t* outer_cycles);
 

@@ -150,9 +150,9 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
             for (int j = 0; j < i; ++j) sym_model = sym_model && pb->model.tensor[i * pb->dim + j] == pb->model.tensor[j * pb->dim + i];
     MixedWork mw;
     if (mixed) {
-        mw = mixed_workspace(ctx, n);
         build_half(pb);
-        K.val32.alloc(pb->sell.slots32());
+        mw = mixed_workspace(ctx, n, &pb->sell);
+        alloc_f32(ctx, &K);
     }
     update_gval(pb, t0);
     launch_copy(ctx, n, U, b.Uprev);

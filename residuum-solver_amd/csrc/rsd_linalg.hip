@@ -268,6 +268,10 @@ void launch_fill(rsd_ctx* ctx, int64_t n, double* p, double v) {
     if (n) hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, ctx->stream, n, p, v);
 }
 
+void launch_fill_f32(rsd_ctx* ctx, int64_t n, float* p, float v) {
+    if (n) hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, ctx->stream, n, p, v);
+}
+
 CGWork cg_workspace(rsd_ctx* ctx, int64_t n, int64_t max_iters, bool want_hist) {
     CGWork w;
     w.r = (double*)ctx->ws("cg.r", sizeof(double) * (size_t)std::max<int64_t>(n, 1));

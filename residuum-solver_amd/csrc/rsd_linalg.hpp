@@ -27,6 +27,7 @@ void launch_axpby(rsd_ctx* ctx, int64_t n, double a, double b, const double* x, 
 void launch_scal(rsd_ctx* ctx, int64_t n, double a, double* x);
 void launch_copy(rsd_ctx* ctx, int64_t n, const double* x, double* y);
 void launch_fill(rsd_ctx* ctx, int64_t n, double* p, double v);
+void launch_fill_f32(rsd_ctx* ctx, int64_t n, float* p, float v);
 
 CGWork cg_workspace(rsd_ctx* ctx, int64_t n, int64_t max_iters, bool want_hist);
 void cg_launch_init(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,

@@ -41,14 +41,16 @@ __global__ void k_set_flag(int32_t* f, int32_t v) {
 }
 
 // The half layout's FP32 upper values from the FP64 SELL values (one thread
-// per row): the m-th value right of the diagonal part of row r goes to
-// m * pad + r.  With CHECK, the row's values left of the diagonal are also
+// per row): upper entry j of row r (pattern q) goes to r + dst[q][j] (the
+// m-th upper entry to m * pad + r, or in the uniform stencil storage to its
+// offset's plane).  With CHECK, the row's values left of the diagonal are also
 // compared with the FP64 values stored for the mirrored entries, and *asym
 // is set when one differs by more than 1e-10 of the row's largest magnitude
 // (an assembled heat matrix is symmetric to rounding).
 template<bool CHECK>
 __global__ __launch_bounds__(kBlock) void k_to_f32_half(int64_t n, Cols H, const int32_t* __restrict__ mir,
-                                                        const int32_t* __restrict__ low, int64_t pad,
+                                                        const int32_t* __restrict__ low,
+                                                        const int32_t* __restrict__ dst,
                                                         const double* __restrict__ val, float* __restrict__ vh,
                                                         int32_t* __restrict__ asym) {
     const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -56,7 +58,8 @@ __global__ __launch_bounds__(kBlock) void k_to_f32_half(int64_t n, Cols H, const
     const int q = H.pid[r];
     const int len = H.plen[q], lo = low[q];
     const int64_t base = H.slice_ptr[r >> 6] + (r & 63);
-    for (int j = lo; j < len; ++j) vh[(int64_t)(j - lo) * pad + r] = (float)val[base + (int64_t)j * kSlice];
+    const int32_t* pd = dst + q * H.stride;
+    for (int j = lo; j < len; ++j) vh[(uint32_t)r + (uint32_t)pd[j]] = (float)val[base + (int64_t)j * kSlice];
     if constexpr (CHECK) {
         const int32_t* po = H.poff + q * H.stride;
         const int32_t* pm = mir + q * H.stride;
@@ -255,10 +258,19 @@ __global__ void k_inv_diag_f32(int64_t n, const int32_t* __restrict__ diag, cons
 }
 
 // ------------------------------------------------------------------------
-void matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A, bool check_symmetry) {
+void alloc_f32(rsd_ctx* ctx, rsd_matrix* A) {
     build_half(A->pb);
     const Sell& S = A->pb->sell;
-    if (A->val32.n != S.slots32()) A->val32.alloc(S.slots32());
+    if (A->val32.n == S.slots32()) return;
+    A->val32.alloc(S.slots32());
+    // the uniform stencil storage holds zeros where a row lacks an entry:
+    // the conversion writes only the entries that exist
+    if (S.has_uni) launch_fill_f32(ctx, A->val32.n, A->val32.p, 0.0f);
+}
+
+void matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A, bool check_symmetry) {
+    alloc_f32(ctx, A);
+    const Sell& S = A->pb->sell;
     if (S.has_half) {
         if (!A->asym.n) A->asym.alloc(1);
         hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, ctx->stream, A->asym.p, 0);
@@ -266,10 +278,10 @@ void matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A, bool check_symmetry) {
             const dim3 grid((unsigned)nblocks(S.n_rows));
             if (check_symmetry)
                 hipLaunchKernelGGL(k_to_f32_half<true>, grid, dim3(kBlock), 0, ctx->stream, S.n_rows, S.cols32(),
-                                   S.half_mir.p, S.half_low.p, S.half_pad, A->val.p, A->val32.p, A->asym.p);
+                                   S.half_mir.p, S.half_low.p, S.half_dst.p, A->val.p, A->val32.p, A->asym.p);
             else
                 hipLaunchKernelGGL(k_to_f32_half<false>, grid, dim3(kBlock), 0, ctx->stream, S.n_rows, S.cols32(),
-                                   S.half_mir.p, S.half_low.p, S.half_pad, A->val.p, A->val32.p, A->asym.p);
+                                   S.half_mir.p, S.half_low.p, S.half_dst.p, A->val.p, A->val32.p, A->asym.p);
         }
     } else if (A->val.n) {
         hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nblocks(A->val.n)), dim3(kBlock), 0, ctx->stream, A->val.n,
@@ -277,19 +289,24 @@ void matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A, bool check_symmetry) {
     }
 }
 
-void launch_spmv_f32(rsd_ctx* ctx, const rsd_matrix* A, const float* x, float* y) {
+// padded: x is readable (and finite) from x - uni_lo to x + n + uni_hi
+void launch_spmv_f32(rsd_ctx* ctx, const rsd_matrix* A, const float* x, float* y, bool padded) {
     const Sell& S = A->pb->sell;
     const int64_t n = S.n_rows, nb = nblocks(n);
     if (!n) return;
-    RSD_LAUNCH_COLS32(S.mode32(), k_spmv_f32, dim3((unsigned)nb), dim3(kBlock), ctx->stream, n, nb, S.cols32(),
-                      A->val32.p, x, y);
+    RSD_LAUNCH_COLS32(padded ? S.mode_padded() : S.mode32(), k_spmv_f32, dim3((unsigned)nb), dim3(kBlock),
+                      ctx->stream, n, nb, S.cols32(), A->val32.p, x, y);
 }
 
-MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n) {
+MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n, const Sell* S) {
     MixedWork w;
     const size_t f = sizeof(float) * (size_t)std::max<int64_t>(n, 1), dd = sizeof(double) * (size_t)std::max<int64_t>(n, 1);
     w.r64 = (double*)ctx->ws("mx.r64", dd);
-    w.u = (float*)ctx->ws("mx.u", f);
+    // u, the SpMV operand, carries the uniform stencil's padding
+    w.lo = S && S->has_uni ? S->uni_lo : 0;
+    w.hi = S && S->has_uni ? S->uni_hi : 0;
+    w.ubase = (float*)ctx->ws("mx.u", f + sizeof(float) * (size_t)(w.lo + w.hi));
+    w.u = w.ubase + w.lo;
     w.w = (float*)ctx->ws("mx.w", f);
     w.p = (float*)ctx->ws("mx.p", f);
     w.s = (float*)ctx->ws("mx.s", f);
@@ -312,8 +329,8 @@ void mixed_launch_inner_iter(rsd_ctx* ctx, const rsd_matrix* A, int32_t precond,
     double* P = ctx->partials(mixed_partials(n));
     const float* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     hipStream_t s = ctx->stream;
-    RSD_LAUNCH_COLS32(S.mode32(), k_mx_spmv_wu, dim3((unsigned)nb), dim3(kBlock), s, n, nb, S.cols32(), A->val32.p,
-                      w.u, w.w, w.st, P);
+    RSD_LAUNCH_COLS32(w.lo || w.hi ? S.mode_padded() : S.mode32(), k_mx_spmv_wu, dim3((unsigned)nb), dim3(kBlock),
+                      s, n, nb, S.cols32(), A->val32.p, w.u, w.w, w.st, P);
     hipLaunchKernelGGL(k_mx_scalars, dim3(1), dim3(kFold), 0, s, nb, P, nbu, P + nb, w.st);
     hipLaunchKernelGGL(k_mx_update, dim3((unsigned)nbu), dim3(kBlock), 0, s, n, nbu, w.p, w.s, w.d, w.u, w.w, invd,
                        w.st, P + nb);
@@ -334,6 +351,9 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
     const float* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     if (precond == RSD_PC_JACOBI)
         hipLaunchKernelGGL(k_inv_diag_f32, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, st, n, S.diag.p, A->val.p, w.invd);
+    // the padding around u holds zeros (no kernel writes it)
+    launch_fill_f32(ctx, w.lo, w.ubase, 0.0f);
+    launch_fill_f32(ctx, w.hi, w.u + n, 0.0f);
     MixedState* h = nullptr;
     RSD_HIP(hipHostMalloc((void**)&h, sizeof(MixedState), hipHostMallocDefault));
     std::memset(h, 0, sizeof(MixedState));
@@ -414,7 +434,16 @@ int rsd_spmv_f32(rsd_ctx* ctx, rsd_matrix* A, const float* x, float* y) {
     return guard([&] {
         RSD_CHECK(ctx && A && x && y, "null argument");
         RSD_CHECK(A->val32.n == A->pb->sell.slots32(), "rsd_spmv_f32: call rsd_matrix_to_f32 first");
-        launch_spmv_f32(ctx, A, x, y);
+        launch_spmv_f32(ctx, A, x, y, false);
+        RSD_HIP(hipGetLastError());
+    });
+}
+
+int rsd_spmv_f32_padded(rsd_ctx* ctx, rsd_matrix* A, const float* x, float* y) {
+    return guard([&] {
+        RSD_CHECK(ctx && A && x && y, "null argument");
+        RSD_CHECK(A->val32.n == A->pb->sell.slots32(), "rsd_spmv_f32_padded: call rsd_matrix_to_f32 first");
+        launch_spmv_f32(ctx, A, x, y, true);
         RSD_HIP(hipGetLastError());
     });
 }
@@ -424,7 +453,7 @@ int rsd_cg_solve_mixed(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, 
     return guard([&] {
         RSD_CHECK(ctx && A && b && x, "rsd_cg_solve_mixed: null argument");
         RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
-        MixedWork w = mixed_workspace(ctx, A->pb->sell.n_rows);
+        MixedWork w = mixed_workspace(ctx, A->pb->sell.n_rows, &A->pb->sell);
         MixedState s = mixed_run(ctx, A, b, x, tol, max_iters, precond,
                                  inner_floor > 0 ? inner_floor : kMixedDefaultInnerFloor, false, w,
                                  nullptr);

@@ -19,6 +19,8 @@ struct MixedState {
 struct MixedWork {
     double* r64 = nullptr;
     float *u = nullptr, *w = nullptr, *p = nullptr, *s = nullptr, *d = nullptr, *invd = nullptr;
+    float* ubase = nullptr;   // u - lo: u's buffer with the uniform stencil's padding
+    int64_t lo = 0, hi = 0;
     MixedState* st = nullptr;
 };
 
@@ -31,8 +33,15 @@ struct ChunkRunner {
 // the FP32 copy of A's values; check_symmetry: compare every mirrored pair of
 // the half layout (callers whose matrix is symmetric by construction skip it)
 void matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A, bool check_symmetry);
-void launch_spmv_f32(rsd_ctx* ctx, const rsd_matrix* A, const float* x, float* y);
-MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n);
+// padded: x readable (and finite) from x - uni_lo to x + n + uni_hi, so the
+// uniform stencil kernel may run
+void launch_spmv_f32(rsd_ctx* ctx, const rsd_matrix* A, const float* x, float* y, bool padded);
+// S: the matrix layout the solve uses (its uniform stencil pads u); null
+// for none
+MixedWork mixed_workspace(rsd_ctx* ctx, int64_t n, const Sell* S);
+// the FP32 copy's storage, allocated (and zeroed where the layout keeps
+// zeros) once per layout
+void alloc_f32(rsd_ctx* ctx, rsd_matrix* A);
 // row tiles of 256 per block in the inner vector kernels (set-up and
 // update; the SpMV keeps one row per thread): a quarter of the block sums
 // for the one-block fold, and more loads in flight per thread

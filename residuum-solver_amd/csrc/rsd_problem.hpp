@@ -47,9 +47,20 @@ struct Sell {
     DBuf<int32_t> half_val;    // [n_half * pat_stride] m * half_pad + min(c - r, 0)
     DBuf<int32_t> half_len;    // [n_half] entries
     DBuf<int32_t> half_low;    // [n_half] entries left of the diagonal
+    DBuf<int32_t> half_dst;    // [n_half * pat_stride] value index - row of each upper entry (conversion)
     int32_t n_half = 0, half_width = 0;
     int64_t half_pad = 0;      // rows rounded up to 64: the stride between upper-value planes
     bool has_half = false, half_tried = false;
+    // uniform stencil storage of the half values (COL_UNI): one dense plane
+    // per non-negative offset of the union, planes uni_stride apart with row
+    // r at uni_front + r, zeros where a row lacks the entry; the per-pattern
+    // tables (half_val, half_dst) index the same storage
+    bool has_uni = false;
+    int32_t uni_nr = 0, uni_planes = 0;
+    int32_t uni_roff[kUniMaxRuns] = {};
+    int32_t uni_vidx[3 * kUniMaxRuns] = {};
+    int64_t uni_stride = 0, uni_front = 0;
+    int64_t uni_lo = 0, uni_hi = 0;  // padding an x vector needs before and after its n entries
 
     Cols cols() const {
         Cols c;
@@ -67,7 +78,11 @@ struct Sell {
     int mode() const { return has_pat ? COL_PAT : has_off16 ? COL_OFF16 : COL_ABS; }
     // the FP32 copy: symmetric half storage when built, else the FP64 layout
     int mode32() const { return has_half ? COL_HALF : mode(); }
-    int64_t slots32() const { return has_half ? half_pad * half_width : slots; }
+    // the mixed solver's own SpMV, whose x vector carries the padding
+    int mode_padded() const { return has_uni ? COL_UNI : mode32(); }
+    int64_t slots32() const {
+        return has_uni ? uni_stride * uni_planes : has_half ? half_pad * half_width : slots;
+    }
     Cols cols32() const {
         if (!has_half) return cols();
         Cols c = cols();
@@ -76,6 +91,11 @@ struct Sell {
         c.plen = half_len.p;
         c.pval = half_val.p;
         c.stride = pat_stride;
+        if (has_uni) {
+            c.u_nr = uni_nr;
+            for (int k = 0; k < kUniMaxRuns; ++k) c.u_roff[k] = uni_roff[k];
+            for (int k = 0; k < 3 * kUniMaxRuns; ++k) c.u_vidx[k] = uni_vidx[k];
+        }
         return c;
     }
 };
@@ -93,7 +113,9 @@ struct Sell {
 // the same for kernels on the FP32 copy, which may also use COL_HALF
 #define RSD_LAUNCH_COLS32(mode, kern, grid, block, stream, ...)                                      \
     do {                                                                                            \
-        if ((mode) == ::rsd::COL_HALF)                                                              \
+        if ((mode) == ::rsd::COL_UNI)                                                               \
+            hipLaunchKernelGGL(kern<::rsd::COL_UNI>, grid, block, 0, stream, __VA_ARGS__);          \
+        else if ((mode) == ::rsd::COL_HALF)                                                         \
             hipLaunchKernelGGL(kern<::rsd::COL_HALF>, grid, block, 0, stream, __VA_ARGS__);         \
         else                                                                                        \
             RSD_LAUNCH_COLS(mode, kern, grid, block, stream, __VA_ARGS__);                          \

@@ -21,7 +21,13 @@ namespace rsd {
 //              column c >= r that is the row's own m-th upper value
 //              (pval = m n_pad), for c < r the value row c keeps for
 //              column r (pval = m' n_pad + c - r).
-enum ColMode : int { COL_ABS = 0, COL_OFF16 = 1, COL_PAT = 2, COL_HALF = 3 };
+enum ColMode : int { COL_ABS = 0, COL_OFF16 = 1, COL_PAT = 2, COL_HALF = 3, COL_UNI = 4 };
+
+// uniform stencil of the FP32 half storage (COL_UNI): the union of every
+// row's column offsets is u_nr runs of 3 consecutive offsets (9 for hex Q1,
+// 3 for quad Q1); entry 3k+i of every row has column r + u_roff[k] + i and
+// value index r + u_vidx[3k+i], rows lacking the entry hold a zero value
+constexpr int kUniMaxRuns = 9;
 
 struct Cols {
     const int64_t* slice_ptr = nullptr;
@@ -33,6 +39,9 @@ struct Cols {
     const int32_t* plen = nullptr;   // [n_patterns]
     const int32_t* pval = nullptr;   // [n_patterns * stride] value index - row (COL_HALF)
     int32_t stride = 0;
+    int32_t u_nr = 0;                        // COL_UNI runs (3 or 9)
+    int32_t u_roff[kUniMaxRuns] = {};        // first column offset of each run
+    int32_t u_vidx[3 * kUniMaxRuns] = {};    // value index - row of each entry
 };
 
 // Matrix values and column indices are read once per product: they are
@@ -164,10 +173,48 @@ __device__ __forceinline__ float row_dot_half(int64_t r, const Cols& C, const fl
     return run(C.poff + p * C.stride, C.pval + p * C.stride, C.plen[p]);
 }
 
+// Uniform stencil (COL_UNI): every row reads the same NR runs of 3 entries,
+// so there is no pattern id and no table load: the value indices and the
+// run offsets are kernel arguments, and each run's three x values come in
+// one 12-byte load.  x must be readable (and finite) from x - lo to x + n +
+// hi (the layout's padding): a row lacking an entry multiplies a zero value
+// by an x value outside its stencil.  The row sum takes the entries in
+// column order like row_dot_half, plus fmaf(0, x, sum) for the missing
+// ones, which return sum unchanged (sum starts at +0 and never becomes -0):
+// bitwise row_dot_half's result.
+typedef float rsd_float3u __attribute__((ext_vector_type(3), aligned(4)));
+
+template<int NR>
+__device__ __forceinline__ float row_dot_uni(int64_t r, const Cols& C, const float* __restrict__ val,
+                                             const float* __restrict__ x) {
+    const uint32_t ru = (uint32_t)r;
+    const float* xr = x + r;
+    float sum = 0.0f;
+#pragma unroll
+    for (int k0 = 0; k0 < NR; k0 += 3) {
+        rsd_float3u xv[3];
+        float v[9];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            xv[k] = *(const rsd_float3u*)(xr + C.u_roff[k0 + k]);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) v[3 * k + i] = val[ru + (uint32_t)C.u_vidx[3 * (k0 + k) + i]];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            sum = __builtin_fmaf(v[3 * k], xv[k].x, sum);
+            sum = __builtin_fmaf(v[3 * k + 1], xv[k].y, sum);
+            sum = __builtin_fmaf(v[3 * k + 2], xv[k].z, sum);
+        }
+    }
+    return sum;
+}
+
 // FP32 values and vectors, FP32 accumulation (fused multiply-adds)
 template<int MODE>
 __device__ __forceinline__ float row_dot_f32(int64_t r, const Cols& C, const float* __restrict__ val,
                                              const float* __restrict__ x) {
+    if constexpr (MODE == COL_UNI) return C.u_nr == 9 ? row_dot_uni<9>(r, C, val, x) : row_dot_uni<3>(r, C, val, x);
     if constexpr (MODE == COL_HALF) return row_dot_half(r, C, val, x);
     constexpr int U = kRowUnroll;
     const int64_t base = C.slice_ptr[r >> 6] + (r & 63);

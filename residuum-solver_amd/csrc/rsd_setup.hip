@@ -453,13 +453,60 @@ void rsd::build_half(rsd_problem* pb) {
     const int32_t width = n ? *std::max_element(nup.begin(), nup.end()) : 0;
     const int64_t pad = (n + kSlice - 1) / kSlice * kSlice;
     if (width == 0 || pad * (int64_t)width >= (int64_t)INT32_MAX) return;
-    std::vector<int32_t> tval(np * stride, 0);
+    std::vector<int32_t> tval(np * stride, 0), tdst(np * stride, 0);
     for (int64_t q = 0; q < np; ++q)
         for (int32_t j = 0; j < tlen[q]; ++j) {
             const int32_t o = toff[q * stride + j];
             tval[q * stride + j] = (int32_t)(tmir[q * stride + j] * pad + std::min(o, 0));
+            tdst[q * stride + j] = (int32_t)(tmir[q * stride + j] * pad);
         }
+    // Uniform stencil: when the union of the rows' offsets is 3 or 9 runs of
+    // 3 consecutive offsets, closed under negation, the values go to one
+    // dense plane per non-negative offset (zeros where a row lacks it) and
+    // the tables are rewritten to index those planes
+    std::vector<int32_t> uni;
+    for (int64_t q = 0; q < np; ++q) uni.insert(uni.end(), toff.begin() + q * stride, toff.begin() + q * stride + tlen[q]);
+    std::sort(uni.begin(), uni.end());
+    uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
+    bool ok = (uni.size() == 9 || uni.size() == 27);
+    for (size_t k = 0; ok && k < uni.size(); k += 3) ok = uni[k + 1] == uni[k] + 1 && uni[k + 2] == uni[k] + 2;
+    for (size_t k = 0; ok && k < uni.size(); ++k) ok = std::binary_search(uni.begin(), uni.end(), -uni[k]);
+    std::unordered_map<int32_t, int32_t> plane;
+    int64_t front = 0, ustride = 0;
+    int32_t nplanes = 0;
+    if (ok) {
+        for (int32_t o : uni)
+            if (o >= 0) plane[o] = nplanes++;
+        front = -(int64_t)uni.front();
+        ustride = (front + pad + kSlice - 1) / kSlice * kSlice;
+        ok = ustride * nplanes < (int64_t)INT32_MAX;
+    }
+    auto vidx = [&](int32_t o) -> int32_t {  // value index - row of the entry with offset o
+        return o >= 0 ? (int32_t)(plane[o] * ustride + front) : (int32_t)(plane[-o] * ustride + front + o);
+    };
+    if (ok) {
+        for (int64_t q = 0; q < np; ++q)
+            for (int32_t j = 0; j < tlen[q]; ++j) {
+                const int32_t o = toff[q * stride + j];
+                tval[q * stride + j] = vidx(o);
+                tdst[q * stride + j] = o >= 0 ? vidx(o) : 0;
+            }
+        S.uni_nr = (int32_t)uni.size() / 3;
+        for (int32_t k = 0; k < S.uni_nr; ++k) {
+            S.uni_roff[k] = uni[3 * k];
+            for (int i = 0; i < 3; ++i) S.uni_vidx[3 * k + i] = vidx(uni[3 * k + i]);
+        }
+        S.uni_planes = nplanes;
+        S.uni_stride = ustride;
+        S.uni_front = front;
+        // a 12-byte load at r + roff reads up to r + max offset; pad to whole
+        // 64-float groups so the vector's element 0 keeps its alignment
+        S.uni_lo = (front + 4 + kSlice - 1) / kSlice * kSlice;
+        S.uni_hi = ((int64_t)uni.back() + 4 + kSlice - 1) / kSlice * kSlice;
+        S.has_uni = true;
+    }
     hipStream_t st = pb->ctx->stream;
+    S.half_dst.upload(tdst.data(), (int64_t)tdst.size(), st);
     S.half_id.upload(id.data(), n, st);
     S.half_off.upload(toff.data(), (int64_t)toff.size(), st);
     S.half_mir.upload(tmir.data(), (int64_t)tmir.size(), st);
@@ -888,10 +935,20 @@ int rsd_problem_layout_f32(rsd_problem* pb, int64_t* out) {
         RSD_CHECK(pb && out, "null argument");
         build_half(pb);
         const Sell& S = pb->sell;
-        out[0] = S.mode32();
+        out[0] = S.has_uni ? COL_UNI : S.mode32();
         out[1] = S.has_half ? S.n_half : S.n_pat;
-        out[2] = S.half_width;
+        out[2] = S.has_uni ? S.uni_planes : S.half_width;
         out[3] = S.slots32();
+        RSD_HIP(hipStreamSynchronize(pb->ctx->stream));
+    });
+}
+
+int rsd_problem_padding_f32(rsd_problem* pb, int64_t* lo, int64_t* hi) {
+    return guard([&] {
+        RSD_CHECK(pb && lo && hi, "null argument");
+        build_half(pb);
+        *lo = pb->sell.has_uni ? pb->sell.uni_lo : 0;
+        *hi = pb->sell.has_uni ? pb->sell.uni_hi : 0;
         RSD_HIP(hipStreamSynchronize(pb->ctx->stream));
     });
 }

@@ -387,6 +387,12 @@ class Problem:
         check(lib().rsd_problem_layout(self.h, out))
         return tuple(int(v) for v in out)
 
+    def padding_f32(self):
+        """(lo, hi): the zeros rsd_spmv_f32_padded needs around x."""
+        lo, hi = C.c_int64(), C.c_int64()
+        check(lib().rsd_problem_padding_f32(self.h, C.byref(lo), C.byref(hi)))
+        return lo.value, hi.value
+
     def layout_f32(self):
         """Layout of the FP32 copy: (column mode, patterns, upper entries per
         row, slots); mode 3 is the symmetric half storage, else the FP64
@@ -596,6 +602,14 @@ class Matrix:
 
     def spmv_f32(self, x, y):
         check(lib().rsd_spmv_f32(self.pb.ctx.h, self.h, dptr(x), dptr(y)))
+
+    def spmv_f32_padded(self, xp, y):
+        """y = A x by the mixed CG's kernel; xp is a float32 DeviceArray of
+        lo + n + hi entries (Problem.padding_f32) holding x at [lo, lo + n)
+        and zeros around it."""
+        lo, hi = self.pb.padding_f32()
+        assert xp.n == lo + self.pb.n_free + hi and xp.dtype == np.float32
+        check(lib().rsd_spmv_f32_padded(self.pb.ctx.h, self.h, _vp(xp.ptr + 4 * lo), dptr(y)))
 
     def cg_mixed(self, b, x, tol=1e-8, max_iters=1000, precond=PC_JACOBI, inner_floor=0.0):
         rep = Report()

@@ -562,8 +562,13 @@ def test_fp32_half_storage(ctx, case):
         As = A
     else:
         upper = np.array([np.count_nonzero(ci[rp[i]:rp[i + 1]] >= i) for i in range(n)])
-        assert mode32 == 3 and width == upper.max()
-        assert slots == ((n + 63) // 64) * 64 * width
+        if mode32 == 4:  # uniform stencil storage: one dense plane per non-negative offset
+            offs = np.unique(ci - np.repeat(np.arange(n), np.diff(rp)))
+            assert width == np.count_nonzero(offs >= 0) and len(offs) in (9, 27)
+            assert slots >= ((n + 63) // 64) * 64 * width
+        else:
+            assert mode32 == 3 and width == upper.max()
+            assert slots == ((n + 63) // 64) * 64 * width
         As = symmetric_upper(A)
     As32 = As.astype(np.float32)
     K.to_f32()
@@ -587,13 +592,45 @@ def test_fp32_half_storage(ctx, case):
     assert rep.converged == 1 and np.linalg.norm(r) / np.linalg.norm(b) < 1e-8
 
 
+@pytest.mark.parametrize("case", CASES + ["permuted"])
+def test_spmv_f32_padded_equals_per_pattern(ctx, case):
+    """The mixed CG's own FP32 SpMV (uniform stencil: every row the same 27
+    or 9 entries, zeros for the missing ones, x read three at a time from a
+    zero-padded vector) is bitwise the per-pattern product, on every layout;
+    the padding may hold only zeros and is never written."""
+    forms = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 1.0 / 1e-3)]
+    if case == "permuted":
+        pb = permuted_problem(ctx, 6)
+    else:
+        pb, oc = build(ctx, case, kappa_seed=3)
+    K = pb.matrix()
+    pb.assemble_matrix(K, forms)
+    K.to_f32()
+    n = pb.n_free
+    lo, hi = pb.padding_f32()
+    mode32 = pb.layout_f32()[0]
+    say("layout", mode32, "padding", lo, hi)
+    assert (lo > 0 and hi > 0) == (mode32 == 4)
+    rng = np.random.default_rng(16)
+    for trial in range(3):
+        xv = rng.standard_normal(n).astype(np.float32)
+        xp = np.zeros(lo + n + hi, np.float32)
+        xp[lo:lo + n] = xv
+        dxp = ctx.from_numpy(xp)
+        y1, y2 = ctx.empty(n, np.float32), ctx.empty(n, np.float32)
+        K.spmv_f32(ctx.from_numpy(xv), y1)
+        K.spmv_f32_padded(dxp, y2)
+        assert np.array_equal(y1.numpy(), y2.numpy())
+        assert np.array_equal(dxp.numpy(), xp)
+
+
 def test_mixed_rejects_asymmetric_matrix(ctx):
     """A matrix whose mirrored pairs differ beyond rounding cannot go through
     the symmetric FP32 copy: the mixed solve fails with an argument error, and
     the same matrix made symmetric again solves."""
     import residuum_hip as rh
     pb, oc = build(ctx, CASES[2], kappa_seed=9)
-    assert pb.layout_f32()[0] == 3
+    assert pb.layout_f32()[0] in (3, 4)
     forms = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 1.0 / 1e-3)]
     val = oc.assemble_matrix(forms)
     K = pb.matrix()
