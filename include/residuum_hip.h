@@ -430,6 +430,23 @@ int rsd_batch_destroy(rsd_batch* b);
 int rsd_batch_layout(rsd_batch* b, int64_t* total_rows, int64_t* offsets);
 int rsd_cg_solve_batched(rsd_ctx* ctx, rsd_batch* b, const double* rhs, double* x, double tol, int64_t max_iters,
                          int32_t precond, int32_t use_graph, rsd_report* reps);
+/* The batch's schedule (no reference counterpart): sched[4] = iterations per
+ * chunk launch (1..256), row-product load batch (4 or 9), block order (0
+ * problem by problem, 1 interleaved across problems), XCD block remap (0/1).
+ * No schedule changes an operation or its order: results are bitwise the
+ * same under every one.  Default 8, 9, 0, 1.  rsd_batch_autotune times the
+ * 32 combinations of {4, 8, 16, 32} x {4, 9} x {0, 1} x {0, 1} on the batch
+ * itself (about `iters` iterations each from b = 1, x = 0, in its own
+ * scratch vectors, with the chunk launch, state read and synchronisation of
+ * a solve), keeps the fastest and returns it in sched (nullable) with its
+ * time per iteration in us_per_iter (nullable).  cache_path (nullable): a
+ * JSON object of schedules keyed by device architecture, problem count,
+ * rows, nonzeros, column mode and preconditioner; a hit is used without
+ * timing (*from_cache = 1), a miss is timed and written back. */
+int rsd_batch_set_schedule(rsd_batch* b, const int32_t* sched);
+int rsd_batch_get_schedule(rsd_batch* b, int32_t* sched);
+int rsd_batch_autotune(rsd_ctx* ctx, rsd_batch* b, int32_t precond, const char* cache_path, int64_t iters,
+                       int32_t* sched, double* us_per_iter, int32_t* from_cache);
 
 /* ---------------------------------------------------------------------- */
 /* seeded synthetic inputs (no reference counterpart)                       */

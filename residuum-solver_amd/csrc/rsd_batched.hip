@@ -6,15 +6,28 @@
 // own sliced-ELL arrays and stores its block partial in that problem's
 // partial range; one fold block per problem then finishes the sums.  The
 // reduction tree is the one a standalone solve of the problem uses, so each
-// batched solve is bitwise identical to rsd_cg_solve_f64 on its own.  Converged problems drop out block by block while the others
-// continue; one chunk of iterations for the whole batch is a single graph
-// launch.
+// batched solve is bitwise identical to rsd_cg_solve_f64 on its own.
+// Converged problems drop out block by block while the others continue; one
+// chunk of iterations for the whole batch is a single graph launch.
+//
+// Schedule: the chunk length, the row product's load batch, the order in
+// which the launch grid visits the problems' blocks (problem by problem, or
+// interleaved one block per problem at a time) and the XCD block remap.
+// None of them changes an operation or its order, so every schedule gives
+// the same bits; rsd_batch_autotune times them on the batch and keeps the
+// fastest, cached in a JSON file keyed by the device and the batch.
 #include "rsd_batched.hpp"
 #include "rsd_linalg.hpp"
 #include "rsd_rowdot.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
 
 namespace rsd {
 
@@ -29,55 +42,64 @@ struct BatchDev {
     const int64_t* const* slice_ptr;  // per problem
     const int32_t* const* rp;
     const int32_t* const* col;
+    const uint8_t* const* pid;        // per problem, shared offset patterns (COL_PAT)
+    const int32_t* const* poff;
+    const int32_t* const* plen;
+    const int32_t* stride;            // [np]
     const double* const* val;
     const int32_t* const* diag;
-    const int32_t* blk2prob;          // [total blocks]
+    const int32_t* blk2prob;          // [total blocks] problem of each launched block
+    const int32_t* blk2local;         // [total blocks] its block index within the problem
     const int64_t* first_block;       // [np + 1]
     const int64_t* nrows;             // [np]
     const int64_t* voff;              // [np + 1] vector segment offsets
-    __device__ __forceinline__ Cols cols(int pid) const {
+    int32_t xcd;                      // remap launched blocks XCD by XCD
+    __device__ __forceinline__ Cols cols(int p) const {
         Cols c;
-        c.slice_ptr = slice_ptr[pid];
-        c.rp = rp[pid];
-        c.col = col[pid];
+        c.slice_ptr = slice_ptr[p];
+        c.rp = rp[p];
+        c.col = col[p];
+        c.pid = pid[p];
+        c.poff = poff[p];
+        c.plen = plen[p];
+        c.stride = stride[p];
         return c;
     }
 };
 
-__device__ __forceinline__ void locate(const BatchDev& B, int64_t gb, int& pid, int64_t& lb, int64_t& nb) {
+__device__ __forceinline__ void locate(const BatchDev& B, int64_t total_blocks, int& pid, int64_t& lb, int64_t& nb) {
+    const int64_t gb = B.xcd ? xcd_block(blockIdx.x, total_blocks) : (int64_t)blockIdx.x;
     pid = B.blk2prob[gb];
-    lb = gb - B.first_block[pid];
+    lb = B.blk2local[gb];
     nb = B.first_block[pid + 1] - B.first_block[pid];
 }
 
-__global__ void k_b_inv_diag(BatchDev B, int64_t total_rows, double* __restrict__ invd) {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (g >= total_rows) return;
+__global__ void k_b_inv_diag(BatchDev B, int64_t total_blocks, double* __restrict__ invd) {
     int pid; int64_t lb, nb;
-    locate(B, g / kBlock, pid, lb, nb);
-    const int64_t r = lb * kBlock + (g % kBlock);
+    locate(B, total_blocks, pid, lb, nb);
+    const int64_t r = lb * kBlock + threadIdx.x;
+    const int64_t g = B.voff[pid] + r;
     if (r >= B.nrows[pid]) { invd[g] = 1.0; return; }
     const int32_t s = B.diag[pid][r];
     const double d = s >= 0 ? B.val[pid][s] : 0.0;
     invd[g] = (fabs(d) > 1e-300) ? 1.0 / d : 1.0;
 }
 
+template<int MODE, int U>
 __global__ void k_b_spmv(BatchDev B, int64_t total_blocks, const double* __restrict__ x, double* __restrict__ y) {
-    const int64_t gb = xcd_block(blockIdx.x, total_blocks);
     int pid; int64_t lb, nb;
-    locate(B, gb, pid, lb, nb);
+    locate(B, total_blocks, pid, lb, nb);
     const int64_t r = lb * kBlock + threadIdx.x;
     const int64_t off = B.voff[pid];
-    if (r < B.nrows[pid]) y[off + r] = row_dot_f64<COL_ABS>(r, B.cols(pid), B.val[pid], x + off);
+    if (r < B.nrows[pid]) y[off + r] = row_dot_f64<MODE, U>(r, B.cols(pid), B.val[pid], x + off);
 }
 
 __global__ __launch_bounds__(kBlock) void k_b_init(BatchDev B, int64_t total_blocks, const double* __restrict__ b,
                                                    const double* __restrict__ Ax, double* __restrict__ r,
                                                    double* __restrict__ p, const double* __restrict__ invd,
                                                    double* partials) {
-    const int64_t gb = xcd_block(blockIdx.x, total_blocks);
     int pid; int64_t lb, nb;
-    locate(B, gb, pid, lb, nb);
+    locate(B, total_blocks, pid, lb, nb);
     const int64_t rl = lb * kBlock + threadIdx.x;
     const int64_t i = B.voff[pid] + rl;
     double v[2] = {0.0, 0.0};
@@ -112,17 +134,17 @@ __global__ __launch_bounds__(kFold) void k_b_init_fold(BatchDev B, const double*
     else if (max_iters <= 1) s->stop = 1;
 }
 
+template<int MODE, int U>
 __global__ __launch_bounds__(kBlock) void k_b_spmv_pap(BatchDev B, int64_t total_blocks, const double* __restrict__ p,
                                                        double* __restrict__ Ap, const BState* st, double* partials) {
-    const int64_t gb = xcd_block(blockIdx.x, total_blocks);
     int pid; int64_t lb, nb;
-    locate(B, gb, pid, lb, nb);
+    locate(B, total_blocks, pid, lb, nb);
     if (st[pid].stop) return;
     const int64_t rl = lb * kBlock + threadIdx.x;
     const int64_t off = B.voff[pid];
     double v[1] = {0.0};
     if (rl < B.nrows[pid]) {
-        const double y = row_dot_f64<COL_ABS>(rl, B.cols(pid), B.val[pid], p + off);
+        const double y = row_dot_f64<MODE, U>(rl, B.cols(pid), B.val[pid], p + off);
         Ap[off + rl] = y;
         v[0] = p[off + rl] * y;
     }
@@ -143,9 +165,8 @@ __global__ __launch_bounds__(kBlock) void k_b_update_r(BatchDev B, int64_t total
                                                        const double* __restrict__ p, const double* __restrict__ Ap,
                                                        double* __restrict__ r, const double* __restrict__ invd,
                                                        const BState* st, double* partials) {
-    const int64_t gb = xcd_block(blockIdx.x, total_blocks);
     int pid; int64_t lb, nb;
-    locate(B, gb, pid, lb, nb);
+    locate(B, total_blocks, pid, lb, nb);
     const BState* s = st + pid;
     if (s->stop) return;
     const int64_t rl = lb * kBlock + threadIdx.x;
@@ -187,9 +208,8 @@ __global__ __launch_bounds__(kFold) void k_b_beta(BatchDev B, const double* __re
 __global__ __launch_bounds__(kBlock) void k_b_update_p(BatchDev B, int64_t total_blocks, double* __restrict__ p,
                                                        const double* __restrict__ r, const double* __restrict__ invd,
                                                        const BState* st) {
-    const int64_t gb = xcd_block(blockIdx.x, total_blocks);
     int pid; int64_t lb, nb;
-    locate(B, gb, pid, lb, nb);
+    locate(B, total_blocks, pid, lb, nb);
     const BState* s = st + pid;
     if (s->stop) return;
     const int64_t rl = lb * kBlock + threadIdx.x;
@@ -205,15 +225,29 @@ __global__ __launch_bounds__(kBlock) void k_b_update_p(BatchDev B, int64_t total
 
 using namespace rsd;
 
+namespace rsd {
+// a batch's schedule: none of its fields changes an operation or its order
+struct BSched {
+    int32_t chunk = 8;   // iterations per launch of the chunk graph
+    int32_t unroll = 9;  // row product load batch (4 or 9)
+    int32_t order = 0;   // 0: blocks problem by problem, 1: interleaved across problems
+    int32_t xcd = 1;     // remap launched blocks XCD by XCD
+};
+}  // namespace rsd
+
 struct rsd_batch {
     rsd_ctx* ctx = nullptr;
     int32_t np = 0;
-    int64_t total_rows = 0, total_blocks = 0;
+    int32_t mode = COL_ABS;  // COL_PAT when every problem has shared offset patterns
+    int64_t total_rows = 0, total_blocks = 0, total_nnz = 0;
+    rsd::BSched sched;
     std::vector<int64_t> h_voff, h_nrows, h_first;
     DBuf<const int64_t*> d_slice;
-    DBuf<const int32_t*> d_rp, d_col, d_diag;
+    DBuf<const int32_t*> d_rp, d_col, d_diag, d_poff, d_plen;
+    DBuf<const uint8_t*> d_pid;
+    DBuf<int32_t> d_stride;
     DBuf<const double*> d_val;
-    DBuf<int32_t> blk2prob;
+    DBuf<int32_t> blk2prob[2], blk2local[2];  // per block order
     DBuf<int64_t> first_block, nrows, voff;
     DBuf<double> r, p, Ap, invd;
     DBuf<BState> st;
@@ -223,17 +257,37 @@ struct rsd_batch {
         B.slice_ptr = d_slice.p;
         B.rp = d_rp.p;
         B.col = d_col.p;
+        B.pid = d_pid.p;
+        B.poff = d_poff.p;
+        B.plen = d_plen.p;
+        B.stride = d_stride.p;
         B.val = d_val.p;
         B.diag = d_diag.p;
-        B.blk2prob = blk2prob.p;
+        B.blk2prob = blk2prob[sched.order].p;
+        B.blk2local = blk2local[sched.order].p;
         B.first_block = first_block.p;
         B.nrows = nrows.p;
         B.voff = voff.p;
+        B.xcd = sched.xcd;
         return B;
     }
 };
 
 namespace rsd {
+
+// the SpMV kernels for the batch's column mode and load batch
+#define RSD_B_SPMV(kern, bt, ...)                                                                                  \
+    do {                                                                                                           \
+        const dim3 g_((unsigned)(bt)->total_blocks), b_(kBlock);                                                   \
+        hipStream_t s_ = (bt)->ctx->stream;                                                                        \
+        if ((bt)->mode == COL_PAT) {                                                                               \
+            if ((bt)->sched.unroll == 4) hipLaunchKernelGGL((kern<COL_PAT, 4>), g_, b_, 0, s_, __VA_ARGS__);       \
+            else hipLaunchKernelGGL((kern<COL_PAT, 9>), g_, b_, 0, s_, __VA_ARGS__);                               \
+        } else {                                                                                                   \
+            if ((bt)->sched.unroll == 4) hipLaunchKernelGGL((kern<COL_ABS, 4>), g_, b_, 0, s_, __VA_ARGS__);       \
+            else hipLaunchKernelGGL((kern<COL_ABS, 9>), g_, b_, 0, s_, __VA_ARGS__);                               \
+        }                                                                                                          \
+    } while (0)
 
 static void b_launch_iter(rsd_batch* bt, double* x, double tol, int64_t max_iters, int32_t precond) {
     hipStream_t st = bt->ctx->stream;
@@ -241,8 +295,7 @@ static void b_launch_iter(rsd_batch* bt, double* x, double tol, int64_t max_iter
     const int64_t nb = bt->total_blocks;
     const double* invd = precond == RSD_PC_JACOBI ? bt->invd.p : nullptr;
     const unsigned np = (unsigned)bt->np;
-    hipLaunchKernelGGL(k_b_spmv_pap, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, bt->p.p, bt->Ap.p, bt->st.p,
-                       bt->partials.p);
+    RSD_B_SPMV(k_b_spmv_pap, bt, B, nb, bt->p.p, bt->Ap.p, bt->st.p, bt->partials.p);
     hipLaunchKernelGGL(k_b_alpha, dim3(np), dim3(kFold), 0, st, B, bt->partials.p, bt->st.p);
     hipLaunchKernelGGL(k_b_update_r, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, x, bt->p.p, bt->Ap.p, bt->r.p,
                        invd, bt->st.p, bt->partials.p);
@@ -250,27 +303,48 @@ static void b_launch_iter(rsd_batch* bt, double* x, double tol, int64_t max_iter
     hipLaunchKernelGGL(k_b_update_p, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, bt->p.p, bt->r.p, invd, bt->st.p);
 }
 
-void batched_cg(rsd_batch* bt, const double* b, double* x, double tol, int64_t max_iters, int32_t precond,
-                bool use_graph, std::vector<BState>& out, int64_t chunk) {
+// the solve's start: Jacobi, r = b - A x, p, the initial scalars
+static void b_launch_init(rsd_batch* bt, const double* b, const double* x, double tol, int64_t max_iters,
+                          int32_t precond) {
     hipStream_t st = bt->ctx->stream;
     const BatchDev B = bt->dev();
     const int64_t nb = bt->total_blocks;
     if (precond == RSD_PC_JACOBI)
-        hipLaunchKernelGGL(k_b_inv_diag, dim3((unsigned)nb), dim3(kBlock), 0, st, B, bt->total_rows, bt->invd.p);
-    hipLaunchKernelGGL(k_b_spmv, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, x, bt->Ap.p);
+        hipLaunchKernelGGL(k_b_inv_diag, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, bt->invd.p);
+    RSD_B_SPMV(k_b_spmv, bt, B, nb, x, bt->Ap.p);
     hipLaunchKernelGGL(k_b_init, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, b, bt->Ap.p, bt->r.p, bt->p.p,
                        precond == RSD_PC_JACOBI ? bt->invd.p : nullptr, bt->partials.p);
     hipLaunchKernelGGL(k_b_init_fold, dim3((unsigned)bt->np), dim3(kFold), 0, st, B, bt->partials.p, bt->st.p, tol,
                        max_iters);
     RSD_HIP(hipGetLastError());
+}
+
+struct BGraph {
     hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
-    if (use_graph) {
-        RSD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int64_t k = 0; k < chunk; ++k) b_launch_iter(bt, x, tol, max_iters, precond);
-        RSD_HIP(hipStreamEndCapture(st, &g));
-        RSD_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    hipGraphExec_t e = nullptr;
+    ~BGraph() {
+        if (e) (void)hipGraphExecDestroy(e);
+        if (g) (void)hipGraphDestroy(g);
     }
+};
+
+// one chunk of iterations captured as a graph
+static void b_capture(rsd_batch* bt, double* x, double tol, int64_t max_iters, int32_t precond, BGraph& G) {
+    hipStream_t st = bt->ctx->stream;
+    RSD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int64_t k = 0; k < bt->sched.chunk; ++k) b_launch_iter(bt, x, tol, max_iters, precond);
+    RSD_HIP(hipStreamEndCapture(st, &G.g));
+    RSD_HIP(hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0));
+}
+
+void batched_cg(rsd_batch* bt, const double* b, double* x, double tol, int64_t max_iters, int32_t precond,
+                bool use_graph, std::vector<BState>& out) {
+    hipStream_t st = bt->ctx->stream;
+    const int64_t chunk = bt->sched.chunk;
+    b_launch_init(bt, b, x, tol, max_iters, precond);
+    BGraph G;
+    if (use_graph) b_capture(bt, x, tol, max_iters, precond, G);
+    hipGraphExec_t ge = G.e;
     BState* h = nullptr;
     RSD_HIP(hipHostMalloc((void**)&h, sizeof(BState) * (size_t)bt->np, hipHostMallocDefault));
     int64_t launched = 0;
@@ -290,13 +364,99 @@ void batched_cg(rsd_batch* bt, const double* b, double* x, double tol, int64_t m
         out.assign(h, h + bt->np);
     } catch (...) {
         (void)hipHostFree(h);
-        if (ge) (void)hipGraphExecDestroy(ge);
-        if (g) (void)hipGraphDestroy(g);
         throw;
     }
     (void)hipHostFree(h);
-    if (ge) (void)hipGraphExecDestroy(ge);
-    if (g) (void)hipGraphDestroy(g);
+}
+
+// ------------------------------------------------------------------------
+// autotuner
+// ------------------------------------------------------------------------
+// the cache key: the device architecture and what sets the batch's cost
+static std::string batch_key(const rsd_batch* bt, int32_t precond) {
+    hipDeviceProp_t prop;
+    int dev = 0;
+    RSD_HIP(hipGetDevice(&dev));
+    RSD_HIP(hipGetDeviceProperties(&prop, dev));
+    std::ostringstream k;
+    k << prop.gcnArchName << "/np=" << bt->np << "/rows=" << bt->total_rows << "/nnz=" << bt->total_nnz
+      << "/cols=" << (bt->mode == COL_PAT ? "pat" : "abs") << "/pc=" << (precond == RSD_PC_JACOBI ? "jacobi" : "identity");
+    return k.str();
+}
+
+struct TuneEntry {
+    BSched s;
+    double us = 0.0;
+};
+
+// The cache file holds one entry per line:
+//   "key": {"chunk": C, "unroll": U, "order": O, "xcd": X, "us_per_iter": T},
+// between a line "{" and a line "}" (a JSON object).
+static std::map<std::string, TuneEntry> read_cache(const std::string& path) {
+    std::map<std::string, TuneEntry> m;
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line)) {
+        const size_t q0 = line.find('"'), q1 = q0 == std::string::npos ? q0 : line.find('"', q0 + 1);
+        const size_t br = q1 == std::string::npos ? q1 : line.find('{', q1);
+        if (br == std::string::npos) continue;
+        TuneEntry e;
+        if (std::sscanf(line.c_str() + br, "{\"chunk\": %d, \"unroll\": %d, \"order\": %d, \"xcd\": %d, \"us_per_iter\": %lf",
+                        &e.s.chunk, &e.s.unroll, &e.s.order, &e.s.xcd, &e.us) == 5)
+            m[line.substr(q0 + 1, q1 - q0 - 1)] = e;
+    }
+    return m;
+}
+
+static void write_cache(const std::string& path, const std::map<std::string, TuneEntry>& m) {
+    const std::string tmp = path + ".tmp";
+    {
+        std::ofstream f(tmp);
+        RSD_CHECK(f.good(), "rsd_batch_autotune: cannot write the cache file");
+        f << "{\n";
+        size_t i = 0;
+        for (const auto& [k, e] : m) {
+            char buf[256];
+            std::snprintf(buf, sizeof buf, "{\"chunk\": %d, \"unroll\": %d, \"order\": %d, \"xcd\": %d, \"us_per_iter\": %.3f}",
+                          e.s.chunk, e.s.unroll, e.s.order, e.s.xcd, e.us);
+            f << "\"" << k << "\": " << buf << (++i < m.size() ? ",\n" : "\n");
+        }
+        f << "}\n";
+    }
+    RSD_CHECK(std::rename(tmp.c_str(), path.c_str()) == 0, "rsd_batch_autotune: cannot replace the cache file");
+}
+
+// time one schedule: the real loop (graph launch of a chunk, state read,
+// synchronisation) over about `iters` iterations of a problem that never
+// converges (tol 0), from b = 1, x = 0 in scratch vectors
+static double time_schedule(rsd_batch* bt, int32_t precond, double* b, double* x, int64_t iters) {
+    hipStream_t st = bt->ctx->stream;
+    const int64_t big = int64_t(1) << 40;
+    launch_fill(bt->ctx, bt->total_rows, x, 0.0);
+    b_launch_init(bt, b, x, 0.0, big, precond);
+    BGraph G;
+    b_capture(bt, x, 0.0, big, precond, G);
+    BState* h = nullptr;
+    RSD_HIP(hipHostMalloc((void**)&h, sizeof(BState) * (size_t)bt->np, hipHostMallocDefault));
+    double us = 0.0;
+    try {
+        auto round = [&] {
+            RSD_HIP(hipGraphLaunch(G.e, st));
+            RSD_HIP(hipMemcpyAsync(h, bt->st.p, sizeof(BState) * (size_t)bt->np, hipMemcpyDeviceToHost, st));
+            RSD_HIP(hipStreamSynchronize(st));
+        };
+        round();  // warm-up
+        const int64_t rounds = std::max<int64_t>(1, (iters + bt->sched.chunk - 1) / bt->sched.chunk);
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int64_t k = 0; k < rounds; ++k) round();
+        const auto t1 = std::chrono::steady_clock::now();
+        us = std::chrono::duration<double, std::micro>(t1 - t0).count() / (double)(rounds * bt->sched.chunk);
+    } catch (...) {
+        (void)hipHostFree(h);
+        throw;
+    }
+    (void)hipHostFree(h);
+    return us;
 }
 
 }  // namespace rsd
@@ -310,8 +470,11 @@ int rsd_batch_create(rsd_ctx* ctx, int32_t n_problems, rsd_matrix* const* mats, 
         bt->ctx = ctx;
         bt->np = n_problems;
         std::vector<const int64_t*> sl(n_problems);
-        std::vector<const int32_t*> rp(n_problems), col(n_problems), dg(n_problems);
+        std::vector<const int32_t*> rp(n_problems), col(n_problems), dg(n_problems), poff(n_problems), plen(n_problems);
+        std::vector<const uint8_t*> pid(n_problems);
+        std::vector<int32_t> stride(n_problems, 0);
         std::vector<const double*> val(n_problems);
+        bool all_pat = true;
         bt->h_voff.assign(n_problems + 1, 0);
         bt->h_first.assign(n_problems + 1, 0);
         bt->h_nrows.assign(n_problems, 0);
@@ -322,6 +485,12 @@ int rsd_batch_create(rsd_ctx* ctx, int32_t n_problems, rsd_matrix* const* mats, 
             rp[i] = S.row_ptr.p;
             col[i] = S.col.p;
             dg[i] = S.diag.p;
+            pid[i] = S.pat_id.p;
+            poff[i] = S.pat_off.p;
+            plen[i] = S.pat_len.p;
+            stride[i] = S.pat_stride;
+            all_pat = all_pat && S.has_pat;
+            bt->total_nnz += mats[i]->pb->nnz;
             val[i] = mats[i]->val.p;
             const int64_t nbk = std::max<int64_t>(1, nblocks(S.n_rows));
             bt->h_nrows[i] = S.n_rows;
@@ -330,16 +499,35 @@ int rsd_batch_create(rsd_ctx* ctx, int32_t n_problems, rsd_matrix* const* mats, 
         }
         bt->total_blocks = bt->h_first[n_problems];
         bt->total_rows = bt->h_voff[n_problems];
-        std::vector<int32_t> b2p(bt->total_blocks);
+        bt->mode = all_pat ? COL_PAT : COL_ABS;
+        // block orders: problem by problem, and interleaved (block k of every
+        // problem that has one, then block k + 1, ...)
+        std::vector<int32_t> b2p[2], b2l[2];
         for (int i = 0; i < n_problems; ++i)
-            for (int64_t b = bt->h_first[i]; b < bt->h_first[i + 1]; ++b) b2p[b] = i;
+            for (int64_t b = bt->h_first[i]; b < bt->h_first[i + 1]; ++b) {
+                b2p[0].push_back(i);
+                b2l[0].push_back((int32_t)(b - bt->h_first[i]));
+            }
+        for (int64_t k = 0; (int64_t)b2p[1].size() < bt->total_blocks; ++k)
+            for (int i = 0; i < n_problems; ++i)
+                if (k < bt->h_first[i + 1] - bt->h_first[i]) {
+                    b2p[1].push_back(i);
+                    b2l[1].push_back((int32_t)k);
+                }
         hipStream_t st = ctx->stream;
         bt->d_slice.upload(sl.data(), n_problems, st);
         bt->d_rp.upload(rp.data(), n_problems, st);
         bt->d_col.upload(col.data(), n_problems, st);
         bt->d_diag.upload(dg.data(), n_problems, st);
         bt->d_val.upload(val.data(), n_problems, st);
-        bt->blk2prob.upload(b2p.data(), bt->total_blocks, st);
+        for (int o = 0; o < 2; ++o) {
+            bt->blk2prob[o].upload(b2p[o].data(), bt->total_blocks, st);
+            bt->blk2local[o].upload(b2l[o].data(), bt->total_blocks, st);
+        }
+        bt->d_pid.upload(pid.data(), n_problems, st);
+        bt->d_poff.upload(poff.data(), n_problems, st);
+        bt->d_plen.upload(plen.data(), n_problems, st);
+        bt->d_stride.upload(stride.data(), n_problems, st);
         bt->first_block.upload(bt->h_first.data(), n_problems + 1, st);
         bt->nrows.upload(bt->h_nrows.data(), n_problems, st);
         bt->voff.upload(bt->h_voff.data(), n_problems + 1, st);
@@ -379,7 +567,7 @@ int rsd_cg_solve_batched(rsd_ctx* ctx, rsd_batch* bt, const double* b, double* x
         RSD_CHECK(ctx && bt && b && x && ctx == bt->ctx, "rsd_cg_solve_batched: bad argument");
         RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
         std::vector<BState> s;
-        batched_cg(bt, b, x, tol, max_iters, precond, use_graph != 0, s, 8);
+        batched_cg(bt, b, x, tol, max_iters, precond, use_graph != 0, s);
         if (reps)
             for (int i = 0; i < bt->np; ++i) {
                 CGState c{};
@@ -390,6 +578,72 @@ int rsd_cg_solve_batched(rsd_ctx* ctx, rsd_batch* bt, const double* b, double* x
                 c.stop = s[i].stop;
                 cg_report(c, max_iters, &reps[i]);
             }
+    });
+}
+
+int rsd_batch_set_schedule(rsd_batch* bt, const int32_t* sched) {
+    return guard([&] {
+        RSD_CHECK(bt && sched, "rsd_batch_set_schedule: null argument");
+        RSD_CHECK(sched[0] >= 1 && sched[0] <= 256, "chunk must be 1..256");
+        RSD_CHECK(sched[1] == 4 || sched[1] == 9, "unroll must be 4 or 9");
+        RSD_CHECK(sched[2] == 0 || sched[2] == 1, "order must be 0 or 1");
+        RSD_CHECK(sched[3] == 0 || sched[3] == 1, "xcd must be 0 or 1");
+        bt->sched = BSched{sched[0], sched[1], sched[2], sched[3]};
+    });
+}
+
+int rsd_batch_get_schedule(rsd_batch* bt, int32_t* sched) {
+    return guard([&] {
+        RSD_CHECK(bt && sched, "rsd_batch_get_schedule: null argument");
+        sched[0] = bt->sched.chunk;
+        sched[1] = bt->sched.unroll;
+        sched[2] = bt->sched.order;
+        sched[3] = bt->sched.xcd;
+    });
+}
+
+int rsd_batch_autotune(rsd_ctx* ctx, rsd_batch* bt, int32_t precond, const char* cache_path, int64_t iters,
+                       int32_t* sched, double* us_per_iter, int32_t* from_cache) {
+    return guard([&] {
+        RSD_CHECK(ctx && bt && ctx == bt->ctx, "rsd_batch_autotune: bad argument");
+        RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
+        const std::string key = batch_key(bt, precond);
+        std::map<std::string, TuneEntry> cache;
+        if (cache_path && *cache_path) cache = read_cache(cache_path);
+        TuneEntry best;
+        bool cached = false;
+        if (auto it = cache.find(key); it != cache.end()) {
+            best = it->second;
+            cached = true;
+        } else {
+            DBuf<double> b, x;
+            b.alloc(bt->total_rows);
+            x.alloc(bt->total_rows);
+            launch_fill(ctx, bt->total_rows, b.p, 1.0);
+            best.us = 1e300;
+            for (int32_t chunk : {4, 8, 16, 32})
+                for (int32_t unroll : {4, 9})
+                    for (int32_t order : {0, 1})
+                        for (int32_t xcd : {0, 1}) {
+                            bt->sched = BSched{chunk, unroll, order, xcd};
+                            const double us = time_schedule(bt, precond, b.p, x.p, std::max<int64_t>(iters, 1));
+                            if (us < best.us) best = TuneEntry{bt->sched, us};
+                        }
+            RSD_HIP(hipStreamSynchronize(ctx->stream));
+            if (cache_path && *cache_path) {
+                cache[key] = best;
+                write_cache(cache_path, cache);
+            }
+        }
+        bt->sched = best.s;
+        if (sched) {
+            sched[0] = best.s.chunk;
+            sched[1] = best.s.unroll;
+            sched[2] = best.s.order;
+            sched[3] = best.s.xcd;
+        }
+        if (us_per_iter) *us_per_iter = best.us;
+        if (from_cache) *from_cache = cached ? 1 : 0;
     });
 }
 

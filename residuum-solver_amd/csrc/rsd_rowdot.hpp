@@ -98,11 +98,10 @@ constexpr int kRowUnroll = RSD_ROW_UNROLL;
 // y_r = sum_j val * x[col] over row r, in CSR order, each product and sum
 // rounded separately: the reference's left-to-right row sum
 // (linalg/operations/backend/cpu/MatrixOps.tpp:5-23) bit for bit, whatever
-// the column mode.
-template<int MODE>
+// the column mode and the load batch U (which only groups the loads).
+template<int MODE, int U = kRowUnroll>
 __device__ __forceinline__ double row_dot_f64(int64_t r, const Cols& C, const double* __restrict__ val,
                                               const double* __restrict__ x) {
-    constexpr int U = kRowUnroll;
     const int64_t base = C.slice_ptr[r >> 6] + (r & 63);
     auto run = [&](auto colf, int len) {
         double sum = 0.0;

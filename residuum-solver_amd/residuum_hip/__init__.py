@@ -661,6 +661,28 @@ class Batch:
     def unpack(self, packed):
         return [packed[self.offsets[i]:self.offsets[i] + m.pb.n_free].copy() for i, m in enumerate(self.mats)]
 
+    @property
+    def schedule(self):
+        """(chunk, unroll, order, xcd): the batch's launch schedule."""
+        s = (C.c_int32 * 4)()
+        check(lib().rsd_batch_get_schedule(self.h, s))
+        return tuple(int(v) for v in s)
+
+    @schedule.setter
+    def schedule(self, sched):
+        check(lib().rsd_batch_set_schedule(self.h, (C.c_int32 * 4)(*[int(v) for v in sched])))
+
+    def autotune(self, precond=PC_JACOBI, cache=None, iters=64):
+        """Time every schedule on this batch and keep the fastest (a cached
+        one when `cache`, a JSON file, holds this batch's key); returns
+        (schedule, us per iteration, from_cache)."""
+        s = (C.c_int32 * 4)()
+        us = C.c_double()
+        hit = C.c_int32()
+        check(lib().rsd_batch_autotune(self.ctx.h, self.h, C.c_int32(precond), cache.encode() if cache else None,
+                                       C.c_int64(iters), s, C.byref(us), C.byref(hit)))
+        return tuple(int(v) for v in s), us.value, bool(hit.value)
+
     def cg(self, b, x, tol=1e-8, max_iters=1000, precond=PC_JACOBI, use_graph=True):
         reps = (Report * len(self.mats))()
         check(lib().rsd_cg_solve_batched(self.ctx.h, self.h, dptr(b), dptr(x), C.c_double(tol), C.c_int64(max_iters),

@@ -747,6 +747,80 @@ def test_batched_cg_equals_standalone(ctx, use_graph):
         assert same(xs[i], x1.numpy())
 
 
+def _batch_problems(ctx, sides, with_permuted=False):
+    import residuum_hip as rh
+    pbs, mats, bs = [], [], []
+    for i, s in enumerate(sides):
+        pb = rh.Problem(ctx, 3, (s, s, s), (1, 1, 1), 2, list(range(6)), ["0"] * 6)
+        pb.set_conductivity(1.0, kappa=np.exp(rh.philox_normal(20261202, i, pb.n_elem)))
+        pbs.append(pb)
+    if with_permuted:
+        pbs.append(permuted_problem(ctx, 9))
+    for i, pb in enumerate(pbs):
+        K = pb.matrix()
+        pb.assemble_matrix(K, [(O.DIFFUSION, 1.0)])
+        mats.append(K)
+        bs.append(np.random.default_rng(30 + i).standard_normal(pb.n_free))
+    return pbs, mats, bs
+
+
+@pytest.mark.parametrize("with_permuted", [False, True])
+def test_batched_schedules_bitwise(ctx, with_permuted):
+    """Every schedule (chunk, load batch, block order, XCD remap) gives the
+    same bits, and each problem equals its standalone solve; with a
+    renumbered problem in the batch the row products use explicit columns."""
+    import residuum_hip as rh
+    pbs, mats, bs = _batch_problems(ctx, [4, 7, 3, 6], with_permuted)
+    batch = rh.Batch(ctx, mats)
+    db = ctx.from_numpy(batch.pack(bs))
+    ref = []
+    for pb, K, b in zip(pbs, mats, bs):
+        x1 = pb.vector(0.0)
+        r1 = K.cg(ctx.from_numpy(b), x1, tol=1e-9, max_iters=2000)
+        ref.append((r1.iterations, x1.numpy()))
+    for chunk in (1, 4, 8, 16, 32):
+        for unroll in (4, 9):
+            for order in (0, 1):
+                for xcd in (0, 1):
+                    batch.schedule = (chunk, unroll, order, xcd)
+                    assert batch.schedule == (chunk, unroll, order, xcd)
+                    dx = ctx.from_numpy(np.zeros(batch.total_rows))
+                    reps = batch.cg(db, dx, tol=1e-9, max_iters=2000)
+                    xs = batch.unpack(dx.numpy())
+                    for i, (its, x1) in enumerate(ref):
+                        assert reps[i].converged == 1 and reps[i].iterations == its, (chunk, unroll, order, xcd, i)
+                        assert same(xs[i], x1), (chunk, unroll, order, xcd, i)
+
+
+def test_batched_autotune_cache(ctx, tmp_path):
+    """The autotuner times the 32 schedules, keeps the fastest, writes it to
+    the JSON cache; a second batch of the same shape takes it from there."""
+    import json
+    import residuum_hip as rh
+    pbs, mats, bs = _batch_problems(ctx, [5, 8, 6])
+    cache = str(tmp_path / "tune.json")
+    batch = rh.Batch(ctx, mats)
+    sched, us, hit = batch.autotune(cache=cache, iters=16)
+    say("autotune", sched, us, hit)
+    assert not hit and us > 0 and batch.schedule == sched
+    d = json.load(open(cache))
+    assert len(d) == 1
+    (key, entry), = d.items()
+    assert "/np=3/" in key and (entry["chunk"], entry["unroll"], entry["order"], entry["xcd"]) == sched
+    batch2 = rh.Batch(ctx, mats)
+    sched2, us2, hit2 = batch2.autotune(cache=cache)
+    assert hit2 and sched2 == sched and batch2.schedule == sched
+    # the tuned schedule solves bitwise like the default one
+    db = ctx.from_numpy(batch.pack(bs))
+    dx1, dx2 = ctx.from_numpy(np.zeros(batch.total_rows)), ctx.from_numpy(np.zeros(batch.total_rows))
+    batch.cg(db, dx1, tol=1e-9)
+    batch2.schedule = (8, 9, 0, 1)
+    batch2.cg(db, dx2, tol=1e-9)
+    assert same(dx1.numpy(), dx2.numpy())
+    with pytest.raises(rh.RsdError):
+        batch.schedule = (8, 5, 0, 1)
+
+
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
