@@ -132,6 +132,76 @@ def measured_traffic(n, layout):
         return None
 
 
+def run_batched(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
+    """The batched workload (not the default line): 256 steady heat problems
+    on Q1 hex unit cubes, sides drawn from 16..64 (seeded), per-element
+    conductivity lognormal(0, 1), nodal source U[0, 1], T = 0 on every face,
+    CG + Jacobi to 1e-8 from x = 0; one batched solve under the autotuned
+    schedule against the same problems solved one after another."""
+    sides = np.random.default_rng(20261202).integers(16, 65, args.batch_problems)
+    t0 = time.perf_counter()
+    pbs, mats, bs = [], [], []
+    for i, s_ in enumerate(sides):
+        s_ = int(s_)
+        pb = rh.Problem(ctx, 3, (s_, s_, s_), (1, 1, 1), 2, list(range(6)), ["0"] * 6)
+        pb.set_conductivity(1.0, kappa=np.exp(rh.philox_normal(20261202, i, pb.n_elem)))
+        K = pb.matrix()
+        pb.assemble_matrix(K, [(rh.DIFFUSION, 1.0)])
+        F = pb.vector()
+        pb.assemble_load(F, nodal=ctx.from_numpy(rh.philox_uniform(20261202, 1000 + i, pb.n_nodes)))
+        pbs.append(pb)
+        mats.append(K)
+        bs.append(F)
+    ctx.sync()
+    setup_s = time.perf_counter() - t0
+    rows = [pb.n_free for pb in pbs]
+    nnz = sum(pb.nnz for pb in pbs)
+    batch = rh.Batch(ctx, mats)
+    sched, tune_us, hit = batch.autotune(cache=args.tune_cache, iters=32)
+    db = ctx.from_numpy(batch.pack([F.numpy() for F in bs]))
+    dx = ctx.from_numpy(np.zeros(batch.total_rows))
+    batch.cg(db, dx, tol=1e-8, max_iters=5000)  # warm-up
+    dx = ctx.from_numpy(np.zeros(batch.total_rows))
+    ctx.sync()
+    a, b = new_ev(), new_ev()
+    rec_ev(a, stream)
+    reps = batch.cg(db, dx, tol=1e-8, max_iters=5000)
+    rec_ev(b, stream)
+    batched_ms = ev_ms(a, b)
+    its = [int(r.iterations) for r in reps]
+    assert all(r.converged for r in reps)
+    xs = batch.unpack(dx.numpy())
+    # the same problems one after another
+    xs1 = [pb.vector(0.0) for pb in pbs]
+    ctx.sync()
+    rec_ev(a, stream)
+    its1 = [int(K.cg(F, x1, tol=1e-8, max_iters=5000).iterations) for K, F, x1 in zip(mats, bs, xs1)]
+    rec_ev(b, stream)
+    single_ms = ev_ms(a, b)
+    same_bits = all(np.array_equal(x, x1.numpy()) for x, x1 in zip(xs, xs1)) and its == its1
+    dof_iters = float(sum(n_ * k for n_, k in zip(rows, its)))
+    print(f"# batched: {len(pbs)} problems, {sum(rows)} rows, {nnz} nnz, setup {setup_s:.1f} s; schedule {sched} "
+          f"({'cached' if hit else 'tuned'}, {tune_us:.0f} us/iter); batched {batched_ms:.1f} ms, one by one "
+          f"{single_ms:.1f} ms; bitwise equal: {same_bits}", file=sys.stderr, flush=True)
+    line = {
+        "metric": "batched CG DOF-iters/s (256 uneven Q1 hex problems, CG+Jacobi to 1e-8)",
+        "value": dof_iters / (batched_ms * 1e-3),
+        "unit": "DOF-iters/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic (seeded sides, lognormal(0,1) conductivity, U[0,1] nodal source)",
+        "config": {"workload": "batched steady heat, %d Q1 hex unit cubes, sides 16..64" % len(pbs),
+                   "rows": int(sum(rows)), "nnz": int(nnz)},
+        "batched_ms": batched_ms, "one_by_one_ms": single_ms, "speedup": single_ms / batched_ms,
+        "iterations": {"sum": int(sum(its)), "max": int(max(its))},
+        "schedule": {"chunk": sched[0], "unroll": sched[1], "order": sched[2], "xcd": sched[3],
+                     "from_cache": hit, "tuned_us_per_iter": tune_us},
+        "bitwise_equal_to_one_by_one": bool(same_bits),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +212,10 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=320, help="oracle CG iterations timed for the CPU baseline (about 10 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-matrix-free", action="store_true", help="skip the matrix-free operator solve")
+    ap.add_argument("--workload", choices=["be128", "batched"], default="be128",
+                    help="be128: the Backward Euler line (default); batched: the batched-solve workload")
+    ap.add_argument("--batch-problems", type=int, default=256)
+    ap.add_argument("--tune-cache", default=None, help="JSON schedule cache for the batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -153,6 +227,9 @@ def main():
     ctx = rh.Ctx(0)
     stream = C.c_void_p()
     rh.check(rh.lib().rsd_ctx_stream(ctx.h, C.byref(stream)))
+    if args.workload == "batched":
+        run_batched(args, rh, ctx, stream, new_ev, rec_ev, ev_ms)
+        return
     n = args.n
     dt = 1e-3
     t_setup = time.perf_counter()
