@@ -134,17 +134,17 @@ def measured_traffic(n, layout):
 
 def run_batched(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
     """The batched workload (not the default line): 256 steady heat problems
-    on Q1 hex unit cubes, sides drawn from 16..64 (seeded), per-element
+    on Q1 hex unit cubes, sides 16..64 (Philox, seed 20261202), per-element
     conductivity lognormal(0, 1), nodal source U[0, 1], T = 0 on every face,
     CG + Jacobi to 1e-8 from x = 0; one batched solve under the autotuned
     schedule against the same problems solved one after another."""
-    sides = np.random.default_rng(20261202).integers(16, 65, args.batch_problems)
+    sides = (16 + np.floor(rh.philox_uniform(20261202, 0, args.batch_problems) * 49)).astype(int)
     t0 = time.perf_counter()
     pbs, mats, bs = [], [], []
     for i, s_ in enumerate(sides):
         s_ = int(s_)
         pb = rh.Problem(ctx, 3, (s_, s_, s_), (1, 1, 1), 2, list(range(6)), ["0"] * 6)
-        pb.set_conductivity(1.0, kappa=np.exp(rh.philox_normal(20261202, i, pb.n_elem)))
+        pb.set_conductivity(1.0, kappa=np.exp(rh.philox_normal(20261202, 1 + i, pb.n_elem)))
         K = pb.matrix()
         pb.assemble_matrix(K, [(rh.DIFFUSION, 1.0)])
         F = pb.vector()
