@@ -22,8 +22,11 @@ namespace rsd {
 // true: thread a also forms entry a of the unit mass form's element vector
 // on the full gather of A.mvU, with k_asm_vector's chain of products and
 // sums, and adds it to A.mvF[row(a)] (the same colour order, so bitwise the
-// separate vector assembly).
-template<int D, int P, int NQ, bool STATE, bool DIAG = false, bool MV = false>
+// separate vector assembly).  EB = true: the element pass of the row gather
+// assembly, over every element in index order: thread a writes row a of Ke
+// to A.ebuf[(e * NPE + a) * NPE + b] (and the mass entry to A.fbuf[e * NPE
+// + a]) instead of scattering it.
+template<int D, int P, int NQ, bool STATE, bool DIAG = false, bool MV = false, bool EB = false>
 __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(AsmArgs A, Model m, int64_t c0,
                                                                              int64_t c1) {
     using C = ECfg<D, P, NQ, STATE>;
@@ -40,7 +43,7 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
     const bool has_kap = A.kappa != nullptr;
     double Um[MV ? NPE : 1];
     if (valid) {
-        e = A.order[k];
+        e = EB ? k : A.order[k];
         int32_t nodes[NPE];
         double X[NPE][D];
         load_element<D, P>(A, e, nodes, X);
@@ -116,6 +119,13 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
             }
         }
     }
+    if constexpr (EB) {
+        double* out = A.ebuf + (e * NPE + a) * NPE;
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) out[b] = Ke[b];
+        if constexpr (MV) A.fbuf[e * NPE + a] = Fm;
+        return;
+    }
     if constexpr (MV) {
         const int32_t row = A.t2a[node_a];
         if (row >= 0) A.mvF[row] += Fm;
@@ -130,6 +140,59 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
             const int32_t s = mp[b];
             if (s >= 0) A.val[s] += Ke[b];
         }
+    }
+}
+
+// Row gather pass: one lane per SELL row, one wavefront per slice.  Lane r
+// walks the row's elements in colour order and adds element row a's
+// entries into a wavefront-private LDS accumulator (acc[k * 64 + lane],
+// lane-private, so no barrier), at the row slot position of each column;
+// then writes every slot of its row once, coalesced across the slice, the
+// padding slots and padding rows as zeros.  Each entry is 0 plus its
+// contributions in colour order: bitwise the colour-by-colour scatter into
+// a zeroed value array.  MV: the element mass vectors, summed the same way
+// into mvF[r].  MJ bounds the elements per row (the host picks MJ >=
+// gat_m).
+template<int NPE, int MJ, bool MV>
+__global__ __launch_bounds__(kSlice) void k_gather_rows(const int32_t* __restrict__ gel,
+                                                        const uint32_t* __restrict__ gk, int32_t M,
+                                                        const double* __restrict__ eb, const double* __restrict__ fb,
+                                                        const int64_t* __restrict__ slice_ptr, int64_t n_rows,
+                                                        double* __restrict__ val, double* __restrict__ mvF) {
+    constexpr int K4 = (NPE + 3) / 4;
+    extern __shared__ double acc[];
+    const int lane = threadIdx.x;
+    const int64_t s = blockIdx.x;
+    const int64_t base = slice_ptr[s];
+    const int W = (int)((slice_ptr[s + 1] - base) / kSlice);
+    for (int k = 0; k < W; ++k) acc[k * kSlice + lane] = 0.0;
+    const int32_t* ge = gel + s * M * kSlice + lane;
+    const uint32_t* gw = gk + s * M * K4 * kSlice + lane;
+    int32_t q[MJ];
+#pragma unroll
+    for (int j = 0; j < MJ; ++j) q[j] = j < M ? ge[j * kSlice] : -1;
+    double fm = 0.0;
+#pragma unroll
+    for (int j = 0; j < MJ; ++j) {
+        if (q[j] < 0) break;
+        const double* er = eb + (int64_t)q[j] * NPE;
+        double v[NPE];
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) v[b] = er[b];
+        uint32_t kw[K4];
+#pragma unroll
+        for (int t = 0; t < K4; ++t) kw[t] = gw[(j * K4 + t) * kSlice];
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) {
+            const uint32_t k = (kw[b / 4] >> (8 * (b % 4))) & 0xffu;
+            if (k != 0xffu) acc[k * kSlice + lane] += v[b];
+        }
+        if constexpr (MV) fm += fb[q[j]];
+    }
+    for (int k = 0; k < W; ++k) val[base + k * kSlice + lane] = acc[k * kSlice + lane];
+    if constexpr (MV) {
+        const int64_t r = s * kSlice + lane;
+        if (r < n_rows) mvF[r] = fm;
     }
 }
 
@@ -317,11 +380,59 @@ static void run_matrix_t(rsd_problem* pb, const AsmArgs& A) {
     RSD_HIP(hipGetLastError());
 }
 
+template<int NPE, int MJ, bool MV>
+static void launch_gather_rows(rsd_problem* pb, const AsmArgs& A) {
+    const Sell& S = pb->sell;
+    hipLaunchKernelGGL((k_gather_rows<NPE, MJ, MV>), dim3((unsigned)S.n_slices), dim3(kSlice),
+                       (size_t)pb->max_row * kSlice * sizeof(double), pb->ctx->stream, pb->gat_el.p, pb->gat_k.p,
+                       pb->gat_m, A.ebuf, A.fbuf, S.slice_ptr.p, S.n_rows, A.val, A.mvF);
+}
+
+static bool use_gather(const rsd_problem* pb) { return pb->gather_on && pb->gat_m > 0; }
+
+void gather_reserve(rsd_problem* pb, bool with_mass) {
+    if (!use_gather(pb)) return;
+    const int64_t ne = pb->n_elem, npe = pb->npe;
+    if (pb->ebuf.n < ne * npe * npe) pb->ebuf.alloc(ne * npe * npe);
+    if (with_mass && pb->fbuf.n < ne * npe) pb->fbuf.alloc(ne * npe);
+}
+
+// row gather assembly: the element pass (one launch over all elements),
+// then the row pass
+template<int D, int P, int NQ, bool STATE, bool MV>
+static void run_matrix_gather(rsd_problem* pb, AsmArgs A) {
+    using C = ECfg<D, P, NQ, STATE>;
+    constexpr int NPE = C::NPE;
+    gather_reserve(pb, MV);
+    A.ebuf = pb->ebuf.p;
+    A.fbuf = MV ? pb->fbuf.p : nullptr;
+    const int64_t ne = pb->n_elem;
+    hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, STATE, false, MV, true>), dim3((unsigned)((ne + C::EPB - 1) / C::EPB)),
+                       dim3(C::BLOCK), 0, pb->ctx->stream, A, pb->model, (int64_t)0, ne);
+    RSD_HIP(hipGetLastError());
+    const int M = pb->gat_m;
+    if (M <= 4) launch_gather_rows<NPE, 4, MV>(pb, A);
+    else if (M <= 8) launch_gather_rows<NPE, 8, MV>(pb, A);
+    else launch_gather_rows<NPE, kGatherMaxElems, MV>(pb, A);
+    RSD_HIP(hipGetLastError());
+}
+
 // A.F set: the diagonal (assembleDiagonal) into A.F; else the matrix into
-// A.val, with the mass vector of A.mvU into A.mvF when A.mvF is set
+// A.val, with the mass vector of A.mvU into A.mvF when A.mvF is set (row
+// gather assembly when the problem has its tables, else colour by colour)
 template<int D, int P, int NQ>
 static void run_matrix(rsd_problem* pb, const AsmArgs& A) {
     const bool diag = A.F != nullptr, mv = A.mvF != nullptr;
+    if (!diag && use_gather(pb)) {
+        if (A.need_state) {
+            if (mv) run_matrix_gather<D, P, NQ, true, true>(pb, A);
+            else run_matrix_gather<D, P, NQ, true, false>(pb, A);
+        } else {
+            if (mv) run_matrix_gather<D, P, NQ, false, true>(pb, A);
+            else run_matrix_gather<D, P, NQ, false, false>(pb, A);
+        }
+        return;
+    }
     if (A.need_state) {
         if (diag) run_matrix_t<D, P, NQ, true, true>(pb, A);
         else if (mv) run_matrix_t<D, P, NQ, true, false, true>(pb, A);
@@ -446,7 +557,7 @@ void assemble_matrix(rsd_problem* pb, double time, int32_t nf, const int32_t* ki
     A.time = time;
     A.need_state = forms_need_state(pb, A);
     update_gval(pb, time);
-    launch_fill(pb->ctx, K->val.n, K->val.p, 0.0);
+    if (!use_gather(pb)) launch_fill(pb->ctx, K->val.n, K->val.p, 0.0);  // the row pass writes every slot
     dispatch_matrix(pb, A);
 }
 
@@ -486,8 +597,10 @@ void assemble_matrix_mass(rsd_problem* pb, double time, int32_t nf, const int32_
     A.mvF = Fm;
     // the state gather (if any) and the mass gather read the same values
     update_gval(pb, A.need_state ? time : time_m);
-    launch_fill(pb->ctx, K->val.n, K->val.p, 0.0);
-    launch_fill(pb->ctx, pb->n_free, Fm, 0.0);
+    if (!use_gather(pb)) {  // the row pass writes every slot and every entry of Fm
+        launch_fill(pb->ctx, K->val.n, K->val.p, 0.0);
+        launch_fill(pb->ctx, pb->n_free, Fm, 0.0);
+    }
     dispatch_matrix(pb, A);
 }
 

@@ -28,6 +28,8 @@ struct AsmArgs {
     const int32_t* stop;  // nullable: vector kernels return at once when *stop != 0
     const double* mvU;    // matrix kernels, nullable: also scatter (M_e Ue)[a] of mvU into mvF
     double* mvF;
+    double* ebuf;         // matrix kernels writing element matrices (row gather assembly)
+    double* fbuf;         // and, with mvU, the element mass vectors
 };
 
 AsmArgs base_args(rsd_problem* pb);
@@ -51,6 +53,10 @@ void assemble_matrix_mass(rsd_problem* pb, double time, int32_t nf, const int32_
 // whether assemble_matrix_mass applies: one set of Dirichlet values serves
 // both gathers
 bool matrix_mass_fusable(const rsd_problem* pb, int32_t nf, const int32_t* kinds);
+// reserves the element buffers of the row gather assembly (call before a
+// stream capture that assembles matrices); with_mass: also the element mass
+// vectors of assemble_matrix_mass
+void gather_reserve(rsd_problem* pb, bool with_mass);
 void assemble_load(rsd_problem* pb, double time, const Expr* src, const double* nodal, double* F);
 void apply_essential(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs, double* F);
 const Expr* cached_source(rsd_problem* pb, const char* text);

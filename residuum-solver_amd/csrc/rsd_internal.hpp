@@ -58,6 +58,8 @@ int guard(F&& f) {
 // ------------------------------------------------------------------------
 constexpr int kBlock = 256;       // threads per block, 4 wave64
 constexpr int kSlice = 64;        // SELL slice height = one wavefront
+constexpr int kGatherMaxRow = 64;    // widest row the row gather assembly accumulates (LDS: 32 KB per wavefront)
+constexpr int kGatherMaxElems = 16;  // most elements around a node the row gather assembly takes
 constexpr int kXcds = 8;          // MI355X: 8 XCDs, blocks dispatched round-robin
 
 inline int64_t nblocks(int64_t n) { return (n + kBlock - 1) / kBlock; }

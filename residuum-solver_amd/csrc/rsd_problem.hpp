@@ -156,6 +156,16 @@ struct rsd_problem {
     rsd::DBuf<int32_t> border;    // elements with a constrained node, colour-major
     rsd::DBuf<int32_t> map_sell;  // [n_elem * npe * npe] SELL slot or -1
     rsd::DBuf<int32_t> csr_slot;  // [nnz] SELL slot of each CSR entry (matrices made from a CSR)
+    // row gather assembly: per free row (SELL order, 64-row slices) the row's
+    // elements in colour order, at most one per colour; gat_m = 0 when the
+    // tables are not built
+    int32_t gat_m = 0;            // element slots per row
+    bool gather_on = true;        // matrices assemble by row gather when the tables exist
+    rsd::DBuf<int32_t> gat_el;    // [n_slices * gat_m * 64] e * npe + a, -1 past the row's elements
+    rsd::DBuf<uint32_t> gat_k;    // [n_slices * gat_m * k4 * 64] row slot position of column b (byte b % 4 of
+                                  // dword b / 4, k4 = ceil(npe / 4)); 0xff: constrained column
+    rsd::DBuf<double> ebuf;       // [n_elem * npe * npe] element matrices, reserved on first use
+    rsd::DBuf<double> fbuf;       // [n_elem * npe] element mass vectors, reserved on first use
     rsd::DBuf<double> gval;       // [n_nodes] Dirichlet value (0 on free nodes)
     rsd::DBuf<double> kappa;      // [n_elem] or empty
     const double* kappa_ext = nullptr;  // device kappa supplied by the caller

@@ -676,6 +676,69 @@ static void face_nodes(int dim, int p, int f, std::vector<int>& out) {
         }
 }
 
+// Row gather tables.  Free row r (slice r / 64, lane r % 64) lists the
+// elements around its node in colour order -- distinct colours, since
+// elements sharing a node never share one -- as e * npe + a (a: the node's
+// local index), and for each local column b the position of column
+// t2a(ien[e][b]) in the row's sorted column list (0xff when constrained).
+// Summing the element rows in that order gives every entry its
+// contributions in the colour order of the element scatter.  Not built
+// when an element repeats a node, a row is wider than the gather kernel's
+// accumulator or e * npe overflows int32.
+static void build_gather(rsd_problem* pb, const std::vector<int64_t>& n2e_ptr, const std::vector<int32_t>& n2e) {
+    pb->gat_m = 0;
+    const int npe = pb->npe;
+    const int64_t n = pb->n_free, ne = pb->n_elem;
+    const Sell& S = pb->sell;
+    if (n == 0 || ne == 0 || pb->max_row > kGatherMaxRow || ne * npe >= (int64_t)INT32_MAX) return;
+    for (int64_t e = 0; e < ne; ++e)
+        for (int a = 0; a < npe; ++a)
+            for (int b = 0; b < a; ++b)
+                if (pb->h_ien[e * npe + a] == pb->h_ien[e * npe + b]) return;
+    int64_t M = 0;
+    for (int64_t r = 0; r < n; ++r) {
+        const int32_t v = pb->h_a2t[r];
+        M = std::max(M, n2e_ptr[v + 1] - n2e_ptr[v]);
+    }
+    if (M == 0 || M > kGatherMaxElems) return;
+    const int k4 = (npe + 3) / 4;
+    std::vector<int32_t> gel((size_t)S.n_slices * M * kSlice, -1);
+    std::vector<uint32_t> gk((size_t)S.n_slices * M * k4 * kSlice, 0xffffffffu);
+    std::atomic<bool> ok{true};
+    parallel_for(n, [&](int64_t lo, int64_t hi) {
+        std::vector<std::pair<int32_t, int32_t>> el;
+        for (int64_t r = lo; r < hi; ++r) {
+            const int32_t v = pb->h_a2t[r];
+            el.clear();
+            for (int64_t k = n2e_ptr[v]; k < n2e_ptr[v + 1]; ++k) el.emplace_back(pb->h_colour[n2e[k]], n2e[k]);
+            std::sort(el.begin(), el.end());
+            const int64_t s = r / kSlice, lane = r % kSlice;
+            const int32_t* beg = pb->h_ci.data() + pb->h_rp[r];
+            const int32_t* end = pb->h_ci.data() + pb->h_rp[r + 1];
+            for (size_t j = 0; j < el.size(); ++j) {
+                if (j && el[j].first == el[j - 1].first) ok = false;
+                const int64_t e = el[j].second;
+                int a = 0;
+                while (a < npe && pb->h_ien[e * npe + a] != v) ++a;
+                gel[((size_t)s * M + j) * kSlice + lane] = (int32_t)(e * npe + a);
+                for (int b = 0; b < npe; ++b) {
+                    const int32_t c = pb->h_t2a[pb->h_ien[e * npe + b]];
+                    uint32_t kb = 0xffu;
+                    if (c >= 0) kb = (uint32_t)(std::lower_bound(beg, end, c) - beg);
+                    uint32_t& w = gk[(((size_t)s * M + j) * k4 + b / 4) * kSlice + lane];
+                    w = (w & ~(0xffu << (8 * (b % 4)))) | (kb << (8 * (b % 4)));
+                }
+            }
+        }
+    });
+    if (!ok) return;
+    hipStream_t st = pb->ctx->stream;
+    pb->gat_el.upload(gel.data(), (int64_t)gel.size(), st);
+    pb->gat_k.upload(gk.data(), (int64_t)gk.size(), st);
+    RSD_HIP(hipStreamSynchronize(st));
+    pb->gat_m = (int32_t)M;
+}
+
 static void build_problem(rsd_problem* pb) {
     const int dim = pb->dim, npe = pb->npe;
     const int64_t ne = pb->n_elem, nn = pb->n_nodes;
@@ -809,6 +872,7 @@ static void build_problem(rsd_problem* pb) {
         }
         pb->border.upload(border.data(), (int64_t)border.size(), st);
     }
+    build_gather(pb, n2e_ptr, n2e);
     // device copies
     pb->xyz.upload(pb->h_xyz.data(), nn * dim, st);
     pb->ien.upload(pb->h_ien.data(), ne * npe, st);
@@ -839,6 +903,20 @@ static void init_problem(rsd_ctx* ctx, rsd_problem* pb, int32_t dim, const int32
         if (e.uses_t) pb->bc_time_dependent = true;
         pb->bc_exprs.push_back(e);
     }
+}
+
+int rsd_problem_set_assembly(rsd_problem* pb, int32_t mode, int32_t* in_use) {
+    return guard([&] {
+        RSD_CHECK(pb, "rsd_problem_set_assembly: null problem");
+        RSD_CHECK(mode == 0 || mode == 1, "rsd_problem_set_assembly: mode must be 0 (colour scatter) or 1 (row gather)");
+        RSD_HIP(hipStreamSynchronize(pb->ctx->stream));
+        pb->gather_on = mode == 1;
+        if (!pb->gather_on) {  // the element buffers are of no further use
+            pb->ebuf.reset();
+            pb->fbuf.reset();
+        }
+        if (in_use) *in_use = pb->gather_on && pb->gat_m > 0 ? 1 : 0;
+    });
 }
 
 int rsd_problem_create_block(rsd_ctx* ctx, int32_t dim, const int64_t* nel, const double* ext, const int32_t* p,

@@ -387,6 +387,13 @@ class Problem:
         check(lib().rsd_problem_layout(self.h, out))
         return tuple(int(v) for v in out)
 
+    def set_assembly(self, mode):
+        """Matrix assembly scheme: 1 row gather (default), 0 colour-by-colour
+        element scatter; returns 1 when the row gather is in use."""
+        n = C.c_int32()
+        check(lib().rsd_problem_set_assembly(self.h, C.c_int32(mode), C.byref(n)))
+        return n.value
+
     def padding_f32(self):
         """(lo, hi): the zeros rsd_spmv_f32_padded needs around x."""
         lo, hi = C.c_int64(), C.c_int64()

@@ -128,6 +128,68 @@ def test_assemble_matrix_bitexact(ctx, case, fs):
     assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
 
 
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+@pytest.mark.parametrize("case", CASES + [(3, (5, 4, 9), (1, 1, 1), 2, [0, 4], ["x", "1"], None)])
+def test_assembly_schemes_bitwise(ctx, case):
+    """The two matrix assembly schemes -- row gather (element pass, then one
+    pass per row summing its elements' rows in colour order) and the
+    colour-by-colour element scatter -- give the same bits for every form
+    set and for a temperature-dependent Jacobian; a product with each
+    matrix agrees bitwise too, so the padding slots hold the same zeros.
+    The gather runs first, then the scatter, then the gather again on a
+    problem whose value arrays were written by the other scheme."""
+    import residuum_hip as rh
+    dim, n, p, nq, tags, exprs, ext = case
+    pb, oc = build(ctx, case, kappa_seed=17)
+    assert pb.set_assembly(1) == 1
+    x = ctx.from_numpy(np.random.default_rng(3).standard_normal(oc.n_free))
+    got = {}
+    K = pb.matrix()
+    for mode in (1, 0, 1):
+        pb.set_assembly(mode)
+        for fs in sorted(FORMSETS):
+            pb.assemble_matrix(K, FORMSETS[fs])
+            y = pb.vector()
+            K.spmv(x, y)
+            got.setdefault(fs, []).append((bits(K.values()), bits(y.numpy())))
+    for fs, runs in got.items():
+        say("form set", fs)
+        for v, y in runs[1:]:
+            assert np.array_equal(v, runs[0][0]) and np.array_equal(y, runs[0][1])
+    if dim == 2:
+        tensor = (1.0, 0.2, 0.2, 0.1)
+        pj = rh.Problem(ctx, dim, n, p, nq, tags, exprs, ext=ext)
+        pj.set_conductivity(1.0, k_expr="1.0 + 0.5*T", dk_expr="0.5", tensor=tensor)
+        U = ctx.from_numpy(np.random.default_rng(2).standard_normal(oc.n_free))
+        forms = [(O.DIFFUSION, 1.0), (O.TDIFFUSION, 1.0)]
+        J = pj.matrix()
+        vals = []
+        for mode in (1, 0):
+            pj.set_assembly(mode)
+            pj.assemble_matrix(J, forms, U=U)
+            vals.append(bits(J.values()))
+        assert np.array_equal(vals[0], vals[1])
+
+
+def test_be_run_assembly_schemes_bitwise(ctx):
+    """Backward Euler through the fused K + M/dt and M U_prev assembly,
+    captured as a graph: both assembly schemes give the oracle's solution
+    bit for bit."""
+    case = (3, (6, 5, 10), (1, 1, 1), 2, [0, 1], ["1", "0"], None)
+    pb, oc = build(ctx, case, kappa_seed=20261202)
+    out = oc.be_run(0.0, 0.005, 1e-3, cg_tol=1e-8)
+    for mode in (1, 0):
+        pb.set_assembly(mode)
+        U = pb.vector(0.0)
+        steps, iters = pb.be_run(U, 0.0, 0.005, 1e-3, tol=1e-8, use_graph=True)
+        say("assembly", mode, "iters", list(iters))
+        assert steps == out["steps"] == 5 and list(iters) == list(out["iters"])
+        assert np.array_equal(bits(U.numpy()), bits(out["U"]))
+
+
 @pytest.mark.parametrize("case", CASES[:2])
 def test_assemble_jacobian_tdep_bitexact(ctx, case):
     import residuum_hip as rh
