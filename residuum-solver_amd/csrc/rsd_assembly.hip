@@ -26,9 +26,19 @@ namespace rsd {
 // assembly, over every element in index order: thread a writes row a of Ke
 // to A.ebuf[(e * NPE + a) * NPE + b] (and the mass entry to A.fbuf[e * NPE
 // + a]) instead of scattering it.
-template<int D, int P, int NQ, bool STATE, bool DIAG = false, bool MV = false, bool EB = false>
+//
+// FPAIR = true: the Backward Euler form pair (unscaled isotropic diffusion,
+// then the scaled mass form) on a state-free model, straight-line: per
+// point and column b, Ke[b] += (sum_i da_i (k db_i)) meas w, then Ke[b] +=
+// coef ((rho cp N_a N_b) meas w) -- each entry's chain of the generic loop,
+// except that the integrand's sum starts from its first product instead of
+// 0 + first product.  The two differ only in the sign of a zero integrand,
+// and Ke[b], a sum started at +0, absorbs that: x + (+0) and x + (-0) are
+// both x for x != 0, and both +0 for x = +0 (a sum started at +0 is never -0).
+template<int D, int P, int NQ, bool STATE, bool DIAG = false, bool MV = false, bool EB = false, bool FPAIR = false>
 __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(AsmArgs A, Model m, int64_t c0,
                                                                              int64_t c1) {
+    static_assert(!FPAIR || (!STATE && !DIAG), "the form pair path is for state-free matrices");
     using C = ECfg<D, P, NQ, STATE>;
     constexpr int NPE = C::NPE;
     constexpr int NB = DIAG ? 1 : NPE;  // entries of row a this thread forms
@@ -65,6 +75,34 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
 #pragma unroll
     for (int i = 0; i < NB; ++i) Ke[i] = 0.0;
     double Fm = 0.0;
+    if constexpr (FPAIR) {
+        const double coef = A.coefs[1];
+#pragma unroll 1
+        for (int q = 0; q < C::NQP; ++q) {
+            const double* r = rec + q * C::REC;
+            const double* tq = A.ntab + q * (NPE + 1);  // uniform: scalar loads
+            const double w = tq[NPE];
+            double N[NPE];
+#pragma unroll
+            for (int b = 0; b < NPE; ++b) N[b] = tq[b];
+            const double meas = r[C::R_MEAS];
+            const double ra = m.rho * m.cp * tq[a];
+            double da[D];
+#pragma unroll
+            for (int i = 0; i < D; ++i) da[i] = r[C::R_DN + a * D + i];
+#pragma unroll
+            for (int b = 0; b < NPE; ++b) {
+                const double rn = ra * N[b];
+                if constexpr (MV) Fm += rn * Um[b] * meas * w;
+                const double* db = r + C::R_DN + b * D;
+                double it = da[0] * (kel * db[0]);
+#pragma unroll
+                for (int i = 1; i < D; ++i) it += da[i] * (kel * db[i]);
+                Ke[b] += it * meas * w;
+                Ke[b] += coef * (rn * meas * w);
+            }
+        }
+    } else
     for (int q = 0; q < C::NQP; ++q) {
         const double* r = rec + q * C::REC;
         double xi[3];
@@ -146,13 +184,17 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
 // Row gather pass: one lane per SELL row, one wavefront per slice.  Lane r
 // walks the row's elements in colour order and adds element row a's
 // entries into a wavefront-private LDS accumulator (acc[k * 64 + lane],
-// lane-private, so no barrier), at the row slot position of each column;
+// lane-private, so no barrier) at the row slot position k of each column;
 // then writes every slot of its row once, coalesced across the slice, the
 // padding slots and padding rows as zeros.  Each entry is 0 plus its
 // contributions in colour order: bitwise the colour-by-colour scatter into
-// a zeroed value array.  MV: the element mass vectors, summed the same way
-// into mvF[r].  MJ bounds the elements per row (the host picks MJ >=
-// gat_m).
+// a zeroed value array.  Branch-free: a constrained column, and every
+// column of a missing element (past the row's count, read from element row
+// 0), goes to dummy slot W + b, so one element's NPE slots are distinct and
+// its reads, adds and writes issue as three batches.  MV: the element mass
+// vectors summed the same way into mvF[r] (a missing element adds +0, a
+// no-op on a sum started at +0).  MJ bounds the elements per row (the host
+// picks MJ >= gat_m).
 template<int NPE, int MJ, bool MV>
 __global__ __launch_bounds__(kSlice) void k_gather_rows(const int32_t* __restrict__ gel,
                                                         const uint32_t* __restrict__ gk, int32_t M,
@@ -160,7 +202,7 @@ __global__ __launch_bounds__(kSlice) void k_gather_rows(const int32_t* __restric
                                                         const int64_t* __restrict__ slice_ptr, int64_t n_rows,
                                                         double* __restrict__ val, double* __restrict__ mvF) {
     constexpr int K4 = (NPE + 3) / 4;
-    extern __shared__ double acc[];
+    extern __shared__ double acc[];  // [(W + NPE) * 64]: the row slots, then the dummy slots
     const int lane = threadIdx.x;
     const int64_t s = blockIdx.x;
     const int64_t base = slice_ptr[s];
@@ -169,25 +211,40 @@ __global__ __launch_bounds__(kSlice) void k_gather_rows(const int32_t* __restric
     const int32_t* ge = gel + s * M * kSlice + lane;
     const uint32_t* gw = gk + s * M * K4 * kSlice + lane;
     int32_t q[MJ];
+    uint32_t kw[MJ][K4];
 #pragma unroll
-    for (int j = 0; j < MJ; ++j) q[j] = j < M ? ge[j * kSlice] : -1;
+    for (int j = 0; j < MJ; ++j) {
+        q[j] = j < M ? ge[j * kSlice] : -1;
+#pragma unroll
+        for (int t = 0; t < K4; ++t) kw[j][t] = j < M ? gw[(j * K4 + t) * kSlice] : 0xffffffffu;
+    }
+    double v[MJ][NPE];
+    double fv[MJ];
+#pragma unroll
+    for (int j = 0; j < MJ; ++j) {
+        const int64_t row = q[j] < 0 ? 0 : q[j];
+        const double* er = eb + row * NPE;
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) v[j][b] = er[b];
+        if constexpr (MV) fv[j] = fb[row];
+    }
     double fm = 0.0;
 #pragma unroll
     for (int j = 0; j < MJ; ++j) {
-        if (q[j] < 0) break;
-        const double* er = eb + (int64_t)q[j] * NPE;
-        double v[NPE];
-#pragma unroll
-        for (int b = 0; b < NPE; ++b) v[b] = er[b];
-        uint32_t kw[K4];
-#pragma unroll
-        for (int t = 0; t < K4; ++t) kw[t] = gw[(j * K4 + t) * kSlice];
+        int at[NPE];
+        double t[NPE];
 #pragma unroll
         for (int b = 0; b < NPE; ++b) {
-            const uint32_t k = (kw[b / 4] >> (8 * (b % 4))) & 0xffu;
-            if (k != 0xffu) acc[k * kSlice + lane] += v[b];
+            const uint32_t k = (kw[j][b / 4] >> (8 * (b % 4))) & 0xffu;
+            at[b] = ((k == 0xffu) ? W + b : (int)k) * kSlice + lane;
         }
-        if constexpr (MV) fm += fb[q[j]];
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) t[b] = acc[at[b]];
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) t[b] += v[j][b];
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) acc[at[b]] = t[b];
+        if constexpr (MV) fm += q[j] < 0 ? 0.0 : fv[j];
     }
     for (int k = 0; k < W; ++k) val[base + k * kSlice + lane] = acc[k * kSlice + lane];
     if constexpr (MV) {
@@ -366,17 +423,54 @@ static void run_adj(rsd_problem* pb, const AsmArgs& A, const Model& unit, const 
     RSD_HIP(hipGetLastError());
 }
 
+// the Backward Euler form pair on a state-free isotropic model
+static bool form_pair(const rsd_problem* pb, const AsmArgs& A) {
+    return !A.need_state && !pb->model.aniso && A.nf == 2 && A.kinds[0] == RSD_FORM_DIFFUSION &&
+           A.kinds[1] == (RSD_FORM_MASS | RSD_FORM_SCALED);
+}
+
 template<int D, int P, int NQ, bool STATE, bool DIAG, bool MV = false>
 static void run_matrix_t(rsd_problem* pb, const AsmArgs& A) {
     using C = ECfg<D, P, NQ, STATE>;
     hipStream_t st = pb->ctx->stream;
+    const bool pair = !STATE && !DIAG && form_pair(pb, A);
     for (int c = 0; c < pb->n_colours(); ++c) {
         const int64_t c0 = pb->colour_off[c], c1 = pb->colour_off[c + 1];
         if (c1 <= c0) continue;
-        hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, STATE, DIAG, MV>),
-                           dim3((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB)), dim3(C::BLOCK), 0, st, A, pb->model,
-                           c0, c1);
+        const dim3 grid((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB));
+        if constexpr (!STATE && !DIAG) {
+            if (pair) {
+                hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, false, false, MV, false, true>), grid, dim3(C::BLOCK), 0,
+                                   st, A, pb->model, c0, c1);
+                continue;
+            }
+        }
+        hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, STATE, DIAG, MV>), grid, dim3(C::BLOCK), 0, st, A, pb->model, c0,
+                           c1);
     }
+    RSD_HIP(hipGetLastError());
+}
+
+template<int D, int P, int NQ>
+__global__ void k_shape_table(double* __restrict__ tab) {
+    constexpr int NPE = Elem<D, P>::NPE;
+    constexpr int NQP = D == 2 ? NQ * NQ : NQ * NQ * NQ;
+    const int q = threadIdx.x;
+    if (q >= NQP) return;
+    double xi[3];
+    const double w = point_coords<D, NQ>(q, xi);
+    double N[NPE];
+    shape_values<D, P>(xi, N);
+    for (int b = 0; b < NPE; ++b) tab[q * (NPE + 1) + b] = N[b];
+    tab[q * (NPE + 1) + NPE] = w;
+}
+
+template<int D, int P, int NQ>
+static void run_shape_table(rsd_problem* pb) {
+    constexpr int NPE = Elem<D, P>::NPE;
+    constexpr int NQP = D == 2 ? NQ * NQ : NQ * NQ * NQ;
+    pb->ntab.alloc(NQP * (NPE + 1));
+    hipLaunchKernelGGL((k_shape_table<D, P, NQ>), dim3(1), dim3(128), 0, pb->ctx->stream, pb->ntab.p);
     RSD_HIP(hipGetLastError());
 }
 
@@ -384,7 +478,7 @@ template<int NPE, int MJ, bool MV>
 static void launch_gather_rows(rsd_problem* pb, const AsmArgs& A) {
     const Sell& S = pb->sell;
     hipLaunchKernelGGL((k_gather_rows<NPE, MJ, MV>), dim3((unsigned)S.n_slices), dim3(kSlice),
-                       (size_t)pb->max_row * kSlice * sizeof(double), pb->ctx->stream, pb->gat_el.p, pb->gat_k.p,
+                       (size_t)(pb->max_row + NPE) * kSlice * sizeof(double), pb->ctx->stream, pb->gat_el.p, pb->gat_k.p,
                        pb->gat_m, A.ebuf, A.fbuf, S.slice_ptr.p, S.n_rows, A.val, A.mvF);
 }
 
@@ -407,8 +501,18 @@ static void run_matrix_gather(rsd_problem* pb, AsmArgs A) {
     A.ebuf = pb->ebuf.p;
     A.fbuf = MV ? pb->fbuf.p : nullptr;
     const int64_t ne = pb->n_elem;
-    hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, STATE, false, MV, true>), dim3((unsigned)((ne + C::EPB - 1) / C::EPB)),
-                       dim3(C::BLOCK), 0, pb->ctx->stream, A, pb->model, (int64_t)0, ne);
+    const dim3 grid((unsigned)((ne + C::EPB - 1) / C::EPB));
+    bool done = false;
+    if constexpr (!STATE) {
+        if (form_pair(pb, A)) {
+            hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, false, false, MV, true, true>), grid, dim3(C::BLOCK), 0,
+                               pb->ctx->stream, A, pb->model, (int64_t)0, ne);
+            done = true;
+        }
+    }
+    if (!done)
+        hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, STATE, false, MV, true>), grid, dim3(C::BLOCK), 0, pb->ctx->stream,
+                           A, pb->model, (int64_t)0, ne);
     RSD_HIP(hipGetLastError());
     const int M = pb->gat_m;
     if (M <= 4) launch_gather_rows<NPE, 4, MV>(pb, A);
@@ -501,6 +605,19 @@ static void dispatch_matrix(rsd_problem* pb, const AsmArgs& A) {
     throw Error(RSD_ERR_ARG, "discretisation not supported by the HIP element kernels (dim/p/nq)");
 }
 
+void build_shape_table(rsd_problem* pb) {
+    if (pb->dim == 2 && pb->p == 1 && pb->nq == 1) { run_shape_table<2, 1, 1>(pb); return; }
+    if (pb->dim == 2 && pb->p == 1 && pb->nq == 2) { run_shape_table<2, 1, 2>(pb); return; }
+    if (pb->dim == 2 && pb->p == 1 && pb->nq == 3) { run_shape_table<2, 1, 3>(pb); return; }
+    if (pb->dim == 2 && pb->p == 2 && pb->nq == 2) { run_shape_table<2, 2, 2>(pb); return; }
+    if (pb->dim == 2 && pb->p == 2 && pb->nq == 3) { run_shape_table<2, 2, 3>(pb); return; }
+    if (pb->dim == 2 && pb->p == 2 && pb->nq == 4) { run_shape_table<2, 2, 4>(pb); return; }
+    if (pb->dim == 3 && pb->p == 1 && pb->nq == 1) { run_shape_table<3, 1, 1>(pb); return; }
+    if (pb->dim == 3 && pb->p == 1 && pb->nq == 2) { run_shape_table<3, 1, 2>(pb); return; }
+    if (pb->dim == 3 && pb->p == 1 && pb->nq == 3) { run_shape_table<3, 1, 3>(pb); return; }
+    pb->ntab.reset();  // other discretisations have no element kernels
+}
+
 static void dispatch_vector(rsd_problem* pb, const AsmArgs& A, const Expr& src, int mode) {
     if (pb->dim == 2 && pb->p == 1 && pb->nq == 1) { run_vector<2, 1, 1>(pb, A, src, mode); return; }
     if (pb->dim == 2 && pb->p == 1 && pb->nq == 2) { run_vector<2, 1, 2>(pb, A, src, mode); return; }
@@ -523,6 +640,7 @@ AsmArgs base_args(rsd_problem* pb) {
     A.order = pb->order.p;
     A.map = pb->map_sell.p;
     A.kappa = pb->kappa_ptr();
+    A.ntab = pb->ntab.p;
     return A;
 }
 

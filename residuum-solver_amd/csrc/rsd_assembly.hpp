@@ -30,6 +30,7 @@ struct AsmArgs {
     double* mvF;
     double* ebuf;         // matrix kernels writing element matrices (row gather assembly)
     double* fbuf;         // and, with mvU, the element mass vectors
+    const double* ntab;   // Gauss point table: N_b (b < npe) and w per point, npe + 1 doubles a point
 };
 
 AsmArgs base_args(rsd_problem* pb);
@@ -57,6 +58,9 @@ bool matrix_mass_fusable(const rsd_problem* pb, int32_t nf, const int32_t* kinds
 // stream capture that assembles matrices); with_mass: also the element mass
 // vectors of assemble_matrix_mass
 void gather_reserve(rsd_problem* pb, bool with_mass);
+// the Gauss point table (shape values and weights), computed on the device
+// by the element kernels' own functions; part of problem set-up
+void build_shape_table(rsd_problem* pb);
 void assemble_load(rsd_problem* pb, double time, const Expr* src, const double* nodal, double* F);
 void apply_essential(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs, double* F);
 const Expr* cached_source(rsd_problem* pb, const char* text);

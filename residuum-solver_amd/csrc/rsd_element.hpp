@@ -222,7 +222,13 @@ struct ECfg {
     static constexpr int R_DN = 0, R_MEAS = GRAD ? NPE * D : 0, R_SRC = R_MEAS + 1;
     static constexpr int R_K = R_SRC + 1, R_DK = R_K + 1, R_TR = R_DK + 1, R_GT = R_TR + 1;
     static constexpr int REC = STATE ? R_GT + D : R_SRC + 1;
-    static constexpr int ELEM_DOUBLES = NQP * REC;
+    // element records ELEM_DOUBLES apart, a stride of 2 mod 16 doubles (4
+    // mod 32 dwords): the 8 elements of a wavefront, whose threads read the
+    // same record offset in one instruction, start on 8 distinct groups of 4
+    // LDS banks instead of one (an unpadded stride that is a multiple of 16
+    // doubles puts them all on the same bank)
+    static constexpr int ELEM_RAW = NQP * REC;
+    static constexpr int ELEM_DOUBLES = ELEM_RAW + (18 - ELEM_RAW % 16) % 16;
     static constexpr int LDS_BUDGET = 40 * 1024;
     static constexpr int EPB_RAW = LDS_BUDGET / (ELEM_DOUBLES * 8);
     static constexpr int EPB_MAX = 256 / NPE;

@@ -166,6 +166,7 @@ struct rsd_problem {
                                   // dword b / 4, k4 = ceil(npe / 4)); 0xff: constrained column
     rsd::DBuf<double> ebuf;       // [n_elem * npe * npe] element matrices, reserved on first use
     rsd::DBuf<double> fbuf;       // [n_elem * npe] element mass vectors, reserved on first use
+    rsd::DBuf<double> ntab;       // [n_qp * (npe + 1)] shape values N_b and weight w of each Gauss point
     rsd::DBuf<double> gval;       // [n_nodes] Dirichlet value (0 on free nodes)
     rsd::DBuf<double> kappa;      // [n_elem] or empty
     const double* kappa_ext = nullptr;  // device kappa supplied by the caller

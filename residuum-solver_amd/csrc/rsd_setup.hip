@@ -7,6 +7,7 @@
 // oracle and golden vectors); the pattern build is node-centric and
 // multi-threaded rather than the reference's two-pass touch list.
 #include "rsd_problem.hpp"
+#include "rsd_assembly.hpp"
 
 #include <algorithm>
 #include <cctype>
@@ -873,6 +874,7 @@ static void build_problem(rsd_problem* pb) {
         pb->border.upload(border.data(), (int64_t)border.size(), st);
     }
     build_gather(pb, n2e_ptr, n2e);
+    build_shape_table(pb);
     // device copies
     pb->xyz.upload(pb->h_xyz.data(), nn * dim, st);
     pb->ien.upload(pb->h_ien.data(), ne * npe, st);
