@@ -137,7 +137,9 @@ def run_batched(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
     on Q1 hex unit cubes, sides 16..64 (Philox, seed 20261202), per-element
     conductivity lognormal(0, 1), nodal source U[0, 1], T = 0 on every face,
     CG + Jacobi to 1e-8 from x = 0; one batched solve under the autotuned
-    schedule against the same problems solved one after another."""
+    schedule against the same problems solved one after another, then the
+    batched mixed-precision solve (FP32 bulk, FP64 refinement) against the
+    same FP64 solves."""
     sides = (16 + np.floor(rh.philox_uniform(20261202, 0, args.batch_problems) * 49)).astype(int)
     t0 = time.perf_counter()
     pbs, mats, bs = [], [], []
@@ -183,6 +185,29 @@ def run_batched(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
     print(f"# batched: {len(pbs)} problems, {sum(rows)} rows, {nnz} nnz, setup {setup_s:.1f} s; schedule {sched} "
           f"({'cached' if hit else 'tuned'}, {tune_us:.0f} us/iter); batched {batched_ms:.1f} ms, one by one "
           f"{single_ms:.1f} ms; bitwise equal: {same_bits}", file=sys.stderr, flush=True)
+    # FP32 bulk with FP64 refinement: the batched mixed solve (first call
+    # converts the FP32 copies and sets up the tables, so it is a warm-up)
+    dm = ctx.from_numpy(np.zeros(batch.total_rows))
+    batch.cg_mixed(db, dm, tol=1e-8, max_iters=5000)
+    dm = ctx.from_numpy(np.zeros(batch.total_rows))
+    ctx.sync()
+    rec_ev(a, stream)
+    mreps, mcyc = batch.cg_mixed(db, dm, tol=1e-8, max_iters=5000)
+    rec_ev(b, stream)
+    mixed_ms = ev_ms(a, b)
+    assert all(r.converged for r in mreps)
+    mits = [int(r.iterations) for r in mreps]
+    xm = batch.unpack(dm.numpy())
+    m_same = True
+    for K, F, x, it_, c_ in zip(mats, bs, xm, mits, mcyc):
+        x1 = K.pb.vector(0.0)
+        r1, c1 = K.cg_mixed(F, x1, tol=1e-8, max_iters=5000)
+        m_same = m_same and np.array_equal(x, x1.numpy()) and int(r1.iterations) == it_ and c1 == c_
+    m_diff = max(float(np.max(np.abs(x - x64)) / max(np.max(np.abs(x64)), 1e-300)) for x, x64 in zip(xm, xs))
+    m_dof_iters = float(sum(n_ * k for n_, k in zip(rows, mits)))
+    print(f"# batched mixed: {mixed_ms:.1f} ms ({sum(mits)} inner iterations, {max(mits)} at most, outer cycles "
+          f"{min(mcyc)}..{max(mcyc)}); bitwise equal to the standalone mixed solves: {m_same}; max relative "
+          f"difference to FP64 {m_diff:.2e}", file=sys.stderr, flush=True)
     line = {
         "metric": "batched CG DOF-iters/s (256 uneven Q1 hex problems, CG+Jacobi to 1e-8)",
         "value": dof_iters / (batched_ms * 1e-3),
@@ -198,6 +223,11 @@ def run_batched(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
         "schedule": {"chunk": sched[0], "unroll": sched[1], "order": sched[2], "xcd": sched[3],
                      "from_cache": hit, "tuned_us_per_iter": tune_us},
         "bitwise_equal_to_one_by_one": bool(same_bits),
+        "mixed": {"batched_ms": mixed_ms, "dof_iters_per_s": m_dof_iters / (mixed_ms * 1e-3),
+                  "iterations": {"sum": int(sum(mits)), "max": int(max(mits))},
+                  "outer_cycles": {"min": int(min(mcyc)), "max": int(max(mcyc))},
+                  "vs_one_by_one_fp64": single_ms / mixed_ms, "vs_batched_fp64": batched_ms / mixed_ms,
+                  "bitwise_equal_to_standalone_mixed": bool(m_same), "max_rel_diff_vs_fp64": m_diff},
     }
     print(json.dumps(line), flush=True)
 

@@ -440,6 +440,18 @@ int rsd_batch_destroy(rsd_batch* b);
 int rsd_batch_layout(rsd_batch* b, int64_t* total_rows, int64_t* offsets);
 int rsd_cg_solve_batched(rsd_ctx* ctx, rsd_batch* b, const double* rhs, double* x, double tol, int64_t max_iters,
                          int32_t precond, int32_t use_graph, rsd_report* reps);
+/* Batched mixed precision (no reference counterpart): every problem of the
+ * batch runs rsd_cg_solve_mixed -- FP32 inner CG on the FP32 copy (refreshed
+ * from each matrix's values by every call), FP64 outer residual -- in one
+ * set of launches; each problem's result, report and outer cycle count
+ * (outer_cycles: host [n_problems], nullable) are bitwise those of
+ * rsd_cg_solve_mixed on that problem alone.  The problems' FP32 layouts
+ * (rsd_problem_layout_f32) must agree; RSD_ERR_ARG names a problem whose
+ * matrix fails the FP32 copy's symmetry check.  The batch's schedule
+ * applies (chunk: inner iterations per graph launch). */
+int rsd_cg_solve_batched_mixed(rsd_ctx* ctx, rsd_batch* b, const double* rhs, double* x, double tol,
+                               int64_t max_iters, int32_t precond, double inner_floor, rsd_report* reps,
+                               int64_t* outer_cycles);
 /* The batch's schedule (no reference counterpart): sched[4] = iterations per
  * chunk launch (1..256), row-product load batch (4 or 9), block order (0
  * problem by problem, 1 interleaved across problems), XCD block remap (0/1).

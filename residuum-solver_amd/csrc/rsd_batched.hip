@@ -18,12 +18,14 @@
 // fastest, cached in a JSON file keyed by the device and the batch.
 #include "rsd_batched.hpp"
 #include "rsd_linalg.hpp"
+#include "rsd_mixed.hpp"
 #include "rsd_rowdot.hpp"
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <map>
 #include <sstream>
@@ -221,6 +223,260 @@ __global__ __launch_bounds__(kBlock) void k_b_update_p(BatchDev B, int64_t total
     }
 }
 
+// ------------------------------------------------------------------------
+// Batched mixed precision: each problem runs the standalone mixed solve
+// (rsd_mixed.hip) -- FP64 outer residual, FP32 inner CG on the problem's
+// FP32 copy -- with the outer loop's decisions (first residual,
+// convergence, inner tolerance and cap) made per problem on the device, in
+// the host's double arithmetic.  The FP64 kernels use the 256-row block
+// table, the inner set-up and update the 1024-row tile table of a
+// standalone mixed solve; each problem's partials sit in its own range,
+// laid out as a standalone solve's, so every problem is bitwise its
+// rsd_cg_solve_mixed.  The FP32 products walk the half layout's per-pattern
+// tables, bitwise the uniform stencil kernel a standalone solve uses.
+// ------------------------------------------------------------------------
+struct BMState {
+    MixedState m;
+    int32_t done;  // the problem's outer loop has ended (converged or given up)
+};
+
+struct BMixDev {
+    const Cols* c32;             // [np] FP32 columns
+    const float* const* val32;   // [np]
+    const int32_t* t2prob;       // [total tile blocks]
+    const int32_t* t2local;
+    const int64_t* first_t;      // [np + 1]
+    const int64_t* pbase;        // [np] partial range: nb block sums, then 2 nbu
+};
+
+__device__ __forceinline__ void locate_t(const BatchDev& B, const BMixDev& M, int64_t total_t, int& pid, int64_t& lb,
+                                         int64_t& nbu) {
+    const int64_t gb = B.xcd ? xcd_block(blockIdx.x, total_t) : (int64_t)blockIdx.x;
+    pid = M.t2prob[gb];
+    lb = M.t2local[gb];
+    nbu = M.first_t[pid + 1] - M.first_t[pid];
+}
+
+__global__ void k_bm_inv_diag(BatchDev B, int64_t total_blocks, float* __restrict__ invd) {
+    int pid; int64_t lb, nb;
+    locate(B, total_blocks, pid, lb, nb);
+    const int64_t r = lb * kBlock + threadIdx.x;
+    const int64_t g = B.voff[pid] + r;
+    if (r >= B.nrows[pid]) { invd[g] = 1.0f; return; }
+    const int32_t s = B.diag[pid][r];
+    const double d = s >= 0 ? B.val[pid][s] : 0.0;
+    invd[g] = (float)((fabs(d) > 1e-300) ? 1.0 / d : 1.0);
+}
+
+// r = b - A x (FP64), block sums of ||r||^2
+template<int MODE, int U>
+__global__ __launch_bounds__(kBlock) void k_bm_residual(BatchDev B, BMixDev M, int64_t total_blocks,
+                                                        const double* __restrict__ b, const double* __restrict__ x,
+                                                        double* __restrict__ r64, const BMState* st, double* part) {
+    int pid; int64_t lb, nb;
+    locate(B, total_blocks, pid, lb, nb);
+    if (st[pid].done) return;
+    const int64_t rl = lb * kBlock + threadIdx.x;
+    const int64_t off = B.voff[pid];
+    double v[1] = {0.0};
+    if (rl < B.nrows[pid]) {
+        const double ri = b[off + rl] + -1.0 * row_dot_f64<MODE, U>(rl, B.cols(pid), B.val[pid], x + off);
+        r64[off + rl] = ri;
+        v[0] = ri * ri;
+    }
+    block_partials<1>(v, lb, nb, part + M.pbase[pid]);
+}
+
+// one block per problem: the outer loop's step of the standalone solve
+__global__ __launch_bounds__(kFold) void k_bm_outer(BatchDev B, BMixDev M, const double* __restrict__ part,
+                                                    BMState* st, double tol, int64_t max_iters, double inner_floor) {
+    const int pid = blockIdx.x;
+    BMState* s = st + pid;
+    if (s->done) return;
+    const int64_t nb = B.first_block[pid + 1] - B.first_block[pid];
+    double out[1];
+    fold_partials<1>(nb, part + M.pbase[pid], out);
+    if (threadIdx.x != 0) return;
+    MixedState& m = s->m;
+    const double res = sqrt(out[0]);
+    if (m.cycles == 0) m.res0 = res;
+    m.res = res;
+    const double rel = m.res / (m.res0 + 1e-50);
+    if (rel < tol) {
+        m.converged = 1;
+        m.stop = 1;
+        s->done = 1;
+        return;
+    }
+    if (m.cycles >= kMixedMaxCycles || m.total_inner >= max_iters || !(rel == rel)) {
+        m.stop = 1;
+        s->done = 1;
+        return;
+    }
+    double it = 0.5 * tol * (m.res0 + 1e-50) / m.res;
+    it = (inner_floor < it) ? it : inner_floor;  // max(inner_floor, it)
+    it = (it < 0.5) ? it : 0.5;                  // min(0.5, it)
+    m.in_tol = it;
+    m.in_max = max_iters - m.total_inner > 1 ? max_iters - m.total_inner : 1;
+    m.cycles = m.cycles + 1;
+    m.it = 0;
+    m.stop = 0;
+}
+
+// inner set-up (k_mx_init per problem)
+__global__ __launch_bounds__(kBlock) void k_bm_init(BatchDev B, BMixDev M, int64_t total_t,
+                                                    const double* __restrict__ r64, float* __restrict__ u,
+                                                    float* __restrict__ p, float* __restrict__ s, float* __restrict__ d,
+                                                    const float* __restrict__ invd, const BMState* st, double* part) {
+    int pid; int64_t lb, nbu;
+    locate_t(B, M, total_t, pid, lb, nbu);
+    const BMState* bs = st + pid;
+    if (bs->done) return;
+    const int64_t n = B.nrows[pid], off = B.voff[pid];
+    const double sc = bs->m.res > 0.0 ? 1.0 / bs->m.res : 0.0;
+    double v[2] = {0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < kMxTiles; ++t) {
+        const int64_t rl = (lb * kMxTiles + t) * kBlock + threadIdx.x;
+        if (rl < n) {
+            const int64_t i = off + rl;
+            const float ri = (float)(r64[i] * sc);
+            const float ui = invd ? ri * invd[i] : ri;
+            u[i] = ui;
+            p[i] = 0.0f;
+            s[i] = 0.0f;
+            d[i] = 0.0f;
+            v[0] += (double)ri * (double)ui;
+            v[1] += (double)ri * (double)ri;
+        }
+    }
+    const int64_t nb = B.first_block[pid + 1] - B.first_block[pid];
+    block_partials<2>(v, lb, nbu, part + M.pbase[pid] + nb);
+}
+
+// w = A u (FP32), block sums of w.u (k_mx_spmv_wu per problem)
+template<int MODE>
+__global__ __launch_bounds__(kBlock) void k_bm_spmv_wu(BatchDev B, BMixDev M, int64_t total_blocks,
+                                                       const float* __restrict__ u, float* __restrict__ w,
+                                                       const BMState* st, double* part) {
+    int pid; int64_t lb, nb;
+    locate(B, total_blocks, pid, lb, nb);
+    if (st[pid].m.stop) return;
+    const int64_t rl = lb * kBlock + threadIdx.x;
+    const int64_t off = B.voff[pid];
+    double v[1] = {0.0};
+    if (rl < B.nrows[pid]) {
+        const float wi = row_dot_f32<MODE>(rl, M.c32[pid], M.val32[pid], u + off);
+        w[off + rl] = wi;
+        v[0] = (double)wi * (double)u[off + rl];
+    }
+    block_partials<1>(v, lb, nb, part + M.pbase[pid]);
+}
+
+// one block per problem: k_mx_scalars
+__global__ __launch_bounds__(kFold) void k_bm_scalars(BatchDev B, BMixDev M, const double* __restrict__ part,
+                                                      BMState* st) {
+    const int pid = blockIdx.x;
+    MixedState* ms = &st[pid].m;
+    if (ms->stop) return;
+    const int64_t nb = B.first_block[pid + 1] - B.first_block[pid], nbu = M.first_t[pid + 1] - M.first_t[pid];
+    const double* P0 = part + M.pbase[pid];
+    const double* P12 = P0 + nb;
+    const int64_t k = ms->it;
+    const double gam_prev = ms->gam, alp_prev = ms->alp, in_res0 = ms->in_res0, in_tol = ms->in_tol;
+    const int64_t in_max = ms->in_max;
+    double o[3];
+    const double* pk[3] = {P0, P12, P12 + nbu};
+    const int64_t nbk[3] = {nb, nbu, nbu};
+    fold_partials<3>(pk, nbk, o);
+    if (threadIdx.x != 0) return;
+    const double del = o[0], gam = o[1], res = sqrt(o[2]);
+    const double r0 = k == 0 ? res : in_res0;
+    ms->total_inner += 1;
+    if (k == 0) ms->in_res0 = res;
+    if (res / (r0 + 1e-50) < in_tol || !(res == res) || k >= in_max) {
+        ms->stop = 1;
+        return;
+    }
+    double alpha, beta;
+    if (k == 0) {
+        beta = 0.0;
+        alpha = gam / del;
+    } else {
+        beta = gam / gam_prev;
+        alpha = gam / (del - beta * gam / alp_prev);
+    }
+    ms->alpha = alpha;
+    ms->beta = beta;
+    ms->gam = gam;
+    ms->alp = alpha;
+    ms->it = k + 1;
+}
+
+// k_mx_update per problem
+__global__ __launch_bounds__(kBlock) void k_bm_update(BatchDev B, BMixDev M, int64_t total_t, float* __restrict__ p,
+                                                      float* __restrict__ s, float* __restrict__ d,
+                                                      float* __restrict__ u, const float* __restrict__ w,
+                                                      const float* __restrict__ invd, const BMState* st,
+                                                      double* part) {
+    int pid; int64_t lb, nbu;
+    locate_t(B, M, total_t, pid, lb, nbu);
+    const MixedState* ms = &st[pid].m;
+    if (ms->stop) return;
+    const int64_t n = B.nrows[pid], off = B.voff[pid];
+    const float a = (float)ms->alpha, bt = (float)ms->beta;
+    constexpr int T = kMxTiles;
+    int64_t idx[T];
+    float pv[T], sv[T], dv[T], uv[T], wv[T], mv[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int64_t rl = (lb * T + t) * kBlock + threadIdx.x;
+        const bool in = rl < n;
+        idx[t] = in ? off + rl : -1;
+        pv[t] = in ? p[off + rl] : 0.0f;
+        sv[t] = in ? s[off + rl] : 0.0f;
+        dv[t] = in ? d[off + rl] : 0.0f;
+        uv[t] = in ? u[off + rl] : 0.0f;
+        wv[t] = in ? w[off + rl] : 0.0f;
+        mv[t] = (in && invd) ? invd[off + rl] : 1.0f;
+    }
+    double v[2] = {0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        if (idx[t] < 0) continue;
+        const int64_t i = idx[t];
+        const float pi = __builtin_fmaf(bt, pv[t], uv[t]);
+        const float si = __builtin_fmaf(bt, sv[t], wv[t]);
+        p[i] = pi;
+        s[i] = si;
+        d[i] = __builtin_fmaf(a, pi, dv[t]);
+        float ui, ri;
+        if (invd) {
+            ui = __builtin_fmaf(-a, mv[t] * si, uv[t]);
+            ri = ui / mv[t];
+        } else {
+            ui = __builtin_fmaf(-a, si, uv[t]);
+            ri = ui;
+        }
+        u[i] = ui;
+        v[0] += (double)ri * (double)ui;
+        v[1] += (double)ri * (double)ri;
+    }
+    const int64_t nb = B.first_block[pid + 1] - B.first_block[pid];
+    block_partials<2>(v, lb, nbu, part + M.pbase[pid] + nb);
+}
+
+// x += res d for every problem whose inner solve ran this cycle
+__global__ void k_bm_update_x(BatchDev B, int64_t total_blocks, double* __restrict__ x, const float* __restrict__ d,
+                              const BMState* st) {
+    int pid; int64_t lb, nb;
+    locate(B, total_blocks, pid, lb, nb);
+    if (st[pid].done) return;
+    const int64_t rl = lb * kBlock + threadIdx.x;
+    const int64_t i = B.voff[pid] + rl;
+    if (rl < B.nrows[pid]) x[i] += st[pid].m.res * (double)d[i];
+}
+
 }  // namespace rsd
 
 using namespace rsd;
@@ -252,6 +508,27 @@ struct rsd_batch {
     DBuf<double> r, p, Ap, invd;
     DBuf<BState> st;
     DBuf<double> partials;
+    // batched mixed precision, set up on first use
+    std::vector<rsd_matrix*> mats;
+    int32_t mode32 = -1;  // FP32 column mode of every problem
+    int64_t total_t = 0;  // 1024-row tile blocks
+    DBuf<Cols> d_c32;
+    DBuf<const float*> d_val32;
+    DBuf<int32_t> t2prob[2], t2local[2];
+    DBuf<int64_t> first_t, pbase;
+    DBuf<float> mu, mw, mp, ms, md, minvd;
+    DBuf<double> r64, mpart;
+    DBuf<BMState> mst;
+    BMixDev mdev() const {
+        BMixDev M;
+        M.c32 = d_c32.p;
+        M.val32 = d_val32.p;
+        M.t2prob = t2prob[sched.order].p;
+        M.t2local = t2local[sched.order].p;
+        M.first_t = first_t.p;
+        M.pbase = pbase.p;
+        return M;
+    }
     BatchDev dev() const {
         BatchDev B;
         B.slice_ptr = d_slice.p;
@@ -369,6 +646,167 @@ void batched_cg(rsd_batch* bt, const double* b, double* x, double tol, int64_t m
     (void)hipHostFree(h);
 }
 
+// the batched mixed solve's tables and vectors; the FP32 copies are
+// refreshed by every solve
+static void bm_setup(rsd_batch* bt) {
+    rsd_ctx* ctx = bt->ctx;
+    hipStream_t st = ctx->stream;
+    const int np = bt->np;
+    if (bt->mode32 < 0) {
+        std::vector<Cols> c32(np);
+        std::vector<int64_t> first{0}, pbase(np);
+        int64_t pb_acc = 0;
+        for (int i = 0; i < np; ++i) {
+            rsd_matrix* A = bt->mats[i];
+            alloc_f32(ctx, A);  // builds the half layout when the pattern allows it
+            const Sell& S = A->pb->sell;
+            // symmetric half storage, else the FP32 copy in the sliced ELL
+            // layout walked by absolute columns (bitwise any column mode)
+            const int m = S.has_half ? COL_HALF : COL_ABS;
+            RSD_CHECK(i == 0 || m == bt->mode32, "batched mixed CG: the problems' FP32 layouts differ");
+            bt->mode32 = m;
+            c32[i] = S.cols32();
+            const int64_t nb = std::max<int64_t>(1, nblocks(S.n_rows)), nbu = mixed_blocks(S.n_rows);
+            first.push_back(first.back() + nbu);
+            pbase[i] = pb_acc;
+            pb_acc += nb + 2 * nbu;
+        }
+        bt->total_t = first.back();
+        std::vector<int32_t> t2p[2], t2l[2];
+        for (int i = 0; i < np; ++i)
+            for (int64_t b = first[i]; b < first[i + 1]; ++b) {
+                t2p[0].push_back(i);
+                t2l[0].push_back((int32_t)(b - first[i]));
+            }
+        for (int64_t k = 0; (int64_t)t2p[1].size() < bt->total_t; ++k)
+            for (int i = 0; i < np; ++i)
+                if (k < first[i + 1] - first[i]) {
+                    t2p[1].push_back(i);
+                    t2l[1].push_back((int32_t)k);
+                }
+        bt->d_c32.upload(c32.data(), np, st);
+        for (int o = 0; o < 2; ++o) {
+            bt->t2prob[o].upload(t2p[o].data(), bt->total_t, st);
+            bt->t2local[o].upload(t2l[o].data(), bt->total_t, st);
+        }
+        bt->first_t.upload(first.data(), np + 1, st);
+        bt->pbase.upload(pbase.data(), np, st);
+        const int64_t n = bt->total_rows;
+        for (DBuf<float>* v : {&bt->mu, &bt->mw, &bt->mp, &bt->ms, &bt->md, &bt->minvd}) {
+            v->alloc(n);
+            launch_fill_f32(ctx, n, v->p, 0.0f);
+        }
+        bt->r64.alloc(n);
+        bt->mpart.alloc(pb_acc);
+        bt->mst.alloc(np);
+    }
+    std::vector<const float*> v32(np);
+    for (int i = 0; i < np; ++i) {
+        matrix_to_f32(ctx, bt->mats[i], true);
+        v32[i] = bt->mats[i]->val32.p;
+    }
+    bt->d_val32.upload(v32.data(), np, st);
+    // the FP32 conversion's symmetry check, as the standalone solve reports it
+    for (int i = 0; i < np; ++i) {
+        const rsd_matrix* A = bt->mats[i];
+        if (!A->pb->sell.has_half || !A->asym.n) continue;
+        int32_t flag = 0;
+        RSD_HIP(hipMemcpyAsync(&flag, A->asym.p, sizeof flag, hipMemcpyDeviceToHost, st));
+        RSD_HIP(hipStreamSynchronize(st));
+        if (flag)
+            throw Error(RSD_ERR_ARG, "batched mixed CG: problem " + std::to_string(i) + "'s matrix is not symmetric "
+                                     "(a value differs from its mirror by more than 1e-10 of its row's largest entry)");
+    }
+}
+
+#define RSD_BM_SPMV32(bt, grid, ...)                                                                               \
+    do {                                                                                                           \
+        hipStream_t s_ = (bt)->ctx->stream;                                                                        \
+        if ((bt)->mode32 == COL_HALF) hipLaunchKernelGGL((k_bm_spmv_wu<COL_HALF>), grid, dim3(kBlock), 0, s_, __VA_ARGS__); \
+        else hipLaunchKernelGGL((k_bm_spmv_wu<COL_ABS>), grid, dim3(kBlock), 0, s_, __VA_ARGS__);                \
+    } while (0)
+
+static void bm_launch_inner_iter(rsd_batch* bt, int32_t precond) {
+    hipStream_t st = bt->ctx->stream;
+    const BatchDev B = bt->dev();
+    const BMixDev M = bt->mdev();
+    const float* invd = precond == RSD_PC_JACOBI ? bt->minvd.p : nullptr;
+    RSD_BM_SPMV32(bt, dim3((unsigned)bt->total_blocks), B, M, bt->total_blocks, bt->mu.p, bt->mw.p, bt->mst.p,
+                  bt->mpart.p);
+    hipLaunchKernelGGL(k_bm_scalars, dim3((unsigned)bt->np), dim3(kFold), 0, st, B, M, bt->mpart.p, bt->mst.p);
+    hipLaunchKernelGGL(k_bm_update, dim3((unsigned)bt->total_t), dim3(kBlock), 0, st, B, M, bt->total_t, bt->mp.p,
+                       bt->ms.p, bt->md.p, bt->mu.p, bt->mw.p, invd, bt->mst.p, bt->mpart.p);
+}
+
+void batched_mixed(rsd_batch* bt, const double* b, double* x, double tol, int64_t max_iters, int32_t precond,
+                   double inner_floor, std::vector<BMState>& out) {
+    hipStream_t st = bt->ctx->stream;
+    bm_setup(bt);
+    const BatchDev B = bt->dev();
+    const BMixDev M = bt->mdev();
+    const int np = bt->np;
+    const int64_t nb = bt->total_blocks;
+    if (precond == RSD_PC_JACOBI)
+        hipLaunchKernelGGL(k_bm_inv_diag, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, bt->minvd.p);
+    const float* invd = precond == RSD_PC_JACOBI ? bt->minvd.p : nullptr;
+    BMState* h = nullptr;
+    RSD_HIP(hipHostMalloc((void**)&h, sizeof(BMState) * (size_t)np, hipHostMallocDefault));
+    std::memset(h, 0, sizeof(BMState) * (size_t)np);
+    for (int i = 0; i < np; ++i) h[i].m.in_max = max_iters;
+    BGraph G;
+    try {
+        RSD_HIP(hipMemcpyAsync(bt->mst.p, h, sizeof(BMState) * (size_t)np, hipMemcpyHostToDevice, st));
+        // one chunk of inner iterations for the whole batch, as a graph
+        RSD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int64_t k = 0; k < bt->sched.chunk; ++k) bm_launch_inner_iter(bt, precond);
+        RSD_HIP(hipStreamEndCapture(st, &G.g));
+        RSD_HIP(hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0));
+        auto read_states = [&] {
+            RSD_HIP(hipMemcpyAsync(h, bt->mst.p, sizeof(BMState) * (size_t)np, hipMemcpyDeviceToHost, st));
+            RSD_HIP(hipStreamSynchronize(st));
+        };
+        for (int cycle = 0;; ++cycle) {
+            RSD_CHECK(cycle <= kMixedMaxCycles + 1, "batched mixed CG: outer loop did not end");
+            if (bt->mode == COL_PAT) {
+                if (bt->sched.unroll == 4)
+                    hipLaunchKernelGGL((k_bm_residual<COL_PAT, 4>), dim3((unsigned)nb), dim3(kBlock), 0, st, B, M, nb,
+                                       b, x, bt->r64.p, bt->mst.p, bt->mpart.p);
+                else
+                    hipLaunchKernelGGL((k_bm_residual<COL_PAT, 9>), dim3((unsigned)nb), dim3(kBlock), 0, st, B, M, nb,
+                                       b, x, bt->r64.p, bt->mst.p, bt->mpart.p);
+            } else {
+                hipLaunchKernelGGL((k_bm_residual<COL_ABS, 9>), dim3((unsigned)nb), dim3(kBlock), 0, st, B, M, nb, b,
+                                   x, bt->r64.p, bt->mst.p, bt->mpart.p);
+            }
+            hipLaunchKernelGGL(k_bm_outer, dim3((unsigned)np), dim3(kFold), 0, st, B, M, bt->mpart.p, bt->mst.p, tol,
+                               max_iters, inner_floor);
+            read_states();
+            bool all_done = true;
+            for (int i = 0; i < np; ++i) all_done = all_done && h[i].done;
+            if (all_done) break;
+            hipLaunchKernelGGL(k_bm_init, dim3((unsigned)bt->total_t), dim3(kBlock), 0, st, B, M, bt->total_t,
+                               bt->r64.p, bt->mu.p, bt->mp.p, bt->ms.p, bt->md.p, invd, bt->mst.p, bt->mpart.p);
+            int64_t launched = 0;
+            for (;;) {
+                RSD_HIP(hipGraphLaunch(G.e, st));
+                launched += bt->sched.chunk;
+                read_states();
+                bool stopped = true;
+                for (int i = 0; i < np; ++i) stopped = stopped && h[i].m.stop;
+                if (stopped) break;
+                RSD_CHECK(launched <= max_iters + 2 * bt->sched.chunk, "batched mixed CG: inner solve did not stop");
+            }
+            hipLaunchKernelGGL(k_bm_update_x, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, x, bt->md.p, bt->mst.p);
+            RSD_HIP(hipGetLastError());
+        }
+        out.assign(h, h + np);
+    } catch (...) {
+        (void)hipHostFree(h);
+        throw;
+    }
+    (void)hipHostFree(h);
+}
+
 // ------------------------------------------------------------------------
 // autotuner
 // ------------------------------------------------------------------------
@@ -469,6 +907,7 @@ int rsd_batch_create(rsd_ctx* ctx, int32_t n_problems, rsd_matrix* const* mats, 
         auto bt = std::make_unique<rsd_batch>();
         bt->ctx = ctx;
         bt->np = n_problems;
+        bt->mats.assign(mats, mats + n_problems);
         std::vector<const int64_t*> sl(n_problems);
         std::vector<const int32_t*> rp(n_problems), col(n_problems), dg(n_problems), poff(n_problems), plen(n_problems);
         std::vector<const uint8_t*> pid(n_problems);
@@ -578,6 +1017,28 @@ int rsd_cg_solve_batched(rsd_ctx* ctx, rsd_batch* bt, const double* b, double* x
                 c.stop = s[i].stop;
                 cg_report(c, max_iters, &reps[i]);
             }
+    });
+}
+
+int rsd_cg_solve_batched_mixed(rsd_ctx* ctx, rsd_batch* bt, const double* b, double* x, double tol,
+                               int64_t max_iters, int32_t precond, double inner_floor, rsd_report* reps,
+                               int64_t* outer_cycles) {
+    return guard([&] {
+        RSD_CHECK(ctx && bt && b && x && ctx == bt->ctx, "rsd_cg_solve_batched_mixed: bad argument");
+        RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
+        std::vector<BMState> s;
+        batched_mixed(bt, b, x, tol, max_iters, precond, inner_floor > 0 ? inner_floor : kMixedDefaultInnerFloor, s);
+        for (int i = 0; i < bt->np; ++i) {
+            const MixedState& m = s[i].m;
+            if (reps) {
+                reps[i].converged = m.converged;
+                reps[i].iterations = m.total_inner;
+                reps[i].initial_residual = m.res0;
+                reps[i].final_residual = m.res;
+                reps[i].final_residual_rel = m.res / (m.res0 + 1e-50);
+            }
+            if (outer_cycles) outer_cycles[i] = m.cycles;
+        }
     });
 }
 

@@ -696,6 +696,16 @@ class Batch:
                                          C.c_int32(precond), C.c_int32(int(use_graph)), reps))
         return list(reps)
 
+    def cg_mixed(self, b, x, tol=1e-8, max_iters=1000, precond=PC_JACOBI, inner_floor=0.0):
+        """Every problem's mixed-precision solve (FP32 inner CG, FP64 outer
+        residual) in one set of launches; returns (reports, outer cycles)."""
+        reps = (Report * len(self.mats))()
+        cyc = np.zeros(len(self.mats), np.int64)
+        check(lib().rsd_cg_solve_batched_mixed(self.ctx.h, self.h, dptr(b), dptr(x), C.c_double(tol),
+                                               C.c_int64(max_iters), C.c_int32(precond), C.c_double(inner_floor),
+                                               reps, cyc.ctypes.data_as(_i64)))
+        return list(reps), [int(c) for c in cyc]
+
 
 def philox_normal(seed, stream, n):
     out = np.zeros(n)

@@ -826,6 +826,65 @@ def _batch_problems(ctx, sides, with_permuted=False):
     return pbs, mats, bs
 
 
+def test_batched_mixed_bitwise(ctx):
+    """The batched mixed-precision solve: each problem's solution, inner
+    iteration count, outer cycle count and residuals are bitwise its
+    standalone mixed solve (FP32 half storage, uniform stencil kernel) under
+    three schedules, and agree with its FP64 solve to the tolerance."""
+    import residuum_hip as rh
+    pbs, mats, bs = _batch_problems(ctx, [4, 7, 3, 6, 5])
+    batch = rh.Batch(ctx, mats)
+    db = ctx.from_numpy(batch.pack(bs))
+    ref = []
+    for pb, K, b in zip(pbs, mats, bs):
+        x1 = pb.vector(0.0)
+        r1, c1 = K.cg_mixed(ctx.from_numpy(b), x1, tol=1e-8, max_iters=2000)
+        x2 = pb.vector(0.0)
+        K.cg(ctx.from_numpy(b), x2, tol=1e-12, max_iters=4000)
+        ref.append((r1, c1, x1.numpy(), x2.numpy()))
+    for sched in ((8, 9, 0, 1), (4, 4, 1, 0), (32, 9, 1, 1)):
+        batch.schedule = sched
+        dx = ctx.from_numpy(np.zeros(batch.total_rows))
+        reps, cyc = batch.cg_mixed(db, dx, tol=1e-8, max_iters=2000)
+        xs = batch.unpack(dx.numpy())
+        say("schedule", sched, "iterations", [r.iterations for r in reps], "cycles", cyc)
+        for i, (r1, c1, x1, x64) in enumerate(ref):
+            assert reps[i].converged == r1.converged == 1
+            assert reps[i].iterations == r1.iterations and cyc[i] == c1 and c1 >= 2
+            assert reps[i].final_residual == r1.final_residual and reps[i].initial_residual == r1.initial_residual
+            assert np.array_equal(bits(xs[i]), bits(x1))
+            assert np.max(np.abs(xs[i] - x64)) <= 1e-6 * np.max(np.abs(x64))
+
+
+def test_batched_mixed_explicit_columns(ctx):
+    """Renumbered problems (no shared offset patterns, so the FP32 copy
+    keeps the sliced ELL layout) batch among themselves bitwise their
+    standalone mixed solves; a batch mixing them with half-storage problems
+    is refused."""
+    import residuum_hip as rh
+    pbs = [permuted_problem(ctx, 9), permuted_problem(ctx, 4)]
+    mats, bs = [], []
+    for i, pb in enumerate(pbs):
+        K = pb.matrix()
+        pb.assemble_matrix(K, [(O.DIFFUSION, 1.0)])
+        mats.append(K)
+        bs.append(np.random.default_rng(40 + i).standard_normal(pb.n_free))
+    batch = rh.Batch(ctx, mats)
+    dx = ctx.from_numpy(np.zeros(batch.total_rows))
+    reps, cyc = batch.cg_mixed(ctx.from_numpy(batch.pack(bs)), dx, tol=1e-8, max_iters=2000)
+    xs = batch.unpack(dx.numpy())
+    for i, (pb, K, b) in enumerate(zip(pbs, mats, bs)):
+        x1 = pb.vector(0.0)
+        r1, c1 = K.cg_mixed(ctx.from_numpy(b), x1, tol=1e-8, max_iters=2000)
+        assert reps[i].iterations == r1.iterations and cyc[i] == c1 and reps[i].converged == 1
+        assert np.array_equal(bits(xs[i]), bits(x1.numpy()))
+    _, mats2, _ = _batch_problems(ctx, [4])
+    mixed = rh.Batch(ctx, mats2 + mats[:1])
+    dz = ctx.from_numpy(np.zeros(mixed.total_rows))
+    with pytest.raises(rh.RsdError, match="FP32 layouts differ"):
+        mixed.cg_mixed(ctx.from_numpy(np.ones(mixed.total_rows)), dz)
+
+
 @pytest.mark.parametrize("with_permuted", [False, True])
 def test_batched_schedules_bitwise(ctx, with_permuted):
     """Every schedule (chunk, load batch, block order, XCD remap) gives the
