@@ -93,7 +93,9 @@ def test_batched_256_problems(ctx):
     elements (Philox seed 20261202), kappa_e lognormal(0, 1), nodal source
     U[0, 1], T = 0 on all faces; CG + Jacobi to 1e-8 from x = 0.  One batched
     solve (one set of launches for all problems) against 256 standalone
-    FP64 solves: every problem bitwise equal, iteration counts equal."""
+    FP64 solves: every problem bitwise equal, iteration counts equal.  Then
+    the batched mixed-precision solve: every problem within 1e-6 of its FP64
+    solution, and every 16th bitwise its standalone mixed solve."""
     import residuum_hip as rh
     u = rh.philox_uniform(20261202, 0, 256)
     sides = (16 + np.floor(u * 49)).astype(int)
@@ -142,6 +144,26 @@ def test_batched_256_problems(ctx):
     say(f"batched: {tb*1e3:.0f} ms for {iters_b} problem-iterations "
         f"({sum(pb.n_free * r.iterations for pb, r in zip(pbs, reps)) / tb:.3e} DOF-iters/s); "
         f"256 standalone solves: {ts*1e3:.0f} ms; speed-up {ts / tb:.2f}x")
+    # FP32 bulk with FP64 refinement: the batched mixed solve, every problem
+    # bitwise its standalone mixed solve, within 1e-6 of the FP64 solution
+    dm = ctx.from_numpy(np.zeros(batch.total_rows))
+    tm = time.perf_counter()
+    mreps, mcyc = batch.cg_mixed(db, dm, tol=1e-8, max_iters=5000)
+    tm = time.perf_counter() - tm
+    xm = batch.unpack(dm.numpy())
+    for i, (pb, K, b) in enumerate(zip(pbs, mats, bs)):
+        assert mreps[i].converged == 1
+        assert np.max(np.abs(xm[i] - xs[i])) <= 1e-6 * np.max(np.abs(xs[i])), i
+        if i % 16 == 0:
+            x1 = pb.vector(0.0)
+            d_b = ctx.from_numpy(b)
+            r1, c1 = K.cg_mixed(d_b, x1, tol=1e-8, max_iters=5000)
+            assert mreps[i].iterations == r1.iterations and mcyc[i] == c1, i
+            assert np.array_equal(xm[i], x1.numpy()), i
+            x1.free()
+            d_b.free()
+    say(f"batched mixed: {tm*1e3:.0f} ms (first call, FP32 copies included) for "
+        f"{sum(r.iterations for r in mreps)} inner iterations, outer cycles {min(mcyc)}..{max(mcyc)}")
 
 
 @pytest.mark.gpu
