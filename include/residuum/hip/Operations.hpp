@@ -3,7 +3,7 @@
 // The reference declares its BLAS-1 and matrix operations as free function
 // templates (linalg/operations/VectorOps.hpp:11-37, MatrixOps.hpp:11-15)
 // whose bodies come from backend/cpu/*.tpp.  These non-template overloads
-// for Vector<double, Hip> and CSRMatrix<double, Hip> are preferred over
+// for Vector<double, Hip>, Vector<float, Hip> and CSRMatrix<double, Hip> are preferred over
 // those templates by overload resolution, and run on the device:
 //   dot / norm      fixed-tree reductions (bitwise reproducible; the sum
 //                   order differs from the CPU loop, so results agree with
@@ -26,6 +26,7 @@
 namespace residuum {
 namespace hip {
 using HipVector = linalg::types::Vector<double, linalg::types::backend::Hip>;
+using HipVectorF = linalg::types::Vector<float, linalg::types::backend::Hip>;
 using HipCSRMatrix = linalg::types::CSRMatrix<double, linalg::types::backend::Hip>;
 static_assert(sizeof(Index) == sizeof(uint64_t), "the device CSR uses 64-bit indices like the reference's size_t");
 }  // namespace hip
@@ -75,6 +76,46 @@ inline double norm(const hip::HipVector& x) {
     double r = 0.0;
     hip::check(rsd_norm2_f64(hip::Context::get(), (int64_t)x.size(), x.data(), nullptr, &r));
     return r;
+}
+
+// The same operations on float vectors: elementwise bitwise the CPU loop;
+// dot and norm sum exact FP64 products (the CPU loop sums in float) and
+// return the result rounded to float.
+inline float dot(const hip::HipVectorF& a, const hip::HipVectorF& b) {
+    assert(a.size() == b.size());
+    double r = 0.0;
+    hip::check(rsd_dot_f32(hip::Context::get(), (int64_t)a.size(), a.data(), b.data(), nullptr, &r));
+    return (float)r;
+}
+
+inline void axpy(const float alpha, const hip::HipVectorF& x, hip::HipVectorF& y) {
+    assert(x.size() == y.size());
+    hip::check(rsd_axpy_f32(hip::Context::get(), (int64_t)x.size(), alpha, x.data(), y.data()));
+}
+
+inline void axpby(const float alpha, const float beta, const hip::HipVectorF& x, hip::HipVectorF& y) {
+    assert(x.size() == y.size());
+    hip::check(rsd_axpby_f32(hip::Context::get(), (int64_t)x.size(), alpha, beta, x.data(), y.data()));
+}
+
+inline void scal(const float alpha, hip::HipVectorF& x) {
+    hip::check(rsd_scal_f32(hip::Context::get(), (int64_t)x.size(), alpha, x.data()));
+}
+
+inline void copy(const hip::HipVectorF& x, hip::HipVectorF& y) {
+    assert(x.size() == y.size());
+    hip::check(rsd_copy_f32(hip::Context::get(), (int64_t)x.size(), x.data(), y.data()));
+}
+
+inline void multiply(const hip::HipVectorF& x, const hip::HipVectorF& y, hip::HipVectorF& z) {
+    assert(x.size() == y.size() && y.size() == z.size());
+    hip::check(rsd_mul_f32(hip::Context::get(), (int64_t)x.size(), x.data(), y.data(), z.data()));
+}
+
+inline float norm(const hip::HipVectorF& x) {
+    double r = 0.0;
+    hip::check(rsd_norm2_f32(hip::Context::get(), (int64_t)x.size(), x.data(), nullptr, &r));
+    return (float)r;
 }
 
 // MatrixOps.tpp:5-23

@@ -105,6 +105,18 @@ int rsd_axpby_f64(rsd_ctx* ctx, int64_t n, double alpha, double beta, const doub
 int rsd_scal_f64(rsd_ctx* ctx, int64_t n, double alpha, double* x);
 int rsd_copy_f64(rsd_ctx* ctx, int64_t n, const double* x, double* y);
 int rsd_mul_f64(rsd_ctx* ctx, int64_t n, const double* x, const double* y, double* z);        /* z = x .* y */
+/* The same operations on FP32 vectors (Vector<float>, VectorOps.tpp:7-86):
+ * the elementwise ones in float, each product and sum rounded once as in the
+ * CPU loop (bitwise it); dot and norm2 form each product exactly in FP64 and
+ * sum with the FP64 tree above, writing a double (the CPU loop accumulates
+ * in float, so the two agree to float rounding). */
+int rsd_dot_f32(rsd_ctx* ctx, int64_t n, const float* x, const float* y, double* d_out, double* h_out);
+int rsd_norm2_f32(rsd_ctx* ctx, int64_t n, const float* x, double* d_out, double* h_out);
+int rsd_axpy_f32(rsd_ctx* ctx, int64_t n, float alpha, const float* x, float* y);
+int rsd_axpby_f32(rsd_ctx* ctx, int64_t n, float alpha, float beta, const float* x, float* y);
+int rsd_scal_f32(rsd_ctx* ctx, int64_t n, float alpha, float* x);
+int rsd_copy_f32(rsd_ctx* ctx, int64_t n, const float* x, float* y);
+int rsd_mul_f32(rsd_ctx* ctx, int64_t n, const float* x, const float* y, float* z);
 
 /* ---------------------------------------------------------------------- */
 /* problem: mesh + topological DOFs + CSR pattern + element->nnz map +     */

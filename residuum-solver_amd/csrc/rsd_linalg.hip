@@ -46,6 +46,42 @@ __global__ void k_mul(int64_t n, const double* __restrict__ x, const double* __r
     if (i < n) z[i] = x[i] * y[i];
 }
 
+// FP32 vectors (Vector<float>): the elementwise operations in float, each
+// product and sum rounded once as in the CPU loop
+__global__ void k_axpy_f32(int64_t n, float a, const float* __restrict__ x, float* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+
+__global__ void k_axpby_f32(int64_t n, float a, float b, const float* __restrict__ x, float* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] = b * y[i] + a * x[i];
+}
+
+__global__ void k_scal_f32(int64_t n, float a, float* __restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] *= a;
+}
+
+__global__ void k_copy_f32(int64_t n, const float* __restrict__ x, float* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] = x[i];
+}
+
+__global__ void k_mul_f32(int64_t n, const float* __restrict__ x, const float* __restrict__ y, float* __restrict__ z) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) z[i] = x[i] * y[i];
+}
+
+// FP32 dot: each product exact in FP64, summed with the FP64 tree
+__global__ __launch_bounds__(kBlock) void k_dot_f32(int64_t n, int64_t nb, const float* __restrict__ x,
+                                                    const float* __restrict__ y, double* partials) {
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    double v[1] = {i < n ? (double)x[i] * (double)y[i] : 0.0};
+    block_partials<1>(v, lb, nb, partials);
+}
+
 __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, int64_t nb, const double* __restrict__ x,
                                                 const double* __restrict__ y, double* partials) {
     const int64_t lb = xcd_block(blockIdx.x, nb);
@@ -436,6 +472,60 @@ int rsd_mul_f64(rsd_ctx* ctx, int64_t n, const double* x, const double* y, doubl
         if (n) hipLaunchKernelGGL(k_mul, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, ctx->stream, n, x, y, z);
         RSD_HIP(hipGetLastError());
     });
+}
+
+static void dot_f32(rsd_ctx* ctx, int64_t n, const float* x, const float* y, double* d_out, bool take_sqrt) {
+    const int64_t nb = std::max<int64_t>(1, nblocks(n));
+    double* part = ctx->partials(nb);
+    hipLaunchKernelGGL(k_dot_f32, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, x, y, part);
+    hipLaunchKernelGGL(k_dot_fold, dim3(1), dim3(kFold), 0, ctx->stream, nb, part, d_out, (int32_t)take_sqrt);
+}
+
+int rsd_dot_f32(rsd_ctx* ctx, int64_t n, const float* x, const float* y, double* d_out, double* h_out) {
+    return guard([&] {
+        RSD_CHECK(ctx && n >= 0 && (n == 0 || (x && y)), "rsd_dot_f32: bad argument");
+        double* o = d_out ? d_out : ctx->d_scalars;
+        dot_f32(ctx, n, x, y, o, false);
+        RSD_HIP(hipGetLastError());
+        finish_scalar(ctx, o, h_out);
+    });
+}
+
+int rsd_norm2_f32(rsd_ctx* ctx, int64_t n, const float* x, double* d_out, double* h_out) {
+    return guard([&] {
+        RSD_CHECK(ctx && n >= 0 && (n == 0 || x), "rsd_norm2_f32: bad argument");
+        double* o = d_out ? d_out : ctx->d_scalars;
+        dot_f32(ctx, n, x, x, o, true);
+        RSD_HIP(hipGetLastError());
+        finish_scalar(ctx, o, h_out);
+    });
+}
+
+#define RSD_F32_LAUNCH(kern, n, ...)                                                                              \
+    do {                                                                                                          \
+        if (n) hipLaunchKernelGGL(kern, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, ctx->stream, n, __VA_ARGS__); \
+        RSD_HIP(hipGetLastError());                                                                               \
+    } while (0)
+
+int rsd_axpy_f32(rsd_ctx* ctx, int64_t n, float a, const float* x, float* y) {
+    return guard([&] { RSD_CHECK(ctx && n >= 0 && (n == 0 || (x && y)), "rsd_axpy_f32: bad argument");
+                       RSD_F32_LAUNCH(k_axpy_f32, n, a, x, y); });
+}
+int rsd_axpby_f32(rsd_ctx* ctx, int64_t n, float a, float b, const float* x, float* y) {
+    return guard([&] { RSD_CHECK(ctx && n >= 0 && (n == 0 || (x && y)), "rsd_axpby_f32: bad argument");
+                       RSD_F32_LAUNCH(k_axpby_f32, n, a, b, x, y); });
+}
+int rsd_scal_f32(rsd_ctx* ctx, int64_t n, float a, float* x) {
+    return guard([&] { RSD_CHECK(ctx && n >= 0 && (n == 0 || x), "rsd_scal_f32: bad argument");
+                       RSD_F32_LAUNCH(k_scal_f32, n, a, x); });
+}
+int rsd_copy_f32(rsd_ctx* ctx, int64_t n, const float* x, float* y) {
+    return guard([&] { RSD_CHECK(ctx && n >= 0 && (n == 0 || (x && y)), "rsd_copy_f32: bad argument");
+                       RSD_F32_LAUNCH(k_copy_f32, n, x, y); });
+}
+int rsd_mul_f32(rsd_ctx* ctx, int64_t n, const float* x, const float* y, float* z) {
+    return guard([&] { RSD_CHECK(ctx && n >= 0 && (n == 0 || (x && y && z)), "rsd_mul_f32: bad argument");
+                       RSD_F32_LAUNCH(k_mul_f32, n, x, y, z); });
 }
 
 int rsd_spmv_f64(rsd_ctx* ctx, rsd_matrix* A, const double* x, double* y) {

@@ -137,31 +137,47 @@ class Ctx:
             check(lib().rsd_memset_zero(self.h, _vp(d.ptr), C.c_size_t(d.nbytes)))
         return d
 
-    # -- BLAS-1 -------------------------------------------------------------
+    # -- BLAS-1 (FP64 or FP32 device arrays; FP32 dot / norm2 sum in FP64) --
+    @staticmethod
+    def _kind(*arrs):
+        def f32(a):
+            return str(a.dtype) in ("float32", "torch.float32")
+
+        if any(f32(a) != f32(arrs[0]) for a in arrs):
+            raise TypeError("BLAS-1: mixed FP32 / FP64 operands")
+        return ("f32", C.c_float) if f32(arrs[0]) else ("f64", C.c_double)
+
     def dot(self, x, y):
+        k, _ = self._kind(x, y)
         out = C.c_double()
-        check(lib().rsd_dot_f64(self.h, C.c_int64(x.n), dptr(x), dptr(y), None, C.byref(out)))
+        check(getattr(lib(), "rsd_dot_" + k)(self.h, C.c_int64(x.n), dptr(x), dptr(y), None, C.byref(out)))
         return out.value
 
     def norm2(self, x):
+        k, _ = self._kind(x)
         out = C.c_double()
-        check(lib().rsd_norm2_f64(self.h, C.c_int64(x.n), dptr(x), None, C.byref(out)))
+        check(getattr(lib(), "rsd_norm2_" + k)(self.h, C.c_int64(x.n), dptr(x), None, C.byref(out)))
         return out.value
 
     def axpy(self, a, x, y):
-        check(lib().rsd_axpy_f64(self.h, C.c_int64(x.n), C.c_double(a), dptr(x), dptr(y)))
+        k, sc = self._kind(x, y)
+        check(getattr(lib(), "rsd_axpy_" + k)(self.h, C.c_int64(x.n), sc(a), dptr(x), dptr(y)))
 
     def axpby(self, a, b, x, y):
-        check(lib().rsd_axpby_f64(self.h, C.c_int64(x.n), C.c_double(a), C.c_double(b), dptr(x), dptr(y)))
+        k, sc = self._kind(x, y)
+        check(getattr(lib(), "rsd_axpby_" + k)(self.h, C.c_int64(x.n), sc(a), sc(b), dptr(x), dptr(y)))
 
     def scal(self, a, x):
-        check(lib().rsd_scal_f64(self.h, C.c_int64(x.n), C.c_double(a), dptr(x)))
+        k, sc = self._kind(x)
+        check(getattr(lib(), "rsd_scal_" + k)(self.h, C.c_int64(x.n), sc(a), dptr(x)))
 
     def copy(self, x, y):
-        check(lib().rsd_copy_f64(self.h, C.c_int64(x.n), dptr(x), dptr(y)))
+        k, _ = self._kind(x, y)
+        check(getattr(lib(), "rsd_copy_" + k)(self.h, C.c_int64(x.n), dptr(x), dptr(y)))
 
     def mul(self, x, y, z):
-        check(lib().rsd_mul_f64(self.h, C.c_int64(x.n), dptr(x), dptr(y), dptr(z)))
+        k, _ = self._kind(x, y, z)
+        check(getattr(lib(), "rsd_mul_" + k)(self.h, C.c_int64(x.n), dptr(x), dptr(y), dptr(z)))
 
     def fill(self, x, v):
         if x.dtype == np.float32:

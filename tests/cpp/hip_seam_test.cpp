@@ -162,6 +162,49 @@ int main() {
         ops::copy(gz, gy);
         report("copy_bitwise", down(gy) == host(cy));
     }
+    // BLAS-1 on float vectors: the reference's float templates on the CPU
+    // tag against the float overloads on the Hip tag
+    {
+        using VecCf = linalg::types::Vector<float, CPU>;
+        using VecHf = linalg::types::Vector<float, Hip>;
+        VecCf fx(n), fy(n), fz(n);
+        VecHf hxf(n), hyf(n), hzf(n);
+        std::vector<float> tx(n), ty(n);
+        for (Index i = 0; i < n; ++i) {
+            tx[i] = (float)(0.5 + 0.25 * std::sin(0.37 * i));
+            ty[i] = (float)(0.75 + 0.25 * std::cos(0.11 * i));
+        }
+        std::copy(tx.begin(), tx.end(), fx.data());
+        std::copy(ty.begin(), ty.end(), fy.data());
+        Hip::copy<float>(hxf.data(), tx.data(), n, linalg::types::backend::CopyKind::HostToDevice);
+        Hip::copy<float>(hyf.data(), ty.data(), n, linalg::types::backend::CopyKind::HostToDevice);
+        auto downf = [&](const VecHf& v) {
+            std::vector<float> h(v.size());
+            Hip::copy<float>(h.data(), v.data(), v.size(), linalg::types::backend::CopyKind::DeviceToHost);
+            return h;
+        };
+        auto hostf = [](const VecCf& v) { return std::vector<float>(v.data(), v.data() + v.size()); };
+        // positive terms: the float loop's rounding error stays near n eps
+        const float dc = ops::dot(fx, fy), dg = ops::dot(hxf, hyf);
+        report("dot_f32", std::fabs(dc - dg) <= 2e-3f * std::fabs(dc), std::to_string(dc) + " vs " + std::to_string(dg));
+        const float nc = ops::norm(fx), ng = ops::norm(hxf);
+        report("norm_f32", std::fabs(nc - ng) <= 1e-3f * nc);
+        ops::axpy(0.3f, fx, fy);
+        ops::axpy(0.3f, hxf, hyf);
+        report("axpy_f32_bitwise", downf(hyf) == hostf(fy));
+        ops::axpby(0.7f, -1.3f, fx, fy);
+        ops::axpby(0.7f, -1.3f, hxf, hyf);
+        report("axpby_f32_bitwise", downf(hyf) == hostf(fy));
+        ops::scal(1.7f, fy);
+        ops::scal(1.7f, hyf);
+        report("scal_f32_bitwise", downf(hyf) == hostf(fy));
+        ops::multiply(fx, fy, fz);
+        ops::multiply(hxf, hyf, hzf);
+        report("multiply_f32_bitwise", downf(hzf) == hostf(fz));
+        ops::copy(fz, fy);
+        ops::copy(hzf, hyf);
+        report("copy_f32_bitwise", downf(hyf) == hostf(fy));
+    }
     MatC Ac(n, n);
     MatH Ag(n, n);
     load(Ac, hA);

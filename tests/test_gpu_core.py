@@ -107,6 +107,37 @@ def test_blas1(ctx):
     assert np.all(dx.numpy() == 4.25)
 
 
+def test_blas1_f32(ctx):
+    """FP32 vectors: the elementwise operations bitwise numpy's float32
+    arithmetic (each product and sum rounded once); dot and norm2 sum the
+    exact products in FP64 (a fixed tree, so to 1e-12 of numpy's FP64 dot)."""
+    rng = np.random.default_rng(4)
+    n = 3001
+    x = rng.standard_normal(n).astype(np.float32)
+    y = rng.standard_normal(n).astype(np.float32)
+    dx, dy, dz = ctx.from_numpy(x), ctx.from_numpy(y), ctx.empty(n, np.float32)
+    f = np.float32
+    d64 = np.dot(x.astype(np.float64), y.astype(np.float64))
+    assert abs(ctx.dot(dx, dy) - d64) <= 1e-12 * np.abs(x.astype(np.float64) * y).sum()
+    n64 = np.sqrt(np.dot(x.astype(np.float64), x.astype(np.float64)))
+    assert abs(ctx.norm2(dx) - n64) <= 1e-13 * n64
+    ctx.axpy(0.3, dx, dy)
+    y1 = y + f(0.3) * x
+    assert same(dy.numpy(), y1) and dy.numpy().dtype == np.float32
+    ctx.axpby(0.7, -1.5, dx, dy)
+    y2 = f(-1.5) * y1 + f(0.7) * x
+    assert same(dy.numpy(), y2)
+    ctx.scal(2.5, dx)
+    x2 = x * f(2.5)
+    assert same(dx.numpy(), x2)
+    ctx.mul(dx, dy, dz)
+    assert same(dz.numpy(), x2 * y2)
+    ctx.copy(dz, dx)
+    assert same(dx.numpy(), x2 * y2)
+    with pytest.raises(TypeError):
+        ctx.axpy(1.0, dx, ctx.from_numpy(np.zeros(n)))
+
+
 FORMSETS = {
     "diffusion": [(O.DIFFUSION, 1.0)],
     "mass": [(O.MASS, 1.0)],

@@ -21,4 +21,5 @@ def test_reference_templates_on_hip_tag():
     print(r.stderr)
     fails = [ln for ln in r.stdout.splitlines() if ln.startswith("FAIL")]
     assert r.returncode == 0 and not fails, "\n".join(fails) or r.stderr
-    assert sum(ln.startswith("PASS") for ln in r.stdout.splitlines()) >= 25
+    passed = [ln for ln in r.stdout.splitlines() if ln.startswith("PASS")]
+    assert len(passed) >= 34 and sum("_f32" in ln for ln in passed) == 7
