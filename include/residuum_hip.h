@@ -275,6 +275,19 @@ int rsd_cg_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
  * restart; rep->iterations counts Arnoldi steps. */
 int rsd_gmres_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int32_t tol_type,
                         int64_t max_iters, int32_t krylov, int32_t precond, rsd_report* rep);
+/* Mixed-precision GMRES (no reference counterpart; the reference is FP64
+ * only): FP64 iterative refinement -- r = b - A x with the FP64 values,
+ * stopping on ||r|| / (||r_0|| + 1e-50) < tol, x += ||r|| d -- around an
+ * inner right-preconditioned GMRES(krylov) on A d = r / ||r|| with a full
+ * FP32 copy of A's values (any matrix, symmetric or not), FP32 basis and
+ * products, Gram-Schmidt and norm dots summed in FP64, Hessenberg and Givens
+ * in FP64.  The inner relative tolerance is max(inner_floor, 0.5 tol
+ * ||r_0|| / ||r||), capped at 0.5 (inner_floor <= 0: 1e-4).
+ * rep->iterations counts inner Arnoldi steps; outer_cycles (nullable) the
+ * refinements. */
+int rsd_gmres_solve_mixed(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
+                          int32_t krylov, int32_t precond, double inner_floor, rsd_report* rep,
+                          int64_t* outer_cycles);
 
 /* ---------------------------------------------------------------------- */
 /* raw device CSR in the reference's layout (linalg/types/CSRMatrix.hpp:14-88:
@@ -419,7 +432,9 @@ int rsd_be_run(rsd_ctx* ctx, rsd_problem* pb, double t0, double tf, double dt,
  * 60-116): per iteration assemble the load, R = F - K(U) U with Dirichlet
  * values gathered in, ||R||, stop if ||R|| < abs_tol or ||R||/||R_0|| <
  * rel_tol, else reassemble J = K + K_T in place, dU = 0, solve J dU = R
- * (lin_solver 0 CG, 1 GMRES(krylov)), U += dU.  newton_iters receives the
+ * (lin_solver 0 CG, 1 GMRES(krylov), 2 rsd_gmres_solve_mixed's
+ * mixed-precision GMRES(krylov), 3 rsd_cg_solve_mixed's mixed-precision CG
+ * on a full FP32 copy of J, which need not be symmetric), U += dU.  newton_iters receives the
  * iteration at which it converged (-1 not converged); res_hist (host,
  * nullable, >= max_newton) receives ||R||, lin_iters (host, nullable) the
  * inner iterations per Newton step. */

@@ -301,10 +301,22 @@ int64_t newton_run(rsd_problem* pb, const char* src_text, double abs_tol, double
             const CGState s = cg_run(ctx, &J, R, dU, lin_tol, lin_max, precond, w);
             ok = s.converged;
             li = s.converged ? s.iter : lin_max;
-        } else {
+        } else if (lin_solver == 1) {
             const GMRESResult g = gmres_run(ctx, &J, R, dU, lin_tol, lin_max, krylov, precond);
             ok = g.converged;
             li = g.iterations;
+        } else if (lin_solver == 2) {  // FP32 inner GMRES with FP64 refinement
+            const MixedGMResult g = gmres_mixed_run(ctx, &J, R, dU, lin_tol, lin_max, krylov, precond,
+                                                    kMixedDefaultInnerFloor);
+            ok = g.converged;
+            li = g.iterations;
+        } else {  // FP32 inner CG on the full FP32 copy (J need not be symmetric), FP64 refinement
+            MixedWork mw = mixed_workspace(ctx, n, nullptr);
+            mw.full = true;
+            const MixedState s = mixed_run(ctx, &J, R, dU, lin_tol, lin_max, precond, kMixedDefaultInnerFloor, false,
+                                           mw, nullptr);
+            ok = s.converged;
+            li = s.total_inner;
         }
         if (lin_iters) lin_iters[it] = li;
         if (!ok) return -2;
@@ -344,7 +356,8 @@ int rsd_newton_run(rsd_ctx* ctx, rsd_problem* pb, const char* src_expr, double a
     const int rc = guard([&] {
         RSD_CHECK(ctx && pb && d_U, "rsd_newton_run: null argument");
         RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
-        RSD_CHECK(lin_solver == 0 || lin_solver == 1, "lin_solver must be 0 (CG) or 1 (GMRES)");
+        RSD_CHECK(lin_solver >= 0 && lin_solver <= 3,
+                  "lin_solver must be 0 (CG), 1 (GMRES), 2 (mixed GMRES) or 3 (mixed CG)");
         RSD_CHECK(krylov >= 1 && krylov <= 512, "krylov dimension outside [1, 512]");
         its = newton_run(pb, src_expr, abs_tol, rel_tol, max_newton, lin_solver, lin_tol, lin_max_iters, krylov,
                          precond, d_U, res_hist, lin_iters);

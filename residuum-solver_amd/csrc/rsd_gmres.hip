@@ -11,6 +11,7 @@
 // the state once per cycle.
 #include "rsd_gmres.hpp"
 #include "rsd_linalg.hpp"
+#include "rsd_mixed.hpp"
 #include "rsd_rowdot.hpp"
 
 #include <algorithm>
@@ -264,9 +265,250 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
     return res;
 }
 
+// ------------------------------------------------------------------------
+// Mixed precision GMRES: iterative refinement around an FP32 inner solve.
+// Outer (FP64): r = b - A x with the FP64 values, stop on ||r|| / (||r_0||
+// + 1e-50) < tol, x += ||r|| d.  Inner: right-preconditioned GMRES(m) on
+// A d = r / ||r|| with the FP32 copy of the values (any matrix: the full
+// sliced ELL layout, not the symmetric half storage), FP32 basis vectors,
+// products and vector updates; the Gram-Schmidt and norm dots sum exact
+// FP64 products; Hessenberg, Givens and back-substitution in FP64 (the
+// kernels of the FP64 solver).  The inner tolerance is the mixed CG's:
+// max(floor, 0.5 tol ||r_0|| / ||r||), capped at 0.5.
+// ------------------------------------------------------------------------
+__global__ void k_gm32_convert(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void k_gm32_inv_diag(int64_t n, const int32_t* __restrict__ diag, const double* __restrict__ val,
+                                float* __restrict__ inv) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int32_t s = diag[i];
+    const double d = s >= 0 ? val[s] : 0.0;
+    inv[i] = (float)((fabs(d) > 1e-300) ? 1.0 / d : 1.0);
+}
+
+// w = A32 (M q), FP32 accumulation
+template<int MODE>
+__global__ __launch_bounds__(kBlock) void k_gm32_apply(int64_t n, int64_t nb, Cols C, const float* __restrict__ val,
+                                                       const float* __restrict__ q, const float* __restrict__ invd,
+                                                       float* __restrict__ w, const GMState* st) {
+    if (st->stop) return;
+    const int64_t r = xcd_block(blockIdx.x, nb) * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const int64_t base = C.slice_ptr[r >> 6] + (r & 63);
+    const RowCols<MODE> rc(r, C);
+    float sum = 0.0f;
+    for (int j = 0; j < rc.len; ++j) {
+        const int64_t s = base + (int64_t)j * kSlice;
+        const int c = rc.col(C, r, s, j);
+        const float z = invd ? q[c] * invd[c] : q[c];
+        sum = __builtin_fmaf(stream_load(val + s), z, sum);
+    }
+    w[r] = sum;
+}
+
+// inner residual q0 = r64 / ||r|| - A32 d (d = 0 on the first inner cycle:
+// no product), block sums of ||q0||^2
+template<int MODE>
+__global__ __launch_bounds__(kBlock) void k_gm32_inres(int64_t n, int64_t nb, Cols C, const float* __restrict__ val,
+                                                       const double* __restrict__ r64, double sc,
+                                                       const float* __restrict__ d, int32_t first,
+                                                       float* __restrict__ q0, double* partials) {
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t k = lb * kBlock + threadIdx.x;
+    double v[1] = {0.0};
+    if (k < n) {
+        float qk = (float)(r64[k] * sc);
+        if (!first) qk = qk - row_dot_f32<MODE>(k, C, val, d);
+        q0[k] = qk;
+        v[0] = (double)qk * (double)qk;
+    }
+    block_partials<1>(v, lb, nb, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gm32_mgs(int64_t n, int64_t nb, float* __restrict__ w,
+                                                     const float* __restrict__ Qprev, const float* __restrict__ Qi,
+                                                     int32_t i, int32_t j, int32_t m, const double* H,
+                                                     const GMState* st, double* partials) {
+    if (st->stop) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t k = lb * kBlock + threadIdx.x;
+    const float hprev = i > 0 ? (float)H[(int64_t)j * (m + 1) + (i - 1)] : 0.0f;
+    double v[1] = {0.0};
+    if (k < n) {
+        float wk = w[k];
+        if (i > 0) {
+            wk = __builtin_fmaf(-hprev, Qprev[k], wk);
+            w[k] = wk;
+        }
+        v[0] = Qi ? (double)Qi[k] * (double)wk : (double)wk * (double)wk;
+    }
+    block_partials<1>(v, lb, nb, partials);
+}
+
+__global__ void k_gm32_newq(int64_t n, const float* __restrict__ w, float* __restrict__ q, const GMState* st) {
+    if (st->stop || st->hnext <= 1e-14) return;
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k < n) q[k] = w[k] * (float)(1.0 / st->hnext);
+}
+
+__global__ void k_gm32_cycle_start(int64_t n, float* __restrict__ q0, double* g, int32_t m, GMState* st) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const double beta = st->beta;
+    if (k < n) q0[k] *= (float)(1.0 / beta);
+    if (k == 0) {
+        g[0] = beta;
+        for (int i = 1; i <= m; ++i) g[i] = 0.0;
+    }
+}
+
+// d += M (sum_i y_i q_i), the sum in FP64
+__global__ void k_gm32_update_d(int64_t n, const float* const* __restrict__ Q, const double* __restrict__ y,
+                                const float* __restrict__ invd, float* __restrict__ d, const GMState* st) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    double z = 0.0;
+    for (int i = 0; i < st->mloc; ++i) z += y[i] * (double)Q[i][k];
+    if (invd) z *= (double)invd[k];
+    d[k] = (float)((double)d[k] + z);
+}
+
+__global__ void k_gm32_update_x(int64_t n, double* __restrict__ x, const float* __restrict__ d, double res) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k < n) x[k] += res * (double)d[k];
+}
+
+MixedGMResult gmres_mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol,
+                              int64_t max_iters, int32_t m, int32_t precond, double inner_floor) {
+    const Sell& S = A->pb->sell;
+    const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n)), nbe = nblocks(std::max<int64_t>(n, 1));
+    hipStream_t st = ctx->stream;
+    const size_t fb = sizeof(float) * (size_t)std::max<int64_t>(n, 1);
+    std::vector<float*> hq(m + 1);
+    for (int i = 0; i <= m; ++i) hq[i] = (float*)ctx->ws("gm32.Q" + std::to_string(i), fb);
+    float* w = (float*)ctx->ws("gm32.w", fb);
+    float* d = (float*)ctx->ws("gm32.d", fb);
+    float* invd = precond == RSD_PC_JACOBI ? (float*)ctx->ws("gm32.invd", fb) : nullptr;
+    double* r64 = (double*)ctx->ws("gm32.r64", sizeof(double) * (size_t)std::max<int64_t>(n, 1));
+    double* H = (double*)ctx->ws("gm.H", sizeof(double) * (size_t)(m + 1) * m);
+    double* cs = (double*)ctx->ws("gm.cs", sizeof(double) * (size_t)(m + 1));
+    double* sn = (double*)ctx->ws("gm.sn", sizeof(double) * (size_t)(m + 1));
+    double* g = (double*)ctx->ws("gm.g", sizeof(double) * (size_t)(m + 1));
+    double* y = (double*)ctx->ws("gm.y", sizeof(double) * (size_t)(m + 1));
+    float** dQ = (float**)ctx->ws("gm32.Qptr", sizeof(float*) * (size_t)(m + 1));
+    GMState* so = (GMState*)ctx->ws("gm32.outer", sizeof(GMState));
+    GMState* si = (GMState*)ctx->ws("gm32.inner", sizeof(GMState));
+    double* part = ctx->partials(nb);
+    if (A->val32f.n != S.slots) A->val32f.alloc(S.slots);
+    if (S.slots)
+        hipLaunchKernelGGL(k_gm32_convert, dim3((unsigned)nblocks(S.slots)), dim3(kBlock), 0, st, S.slots, A->val.p,
+                           A->val32f.p);
+    if (invd)
+        hipLaunchKernelGGL(k_gm32_inv_diag, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, S.diag.p, A->val.p, invd);
+    RSD_HIP(hipMemcpyAsync(dQ, hq.data(), sizeof(float*) * (size_t)(m + 1), hipMemcpyHostToDevice, st));
+    RSD_HIP(hipMemsetAsync(so, 0, sizeof(GMState), st));
+    GMState* h = nullptr;
+    RSD_HIP(hipHostMalloc((void**)&h, sizeof(GMState), hipHostMallocDefault));
+    auto read = [&](const GMState* s) {
+        RSD_HIP(hipMemcpyAsync(h, s, sizeof(GMState), hipMemcpyDeviceToHost, st));
+        RSD_HIP(hipStreamSynchronize(st));
+    };
+    auto outer_residual = [&](int32_t first) {
+        RSD_LAUNCH_COLS(S.mode(), k_gm_residual, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(), A->val.p, b, x,
+                        r64, part);
+        hipLaunchKernelGGL(k_gm_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part, so, first);
+        read(so);
+        return h->beta;
+    };
+    MixedGMResult out{};
+    try {
+        const double res0 = outer_residual(1);
+        double res = res0;
+        out.initial_residual = res0;
+        int64_t total = 0, cycles = 0;
+        for (;;) {
+            const double rel = res / (res0 + 1e-50);
+            if (rel < tol) { out.converged = 1; break; }
+            if (cycles >= kMixedMaxCycles || total >= max_iters || !(rel == rel)) break;
+            double in_tol = 0.5 * tol * (res0 + 1e-50) / res;
+            in_tol = std::min(0.5, std::max(inner_floor, in_tol));
+            const int64_t cap = std::max<int64_t>(1, max_iters - total);
+            launch_fill_f32(ctx, n, d, 0.0f);
+            RSD_HIP(hipMemsetAsync(si, 0, sizeof(GMState), st));
+            for (int32_t first = 1;; first = 0) {
+                RSD_LAUNCH_COLS(S.mode(), k_gm32_inres, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(),
+                                A->val32f.p, r64, 1.0 / res, d, first, hq[0], part);
+                hipLaunchKernelGGL(k_gm_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part, si, first);
+                if (!first) {  // a restart: the true inner residual may already meet the tolerance
+                    read(si);
+                    if (h->beta / (h->res0 + 1e-50) < in_tol) break;
+                }
+                hipLaunchKernelGGL(k_gm_reset_flags, dim3(1), dim3(1), 0, st, si);
+                hipLaunchKernelGGL(k_gm32_cycle_start, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, hq[0], g, m, si);
+                for (int j = 0; j < m; ++j) {
+                    RSD_LAUNCH_COLS(S.mode(), k_gm32_apply, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(),
+                                    A->val32f.p, hq[j], invd, w, si);
+                    for (int i = 0; i <= j + 1; ++i) {
+                        const bool norm = i == j + 1;
+                        hipLaunchKernelGGL(k_gm32_mgs, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
+                                           i > 0 ? hq[i - 1] : nullptr, norm ? nullptr : hq[i], i, j, m, H, si, part);
+                        hipLaunchKernelGGL(k_gm_mgs_fold, dim3(1), dim3(kFold), 0, st, nb, part, i, j, m,
+                                           (int32_t)norm, H, si);
+                    }
+                    hipLaunchKernelGGL(k_gm32_newq, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, w, hq[j + 1], si);
+                    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, st, j, m, H, cs, sn, g, si, in_tol, 0, cap);
+                    RSD_HIP(hipGetLastError());
+                }
+                hipLaunchKernelGGL(k_gm_backsub, dim3(1), dim3(1), 0, st, m, H, g, y, si);
+                hipLaunchKernelGGL(k_gm32_update_d, dim3((unsigned)nbe), dim3(kBlock), 0, st, n,
+                                   (const float* const*)dQ, y, invd, d, si);
+                RSD_HIP(hipGetLastError());
+                read(si);
+                if (h->stop) break;  // tolerance, breakdown or the iteration cap
+            }
+            total += h->total;
+            ++cycles;
+            hipLaunchKernelGGL(k_gm32_update_x, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, x, d, res);
+            res = outer_residual(0);
+        }
+        out.iterations = total;
+        out.cycles = cycles;
+        out.final_residual = res;
+        out.final_residual_rel = res / (res0 + 1e-50);
+    } catch (...) {
+        (void)hipHostFree(h);
+        throw;
+    }
+    (void)hipHostFree(h);
+    return out;
+}
+
 }  // namespace rsd
 
 using namespace rsd;
+
+extern "C" int rsd_gmres_solve_mixed(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol,
+                                     int64_t max_iters, int32_t krylov, int32_t precond, double inner_floor,
+                                     rsd_report* rep, int64_t* outer_cycles) {
+    return guard([&] {
+        RSD_CHECK(ctx && A && b && x, "rsd_gmres_solve_mixed: null argument");
+        RSD_CHECK(krylov >= 1 && krylov <= 512, "krylov dimension outside [1, 512]");
+        RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
+        const MixedGMResult r = gmres_mixed_run(ctx, A, b, x, tol, max_iters, krylov, precond,
+                                                inner_floor > 0 ? inner_floor : kMixedDefaultInnerFloor);
+        if (rep) {
+            rep->converged = r.converged;
+            rep->iterations = r.iterations;
+            rep->initial_residual = r.initial_residual;
+            rep->final_residual = r.final_residual;
+            rep->final_residual_rel = r.final_residual_rel;
+        }
+        if (outer_cycles) *outer_cycles = r.cycles;
+    });
+}
 
 extern "C" int rsd_gmres_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol,
                                    int32_t tol_type, int64_t max_iters, int32_t krylov, int32_t precond,

@@ -329,8 +329,12 @@ void mixed_launch_inner_iter(rsd_ctx* ctx, const rsd_matrix* A, int32_t precond,
     double* P = ctx->partials(mixed_partials(n));
     const float* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     hipStream_t s = ctx->stream;
-    RSD_LAUNCH_COLS32(w.lo || w.hi ? S.mode_padded() : S.mode32(), k_mx_spmv_wu, dim3((unsigned)nb), dim3(kBlock),
-                      s, n, nb, S.cols32(), A->val32.p, w.u, w.w, w.st, P);
+    if (w.full)
+        RSD_LAUNCH_COLS(S.mode(), k_mx_spmv_wu, dim3((unsigned)nb), dim3(kBlock), s, n, nb, S.cols(), A->val32f.p, w.u,
+                        w.w, w.st, P);
+    else
+        RSD_LAUNCH_COLS32(w.lo || w.hi ? S.mode_padded() : S.mode32(), k_mx_spmv_wu, dim3((unsigned)nb),
+                          dim3(kBlock), s, n, nb, S.cols32(), A->val32.p, w.u, w.w, w.st, P);
     hipLaunchKernelGGL(k_mx_scalars, dim3(1), dim3(kFold), 0, s, nb, P, nbu, P + nb, w.st);
     hipLaunchKernelGGL(k_mx_update, dim3((unsigned)nbu), dim3(kBlock), 0, s, n, nbu, w.p, w.s, w.d, w.u, w.w, invd,
                        w.st, P + nb);
@@ -346,7 +350,14 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
     const Sell& S = A->pb->sell;
     const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n)), nbt = mixed_blocks(n);
     hipStream_t st = ctx->stream;
-    if (!fresh_f32) matrix_to_f32(ctx, A, true);
+    if (w.full) {
+        if (A->val32f.n != S.slots) A->val32f.alloc(S.slots);
+        if (S.slots)
+            hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nblocks(S.slots)), dim3(kBlock), 0, st, S.slots, A->val.p,
+                               A->val32f.p);
+    } else if (!fresh_f32) {
+        matrix_to_f32(ctx, A, true);
+    }
     double* part = ctx->partials(mixed_partials(n));
     const float* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     if (precond == RSD_PC_JACOBI)
@@ -369,7 +380,7 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
             RSD_LAUNCH_COLS(S.mode(), k_mx_residual, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(), A->val.p, b,
                             x, w.r64, part);
             hipLaunchKernelGGL(k_mx_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part,
-                               S.has_half ? A->asym.p : nullptr, w.st);
+                               S.has_half && !w.full ? A->asym.p : nullptr, w.st);
             RSD_HIP(hipMemcpyAsync(h, w.st, sizeof(MixedState), hipMemcpyDeviceToHost, st));
             RSD_HIP(hipStreamSynchronize(st));
             if (h->asym)

@@ -22,6 +22,10 @@ struct MixedWork {
     float* ubase = nullptr;   // u - lo: u's buffer with the uniform stencil's padding
     int64_t lo = 0, hi = 0;
     MixedState* st = nullptr;
+    // the inner products use the full FP32 copy in the FP64 layout (val32f)
+    // instead of the symmetric half storage: for matrices that are not
+    // symmetric (a Newton Jacobian with a temperature-dependent conductivity)
+    bool full = false;
 };
 
 // replays a fixed chunk of inner iterations (a hipGraph in the BE driver)

@@ -186,6 +186,7 @@ struct rsd_matrix {
     std::unique_ptr<rsd_problem> own_pb;  // pattern-only problem of a matrix made from a CSR
     rsd::DBuf<double> val;    // [slots] SELL order
     rsd::DBuf<float> val32;   // [sell.slots32()] FP32 copy for the mixed-precision path
+    rsd::DBuf<float> val32f;  // [sell.slots] full FP32 copy in the FP64 layout (mixed GMRES, any matrix)
     rsd::DBuf<int32_t> asym;  // [1] set by the FP32 conversion when the half layout's symmetry check fails
 };
 

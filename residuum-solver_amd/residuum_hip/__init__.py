@@ -555,6 +555,9 @@ class Problem:
 
     def newton(self, U, src=None, abs_tol=1e-10, rel_tol=1e-10, max_newton=50, lin_solver=1, lin_tol=1e-12,
                lin_max=5000, krylov=30, precond=PC_JACOBI):
+        """Steady Newton; lin_solver 0 CG, 1 GMRES(krylov), 2 mixed-precision
+        GMRES(krylov), 3 mixed-precision CG (full FP32 copy of J).  Returns (iteration it converged at or < 0, ||R||
+        history, inner iterations per Newton step)."""
         its = C.c_int64()
         rh = np.zeros(max_newton)
         li = np.zeros(max_newton, np.int64)
@@ -564,7 +567,6 @@ class Problem:
                                    dptr(U), C.byref(its), rh.ctypes.data_as(_d), li.ctypes.data_as(_i64)))
         k = its.value
         return k, rh[:max(k, 0) + 1], li[:max(k, 0)]
-
 
     def adjoint(self, d_kappa, U0, t0, dt, n_steps, obs, t_obs, tol=1e-13, max_iters=10000, want_grad=True,
                 d_grad=None, d_UN=None):
@@ -619,6 +621,16 @@ class Matrix:
         check(lib().rsd_gmres_solve_f64(self.pb.ctx.h, self.h, dptr(b), dptr(x), C.c_double(tol), C.c_int32(tol_type),
                                         C.c_int64(max_iters), C.c_int32(krylov), C.c_int32(precond), C.byref(rep)))
         return rep
+
+    def gmres_mixed(self, b, x, tol=1e-10, max_iters=1000, krylov=30, precond=PC_JACOBI, inner_floor=0.0):
+        """FP32 inner GMRES(krylov) with FP64 refinement; returns (report,
+        outer cycles)."""
+        rep = Report()
+        cyc = C.c_int64()
+        check(lib().rsd_gmres_solve_mixed(self.pb.ctx.h, self.h, dptr(b), dptr(x), C.c_double(tol),
+                                          C.c_int64(max_iters), C.c_int32(krylov), C.c_int32(precond),
+                                          C.c_double(inner_floor), C.byref(rep), C.byref(cyc)))
+        return rep, cyc.value
 
     def to_f32(self):
         check(lib().rsd_matrix_to_f32(self.pb.ctx.h, self.h))

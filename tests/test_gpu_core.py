@@ -545,6 +545,64 @@ def test_gmres_bitexact(ctx, case, precond):
         assert same(dx.numpy(), xo)
 
 
+@pytest.mark.parametrize("case", CASES[:3])
+@pytest.mark.parametrize("precond", [0, 1])
+def test_gmres_mixed(ctx, case, precond):
+    """Mixed-precision GMRES on the non-symmetric tangent Jacobian: its
+    solution's FP64 relative residual, recomputed here from an FP64 product,
+    is below the tolerance, which one FP32 inner solve cannot reach (so
+    several refinements ran), and it agrees with the FP64 GMRES solution."""
+    import residuum_hip as rh
+    dim, n, p, nq, tags, exprs, ext = case
+    tensor = (1.0, 0.2, 0.2, 0.1) if dim == 2 else (1, 0.2, 0, 0.2, 0.1, 0, 0, 0, 1)
+    pb = rh.Problem(ctx, dim, n, p, nq, tags, exprs, ext=ext)
+    pb.set_conductivity(1.0, k_expr="1.0 + 0.5*T", dk_expr="0.5", tensor=tensor)
+    nf = pb.n_free
+    U = np.random.default_rng(4).random(nf)
+    J = pb.matrix()
+    pb.assemble_matrix(J, [(O.DIFFUSION, 1.0), (O.TDIFFUSION, 1.0)], U=ctx.from_numpy(U))
+    b = np.random.default_rng(5).standard_normal(nf)
+    db = ctx.from_numpy(b)
+    ref = pb.vector(0.0)
+    J.gmres(db, ref, tol=1e-13, max_iters=4000, krylov=30, precond=precond)
+    for krylov in (5, 30):
+        dx = pb.vector(0.0)
+        rep, cyc = J.gmres_mixed(db, dx, tol=1e-10, max_iters=4000, krylov=krylov, precond=precond)
+        y = pb.vector()
+        J.spmv(dx, y)
+        rel = np.linalg.norm(b - y.numpy()) / np.linalg.norm(b)
+        say("gmres mixed", krylov, precond, rep.iterations, cyc, rel)
+        assert rep.converged == 1 and cyc >= 2 and rep.iterations > 0
+        assert rep.final_residual_rel < 1e-10 and rel < 1.5e-10
+        x = dx.numpy()
+        assert np.max(np.abs(x - ref.numpy())) <= 1e-8 * np.max(np.abs(ref.numpy()))
+
+
+def test_newton_mixed_inner(ctx):
+    """Newton with mixed-precision inner solves (GMRES, and CG on the full
+    FP32 copy of the non-symmetric Jacobian) on the anisotropic Q2 problem of
+    the bitwise Newton test: the same Newton iteration count and the
+    FP64-GMRES solution to 1e-9 (the inner solves stop at 1e-12 either way,
+    so the iterates differ only by their rounding)."""
+    import residuum_hip as rh
+    dim, n, p, nq = 2, (12, 10), (2, 2), 3
+    tensor = (1.0, 0.0, 0.0, 0.1)
+    src = "10*sin(pi*x)*sin(pi*y)"
+    out = []
+    for lin_solver in (1, 2, 3):
+        pb = rh.Problem(ctx, dim, n, p, nq, [0, 1, 2, 3], ["0"] * 4)
+        pb.set_conductivity(1.0, k_expr="1.0 + 0.5*T", dk_expr="0.5", tensor=tensor)
+        dU = pb.vector(0.0)
+        its, res, lin = pb.newton(dU, src=src, abs_tol=1e-10, rel_tol=1e-10, lin_solver=lin_solver, lin_tol=1e-12,
+                                  krylov=30)
+        say("newton", lin_solver, its, list(lin), list(res))
+        assert its > 0
+        out.append((its, dU.numpy()))
+    for its, u in out[1:]:
+        assert its == out[0][0]
+        assert np.max(np.abs(u - out[0][1])) <= 1e-9 * np.max(np.abs(out[0][1]))
+
+
 def test_newton_bitexact(ctx):
     import residuum_hip as rh
     dim, n, p, nq = 2, (12, 10), (2, 2), 3
