@@ -30,6 +30,13 @@ SteadyStage.hpp:60-84; nonlinear conductivity: Newton, Newton.hpp:32-83) or
 the transient Backward Euler loop (BackwardEulerStage.hpp:71-163) with the
 CSR operator and CG (rsd_be_run, the whole loop on the device) or, for the
 FEM operator or GMRES, one device solve per step.
+
+A linear-solver node may carry `precision: mixed` (a device extension; the
+default `fp64` is the reference's behaviour): FP32 inner solves with FP64
+iterative refinement -- rsd_cg_solve_mixed / rsd_gmres_solve_mixed for a
+steady solve, the mixed Backward Euler loop, and Newton's mixed inner
+solves (GMRES, or CG on a full FP32 copy of the Jacobian).  It needs the
+csr operator.
 """
 import argparse
 import os
@@ -83,9 +90,14 @@ def _linear(node, where):
                max_iterations=int(_opt(node, "max_iterations", 1000)),
                krylov_dim=int(_opt(node, "krylov_dim", 50)),
                operator=_req(op, "type", where + ".operator") if op else "csr",
-               preconditioner=_req(pc, "type", where + ".preconditioner") if pc else "identity")
+               preconditioner=_req(pc, "type", where + ".preconditioner") if pc else "identity",
+               precision=str(_opt(node, "precision", "fp64")))
     if cfg["operator"] not in ("csr", "fem"):
         raise ConfigError(f"Unknown linear solver operator type: {cfg['operator']}. Valid options: csr, fem")
+    if cfg["precision"] not in ("fp64", "mixed"):
+        raise ConfigError(f"{where}: unknown precision '{cfg['precision']}'. Valid options: fp64, mixed")
+    if cfg["precision"] == "mixed" and cfg["operator"] != "csr":
+        raise ConfigError(f"{where}: mixed precision needs the csr operator")
     if cfg["preconditioner"] not in ("identity", "jacobi"):
         raise ConfigError(f"Unknown preconditioner type: {cfg['preconditioner']}. Valid options: identity, jacobi")
     return cfg
@@ -402,8 +414,14 @@ def _solve(pb, K, forms, lin, F, U, state=None, time=0.0):
             raise ConfigError("the matrix-free operator runs with cg on the device")
         return pb.cg_fem(F, U, forms, tol=lin["tolerance"], max_iters=lin["max_iterations"], precond=_precond(lin),
                          U=state, time=time)
+    mixed = lin["precision"] == "mixed"
     if lin["type"] == "cg":
+        if mixed:
+            return K.cg_mixed(F, U, tol=lin["tolerance"], max_iters=lin["max_iterations"], precond=_precond(lin))[0]
         return K.cg(F, U, tol=lin["tolerance"], max_iters=lin["max_iterations"], precond=_precond(lin))
+    if mixed:
+        return K.gmres_mixed(F, U, tol=lin["tolerance"], max_iters=lin["max_iterations"], krylov=lin["krylov_dim"],
+                             precond=_precond(lin))[0]
     return K.gmres(F, U, tol=lin["tolerance"], max_iters=lin["max_iterations"], krylov=lin["krylov_dim"],
                    precond=_precond(lin))
 
@@ -433,7 +451,9 @@ def _steady(pb, cfg, U, src_text, src_nodal, quiet, mon=None):
             raise ConfigError("Newton runs with the CSR operator on the device")
         t0 = _time.perf_counter()
         its, res, lin_its = pb.newton(U, src=src_text, abs_tol=nl["abs_tol"], rel_tol=nl["rel_tol"],
-                                      max_newton=nl["max_iterations"], lin_solver=1 if lin["type"] == "gmres" else 0,
+                                      max_newton=nl["max_iterations"],
+                                      lin_solver={("cg", "fp64"): 0, ("gmres", "fp64"): 1, ("gmres", "mixed"): 2,
+                                                  ("cg", "mixed"): 3}[(lin["type"], lin["precision"])],
                                       lin_tol=lin["tolerance"], lin_max=lin["max_iterations"],
                                       krylov=lin["krylov_dim"], precond=_precond(lin))
         conv = its >= 0
@@ -475,7 +495,7 @@ def _transient(pb, cfg, U, src_text, src_nodal, mesh, quiet, mon=None):
     start = _time.perf_counter()
     if lin["operator"] == "csr" and lin["type"] == "cg" and src_nodal is None and cfg["output"] is None and not mon:
         steps, iters = pb.be_run(U, t0, tf, dt, src=src_text, tol=lin["tolerance"], max_iters=lin["max_iterations"],
-                                 precond=_precond(lin))
+                                 precond=_precond(lin), mixed=lin["precision"] == "mixed")
         _log(f"backward euler: {steps} steps on the device, {int(np.sum(iters))} CG iterations "
              f"({_time.perf_counter() - start:.3f} s)", quiet)
         return dict(converged=True, steps=int(steps), iterations=[int(v) for v in iters])

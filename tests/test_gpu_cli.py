@@ -50,6 +50,41 @@ def test_steady_bilinear(ctx, tmp_path, operator, precond):
     assert np.max(np.abs(s["U"] - (1 - xy[:, 0]) * (1 - xy[:, 1]))) <= 1e-10
 
 
+@pytest.mark.parametrize("solver", ["cg", "gmres"])
+def test_steady_mixed_precision(ctx, tmp_path, solver):
+    """`precision: mixed` runs the steady solve with FP32 inner solves and
+    FP64 refinement: the bilinear solution to 1e-10, as in FP64."""
+    from residuum_hip import cli
+    d = bench_config(tmp_path)
+    d["solver"]["linear"].update(type=solver, precision="mixed", krylov_dim=30)
+    d["solver"]["linear"]["operator"]["type"] = "csr"
+    d["solver"]["linear"]["preconditioner"]["type"] = "jacobi"
+    s = cli.run_heat(write_cfg(tmp_path, d), quiet=True, ctx=ctx)
+    say("steady mixed", solver, s["iterations"], s["final_residual"])
+    assert s["converged"]
+    xy = nodes_xy(ctx, str(tmp_path / "square_n16.pmsh"))
+    assert np.max(np.abs(s["U"] - (1 - xy[:, 0]) * (1 - xy[:, 1]))) <= 1e-10
+
+
+def test_transient_mixed_precision(ctx, tmp_path):
+    """The Backward Euler loop on the device with `precision: mixed` agrees
+    with the FP64 loop to the solver tolerance."""
+    from residuum_hip import cli
+    d = bench_config(tmp_path)
+    for bc in d["boundary_conditions"]:
+        bc["read"]["expression"] = "100.0"
+    d["physics"]["models"]["source"]["read"]["expression"] = "10*x*y"
+    d["solver"]["driver"]["type"] = "transient"
+    d["solver"]["timestepper"] = {"type": "backward_euler", "t0": 0.0, "tf": 0.1, "step_size": {"mode": "constant", "dt": 0.02}}
+    d["solver"]["linear"] = {"type": "cg", "operator": {"type": "csr"}, "preconditioner": {"type": "jacobi"},
+                             "tolerance": 1e-10, "max_iterations": 5000}
+    s64 = cli.run_heat(write_cfg(tmp_path, d), quiet=True, ctx=ctx)
+    d["solver"]["linear"]["precision"] = "mixed"
+    smx = cli.run_heat(write_cfg(tmp_path, d, "mixed.yaml"), quiet=True, ctx=ctx)
+    assert s64["steps"] == smx["steps"] == 5
+    assert np.max(np.abs(smx["U"] - s64["U"])) <= 1e-8 * np.max(np.abs(s64["U"]))
+
+
 def test_transient_paths_agree(ctx, tmp_path):
     """Backward Euler from a configuration: the whole loop on the device
     (CSR + CG) equals Problem.be_run bit for bit; the matrix-free operator

@@ -95,7 +95,20 @@ def test_heat_config_parse():
     assert cfg["source"] == ("expression", "0.0") and cfg["initial"] == ("expression", "0.0")
     assert cfg["driver"] == "steady"
     assert cfg["linear"] == dict(type="cg", tolerance=1e-12, max_iterations=10000, krylov_dim=50, operator="fem",
-                                 preconditioner="identity")
+                                 preconditioner="identity", precision="fp64")
+
+
+def test_heat_config_precision(tmp_path):
+    """`precision: mixed` on a linear-solver node selects the mixed solves;
+    fp64 is the default."""
+    import yaml
+    from residuum_hip import cli
+    d = yaml.safe_load(open(os.path.join(GOLDEN, "heateq_square_n16.yaml")))
+    d["solver"]["linear"]["operator"]["type"] = "csr"
+    d["solver"]["linear"]["precision"] = "mixed"
+    p = tmp_path / "c.yaml"
+    p.write_text(yaml.safe_dump(d))
+    assert cli.parse_heat_config(str(p))["linear"]["precision"] == "mixed"
 
 
 @pytest.mark.parametrize("edit,msg", [
@@ -105,6 +118,8 @@ def test_heat_config_parse():
     (lambda d: d["solver"]["linear"]["preconditioner"].update(type="ilu"), "preconditioner"),
     (lambda d: d["solver"]["driver"].update(type="pseudo_transient"), "driver"),
     (lambda d: d.pop("solver"), "solver"),
+    (lambda d: d["solver"]["linear"].update(precision="fp16"), "precision"),
+    (lambda d: d["solver"]["linear"].update(precision="mixed"), "csr operator"),
 ])
 def test_heat_config_refusals(edit, msg, tmp_path):
     """What the device path does not run is refused with a message."""
