@@ -87,7 +87,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv_f32(int64_t n, int64_t nb, Cols
 template<int MODE>
 __global__ __launch_bounds__(kBlock) void k_mx_residual(int64_t n, int64_t nb, Cols C, const double* __restrict__ val,
                                                         const double* __restrict__ b, const double* __restrict__ x,
-                                                        double* __restrict__ r64, double* partials) {
+                                                        double* __restrict__ r64, double* partials,
+                                                        const MixedState* st) {
+    if (st->done) return;
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t i = lb * kBlock + threadIdx.x;
     double v[1] = {0.0};
@@ -99,16 +101,35 @@ __global__ __launch_bounds__(kBlock) void k_mx_residual(int64_t n, int64_t nb, C
     block_partials<1>(v, lb, nb, partials);
 }
 
-__global__ __launch_bounds__(kFold) void k_mx_residual_fold(int64_t nb, const double* __restrict__ partials,
-                                                            const int32_t* __restrict__ asym, MixedState* st) {
+// the outer loop's step, on the device: ||r||, then (in the host solver's
+// double arithmetic) convergence, the caps, or the next inner solve's
+// tolerance max(floor, 0.5 tol ||r_0|| / ||r||) capped at 0.5 and its cap
+__global__ __launch_bounds__(kFold) void k_mx_outer(int64_t nb, const double* __restrict__ partials,
+                                                    const int32_t* __restrict__ asym, MixedState* st, double tol,
+                                                    int64_t max_iters, double inner_floor) {
+    if (st->done) return;
     double out[1];
     fold_partials<1>(nb, partials, out);
-    if (threadIdx.x == 0) {
-        const double res = sqrt(out[0]);
-        if (st->cycles == 0) st->res0 = res;
-        st->res = res;
-        st->asym = asym ? *asym : 0;
+    if (threadIdx.x != 0) return;
+    const double res = sqrt(out[0]);
+    if (st->cycles == 0) st->res0 = res;
+    st->res = res;
+    st->asym = asym ? *asym : 0;
+    const double rel = res / (st->res0 + 1e-50);
+    if (st->asym || rel < tol || st->cycles >= kMixedMaxCycles || st->total_inner >= max_iters || !(rel == rel)) {
+        st->converged = !st->asym && rel < tol;
+        st->done = 1;
+        st->stop = 1;
+        return;
     }
+    double it = 0.5 * tol * (st->res0 + 1e-50) / res;
+    it = (inner_floor < it) ? it : inner_floor;  // max(inner_floor, it)
+    it = (it < 0.5) ? it : 0.5;                  // min(0.5, it)
+    st->in_tol = it;
+    st->in_max = max_iters - st->total_inner > 1 ? max_iters - st->total_inner : 1;
+    st->cycles = st->cycles + 1;
+    st->it = 0;
+    st->stop = 0;
 }
 
 // inner set-up: r = r64 / ||r||, u = M r, p = s = d = 0; block sums of r.u,
@@ -118,6 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_init(int64_t n, int64_t nb, const
                                                     float* __restrict__ s, float* __restrict__ d,
                                                     const float* __restrict__ invd, const MixedState* st,
                                                     double* P12) {
+    if (st->done) return;
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const double sc = st->res > 0.0 ? 1.0 / st->res : 0.0;
     double v[2] = {0.0, 0.0};
@@ -244,6 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_mx_update(int64_t n, int64_t nb, flo
 }
 
 __global__ void k_mx_update_x(int64_t n, double* __restrict__ x, const float* __restrict__ d, const MixedState* st) {
+    if (st->done) return;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) x[i] += st->res * (double)d[i];
 }
@@ -376,45 +399,38 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
         else for (int64_t k = 0; k < kInnerChunk; ++k) mixed_launch_inner_iter(ctx, A, precond, w);
     };
     try {
+        // Per cycle: the residual and the outer decision (device), the inner
+        // set-up and the chunks the same cycle of the previous solve needed,
+        // then one look at the state; every kernel returns at once once the
+        // solve is done or the inner solve has stopped.
+        const bool known = hint && hint[0] > 0;
         for (int cycle = 0;; ++cycle) {
+            // the outer decision ends the loop by cycle kMixedMaxCycles; hint has kMixedMaxCycles + 1 entries
+            RSD_CHECK(cycle <= kMixedMaxCycles, "mixed CG: outer loop did not end");
             RSD_LAUNCH_COLS(S.mode(), k_mx_residual, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(), A->val.p, b,
-                            x, w.r64, part);
-            hipLaunchKernelGGL(k_mx_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part,
-                               S.has_half && !w.full ? A->asym.p : nullptr, w.st);
-            RSD_HIP(hipMemcpyAsync(h, w.st, sizeof(MixedState), hipMemcpyDeviceToHost, st));
-            RSD_HIP(hipStreamSynchronize(st));
-            if (h->asym)
-                throw Error(RSD_ERR_ARG, "mixed CG: the matrix is not symmetric (a value differs from its mirror by "
-                                         "more than 1e-10 of its row's largest entry)");
-            const double rel = h->res / (h->res0 + 1e-50);
-            if (rel < tol) { h->converged = 1; break; }
-            if (cycle >= kMixedMaxCycles || h->total_inner >= max_iters || !(rel == rel)) break;
-            // inner tolerance: what is still missing, never below the FP32 floor
-            double it = 0.5 * tol * (h->res0 + 1e-50) / h->res;
-            it = std::min(0.5, std::max(inner_floor, it));
-            h->in_tol = it;
-            h->in_max = std::max<int64_t>(1, max_iters - h->total_inner);
-            h->cycles = cycle + 1;
-            h->it = 0;
-            h->stop = 0;
-            RSD_HIP(hipMemcpyAsync(w.st, h, sizeof(MixedState), hipMemcpyHostToDevice, st));
+                            x, w.r64, part, w.st);
+            hipLaunchKernelGGL(k_mx_outer, dim3(1), dim3(kFold), 0, st, nb, part,
+                               S.has_half && !w.full ? A->asym.p : nullptr, w.st, tol, max_iters, inner_floor);
             hipLaunchKernelGGL(k_mx_init, dim3((unsigned)nbt), dim3(kBlock), 0, st, n, nbt, w.r64, w.u, w.p, w.s,
                                w.d, invd, w.st, part + nb);
             const int64_t before = h->total_inner;
-            // queue what the same cycle of the previous solve needed (minus a
-            // margin) before the first look at the stop flag
-            const int64_t ahead = (hint && hint[cycle] > 2) ? std::max<int64_t>(1, (hint[cycle] - 2) / kInnerChunk) : 1;
+            const int64_t hc = hint ? hint[cycle] : 0;
+            const int64_t ahead = !known ? 1 : hc > 2 ? std::max<int64_t>(1, (hc - 2) / kInnerChunk) : (hc > 0 ? 1 : 0);
             int64_t launched = 0;
             for (int64_t k = 0; k < ahead; ++k) { chunk(); launched += kInnerChunk; }
             for (;;) {
                 RSD_HIP(hipMemcpyAsync(h, w.st, sizeof(MixedState), hipMemcpyDeviceToHost, st));
                 RSD_HIP(hipStreamSynchronize(st));
-                if (h->stop) break;
+                if (h->asym)
+                    throw Error(RSD_ERR_ARG, "mixed CG: the matrix is not symmetric (a value differs from its mirror by "
+                                             "more than 1e-10 of its row's largest entry)");
+                if (h->done || h->stop) break;
                 RSD_CHECK(launched <= max_iters + 2 * kInnerChunk, "mixed CG: inner solve did not stop");
                 chunk();
                 launched += kInnerChunk;
             }
             if (hint) hint[cycle] = h->total_inner - before;
+            if (h->done) break;
             hipLaunchKernelGGL(k_mx_update_x, dim3((unsigned)nblocks(n)), dim3(kBlock), 0, st, n, x, w.d, w.st);
             RSD_HIP(hipGetLastError());
         }

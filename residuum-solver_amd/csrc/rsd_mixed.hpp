@@ -14,6 +14,7 @@ struct MixedState {
     int64_t it;                // inner iteration index
     int64_t in_max, total_inner;  // inner iteration cap; SpMVs of all inner solves
     int32_t cycles, converged, stop, asym;  // asym: the FP32 copy's symmetry check failed
+    int32_t done;              // the outer loop has ended (converged, asym, or a cap reached)
 };
 
 struct MixedWork {
