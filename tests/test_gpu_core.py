@@ -208,10 +208,11 @@ def test_assembly_schemes_bitwise(ctx, case):
 def test_be_run_assembly_schemes_bitwise(ctx):
     """Backward Euler through the fused K + M/dt and M U_prev assembly,
     captured as a graph: both assembly schemes give the oracle's solution
-    bit for bit."""
+    bit for bit, and the same mixed-precision solution."""
     case = (3, (6, 5, 10), (1, 1, 1), 2, [0, 1], ["1", "0"], None)
     pb, oc = build(ctx, case, kappa_seed=20261202)
     out = oc.be_run(0.0, 0.005, 1e-3, cg_tol=1e-8)
+    mixed = []
     for mode in (1, 0):
         pb.set_assembly(mode)
         U = pb.vector(0.0)
@@ -219,6 +220,10 @@ def test_be_run_assembly_schemes_bitwise(ctx):
         say("assembly", mode, "iters", list(iters))
         assert steps == out["steps"] == 5 and list(iters) == list(out["iters"])
         assert np.array_equal(bits(U.numpy()), bits(out["U"]))
+        Um = pb.vector(0.0)
+        steps, iters = pb.be_run(Um, 0.0, 0.005, 1e-3, tol=1e-8, mixed=True, use_graph=True)
+        mixed.append((list(iters), bits(Um.numpy())))
+    assert mixed[0][0] == mixed[1][0] and np.array_equal(mixed[0][1], mixed[1][1])
 
 
 @pytest.mark.parametrize("case", CASES[:2])
