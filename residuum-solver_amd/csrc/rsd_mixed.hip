@@ -415,9 +415,19 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
                                w.d, invd, w.st, part + nb);
             const int64_t before = h->total_inner;
             const int64_t hc = hint ? hint[cycle] : 0;
-            const int64_t ahead = !known ? 1 : hc > 2 ? std::max<int64_t>(1, (hc - 2) / kInnerChunk) : (hc > 0 ? 1 : 0);
             int64_t launched = 0;
-            for (int64_t k = 0; k < ahead; ++k) { chunk(); launched += kInnerChunk; }
+            if (!known) {
+                chunk();
+                launched = kInnerChunk;
+            } else if (hc > 0) {
+                // the same cycle of the previous solve's iterations and one
+                // more: whole chunks, then single iterations, so the look at
+                // the state usually finds the inner solve stopped with at most
+                // a few launches left over
+                const int64_t want = hc + 1;
+                for (; launched + kInnerChunk <= want; launched += kInnerChunk) chunk();
+                for (; launched < want; ++launched) mixed_launch_inner_iter(ctx, A, precond, w);
+            }
             for (;;) {
                 RSD_HIP(hipMemcpyAsync(h, w.st, sizeof(MixedState), hipMemcpyDeviceToHost, st));
                 RSD_HIP(hipStreamSynchronize(st));
