@@ -257,9 +257,13 @@ __global__ __launch_bounds__(kSlice) void k_gather_rows(const int32_t* __restric
 //   mode 0: forms applied to a selected state buffer (assembleVector)
 //   mode 1: load vector (SourceForm / NodalSourceForm)
 //   mode 2: essential lift, F -= forms(constrained values)
-template<int D, int P, int NQ, bool STATE, bool GRAD, int MODE>
+// EBV (the lift by row gather, mode 2): over every border element at once,
+// thread a writes its scatter term -1.0 * Fe to A.ebuf[k * NPE + a] (k the
+// border position) instead of adding it to F.
+template<int D, int P, int NQ, bool STATE, bool GRAD, int MODE, bool EBV = false>
 __global__ __launch_bounds__((ECfg<D, P, NQ, STATE, GRAD>::BLOCK)) void k_asm_vector(AsmArgs A, Model m, Expr src,
                                                                                    int64_t c0, int64_t c1) {
+    static_assert(!EBV || MODE == 2, "element vectors to a buffer: the lift only");
     using C = ECfg<D, P, NQ, STATE, GRAD>;
     constexpr int mode = MODE;
     if (A.stop && *A.stop) return;
@@ -355,8 +359,24 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE, GRAD>::BLOCK)) void k_asm_ve
             if (scaled) Fe += coef * buf;
         }
     }
+    if constexpr (EBV) {
+        A.ebuf[k * NPE + a] = -1.0 * Fe;
+        return;
+    }
     const int32_t row = A.t2a[nodes[a]];
     if (row >= 0) A.F[row] += mode == 2 ? -1.0 * Fe : Fe;
+}
+
+// the lift's row pass: each listed row adds its border elements' terms in
+// border (colour) order, F = ((F + t_1) + t_2) + ..., bitwise the scatter
+__global__ void k_lift_rows(int64_t nrows, const int32_t* __restrict__ rows, const int32_t* __restrict__ off,
+                            const int32_t* __restrict__ idx, const double* __restrict__ terms, double* __restrict__ F) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nrows) return;
+    const int32_t r = rows[i];
+    double v = F[r];
+    for (int32_t j = off[i]; j < off[i + 1]; ++j) v += terms[idx[j]];
+    F[r] = v;
 }
 
 // adjoint contribution per element (no colouring: one output per element):
@@ -554,6 +574,19 @@ static void run_vector_t(rsd_problem* pb, AsmArgs A, const Expr& src) {
     hipStream_t st = pb->ctx->stream;
     const bool lift = MODE == 2;
     if (lift) A.order = pb->border.p;
+    if constexpr (MODE == 2) {
+        if (use_gather(pb) && pb->lift_nrows > 0) {  // two launches instead of one per colour
+            const int64_t nbd = pb->border.n;
+            A.ebuf = pb->lift_buf.p;
+            hipLaunchKernelGGL((k_asm_vector<D, P, NQ, STATE, GRAD, 2, true>),
+                               dim3((unsigned)((nbd + C::EPB - 1) / C::EPB)), dim3(C::BLOCK), 0, st, A, pb->model,
+                               src, (int64_t)0, nbd);
+            hipLaunchKernelGGL(k_lift_rows, dim3((unsigned)nblocks(pb->lift_nrows)), dim3(kBlock), 0, st,
+                               pb->lift_nrows, pb->lift_row.p, pb->lift_off.p, pb->lift_idx.p, pb->lift_buf.p, A.F);
+            RSD_HIP(hipGetLastError());
+            return;
+        }
+    }
     const std::vector<int64_t>& off = lift ? pb->bcolour_off : pb->colour_off;
     for (int c = 0; c < pb->n_colours(); ++c) {
         const int64_t c0 = off[c], c1 = off[c + 1];

@@ -167,6 +167,14 @@ struct rsd_problem {
     rsd::DBuf<double> ebuf;       // [n_elem * npe * npe] element matrices, reserved on first use
     rsd::DBuf<double> fbuf;       // [n_elem * npe] element mass vectors, reserved on first use
     rsd::DBuf<double> ntab;       // [n_qp * (npe + 1)] shape values N_b and weight w of each Gauss point
+    // lift by row gather: the free rows of the elements touching a
+    // constrained node, each with its (border position k * npe + a) list in
+    // border (colour-major) order
+    int64_t lift_nrows = 0;
+    rsd::DBuf<int32_t> lift_row;  // [lift_nrows]
+    rsd::DBuf<int32_t> lift_off;  // [lift_nrows + 1]
+    rsd::DBuf<int32_t> lift_idx;  // [lift_off[lift_nrows]]
+    rsd::DBuf<double> lift_buf;   // [border elements * npe] element lift vectors
     rsd::DBuf<double> gval;       // [n_nodes] Dirichlet value (0 on free nodes)
     rsd::DBuf<double> kappa;      // [n_elem] or empty
     const double* kappa_ext = nullptr;  // device kappa supplied by the caller

@@ -872,6 +872,34 @@ static void build_problem(rsd_problem* pb) {
             pb->bcolour_off[c + 1] = (int64_t)border.size();
         }
         pb->border.upload(border.data(), (int64_t)border.size(), st);
+        // the lift's row lists: entry k * npe + a of border element k goes
+        // to free row t2a(ien[e][a]); the entries of a row in border order
+        // are its contributions in colour order
+        std::vector<std::pair<int32_t, int32_t>> ent;
+        const bool fits = (int64_t)border.size() * npe < (int64_t)INT32_MAX;
+        for (size_t k = 0; fits && k < border.size(); ++k)
+            for (int a = 0; a < npe; ++a) {
+                const int32_t r = pb->h_t2a[pb->h_ien[(int64_t)border[k] * npe + a]];
+                if (r >= 0) ent.emplace_back(r, (int32_t)(k * npe + a));
+            }
+        std::stable_sort(ent.begin(), ent.end(),
+                         [](const std::pair<int32_t, int32_t>& x, const std::pair<int32_t, int32_t>& y) {
+                             return x.first < y.first;
+                         });
+        std::vector<int32_t> rows, off{0}, idx;
+        for (size_t i = 0; i < ent.size(); ++i) {
+            if (i == 0 || ent[i].first != ent[i - 1].first) {
+                if (i) off.push_back((int32_t)idx.size());
+                rows.push_back(ent[i].first);
+            }
+            idx.push_back(ent[i].second);
+        }
+        if (!rows.empty()) off.push_back((int32_t)idx.size());
+        pb->lift_nrows = (int64_t)rows.size();
+        pb->lift_row.upload(rows.data(), (int64_t)rows.size(), st);
+        pb->lift_off.upload(off.data(), (int64_t)off.size(), st);
+        pb->lift_idx.upload(idx.data(), (int64_t)idx.size(), st);
+        pb->lift_buf.alloc((int64_t)border.size() * npe);
     }
     build_gather(pb, n2e_ptr, n2e);
     build_shape_table(pb);

@@ -169,7 +169,8 @@ def test_assembly_schemes_bitwise(ctx, case):
     pass per row summing its elements' rows in colour order) and the
     colour-by-colour element scatter -- give the same bits for every form
     set and for a temperature-dependent Jacobian; a product with each
-    matrix agrees bitwise too, so the padding slots hold the same zeros.
+    matrix agrees bitwise too, so the padding slots hold the same zeros;
+    the essential lift by row gather equals the colour-by-colour lift.
     The gather runs first, then the scatter, then the gather again on a
     problem whose value arrays were written by the other scheme."""
     import residuum_hip as rh
@@ -190,6 +191,14 @@ def test_assembly_schemes_bitwise(ctx, case):
         say("form set", fs)
         for v, y in runs[1:]:
             assert np.array_equal(v, runs[0][0]) and np.array_equal(y, runs[0][1])
+    # the essential lift: by row gather (two launches) or colour by colour
+    lifts = []
+    for mode in (1, 0):
+        pb.set_assembly(mode)
+        F = ctx.from_numpy(np.random.default_rng(8).standard_normal(oc.n_free))
+        pb.apply_essential(F, FORMSETS["be"])
+        lifts.append(bits(F.numpy()))
+    assert np.array_equal(lifts[0], lifts[1])
     if dim == 2:
         tensor = (1.0, 0.2, 0.2, 0.1)
         pj = rh.Problem(ctx, dim, n, p, nq, tags, exprs, ext=ext)
