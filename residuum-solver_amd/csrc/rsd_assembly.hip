@@ -195,12 +195,16 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE>::BLOCK)) void k_asm_matrix(A
 // vectors summed the same way into mvF[r] (a missing element adds +0, a
 // no-op on a sum started at +0).  MJ bounds the elements per row (the host
 // picks MJ >= gat_m).
-template<int NPE, int MJ, bool MV>
+// AX (with the mass vector): also axo[r] = sum_j value_j x[col_j] from 0
+// in CSR order, bitwise row_dot_f64 on the values just written (AX 1: the
+// row's shared offset pattern, 2: absolute columns).
+template<int NPE, int MJ, bool MV, int AX = 0>
 __global__ __launch_bounds__(kSlice) void k_gather_rows(const int32_t* __restrict__ gel,
                                                         const uint32_t* __restrict__ gk, int32_t M,
                                                         const double* __restrict__ eb, const double* __restrict__ fb,
                                                         const int64_t* __restrict__ slice_ptr, int64_t n_rows,
-                                                        double* __restrict__ val, double* __restrict__ mvF) {
+                                                        double* __restrict__ val, double* __restrict__ mvF, Cols C,
+                                                        const double* __restrict__ axx, double* __restrict__ axo) {
     constexpr int K4 = (NPE + 3) / 4;
     extern __shared__ double acc[];  // [(W + NPE) * 64]: the row slots, then the dummy slots
     const int lane = threadIdx.x;
@@ -250,6 +254,23 @@ __global__ __launch_bounds__(kSlice) void k_gather_rows(const int32_t* __restric
     if constexpr (MV) {
         const int64_t r = s * kSlice + lane;
         if (r < n_rows) mvF[r] = fm;
+    }
+    if constexpr (AX != 0) {
+        const int64_t r = s * kSlice + lane;
+        if (r < n_rows) {
+            double sum = 0.0;
+            if constexpr (AX == 1) {
+                sum = pat_rows<double>(r, C, [&](const int32_t* po, int len) {
+                    double t = 0.0;
+                    for (int j = 0; j < len; ++j) t += acc[j * kSlice + lane] * axx[(int)r + po[j]];
+                    return t;
+                });
+            } else {
+                const int len = C.rp[r + 1] - C.rp[r];
+                for (int j = 0; j < len; ++j) sum += acc[j * kSlice + lane] * axx[C.col[base + (int64_t)j * kSlice + lane]];
+            }
+            axo[r] = sum;
+        }
     }
 }
 
@@ -497,12 +518,26 @@ static void run_shape_table(rsd_problem* pb) {
 template<int NPE, int MJ, bool MV>
 static void launch_gather_rows(rsd_problem* pb, const AsmArgs& A) {
     const Sell& S = pb->sell;
-    hipLaunchKernelGGL((k_gather_rows<NPE, MJ, MV>), dim3((unsigned)S.n_slices), dim3(kSlice),
-                       (size_t)(pb->max_row + NPE) * kSlice * sizeof(double), pb->ctx->stream, pb->gat_el.p, pb->gat_k.p,
-                       pb->gat_m, A.ebuf, A.fbuf, S.slice_ptr.p, S.n_rows, A.val, A.mvF);
+    const dim3 grid((unsigned)S.n_slices), block(kSlice);
+    const size_t lds = (size_t)(pb->max_row + NPE) * kSlice * sizeof(double);
+    hipStream_t st = pb->ctx->stream;
+    const Cols C = S.cols();
+    if (MV && A.axo) {
+        if (S.has_pat)
+            hipLaunchKernelGGL((k_gather_rows<NPE, MJ, true, 1>), grid, block, lds, st, pb->gat_el.p, pb->gat_k.p,
+                               pb->gat_m, A.ebuf, A.fbuf, S.slice_ptr.p, S.n_rows, A.val, A.mvF, C, A.axx, A.axo);
+        else
+            hipLaunchKernelGGL((k_gather_rows<NPE, MJ, true, 2>), grid, block, lds, st, pb->gat_el.p, pb->gat_k.p,
+                               pb->gat_m, A.ebuf, A.fbuf, S.slice_ptr.p, S.n_rows, A.val, A.mvF, C, A.axx, A.axo);
+        return;
+    }
+    hipLaunchKernelGGL((k_gather_rows<NPE, MJ, MV>), grid, block, lds, st, pb->gat_el.p, pb->gat_k.p, pb->gat_m,
+                       A.ebuf, A.fbuf, S.slice_ptr.p, S.n_rows, A.val, A.mvF, C, A.axx, A.axo);
 }
 
 static bool use_gather(const rsd_problem* pb) { return pb->gather_on && pb->gat_m > 0; }
+
+bool gather_forms_product(const rsd_problem* pb) { return use_gather(pb); }
 
 void gather_reserve(rsd_problem* pb, bool with_mass) {
     if (!use_gather(pb)) return;
@@ -734,9 +769,10 @@ bool matrix_mass_fusable(const rsd_problem* pb, int32_t nf, const int32_t* kinds
 
 void assemble_matrix_mass(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
                           const double* U, const double* Udot, rsd_matrix* K, const double* Um, double time_m,
-                          double* Fm) {
+                          double* Fm, const double* axx, double* axo) {
     RSD_CHECK(K->pb == pb, "matrix belongs to another problem");
     RSD_CHECK(matrix_mass_fusable(pb, nf, kinds), "matrix and mass vector need Dirichlet values at two times");
+    RSD_CHECK(!axo || (axx && gather_forms_product(pb)), "the product with the matrix needs the row gather pass");
     AsmArgs A = base_args(pb);
     set_forms(A, nf, kinds, coefs);
     A.U = U;
@@ -746,6 +782,8 @@ void assemble_matrix_mass(rsd_problem* pb, double time, int32_t nf, const int32_
     A.need_state = forms_need_state(pb, A);
     A.mvU = Um;
     A.mvF = Fm;
+    A.axx = axx;
+    A.axo = axo;
     // the state gather (if any) and the mass gather read the same values
     update_gval(pb, A.need_state ? time : time_m);
     if (!use_gather(pb)) {  // the row pass writes every slot and every entry of Fm

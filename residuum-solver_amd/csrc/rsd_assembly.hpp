@@ -31,6 +31,8 @@ struct AsmArgs {
     double* ebuf;         // matrix kernels writing element matrices (row gather assembly)
     double* fbuf;         // and, with mvU, the element mass vectors
     const double* ntab;   // Gauss point table: N_b (b < npe) and w per point, npe + 1 doubles a point
+    const double* axx;    // row gather with the mass vector, nullable: also axo = A axx, each row's
+    double* axo;          // product bitwise the FP64 row product (sequential, CSR order)
 };
 
 AsmArgs base_args(rsd_problem* pb);
@@ -50,7 +52,9 @@ void assemble_diagonal(rsd_problem* pb, double time, int32_t nf, const int32_t* 
 // uses those at `time`, so the two times must give the same values.
 void assemble_matrix_mass(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,
                           const double* U, const double* Udot, rsd_matrix* K, const double* Um, double time_m,
-                          double* Fm);
+                          double* Fm, const double* axx = nullptr, double* axo = nullptr);
+// whether assemble_matrix_mass can also form axo = K axx (the row gather pass)
+bool gather_forms_product(const rsd_problem* pb);
 // whether assemble_matrix_mass applies: one set of Dirichlet values serves
 // both gathers
 bool matrix_mass_fusable(const rsd_problem* pb, int32_t nf, const int32_t* kinds);

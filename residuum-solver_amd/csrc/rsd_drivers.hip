@@ -25,8 +25,9 @@ struct BEBuffers {
 };
 
 // BackwardEulerStage::assemble (BackwardEulerStage.hpp:221-246)
+// axo (nullable): the row gather pass also forms axo = (K + M/dt) U
 void be_assemble(rsd_problem* pb, rsd_matrix* K, const BEBuffers& b, double* U, double ct, double dt,
-                 const Expr* src) {
+                 const Expr* src, double* axo = nullptr) {
     rsd_ctx* ctx = pb->ctx;
     const int64_t n = pb->n_free;
     const int32_t kinds[2] = {RSD_FORM_DIFFUSION, RSD_FORM_MASS | RSD_FORM_SCALED};
@@ -37,9 +38,10 @@ void be_assemble(rsd_problem* pb, rsd_matrix* K, const BEBuffers& b, double* U, 
     launch_axpby(ctx, n, -1.0 / dt, 1.0 / dt, b.Uprev, b.Udot);
     if (matrix_mass_fusable(pb, 2, kinds)) {
         // K + M/dt and M U_prev from one element pass
-        assemble_matrix_mass(pb, ct, 2, kinds, coefs, U, b.Udot, K, b.Uprev, ct - dt, b.ms);
+        assemble_matrix_mass(pb, ct, 2, kinds, coefs, U, b.Udot, K, b.Uprev, ct - dt, b.ms, axo ? U : nullptr, axo);
         assemble_load(pb, ct, src, nullptr, b.F);
     } else {
+        RSD_CHECK(!axo, "be_assemble: the product with the matrix comes only from the fused pass");
         assemble_matrix(pb, ct, 2, kinds, coefs, U, b.Udot, K);
         assemble_load(pb, ct, src, nullptr, b.F);
         assemble_vector(pb, ct - dt, 1, kmass, one, b.Uprev, nullptr, nullptr, RSD_SEL_VALUE, b.ms);
@@ -150,10 +152,13 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
         for (int i = 0; i < pb->dim; ++i)
             for (int j = 0; j < i; ++j) sym_model = sym_model && pb->model.tensor[i * pb->dim + j] == pb->model.tensor[j * pb->dim + i];
     MixedWork mw;
+    double* ax0 = nullptr;  // (K + M/dt) U from the assembly's row pass: the solve's first product
     if (mixed) {
         build_half(pb);
         mw = mixed_workspace(ctx, n, &pb->sell);
         alloc_f32(ctx, &K);
+        const int32_t kinds[2] = {RSD_FORM_DIFFUSION, RSD_FORM_MASS | RSD_FORM_SCALED};
+        if (gather_forms_product(pb) && matrix_mass_fusable(pb, 2, kinds)) ax0 = (double*)ctx->ws("be.Ax0", vb);
     }
     update_gval(pb, t0);
     launch_copy(ctx, n, U, b.Uprev);
@@ -172,7 +177,7 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
             ev.rec(ev.a, st);
             if (graphable) {
                 if (!step_graph.e || graph_dt != dt) {
-                    capture(st, step_graph, [&] { be_assemble(pb, &K, b, U, ct, dt, src); });
+                    capture(st, step_graph, [&] { be_assemble(pb, &K, b, U, ct, dt, src, ax0); });
                     graph_dt = dt;
                 }
                 if (!chunk_graph.e)
@@ -181,7 +186,7 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
                     });
                 RSD_HIP(hipGraphLaunch(step_graph.e, st));
             } else {
-                be_assemble(pb, &K, b, U, ct, dt, src);
+                be_assemble(pb, &K, b, U, ct, dt, src, ax0);
             }
             ev.rec(ev.b, st);
             GraphChunk gc;
@@ -191,7 +196,7 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
             // symmetric to rounding: the FP32 conversion skips the pair check
             matrix_to_f32(ctx, &K, !sym_model);
             const MixedState ms = mixed_run(ctx, &K, b.F, U, tol, max_iters, precond, kMixedInnerFloor, true, mw,
-                                            graphable ? &gc : nullptr, mixed_hint);
+                                            graphable ? &gc : nullptr, mixed_hint, ax0);
             ev.rec(ev.c, st);
             ev.accumulate();
             if (iters) iters[step] = ms.total_inner;

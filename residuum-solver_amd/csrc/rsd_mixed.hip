@@ -101,6 +101,23 @@ __global__ __launch_bounds__(kBlock) void k_mx_residual(int64_t n, int64_t nb, C
     block_partials<1>(v, lb, nb, partials);
 }
 
+// the first residual from a given A x: b + -1.0 (A x), the row product's
+// rounding sequence with the product already formed
+__global__ __launch_bounds__(kBlock) void k_mx_residual_ax(int64_t n, int64_t nb, const double* __restrict__ b,
+                                                           const double* __restrict__ ax, double* __restrict__ r64,
+                                                           double* partials, const MixedState* st) {
+    if (st->done) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t i = lb * kBlock + threadIdx.x;
+    double v[1] = {0.0};
+    if (i < n) {
+        const double ri = b[i] + -1.0 * ax[i];
+        r64[i] = ri;
+        v[0] = ri * ri;
+    }
+    block_partials<1>(v, lb, nb, partials);
+}
+
 // the outer loop's step, on the device: ||r||, then (in the host solver's
 // double arithmetic) convergence, the caps, or the next inner solve's
 // tolerance max(floor, 0.5 tol ||r_0|| / ||r||) capped at 0.5 and its cap
@@ -369,7 +386,7 @@ constexpr int64_t kInnerChunk = 8;
 // unless fresh_f32 is already true).  Returns the final state.
 MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
                      int32_t precond, double inner_floor, bool fresh_f32, const MixedWork& w, ChunkRunner* runner,
-                     int64_t* hint) {
+                     int64_t* hint, const double* ax0) {
     const Sell& S = A->pb->sell;
     const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n)), nbt = mixed_blocks(n);
     hipStream_t st = ctx->stream;
@@ -407,8 +424,12 @@ MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
         for (int cycle = 0;; ++cycle) {
             // the outer decision ends the loop by cycle kMixedMaxCycles; hint has kMixedMaxCycles + 1 entries
             RSD_CHECK(cycle <= kMixedMaxCycles, "mixed CG: outer loop did not end");
-            RSD_LAUNCH_COLS(S.mode(), k_mx_residual, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(), A->val.p, b,
-                            x, w.r64, part, w.st);
+            if (cycle == 0 && ax0)
+                hipLaunchKernelGGL(k_mx_residual_ax, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, b, ax0, w.r64,
+                                   part, w.st);
+            else
+                RSD_LAUNCH_COLS(S.mode(), k_mx_residual, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(),
+                                A->val.p, b, x, w.r64, part, w.st);
             hipLaunchKernelGGL(k_mx_outer, dim3(1), dim3(kFold), 0, st, nb, part,
                                S.has_half && !w.full ? A->asym.p : nullptr, w.st, tol, max_iters, inner_floor);
             hipLaunchKernelGGL(k_mx_init, dim3((unsigned)nbt), dim3(kBlock), 0, st, n, nbt, w.r64, w.u, w.p, w.s,

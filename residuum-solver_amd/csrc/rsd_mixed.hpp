@@ -62,8 +62,11 @@ constexpr int kMixedMaxCycles = 40;
 constexpr double kMixedDefaultInnerFloor = 1e-4;
 // hint: optional [kMixedMaxCycles] inner iteration counts of a previous solve,
 // used to queue chunks before the first look at the stop flag (updated)
+// ax0 (optional): A x for the x passed in, formed elsewhere bitwise as the
+// FP64 row product (the assembly's row pass); the first residual then needs
+// no product
 MixedState mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
                      int32_t precond, double inner_floor, bool fresh_f32, const MixedWork& w, ChunkRunner* runner,
-                     int64_t* hint = nullptr);
+                     int64_t* hint = nullptr, const double* ax0 = nullptr);
 
 }  // namespace rsd
