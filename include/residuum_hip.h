@@ -161,15 +161,16 @@ int rsd_problem_layout(rsd_problem* pb, int64_t* out);
  * upper entries per row (half storage) or planes (uniform stencil; 0
  * otherwise), FP32 slots */
 int rsd_problem_layout_f32(rsd_problem* pb, int64_t* out);
-/* Matrix assembly scheme (no reference counterpart; both give bitwise the
- * same matrices).  mode 1 (the default): row gather -- one pass writes every
- * element matrix to an element buffer (n_elem * npe^2 doubles, reserved on
- * first use), a second pass has each row sum its elements' rows in colour
- * order and write its values once; used when the problem's gather tables
- * exist (elements around a node at most 16, rows at most 64 entries).  mode
- * 0: the element scatter, colour by colour, into a zeroed value array (frees
- * the element buffers).  *in_use (nullable) receives 1 when the row gather
- * is in use. */
+/* Assembly scheme (no reference counterpart; both give bitwise the same
+ * matrices and lifted vectors).  mode 1 (the default): row gather -- one
+ * pass writes every element matrix to an element buffer (n_elem * npe^2
+ * doubles, reserved on first use), a second pass has each row sum its
+ * elements' rows in colour order and write its values once; the essential
+ * lift likewise (the border elements' terms, then each affected row adds
+ * them in colour order); used when the problem's gather tables exist
+ * (elements around a node at most 16, rows at most 64 entries).  mode 0:
+ * the element scatter, colour by colour (frees the element buffers).
+ * *in_use (nullable) receives 1 when the row gather is in use. */
 int rsd_problem_set_assembly(rsd_problem* pb, int32_t mode, int32_t* in_use);
 /* padding the uniform stencil kernel needs around its x vector (no reference
  * counterpart): rsd_spmv_f32_padded reads x[-lo .. n + hi - 1], which must
