@@ -281,10 +281,17 @@ __global__ __launch_bounds__(kSlice) void k_gather_rows(const int32_t* __restric
 // EBV (the lift by row gather, mode 2): over every border element at once,
 // thread a writes its scatter term -1.0 * Fe to A.ebuf[k * NPE + a] (k the
 // border position) instead of adding it to F.
-template<int D, int P, int NQ, bool STATE, bool GRAD, int MODE, bool EBV = false>
+// FPAIR (mode 0, state-free isotropic model, the Backward Euler form pair):
+// straight-line per point -- Fe += ((sum_i da_i (k db_i)) Ue_b) meas w over
+// b, then buf = sum_b (((rho cp N_a) N_b) Ue_b) meas w and Fe += coef buf --
+// the generic loop's chains, the integrand's sum started from its first
+// product (a sign of zero at most, which Fe, a sum started at +0, absorbs),
+// shape values from the Gauss point table.
+template<int D, int P, int NQ, bool STATE, bool GRAD, int MODE, bool EBV = false, bool FPAIR = false>
 __global__ __launch_bounds__((ECfg<D, P, NQ, STATE, GRAD>::BLOCK)) void k_asm_vector(AsmArgs A, Model m, Expr src,
                                                                                    int64_t c0, int64_t c1) {
     static_assert(!EBV || MODE == 2, "element vectors to a buffer: the lift only");
+    static_assert(!FPAIR || (MODE == 0 && !STATE && GRAD), "the form pair path: operand / state products");
     using C = ECfg<D, P, NQ, STATE, GRAD>;
     constexpr int mode = MODE;
     if (A.stop && *A.stop) return;
@@ -337,6 +344,32 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE, GRAD>::BLOCK)) void k_asm_ve
     double kel = m.k_scalar;
     if (has_kap) kel = kap * kel;
     double Fe = 0.0;
+    if constexpr (FPAIR) {
+        const double coef = A.coefs[1];
+#pragma unroll 1
+        for (int q = 0; q < C::NQP; ++q) {
+            const double* r = rec + q * C::REC;
+            const double* tq = A.ntab + q * (NPE + 1);  // uniform: scalar loads
+            const double w = tq[NPE];
+            const double meas = r[C::R_MEAS];
+            double da[D];
+#pragma unroll
+            for (int i = 0; i < D; ++i) da[i] = r[C::R_DN + a * D + i];
+#pragma unroll
+            for (int b = 0; b < NPE; ++b) {
+                const double* db = r + C::R_DN + b * D;
+                double it = da[0] * (kel * db[0]);
+#pragma unroll
+                for (int i = 1; i < D; ++i) it += da[i] * (kel * db[i]);
+                Fe += it * Ue[b] * meas * w;
+            }
+            const double ra = m.rho * m.cp * tq[a];
+            double buf = 0.0;
+#pragma unroll
+            for (int b = 0; b < NPE; ++b) buf += ra * tq[b] * Ue[b] * meas * w;
+            Fe += coef * buf;
+        }
+    } else
     for (int q = 0; q < C::NQP; ++q) {
         const double* r = rec + q * C::REC;
         double xi[3];
@@ -623,11 +656,20 @@ static void run_vector_t(rsd_problem* pb, AsmArgs A, const Expr& src) {
         }
     }
     const std::vector<int64_t>& off = lift ? pb->bcolour_off : pb->colour_off;
+    bool pair = false;
+    if constexpr (MODE == 0 && !STATE && GRAD) pair = form_pair(pb, A);
     for (int c = 0; c < pb->n_colours(); ++c) {
         const int64_t c0 = off[c], c1 = off[c + 1];
         if (c1 <= c0) continue;
-        hipLaunchKernelGGL((k_asm_vector<D, P, NQ, STATE, GRAD, MODE>),
-                           dim3((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB)), dim3(C::BLOCK), 0, st, A, pb->model,
+        const dim3 grid((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB));
+        if constexpr (MODE == 0 && !STATE && GRAD) {
+            if (pair) {
+                hipLaunchKernelGGL((k_asm_vector<D, P, NQ, false, true, 0, false, true>), grid, dim3(C::BLOCK), 0, st, A,
+                                   pb->model, src, c0, c1);
+                continue;
+            }
+        }
+        hipLaunchKernelGGL((k_asm_vector<D, P, NQ, STATE, GRAD, MODE>), grid, dim3(C::BLOCK), 0, st, A, pb->model,
                            src, c0, c1);
     }
     RSD_HIP(hipGetLastError());
