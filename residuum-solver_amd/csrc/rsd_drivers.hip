@@ -236,10 +236,15 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
             else for (int64_t k = 0; k < kChunk; ++k) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);
             launched += kChunk;
         };
-        // queue the chunks the previous step needed (minus a margin) before
-        // the first look at the stop flag: one host round trip per step
-        // instead of one per chunk when the iteration count is steady
-        for (int64_t ahead = (last_iters - 2) / kChunk - 1; ahead > 0 && launched < max_iters; --ahead) launch_chunk();
+        // queue the previous step's iteration count and one more (whole
+        // chunks, then single iterations) before the first look at the stop
+        // flag: one host round trip per step when the count is steady, and
+        // at most a few launches left over after the stop
+        if (last_iters > 0) {
+            const int64_t want = std::min(last_iters + 1, max_iters);
+            while (launched + kChunk <= want) launch_chunk();
+            for (; launched < want; ++launched) cg_launch_iter(ctx, &K, U, tol, max_iters, precond, w);
+        }
         for (;;) {
             RSD_HIP(hipMemcpyAsync(pin.p, w.st, sizeof(CGState), hipMemcpyDeviceToHost, st));
             RSD_HIP(hipStreamSynchronize(st));
