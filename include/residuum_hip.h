@@ -76,6 +76,10 @@ int rsd_ctx_sync(rsd_ctx* ctx);
 int rsd_ctx_stream(rsd_ctx* ctx, void** stream);
 const char* rsd_last_error_string(void);
 const char* rsd_version(void);
+/* build stamp (no reference counterpart): the first 16 hex digits of the
+ * SHA-256 of this header as the library was compiled against it; a binding
+ * compares it with the header it reads, so a stale library is refused */
+const char* rsd_abi_hash(void);
 
 /* ---------------------------------------------------------------------- */
 /* Backend-tag primitives: CPU.hpp:15-40 alloc/copy/zero/set               */

@@ -254,6 +254,8 @@ double orc_adjoint_grad(const orc_problem* P, double t0, double dt, int64_t n_st
 /* Philox4x32-10 (Salmon et al., SC'11) -> N(0,1) via Box-Muller.
  * out[i] = normal sample for counter (i, stream) with key seed. */
 void orc_normal_field(uint64_t seed, uint32_t stream, int64_t n, double* out);
+/* the bare Philox4x32-10 block: out = philox(ctr, key), ten rounds */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void orc_uniform_field(uint64_t seed, uint32_t stream, int64_t n, double* out);
 
 #ifdef __cplusplus

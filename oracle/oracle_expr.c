@@ -190,6 +190,12 @@ static void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
     }
 }
 
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    philox(c, key[0], key[1]);
+    for (int i = 0; i < 4; ++i) out[i] = c[i];
+}
+
 static double u53(uint32_t a, uint32_t b) {
     uint64_t v = (((uint64_t)a << 32) | b) >> 11;
     return ((double)v + 0.5) * (1.0 / 9007199254740992.0);

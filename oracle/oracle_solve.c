@@ -458,12 +458,13 @@ double orc_adjoint_grad(const orc_problem* P, double t0, double dt, int64_t n_st
     for (int64_t k = 0; k < n_obs; ++k) {
         const double d = UN[obs[k]] - t_obs[k];
         loss += d * d;
-        rhs[obs[k]] = d;
+        rhs[obs[k]] += d;  /* a repeated index contributes once per observation, as in the loss */
     }
     loss = 0.5 * loss;
+    /* the element diffusion matrix at unit per-element factor: kappa_e is
+     * taken out, the model's k_scalar and tensor stay */
     orc_problem Ph = *P;
     Ph.kappa_e = NULL;
-    Ph.k_scalar = 1.0;
     double Ke[64 * 64], Te[64], Le[64];
     for (int64_t e = 0; e < P->n_elem; ++e) grad[e] = 0.0;
     for (int64_t k = n_steps; k >= 1; --k) {

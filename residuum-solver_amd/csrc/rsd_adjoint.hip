@@ -7,9 +7,11 @@
 // Loss:     L = 1/2 sum_obs (u^N_i - t_i)^2.
 // Backward: A lam_N = dL/du^N,  A lam_k = (1/dt) M lam_{k+1}  (A, M symmetric),
 //           dL/dtheta_e = -sum_{k=N..1} kappa_e lam_e^k . (Khat_e T_e^k),
-//           kappa_e = exp(theta_e), Khat_e the element diffusion matrix at
-//           unit conductivity, T_e^k the element temperatures with their
-//           Dirichlet values.
+//           kappa_e = exp(theta_e), Khat_e the element diffusion matrix
+//           with kappa_e taken out (the model's k_scalar and tensor kept),
+//           T_e^k the element temperatures with their Dirichlet values.
+// The conductivity must not depend on T: the run is linear.  A repeated
+// observation index counts once per observation in both L and dL/du^N.
 #include "rsd_assembly.hpp"
 #include "rsd_linalg.hpp"
 
@@ -76,7 +78,7 @@ double adjoint_run(rsd_problem* pb, const double* d_kappa, double t0, double dt,
         RSD_CHECK(obs[k] >= 0 && obs[k] < n, "observation index outside the free DOFs");
         const double d = hU[obs[k]] - t_obs[k];
         loss += d * d;
-        hr[obs[k]] = d;
+        hr[obs[k]] += d;  // a repeated index contributes once per observation, as in the loss
     }
     loss = 0.5 * loss;
     if (!want_grad) {
@@ -114,6 +116,8 @@ extern "C" int rsd_adjoint_grad_logk(rsd_ctx* ctx, rsd_problem* pb, const double
         RSD_CHECK(dt > 0.0 && n_steps >= 1, "need dt > 0 and n_steps >= 1");
         RSD_CHECK(n_obs >= 0 && (n_obs == 0 || (obs && t_obs)), "bad observations");
         RSD_CHECK(!want_grad || d_grad, "gradient output missing");
+        RSD_CHECK(!pb->model.has_k && !pb->model.has_dk,
+                  "rsd_adjoint_grad_logk: the conductivity must not depend on T (the gradient is for a linear run)");
         *loss = adjoint_run(pb, d_kappa, t0, dt, n_steps, cg_tol, cg_max_iters, d_U0, n_obs, obs, t_obs, d_grad, d_UN,
                             want_grad != 0);
     });

@@ -893,8 +893,10 @@ void adjoint_grad_step(rsd_problem* pb, double time, const double* U, const doub
     A.operand = lam;
     A.time = time;
     update_gval(pb, time);
+    // the element diffusion matrix with the per-element factor taken out:
+    // kappa_e * Khat_e is the forward's element matrix, so Khat_e keeps the
+    // model's k_scalar and tensor
     Model unit = pb->model;
-    unit.k_scalar = 1.0;
     unit.has_k = 0;
     unit.has_dk = 0;
     if (pb->dim == 2 && pb->p == 1 && pb->nq == 2) { run_adj<2, 1, 2>(pb, A, unit, kappa, grad); return; }

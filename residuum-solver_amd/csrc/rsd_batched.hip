@@ -830,6 +830,13 @@ struct TuneEntry {
 // The cache file holds one entry per line:
 //   "key": {"chunk": C, "unroll": U, "order": O, "xcd": X, "us_per_iter": T},
 // between a line "{" and a line "}" (a JSON object).
+// the ranges rsd_batch_set_schedule accepts
+static bool valid_schedule(const BSched& s) {
+    return s.chunk >= 1 && s.chunk <= 256 && (s.unroll == 4 || s.unroll == 9) && (s.order == 0 || s.order == 1) &&
+           (s.xcd == 0 || s.xcd == 1);
+}
+
+// entries outside the accepted ranges are dropped, so their keys are tuned again
 static std::map<std::string, TuneEntry> read_cache(const std::string& path) {
     std::map<std::string, TuneEntry> m;
     std::ifstream f(path);
@@ -840,7 +847,8 @@ static std::map<std::string, TuneEntry> read_cache(const std::string& path) {
         if (br == std::string::npos) continue;
         TuneEntry e;
         if (std::sscanf(line.c_str() + br, "{\"chunk\": %d, \"unroll\": %d, \"order\": %d, \"xcd\": %d, \"us_per_iter\": %lf",
-                        &e.s.chunk, &e.s.unroll, &e.s.order, &e.s.xcd, &e.us) == 5)
+                        &e.s.chunk, &e.s.unroll, &e.s.order, &e.s.xcd, &e.us) == 5 &&
+            valid_schedule(e.s))
             m[line.substr(q0 + 1, q1 - q0 - 1)] = e;
     }
     return m;

@@ -60,7 +60,8 @@ int64_t mixed_partials(int64_t n);
 void mixed_launch_inner_iter(rsd_ctx* ctx, const rsd_matrix* A, int32_t precond, const MixedWork& w);
 constexpr int kMixedMaxCycles = 40;
 constexpr double kMixedDefaultInnerFloor = 1e-4;
-// hint: optional [kMixedMaxCycles] inner iteration counts of a previous solve,
+// hint: optional [kMixedMaxCycles + 1] inner iteration counts of a previous solve
+// (entries 0 .. kMixedMaxCycles are read and written),
 // used to queue chunks before the first look at the stop flag (updated)
 // ax0 (optional): A x for the x passed in, formed elsewhere bitwise as the
 // FP64 row product (the assembly's row pass); the first residual then needs

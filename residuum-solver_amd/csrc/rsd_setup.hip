@@ -574,6 +574,11 @@ extern "C" {
 const char* rsd_last_error_string(void) { return g_last_error.c_str(); }
 const char* rsd_version(void) { return "residuum-hip 0.1 (gfx950)"; }
 
+#ifndef RSD_ABI_HASH
+#error "RSD_ABI_HASH (the SHA-256 prefix of include/residuum_hip.h) is set by the Makefile"
+#endif
+const char* rsd_abi_hash(void) { return RSD_ABI_HASH; }
+
 int rsd_ctx_create(int device, rsd_ctx** out) {
     return guard([&] {
         RSD_CHECK(out, "rsd_ctx_create: null out");

@@ -12,6 +12,8 @@ import weakref
 
 import numpy as np
 
+from . import _abi
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libresiduum_hip.so")
 
@@ -52,8 +54,10 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RsdError(f"{LIB_PATH} not built: run __graft_entry__.build() or make -C residuum-solver_amd")
         L = C.CDLL(LIB_PATH)
-        L.rsd_last_error_string.restype = C.c_char_p
-        L.rsd_version.restype = C.c_char_p
+        try:
+            _abi.bind(L)
+        except RuntimeError as e:
+            raise RsdError(f"{LIB_PATH}: {e}") from None
         _lib = L
     return _lib
 

@@ -864,6 +864,43 @@ def test_adjoint_bitexact_and_fd(ctx):
         assert abs(fd - g @ d) <= 1e-6 * abs(fd)
 
 
+def test_adjoint_scaled_conductivity_repeated_observation(ctx):
+    """k = 2 (the forward's element matrices are kappa_e * 2 * Khat) and an
+    observation index listed twice: bitwise the oracle's adjoint, whose
+    gradient is pinned by central differences on the CPU, and central
+    differences of the device loss here."""
+    import residuum_hip as rh
+    dim, n, p, nq, tags, exprs = 3, (4, 5, 3), (1, 1, 1), 2, [4], ["1"]
+    pb = rh.Problem(ctx, dim, n, p, nq, tags, exprs)
+    oc = O.Case(dim, n, p, nq, tags, exprs, order=O.ORDER_GPU)
+    theta = 0.4 * O.normal_field(11, 0, oc.n_elem)
+    pb.set_conductivity(2.0)
+    oc.k_scalar = 2.0
+    oc.kappa = np.exp(theta)
+    oc.set_order(O.ORDER_GPU)
+    obs = np.concatenate([np.arange(1, oc.n_free, 5), [3, 3]]).astype(np.int64)
+    tobs = np.linspace(0.05, 0.4, obs.size)
+    dk, dU0, dg = ctx.from_numpy(np.exp(theta)), pb.vector(0.0), ctx.empty(oc.n_elem)
+    loss = pb.adjoint(dk, dU0, 0.0, 5e-3, 3, obs, tobs, tol=1e-13, d_grad=dg)
+    lo, go, _ = oc.adjoint(0.0, 5e-3, 3, obs, tobs, cg_tol=1e-13)
+    assert loss == lo
+    assert same(dg.numpy(), go)
+    g = dg.numpy()
+    rng = np.random.default_rng(5)
+    for _ in range(2):
+        d = rng.standard_normal(oc.n_elem)
+        lp = pb.adjoint(ctx.from_numpy(np.exp(theta + 1e-6 * d)), dU0, 0.0, 5e-3, 3, obs, tobs, tol=1e-13,
+                        want_grad=False)
+        lm = pb.adjoint(ctx.from_numpy(np.exp(theta - 1e-6 * d)), dU0, 0.0, 5e-3, 3, obs, tobs, tol=1e-13,
+                        want_grad=False)
+        fd = (lp - lm) / 2e-6
+        assert abs(fd - g @ d) <= 1e-6 * abs(fd)
+    # a temperature-dependent conductivity is refused (the gradient is for a linear run)
+    pb.set_conductivity(1.0, k_expr="1+0.5*T", dk_expr="0.5")
+    with pytest.raises(rh.RsdError):
+        pb.adjoint(dk, dU0, 0.0, 5e-3, 1, obs, tobs, tol=1e-13, d_grad=dg)
+
+
 def test_adjoint_torch_autograd(ctx):
     torch = pytest.importorskip("torch")
     import residuum_hip as rh
@@ -1043,6 +1080,13 @@ def test_batched_autotune_cache(ctx, tmp_path):
     assert same(dx1.numpy(), dx2.numpy())
     with pytest.raises(rh.RsdError):
         batch.schedule = (8, 5, 0, 1)
+    # a cache entry outside the accepted ranges is dropped and tuned again
+    entry.update(order=7, chunk=0)
+    json.dump({key: entry}, open(cache, "w"))
+    batch3 = rh.Batch(ctx, mats)
+    sched3, us3, hit3 = batch3.autotune(cache=cache, iters=16)
+    assert not hit3 and batch3.schedule == sched3 and sched3[2] in (0, 1) and 1 <= sched3[0] <= 256
+    assert json.load(open(cache))[key]["order"] in (0, 1)
 
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")

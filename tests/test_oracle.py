@@ -249,6 +249,32 @@ def test_adjoint_matches_finite_differences():
         assert abs(fd - grad @ d) <= 1e-6 * max(abs(fd), 1e-12)
 
 
+def test_adjoint_scaled_conductivity_and_repeated_observation():
+    """The adjoint with a conductivity value k = 2 (the element matrices carry
+    kappa_e * k) and an observation index listed twice (the loss counts it
+    twice): central differences of the loss itself pin both."""
+    c = O.Case(3, (3, 4, 3), (1, 1, 1), 2, [4], ["1"])
+    c.k_scalar = 2.0
+    theta = 0.4 * O.normal_field(11, 0, c.n_elem)
+    obs = np.concatenate([np.arange(1, c.n_free, 5), [3, 3]])
+    tobs = np.linspace(0.05, 0.4, obs.size)
+    c.kappa = np.exp(theta)
+    c.set_order(O.ORDER_REF)
+    loss, grad, UN = c.adjoint(0.0, 5e-3, 3, obs, tobs)
+    assert np.isclose(loss, 0.5 * np.sum((UN[obs] - tobs) ** 2), rtol=1e-14)
+    rng = np.random.default_rng(3)
+    h = 1e-6
+    for _ in range(3):
+        d = rng.standard_normal(c.n_elem)
+        ls = []
+        for s_ in (1, -1):
+            c.kappa = np.exp(theta + s_ * h * d)
+            c.set_order(O.ORDER_REF)
+            ls.append(c.adjoint(0.0, 5e-3, 3, obs, tobs)[0])
+        fd = (ls[0] - ls[1]) / (2 * h)
+        assert abs(fd - grad @ d) <= 1e-6 * max(abs(fd), 1e-12)
+
+
 # ---------------------------------------------------------------------------
 # live comparison with the reference harness (only where it was built)
 # ---------------------------------------------------------------------------
