@@ -62,6 +62,7 @@ typedef struct rsd_ctx rsd_ctx;
 typedef struct rsd_problem rsd_problem;
 typedef struct rsd_matrix rsd_matrix;
 typedef struct rsd_batch rsd_batch;
+typedef struct rsd_expr rsd_expr;
 
 /* ---------------------------------------------------------------------- */
 /* context                                                                 */
@@ -212,6 +213,25 @@ int rsd_problem_mesh_tag(rsd_problem* pb, uint64_t* out);
  * condition given by an expression (InitialConditionConfigParser.cpp with
  * read mode "expression"), same grammar as the boundary expressions */
 int rsd_problem_eval_expr(rsd_ctx* ctx, rsd_problem* pb, const char* expr, double time, double* d_out);
+/* Expressions on the host, in the grammar the device evaluates (the
+ * Dirichlet values, loads and k(T) above): the host side of a function that
+ * also runs on the device, e.g. a model evaluated by the reference's CPU
+ * assembly.  Replaces the exprtk-backed ScalarExpression /
+ * TemperatureExpression call (src/utils/expression/ScalarExpression.cpp:3-30,
+ * equation/heateq/evaluator/TemperatureExpression.hpp:38-51); variables t x y
+ * z T.  rsd_expr_uses tells whether the expression reads t and T. */
+int rsd_expr_compile(const char* text, rsd_expr** out);
+int rsd_expr_eval(const rsd_expr* e, double t, double x, double y, double z, double T, double* out);
+int rsd_expr_uses(const rsd_expr* e, int32_t* uses_t, int32_t* uses_T);
+int rsd_expr_destroy(rsd_expr* e);
+/* Dirichlet values given by the caller instead of the problem's expressions
+ * (gatherElementVector's Full / Constrained modes evaluate the registered
+ * boundary functions at the constrained nodes, ElementMap.tpp:39-72): host
+ * values [n_nodes - n_free] at the constrained nodes in ascending node
+ * order, valid for `time` only -- an assembly at another time fails with
+ * RSD_ERR_ARG until values for it are given.  values == NULL returns the
+ * problem to its expressions. */
+int rsd_problem_set_dirichlet_values(rsd_ctx* ctx, rsd_problem* pb, double time, const double* values);
 /* topo->alg map (-1 constrained), alg->topo, constraint tag per node */
 int rsd_problem_get_dofs(rsd_problem* pb, int32_t* topo_to_alg, int32_t* alg_to_topo, int32_t* ctag);
 /* CSR pattern as the reference's size_t arrays, widened to int64 */
@@ -243,6 +263,9 @@ int rsd_matrix_zero(rsd_ctx* ctx, rsd_matrix* m);                        /* CSRM
 /* values in CSR order, host [nnz] */
 int rsd_matrix_get_values(rsd_ctx* ctx, rsd_matrix* m, double* csr_values);
 int rsd_matrix_set_values(rsd_ctx* ctx, rsd_matrix* m, const double* csr_values);
+/* the values in CSR order into device memory [nnz] (a CSRMatrix<Real,Hip>'s
+ * data(), CSRMatrix.hpp:76-78); rsd_matrix_load_csr below is the reverse */
+int rsd_matrix_store_csr(rsd_ctx* ctx, rsd_matrix* m, double* d_csr_values);
 
 /* y = A x: operations::matvec (linalg/operations/backend/cpu/MatrixOps.tpp:5-23),
  * row sums in the reference's left-to-right order (bitwise equal) */
@@ -319,6 +342,8 @@ int rsd_inv_diag_f64(rsd_ctx* ctx, int64_t n, const double* d, double* inv);
 int rsd_matrix_from_csr(rsd_ctx* ctx, int64_t n_rows, const uint64_t* row_ptr, const uint64_t* col_idx,
                         rsd_matrix** out);
 int rsd_matrix_load_csr(rsd_ctx* ctx, rsd_matrix* A, const double* val);
+/* (rsd_matrix_load_csr also loads a problem's matrix, rsd_matrix_create,
+ * from device values in the problem's CSR order) */
 
 /* Mixed precision (no reference counterpart; the reference is FP64 only).
  * rsd_matrix_to_f32 refreshes the FP32 copy of A's values; rsd_spmv_f32 is
@@ -366,6 +391,21 @@ int rsd_assemble_load(rsd_ctx* ctx, rsd_problem* pb, double time, const char* sr
 /* essential lift F -= A_fc g (BoundaryApplicator.tpp:7-64) */
 int rsd_apply_essential(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t n_forms,
                         const int32_t* kinds, const double* coefs, double* F);
+/* Natural (flux) boundary conditions: BoundaryApplicator::applyNaturalBCs
+ * (fem/boundary/backend/cpu/BoundaryApplicator.tpp:66-140) with
+ * FluxBoundaryForm (equation/heateq/form/FluxBoundaryForm.hpp:11-38) or
+ * NodalFluxForm (NodalFluxForm.hpp:12-55).  F += for every element
+ * (ascending) and face (ascending) whose tag has entries, the face vector
+ * Fe[a] = sum over the face's Gauss points (nq per direction), the tag's
+ * entries in order and sD of (q_sD N_a) n_sD w, n the unnormalised cofactor
+ * normal; added at the face's free nodes in that order, bitwise the serial
+ * loop.  Entry k (at most 8) applies to faces tagged entry_tags[k]; its flux
+ * vector q is the nodal field d_nodal_flux[k] (device [n_nodes * dim],
+ * interpolated with the face shape values) when d_nodal_flux and that
+ * pointer are non-null, else the dim expressions flux_exprs[k * dim + sD]
+ * of (t, x, y, z). */
+int rsd_apply_natural(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t n_entries, const int32_t* entry_tags,
+                      const char* const* flux_exprs, const double* const* d_nodal_flux, double* F);
 /* Assembler::assembleDiagonal (Assembler.tpp:142-197): diag [n_free] zeroed,
  * then each element's Ke[a][a] added in colour order; bitwise the diagonal
  * of rsd_assemble_matrix with the same forms.  FEMOperator::diagonal. */

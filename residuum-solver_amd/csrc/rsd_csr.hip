@@ -184,7 +184,11 @@ int rsd_matrix_from_csr(rsd_ctx* ctx, int64_t n_rows, const uint64_t* row_ptr, c
 int rsd_matrix_load_csr(rsd_ctx* ctx, rsd_matrix* A, const double* val) {
     return guard([&] {
         RSD_CHECK(ctx && A && val, "rsd_matrix_load_csr: null argument");
-        RSD_CHECK(A->pb->csr_slot.n == A->pb->nnz, "rsd_matrix_load_csr: matrix was not made from a CSR");
+        if (A->pb->csr_slot.n != A->pb->nnz) {  // a problem's matrix: slots from the sliced layout
+            RSD_CHECK(ctx == A->pb->ctx, "matrix belongs to another context");
+            launch_sell_csr(A, const_cast<double*>(val), false);
+            return;
+        }
         if (A->pb->nnz)
             hipLaunchKernelGGL(k_csr_to_sell, dim3((unsigned)nblocks(A->pb->nnz)), dim3(kBlock), 0, ctx->stream,
                                A->pb->nnz, A->pb->csr_slot.p, val, A->val.p);

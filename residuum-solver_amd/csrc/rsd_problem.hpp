@@ -125,6 +125,7 @@ struct Sell {
 
 namespace rsd {
 struct ReactionPlan;  // boundary-reaction plan, built on first use per tag list
+struct NaturalPlan;   // natural boundary condition plan, built on first use per tag list
 }
 
 struct rsd_problem {
@@ -147,6 +148,11 @@ struct rsd_problem {
     bool bc_time_dependent = false;
     double gval_time = 0.0;
     bool gval_valid = false;
+    // Dirichlet values given by the caller for one time at a time
+    // (rsd_problem_set_dirichlet_values) instead of the expressions
+    bool gval_external = false;
+    rsd::DBuf<int32_t> cnodes;    // [n_nodes - n_free] constrained nodes, ascending
+    rsd::DBuf<double> cvals;      // their values, staged
     // device arrays
     rsd::DBuf<double> xyz;        // [n_nodes * dim]
     rsd::DBuf<int32_t> ien;       // [n_elem * npe]
@@ -184,6 +190,7 @@ struct rsd_problem {
     std::string src_text;
     rsd::Expr src_expr;
     std::shared_ptr<rsd::ReactionPlan> reaction;
+    std::shared_ptr<rsd::NaturalPlan> natural;
 
     const double* kappa_ptr() const { return kappa_ext ? kappa_ext : (kappa.n ? kappa.p : nullptr); }
     int n_colours() const { return (int)colour_off.size() - 1; }
@@ -201,6 +208,9 @@ struct rsd_matrix {
 namespace rsd {
 // refresh the per-node Dirichlet values for `time` (no-op when cached)
 void update_gval(rsd_problem* pb, double time);
+// a problem matrix's values between the sliced layout and CSR order (device
+// memory both): to_csr copies SELL -> CSR, else CSR -> SELL
+void launch_sell_csr(const rsd_matrix* m, double* csr, bool to_csr);
 // sliced ELL layout of pb->h_rp / h_ci, uploaded
 void build_sell(rsd_problem* pb, const std::vector<int32_t>& rowlen);
 // the symmetric half layout of the FP32 copy, tried once per problem (host

@@ -8,6 +8,7 @@
 // multi-threaded rather than the reference's two-pass touch list.
 #include "rsd_problem.hpp"
 #include "rsd_assembly.hpp"
+#include "rsd_linalg.hpp"
 
 #include <algorithm>
 #include <cctype>
@@ -207,6 +208,35 @@ __global__ void k_gval(int64_t n_nodes, int32_t dim, const double* __restrict__ 
     g[i] = v;
 }
 
+// g[node[i]] = v[i]: caller-given Dirichlet values at the constrained nodes
+__global__ void k_scatter_gval(int64_t n, const int32_t* __restrict__ node, const double* __restrict__ v,
+                               double* __restrict__ g) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) g[node[i]] = v[i];
+}
+
+// SELL <-> CSR order for a problem's matrix, one thread per row: CSR entry
+// j of row r sits at slot slice_ptr[r / 64] + 64 j + r % 64
+__global__ void k_sell_csr(int64_t n, const int64_t* __restrict__ slice_ptr, const int32_t* __restrict__ rp,
+                           double* __restrict__ sell, double* __restrict__ csr, int32_t to_csr) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t base = slice_ptr[r / kSlice] + r % kSlice;
+    const int32_t b = rp[r], len = rp[r + 1] - b;
+    for (int32_t j = 0; j < len; ++j) {
+        if (to_csr) csr[b + j] = sell[base + (int64_t)j * kSlice];
+        else sell[base + (int64_t)j * kSlice] = csr[b + j];
+    }
+}
+
+void launch_sell_csr(const rsd_matrix* m, double* csr, bool to_csr) {
+    const rsd_problem* pb = m->pb;
+    if (pb->n_free)
+        hipLaunchKernelGGL(k_sell_csr, dim3((unsigned)nblocks(pb->n_free)), dim3(kBlock), 0, pb->ctx->stream, pb->n_free,
+                           pb->sell.slice_ptr.p, pb->sell.row_ptr.p, m->val.p, csr, to_csr ? 1 : 0);
+    RSD_HIP(hipGetLastError());
+}
+
 // f(t, x, y, z) at the node of every free DOF, into the free vector
 __global__ void k_eval_free(int64_t n_nodes, int32_t dim, const double* __restrict__ xyz,
                             const int32_t* __restrict__ t2a, Expr e, double time, double* __restrict__ out) {
@@ -223,6 +253,11 @@ __global__ void k_eval_free(int64_t n_nodes, int32_t dim, const double* __restri
 using namespace rsd;
 
 void rsd::update_gval(rsd_problem* pb, double time) {
+    if (pb->gval_external) {
+        RSD_CHECK(pb->gval_valid && pb->gval_time == time,
+                  "the Dirichlet values at this time were not given (rsd_problem_set_dirichlet_values)");
+        return;
+    }
     if (pb->gval_valid && (!pb->bc_time_dependent || pb->gval_time == time)) return;
     const int64_t nb = nblocks(pb->n_nodes);
     hipLaunchKernelGGL(k_gval, dim3((unsigned)nb), dim3(kBlock), 0, pb->ctx->stream, pb->n_nodes, pb->dim,
@@ -1017,6 +1052,72 @@ int rsd_problem_eval_expr(rsd_ctx* ctx, rsd_problem* pb, const char* expr, doubl
     });
 }
 
+struct rsd_expr {
+    rsd::Expr e;
+};
+
+int rsd_expr_compile(const char* text, rsd_expr** out) {
+    return guard([&] {
+        RSD_CHECK(text && out, "rsd_expr_compile: null argument");
+        auto h = std::make_unique<rsd_expr>();
+        h->e = compile_expr(text);
+        *out = h.release();
+    });
+}
+
+int rsd_expr_eval(const rsd_expr* e, double t, double x, double y, double z, double T, double* out) {
+    return guard([&] {
+        RSD_CHECK(e && out, "rsd_expr_eval: null argument");
+        *out = eval_expr_host(e->e, t, x, y, z, T);
+    });
+}
+
+int rsd_expr_uses(const rsd_expr* e, int32_t* uses_t, int32_t* uses_T) {
+    return guard([&] {
+        RSD_CHECK(e, "rsd_expr_uses: null argument");
+        if (uses_t) *uses_t = e->e.uses_t;
+        if (uses_T) *uses_T = e->e.uses_T;
+    });
+}
+
+int rsd_expr_destroy(rsd_expr* e) {
+    return guard([&] { delete e; });
+}
+
+int rsd_problem_set_dirichlet_values(rsd_ctx* ctx, rsd_problem* pb, double time, const double* values) {
+    return guard([&] {
+        RSD_CHECK(ctx && pb, "rsd_problem_set_dirichlet_values: null argument");
+        RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
+        if (!values) {  // back to the expressions
+            pb->gval_external = false;
+            pb->gval_valid = false;
+            return;
+        }
+        const int64_t nc = pb->n_nodes - pb->n_free;
+        if (pb->cnodes.n != nc) {
+            std::vector<int32_t> cn;
+            cn.reserve((size_t)nc);
+            for (int64_t i = 0; i < pb->n_nodes; ++i)
+                if (pb->h_t2a[i] < 0) cn.push_back((int32_t)i);
+            RSD_CHECK((int64_t)cn.size() == nc, "rsd_problem_set_dirichlet_values: one DOF per node expected");
+            pb->cnodes.upload(cn.data(), nc, ctx->stream);
+            pb->cvals.alloc(nc);
+            RSD_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        if (!pb->gval_external) launch_fill(ctx, pb->n_nodes, pb->gval.p, 0.0);
+        if (nc) {
+            RSD_HIP(hipMemcpyAsync(pb->cvals.p, values, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(k_scatter_gval, dim3((unsigned)nblocks(nc)), dim3(kBlock), 0, ctx->stream, nc, pb->cnodes.p,
+                               pb->cvals.p, pb->gval.p);
+            RSD_HIP(hipGetLastError());
+            RSD_HIP(hipStreamSynchronize(ctx->stream));  // values is the caller's host memory
+        }
+        pb->gval_external = true;
+        pb->gval_valid = true;
+        pb->gval_time = time;
+    });
+}
+
 int rsd_problem_destroy(rsd_problem* pb) {
     return guard([&] {
         if (!pb) return;
@@ -1203,6 +1304,14 @@ int rsd_matrix_get_values(rsd_ctx* ctx, rsd_matrix* m, double* out) {
         RSD_HIP(hipStreamSynchronize(ctx->stream));
         for (int64_t r = 0; r < pb->n_free; ++r)
             for (int64_t j = 0; j < pb->h_rp[r + 1] - pb->h_rp[r]; ++j) out[pb->h_rp[r] + j] = h[sell_slot(pb, r, j)];
+    });
+}
+
+int rsd_matrix_store_csr(rsd_ctx* ctx, rsd_matrix* m, double* d_csr) {
+    return guard([&] {
+        RSD_CHECK(ctx && m && d_csr, "rsd_matrix_store_csr: null argument");
+        RSD_CHECK(ctx == m->pb->ctx, "matrix belongs to another context");
+        launch_sell_csr(m, d_csr, true);
     });
 }
 
