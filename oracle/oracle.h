@@ -248,6 +248,14 @@ double orc_adjoint_grad(const orc_problem* P, double t0, double dt, int64_t n_st
                         const double* U0, int64_t n_obs, const int64_t* obs, const double* t_obs,
                         double* grad, double* U_final);
 
+/* natural (flux) boundary conditions, BoundaryApplicator.tpp:66-140 with
+ * FluxBoundaryForm / NodalFluxForm: F += each tagged face's vector; entry k
+ * applies to faces tagged entry_tags[k], its flux from nodal[k] ([n_nodes *
+ * dim]) when nodal and nodal[k] are non-NULL, else from the dim compiled
+ * expressions exprs[k * dim + sD] */
+int orc_apply_natural(const orc_problem* P, double time, const int32_t* rng, int32_t n_entries,
+                      const int32_t* entry_tags, const void* const* exprs, const double* const* nodal, double* F);
+
 /* ------------------------------------------------------------------ */
 /* seeded inputs                                                        */
 /* ------------------------------------------------------------------ */

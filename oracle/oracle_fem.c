@@ -846,3 +846,66 @@ int orc_boundary_flux(const orc_problem* P, double time, const double* U, const 
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* natural (flux) boundary conditions: BoundaryApplicator<CPU>::
+ * applyNaturalBCs (fem/boundary/backend/cpu/BoundaryApplicator.tpp:66-140)
+ * with FluxBoundaryForm (equation/heateq/form/FluxBoundaryForm.hpp:24-36)
+ * or NodalFluxForm (NodalFluxForm.hpp:28-51) at the points of
+ * EvalQuadraturePointBoundary::evaluate (EvalQuadraturePointBoundary.hpp:
+ * 73-101; Nface = the volume shape values at the face nodes).  Elements in
+ * ascending order, faces ascending; a face whose tag has entries gets
+ * Fe (zeroed), every point adds each entry of its tag in order,
+ *   Fe[a] += (q_sD Nface_a) n_sD w  (sD ascending),
+ * then Fe is scattered into F at the free face nodes (scatterElementVector,
+ * ElementMap.tpp:77-92, coefficient 1).  Entry k: nodal[k] != NULL gives
+ * nodal flux vectors [n_nodes * dim] (q = sum_b Nface_b q_b from 0), else
+ * the dim expressions exprs[k * dim + sD] of (t, x, y, z).                */
+/* ------------------------------------------------------------------ */
+int orc_apply_natural(const orc_problem* P, double time, const int32_t* rng, int32_t n_entries,
+                      const int32_t* entry_tags, const void* const* exprs, const double* const* nodal, double* F) {
+    double coords[3 * MAXNPE], Fe[MAXNPE], fxi[3 * 25], fw[25], nref[3];
+    int64_t fids[MAXNPE];
+    const int n = P->npe, d = P->dim, nf = 2 * d;
+    for (int64_t e = 0; e < P->n_elem; ++e) {
+        const int64_t* nodes = P->ien + e * n;
+        const int32_t* rt = rng + e * nf;
+        int any = 0;
+        for (int f = 0; f < nf; ++f)
+            for (int k = 0; k < n_entries; ++k) any |= rt[f] >= 0 && rt[f] == entry_tags[k];
+        if (!any) continue;
+        element_coords(P, nodes, coords);
+        for (int f = 0; f < nf; ++f) {
+            int has = 0;
+            for (int k = 0; k < n_entries; ++k) has |= rt[f] >= 0 && rt[f] == entry_tags[k];
+            if (!has) continue;
+            const int npf = face_nodes(d, P->p, f, fids);
+            for (int a = 0; a < npf; ++a) Fe[a] = 0.0;
+            const int fq = face_points(P, f, fxi, fw, nref);
+            for (int q = 0; q < fq; ++q) {
+                qp_t Q;
+                qp_evaluate(P, coords, &fxi[d * q], fw[q], &Q);
+                double nv[3];
+                boundary_normal(d, Q.J, nref, nv);
+                for (int k = 0; k < n_entries; ++k) {
+                    if (entry_tags[k] != rt[f]) continue;
+                    double val[3] = {0.0, 0.0, 0.0};
+                    if (nodal && nodal[k]) {
+                        for (int b = 0; b < npf; ++b)
+                            for (int s = 0; s < d; ++s) val[s] += Q.N[fids[b]] * nodal[k][nodes[fids[b]] * d + s];
+                    } else {
+                        for (int s = 0; s < d; ++s)
+                            val[s] = orc_expr_eval(exprs[k * d + s], time, Q.x[0], Q.x[1], d == 3 ? Q.x[2] : 0.0, 0.0);
+                    }
+                    for (int a = 0; a < npf; ++a)
+                        for (int s = 0; s < d; ++s) Fe[a] += ((val[s] * Q.N[fids[a]]) * nv[s]) * fw[q];
+                }
+            }
+            for (int a = 0; a < npf; ++a) {
+                const int32_t r = P->topo_to_alg[nodes[fids[a]]];
+                if (r >= 0) F[r] += 1.0 * Fe[a];
+            }
+        }
+    }
+    return 0;
+}

@@ -31,6 +31,10 @@
 #include "fem/assembly/Assembler.hpp"
 #include "fem/boundary/BoundaryApplicator.hpp"
 #include "fem/boundary/EssentialBoundaryRegistry.hpp"
+#include "fem/boundary/NaturalBoundaryRegistry.hpp"
+#include "equation/heateq/boundary/BoundaryFluxFunction.hpp"
+#include "equation/heateq/form/FluxBoundaryForm.hpp"
+#include "equation/heateq/form/NodalFluxForm.hpp"
 #include "fem/basis/LagrangeQuad.hpp"
 #include "fem/basis/LagrangeHex.hpp"
 #include "fem/quadrature/GaussQuadratureQuad.hpp"
@@ -145,6 +149,31 @@ struct HarnessCp {
     void evalGradient(QP& qp) const { qp.dcpdT = 0.0; }
 };
 
+// flux vector kinds shared with the tests (tests/oracle_py.py FLUX_KINDS):
+// 0 (1.5, -0.5, 0.25), 1 (x, y*y, x*z), 2 (sin(pi*x) + t, cos(y), x*y)
+template<int SD>
+struct FluxKindFn {
+    int kind = 0;
+    void operator()(Real t, const Real* x, Real* out) const {
+        const Real X = x[0], Y = x[1], Z = SD == 3 ? x[2] : 0.0;
+        Real v[3];
+        if (kind == 0) { v[0] = 1.5; v[1] = -0.5; v[2] = 0.25; }
+        else if (kind == 1) { v[0] = X; v[1] = Y * Y; v[2] = X * Z; }
+        else { v[0] = std::sin(M_PI * X) + t; v[1] = std::cos(Y); v[2] = X * Y; }
+        for (int i = 0; i < SD; ++i) out[i] = v[i];
+    }
+};
+
+// nodal flux vectors [n_nodes * SD]
+template<int SD>
+struct NodalVec {
+    static constexpr Index NumComponents = SD;
+    const Real* data = nullptr;
+    void eval(Index node, Real* out) const {
+        for (int i = 0; i < SD; ++i) out[i] = data[node * SD + i];
+    }
+};
+
 struct NodalArray {
     static constexpr Index NumComponents = 1;
     const Real* data = nullptr;
@@ -169,6 +198,8 @@ struct Base {
                                    const int32_t* tags, Real* result, Real* measure) = 0;
     virtual void boundary_flux(Real time, const Real* U, int n_tags, const int32_t* tags, Real* result,
                                Real* measure) = 0;
+    virtual void natural(Real time, int n_entries, const int32_t* tags, const int32_t* kinds,
+                         const Real* const* nodal, Real* F) = 0;
     virtual int cg_fem(int set, Real coef, const Real* U, int pc, const Real* rhs, Real* x, Real tol, int64_t maxit,
                        int64_t* iters) = 0;
     virtual int be_run(Real t0, Real tf, Real dt, Real tol, int64_t maxit, Real* U, int64_t* iters) = 0;
@@ -393,6 +424,41 @@ struct Ctx : Base {
             result[k] = reg.result(tags[k])[0];
             measure[k] = reg.entries().at(tags[k]).measureSum;
         }
+    }
+
+    // BoundaryApplicator<CPU>::applyNaturalBCs (BoundaryApplicator.tpp:66-140): a
+    // NaturalBoundaryRegistry with one FluxBoundaryForm (flux kind) or
+    // NodalFluxForm (nodal vectors) entry per (tag, kind) pair, in order
+    void natural(Real time, int n_entries, const int32_t* tags, const int32_t* kinds, const Real* const* nodal,
+                 Real* F) override {
+        using QuadBdy = std::conditional_t<D == 2, fem::quadrature::GaussQuadrature1D, fem::quadrature::GaussQuadratureQuad>;
+        using QPB = equation::heateq::evaluator::EvalQuadraturePointBoundary<EvalEle, Basis, fem::geometry::JacobianTransform<D, D>>;
+        using FluxFn = equation::heateq::BoundaryFluxFunction<D, 1, FluxKindFn<D>>;
+        using FluxForms = fem::form::FormRegistry<equation::heateq::FluxBoundaryForm<QPB, FluxFn>>;
+        using NodalForms = fem::form::FormRegistry<equation::heateq::NodalFluxForm<QPB, NodalVec<D>, 1, D>>;
+        struct DefaultB { void eval(QPB&) const {} void evalGradient(QPB&) const {} };
+        const Index nb = (Index)std::llround(std::pow((double)quad.numPointsTotal(), 1.0 / D));
+        QuadBdy qb = [&] { if constexpr (D == 2) return QuadBdy(nb); else return QuadBdy(nb, nb); }();
+        fem::boundary::NaturalBoundaryRegistry<QPB> reg;
+        DefaultB model;
+        std::vector<std::unique_ptr<FluxForms>> ff;
+        std::vector<std::unique_ptr<NodalForms>> nf;
+        for (int k = 0; k < n_entries; ++k) {
+            FluxKindFn<D> fn; fn.kind = kinds[k];
+            auto bc = std::shared_ptr<fem::boundary::BoundaryCondition<FluxFn>>(new fem::boundary::BoundaryCondition<FluxFn>{
+                tags[k], {fem::boundary::BCCategory::Natural}, FluxFn{fn}});
+            if (nodal && nodal[k]) {
+                NodalVec<D> nv; nv.data = nodal[k];
+                nf.push_back(std::make_unique<NodalForms>(equation::heateq::NodalFluxForm<QPB, NodalVec<D>, 1, D>{nv}));
+                reg.template registerBC<FluxFn>(bc, *nf.back(), model);
+            } else {
+                ff.push_back(std::make_unique<FluxForms>(equation::heateq::FluxBoundaryForm<QPB, FluxFn>{fn}));
+                reg.template registerBC<FluxFn>(bc, *ff.back(), model);
+            }
+        }
+        VecT out = vec_from(F);
+        app.template applyNaturalBCs<1, EvalEle, QPB, QuadBdy>(mesh, *topo, reg, time, ele, qb, out);
+        for (Index i = 0; i < out.size(); ++i) F[i] = out.data()[i];
     }
 
     // cg::Solver over the matrix-free FEMOperator (FEMOperator.hpp:27-71); set 0 diffusion, 2 BE
@@ -684,6 +750,13 @@ void ref_boundary_flux(void* h, double time, const double* U, int n_tags, const 
     Base* b = static_cast<Base*>(h);
     with_kappa(b, nullptr);
     b->boundary_flux(time, U, n_tags, tags, result, measure);
+}
+
+void ref_natural(void* h, double time, int n_entries, const int32_t* tags, const int32_t* kinds,
+                 const double* const* nodal, double* F) {
+    Base* b = static_cast<Base*>(h);
+    with_kappa(b, nullptr);
+    b->natural(time, n_entries, tags, kinds, nodal, F);
 }
 
 int ref_be_run(void* h, double t0, double tf, double dt, double tol, int64_t maxit, const double* kappa,

@@ -497,6 +497,26 @@ class Problem:
         nf, kinds, coefs = _forms(forms)
         check(lib().rsd_apply_essential(self.ctx.h, self.h, C.c_double(time), C.c_int32(nf), kinds, coefs, dptr(F)))
 
+    def apply_natural(self, F, entries, time=0.0):
+        """BoundaryApplicator::applyNaturalBCs: F += the flux terms of the
+        tagged faces.  entries: list of (tag, flux), flux either a sequence
+        of dim expression strings q_sD(t, x, y, z) or a device array of
+        nodal flux vectors [n_nodes * dim]."""
+        n = len(entries)
+        d = self.dim
+        tags = (C.c_int32 * max(n, 1))(*[int(t) for t, _ in entries])
+        ex = (C.c_char_p * max(n * d, 1))()
+        nod = (C.c_void_p * max(n, 1))()
+        for k, (_, fl) in enumerate(entries):
+            if isinstance(fl, (list, tuple)):
+                if len(fl) != d:
+                    raise RsdError(f"a flux needs {d} expressions")
+                for i, e in enumerate(fl):
+                    ex[k * d + i] = e.encode()
+            else:
+                nod[k] = dptr(fl).value
+        check(lib().rsd_apply_natural(self.ctx.h, self.h, C.c_double(time), C.c_int32(n), tags, ex, nod, dptr(F)))
+
     def assemble_diagonal(self, diag, forms, U=None, time=0.0):
         """Assembler::assembleDiagonal: the diagonal of the forms' matrix."""
         nf, kinds, coefs = _forms(forms)

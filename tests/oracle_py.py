@@ -135,6 +135,10 @@ def block_mesh(dim, n, ext, p):
     return xyz, ien, rng
 
 
+# flux vector kinds of the reference harness (oracle/ref_harness.cpp FluxKindFn)
+FLUX_KINDS = [("1.5", "-0.5", "0.25"), ("x", "y*y", "x*z"), ("sin(pi*x)+t", "cos(y)", "x*y")]
+
+
 @dataclass
 class Case:
     """A discretised heat problem on a block mesh, with every oracle array."""
@@ -256,6 +260,29 @@ class Case:
         lib().orc_boundary_flux(C.byref(self.P), C.c_double(time), dp(U), ip(self.rng), len(t), ip(t), dp(res),
                                 dp(meas))
         return res, meas
+
+    def apply_natural(self, entries, F, time=0.0):
+        """F += natural (flux) BC terms; entries: list of (tag, flux) with
+        flux a kind number (FLUX_KINDS expressions) or a nodal array
+        [n_nodes * dim]."""
+        d = self.dim
+        n = len(entries)
+        tags = np.array([t for t, _ in entries], np.int32)
+        keep, ex, nod = [], (C.c_void_p * max(n * d, 1))(), (C.c_void_p * max(n, 1))()
+        for k, (_, fl) in enumerate(entries):
+            if isinstance(fl, (int, np.integer)):
+                for i in range(d):
+                    e = Expr(FLUX_KINDS[fl][i])
+                    keep.append(e)
+                    ex[k * d + i] = e.ptr
+            else:
+                a = np.ascontiguousarray(fl, np.float64)
+                keep.append(a)
+                nod[k] = a.ctypes.data
+        out = np.array(F, np.float64)
+        lib().orc_apply_natural(C.byref(self.P), C.c_double(time), ip(self.rng), C.c_int32(n), ip(tags), ex, nod,
+                                dp(out))
+        return out
 
     def boundary_reaction(self, tags, U, Udot=None, src=None, time=0.0):
         """Weak boundary reaction per tag (sum, face measure), ascending
@@ -464,6 +491,20 @@ class RefCase:
         res, meas = np.zeros(len(t)), np.zeros(len(t))
         ref().ref_boundary_flux(self.h, C.c_double(time), dp(U), C.c_int(len(t)), ip(t), dp(res), dp(meas))
         return res, meas
+
+    def natural(self, entries, F, time=0.0):
+        n = len(entries)
+        tags = np.array([t for t, _ in entries], np.int32)
+        kinds = np.array([fl if isinstance(fl, (int, np.integer)) else 0 for _, fl in entries], np.int32)
+        keep, nod = [], (C.c_void_p * max(n, 1))()
+        for k, (_, fl) in enumerate(entries):
+            if not isinstance(fl, (int, np.integer)):
+                a = np.ascontiguousarray(fl, np.float64)
+                keep.append(a)
+                nod[k] = a.ctypes.data
+        out = np.array(F, np.float64)
+        ref().ref_natural(self.h, C.c_double(time), C.c_int(n), ip(tags), ip(kinds), nod, dp(out))
+        return out
 
     def boundary_reaction(self, tags, U, Udot=None, src_kind=0, time=0.0):
         t = np.array(tags, np.int32)

@@ -413,6 +413,56 @@ def test_boundary_flux_bitexact(ctx, case):
     assert same(got[1], pb.boundary_reaction(tags, ctx.from_numpy(U), time=0.25)[1])
 
 
+NATURAL_ENTRIES = {2: [(1, 0), (2, 1), (3, "nodal"), (2, "nodal"), (0, 1)],
+                   3: [(0, 1), (1, 0), (5, "nodal"), (2, 1), (3, "nodal"), (2, 0)]}
+
+
+def _device_entries(ctx, oc, spec, seed):
+    """(tag, flux) lists for the oracle and the device: kinds become the
+    reference harness's flux expressions, "nodal" a seeded nodal field."""
+    rng = np.random.default_rng(seed)
+    ho, dv = [], []
+    for tag, fl in spec:
+        if fl == "nodal":
+            a = rng.standard_normal(oc.n_nodes * oc.dim)
+            ho.append((tag, a))
+            dv.append((tag, ctx.from_numpy(a)))
+        else:
+            ho.append((tag, fl))
+            dv.append((tag, O.FLUX_KINDS[fl][:oc.dim]))
+    return ho, dv
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_natural_bc_bitexact(ctx, case):
+    """Natural (flux) boundary conditions (BoundaryApplicator.tpp:66-140,
+    FluxBoundaryForm / NodalFluxForm) on the device: F after the face terms
+    bitwise the oracle's serial loop (pinned against the reference's own
+    applyNaturalBCs), on constrained and free faces, several entries on one
+    tag, at two times; the sin/cos flux within 1e-14 (device libm)."""
+    pb, oc = build(ctx, case)
+    ho, dv = _device_entries(ctx, oc, NATURAL_ENTRIES[case[0]], 21)
+    F0 = np.random.default_rng(22).standard_normal(oc.n_free)
+    for time in (0.0, 0.4):
+        F = ctx.from_numpy(F0)
+        pb.apply_natural(F, dv, time=time)
+        want = oc.apply_natural(ho, F0, time=time)
+        say("natural", time, F.numpy()[:3], want[:3])
+        assert same(F.numpy(), want)
+    # the time-dependent, transcendental flux (kind 2)
+    F = ctx.from_numpy(F0)
+    pb.apply_natural(F, [(1, O.FLUX_KINDS[2][:oc.dim]), (3, O.FLUX_KINDS[2][:oc.dim])], time=0.3)
+    want = oc.apply_natural([(1, 2), (3, 2)], F0, time=0.3)
+    assert np.max(np.abs(F.numpy() - want)) <= 1e-14 * np.max(np.abs(want))
+    # no entries: F unchanged; a flux with the wrong arity is refused
+    F = ctx.from_numpy(F0)
+    pb.apply_natural(F, [])
+    assert same(F.numpy(), F0)
+    import residuum_hip as rh
+    with pytest.raises(rh.RsdError):
+        pb.apply_natural(F, [(0, ("1",) * (oc.dim + 1))])
+
+
 def test_boundary_flux_nonlinear_anisotropic_bitexact(ctx):
     """k(T) times an anisotropic tensor: K = k(T) shape at each face point."""
     import residuum_hip as rh

@@ -306,6 +306,72 @@ def test_live_reference(spec):
 
 
 # ---------------------------------------------------------------------------
+# natural (flux) boundary conditions (BoundaryApplicator.tpp:66-140)
+# ---------------------------------------------------------------------------
+NATURAL_CASES = [
+    # mesh, Dirichlet tags / kinds, natural entries (tag, flux kind or "nodal")
+    ((2, (6, 5), (1, 1), 2, [0], [7], (0.0, 2.0, -1.0, 1.0)), [(1, 0), (2, 1), (3, 2), (2, "nodal")]),
+    ((2, (4, 3), (2, 2), 3, [3], [6], None), [(0, 1), (1, 2), (2, "nodal"), (0, 0)]),
+    ((3, (3, 4, 2), (1, 1, 1), 2, [4], [6], None), [(0, 2), (1, 0), (5, "nodal"), (2, 1), (3, 2)]),
+    ((3, (2, 2, 3), (1, 1, 1), 2, [], [], None), [(4, 1), (5, 2), (0, "nodal")]),
+]
+
+
+def _natural_entries(c, spec, seed=3):
+    rng = np.random.default_rng(seed)
+    out = []
+    for tag, fl in spec:
+        out.append((tag, rng.standard_normal(c.n_nodes * c.dim) if fl == "nodal" else fl))
+    return out
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref/libref.so not built (needs /root/reference)")
+@pytest.mark.parametrize("case", NATURAL_CASES)
+def test_natural_bc_live_reference(case):
+    """The oracle's natural BC terms bit for bit against the reference's own
+    applyNaturalBCs with FluxBoundaryForm / NodalFluxForm entries (several
+    entries on one tag, a tag with no flux, a constrained face among them)."""
+    (dim, n, p, nq, tags, kinds, ext), spec = case
+    c = O.Case(dim, n, p, nq, tags, [O.KIND_EXPR[k] for k in kinds], ext=ext)
+    r = O.RefCase(dim, n, p, nq, tags, kinds, ext=ext)
+    entries = _natural_entries(c, spec)
+    F0 = np.random.default_rng(9).standard_normal(c.n_free)
+    for time in (0.0, 0.7):
+        got = c.apply_natural(entries, F0, time=time)
+        want = r.natural(entries, F0, time=time)
+        assert same(got, want)
+        assert not np.array_equal(got, F0)
+
+
+def test_natural_bc_closed_forms():
+    """Flux integrals the quadrature computes exactly: the free and the
+    constrained nodes' face terms add up to the boundary integral of q.n.
+    Unit square, no Dirichlet faces, so every node is free: constant
+    q = (1.5, -0.5) on x = 0 gives -1.5; q = (x, y^2) on x = 1 gives 1 and on
+    y = 1 gives 1; on the unit cube q = (x, y^2, x z) on z = 1 (tag 5) gives
+    the integral of x over the face, 1/2."""
+    c = O.Case(2, (5, 3), (1, 1), 2, [], [])
+    z = np.zeros(c.n_free)
+    assert np.isclose(c.apply_natural([(0, 0)], z).sum(), -1.5, rtol=0, atol=1e-14)
+    assert np.isclose(c.apply_natural([(1, 1)], z).sum(), 1.0, rtol=0, atol=1e-14)
+    assert np.isclose(c.apply_natural([(3, 1)], z).sum(), 1.0, rtol=0, atol=1e-14)
+    # the same flux on two entries of one tag doubles the terms (the two
+    # entries interleave per point, so to rounding)
+    one = c.apply_natural([(1, 1)], z)
+    assert np.allclose(c.apply_natural([(1, 1), (1, 1)], z), 2 * one, rtol=1e-14, atol=0)
+    c3 = O.Case(3, (2, 3, 2), (1, 1, 1), 2, [], [])
+    z3 = np.zeros(c3.n_free)
+    assert np.isclose(c3.apply_natural([(5, 1)], z3).sum(), 0.5, rtol=0, atol=1e-14)
+    # the nodal form interpolates: nodal values of a linear flux reproduce it
+    xyz = c3.xyz.reshape(-1, 3)
+    nod = np.stack([xyz[:, 0], 2.0 + 0 * xyz[:, 0], xyz[:, 0] * 0.5], axis=1).ravel()
+    assert np.isclose(c3.apply_natural([(5, nod)], z3).sum(), 0.25, rtol=0, atol=1e-14)
+    # Q2 quads: the flux (x, y^2) on y = 1 (tag 3) integrates to 1 as well
+    c2 = O.Case(2, (3, 2), (2, 2), 3, [], [])
+    assert np.isclose(c2.apply_natural([(3, 1)], np.zeros(c2.n_free)).sum(), 1.0, rtol=0, atol=1e-14)
+
+
+# ---------------------------------------------------------------------------
 # the matrix-free operator (FEMOperator.hpp:27-71) and assembleDiagonal
 # ---------------------------------------------------------------------------
 DIAG_CASES = [
