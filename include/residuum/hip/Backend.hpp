@@ -15,6 +15,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 
 #include "core/Types.hpp"
 #include "linalg/types/backend/cuda/CopyKind.hpp"
@@ -67,6 +68,47 @@ private:
     rsd_ctx* ctx_ = nullptr;
 };
 
+namespace detail {
+
+// CSR patterns made by Assembler<Hip>::createMatrix, by row pointer: the
+// solvers then run on the device problem's own sliced layout of that
+// pattern instead of rebuilding it from the CSR arrays.  An entry goes when
+// either array is freed.
+struct PatternOwner {
+    const void* colIdx;
+    rsd_problem* pb;
+};
+inline std::unordered_map<const void*, PatternOwner>& patterns() {
+    thread_local std::unordered_map<const void*, PatternOwner> m;
+    return m;
+}
+inline void remember_pattern(const void* rowPtr, const void* colIdx, rsd_problem* pb) {
+    patterns()[rowPtr] = PatternOwner{colIdx, pb};
+}
+inline rsd_problem* pattern_problem(const void* rowPtr, const void* colIdx) {
+    auto& m = patterns();
+    auto it = m.find(rowPtr);
+    return (it != m.end() && it->second.colIdx == colIdx) ? it->second.pb : nullptr;
+}
+inline void forget_pointer(const void* p) {
+    auto& m = patterns();
+    if (m.empty()) return;
+    m.erase(p);
+    for (auto it = m.begin(); it != m.end();) {
+        if (it->second.colIdx == p) it = m.erase(it);
+        else ++it;
+    }
+}
+inline void forget_problem(const rsd_problem* pb) {
+    auto& m = patterns();
+    for (auto it = m.begin(); it != m.end();) {
+        if (it->second.pb == pb) it = m.erase(it);
+        else ++it;
+    }
+}
+
+}  // namespace detail
+
 }  // namespace hip
 
 namespace linalg {
@@ -76,7 +118,9 @@ namespace backend {
 template<typename T>
 struct HipDeleter {
     void operator()(T* p) const noexcept {
-        if (p) rsd_free(hip::Context::get(), p);
+        if (!p) return;
+        hip::detail::forget_pointer(p);
+        rsd_free(hip::Context::get(), p);
     }
 };
 

@@ -17,6 +17,7 @@
 
 #include <cassert>
 
+#include "residuum/hip/DOFNorms.hpp"
 #include "linalg/operations/MatrixOps.hpp"
 #include "linalg/operations/VectorOps.hpp"
 #include "linalg/types/CSRMatrix.hpp"

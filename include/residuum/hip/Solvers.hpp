@@ -139,8 +139,11 @@ public:
         if (!m_ || rp != rp_ || ci != ci_ || A.nRows() != n_) {
             if (m_) rsd_matrix_destroy(m_);
             m_ = nullptr;
-            check(rsd_matrix_from_csr(Context::get(), (int64_t)A.nRows(), reinterpret_cast<const uint64_t*>(rp),
-                                      reinterpret_cast<const uint64_t*>(ci), &m_));
+            if (rsd_problem* pb = detail::pattern_problem(rp, ci))
+                check(rsd_matrix_create(pb, &m_));  // the assembler's pattern: its sliced layout as it is
+            else
+                check(rsd_matrix_from_csr(Context::get(), (int64_t)A.nRows(), reinterpret_cast<const uint64_t*>(rp),
+                                          reinterpret_cast<const uint64_t*>(ci), &m_));
             rp_ = rp;
             ci_ = ci;
             n_ = A.nRows();

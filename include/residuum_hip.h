@@ -87,6 +87,11 @@ const char* rsd_abi_hash(void);
 /* (CUDA.hpp:30-64 cudaMalloc / cudaMemcpy / cudaMemset / cudaFree)        */
 /* ---------------------------------------------------------------------- */
 int rsd_malloc(rsd_ctx* ctx, size_t bytes, void** out);
+/* *out = 1 when p is device memory (rsd_malloc / hipMalloc), 0 for host
+ * memory: lets host code that reads a vector's data() (the reference's
+ * fem/dof/DOFResidualNorms.hpp:14-53 behind its loggers and NewtonRunner)
+ * tell a Hip vector from a CPU one */
+int rsd_pointer_is_device(const void* p, int32_t* out);
 int rsd_free(rsd_ctx* ctx, void* p);
 int rsd_memcpy(rsd_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
 int rsd_memset_zero(rsd_ctx* ctx, void* p, size_t bytes);
@@ -148,6 +153,10 @@ int rsd_problem_create(rsd_ctx* ctx, int32_t dim, const int32_t* p, int32_t nq, 
                        int64_t n_elem, const double* xyz, const int64_t* ien, const int32_t* rng,
                        int32_t n_bc, const int32_t* bc_tags, const char* const* bc_exprs, rsd_problem** out);
 int rsd_problem_destroy(rsd_problem* pb);
+/* the Gauss rule (nq points per direction, 1..5) of the element and face
+ * integrals from now on; the mesh, DOF maps and pattern do not depend on it
+ * (the reference passes the rule to each assembly call, Assembler.hpp:58-66) */
+int rsd_problem_set_quadrature(rsd_ctx* ctx, rsd_problem* pb, int32_t nq);
 /* out[8] = n_nodes, n_elem, npe, n_free, nnz, n_colours, sell_slots, max_row */
 int rsd_problem_sizes(rsd_problem* pb, int64_t* out);
 int rsd_problem_get_mesh(rsd_problem* pb, double* xyz, int64_t* ien, int32_t* rng);

@@ -1052,6 +1052,18 @@ int rsd_problem_eval_expr(rsd_ctx* ctx, rsd_problem* pb, const char* expr, doubl
     });
 }
 
+int rsd_pointer_is_device(const void* p, int32_t* out) {
+    return guard([&] {
+        RSD_CHECK(out, "rsd_pointer_is_device: null argument");
+        *out = 0;
+        if (!p) return;
+        hipPointerAttribute_t a{};
+        const hipError_t e = hipPointerGetAttributes(&a, p);
+        (void)hipGetLastError();  // a plain host pointer is not an error of the caller's
+        *out = (e == hipSuccess && a.type == hipMemoryTypeDevice) ? 1 : 0;
+    });
+}
+
 struct rsd_expr {
     rsd::Expr e;
 };
@@ -1115,6 +1127,19 @@ int rsd_problem_set_dirichlet_values(rsd_ctx* ctx, rsd_problem* pb, double time,
         pb->gval_external = true;
         pb->gval_valid = true;
         pb->gval_time = time;
+    });
+}
+
+int rsd_problem_set_quadrature(rsd_ctx* ctx, rsd_problem* pb, int32_t nq) {
+    return guard([&] {
+        RSD_CHECK(ctx && pb, "rsd_problem_set_quadrature: null argument");
+        RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
+        RSD_CHECK(nq >= 1 && nq <= 5, "quadrature points outside [1, 5]");
+        if (nq == pb->nq) return;
+        pb->nq = nq;
+        build_shape_table(pb);
+        pb->reaction.reset();  // its face points depend on the rule
+        RSD_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
 
