@@ -285,6 +285,7 @@ public:
 
     const mesh::Mesh& mesh() const { return mesh_; }
     const topology::TopologicalDOF<NumDOFs>& topology() const { return topoDOF_; }
+    const fem::boundary::EssentialBoundaryRegistry& essentialRegistry() const { return essentialBCs_; }
 
 private:
     // HeatProblem.tpp:178-214: a nodal field into the free DOFs

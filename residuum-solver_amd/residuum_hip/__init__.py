@@ -381,7 +381,11 @@ class Problem:
          self.max_row) = [int(v) for v in s]
 
     def close(self):
+        """Release the matrices made on this problem, then the problem (a
+        matrix's destroy reads its problem)."""
         if self.h:
+            for m in list(getattr(self, "_matrices", ())):
+                m.close()
             lib().rsd_problem_destroy(self.h)
             self.h = _vp()
 
@@ -613,6 +617,9 @@ class Matrix:
         self.h = _vp()
         check(lib().rsd_matrix_create(pb.h, C.byref(self.h)))
         pb.ctx._adopt(self)
+        if not hasattr(pb, "_matrices"):
+            pb._matrices = weakref.WeakSet()
+        pb._matrices.add(self)
 
     def close(self):
         if self.h:

@@ -10,8 +10,9 @@ here follow its parsers key for key (src/solver/parser/*,
 src/application/heateq/parser/*).  `backend:` is optional; when present it
 must be `hip` (the reference's parser knows cpu and cuda,
 BackendConfigParser.cpp:6-14).  What the device path does not run is refused
-with a message, never run on the host: natural (flux) boundary conditions,
-boundary values or sources other than expressions or (sources) nodal files,
+with a message, never run on the host: fluxes from nodal files or with the
+Newton solver, boundary values or sources other than expressions or
+(sources) nodal files,
 weak flux monitors together with a nodal-file source, non-Lagrange meshes,
 the pseudo-transient driver, and time steppers other than Backward Euler
 with a constant step.
@@ -176,17 +177,35 @@ def parse_heat_config(path):
         if _opt(src, "type", "volumetric") != "volumetric":
             raise ConfigError("physics.models.source: only volumetric sources")
         cfg["source"] = _read_field(src, "physics.models.source", allow_file=True)
-    # boundary conditions (BoundaryConditionConfigParser.cpp)
-    tags, exprs = [], []
+    # boundary conditions (BoundaryConditionConfigParser.cpp:26-60): each form
+    # registers the condition, value_bc as essential, flux_bc as natural
+    # (HeatProblem.tpp:62-112); a flux reads one expression per spatial
+    # component
+    tags, exprs, natural = [], [], []
     for k, bc in enumerate(_opt(root, "boundary_conditions", []) or []):
         where = f"boundary_conditions[{k}]"
         btype = _req(bc, "type", where)
-        if btype != "value":
-            raise ConfigError(f"{where}: '{btype}' (natural) boundary conditions are not supported on the device")
-        mode, text = _read_field(bc, where)
-        tags.append(int(_req(bc, "boundary", where)))
-        exprs.append(text)
-    cfg["bc_tags"], cfg["bc_exprs"] = tags, exprs
+        if btype not in ("value", "flux"):
+            raise ConfigError(f"Unknown boundary type: {btype}")
+        tag = int(_req(bc, "boundary", where))
+        read = _req(bc, "read", where)
+        mode = _opt(read, "mode", "expression")
+        for form in _req(bc, "forms", where):
+            if form == "value_bc":
+                if mode != "expression" or btype != "value":
+                    raise ConfigError(f"{where}: boundary values other than expressions are not run on the device")
+                tags.append(tag)
+                exprs.append(str(_req(read, "expression", where + ".read")))
+            elif form == "flux_bc":
+                if mode != "expression":
+                    raise ConfigError(f"{where}: a flux from a nodal file is not run on the device")
+                q = _req(read, "expression", where + ".read")
+                if btype != "flux" or not isinstance(q, (list, tuple)):
+                    raise ConfigError(f"{where}: a flux_bc reads one expression per spatial component")
+                natural.append((tag, [str(e) for e in q]))
+            else:
+                raise ConfigError(f"Unknown boundary form: {form}")
+    cfg["bc_tags"], cfg["bc_exprs"], cfg["natural"] = tags, exprs, natural
     ic = root.get("initial_condition")
     cfg["initial"] = _read_field(ic, "initial_condition", allow_file=True) if ic else ("expression", "0")
     cfg["monitors"] = _monitors(root, base)
@@ -363,6 +382,9 @@ def run_heat(config_path, quiet=False, ctx=None):
     ctx = ctx or Ctx(0)
     try:
         pb = Problem.from_pmsh(ctx, cfg["mesh"], nq[0], cfg["bc_tags"], cfg["bc_exprs"])
+        for tag, q in cfg["natural"]:
+            if len(q) != mesh["sdim"]:
+                raise ConfigError("HeatProblem: flux boundary 'expression' must have SpatialDim components")
         c = cfg["conductivity"]
         tensor = None if c["tensor"] is None else np.array(c["tensor"], np.float64).ravel()
         pb.set_conductivity(c["value"], k_expr=c["k_expr"], dk_expr=c["dk_expr"], tensor=tensor)
@@ -447,6 +469,9 @@ def _steady(pb, cfg, U, src_text, src_nodal, quiet, mon=None):
         lin = nl["linear"] or cfg["linear"]
         if src_nodal is not None:
             raise ConfigError("a nodal-file source with the Newton solver is not run on the device")
+        if cfg["natural"]:
+            raise ConfigError("flux boundary conditions with the Newton solver are run by hip::HeatProblem, "
+                              "not by this command line")
         if lin["operator"] != "csr":
             raise ConfigError("Newton runs with the CSR operator on the device")
         t0 = _time.perf_counter()
@@ -466,6 +491,8 @@ def _steady(pb, cfg, U, src_text, src_nodal, quiet, mon=None):
     forms = [(DIFFUSION, 1.0)]
     F = pb.vector()
     pb.assemble_load(F, src=src_text, nodal=src_nodal)
+    if cfg["natural"]:
+        pb.apply_natural(F, cfg["natural"])  # SteadyStage::assemble, SteadyStage.hpp:68-70
     pb.apply_essential(F, forms)
     K = None
     if lin["operator"] == "csr":
@@ -493,7 +520,8 @@ def _transient(pb, cfg, U, src_text, src_nodal, mesh, quiet, mon=None):
     if mon:
         mon.evaluate(0, t0, U, Udot)
     start = _time.perf_counter()
-    if lin["operator"] == "csr" and lin["type"] == "cg" and src_nodal is None and cfg["output"] is None and not mon:
+    if lin["operator"] == "csr" and lin["type"] == "cg" and src_nodal is None and cfg["output"] is None and not mon \
+            and not cfg["natural"]:
         steps, iters = pb.be_run(U, t0, tf, dt, src=src_text, tol=lin["tolerance"], max_iters=lin["max_iterations"],
                                  precond=_precond(lin), mixed=lin["precision"] == "mixed")
         _log(f"backward euler: {steps} steps on the device, {int(np.sum(iters))} CG iterations "
@@ -515,6 +543,8 @@ def _transient(pb, cfg, U, src_text, src_nodal, mesh, quiet, mon=None):
         pb.assemble_load(F, src=src_text, nodal=src_nodal, time=ct)
         pb.assemble_vector(ms, [(MASS, 1.0)], U=Uprev, selector=SEL_VALUE, time=ct - h)
         pb.ctx.axpy(1.0 / h, ms, F)
+        if cfg["natural"]:
+            pb.apply_natural(F, cfg["natural"], time=ct)  # BackwardEulerStage.hpp:93
         pb.apply_essential(F, forms, time=ct)
         if K is not None:
             pb.assemble_matrix(K, forms, time=ct)

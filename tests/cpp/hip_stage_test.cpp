@@ -292,7 +292,7 @@ int main(int argc, char** argv) {
             nl.absoluteTolerance = 1e-14;
             nl.relativeTolerance = 1e-10;
             nl.maxIterations = 20;
-            nl.linearSolver = linear(sc::LinearSolverConfig::Type::GMRES, 1e-13, 20000);
+            nl.linearSolver = linear(sc::LinearSolverConfig::Type::GMRES, 1e-11, 20000);
             c.cfg.solver.nonlinear = nl;
             c.cfg.solver.timestepper->nonlinearSolver = nl;
             c.cfg.solver.timestepper->linearSolver = nl.linearSolver;

@@ -208,3 +208,36 @@ def test_monitors_steady_and_transient(ctx, tmp_path):
     assert ticks[-1][2] == want
     rows = open(tmp_path / "mon" / "net_influx.csv").read().splitlines()
     assert len(rows) == 7 and rows[0] == "tick,time,net_influx [W]"
+
+
+@pytest.mark.parametrize("operator", ["csr", "fem"])
+def test_flux_boundary_condition(ctx, tmp_path, operator):
+    """A flux_bc (BoundaryConditionConfigParser.cpp:43-50, applied by
+    applyNaturalBCs) on x = 1 in place of the value there: with the flux
+    K grad T of T = (1-x)(1-y) the discrete solution is still that bilinear
+    field (the face integral of the flux is exact with 2 Gauss points), both
+    steady and as a steady state kept by the Backward Euler loop."""
+    from residuum_hip import cli
+    d = bench_config(tmp_path)
+    d["solver"]["linear"]["operator"]["type"] = operator
+    d["solver"]["linear"]["preconditioner"]["type"] = "jacobi"
+    for bc in d["boundary_conditions"]:
+        if bc["boundary"] == 1:
+            bc.update(type="flux", forms=["flux_bc"])
+            bc["read"] = {"mode": "expression", "expression": ["-(1-y)", "-(1-x)"], "unit": "W/m^2"}
+    s = cli.run_heat(write_cfg(tmp_path, d), quiet=True, ctx=ctx)
+    xy = nodes_xy(ctx, str(tmp_path / "square_n16.pmsh"), tags=(0, 2, 3), exprs=("0",) * 3)
+    exact = (1 - xy[:, 0]) * (1 - xy[:, 1])
+    say("flux steady", operator, s["iterations"], np.max(np.abs(s["U"] - exact)))
+    assert s["converged"] and s["U"].size == exact.size
+    assert np.max(np.abs(s["U"] - exact)) <= 1e-9
+    d["solver"]["driver"]["type"] = "transient"
+    d["solver"]["timestepper"] = {"type": "backward_euler", "t0": 0.0, "tf": 0.06,
+                                  "step_size": {"mode": "constant", "dt": 0.02}}
+    d["initial_condition"] = {"read": {"mode": "expression", "expression": "(1-x)*(1-y)", "unit": "K"}}
+    d["materials"]["density"] = {"value": 1.0, "unit": "kg/m^3"}
+    d["materials"]["specific_heat"] = {"type": "constant", "value": 1.0, "unit": "J/(kg*K)"}
+    s = cli.run_heat(write_cfg(tmp_path, d, "transient.yaml"), quiet=True, ctx=ctx)
+    say("flux transient", operator, s["iterations"], np.max(np.abs(s["U"] - exact)))
+    assert s["converged"] and s["steps"] == 3
+    assert np.max(np.abs(s["U"] - exact)) <= 1e-9

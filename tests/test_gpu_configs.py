@@ -167,6 +167,46 @@ def test_batched_256_problems(ctx):
 
 
 @pytest.mark.gpu
+def test_config1_steady_square_64(ctx):
+    """The plain FP64 workload as stated: 64x64 Q1 quads on the unit square
+    (4,225 nodes, 3,969 free), 2x2 Gauss, k = 1, Dirichlet T = (1-x)(1-y)
+    on all four sides, no source, CG + Jacobi to 1e-10 relative from 0.
+    The solve is bitwise the oracle's in the kernels' order (iterations,
+    report, solution); the discrete solution is the bilinear field itself,
+    so the nodal values match (1-x)(1-y) to the solver tolerance; against
+    the reference-order oracle at 1e-13 the solutions agree to 1e-12."""
+    import residuum_hip as rh
+    dim, n, p, nq, tags, g = 2, (64, 64), (1, 1), 2, [0, 1, 2, 3], ["(1-x)*(1-y)"] * 4
+    pb = rh.Problem(ctx, dim, n, p, nq, tags, g)
+    oc = O.Case(dim, n, p, nq, tags, g, order=O.ORDER_GPU)
+    assert pb.n_nodes == 4225 and pb.n_free == 3969
+    forms = [(rh.DIFFUSION, 1.0)]
+    K, F, U = pb.matrix(), pb.vector(), pb.vector(0.0)
+    pb.assemble_matrix(K, forms)
+    pb.assemble_load(F)
+    pb.apply_essential(F, forms)
+    t0 = time.perf_counter()
+    rep = K.cg(F, U, tol=1e-10, max_iters=2000)
+    dt = time.perf_counter() - t0
+    Uo, ro = oc.steady(solver=0, tol=1e-10, max_iters=2000)
+    say(f"config 1: {rep.iterations} CG iterations ({ro.iterations} oracle) in {dt*1e3:.1f} ms, "
+        f"{pb.n_free * rep.iterations / dt:.3e} DOF-iters/s (one solve, host timing)")
+    assert rep.converged == 1 and rep.iterations == ro.iterations
+    assert np.array_equal(U.numpy(), Uo)
+    xyz = oc.xyz.reshape(-1, 2)[oc.a2t]
+    exact = (1 - xyz[:, 0]) * (1 - xyz[:, 1])
+    err = np.max(np.abs(U.numpy() - exact))
+    say(f"max |T - (1-x)(1-y)| at the nodes: {err:.2e}")
+    assert err < 1e-7
+    oc.set_order(O.ORDER_REF)
+    Ur, _ = oc.steady(solver=0, tol=1e-13, max_iters=4000)
+    U2 = pb.vector(0.0)
+    K.cg(F, U2, tol=1e-13, max_iters=4000)
+    assert np.max(np.abs(U2.numpy() - Ur)) <= 1e-12 * np.max(np.abs(Ur))
+    assert np.max(np.abs(U2.numpy() - exact)) < 1e-10
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("n,lin_solver,lin_tol", [(512, 0, 1e-10), (128, 1, 1e-12), (128, 2, 1e-10)],
                          ids=["512-cg", "128-gmres30", "128-gmres30-mixed"])
 def test_newton_q2(ctx, n, lin_solver, lin_tol):
