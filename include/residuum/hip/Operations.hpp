@@ -12,8 +12,22 @@
 //   matvec / lump   row sums left to right, bitwise the CPU loop
 // A template of the reference that calls operations::* on Hip vectors must
 // see these declarations: include this header before the reference's
-// header that defines that template.
+// header that defines that template.  A qualified call such as
+// operations::copy(b, W.r) in a template body binds to the overloads
+// declared before that body; one included earlier would compile and then
+// run the CPU loops on device memory, so that order is refused here.
 #pragma once
+
+#if defined(RESIDUUM_LINALG_SOLVER_ITERATIVE_CG_SOLVER_HPP) ||          \
+    defined(RESIDUUM_LINALG_SOLVER_ITERATIVE_GMRES_SOLVER_HPP) ||       \
+    defined(RESIDUUM_LINALG_OPERATOR_CSROPERATOR_HPP) ||                \
+    defined(RESIDUUM_LINALG_SOLVER_PRECONDITIONER_IDENTITY_HPP) ||      \
+    defined(RESIDUUM_LINALG_SOLVER_PRECONDITIONER_JACOBI_HPP) ||        \
+    defined(RESIDUUM_SOLVER_STAGE_BACKWARDEULERSTAGE_HPP) ||            \
+    defined(RESIDUUM_SOLVER_STAGE_STEADYSTAGE_HPP) ||                   \
+    defined(RESIDUUM_APPLICATION_HEATEQ_PROBLEM_HEATPROBLEM_HPP)
+#error "residuum/hip/Operations.hpp must be included before the reference headers whose templates call operations::*"
+#endif
 
 #include <cassert>
 

@@ -24,6 +24,13 @@
 #include <type_traits>
 #include <vector>
 
+// first: the reference's cg / gmres Solver bodies (Solver.tpp) call
+// operations::* with qualified names, which bind to the overloads declared
+// before them -- the Hip ones must precede those bodies, or the generic
+// solvers (used over operators this header does not specialise, e.g.
+// FEMOperator) would run the CPU loops on device memory
+#include "residuum/hip/Operations.hpp"
+
 #include "linalg/operator/CSROperator.hpp"
 #include "linalg/operator/Operator.hpp"
 #include "linalg/solver/base/SolverReport.hpp"
@@ -31,7 +38,6 @@
 #include "linalg/solver/iterative/gmres/Solver.hpp"
 #include "linalg/solver/preconditioner/Identity.hpp"
 #include "linalg/solver/preconditioner/Jacobi.hpp"
-#include "residuum/hip/Operations.hpp"
 
 namespace residuum {
 namespace linalg {

@@ -54,10 +54,13 @@ struct Result {
     bool converged = false;
     std::vector<double> U;
     std::vector<long> linIters, newtonIters;  // per solve, from the CSV loggers
+    std::vector<double> linStart;              // per linear solve: the initial residual norm
 };
 
-// iterations per solve: the largest `iter` logged under each outer tick
-static std::vector<long> iterations(const std::string& csv) {
+// iterations per solve: the largest `iter` logged under each outer tick;
+// start (optional) receives the third column of each tick's first row (the
+// linear loggers' abs residual at iteration 0)
+static std::vector<long> iterations(const std::string& csv, std::vector<double>* start = nullptr) {
     std::ifstream f(csv);
     std::string line;
     std::getline(f, line);
@@ -65,15 +68,42 @@ static std::vector<long> iterations(const std::string& csv) {
     std::vector<long> order;
     while (std::getline(f, line)) {
         std::stringstream ss(line);
-        double tick = 0, it = 0;
+        double tick = 0, it = 0, a = 0;
         char c;
-        ss >> tick >> c >> it;
-        if (!last.count((long)tick)) order.push_back((long)tick);
+        ss >> tick >> c >> it >> c >> a;
+        if (!last.count((long)tick)) {
+            order.push_back((long)tick);
+            if (start) start->push_back(a);
+        }
         last[(long)tick] = (long)it;
     }
     std::vector<long> out;
     for (long t : order) out.push_back(last[t]);
     return out;
+}
+
+// Linear iteration counts must be equal on both tags, except for a Newton
+// solve that starts below 1e-8 of its sequence's first residual: its
+// right-hand side is the residual near convergence, whose rounding (about
+// 1e-16 of the assembled terms) differs between the tags because their
+// reductions sum in different orders, i.e. it differs from the other tag's
+// right-hand side by ~1e-7 relative, and the Krylov solver may need a few
+// iterations more or less on it.  Those counts must agree within 20 %.
+static bool counts_agree(const Result& a, const Result& b) {
+    if (a.linIters.empty() || a.linIters.size() != b.linIters.size() || a.linStart.size() != a.linIters.size())
+        return false;
+    std::vector<bool> exact(a.linIters.size(), true);
+    size_t j = 0;
+    for (long k : a.newtonIters) {
+        const double first = j < a.linStart.size() ? a.linStart[j] : 0.0;
+        for (long i = 0; i < k && j < a.linStart.size(); ++i, ++j)
+            if (a.linStart[j] < 1e-8 * first) exact[j] = false;
+    }
+    for (size_t i = 0; i < a.linIters.size(); ++i) {
+        const long d = std::labs(a.linIters[i] - b.linIters[i]);
+        if (exact[i] ? d != 0 : 5 * d > std::max(a.linIters[i], b.linIters[i])) return false;
+    }
+    return true;
 }
 
 template<typename BackendT, Index D, mesh::ElementFamily Family>
@@ -121,7 +151,7 @@ Result run(const Case& c, const std::string& dir, const std::string& tag) {
         r.U.resize(pb.U().size());
         BackendT::template copy<Real>(r.U.data(), pb.U().data(), r.U.size(), linalg::types::backend::CopyKind::DeviceToHost);
     }
-    r.linIters = iterations(cfg.logging.linear.csvFile);
+    r.linIters = iterations(cfg.logging.linear.csvFile, &r.linStart);
     r.newtonIters = iterations(cfg.logging.nonlinear.csvFile);
     return r;
 }
@@ -141,7 +171,7 @@ void compare(const Case& c, const std::string& dir, bool cpu_only) {
         for (long x : v) s += std::to_string(x) + " ";
         return s;
     };
-    report(c.name + "_linear_iterations", a.linIters == b.linIters && !a.linIters.empty(),
+    report(c.name + "_linear_iterations", counts_agree(a, b),
            "cpu [" + show(a.linIters) + "] hip [" + show(b.linIters) + "]");
     report(c.name + "_newton_iterations", a.newtonIters == b.newtonIters,
            "cpu [" + show(a.newtonIters) + "] hip [" + show(b.newtonIters) + "]");

@@ -114,7 +114,10 @@ def test_heat_config_precision(tmp_path):
 @pytest.mark.parametrize("edit,msg", [
     (lambda d: d.update(backend="cpu"), "backend"),
     (lambda d: d["solver"]["linear"].update(type="bicgstab"), "bicgstab"),
-    (lambda d: d["boundary_conditions"][0].update(type="flux"), "natural"),
+    (lambda d: d["boundary_conditions"][0].update(type="flux", forms=["flux_bc"]), "one expression per spatial"),
+    (lambda d: d["boundary_conditions"][0].update(type="flux", forms=["flux_bc"], read={"mode": "file", "file": "q"}),
+     "nodal file"),
+    (lambda d: d["boundary_conditions"][0].update(type="flux"), "other than expressions"),
     (lambda d: d["solver"]["linear"]["preconditioner"].update(type="ilu"), "preconditioner"),
     (lambda d: d["solver"]["driver"].update(type="pseudo_transient"), "driver"),
     (lambda d: d.pop("solver"), "solver"),
