@@ -136,8 +136,10 @@ def test_steady_nonlinear_kirchhoff(ctx, tmp_path):
         "materials": {"conductivity": {"type": "temperature_dependent_isotropic", "value_expression": "1.0 + 0.5*T",
                                        "gradient_expression": "0.5", "unit": "W/(m*K)"}},
         "boundary_conditions": [
-            {"boundary": 0, "type": "value", "read": {"mode": "expression", "expression": "0.0"}},
-            {"boundary": 1, "type": "value", "read": {"mode": "expression", "expression": "1.0"}}],
+            {"boundary": 0, "type": "value", "read": {"mode": "expression", "expression": "0.0", "unit": "K"},
+             "forms": ["value_bc"], "model": "constant"},
+            {"boundary": 1, "type": "value", "read": {"mode": "expression", "expression": "1.0", "unit": "K"},
+             "forms": ["value_bc"], "model": "constant"}],
         "solver": {"driver": {"type": "steady"},
                    "nonlinear": {"type": "newton", "absolute_tolerance": 1e-10, "relative_tolerance": 1e-10,
                                  "max_iterations": 30,
