@@ -115,6 +115,15 @@ int rsd_axpby_f64(rsd_ctx* ctx, int64_t n, double alpha, double beta, const doub
 int rsd_scal_f64(rsd_ctx* ctx, int64_t n, double alpha, double* x);
 int rsd_copy_f64(rsd_ctx* ctx, int64_t n, const double* x, double* y);
 int rsd_mul_f64(rsd_ctx* ctx, int64_t n, const double* x, const double* y, double* z);        /* z = x .* y */
+/* rsd_dot_f64 for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops); no counterpart in the reference.  The same
+ * kernels and result bits as rsd_dot_f64, the block partials in a static
+ * 512 KB device array instead of the context's partials area: no allocation
+ * on any call.  d_out (device) receives x . y; nothing is synchronised.
+ * n <= 2^24, else RSD_ERR_ARG.  The array is one per device, shared by every
+ * call on that device: calls that overlap on one device (other streams or
+ * contexts) race. */
+int rsd_dot_f64_noalloc(rsd_ctx* ctx, int64_t n, const double* x, const double* y, double* d_out);
 /* The same operations on FP32 vectors (Vector<float>, VectorOps.tpp:7-86):
  * the elementwise ones in float, each product and sum rounded once as in the
  * CPU loop (bitwise it); dot and norm2 form each product exactly in FP64 and

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 namespace rsd {
 
@@ -404,12 +405,57 @@ CGState cg_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* x, do
     return h;
 }
 
+// ------------------------------------------------------------------------
+// The dot product for callers that cannot allocate: the block partials live
+// in a file-scope device array instead of the context's (grown on demand)
+// partials area.  Each device holds its own copy of the array; its address
+// there is looked up once, with that device current, and kept.
+// ------------------------------------------------------------------------
+constexpr int64_t kScratchPartials = 65536;  // 512 KB: vectors of up to 2^24 entries
+static __device__ __align__(256) double scratch[kScratchPartials];
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kScratchDevices] = {};
+
+static double* scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    RSD_CHECK(device >= 0 && device < kScratchDevices, "rsd_dot_f64_noalloc: device index out of range");
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        RSD_HIP(hipGetDevice(&cur));
+        if (cur != device) RSD_HIP(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        RSD_HIP(e);
+        RSD_CHECK(p != nullptr, "rsd_dot_f64_noalloc: no device address for the scratch array");
+        g_scratch_ptr[device] = (double*)p;
+    }
+    return g_scratch_ptr[device];
+}
+
 }  // namespace rsd
 
 // ------------------------------------------------------------------------
 // C-ABI
 // ------------------------------------------------------------------------
 extern "C" {
+
+int rsd_dot_f64_noalloc(rsd_ctx* ctx, int64_t n, const double* x, const double* y, double* d_out) {
+    return guard([&] {
+        RSD_CHECK(ctx && n >= 0 && (n == 0 || (x && y)) && d_out, "rsd_dot_f64_noalloc: bad argument");
+        const int64_t nb = std::max<int64_t>(1, nblocks(n));
+        RSD_CHECK(nb <= kScratchPartials, "rsd_dot_f64_noalloc: vectors longer than 2^24 entries (the scratch array "
+                                          "holds 65536 block partials of 256 entries)");
+        double* part = scratch_ptr(ctx->device);
+        // every block writes its partial and the fold reads exactly those:
+        // nothing in the array needs clearing between calls
+        hipLaunchKernelGGL(k_dot, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, x, y, part);
+        hipLaunchKernelGGL(k_dot_fold, dim3(1), dim3(kFold), 0, ctx->stream, nb, part, d_out, 0);
+        RSD_HIP(hipGetLastError());
+    });
+}
 
 int rsd_fill_f64(rsd_ctx* ctx, double* p, int64_t n, double v) {
     return guard([&] {
