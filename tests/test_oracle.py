@@ -175,6 +175,35 @@ def test_kirchhoff_newton():
     assert np.max(np.abs(U - theta)) < 1e-6
 
 
+def test_q2_newton_gmres30_growth():
+    """Restarted GMRES(30) + Jacobi on the Newton Jacobian of the Q2
+    anisotropic k(T) problem (k = (1 + 0.5 T) diag(1, 0.1), f = 10 sin(pi x)
+    sin(pi y), T = 0 on the boundary) after one Newton step, with that
+    step's residual as right-hand side.  The Arnoldi steps to a fixed
+    relative tolerance grow 2.7-4.4x per mesh halving (the condition
+    number grows 4x and restarting discards the Krylov space), which is what
+    makes an inexact inner tolerance necessary at 512^2 (~1.05M DOFs), where
+    1e-3 already takes ~30k steps.  The 64^2 counts are the ones the device
+    GMRES reports in the GPU reduction order."""
+    src = "10*sin(pi*x)*sin(pi*y)"
+    steps = {}
+    for n in (16, 32, 64):
+        c = O.Case(2, (n, n), (2, 2), 3, [0, 1, 2, 3], ["0"] * 4, k_expr="1.0 + 0.5*T", dk_expr="0.5",
+                   tensor=(1.0, 0.0, 0.0, 0.1), order=O.ORDER_GPU)
+        U, _, _, _ = c.newton(src=src, abs_tol=0.0, rel_tol=0.0, max_newton=1, lin_solver=1, lin_tol=1e-10,
+                              lin_max=100000)
+        J = c.assemble_matrix([(O.DIFFUSION, 1.0), (O.TDIFFUSION, 1.0)], U=U)
+        R = c.load(src=src) - c.assemble_vector([(O.DIFFUSION, 1.0)], U=U)
+        for tol in (1e-4, 1e-8):
+            x, rep = c.gmres(J, R, tol=tol, max_iters=100000, krylov=30)
+            assert rep.converged and rep.final_residual_rel < tol
+            steps[n, tol] = rep.iterations
+    print(steps)
+    assert steps[64, 1e-4] == 1168
+    for tol in (1e-4, 1e-8):
+        assert steps[32, tol] > 2.4 * steps[16, tol] and steps[64, tol] > 4.0 * steps[32, tol]
+
+
 def test_cg_gmres_2x2_known():
     """tests/unit/linalg/solver/iterative/cg/Solver.cpp: a 2x2 SPD system."""
     c = O.Case(2, (1, 1), (1, 1), 2, [], [])  # 4 free nodes, a full 4x4 pattern
