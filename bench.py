@@ -96,36 +96,145 @@ def cpu_info():
     return model, os.cpu_count()
 
 
-def cpu_baseline(pb, K, b_host, iters):
-    """The oracle's serial PCG (reference order) on the same assembled matrix:
-    a bounded sample of `iters` iterations, timed on one host core."""
+def _cores():
+    """The cores this process may run on (the box's share, not the machine's)."""
+    try:
+        return sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        return list(range(os.cpu_count() or 1))
+
+
+class _pinned:
+    """Run a block on one core (the reference is single-threaded): the
+    same effect as `taskset -c <core>` for this thread's timing."""
+
+    def __init__(self, core):
+        self.core = core
+
+    def __enter__(self):
+        self.old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {self.core})
+        return self
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.old)
+
+
+def cpu_baseline(pb, K, b_host, iters, kappa, dt):
+    """Serial CPU CG on the same assembled matrix, one pinned core, as a
+    differential sample: the rate is nfree * (k2 - k1) / (t(k2) - t(k1)),
+    so copies and first touches that do not scale with the iterations drop
+    out.  The reference's own cg::Solver with Jacobi (oracle/_ref/libref.so,
+    compiled from the reference's headers at -O3) when that harness is
+    built, kind "reference"; the oracle's C restatement beside it (the port
+    over reference time ratio is the fidelity of the port); the oracle
+    alone, kind "port", when the harness is absent.  Also the reference's
+    assembleMatrix (diffusion with the per-element conductivity) over every
+    element, in elements/s."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py as O
+    core = _cores()[0]
     rp, ci = pb.pattern()
     val = K.values()
-    x = np.zeros(pb.n_free)
-    rep = O.OrcReport()
-    t0 = time.perf_counter()
-    O.lib().orc_cg(C.c_int64(pb.n_free), O.lp(rp), O.lp(ci), O.dp(val), O.dp(b_host), O.dp(x), C.c_double(1e-300),
-                   C.c_int64(iters + 1), 1, O.ORDER_REF, C.byref(rep), None)
-    dt = time.perf_counter() - t0
-    return pb.n_free * rep.iterations / dt, rep.iterations, dt
+    n = pb.n_free
+    k1 = 20
+
+    def port(k):
+        x = np.zeros(n)
+        rep = O.OrcReport()
+        t0 = time.perf_counter()
+        O.lib().orc_cg(C.c_int64(n), O.lp(rp), O.lp(ci), O.dp(val), O.dp(b_host), O.dp(x), C.c_double(1e-300),
+                       C.c_int64(k), 1, O.ORDER_REF, C.byref(rep), None)
+        return time.perf_counter() - t0, int(rep.iterations)
+
+    out = {}
+    with _pinned(core):
+        if O.have_ref():
+            dim_n = round(pb.n_elem ** (1.0 / 3.0))
+            r = O.RefCase(3, (dim_n, dim_n, dim_n), (1, 1, 1), 2, [0, 1], [6, 0])
+            assert r.n_free == n and r.nnz == pb.nnz, "reference harness built a different problem"
+
+            def ref(k):
+                t0 = time.perf_counter()
+                _, _, it = r.solve(0, 1, val, b_host, tol=1e-300, max_iters=k)
+                return time.perf_counter() - t0, it
+            (ta, ia), (tb, ib) = ref(k1), ref(iters)
+            out["ref_rate"], out["ref_iters"], out["ref_s"] = n * (ib - ia) / (tb - ta), ib - ia, tb - ta
+            # the Backward Euler operator K + M/dt (the harness's set 2),
+            # the matrix each step assembles; its values against the
+            # device's (reference vs GPU summation order)
+            t0 = time.perf_counter()
+            vref = r.assemble_matrix(2, 1.0 / dt, kappa=kappa)
+            out["ref_asm_s"] = time.perf_counter() - t0
+            out["ref_asm_elements_per_s"] = pb.n_elem / out["ref_asm_s"]
+            out["ref_vs_gpu_matrix_max_rel_diff"] = float(np.max(np.abs(vref - val)) / np.max(np.abs(vref)))
+            del r
+        kp = iters if "ref_rate" not in out else max(k1 + 100, iters // 2)
+        (ta, ia), (tb, ib) = port(k1), port(kp)
+        out["port_rate"], out["port_iters"], out["port_s"] = n * (ib - ia) / (tb - ta), ib - ia, tb - ta
+    out["core"] = core
+    return out
+
+
+def oracle_be_step(n, path):
+    """Child process (started before the parent touches the GPU, pinned to
+    its own core): one Backward Euler step of the benchmark problem through
+    the oracle in the GPU summation order, U saved to `path` (.npy).  The
+    parent compares its FP64 step (bitwise expected) and its mixed step."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    kappa = np.exp(0.5 * O.normal_field(20261201, 0, n ** 3))
+    c = O.Case(3, (n, n, n), (1, 1, 1), 2, [0, 1], ["1", "0"], kappa=kappa, order=O.ORDER_GPU)
+    out = c.be_run(0.0, 1e-3, 1e-3, cg_tol=1e-8, cg_max=5000)
+    np.save(path, out["U"])
+    np.save(path[:-4] + "_iters.npy", out["iters"])
+
+
+def start_oracle_check(n):
+    import tempfile
+    cores = _cores()
+    d = tempfile.mkdtemp(prefix="rsd_bench_")
+    path = os.path.join(d, "oracle_U.npy")
+    proc = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--oracle-be-step", str(n), path],
+                            preexec_fn=(lambda: os.sched_setaffinity(0, {cores[-1]})) if len(cores) > 1 else None)
+    return proc, path, d
+
+
+def finish_oracle_check(proc, path, d, U64, Umx, it64, itmx, timeout=600):
+    import shutil
+    try:
+        rc = proc.wait(timeout=timeout)
+        if rc != 0:
+            return {"error": "oracle child exited with %d" % rc}
+        Uo = np.load(path)
+        ito = np.load(path[:-4] + "_iters.npy")
+        scale = max(float(np.max(np.abs(Uo))), 1e-300)
+        return {"steps": 1, "oracle": "C restatement, GPU summation order, FP64, 1 core",
+                "oracle_cg_iters": int(ito[0]), "fp64_cg_iters": int(it64), "mixed_inner_iters": int(itmx),
+                "fp64_bitwise": bool(np.array_equal(U64, Uo)),
+                "fp64_max_rel_err": float(np.max(np.abs(U64 - Uo)) / scale),
+                "mixed_max_rel_err": float(np.max(np.abs(Umx - Uo)) / scale)}
+    except subprocess.TimeoutExpired:
+        proc.kill()
+        return {"error": "oracle child timed out"}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 LAYOUTS = ["int32 columns", "int16 offsets", "shared offset patterns", "symmetric half storage",
            "symmetric half storage, uniform 27-point stencil"]
 
 
-def measured_traffic(n, layout):
-    """HBM bytes per k_spmv_f32 launch at this size, from the committed
-    FETCH_SIZE / WRITE_SIZE counter profile (scaled by the calibration copy's
-    counters); None when no profile for this size and FP32 layout is
-    committed."""
-    path = os.path.join(ROOT, "profiles", "spmv_f32_%dcube_hbm_counters.json" % n)
+def measured_traffic(n, kernel):
+    """HBM bytes per launch of `kernel` at this size, from the committed
+    FETCH_SIZE / WRITE_SIZE counter profile of the benchmark (written by
+    --summarize-rocprof hbm, scaled by the calibration copy's counters);
+    None when no profile of this size records that kernel."""
+    path = os.path.join(ROOT, "profiles", "hbm_counters_be_%dcube.json" % n)
     try:
         with open(path) as f:
             prof = json.load(f)
-        if prof.get("layout") != layout:
+        if prof.get("kernel") != kernel:
             return None
         return float(prof["traffic_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
@@ -232,6 +341,143 @@ def run_batched(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
     print(json.dumps(line), flush=True)
 
 
+def _short_kernel(name):
+    """`void rsd::k_spmv_f64<2>(long, ...)` -> `k_spmv_f64<2>`."""
+    name = name.strip().strip('"')
+    if name.startswith("void "):
+        name = name[5:]
+    name = name.replace("rsd::", "")
+    depth = 0
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return name[:i].strip()
+    return name.strip()
+
+
+def _csv_rows(dirname, suffix):
+    import csv
+    import glob
+    rows = []
+    for path in sorted(glob.glob(os.path.join(dirname, "**", "*" + suffix), recursive=True)):
+        with open(path, newline="") as f:
+            rows += [{k.strip().lower(): v for k, v in r.items() if k} for r in csv.DictReader(f)]
+    return rows
+
+
+def _col(row, *names, default=None):
+    for n in names:
+        if n in row and row[n] not in (None, ""):
+            return row[n]
+    return default
+
+
+def summarize_rocprof(argv):
+    """Profile summaries from rocprofv3 CSV output (--output-format csv):
+      trace DIR OUT.md [TITLE [COMMAND]]
+          kernel_stats (calls, average / min / max duration, share of GPU
+          time) joined with kernel_trace's LDS and scratch sizes per
+          workgroup;
+      hbm FETCH_DIR WRITE_DIR OUT.md [TITLE [COMMAND [KERNEL=BYTES ...]]]
+          FETCH_SIZE and WRITE_SIZE (KiB per dispatch) per kernel and grid,
+          scaled by what the same counters read for the calibration copy
+          (k_copy over 2^26 doubles: 512 MiB read and 512 MiB written);
+          KERNEL=BYTES pairs give algorithmic bytes per launch, and a JSON
+          beside OUT.md records the first such kernel's traffic."""
+    mode = argv[0]
+    if mode == "trace":
+        d, out = argv[1], argv[2]
+        title = argv[3] if len(argv) > 3 else "Kernel trace summary"
+        cmd = argv[4] if len(argv) > 4 else None
+        stats = _csv_rows(d, "kernel_stats.csv")
+        trace = _csv_rows(d, "kernel_trace.csv")
+        res = {}
+        for r in trace:
+            k = _short_kernel(_col(r, "kernel_name", default=""))
+            lds = int(float(_col(r, "lds_block_size", "group_segment_size", default=0)))
+            sc = int(float(_col(r, "scratch_size", "private_segment_size", default=0)))
+            res[k] = (lds, sc)
+        lines = [f"# {title}", ""]
+        if cmd:
+            lines += ["Command:", "", "    " + cmd, ""]
+        lines += [f"Summarised by `python bench.py --summarize-rocprof trace <dir> <out.md>` from `{os.path.basename(d.rstrip('/'))}`.", "",
+                  "| kernel | calls | avg us | min us | max us | share of GPU time | LDS B | scratch B |",
+                  "|---|---|---|---|---|---|---|---|"]
+        stats.sort(key=lambda r: -float(_col(r, "totaldurationns", default=0)))
+        for r in stats:
+            k = _short_kernel(_col(r, "name", "kernel_name", default=""))
+            lds, sc = res.get(k, ("-", "-"))
+            lines.append(f"| `{k}` | {int(float(_col(r, 'calls', default=0)))} | "
+                         f"{float(_col(r, 'averagens', default=0)) / 1e3:.1f} | "
+                         f"{float(_col(r, 'minns', default=0)) / 1e3:.1f} | {float(_col(r, 'maxns', default=0)) / 1e3:.1f} | "
+                         f"{float(_col(r, 'percentage', default=0)):.2f}% | {lds} | {sc} |")
+        with open(out, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        print(f"wrote {out} ({len(stats)} kernels)")
+        return
+    if mode == "hbm":
+        fd, wd, out = argv[1], argv[2], argv[3]
+        title = argv[4] if len(argv) > 4 else "HBM traffic per launch"
+        cmd = argv[5] if len(argv) > 5 else None
+        alg = {}
+        for a in argv[6:]:
+            k, _, v = a.rpartition("=")
+            alg[k] = float(v)
+
+        def per_kernel(d, counter):
+            acc = {}
+            for r in _csv_rows(d, "counter_collection.csv"):
+                if _col(r, "counter_name", default="") != counter:
+                    continue
+                k = (_short_kernel(_col(r, "kernel_name", default="")), int(float(_col(r, "grid_size", default=0))))
+                s_, n_ = acc.get(k, (0.0, 0))
+                acc[k] = (s_ + float(_col(r, "counter_value", default=0)), n_ + 1)
+            return {k: (v / n, n) for k, (v, n) in acc.items()}
+        fetch, write = per_kernel(fd, "FETCH_SIZE"), per_kernel(wd, "WRITE_SIZE")
+        calib = ("k_copy", 1 << 26)
+        if calib not in fetch or calib not in write:
+            raise SystemExit("no calibration copy (k_copy over 2^26 doubles) in the counter output")
+        fs, ws = 524288.0 / fetch[calib][0], 524288.0 / write[calib][0]
+        lines = [f"# {title}", ""]
+        if cmd:
+            lines += ["Commands:", ""] + ["    " + c for c in cmd.split(" ;; ")] + [""]
+        lines += [f"Summarised by `python bench.py --summarize-rocprof hbm <fetch dir> <write dir> <out.md>`.", "",
+                  f"Calibration: `k_copy` of 2^26 doubles (512 MiB read, 512 MiB written) reads FETCH_SIZE = "
+                  f"{fetch[calib][0]:.1f} KiB and WRITE_SIZE = {write[calib][0]:.1f} KiB per dispatch, so FETCH_SIZE "
+                  f"is scaled by {fs:.4f} and WRITE_SIZE by {ws:.4f}.", "",
+                  "| kernel (grid) | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes (scaled) | algorithmic bytes | traffic / algorithmic |",
+                  "|---|---|---|---|---|---|---|"]
+        rows = []
+        for k in fetch:
+            if k not in write:
+                continue
+            byts = 1024.0 * (fetch[k][0] * fs + write[k][0] * ws)
+            rows.append((byts, k))
+        rows.sort(reverse=True)
+        record = None
+        for byts, k in rows:
+            a = alg.get(k[0])
+            lines.append(f"| `{k[0]}` ({k[1]}) | {fetch[k][1]} | {fetch[k][0]:.1f} | {write[k][0]:.1f} | "
+                         f"{byts / 1e6:.1f} MB | {a / 1e6:.1f} MB | {byts / a:.3f} |" if a else
+                         f"| `{k[0]}` ({k[1]}) | {fetch[k][1]} | {fetch[k][0]:.1f} | {write[k][0]:.1f} | "
+                         f"{byts / 1e6:.1f} MB | - | - |")
+            if a and record is None and k[0] == next(iter(alg)):
+                record = {"kernel": k[0], "grid": k[1], "fetch_kib": fetch[k][0], "write_kib": write[k][0],
+                          "fetch_scale": fs, "write_scale": ws, "traffic_bytes_per_launch": byts,
+                          "algorithmic_bytes_per_launch": a, "source": os.path.basename(out)}
+        with open(out, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        if record is not None:
+            with open(out[:-3] + ".json", "w") as f:
+                json.dump(record, f, indent=1)
+        print(f"wrote {out} ({len(rows)} kernels)")
+        return
+    raise SystemExit("--summarize-rocprof: mode is 'trace' or 'hbm'")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,12 +492,29 @@ def main():
                     help="be128: the Backward Euler line (default); batched: the batched-solve workload")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--tune-cache", default=None, help="JSON schedule cache for the batched workload")
+    ap.add_argument("--no-oracle-check", action="store_true",
+                    help="skip the one-step comparison with the oracle (a CPU child process, ~1 min)")
+    ap.add_argument("--oracle-be-step", nargs=2, metavar=("N", "OUT"), help=argparse.SUPPRESS)
+    ap.add_argument("--summarize-rocprof", nargs="+", metavar="ARG",
+                    help="write profile summaries from rocprofv3 CSV output instead of running: "
+                         "'trace DIR OUT.md [TITLE]' or 'hbm FETCH_DIR WRITE_DIR OUT.md [TITLE]'")
     args = ap.parse_args()
+    if args.oracle_be_step:
+        oracle_be_step(int(args.oracle_be_step[0]), args.oracle_be_step[1])
+        return
+    if args.summarize_rocprof:
+        summarize_rocprof(args.summarize_rocprof)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    # the oracle's step runs in a child process beside the GPU work; it is
+    # started before this process touches the GPU
+    check = None
+    if args.workload == "be128" and not args.no_cpu and not args.no_oracle_check:
+        check = start_oracle_check(args.n)
     import residuum_hip as rh
     new_ev, rec_ev, ev_ms = hip_events()
     ctx = rh.Ctx(0)
@@ -302,6 +565,14 @@ def main():
     Ud = results["fp64"]["U"].numpy()
     Um = results["mixed"]["U"].numpy()
     mixed_vs_fp64_err = float(np.max(np.abs(Um - Ud)) / max(np.max(np.abs(Ud)), 1e-300))
+    # one step from T = 0 in each precision, for the comparison with the oracle
+    one = {}
+    if check is not None:
+        for mode in ("fp64", "mixed"):
+            U1 = pb.vector(0.0)
+            _, it1 = pb.be_run(U1, 0.0, dt, dt, tol=1e-8, max_iters=5000, mixed=mode == "mixed")
+            one[mode] = (U1.numpy(), int(it1[0]))
+            U1.free()
 
     # SpMV roofline: the dominant kernel of each path, timed alone on the same matrix
     K = pb.matrix()
@@ -398,12 +669,29 @@ def main():
     cpu = None
     if not args.no_cpu:
         b_host = np.ones(nfree)
-        rate, cit, cs = cpu_baseline(pb, K, b_host, args.cpu_iters)
+        cb = cpu_baseline(pb, K, b_host, args.cpu_iters, kappa, dt)
         model, ncpu = cpu_info()
-        cpu = {"value": rate, "unit": "DOF-iters/s", "cores": 1, "kind": "port",
-               "sample": f"{cit} Jacobi-PCG iterations of the oracle's serial reference-order CG on the same "
-                         f"{n}^3 BE operator ({nfree} DOFs), {cs:.1f} s on 1 of {ncpu} cores ({model})"}
-        print(f"# cpu: {rate:.3e} DOF-iters/s ({cit} iters in {cs:.1f}s)", file=sys.stderr, flush=True)
+        ncores = len(_cores())
+        if "ref_rate" in cb:
+            cpu = {"value": cb["ref_rate"], "unit": "DOF-iters/s", "cores": 1, "kind": "reference",
+                   "sample": f"{cb['ref_iters']} Jacobi-PCG iterations (differential: {args.cpu_iters} minus 20) of "
+                             f"the reference's cg::Solver on the same {n}^3 BE operator ({nfree} DOFs), "
+                             f"{cb['ref_s']:.1f} s pinned to core {cb['core']} of {ncores} ({model}, {ncpu} "
+                             f"online); the reference's assembleMatrix of K + M/dt over {pb.n_elem} elements "
+                             f"timed once",
+                   "port_value": cb["port_rate"], "port_over_reference_time": cb["ref_rate"] / cb["port_rate"],
+                   "assembly_elements_per_s": cb["ref_asm_elements_per_s"],
+                   "ref_vs_gpu_matrix_max_rel_diff": cb["ref_vs_gpu_matrix_max_rel_diff"]}
+        else:
+            cpu = {"value": cb["port_rate"], "unit": "DOF-iters/s", "cores": 1, "kind": "port",
+                   "sample": f"{cb['port_iters']} Jacobi-PCG iterations (differential) of the oracle's serial "
+                             f"reference-order CG on the same {n}^3 BE operator ({nfree} DOFs), {cb['port_s']:.1f} s "
+                             f"pinned to core {cb['core']} of {ncores} ({model})"}
+        print(f"# cpu: {cpu['value']:.3e} DOF-iters/s ({cpu['kind']})", file=sys.stderr, flush=True)
+    oracle_check = None
+    if check is not None:
+        oracle_check = finish_oracle_check(*check, one["fp64"][0], one["mixed"][0], one["fp64"][1], one["mixed"][1])
+        print(f"# oracle check: {oracle_check}", file=sys.stderr, flush=True)
 
     mx, fp = results["mixed"], results["fp64"]
     steps = args.steps
@@ -435,11 +723,13 @@ def main():
                      "ms_per_step": mx["assembly_ms"] / steps, "elements": pb.n_elem,
                      "work": "K + M/dt and M U_prev (one element pass), load, essential lift"},
         "mixed_vs_fp64_solution_max_rel_diff": mixed_vs_fp64_err,
+        "oracle_check": oracle_check,
         "spmv": spmv,
         "matrix_free": matrix_free,
         "roofline": {"bound": "hbm", "kernel": "k_spmv_f32<%d> (fp32 values, %s)" % (mode32, layout32),
                      "achieved": spmv["spmv_f32"]["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": spmv["spmv_f32"]["gbs"] / PEAK_HBM_GBS, "traffic": measured_traffic(n, layout32)},
+                     "frac": spmv["spmv_f32"]["gbs"] / PEAK_HBM_GBS,
+                     "traffic": measured_traffic(n, "k_spmv_f32<%d>" % mode32)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
