@@ -392,6 +392,7 @@ def summarize_rocprof(argv):
         d, out = argv[1], argv[2]
         title = argv[3] if len(argv) > 3 else "Kernel trace summary"
         cmd = argv[4] if len(argv) > 4 else None
+        notes = [a[5:] for a in argv[5:] if a.startswith("note:")]
         stats = _csv_rows(d, "kernel_stats.csv")
         trace = _csv_rows(d, "kernel_trace.csv")
         res = {}
@@ -403,7 +404,10 @@ def summarize_rocprof(argv):
         lines = [f"# {title}", ""]
         if cmd:
             lines += ["Command:", "", "    " + cmd, ""]
-        lines += [f"Summarised by `python bench.py --summarize-rocprof trace <dir> <out.md>` from `{os.path.basename(d.rstrip('/'))}`.", "",
+        lines += [f"Summarised by `python bench.py --summarize-rocprof trace <dir> <out.md>` from `{os.path.basename(d.rstrip('/'))}`.", ""]
+        for t in notes:
+            lines += [t, ""]
+        lines += [
                   "| kernel | calls | avg us | min us | max us | share of GPU time | LDS B | scratch B |",
                   "|---|---|---|---|---|---|---|---|"]
         stats.sort(key=lambda r: -float(_col(r, "totaldurationns", default=0)))
@@ -422,8 +426,11 @@ def summarize_rocprof(argv):
         fd, wd, out = argv[1], argv[2], argv[3]
         title = argv[4] if len(argv) > 4 else "HBM traffic per launch"
         cmd = argv[5] if len(argv) > 5 else None
-        alg = {}
+        alg, notes = {}, []
         for a in argv[6:]:
+            if a.startswith("note:"):
+                notes.append(a[5:])
+                continue
             k, _, v = a.rpartition("=")
             alg[k] = float(v)
 
@@ -447,8 +454,10 @@ def summarize_rocprof(argv):
         lines += [f"Summarised by `python bench.py --summarize-rocprof hbm <fetch dir> <write dir> <out.md>`.", "",
                   f"Calibration: `k_copy` of 2^26 doubles (512 MiB read, 512 MiB written) reads FETCH_SIZE = "
                   f"{fetch[calib][0]:.1f} KiB and WRITE_SIZE = {write[calib][0]:.1f} KiB per dispatch, so FETCH_SIZE "
-                  f"is scaled by {fs:.4f} and WRITE_SIZE by {ws:.4f}.", "",
-                  "| kernel (grid) | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes (scaled) | algorithmic bytes | traffic / algorithmic |",
+                  f"is scaled by {fs:.4f} and WRITE_SIZE by {ws:.4f}.", ""]
+        for t in notes:
+            lines += [t, ""]
+        lines += ["| kernel (grid) | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes (scaled) | algorithmic bytes | traffic / algorithmic |",
                   "|---|---|---|---|---|---|---|"]
         rows = []
         for k in fetch:
