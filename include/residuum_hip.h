@@ -195,6 +195,19 @@ int rsd_problem_layout_f32(rsd_problem* pb, int64_t* out);
  * the element scatter, colour by colour (frees the element buffers).
  * *in_use (nullable) receives 1 when the row gather is in use. */
 int rsd_problem_set_assembly(rsd_problem* pb, int32_t mode, int32_t* in_use);
+/* Element arithmetic (no reference counterpart).  RSD_ARITH_REFERENCE (the
+ * default): every element kernel follows the reference's chain of products
+ * and sums, each rounded separately -- bitwise the CPU loops in the same
+ * element order.  RSD_ARITH_CONTRACTED: the Backward Euler form pair
+ * (isotropic diffusion plus scaled mass, state-free model) -- its matrix
+ * with the mass vector, and its operand products (the matrix-free apply) --
+ * forms per-point factors once and accumulates in fused multiply-adds, about
+ * a quarter of the FP64 operations; entries then differ from the reference
+ * order by a few ulps of the largest entry of their row.  Every other form
+ * and kernel keeps the reference order. */
+#define RSD_ARITH_REFERENCE 0
+#define RSD_ARITH_CONTRACTED 1
+int rsd_problem_set_arithmetic(rsd_problem* pb, int32_t mode);
 /* padding the uniform stencil kernel needs around its x vector (no reference
  * counterpart): rsd_spmv_f32_padded reads x[-lo .. n + hi - 1], which must
  * be finite (zeros); 0 and 0 when the layout is not the uniform stencil */

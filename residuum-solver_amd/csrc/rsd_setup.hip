@@ -989,6 +989,15 @@ int rsd_problem_set_assembly(rsd_problem* pb, int32_t mode, int32_t* in_use) {
     });
 }
 
+int rsd_problem_set_arithmetic(rsd_problem* pb, int32_t mode) {
+    return guard([&] {
+        RSD_CHECK(pb, "rsd_problem_set_arithmetic: null problem");
+        RSD_CHECK(mode == RSD_ARITH_REFERENCE || mode == RSD_ARITH_CONTRACTED,
+                  "rsd_problem_set_arithmetic: mode must be RSD_ARITH_REFERENCE or RSD_ARITH_CONTRACTED");
+        pb->arith = mode;
+    });
+}
+
 int rsd_problem_create_block(rsd_ctx* ctx, int32_t dim, const int64_t* nel, const double* ext, const int32_t* p,
                              int32_t nq, int32_t n_bc, const int32_t* bc_tags, const char* const* bc_exprs,
                              rsd_problem** out) {

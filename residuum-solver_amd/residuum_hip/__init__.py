@@ -24,6 +24,7 @@ SCALED = 0x100
 SEL_VALUE, SEL_RATE, SEL_OPERAND = 0, 1, 2
 PC_IDENTITY, PC_JACOBI = 0, 1
 TOL_RELATIVE, TOL_ABSOLUTE = 0, 1
+ARITH_REFERENCE, ARITH_CONTRACTED = 0, 1
 
 _d = C.POINTER(C.c_double)
 _f = C.POINTER(C.c_float)
@@ -417,6 +418,12 @@ class Problem:
         n = C.c_int32()
         check(lib().rsd_problem_set_assembly(self.h, C.c_int32(mode), C.byref(n)))
         return n.value
+
+    def set_arithmetic(self, mode):
+        """Element arithmetic: ARITH_REFERENCE (default, the reference's
+        operation order, bitwise) or ARITH_CONTRACTED (the Backward Euler
+        form pair in fused multiply-adds)."""
+        check(lib().rsd_problem_set_arithmetic(self.h, C.c_int32(mode)))
 
     def padding_f32(self):
         """(lo, hi): the zeros rsd_spmv_f32_padded needs around x."""

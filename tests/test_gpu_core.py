@@ -235,6 +235,68 @@ def test_be_run_assembly_schemes_bitwise(ctx):
     assert mixed[0][0] == mixed[1][0] and np.array_equal(mixed[0][1], mixed[1][1])
 
 
+CONTRACT_CASES = [(3, (7, 6, 5), (1, 1, 1), 2, [0, 1], ["1", "0"], None),
+                  (2, (9, 7), (2, 2), 3, [0, 1, 2, 3], ["(1-x)*(1-y)"] * 4, None)]
+
+
+@pytest.mark.parametrize("case", CONTRACT_CASES)
+def test_contracted_arithmetic(ctx, case):
+    """RSD_ARITH_CONTRACTED, the Backward Euler form pair in fused
+    multiply-adds, against the oracle's reference-order arithmetic:
+    - matrix entries within 16 ulps of their row's largest |entry|, by both
+      assembly schemes, which agree with each other bit for bit;
+    - the matrix-free operand product within 16 ulps of sum_j |A_ij x_j|;
+    - Backward Euler (10 steps, CG to 1e-13, the fused mass vector) and the
+      matrix-free CG (to 1e-13) within 1e-11 of the oracle's solutions;
+    - back in RSD_ARITH_REFERENCE, the matrix is the oracle's bit for bit."""
+    import residuum_hip as rh
+    pb, oc = build(ctx, case, kappa_seed=29)
+    eps = np.finfo(np.float64).eps
+    dt = 2e-3
+    forms = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 1.0 / dt)]
+    ref = oc.assemble_matrix(forms)
+    rows = np.repeat(np.arange(oc.n_free), np.diff(oc.rp))
+    rowmax = np.zeros(oc.n_free)
+    np.maximum.at(rowmax, rows, np.abs(ref))
+    pb.set_arithmetic(rh.ARITH_CONTRACTED)
+    K = pb.matrix()
+    vals = []
+    for mode in (1, 0):
+        pb.set_assembly(mode)
+        pb.assemble_matrix(K, forms)
+        vals.append(K.values())
+    assert np.array_equal(bits(vals[0]), bits(vals[1]))
+    err = np.max(np.abs(vals[0] - ref) / rowmax[rows]) / eps
+    say(f"matrix: max error {err:.2f} ulps of the row's largest entry")
+    assert err <= 16
+    x = np.random.default_rng(6).standard_normal(oc.n_free)
+    y = pb.vector()
+    pb.fem_apply(y, forms, ctx.from_numpy(x))
+    yo = oc.assemble_vector(forms, operand=x, selector=O.SEL_OPERAND)
+    scale = np.zeros(oc.n_free)
+    np.add.at(scale, rows, np.abs(ref * x[oc.ci]))
+    erry = np.max(np.abs(y.numpy() - yo) / scale) / eps
+    say(f"operand product: max error {erry:.2f} ulps of sum |A_ij x_j|")
+    assert erry <= 16
+    out = oc.be_run(0.0, 10 * dt, dt, cg_tol=1e-13, cg_max=2000)
+    pb.set_assembly(1)
+    U = pb.vector(0.0)
+    steps, _ = pb.be_run(U, 0.0, 10 * dt, dt, tol=1e-13, max_iters=2000)
+    eU = np.max(np.abs(U.numpy() - out["U"])) / np.max(np.abs(out["U"]))
+    say(f"BE 10 steps: steps {steps}, max rel diff {eU:.2e}")
+    assert steps == out["steps"] == 10 and eU <= 1e-11
+    b = np.random.default_rng(7).standard_normal(oc.n_free)
+    dx = pb.vector(0.0)
+    rep = pb.cg_fem(ctx.from_numpy(b), dx, forms, tol=1e-13, max_iters=2000)
+    xo, ro = oc.cg_fem(forms, b, tol=1e-13, max_iters=2000)
+    ex = np.max(np.abs(dx.numpy() - xo)) / np.max(np.abs(xo))
+    say(f"matrix-free CG: iters {rep.iterations} / {ro.iterations}, max rel diff {ex:.2e}")
+    assert rep.converged and ex <= 1e-11
+    pb.set_arithmetic(rh.ARITH_REFERENCE)
+    pb.assemble_matrix(K, forms)
+    assert np.array_equal(bits(K.values()), bits(ref))
+
+
 @pytest.mark.parametrize("case", CASES[:2])
 def test_assemble_jacobian_tdep_bitexact(ctx, case):
     import residuum_hip as rh
