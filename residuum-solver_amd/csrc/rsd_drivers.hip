@@ -56,34 +56,6 @@ struct Pinned {
     ~Pinned() { if (p) (void)hipHostFree(p); }
 };
 
-struct Graph {
-    hipGraph_t g = nullptr;
-    hipGraphExec_t e = nullptr;
-    ~Graph() { reset(); }
-    void reset() {
-        if (e) (void)hipGraphExecDestroy(e);
-        if (g) (void)hipGraphDestroy(g);
-        e = nullptr;
-        g = nullptr;
-    }
-};
-
-template<typename F>
-void capture(hipStream_t st, Graph& G, F&& body) {
-    G.reset();
-    RSD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    try {
-        body();
-    } catch (...) {
-        hipGraph_t tmp = nullptr;
-        (void)hipStreamEndCapture(st, &tmp);
-        if (tmp) (void)hipGraphDestroy(tmp);
-        throw;
-    }
-    RSD_HIP(hipStreamEndCapture(st, &G.g));
-    RSD_HIP(hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0));
-}
-
 }  // namespace
 
 constexpr int64_t kChunk = 8;

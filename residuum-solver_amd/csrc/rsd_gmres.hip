@@ -219,7 +219,11 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
             res.iterations = 0;
         } else {
             int64_t total = 0;
-            for (;;) {
+            // one restart cycle: every launch returns at once after the
+            // state's stop flag is set, and its arguments are the same in
+            // every cycle, so from the second cycle on it is replayed from a
+            // graph (about 2 m + m^2 / 2 launches per cycle)
+            auto cycle = [&] {
                 hipLaunchKernelGGL(k_gm_reset_flags, dim3(1), dim3(1), 0, st, dst);
                 hipLaunchKernelGGL(k_gm_cycle_start, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, hq[0], g, m, dst);
                 for (int j = 0; j < m; ++j) {
@@ -244,6 +248,15 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
                                 A->val.p, b, x, hq[0], part);
                 hipLaunchKernelGGL(k_gm_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part, dst, 0);
                 RSD_HIP(hipGetLastError());
+            };
+            Graph cyc;
+            for (int64_t c = 0;; ++c) {
+                if (c == 0) {
+                    cycle();
+                } else {
+                    if (!cyc.e) capture(st, cyc, cycle);
+                    RSD_HIP(hipGraphLaunch(cyc.e, st));
+                }
                 read();
                 total = h->total;
                 const double beta = h->beta;
@@ -438,14 +451,10 @@ MixedGMResult gmres_mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, doub
             const int64_t cap = std::max<int64_t>(1, max_iters - total);
             launch_fill_f32(ctx, n, d, 0.0f);
             RSD_HIP(hipMemsetAsync(si, 0, sizeof(GMState), st));
-            for (int32_t first = 1;; first = 0) {
-                RSD_LAUNCH_COLS(S.mode(), k_gm32_inres, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(),
-                                A->val32f.p, r64, 1.0 / res, d, first, hq[0], part);
-                hipLaunchKernelGGL(k_gm_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part, si, first);
-                if (!first) {  // a restart: the true inner residual may already meet the tolerance
-                    read(si);
-                    if (h->beta / (h->res0 + 1e-50) < in_tol) break;
-                }
+            // one inner restart cycle; its arguments (in_tol, cap) are fixed
+            // for this refinement, so from the second restart on it is
+            // replayed from a graph
+            auto cycle = [&] {
                 hipLaunchKernelGGL(k_gm_reset_flags, dim3(1), dim3(1), 0, st, si);
                 hipLaunchKernelGGL(k_gm32_cycle_start, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, hq[0], g, m, si);
                 for (int j = 0; j < m; ++j) {
@@ -466,6 +475,22 @@ MixedGMResult gmres_mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, doub
                 hipLaunchKernelGGL(k_gm32_update_d, dim3((unsigned)nbe), dim3(kBlock), 0, st, n,
                                    (const float* const*)dQ, y, invd, d, si);
                 RSD_HIP(hipGetLastError());
+            };
+            Graph cyc;
+            for (int32_t first = 1;; first = 0) {
+                RSD_LAUNCH_COLS(S.mode(), k_gm32_inres, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(),
+                                A->val32f.p, r64, 1.0 / res, d, first, hq[0], part);
+                hipLaunchKernelGGL(k_gm_residual_fold, dim3(1), dim3(kFold), 0, st, nb, part, si, first);
+                if (!first) {  // a restart: the true inner residual may already meet the tolerance
+                    read(si);
+                    if (h->beta / (h->res0 + 1e-50) < in_tol) break;
+                }
+                if (first) {
+                    cycle();
+                } else {
+                    if (!cyc.e) capture(st, cyc, cycle);
+                    RSD_HIP(hipGraphLaunch(cyc.e, st));
+                }
                 read(si);
                 if (h->stop) break;  // tolerance, breakdown or the iteration cap
             }

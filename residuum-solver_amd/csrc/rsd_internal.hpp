@@ -266,6 +266,40 @@ __device__ inline double eval_expr(const Expr& e, double t, double x, double y, 
     return st[0];
 }
 
+// an instantiated hipGraph of work recorded on a stream (kernel launches
+// only: a memset node is never captured)
+struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    Graph() = default;
+    Graph(const Graph&) = delete;
+    Graph& operator=(const Graph&) = delete;
+    ~Graph() { reset(); }
+    void reset() {
+        if (e) (void)hipGraphExecDestroy(e);
+        if (g) (void)hipGraphDestroy(g);
+        e = nullptr;
+        g = nullptr;
+    }
+};
+
+// records body()'s launches on st into G (replacing what G held)
+template<typename F>
+void capture(hipStream_t st, Graph& G, F&& body) {
+    G.reset();
+    RSD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        body();
+    } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(st, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+        throw;
+    }
+    RSD_HIP(hipStreamEndCapture(st, &G.g));
+    RSD_HIP(hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0));
+}
+
 // ------------------------------------------------------------------------
 // device buffer helper
 // ------------------------------------------------------------------------
