@@ -189,8 +189,15 @@ public:
         if (transient) linalg::operations::copy(*U_, *U_prev_);
     }
 
+    // the device problem built for this mesh goes with it, so that a later
+    // mesh at the same address cannot match it; matrices still made on it
+    // (a solver's copy) are orphaned by the library and refuse further use
+    ~HeatProblem() {
+        if constexpr (std::is_same_v<BackendT, linalg::types::backend::Hip>) ProblemCache::release(mesh_);
+    }
     HeatProblem(const HeatProblem&) = delete;
     HeatProblem& operator=(const HeatProblem&) = delete;
+
 
     typename B::StiffnessForms stiffnessForms() const { return {}; }
     typename B::MassForms massForms() const { return {}; }

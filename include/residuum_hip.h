@@ -161,6 +161,9 @@ int rsd_block_mesh(int32_t dim, const int64_t* n, const double* ext, const int32
 int rsd_problem_create(rsd_ctx* ctx, int32_t dim, const int32_t* p, int32_t nq, int64_t n_nodes,
                        int64_t n_elem, const double* xyz, const int64_t* ien, const int32_t* rng,
                        int32_t n_bc, const int32_t* bc_tags, const char* const* bc_exprs, rsd_problem** out);
+/* Destroying a problem orphans the matrices made on it: every later call
+ * with such a matrix fails with RSD_ERR_ARG, and rsd_matrix_destroy still
+ * releases it. */
 int rsd_problem_destroy(rsd_problem* pb);
 /* the Gauss rule (nq points per direction, 1..5) of the element and face
  * integrals from now on; the mesh, DOF maps and pattern do not depend on it

@@ -1004,6 +1004,7 @@ int rsd_assemble_matrix(rsd_ctx* ctx, rsd_problem* pb, double time, int32_t nf, 
                         const double* coefs, const double* U, const double* Udot, rsd_matrix* K) {
     return guard([&] {
         RSD_CHECK(ctx && pb && K && (nf == 0 || kinds), "rsd_assemble_matrix: null argument");
+        RSD_LIVE_MATRIX(K);
         RSD_CHECK(ctx == pb->ctx, "problem belongs to another context");
         assemble_matrix(pb, time, nf, kinds, coefs, U, Udot, K);
     });

@@ -926,7 +926,9 @@ int rsd_batch_create(rsd_ctx* ctx, int32_t n_problems, rsd_matrix* const* mats, 
         bt->h_first.assign(n_problems + 1, 0);
         bt->h_nrows.assign(n_problems, 0);
         for (int i = 0; i < n_problems; ++i) {
-            RSD_CHECK(mats[i] && mats[i]->pb->ctx == ctx, "rsd_batch_create: matrix on another context");
+            RSD_CHECK(mats[i], "rsd_batch_create: null matrix");
+            RSD_LIVE_MATRIX(mats[i]);
+            RSD_CHECK(mats[i]->pb->ctx == ctx, "rsd_batch_create: matrix on another context");
             const Sell& S = mats[i]->pb->sell;
             sl[i] = S.slice_ptr.p;
             rp[i] = S.row_ptr.p;

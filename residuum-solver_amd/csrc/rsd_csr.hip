@@ -173,6 +173,7 @@ int rsd_matrix_from_csr(rsd_ctx* ctx, int64_t n_rows, const uint64_t* row_ptr, c
         pb->csr_slot.upload(slot.data(), (int64_t)nnz, st);
         auto m = std::make_unique<rsd_matrix>();
         m->pb = pb.get();
+        m->ctx = ctx;
         m->val.alloc(pb->sell.slots);
         if (pb->sell.slots) RSD_HIP(hipMemsetAsync(m->val.p, 0, sizeof(double) * pb->sell.slots, st));
         m->own_pb = std::move(pb);
@@ -184,6 +185,7 @@ int rsd_matrix_from_csr(rsd_ctx* ctx, int64_t n_rows, const uint64_t* row_ptr, c
 int rsd_matrix_load_csr(rsd_ctx* ctx, rsd_matrix* A, const double* val) {
     return guard([&] {
         RSD_CHECK(ctx && A && val, "rsd_matrix_load_csr: null argument");
+        RSD_LIVE_MATRIX(A);
         if (A->pb->csr_slot.n != A->pb->nnz) {  // a problem's matrix: slots from the sliced layout
             RSD_CHECK(ctx == A->pb->ctx, "matrix belongs to another context");
             launch_sell_csr(A, const_cast<double*>(val), false);

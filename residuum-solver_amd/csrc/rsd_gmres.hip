@@ -520,6 +520,7 @@ extern "C" int rsd_gmres_solve_mixed(rsd_ctx* ctx, rsd_matrix* A, const double* 
                                      rsd_report* rep, int64_t* outer_cycles) {
     return guard([&] {
         RSD_CHECK(ctx && A && b && x, "rsd_gmres_solve_mixed: null argument");
+        RSD_LIVE_MATRIX(A);
         RSD_CHECK(krylov >= 1 && krylov <= 512, "krylov dimension outside [1, 512]");
         RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
         const MixedGMResult r = gmres_mixed_run(ctx, A, b, x, tol, max_iters, krylov, precond,
@@ -540,6 +541,7 @@ extern "C" int rsd_gmres_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b,
                                    rsd_report* rep) {
     return guard([&] {
         RSD_CHECK(ctx && A && b && x, "rsd_gmres_solve_f64: null argument");
+        RSD_LIVE_MATRIX(A);
         RSD_CHECK(krylov >= 1 && krylov <= 512, "krylov dimension outside [1, 512]");
         RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
         RSD_CHECK(tol_type == RSD_TOL_RELATIVE || tol_type == RSD_TOL_ABSOLUTE, "unknown tolerance type");

@@ -39,6 +39,9 @@ void set_last_error(const std::string& msg);
         if (!(cond)) throw ::rsd::Error(RSD_ERR_ARG, msg);                                 \
     } while (0)
 
+// a matrix whose problem has been destroyed (rsd_problem_destroy orphans it)
+#define RSD_LIVE_MATRIX(m) RSD_CHECK((m)->pb != nullptr, "the matrix's problem has been destroyed")
+
 template<typename F>
 int guard(F&& f) {
     try {

@@ -577,6 +577,7 @@ int rsd_mul_f32(rsd_ctx* ctx, int64_t n, const float* x, const float* y, float* 
 int rsd_spmv_f64(rsd_ctx* ctx, rsd_matrix* A, const double* x, double* y) {
     return guard([&] {
         RSD_CHECK(ctx && A && x && y, "rsd_spmv_f64: null argument");
+        RSD_LIVE_MATRIX(A);
         launch_spmv_f64(ctx, A, x, y);
         RSD_HIP(hipGetLastError());
     });
@@ -585,6 +586,7 @@ int rsd_spmv_f64(rsd_ctx* ctx, rsd_matrix* A, const double* x, double* y) {
 int rsd_jacobi_inv_diag(rsd_ctx* ctx, rsd_matrix* A, double* inv) {
     return guard([&] {
         RSD_CHECK(ctx && A && inv, "rsd_jacobi_inv_diag: null argument");
+        RSD_LIVE_MATRIX(A);
         launch_inv_diag(ctx, A, inv);
         RSD_HIP(hipGetLastError());
     });
@@ -594,6 +596,7 @@ int rsd_cg_solve_f64(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, do
                      int64_t max_iters, int32_t precond, rsd_report* rep, double* hist) {
     return guard([&] {
         RSD_CHECK(ctx && A && b && x, "rsd_cg_solve_f64: null argument");
+        RSD_LIVE_MATRIX(A);
         RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
         RSD_CHECK(tol_type == RSD_TOL_RELATIVE || tol_type == RSD_TOL_ABSOLUTE, "unknown tolerance type");
         RSD_CHECK(max_iters >= 0, "max_iters must be >= 0");

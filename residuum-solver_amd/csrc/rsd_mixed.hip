@@ -483,6 +483,7 @@ extern "C" {
 int rsd_matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A) {
     return guard([&] {
         RSD_CHECK(ctx && A, "null argument");
+        RSD_LIVE_MATRIX(A);
         matrix_to_f32(ctx, A, true);
         RSD_HIP(hipGetLastError());
     });
@@ -491,6 +492,7 @@ int rsd_matrix_to_f32(rsd_ctx* ctx, rsd_matrix* A) {
 int rsd_spmv_f32(rsd_ctx* ctx, rsd_matrix* A, const float* x, float* y) {
     return guard([&] {
         RSD_CHECK(ctx && A && x && y, "null argument");
+        RSD_LIVE_MATRIX(A);
         RSD_CHECK(A->val32.n == A->pb->sell.slots32(), "rsd_spmv_f32: call rsd_matrix_to_f32 first");
         launch_spmv_f32(ctx, A, x, y, false);
         RSD_HIP(hipGetLastError());
@@ -500,6 +502,7 @@ int rsd_spmv_f32(rsd_ctx* ctx, rsd_matrix* A, const float* x, float* y) {
 int rsd_spmv_f32_padded(rsd_ctx* ctx, rsd_matrix* A, const float* x, float* y) {
     return guard([&] {
         RSD_CHECK(ctx && A && x && y, "null argument");
+        RSD_LIVE_MATRIX(A);
         RSD_CHECK(A->val32.n == A->pb->sell.slots32(), "rsd_spmv_f32_padded: call rsd_matrix_to_f32 first");
         launch_spmv_f32(ctx, A, x, y, true);
         RSD_HIP(hipGetLastError());
@@ -510,6 +513,7 @@ int rsd_cg_solve_mixed(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, 
                        int32_t precond, double inner_floor, rsd_report* rep, int64_t* outer_cycles) {
     return guard([&] {
         RSD_CHECK(ctx && A && b && x, "rsd_cg_solve_mixed: null argument");
+        RSD_LIVE_MATRIX(A);
         RSD_CHECK(precond == RSD_PC_IDENTITY || precond == RSD_PC_JACOBI, "unknown preconditioner");
         MixedWork w = mixed_workspace(ctx, A->pb->sell.n_rows, &A->pb->sell);
         MixedState s = mixed_run(ctx, A, b, x, tol, max_iters, precond,

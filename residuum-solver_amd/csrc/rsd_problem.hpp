@@ -5,6 +5,8 @@
 #include "rsd_internal.hpp"
 #include "rsd_rowdot.hpp"
 
+#include <unordered_set>
+
 namespace rsd {
 
 // Material model passed by value to the element kernels.
@@ -128,8 +130,17 @@ struct ReactionPlan;  // boundary-reaction plan, built on first use per tag list
 struct NaturalPlan;   // natural boundary condition plan, built on first use per tag list
 }
 
+struct rsd_matrix;
+
 struct rsd_problem {
+    rsd_problem() = default;
+    rsd_problem(const rsd_problem&) = delete;
+    rsd_problem& operator=(const rsd_problem&) = delete;
+    ~rsd_problem();
     rsd_ctx* ctx = nullptr;
+    // the matrices made on this problem (rsd_matrix_create): destroying the
+    // problem orphans them, and every entry point then refuses them
+    std::unordered_set<rsd_matrix*> matrices;
     int32_t dim = 0, p = 1, nq = 2, npe = 0, nfaces = 0;
     int64_t n_nodes = 0, n_elem = 0, n_free = 0, nnz = 0;
     int32_t max_row = 0;
@@ -198,7 +209,8 @@ struct rsd_problem {
 };
 
 struct rsd_matrix {
-    rsd_problem* pb = nullptr;
+    rsd_problem* pb = nullptr;  // null once the problem is destroyed
+    rsd_ctx* ctx = nullptr;     // the problem's context, for the matrix's own release
     std::unique_ptr<rsd_problem> own_pb;  // pattern-only problem of a matrix made from a CSR
     rsd::DBuf<double> val;    // [slots] SELL order
     rsd::DBuf<float> val32;   // [sell.slots32()] FP32 copy for the mixed-precision path

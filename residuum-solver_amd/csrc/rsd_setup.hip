@@ -1152,6 +1152,10 @@ int rsd_problem_set_quadrature(rsd_ctx* ctx, rsd_problem* pb, int32_t nq) {
     });
 }
 
+rsd_problem::~rsd_problem() {
+    for (rsd_matrix* m : matrices) m->pb = nullptr;
+}
+
 int rsd_problem_destroy(rsd_problem* pb) {
     return guard([&] {
         if (!pb) return;
@@ -1303,9 +1307,11 @@ int rsd_matrix_create(rsd_problem* pb, rsd_matrix** out) {
         RSD_CHECK(pb && out, "null argument");
         auto m = std::make_unique<rsd_matrix>();
         m->pb = pb;
+        m->ctx = pb->ctx;
         m->val.alloc(pb->sell.slots);
         if (pb->sell.slots) RSD_HIP(hipMemsetAsync(m->val.p, 0, sizeof(double) * pb->sell.slots, pb->ctx->stream));
         RSD_HIP(hipStreamSynchronize(pb->ctx->stream));
+        pb->matrices.insert(m.get());
         *out = m.release();
     });
 }
@@ -1313,7 +1319,8 @@ int rsd_matrix_create(rsd_problem* pb, rsd_matrix** out) {
 int rsd_matrix_destroy(rsd_matrix* m) {
     return guard([&] {
         if (!m) return;
-        (void)hipStreamSynchronize(m->pb->ctx->stream);
+        if (m->pb) m->pb->matrices.erase(m);
+        if (m->ctx) (void)hipStreamSynchronize(m->ctx->stream);
         delete m;
     });
 }
@@ -1321,6 +1328,7 @@ int rsd_matrix_destroy(rsd_matrix* m) {
 int rsd_matrix_zero(rsd_ctx* ctx, rsd_matrix* m) {
     return guard([&] {
         RSD_CHECK(ctx && m, "null argument");
+        RSD_LIVE_MATRIX(m);
         if (m->val.n) RSD_HIP(hipMemsetAsync(m->val.p, 0, sizeof(double) * m->val.n, ctx->stream));
     });
 }
@@ -1332,6 +1340,7 @@ static int64_t sell_slot(const rsd_problem* pb, int64_t r, int64_t j) {
 int rsd_matrix_get_values(rsd_ctx* ctx, rsd_matrix* m, double* out) {
     return guard([&] {
         RSD_CHECK(ctx && m && out, "null argument");
+        RSD_LIVE_MATRIX(m);
         const rsd_problem* pb = m->pb;
         std::vector<double> h(m->val.n);
         RSD_HIP(hipMemcpyAsync(h.data(), m->val.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
@@ -1344,6 +1353,7 @@ int rsd_matrix_get_values(rsd_ctx* ctx, rsd_matrix* m, double* out) {
 int rsd_matrix_store_csr(rsd_ctx* ctx, rsd_matrix* m, double* d_csr) {
     return guard([&] {
         RSD_CHECK(ctx && m && d_csr, "rsd_matrix_store_csr: null argument");
+        RSD_LIVE_MATRIX(m);
         RSD_CHECK(ctx == m->pb->ctx, "matrix belongs to another context");
         launch_sell_csr(m, d_csr, true);
     });
@@ -1352,6 +1362,7 @@ int rsd_matrix_store_csr(rsd_ctx* ctx, rsd_matrix* m, double* d_csr) {
 int rsd_matrix_set_values(rsd_ctx* ctx, rsd_matrix* m, const double* in) {
     return guard([&] {
         RSD_CHECK(ctx && m && in, "null argument");
+        RSD_LIVE_MATRIX(m);
         const rsd_problem* pb = m->pb;
         std::vector<double> h(m->val.n, 0.0);
         for (int64_t r = 0; r < pb->n_free; ++r)

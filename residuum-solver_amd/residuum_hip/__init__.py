@@ -382,8 +382,8 @@ class Problem:
          self.max_row) = [int(v) for v in s]
 
     def close(self):
-        """Release the matrices made on this problem, then the problem (a
-        matrix's destroy reads its problem)."""
+        """Release the matrices made on this problem, then the problem (the
+        library would otherwise orphan them: usable only for their release)."""
         if self.h:
             for m in list(getattr(self, "_matrices", ())):
                 m.close()

@@ -235,6 +235,25 @@ def test_be_run_assembly_schemes_bitwise(ctx):
     assert mixed[0][0] == mixed[1][0] and np.array_equal(mixed[0][1], mixed[1][1])
 
 
+def test_matrix_outliving_its_problem(ctx):
+    """Destroying a problem orphans the matrices made on it (a C++ solver's
+    copy may outlive the problem): a product with such a matrix fails with
+    RSD_ERR_ARG and says why, and its release still succeeds."""
+    import residuum_hip as rh
+    L = rh.lib()
+    pb = rh.Problem(ctx, 2, (4, 4), (1, 1), 2, [0], ["0"])
+    K = pb.matrix()
+    pb.assemble_matrix(K, [(rh.DIFFUSION, 1.0)])
+    x, y = pb.vector(1.0), pb.vector()
+    rh.check(L.rsd_spmv_f64(ctx.h, K.h, rh.dptr(x), rh.dptr(y)))
+    pb._matrices.discard(K)  # the binding would release the matrix first
+    pb.close()
+    assert L.rsd_spmv_f64(ctx.h, K.h, rh.dptr(x), rh.dptr(y)) == -1  # RSD_ERR_ARG
+    assert b"destroyed" in L.rsd_last_error_string()
+    assert L.rsd_matrix_destroy(K.h) == 0
+    K.h = None
+
+
 CONTRACT_CASES = [(3, (7, 6, 5), (1, 1, 1), 2, [0, 1], ["1", "0"], None),
                   (2, (9, 7), (2, 2), 3, [0, 1, 2, 3], ["(1-x)*(1-y)"] * 4, None)]
 
