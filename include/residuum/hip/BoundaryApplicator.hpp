@@ -19,6 +19,7 @@
 #pragma once
 
 #include <memory>
+#include <mutex>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -66,10 +67,16 @@ public:
         std::shared_ptr<const std::vector<Real>> nodal;    // or nodal vectors
     };
 
-    NaturalBoundaryRegistry() { directory()[static_cast<const Base*>(this)] = this; }
+    NaturalBoundaryRegistry() {
+        std::lock_guard<std::mutex> lk(mutex());
+        directory()[static_cast<const Base*>(this)] = this;
+    }
     NaturalBoundaryRegistry(const NaturalBoundaryRegistry&) = delete;
     NaturalBoundaryRegistry& operator=(const NaturalBoundaryRegistry&) = delete;
-    ~NaturalBoundaryRegistry() { directory().erase(static_cast<const Base*>(this)); }
+    ~NaturalBoundaryRegistry() {
+        std::lock_guard<std::mutex> lk(mutex());
+        directory().erase(static_cast<const Base*>(this));
+    }
 
     // FluxBoundaryForm with the flux vector q(t, x) (HeatProblem.tpp:96-110)
     void registerFlux(Int tag, const Expression<SD, SD>& q) {
@@ -109,15 +116,21 @@ public:
     }
 
     static const NaturalBoundaryRegistry* find(const Base* b) {
+        std::lock_guard<std::mutex> lk(mutex());
         auto& d = directory();
         auto it = d.find(b);
         return it == d.end() ? nullptr : it->second;
     }
 
 private:
+    // every live registry of this type, by its base address (guarded by mutex())
     static std::unordered_map<const Base*, const NaturalBoundaryRegistry*>& directory() {
         static std::unordered_map<const Base*, const NaturalBoundaryRegistry*> d;
         return d;
+    }
+    static std::mutex& mutex() {
+        static std::mutex m;
+        return m;
     }
     equation::heateq::evaluator::DefaultModel<EvalQPT> model_;
     std::vector<std::unique_ptr<FluxForms>> flux_;

@@ -57,6 +57,7 @@ static NaturalPlan& natural_plan(rsd_problem* pb, const std::vector<int32_t>& ta
     NaturalPlan& R = *plan;
     R.tags = tags;
     const int NF = pb->nfaces, npe = pb->npe, D = pb->dim;
+    RSD_CHECK(pb->n_elem * NF < (int64_t)INT32_MAX, "natural boundary conditions: face ids exceed 32 bits");
     std::vector<std::vector<int32_t>> fn(NF);
     for (int f = 0; f < NF; ++f) fn[f] = face_local_nodes(D, pb->p, f);
     const int npf = (int)fn[0].size();
