@@ -206,10 +206,19 @@ int rsd_problem_set_assembly(rsd_problem* pb, int32_t mode, int32_t* in_use);
  * with the mass vector, and its operand products (the matrix-free apply) --
  * forms per-point factors once and accumulates in fused multiply-adds, about
  * a quarter of the FP64 operations; entries then differ from the reference
- * order by a few ulps of the largest entry of their row.  Every other form
- * and kernel keeps the reference order. */
+ * order by a few ulps of the largest entry of their row.  RSD_ARITH_UNIFORM:
+ * as contracted, and on a mesh whose elements are all translates of
+ * element 0 (a block mesh; checked once, within 1e-10 of the element's
+ * extent) with the row gather in use, the pair's element matrix is
+ * kappa_e Kh + coef Mh with Kh and Mh formed once on element 0: the matrix
+ * comes from the row pass alone (no element pass, no element buffer) and
+ * the operand product from an 8 x 8 element product; values then differ
+ * from the per-element reference by the rounding of each element's own
+ * coordinates, about 1e-14 relative.  Every other form and kernel keeps the
+ * reference order. */
 #define RSD_ARITH_REFERENCE 0
 #define RSD_ARITH_CONTRACTED 1
+#define RSD_ARITH_UNIFORM 2
 int rsd_problem_set_arithmetic(rsd_problem* pb, int32_t mode);
 /* padding the uniform stencil kernel needs around its x vector (no reference
  * counterpart): rsd_spmv_f32_padded reads x[-lo .. n + hi - 1], which must

@@ -487,6 +487,151 @@ __global__ __launch_bounds__((ECfg<D, P, NQ, STATE, GRAD>::BLOCK)) void k_asm_ve
     if (row >= 0) A.F[row] += mode == 2 ? -1.0 * Fe : Fe;
 }
 
+// ------------------------------------------------------------------------
+// Uniform geometry (RSD_ARITH_UNIFORM).  On a mesh whose elements are all
+// translates of element 0 (a block mesh), the Backward Euler form pair's
+// element matrix is kappa_e Kh + coef Mh, with Kh (diffusion at unit
+// conductivity, the model's scalar and tensor kept) and Mh (mass) formed
+// once on element 0 in the reference's order.  uh holds Kh then Mh, each
+// [npe * npe] row-major.  The values differ from the per-element ones by
+// the rounding of each element's own coordinates, about 1e-14 relative.
+// ------------------------------------------------------------------------
+
+// the matrix row pass with no element pass: k_gather_rows with element row
+// a of element e formed in place, fma(kappa_e, Kh[a][b], coef Mh[a][b])
+template<int NPE, int MJ, bool MV, int AX = 0>
+__global__ __launch_bounds__(kSlice) void k_uniform_rows(const int32_t* __restrict__ gel,
+                                                         const uint32_t* __restrict__ gk, int32_t M,
+                                                         const double* __restrict__ uh,
+                                                         const double* __restrict__ kappa, double coef,
+                                                         const double* __restrict__ fb,
+                                                         const int64_t* __restrict__ slice_ptr, int64_t n_rows,
+                                                         double* __restrict__ val, double* __restrict__ mvF, Cols C,
+                                                         const double* __restrict__ axx, double* __restrict__ axo) {
+    constexpr int K4 = (NPE + 3) / 4;
+    extern __shared__ double acc[];  // [(W + NPE) * 64]: the row slots, then the dummy slots
+    const int lane = threadIdx.x;
+    const int64_t s = blockIdx.x;
+    const int64_t base = slice_ptr[s];
+    const int W = (int)((slice_ptr[s + 1] - base) / kSlice);
+    for (int k = 0; k < W; ++k) acc[k * kSlice + lane] = 0.0;
+    const int32_t* ge = gel + s * M * kSlice + lane;
+    const uint32_t* gw = gk + s * M * K4 * kSlice + lane;
+    int32_t q[MJ];
+    uint32_t kw[MJ][K4];
+#pragma unroll
+    for (int j = 0; j < MJ; ++j) {
+        q[j] = j < M ? ge[j * kSlice] : -1;
+#pragma unroll
+        for (int t = 0; t < K4; ++t) kw[j][t] = j < M ? gw[(j * K4 + t) * kSlice] : 0xffffffffu;
+    }
+    double kap[MJ];
+    double fv[MJ];
+#pragma unroll
+    for (int j = 0; j < MJ; ++j) {
+        const int64_t row = q[j] < 0 ? 0 : q[j];
+        kap[j] = kappa ? kappa[row / NPE] : 1.0;
+        if constexpr (MV) fv[j] = fb[row];
+    }
+    double fm = 0.0;
+#pragma unroll
+    for (int j = 0; j < MJ; ++j) {
+        const int a = (q[j] < 0 ? 0 : q[j]) % NPE;
+        const double* kh = uh + a * NPE;
+        const double* mh = uh + NPE * NPE + a * NPE;
+        int at[NPE];
+        double t[NPE];
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) {
+            const uint32_t k = (kw[j][b / 4] >> (8 * (b % 4))) & 0xffu;
+            at[b] = ((k == 0xffu) ? W + b : (int)k) * kSlice + lane;
+        }
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) t[b] = acc[at[b]];
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) t[b] += __builtin_fma(kap[j], kh[b], coef * mh[b]);
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) acc[at[b]] = t[b];
+        if constexpr (MV) fm += q[j] < 0 ? 0.0 : fv[j];
+    }
+    for (int k = 0; k < W; ++k) val[base + k * kSlice + lane] = acc[k * kSlice + lane];
+    if constexpr (MV) {
+        const int64_t r = s * kSlice + lane;
+        if (r < n_rows) mvF[r] = fm;
+    }
+    if constexpr (AX != 0) {
+        const int64_t r = s * kSlice + lane;
+        if (r < n_rows) {
+            double sum = 0.0;
+            if constexpr (AX == 1) {
+                sum = pat_rows<double>(r, C, [&](const int32_t* po, int len) {
+                    double t2 = 0.0;
+                    for (int j = 0; j < len; ++j) t2 += acc[j * kSlice + lane] * axx[(int)r + po[j]];
+                    return t2;
+                });
+            } else {
+                const int len = C.rp[r + 1] - C.rp[r];
+                for (int j = 0; j < len; ++j) sum += acc[j * kSlice + lane] * axx[C.col[base + (int64_t)j * kSlice + lane]];
+            }
+            axo[r] = sum;
+        }
+    }
+}
+
+// element vectors of the uniform pair, one thread per element, into
+// vb[e * NPE + a]: MODE 0 the operand product (kappa_e Kh + coef Mh) Ue with
+// the vector kernel's gathers (selector), MODE 1 the unit mass vector Mh Um
+// of the full gather of A.mvU
+template<int NPE, int MODE>
+__global__ __launch_bounds__(kBlock) void k_uniform_elems(int64_t n_elem, AsmArgs A, const double* __restrict__ uh,
+                                                          double* __restrict__ vb) {
+    if (MODE == 0 && A.stop && *A.stop) return;
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n_elem) return;
+    int32_t nodes[NPE];
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) nodes[a] = A.ien[e * NPE + a];
+    double Ue[NPE];
+    if constexpr (MODE == 1) {
+        gather<NPE>(A, nodes, 1, A.mvU, Ue);
+    } else {
+        if (A.selector == RSD_SEL_VALUE) gather<NPE>(A, nodes, 1, A.U, Ue);
+        else if (A.selector == RSD_SEL_RATE) gather<NPE>(A, nodes, 0, A.Udot, Ue);
+        else gather<NPE>(A, nodes, 0, A.operand, Ue);
+    }
+    const double kap = (MODE == 0 && A.kappa) ? A.kappa[e] : 1.0;
+    const double coef = A.coefs[1];
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < NPE; ++b) {
+            const double mh = uh[NPE * NPE + a * NPE + b];
+            const double v = MODE == 1 ? mh : __builtin_fma(kap, uh[a * NPE + b], coef * mh);
+            s = __builtin_fma(v, Ue[b], s);
+        }
+        vb[e * NPE + a] = s;
+    }
+}
+
+// out[r] = the row's element terms in colour order, from 0 (one lane per
+// row, the row gather tables)
+__global__ __launch_bounds__(kSlice) void k_sum_rows(const int32_t* __restrict__ gel, int32_t M,
+                                                     const double* __restrict__ vb, int64_t n_rows,
+                                                     double* __restrict__ out, const int32_t* stop) {
+    if (stop && *stop) return;
+    const int lane = threadIdx.x;
+    const int64_t s = blockIdx.x, r = s * kSlice + lane;
+    if (r >= n_rows) return;
+    const int32_t* ge = gel + s * M * kSlice + lane;
+    double sum = 0.0;
+    for (int j = 0; j < M; ++j) {
+        const int32_t q = ge[j * kSlice];
+        if (q >= 0) sum += vb[q];
+    }
+    out[r] = sum;
+}
+
 // the lift's row pass: each listed row adds its border elements' terms in
 // border (colour) order, F = ((F + t_1) + t_2) + ..., bitwise the scatter
 __global__ void k_lift_rows(int64_t nrows, const int32_t* __restrict__ rows, const int32_t* __restrict__ off,
@@ -580,7 +725,7 @@ static void run_matrix_t(rsd_problem* pb, const AsmArgs& A) {
         const dim3 grid((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB));
         if constexpr (!STATE && !DIAG) {
             if (pair) {
-                if (pb->arith == RSD_ARITH_CONTRACTED)
+                if (pb->arith != RSD_ARITH_REFERENCE)
                     hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, false, false, MV, false, true, true>), grid,
                                        dim3(C::BLOCK), 0, st, A, pb->model, c0, c1);
                 else
@@ -649,12 +794,133 @@ void gather_reserve(rsd_problem* pb, bool with_mass) {
     if (with_mass && pb->fbuf.n < ne * npe) pb->fbuf.alloc(ne * npe);
 }
 
+// every element a translate of element 0 (its node offsets from node 0
+// equal element 0's within 1e-10 of element 0's extent): checked once
+static bool uniform_geometry(rsd_problem* pb) {
+    if (pb->uniform_geom >= 0) return pb->uniform_geom == 1;
+    const int npe = pb->npe, D = pb->dim;
+    bool ok = pb->n_elem > 0 && (int64_t)pb->h_ien.size() == pb->n_elem * npe &&
+              (int64_t)pb->h_xyz.size() == pb->n_nodes * D;
+    std::vector<double> ref((size_t)npe * D);
+    double ext = 0.0;
+    if (ok) {
+        const int32_t* n0 = pb->h_ien.data();
+        for (int a = 0; a < npe; ++a)
+            for (int i = 0; i < D; ++i) {
+                ref[a * D + i] = pb->h_xyz[(size_t)n0[a] * D + i] - pb->h_xyz[(size_t)n0[0] * D + i];
+                ext = std::max(ext, std::fabs(ref[a * D + i]));
+            }
+        ok = ext > 0.0;
+    }
+    const double tol = 1e-10 * ext;
+    for (int64_t e = 1; ok && e < pb->n_elem; ++e) {
+        const int32_t* ne = pb->h_ien.data() + e * npe;
+        for (int a = 1; ok && a < npe; ++a)
+            for (int i = 0; i < D; ++i) {
+                const double d = pb->h_xyz[(size_t)ne[a] * D + i] - pb->h_xyz[(size_t)ne[0] * D + i];
+                if (!(std::fabs(d - ref[a * D + i]) <= tol)) { ok = false; break; }
+            }
+    }
+    pb->uniform_geom = ok ? 1 : 0;
+    return ok;
+}
+
+static bool use_uniform(rsd_problem* pb, const AsmArgs& A) {
+    return pb->arith == RSD_ARITH_UNIFORM && use_gather(pb) && form_pair(pb, A) && uniform_geometry(pb);
+}
+
+// Kh (unit conductivity) and Mh of element 0 by the element kernel, in the
+// reference's order, into pb->uhat
+template<int D, int P, int NQ>
+static const double* uniform_tables(rsd_problem* pb) {
+    using C = ECfg<D, P, NQ, false>;
+    constexpr int NPE = C::NPE;
+    if (pb->uhat.n < 2 * NPE * NPE) pb->uhat.alloc(2 * NPE * NPE);
+    for (int f = 0; f < 2; ++f) {
+        AsmArgs A = base_args(pb);
+        A.kappa = nullptr;
+        A.nf = 1;
+        A.kinds[0] = f == 0 ? RSD_FORM_DIFFUSION : RSD_FORM_MASS;
+        A.coefs[0] = 1.0;
+        A.ebuf = pb->uhat.p + f * NPE * NPE;
+        hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, false, false, false, true>), dim3(1), dim3(C::BLOCK), 0,
+                           pb->ctx->stream, A, pb->model, (int64_t)0, (int64_t)1);
+    }
+    RSD_HIP(hipGetLastError());
+    return pb->uhat.p;
+}
+
+template<int NPE, int MJ, bool MV>
+static void launch_uniform_rows(rsd_problem* pb, const AsmArgs& A, const double* uh) {
+    const Sell& S = pb->sell;
+    const dim3 grid((unsigned)S.n_slices), block(kSlice);
+    const size_t lds = (size_t)(pb->max_row + NPE) * kSlice * sizeof(double);
+    hipStream_t st = pb->ctx->stream;
+    const Cols C = S.cols();
+    const double coef = A.coefs[1];
+    if (MV && A.axo) {
+        if (S.has_pat)
+            hipLaunchKernelGGL((k_uniform_rows<NPE, MJ, true, 1>), grid, block, lds, st, pb->gat_el.p, pb->gat_k.p,
+                               pb->gat_m, uh, A.kappa, coef, A.fbuf, S.slice_ptr.p, S.n_rows, A.val, A.mvF, C, A.axx,
+                               A.axo);
+        else
+            hipLaunchKernelGGL((k_uniform_rows<NPE, MJ, true, 2>), grid, block, lds, st, pb->gat_el.p, pb->gat_k.p,
+                               pb->gat_m, uh, A.kappa, coef, A.fbuf, S.slice_ptr.p, S.n_rows, A.val, A.mvF, C, A.axx,
+                               A.axo);
+        return;
+    }
+    hipLaunchKernelGGL((k_uniform_rows<NPE, MJ, MV>), grid, block, lds, st, pb->gat_el.p, pb->gat_k.p, pb->gat_m, uh,
+                       A.kappa, coef, A.fbuf, S.slice_ptr.p, S.n_rows, A.val, A.mvF, C, A.axx, A.axo);
+}
+
+// the uniform pair's matrix (and mass vector): no element pass, no element
+// buffer; the mass vector's element terms by a light element pass
+template<int D, int P, int NQ, bool MV>
+static void run_matrix_uniform(rsd_problem* pb, AsmArgs A) {
+    constexpr int NPE = Elem<D, P>::NPE;
+    const double* uh = uniform_tables<D, P, NQ>(pb);
+    const int64_t ne = pb->n_elem;
+    if constexpr (MV) {
+        if (pb->fbuf.n < ne * NPE) pb->fbuf.alloc(ne * NPE);
+        A.fbuf = pb->fbuf.p;
+        hipLaunchKernelGGL((k_uniform_elems<NPE, 1>), dim3((unsigned)nblocks(ne)), dim3(kBlock), 0, pb->ctx->stream,
+                           ne, A, uh, A.fbuf);
+    }
+    const int M = pb->gat_m;
+    if (M <= 4) launch_uniform_rows<NPE, 4, MV>(pb, A, uh);
+    else if (M <= 8) launch_uniform_rows<NPE, 8, MV>(pb, A, uh);
+    else launch_uniform_rows<NPE, kGatherMaxElems, MV>(pb, A, uh);
+    RSD_HIP(hipGetLastError());
+}
+
+// the uniform pair applied to a state or operand (mode 0): element vectors,
+// then each row sums its elements' terms in colour order into A.F
+template<int D, int P, int NQ>
+static void run_vector_uniform(rsd_problem* pb, const AsmArgs& A) {
+    constexpr int NPE = Elem<D, P>::NPE;
+    const double* uh = uniform_tables<D, P, NQ>(pb);
+    const int64_t ne = pb->n_elem;
+    if (pb->vbuf.n < ne * NPE) pb->vbuf.alloc(ne * NPE);
+    hipStream_t st = pb->ctx->stream;
+    hipLaunchKernelGGL((k_uniform_elems<NPE, 0>), dim3((unsigned)nblocks(ne)), dim3(kBlock), 0, st, ne, A, uh,
+                       pb->vbuf.p);
+    hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)pb->sell.n_slices), dim3(kSlice), 0, st, pb->gat_el.p, pb->gat_m,
+                       pb->vbuf.p, pb->n_free, A.F, A.stop);
+    RSD_HIP(hipGetLastError());
+}
+
 // row gather assembly: the element pass (one launch over all elements),
 // then the row pass
 template<int D, int P, int NQ, bool STATE, bool MV>
 static void run_matrix_gather(rsd_problem* pb, AsmArgs A) {
     using C = ECfg<D, P, NQ, STATE>;
     constexpr int NPE = C::NPE;
+    if constexpr (!STATE) {
+        if (use_uniform(pb, A)) {
+            run_matrix_uniform<D, P, NQ, MV>(pb, A);
+            return;
+        }
+    }
     gather_reserve(pb, MV);
     A.ebuf = pb->ebuf.p;
     A.fbuf = MV ? pb->fbuf.p : nullptr;
@@ -663,7 +929,7 @@ static void run_matrix_gather(rsd_problem* pb, AsmArgs A) {
     bool done = false;
     if constexpr (!STATE) {
         if (form_pair(pb, A)) {
-            if (pb->arith == RSD_ARITH_CONTRACTED)
+            if (pb->arith != RSD_ARITH_REFERENCE)
                 hipLaunchKernelGGL((k_asm_matrix<D, P, NQ, false, false, MV, true, true, true>), grid, dim3(C::BLOCK),
                                    0, pb->ctx->stream, A, pb->model, (int64_t)0, ne);
             else
@@ -731,14 +997,20 @@ static void run_vector_t(rsd_problem* pb, AsmArgs A, const Expr& src) {
     }
     const std::vector<int64_t>& off = lift ? pb->bcolour_off : pb->colour_off;
     bool pair = false;
-    if constexpr (MODE == 0 && !STATE && GRAD) pair = form_pair(pb, A);
+    if constexpr (MODE == 0 && !STATE && GRAD) {
+        if (use_uniform(pb, A)) {
+            run_vector_uniform<D, P, NQ>(pb, A);
+            return;
+        }
+        pair = form_pair(pb, A);
+    }
     for (int c = 0; c < pb->n_colours(); ++c) {
         const int64_t c0 = off[c], c1 = off[c + 1];
         if (c1 <= c0) continue;
         const dim3 grid((unsigned)((c1 - c0 + C::EPB - 1) / C::EPB));
         if constexpr (MODE == 0 && !STATE && GRAD) {
             if (pair) {
-                if (pb->arith == RSD_ARITH_CONTRACTED)
+                if (pb->arith != RSD_ARITH_REFERENCE)
                     hipLaunchKernelGGL((k_asm_vector<D, P, NQ, false, true, 0, false, true, true>), grid,
                                        dim3(C::BLOCK), 0, st, A, pb->model, src, c0, c1);
                 else
@@ -850,6 +1122,22 @@ static bool forms_need_state(const rsd_problem* pb, const AsmArgs& A) {
         if (k == RSD_FORM_TDIFFUSION || k == RSD_FORM_TMASS) need = true;
     }
     return need;
+}
+
+// what assembling these forms (with the mass vector) will allocate on the
+// device, reserved now: callers that capture assembly into a graph call it
+// first, since no allocation may happen during capture
+void assembly_reserve(rsd_problem* pb, int32_t nf, const int32_t* kinds, bool with_mass) {
+    AsmArgs A = base_args(pb);
+    set_forms(A, nf, kinds, nullptr);
+    A.need_state = forms_need_state(pb, A);
+    if (use_uniform(pb, A)) {
+        const int64_t npe = pb->npe;
+        if (pb->uhat.n < 2 * npe * npe) pb->uhat.alloc(2 * npe * npe);
+        if (with_mass && pb->fbuf.n < pb->n_elem * npe) pb->fbuf.alloc(pb->n_elem * npe);
+    } else {
+        gather_reserve(pb, with_mass);
+    }
 }
 
 void assemble_matrix(rsd_problem* pb, double time, int32_t nf, const int32_t* kinds, const double* coefs,

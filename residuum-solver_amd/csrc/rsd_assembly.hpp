@@ -62,6 +62,9 @@ bool matrix_mass_fusable(const rsd_problem* pb, int32_t nf, const int32_t* kinds
 // stream capture that assembles matrices); with_mass: also the element mass
 // vectors of assemble_matrix_mass
 void gather_reserve(rsd_problem* pb, bool with_mass);
+// what assembling these forms (with the mass vector) will allocate, reserved
+// outside any stream capture
+void assembly_reserve(rsd_problem* pb, int32_t nf, const int32_t* kinds, bool with_mass);
 // the Gauss point table (shape values and weights), computed on the device
 // by the element kernels' own functions; part of problem set-up
 void build_shape_table(rsd_problem* pb);

@@ -112,7 +112,8 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
     rsd_matrix K;
     K.pb = pb;
     K.val.alloc(pb->sell.slots);
-    gather_reserve(pb, true);  // before any capture of the step's assembly
+    const int32_t be_kinds[2] = {RSD_FORM_DIFFUSION, RSD_FORM_MASS | RSD_FORM_SCALED};
+    assembly_reserve(pb, 2, be_kinds, true);  // before any capture of the step's assembly
     const Expr* src = cached_source(pb, src_text);
     const bool graphable = use_graph && !pb->bc_time_dependent && !(src && src->uses_t);
     CGWork w = cg_workspace(ctx, n, max_iters, false);

@@ -178,7 +178,10 @@ struct rsd_problem {
     // tables are not built
     int32_t gat_m = 0;            // element slots per row
     bool gather_on = true;        // matrices assemble by row gather when the tables exist
-    int32_t arith = 0;            // RSD_ARITH_REFERENCE or RSD_ARITH_CONTRACTED (the BE form pair)
+    int32_t arith = 0;            // RSD_ARITH_REFERENCE, _CONTRACTED or _UNIFORM (the BE form pair)
+    int32_t uniform_geom = -1;    // every element a translate of element 0: -1 not yet checked, 0 no, 1 yes
+    rsd::DBuf<double> uhat;       // [2 * npe * npe] Kh then Mh of element 0 (uniform geometry)
+    rsd::DBuf<double> vbuf;       // [n_elem * npe] element vectors of the uniform pair, reserved on first use
     rsd::DBuf<int32_t> gat_el;    // [n_slices * gat_m * 64] e * npe + a, -1 past the row's elements
     rsd::DBuf<uint32_t> gat_k;    // [n_slices * gat_m * k4 * 64] row slot position of column b (byte b % 4 of
                                   // dword b / 4, k4 = ceil(npe / 4)); 0xff: constrained column

@@ -992,8 +992,8 @@ int rsd_problem_set_assembly(rsd_problem* pb, int32_t mode, int32_t* in_use) {
 int rsd_problem_set_arithmetic(rsd_problem* pb, int32_t mode) {
     return guard([&] {
         RSD_CHECK(pb, "rsd_problem_set_arithmetic: null problem");
-        RSD_CHECK(mode == RSD_ARITH_REFERENCE || mode == RSD_ARITH_CONTRACTED,
-                  "rsd_problem_set_arithmetic: mode must be RSD_ARITH_REFERENCE or RSD_ARITH_CONTRACTED");
+        RSD_CHECK(mode == RSD_ARITH_REFERENCE || mode == RSD_ARITH_CONTRACTED || mode == RSD_ARITH_UNIFORM,
+                  "rsd_problem_set_arithmetic: mode must be RSD_ARITH_REFERENCE, _CONTRACTED or _UNIFORM");
         pb->arith = mode;
     });
 }

@@ -24,7 +24,7 @@ SCALED = 0x100
 SEL_VALUE, SEL_RATE, SEL_OPERAND = 0, 1, 2
 PC_IDENTITY, PC_JACOBI = 0, 1
 TOL_RELATIVE, TOL_ABSOLUTE = 0, 1
-ARITH_REFERENCE, ARITH_CONTRACTED = 0, 1
+ARITH_REFERENCE, ARITH_CONTRACTED, ARITH_UNIFORM = 0, 1, 2
 
 _d = C.POINTER(C.c_double)
 _f = C.POINTER(C.c_float)

@@ -316,6 +316,88 @@ def test_contracted_arithmetic(ctx, case):
     assert np.array_equal(bits(K.values()), bits(ref))
 
 
+@pytest.mark.parametrize("case", CONTRACT_CASES)
+def test_uniform_geometry_arithmetic(ctx, case):
+    """RSD_ARITH_UNIFORM on a block mesh: the Backward Euler pair's element
+    matrix as kappa_e Kh + coef Mh (Kh, Mh of element 0), against the
+    oracle's per-element reference order:
+    - matrix entries within 1e-12 of their row's largest |entry| (the
+      per-element coordinate rounding is about 1e-14 relative);
+    - the operand product within 1e-12 of sum_j |A_ij x_j|;
+    - Backward Euler (10 steps, CG to 1e-13, the fused mass vector) and the
+      matrix-free CG (to 1e-13) within 1e-10 of the oracle's solutions, and
+      the mixed-precision Backward Euler loop (with the row pass's product)
+      within 1e-7;
+    - on a mesh with one node moved (not uniform) the mode falls back to the
+      contracted arithmetic, bit for bit."""
+    import residuum_hip as rh
+    dt = 2e-3
+    # a Backward Euler run as the first work in this mode captures the step's
+    # assembly in a graph: everything the mode allocates is reserved before
+    pb0, _ = build(ctx, case, kappa_seed=31)
+    pb0.set_arithmetic(rh.ARITH_UNIFORM)
+    for mixed in (False, True):
+        U0 = pb0.vector(0.0)
+        pb0.be_run(U0, 0.0, 2 * dt, dt, tol=1e-10, use_graph=True, mixed=mixed)
+    pb0.close()
+    pb, oc = build(ctx, case, kappa_seed=31)
+    forms = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 1.0 / dt)]
+    ref = oc.assemble_matrix(forms)
+    rows = np.repeat(np.arange(oc.n_free), np.diff(oc.rp))
+    rowmax = np.zeros(oc.n_free)
+    np.maximum.at(rowmax, rows, np.abs(ref))
+    pb.set_arithmetic(rh.ARITH_UNIFORM)
+    assert pb.set_assembly(1) == 1
+    K = pb.matrix()
+    pb.assemble_matrix(K, forms)
+    err = np.max(np.abs(K.values() - ref) / rowmax[rows])
+    say(f"uniform matrix: max error {err:.2e} of the row's largest entry")
+    assert err <= 1e-12
+    x = np.random.default_rng(6).standard_normal(oc.n_free)
+    y = pb.vector()
+    pb.fem_apply(y, forms, ctx.from_numpy(x))
+    yo = oc.assemble_vector(forms, operand=x, selector=O.SEL_OPERAND)
+    scale = np.zeros(oc.n_free)
+    np.add.at(scale, rows, np.abs(ref * x[oc.ci]))
+    erry = np.max(np.abs(y.numpy() - yo) / scale)
+    say(f"uniform operand product: max error {erry:.2e} of sum |A_ij x_j|")
+    assert erry <= 1e-12
+    out = oc.be_run(0.0, 10 * dt, dt, cg_tol=1e-13, cg_max=2000)
+    U = pb.vector(0.0)
+    steps, _ = pb.be_run(U, 0.0, 10 * dt, dt, tol=1e-13, max_iters=2000)
+    eU = np.max(np.abs(U.numpy() - out["U"])) / np.max(np.abs(out["U"]))
+    Um = pb.vector(0.0)
+    pb.be_run(Um, 0.0, 10 * dt, dt, tol=1e-10, max_iters=2000, mixed=True)
+    eM = np.max(np.abs(Um.numpy() - out["U"])) / np.max(np.abs(out["U"]))
+    say(f"uniform BE 10 steps: FP64 max rel diff {eU:.2e}, mixed {eM:.2e}")
+    assert steps == out["steps"] == 10 and eU <= 1e-10 and eM <= 1e-7
+    b = np.random.default_rng(7).standard_normal(oc.n_free)
+    dx = pb.vector(0.0)
+    rep = pb.cg_fem(ctx.from_numpy(b), dx, forms, tol=1e-13, max_iters=2000)
+    xo, ro = oc.cg_fem(forms, b, tol=1e-13, max_iters=2000)
+    ex = np.max(np.abs(dx.numpy() - xo)) / np.max(np.abs(xo))
+    say(f"uniform matrix-free CG: iters {rep.iterations} / {ro.iterations}, max rel diff {ex:.2e}")
+    assert rep.converged and ex <= 1e-10
+    # one interior-free node moved by a hundredth of an element: not uniform
+    dim, n, p, nq, tags, exprs, ext = case
+    xyz = oc.xyz.copy().reshape(-1, dim)
+    node = oc.ien.reshape(oc.n_elem, -1)[oc.n_elem // 2][0]
+    xyz[node, 0] += 0.01 / n[0]
+    got = []
+    for arith in (rh.ARITH_UNIFORM, rh.ARITH_CONTRACTED):
+        q = rh.Problem.from_mesh(ctx, dim, p, nq, xyz.ravel(), oc.ien, oc.rng, tags, exprs)
+        q.set_conductivity(1.0, kappa=oc.kappa)
+        q.set_arithmetic(arith)
+        Kq = q.matrix()
+        q.assemble_matrix(Kq, forms)
+        yq = q.vector()
+        q.fem_apply(yq, forms, ctx.from_numpy(x))
+        got.append((bits(Kq.values()), bits(yq.numpy())))
+        Kq.close()
+        q.close()
+    assert np.array_equal(got[0][0], got[1][0]) and np.array_equal(got[0][1], got[1][1])
+
+
 @pytest.mark.parametrize("case", CASES[:2])
 def test_assemble_jacobian_tdep_bitexact(ctx, case):
     import residuum_hip as rh
