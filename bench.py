@@ -549,12 +549,15 @@ def main():
           f"{slots32} slots)", file=sys.stderr, flush=True)
 
     results = {}
-    # fp64 and mixed in the reference's element arithmetic; the mixed loop
-    # once more with the Backward Euler form pair contracted (fused
-    # multiply-adds), which changes the matrix by a few ulps
-    for mode in ("fp64", "mixed", "mixed_contracted"):
-        mixed = mode != "fp64"
-        pb.set_arithmetic(rh.ARITH_CONTRACTED if mode == "mixed_contracted" else rh.ARITH_REFERENCE)
+    # fp64 and mixed in the reference's element arithmetic (the FP64 loop is
+    # bitwise the oracle's); then both once more with the uniform-geometry
+    # arithmetic (the block mesh's element matrices from element 0's, within
+    # a few ulps of the per-element ones), which removes the element pass
+    arith = {"fp64": rh.ARITH_REFERENCE, "mixed": rh.ARITH_REFERENCE,
+             "fp64_uniform": rh.ARITH_UNIFORM, "mixed_uniform": rh.ARITH_UNIFORM}
+    for mode in ("fp64", "mixed", "fp64_uniform", "mixed_uniform"):
+        mixed = mode.startswith("mixed")
+        pb.set_arithmetic(arith[mode])
         U = pb.vector(0.0)
         if args.warmup:
             pb.be_run(U, 0.0, args.warmup * dt, dt, tol=1e-8, max_iters=5000, mixed=mixed)
@@ -657,15 +660,24 @@ def main():
             pb.fem_apply(yx, be_forms, bx)
         rec_ev(b_, stream)
         apply_ms = ev_ms(a, b_) / reps
-        pb.set_arithmetic(rh.ARITH_CONTRACTED)
-        pb.fem_apply(yx, be_forms, bx)
+        other = {}
+        for name, ar in (("contracted", rh.ARITH_CONTRACTED), ("uniform", rh.ARITH_UNIFORM)):
+            pb.set_arithmetic(ar)
+            pb.fem_apply(yx, be_forms, bx)
+            ctx.sync()
+            a, b_ = new_ev(), new_ev()
+            rec_ev(a, stream)
+            for _ in range(reps):
+                pb.fem_apply(yx, be_forms, bx)
+            rec_ev(b_, stream)
+            other[name] = ev_ms(a, b_) / reps
+        xs = pb.vector(0.0)
         ctx.sync()
         a, b_ = new_ev(), new_ev()
         rec_ev(a, stream)
-        for _ in range(reps):
-            pb.fem_apply(yx, be_forms, bx)
+        rep_u = pb.cg_fem(bx, xs, be_forms, tol=1e-8, max_iters=5000)
         rec_ev(b_, stream)
-        apply_contracted_ms = ev_ms(a, b_) / reps
+        solve_uniform_ms = ev_ms(a, b_)
         pb.set_arithmetic(rh.ARITH_REFERENCE)
         xs = pb.vector(0.0)
         ctx.sync()
@@ -675,7 +687,10 @@ def main():
         rec_ev(b_, stream)
         solve_ms = ev_ms(a, b_)
         matrix_free = {"apply_ms": apply_ms, "apply_over_spmv_f64": apply_ms * 1e3 / spmv["spmv_f64"]["avg_us"],
-                       "apply_contracted_ms": apply_contracted_ms,
+                       "apply_contracted_ms": other["contracted"], "apply_uniform_ms": other["uniform"],
+                       "apply_uniform_over_spmv_f64": other["uniform"] * 1e3 / spmv["spmv_f64"]["avg_us"],
+                       "uniform_cg_iters": int(rep_u.iterations), "uniform_solve_ms": solve_uniform_ms,
+                       "uniform_dof_iters_per_s": nfree * rep_u.iterations / (solve_uniform_ms * 1e-3),
                        "cg_iters": int(rep.iterations), "solve_ms": solve_ms,
                        "dof_iters_per_s": nfree * rep.iterations / (solve_ms * 1e-3)}
         print(f"# matrix-free: apply {apply_ms:.2f} ms, CG+Jacobi {rep.iterations} iterations in {solve_ms:.1f} ms",
@@ -718,7 +733,8 @@ def main():
         oracle_check = finish_oracle_check(*check, one["fp64"][0], one["mixed"][0], one["fp64"][1], one["mixed"][1])
         print(f"# oracle check: {oracle_check}", file=sys.stderr, flush=True)
 
-    mx, fp, mc = results["mixed"], results["fp64"], results["mixed_contracted"]
+    mx, fp = results["mixed"], results["fp64"]
+    fu, mu = results["fp64_uniform"], results["mixed_uniform"]
     steps = args.steps
     line = {
         "metric": "CG DOF-iters/s + SpMV HBM GB/s, mixed vs FP64 GPU at rel resid 1e-8",
@@ -739,8 +755,15 @@ def main():
                  "assembly_ms": fp["assembly_ms"], "ms_per_step": fp["total_ms"] / steps},
         "mixed": {"dof_iters_per_s": mx["dof_iters_per_s"], "iters": mx["iters"], "solve_ms": mx["solve_ms"],
                   "assembly_ms": mx["assembly_ms"], "ms_per_step": mx["total_ms"] / steps},
-        "mixed_contracted_assembly": {"iters": mc["iters"], "solve_ms": mc["solve_ms"],
-                                      "assembly_ms": mc["assembly_ms"], "ms_per_step": mc["total_ms"] / steps},
+        "uniform_geometry_arithmetic": {
+            "fp64": {"iters": fu["iters"], "solve_ms": fu["solve_ms"], "assembly_ms": fu["assembly_ms"],
+                     "ms_per_step": fu["total_ms"] / steps},
+            "mixed": {"iters": mu["iters"], "solve_ms": mu["solve_ms"], "assembly_ms": mu["assembly_ms"],
+                      "ms_per_step": mu["total_ms"] / steps, "dof_iters_per_s": mu["dof_iters_per_s"]},
+            "step_time_fp64_over_mixed": fu["total_ms"] / mu["total_ms"],
+            "elements_per_s": pb.n_elem * steps / (mu["assembly_ms"] * 1e-3),
+            "fp64_vs_reference_fp64_max_rel_diff": float(np.max(np.abs(fu["U"].numpy() - Ud)) /
+                                                         max(np.max(np.abs(Ud)), 1e-300))},
         "mixed_over_fp64": {"dof_iters_per_s": mx["dof_iters_per_s"] / fp["dof_iters_per_s"],
                             "solve_time": fp["solve_ms"] / mx["solve_ms"],
                             "step_time": fp["total_ms"] / mx["total_ms"]},
