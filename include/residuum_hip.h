@@ -214,8 +214,9 @@ int rsd_problem_set_assembly(rsd_problem* pb, int32_t mode, int32_t* in_use);
  * comes from the row pass alone (no element pass, no element buffer) and
  * the operand product from an 8 x 8 element product; values then differ
  * from the per-element reference by the rounding of each element's own
- * coordinates, about 1e-14 relative.  Every other form and kernel keeps the
- * reference order. */
+ * coordinates, about 1e-14 relative (measured within 3e-15); selecting it
+ * reserves its buffers (2 n_elem npe doubles), so no later call allocates.
+ * Every other form and kernel keeps the reference order. */
 #define RSD_ARITH_REFERENCE 0
 #define RSD_ARITH_CONTRACTED 1
 #define RSD_ARITH_UNIFORM 2

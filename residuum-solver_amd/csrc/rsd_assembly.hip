@@ -1124,6 +1124,17 @@ static bool forms_need_state(const rsd_problem* pb, const AsmArgs& A) {
     return need;
 }
 
+// the uniform-geometry arithmetic's device buffers (tables, element vectors
+// for the mass vector and for operand products), reserved when the mode is
+// selected so that no later call allocates, under stream capture or not
+void uniform_reserve(rsd_problem* pb) {
+    if (!use_gather(pb) || !uniform_geometry(pb)) return;
+    const int64_t npe = pb->npe, m = pb->n_elem * npe;
+    if (pb->uhat.n < 2 * npe * npe) pb->uhat.alloc(2 * npe * npe);
+    if (pb->fbuf.n < m) pb->fbuf.alloc(m);
+    if (pb->vbuf.n < m) pb->vbuf.alloc(m);
+}
+
 // what assembling these forms (with the mass vector) will allocate on the
 // device, reserved now: callers that capture assembly into a graph call it
 // first, since no allocation may happen during capture

@@ -995,6 +995,7 @@ int rsd_problem_set_arithmetic(rsd_problem* pb, int32_t mode) {
         RSD_CHECK(mode == RSD_ARITH_REFERENCE || mode == RSD_ARITH_CONTRACTED || mode == RSD_ARITH_UNIFORM,
                   "rsd_problem_set_arithmetic: mode must be RSD_ARITH_REFERENCE, _CONTRACTED or _UNIFORM");
         pb->arith = mode;
+        if (mode == RSD_ARITH_UNIFORM) uniform_reserve(pb);
     });
 }
 
