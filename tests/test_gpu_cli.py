@@ -118,6 +118,16 @@ def test_transient_paths_agree(ctx, tmp_path):
     nodal, t = pb.read_pndf(str(tmp_path / "out" / "T_000005.pndf"))
     assert abs(t - 0.1) < 1e-12
     assert np.array_equal(nodal, pb.nodal_field(ctx.from_numpy(s_fem["U"]), 0.1))
+    # the element arithmetic of the discretization node: the uniform-geometry
+    # pair on this block mesh, through both operators, within the solver
+    # tolerance of the reference order
+    d.pop("output")
+    d["discretization"]["arithmetic"] = "uniform"
+    for op, ref in (("csr", s_dev["U"]), ("fem", s_fem["U"])):
+        d["solver"]["linear"]["operator"]["type"] = op
+        s_u = cli.run_heat(write_cfg(tmp_path, d, "uniform_%s.yaml" % op), quiet=True, ctx=ctx)
+        say(op, "uniform arithmetic: max rel diff", np.max(np.abs(s_u["U"] - ref)) / np.max(np.abs(ref)))
+        assert s_u["steps"] == 5 and np.max(np.abs(s_u["U"] - ref)) <= 1e-11 * np.max(np.abs(ref))
     assert os.path.getsize(tmp_path / "out" / "T_000005.vtu") > 1000
 
 

@@ -98,6 +98,20 @@ def test_heat_config_parse():
                                  preconditioner="identity", precision="fp64")
 
 
+def test_heat_config_arithmetic(tmp_path):
+    """`discretization: arithmetic:` selects the element arithmetic; the
+    reference's order is the default."""
+    import yaml
+    from residuum_hip import cli
+    d = yaml.safe_load(open(os.path.join(GOLDEN, "heateq_square_n16.yaml")))
+    p = tmp_path / "c.yaml"
+    p.write_text(yaml.safe_dump(d))
+    assert cli.parse_heat_config(str(p))["arithmetic"] == "reference"
+    d["discretization"]["arithmetic"] = "uniform"
+    p.write_text(yaml.safe_dump(d))
+    assert cli.parse_heat_config(str(p))["arithmetic"] == "uniform"
+
+
 def test_heat_config_precision(tmp_path):
     """`precision: mixed` on a linear-solver node selects the mixed solves;
     fp64 is the default."""
@@ -123,6 +137,7 @@ def test_heat_config_precision(tmp_path):
     (lambda d: d.pop("solver"), "solver"),
     (lambda d: d["solver"]["linear"].update(precision="fp16"), "precision"),
     (lambda d: d["solver"]["linear"].update(precision="mixed"), "csr operator"),
+    (lambda d: d["discretization"].update(arithmetic="fast"), "arithmetic"),
 ])
 def test_heat_config_refusals(edit, msg, tmp_path):
     """What the device path does not run is refused with a message."""
