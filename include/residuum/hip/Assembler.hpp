@@ -145,6 +145,10 @@ public:
     rsd_problem* get() const { return pb_; }
     Index numFree() const { return n_free_; }
 
+    // the element arithmetic of this problem (RSD_ARITH_REFERENCE, the
+    // default and bitwise the CPU tag; RSD_ARITH_CONTRACTED; RSD_ARITH_UNIFORM)
+    void setArithmetic(int32_t mode) { check(rsd_problem_set_arithmetic(pb_, mode)); }
+
     void ensureQuadrature(Index nq) {
         if ((int32_t)nq == nq_) return;
         check(rsd_problem_set_quadrature(Context::get(), pb_, (int32_t)nq));
