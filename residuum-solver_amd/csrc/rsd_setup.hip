@@ -984,6 +984,8 @@ int rsd_problem_set_assembly(rsd_problem* pb, int32_t mode, int32_t* in_use) {
         if (!pb->gather_on) {  // the element buffers are of no further use
             pb->ebuf.reset();
             pb->fbuf.reset();
+        } else if (pb->arith == RSD_ARITH_UNIFORM) {
+            uniform_reserve(pb);  // so that no later call allocates
         }
         if (in_use) *in_use = pb->gather_on && pb->gat_m > 0 ? 1 : 0;
     });
