@@ -341,6 +341,73 @@ def run_batched(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
     print(json.dumps(line), flush=True)
 
 
+def renumbered_workload(rh, ctx, stream, n, kappa_elem, dt, capture, new_ev, rec_ev, ev_ms, reps, bytes_of):
+    """The benchmark's operator on the same 128^3 mesh with its nodes
+    renumbered in 8^3-node tiles (tile-major, lexicographic inside a tile):
+    the locality of a good unstructured ordering, but the rows no longer
+    share a few hundred column-offset patterns, so the sliced ELL falls back
+    to explicit columns and the FP32 copy to the full layout.  Times the
+    FP64 and FP32 products and one step's FP64 and mixed solves."""
+    xyz, ien, rng = rh.block_mesh(3, (n, n, n), (0.0, 1.0, 0.0, 1.0, 0.0, 1.0), (1, 1, 1))
+    m = n + 1
+    idx = np.arange(m ** 3)
+    i, j, k = idx % m, (idx // m) % m, idx // (m * m)
+    t = 8
+    nt = (m + t - 1) // t
+    tile = ((k // t) * nt + (j // t)) * nt + (i // t)
+    inner = ((k % t) * t + (j % t)) * t + (i % t)
+    order = np.lexsort((inner, tile))       # old node ids in the new order
+    new_of = np.empty_like(order)
+    new_of[order] = np.arange(order.size)
+    xyz2 = xyz.reshape(-1, 3)[order].ravel()
+    pb = rh.Problem.from_mesh(ctx, 3, (1, 1, 1), 2, xyz2, new_of[ien], rng, [0, 1], ["1", "0"])
+    pb.set_conductivity(1.0, kappa=kappa_elem)
+    pb.set_mass(1.0, 1.0)
+    nfree, nnz = pb.n_free, pb.nnz
+    col_mode, _, _, _ = pb.layout()
+    mode32, _, _, _ = pb.layout_f32()
+    K = pb.matrix()
+    pb.assemble_matrix(K, [(rh.DIFFUSION, 1.0), (rh.MASS | rh.SCALED, 1.0 / dt)], time=dt)
+    K.to_f32()
+    L = rh.lib()
+    x64, y64 = pb.vector(1.0), pb.vector()
+    x32, y32 = ctx.empty(nfree, np.float32), ctx.empty(nfree, np.float32)
+    ctx.fill(x32, 1.0)
+    out = {"numbering": "8^3-node tiles", "layout_f64": LAYOUTS[col_mode], "layout_f32": LAYOUTS[mode32]}
+    for name, fn, byts in (("spmv_f64", lambda: rh.check(L.rsd_spmv_f64(ctx.h, K.h, rh.dptr(x64), rh.dptr(y64))),
+                            bytes_of(col_mode, 8)),
+                           ("spmv_f32", lambda: rh.check(L.rsd_spmv_f32(ctx.h, K.h, rh.dptr(x32), rh.dptr(y32))),
+                            bytes_of(mode32, 4))):
+        for _ in range(3):
+            fn()
+        ctx.sync()
+        replay = capture(fn, reps, stream)
+        replay()
+        ctx.sync()
+        a, b = new_ev(), new_ev()
+        rec_ev(a, stream)
+        replay()
+        rec_ev(b, stream)
+        us = ev_ms(a, b) / reps * 1e3
+        out[name] = {"avg_us": us, "bytes": byts, "gbs": byts / us * 1e-3, "frac": byts / us * 1e-3 / PEAK_HBM_GBS}
+    # one Backward Euler step's solves from T = 0, both precisions
+    for mixed in (False, True):
+        U = pb.vector(0.0)
+        steps, iters = pb.be_run(U, 0.0, dt, dt, tol=1e-8, max_iters=5000, mixed=mixed)
+        tm = pb.last_timing_ms
+        out["mixed" if mixed else "fp64"] = {"iters": int(iters[0]), "solve_ms": tm["solve"],
+                                             "dof_iters_per_s": nfree * int(iters[0]) / (tm["solve"] * 1e-3)}
+    out["mixed_over_fp64_dof_iters_per_s"] = out["mixed"]["dof_iters_per_s"] / out["fp64"]["dof_iters_per_s"]
+    print(f"# renumbered mesh: f64 {LAYOUTS[col_mode]} {out['spmv_f64']['avg_us']:.1f} us "
+          f"({out['spmv_f64']['frac']:.2f} of peak), f32 {LAYOUTS[mode32]} {out['spmv_f32']['avg_us']:.1f} us "
+          f"({out['spmv_f32']['frac']:.2f}); one step: fp64 {out['fp64']['iters']} iters "
+          f"{out['fp64']['solve_ms']:.1f} ms, mixed {out['mixed']['iters']} {out['mixed']['solve_ms']:.1f} ms",
+          file=sys.stderr, flush=True)
+    K.close()
+    pb.close()
+    return out
+
+
 def _short_kernel(name):
     """`void rsd::k_spmv_f64<2>(long, ...)` -> `k_spmv_f64<2>`."""
     name = name.strip().strip('"')
@@ -501,6 +568,8 @@ def main():
                     help="be128: the Backward Euler line (default); batched: the batched-solve workload")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--tune-cache", default=None, help="JSON schedule cache for the batched workload")
+    ap.add_argument("--no-renumbered", action="store_true",
+                    help="skip the same operator on a renumbered mesh (the fallback layouts)")
     ap.add_argument("--no-oracle-check", action="store_true",
                     help="skip the one-step comparison with the oracle (a CPU child process, ~1 min)")
     ap.add_argument("--oracle-be-step", nargs=2, metavar=("N", "OUT"), help=argparse.SUPPRESS)
@@ -696,6 +765,11 @@ def main():
         print(f"# matrix-free: apply {apply_ms:.2f} ms, CG+Jacobi {rep.iterations} iterations in {solve_ms:.1f} ms",
               file=sys.stderr, flush=True)
 
+    renumbered = None
+    if not args.no_renumbered:
+        renumbered = renumbered_workload(rh, ctx, stream, n, kappa, dt, capture, new_ev, rec_ev, ev_ms, args.spmv_reps,
+                                         bytes_of)
+
     # calibration copy of known size (2^26 doubles = 512 MiB each way): the
     # FETCH_SIZE / WRITE_SIZE counters of this dispatch scale the SpMV's
     calib_n = 1 << 26
@@ -776,6 +850,7 @@ def main():
         "oracle_check": oracle_check,
         "spmv": spmv,
         "matrix_free": matrix_free,
+        "renumbered_mesh": renumbered,
         "roofline": {"bound": "hbm", "kernel": "k_spmv_f32<%d> (fp32 values, %s)" % (mode32, layout32),
                      "achieved": spmv["spmv_f32"]["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": spmv["spmv_f32"]["gbs"] / PEAK_HBM_GBS,
