@@ -578,10 +578,12 @@ __global__ __launch_bounds__(kSlice) void k_uniform_rows(const int32_t* __restri
     }
 }
 
-// element vectors of the uniform pair, one thread per element, into
-// vb[e * NPE + a]: MODE 0 the operand product (kappa_e Kh + coef Mh) Ue with
-// the vector kernel's gathers (selector), MODE 1 the unit mass vector Mh Um
-// of the full gather of A.mvU
+// element vectors of the uniform pair, one thread per element: MODE 0 the
+// operand product (kappa_e Kh + coef Mh) Ue with the vector kernel's gathers
+// (selector) into vb[a * n_elem + e] (by local node, so that both the
+// stores here and the row sums' loads are coalesced), MODE 1 the unit mass
+// vector Mh Um of the full gather of A.mvU into vb[e * NPE + a] (the mass
+// vectors' layout the row passes read)
 template<int NPE, int MODE>
 __global__ __launch_bounds__(kBlock) void k_uniform_elems(int64_t n_elem, AsmArgs A, const double* __restrict__ uh,
                                                           double* __restrict__ vb) {
@@ -610,13 +612,15 @@ __global__ __launch_bounds__(kBlock) void k_uniform_elems(int64_t n_elem, AsmArg
             const double v = MODE == 1 ? mh : __builtin_fma(kap, uh[a * NPE + b], coef * mh);
             s = __builtin_fma(v, Ue[b], s);
         }
-        vb[e * NPE + a] = s;
+        if constexpr (MODE == 1) vb[e * NPE + a] = s;
+        else vb[a * n_elem + e] = s;
     }
 }
 
 // out[r] = the row's element terms in colour order, from 0 (one lane per
-// row, the row gather tables)
-__global__ __launch_bounds__(kSlice) void k_sum_rows(const int32_t* __restrict__ gel, int32_t M,
+// row, the row gather tables; vb by local node, vb[a * n_elem + e])
+template<int NPE>
+__global__ __launch_bounds__(kSlice) void k_sum_rows(const int32_t* __restrict__ gel, int32_t M, int64_t n_elem,
                                                      const double* __restrict__ vb, int64_t n_rows,
                                                      double* __restrict__ out, const int32_t* stop) {
     if (stop && *stop) return;
@@ -627,7 +631,7 @@ __global__ __launch_bounds__(kSlice) void k_sum_rows(const int32_t* __restrict__
     double sum = 0.0;
     for (int j = 0; j < M; ++j) {
         const int32_t q = ge[j * kSlice];
-        if (q >= 0) sum += vb[q];
+        if (q >= 0) sum += vb[(int64_t)(q % NPE) * n_elem + q / NPE];
     }
     out[r] = sum;
 }
@@ -904,8 +908,8 @@ static void run_vector_uniform(rsd_problem* pb, const AsmArgs& A) {
     hipStream_t st = pb->ctx->stream;
     hipLaunchKernelGGL((k_uniform_elems<NPE, 0>), dim3((unsigned)nblocks(ne)), dim3(kBlock), 0, st, ne, A, uh,
                        pb->vbuf.p);
-    hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)pb->sell.n_slices), dim3(kSlice), 0, st, pb->gat_el.p, pb->gat_m,
-                       pb->vbuf.p, pb->n_free, A.F, A.stop);
+    hipLaunchKernelGGL(k_sum_rows<NPE>, dim3((unsigned)pb->sell.n_slices), dim3(kSlice), 0, st, pb->gat_el.p,
+                       pb->gat_m, ne, pb->vbuf.p, pb->n_free, A.F, A.stop);
     RSD_HIP(hipGetLastError());
 }
 
