@@ -504,7 +504,7 @@ __global__ __launch_bounds__(kSlice) void k_uniform_rows(const int32_t* __restri
                                                          const uint32_t* __restrict__ gk, int32_t M,
                                                          const double* __restrict__ uh,
                                                          const double* __restrict__ kappa, double coef,
-                                                         const double* __restrict__ fb,
+                                                         const double* __restrict__ fb, int64_t n_elem,
                                                          const int64_t* __restrict__ slice_ptr, int64_t n_rows,
                                                          double* __restrict__ val, double* __restrict__ mvF, Cols C,
                                                          const double* __restrict__ axx, double* __restrict__ axo) {
@@ -531,7 +531,7 @@ __global__ __launch_bounds__(kSlice) void k_uniform_rows(const int32_t* __restri
     for (int j = 0; j < MJ; ++j) {
         const int64_t row = q[j] < 0 ? 0 : q[j];
         kap[j] = kappa ? kappa[row / NPE] : 1.0;
-        if constexpr (MV) fv[j] = fb[row];
+        if constexpr (MV) fv[j] = fb[(row % NPE) * n_elem + row / NPE];  // by local node
     }
     double fm = 0.0;
 #pragma unroll
@@ -578,12 +578,11 @@ __global__ __launch_bounds__(kSlice) void k_uniform_rows(const int32_t* __restri
     }
 }
 
-// element vectors of the uniform pair, one thread per element: MODE 0 the
-// operand product (kappa_e Kh + coef Mh) Ue with the vector kernel's gathers
-// (selector) into vb[a * n_elem + e] (by local node, so that both the
-// stores here and the row sums' loads are coalesced), MODE 1 the unit mass
-// vector Mh Um of the full gather of A.mvU into vb[e * NPE + a] (the mass
-// vectors' layout the row passes read)
+// element vectors of the uniform pair, one thread per element, into
+// vb[a * n_elem + e] (by local node, so that both the stores here and the
+// row passes' loads are coalesced): MODE 0 the operand product (kappa_e Kh +
+// coef Mh) Ue with the vector kernel's gathers (selector), MODE 1 the unit
+// mass vector Mh Um of the full gather of A.mvU
 template<int NPE, int MODE>
 __global__ __launch_bounds__(kBlock) void k_uniform_elems(int64_t n_elem, AsmArgs A, const double* __restrict__ uh,
                                                           double* __restrict__ vb) {
@@ -612,8 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_uniform_elems(int64_t n_elem, AsmArg
             const double v = MODE == 1 ? mh : __builtin_fma(kap, uh[a * NPE + b], coef * mh);
             s = __builtin_fma(v, Ue[b], s);
         }
-        if constexpr (MODE == 1) vb[e * NPE + a] = s;
-        else vb[a * n_elem + e] = s;
+        vb[a * n_elem + e] = s;
     }
 }
 
@@ -865,16 +863,16 @@ static void launch_uniform_rows(rsd_problem* pb, const AsmArgs& A, const double*
     if (MV && A.axo) {
         if (S.has_pat)
             hipLaunchKernelGGL((k_uniform_rows<NPE, MJ, true, 1>), grid, block, lds, st, pb->gat_el.p, pb->gat_k.p,
-                               pb->gat_m, uh, A.kappa, coef, A.fbuf, S.slice_ptr.p, S.n_rows, A.val, A.mvF, C, A.axx,
+                               pb->gat_m, uh, A.kappa, coef, A.fbuf, pb->n_elem, S.slice_ptr.p, S.n_rows, A.val, A.mvF, C, A.axx,
                                A.axo);
         else
             hipLaunchKernelGGL((k_uniform_rows<NPE, MJ, true, 2>), grid, block, lds, st, pb->gat_el.p, pb->gat_k.p,
-                               pb->gat_m, uh, A.kappa, coef, A.fbuf, S.slice_ptr.p, S.n_rows, A.val, A.mvF, C, A.axx,
+                               pb->gat_m, uh, A.kappa, coef, A.fbuf, pb->n_elem, S.slice_ptr.p, S.n_rows, A.val, A.mvF, C, A.axx,
                                A.axo);
         return;
     }
     hipLaunchKernelGGL((k_uniform_rows<NPE, MJ, MV>), grid, block, lds, st, pb->gat_el.p, pb->gat_k.p, pb->gat_m, uh,
-                       A.kappa, coef, A.fbuf, S.slice_ptr.p, S.n_rows, A.val, A.mvF, C, A.axx, A.axo);
+                       A.kappa, coef, A.fbuf, pb->n_elem, S.slice_ptr.p, S.n_rows, A.val, A.mvF, C, A.axx, A.axo);
 }
 
 // the uniform pair's matrix (and mass vector): no element pass, no element
