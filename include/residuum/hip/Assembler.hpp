@@ -146,7 +146,8 @@ public:
     Index numFree() const { return n_free_; }
 
     // the element arithmetic of this problem (RSD_ARITH_REFERENCE, the
-    // default and bitwise the CPU tag; RSD_ARITH_CONTRACTED; RSD_ARITH_UNIFORM)
+    // default and bitwise the CPU tag; RSD_ARITH_CONTRACTED; RSD_ARITH_UNIFORM;
+    // RSD_ARITH_FACTORED)
     void setArithmetic(int32_t mode) { check(rsd_problem_set_arithmetic(pb_, mode)); }
 
     void ensureQuadrature(Index nq) {

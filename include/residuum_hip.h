@@ -216,10 +216,20 @@ int rsd_problem_set_assembly(rsd_problem* pb, int32_t mode, int32_t* in_use);
  * from the per-element reference by the rounding of each element's own
  * coordinates, about 1e-14 relative (measured within 3e-15); selecting it
  * reserves its buffers (2 n_elem npe doubles), so no later call allocates.
- * Every other form and kernel keeps the reference order. */
+ * RSD_ARITH_FACTORED: as contracted, and on Q1 elements of any shape with
+ * the row gather in use, the pair's operand product (the matrix-free apply,
+ * FEMOperator.hpp:38-56) is formed without element matrices: per
+ * quadrature point the geometry, the operand's value and gradient from the
+ * element's multilinear coefficients (sum factorisation of the Q1 basis),
+ * then the flux and mass terms tested in the same coefficients; element
+ * vectors are summed by row in colour order.  Products differ from the
+ * reference order by rounding, about 1e-15 of sum_j |A_ij x_j|; selecting
+ * it reserves n_elem npe doubles.  Every other form and kernel keeps the
+ * reference order. */
 #define RSD_ARITH_REFERENCE 0
 #define RSD_ARITH_CONTRACTED 1
 #define RSD_ARITH_UNIFORM 2
+#define RSD_ARITH_FACTORED 3
 int rsd_problem_set_arithmetic(rsd_problem* pb, int32_t mode);
 /* padding the uniform stencil kernel needs around its x vector (no reference
  * counterpart): rsd_spmv_f32_padded reads x[-lo .. n + hi - 1], which must

@@ -634,6 +634,161 @@ __global__ __launch_bounds__(kSlice) void k_sum_rows(const int32_t* __restrict__
     out[r] = sum;
 }
 
+// ------------------------------------------------------------------------
+// Factored operator product (RSD_ARITH_FACTORED): the Backward Euler form
+// pair applied to an operand on Q1 elements without forming the element
+// matrix.  A Q1 field f = sum_a f_a N_a equals sum_m c_m phi_m, phi_m the
+// monomial prod_d xi_d^(bit d of m), with c = T f and T the tensor product
+// over the directions of the 1D change of basis (f0, f1) -> ((f0 + f1) / 2,
+// (f1 - f0) / 2): the sum factorisation of the Q1 basis.  Per point, the
+// Jacobian, the operand's value and its reference gradient come from the
+// coefficients (d/dxi_d phi_m = phi_{m - e_d} when bit d of m is set);
+// the diffusion flux is f = (k w / |det J|) M^T (M grad_xi u) with M the
+// matrix the reference's gradient transform applies times det J
+// (JacobianTransform.tpp:71-87 with invertMatrix :159-186: in 3D the
+// cofactor matrix, in 2D the adjugate), and the mass term
+// s = coef rho cp w |det J| u, where the reference's measure sqrt(det J^T J)
+// is |det J|.  The test functions accumulate in coefficient space,
+// Z_m = sum_q (phi_m s + sum_d d/dxi_d phi_m f_d), and the element vector is
+// T^T Z.  About 125 FP64 operations per point for the whole element vector,
+// against 20 per entry and point for the element matrix.
+// ------------------------------------------------------------------------
+template<int D>
+__device__ __forceinline__ void q1_coefficients(double (&f)[1 << D]) {
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int a = 0; a < (1 << D); ++a)
+            if (!(a & (1 << d))) {
+                const double lo = f[a], hi = f[a | (1 << d)];
+                f[a] = 0.5 * (lo + hi);
+                f[a | (1 << d)] = 0.5 * (hi - lo);
+            }
+}
+
+// z <- T^T z
+template<int D>
+__device__ __forceinline__ void q1_coefficients_t(double (&z)[1 << D]) {
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int a = 0; a < (1 << D); ++a)
+            if (!(a & (1 << d))) {
+                const double lo = z[a], hi = z[a | (1 << d)];
+                z[a] = 0.5 * (lo - hi);
+                z[a | (1 << d)] = 0.5 * (lo + hi);
+            }
+}
+
+// one thread per element, into vb[a * n_elem + e] (by local node, the layout
+// k_sum_rows reads); the operand by the vector kernel's gathers (selector)
+template<int D, int NQ>
+__global__ __launch_bounds__(kBlock) void k_factored_elems(int64_t n_elem, AsmArgs A, Model m,
+                                                           double* __restrict__ vb) {
+    if (A.stop && *A.stop) return;
+    constexpr int NPE = 1 << D;
+    constexpr int NQP = D == 2 ? NQ * NQ : NQ * NQ * NQ;
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n_elem) return;
+    int32_t nodes[NPE];
+    double X[NPE][D];
+    load_element<D, 1>(A, e, nodes, X);
+    double cu[NPE];
+    if (A.selector == RSD_SEL_VALUE) gather<NPE>(A, nodes, 1, A.U, cu);
+    else if (A.selector == RSD_SEL_RATE) gather<NPE>(A, nodes, 0, A.Udot, cu);
+    else gather<NPE>(A, nodes, 0, A.operand, cu);
+    q1_coefficients<D>(cu);
+    double cx[D][NPE];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+#pragma unroll
+        for (int a = 0; a < NPE; ++a) cx[i][a] = X[a][i];
+        q1_coefficients<D>(cx[i]);
+    }
+    const double kel = A.kappa ? A.kappa[e] * m.k_scalar : m.k_scalar;
+    const double cm = A.coefs[1] * (m.rho * m.cp);
+    double Z[NPE];
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) Z[a] = 0.0;
+    // one point at a time: unrolling the point loop folds the monomials to
+    // constants but takes the 3D kernel from 126 to 168 VGPRs
+#pragma unroll 1
+    for (int q = 0; q < NQP; ++q) {
+        double xi[3];
+        const double w = point_coords<D, NQ>(q, xi);
+        double phi[NPE];
+#pragma unroll
+        for (int mm = 0; mm < NPE; ++mm) {
+            phi[mm] = 1.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                if (mm & (1 << d)) phi[mm] *= xi[d];
+        }
+        double u = 0.0, gu[D], J[D][D];
+#pragma unroll
+        for (int mm = 0; mm < NPE; ++mm) u = __builtin_fma(cu[mm], phi[mm], u);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            gu[d] = 0.0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) J[i][d] = 0.0;
+#pragma unroll
+            for (int mm = 0; mm < NPE; ++mm)
+                if (mm & (1 << d)) {
+                    const double ph = phi[mm ^ (1 << d)];
+                    gu[d] = __builtin_fma(cu[mm], ph, gu[d]);
+#pragma unroll
+                    for (int i = 0; i < D; ++i) J[i][d] = __builtin_fma(cx[i][mm], ph, J[i][d]);
+                }
+        }
+        double M[D][D], det;
+        if constexpr (D == 2) {
+            M[0][0] = J[1][1]; M[0][1] = -J[0][1];
+            M[1][0] = -J[1][0]; M[1][1] = J[0][0];
+            det = __builtin_fma(J[0][0], J[1][1], -(J[0][1] * J[1][0]));
+        } else {
+            M[0][0] = __builtin_fma(J[1][1], J[2][2], -(J[1][2] * J[2][1]));
+            M[0][1] = __builtin_fma(J[1][2], J[2][0], -(J[1][0] * J[2][2]));
+            M[0][2] = __builtin_fma(J[1][0], J[2][1], -(J[1][1] * J[2][0]));
+            M[1][0] = __builtin_fma(J[0][2], J[2][1], -(J[0][1] * J[2][2]));
+            M[1][1] = __builtin_fma(J[0][0], J[2][2], -(J[0][2] * J[2][0]));
+            M[1][2] = __builtin_fma(J[0][1], J[2][0], -(J[0][0] * J[2][1]));
+            M[2][0] = __builtin_fma(J[0][1], J[1][2], -(J[0][2] * J[1][1]));
+            M[2][1] = __builtin_fma(J[0][2], J[1][0], -(J[0][0] * J[1][2]));
+            M[2][2] = __builtin_fma(J[0][0], J[1][1], -(J[0][1] * J[1][0]));
+            det = __builtin_fma(J[0][0], M[0][0], __builtin_fma(J[0][1], M[0][1], J[0][2] * M[0][2]));
+        }
+        const double ad = fabs(det);
+        const double sk = (kel * w) / ad;
+        const double s = (cm * w) * ad * u;
+        double v[D], f[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            v[i] = M[i][0] * gu[0];
+#pragma unroll
+            for (int d = 1; d < D; ++d) v[i] = __builtin_fma(M[i][d], gu[d], v[i]);
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            double t = M[0][d] * v[0];
+#pragma unroll
+            for (int i = 1; i < D; ++i) t = __builtin_fma(M[i][d], v[i], t);
+            f[d] = sk * t;
+        }
+#pragma unroll
+        for (int mm = 0; mm < NPE; ++mm) {
+            double z = __builtin_fma(phi[mm], s, Z[mm]);
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                if (mm & (1 << d)) z = __builtin_fma(phi[mm ^ (1 << d)], f[d], z);
+            Z[mm] = z;
+        }
+    }
+    q1_coefficients_t<D>(Z);
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) vb[a * n_elem + e] = Z[a];
+}
+
 // the lift's row pass: each listed row adds its border elements' terms in
 // border (colour) order, F = ((F + t_1) + t_2) + ..., bitwise the scatter
 __global__ void k_lift_rows(int64_t nrows, const int32_t* __restrict__ rows, const int32_t* __restrict__ off,
@@ -831,6 +986,11 @@ static bool use_uniform(rsd_problem* pb, const AsmArgs& A) {
     return pb->arith == RSD_ARITH_UNIFORM && use_gather(pb) && form_pair(pb, A) && uniform_geometry(pb);
 }
 
+// the factored operator product: Q1 elements, the row gather's tables
+static bool use_factored(const rsd_problem* pb, const AsmArgs& A) {
+    return pb->arith == RSD_ARITH_FACTORED && pb->p == 1 && use_gather(pb) && form_pair(pb, A);
+}
+
 // Kh (unit conductivity) and Mh of element 0 by the element kernel, in the
 // reference's order, into pb->uhat
 template<int D, int P, int NQ>
@@ -909,6 +1069,25 @@ static void run_vector_uniform(rsd_problem* pb, const AsmArgs& A) {
     hipLaunchKernelGGL(k_sum_rows<NPE>, dim3((unsigned)pb->sell.n_slices), dim3(kSlice), 0, st, pb->gat_el.p,
                        pb->gat_m, ne, pb->vbuf.p, pb->n_free, A.F, A.stop);
     RSD_HIP(hipGetLastError());
+}
+
+// the pair applied to a state or operand in factored form (Q1): element
+// vectors, then each row sums its elements' terms in colour order into A.F
+template<int D, int P, int NQ>
+static void run_vector_factored(rsd_problem* pb, const AsmArgs& A) {
+    if constexpr (P == 1) {
+        constexpr int NPE = Elem<D, P>::NPE;
+        const int64_t ne = pb->n_elem;
+        if (pb->vbuf.n < ne * NPE) pb->vbuf.alloc(ne * NPE);
+        hipStream_t st = pb->ctx->stream;
+        hipLaunchKernelGGL((k_factored_elems<D, NQ>), dim3((unsigned)nblocks(ne)), dim3(kBlock), 0, st, ne, A,
+                           pb->model, pb->vbuf.p);
+        hipLaunchKernelGGL(k_sum_rows<NPE>, dim3((unsigned)pb->sell.n_slices), dim3(kSlice), 0, st, pb->gat_el.p,
+                           pb->gat_m, ne, pb->vbuf.p, pb->n_free, A.F, A.stop);
+        RSD_HIP(hipGetLastError());
+    } else {
+        throw Error(RSD_ERR_INTERNAL, "factored operator product: Q1 elements only");
+    }
 }
 
 // row gather assembly: the element pass (one launch over all elements),
@@ -1002,6 +1181,10 @@ static void run_vector_t(rsd_problem* pb, AsmArgs A, const Expr& src) {
     if constexpr (MODE == 0 && !STATE && GRAD) {
         if (use_uniform(pb, A)) {
             run_vector_uniform<D, P, NQ>(pb, A);
+            return;
+        }
+        if (use_factored(pb, A)) {
+            run_vector_factored<D, P, NQ>(pb, A);
             return;
         }
         pair = form_pair(pb, A);
@@ -1127,9 +1310,14 @@ static bool forms_need_state(const rsd_problem* pb, const AsmArgs& A) {
 }
 
 // the uniform-geometry arithmetic's device buffers (tables, element vectors
-// for the mass vector and for operand products), reserved when the mode is
-// selected so that no later call allocates, under stream capture or not
-void uniform_reserve(rsd_problem* pb) {
+// for the mass vector and for operand products), or the factored product's
+// element vectors, reserved when the mode is selected so that no later call
+// allocates, under stream capture or not
+void arith_reserve(rsd_problem* pb) {
+    if (pb->arith == RSD_ARITH_FACTORED) {  // the element vectors of the factored product
+        if (use_gather(pb) && pb->p == 1 && pb->vbuf.n < pb->n_elem * pb->npe) pb->vbuf.alloc(pb->n_elem * pb->npe);
+        return;
+    }
     if (!use_gather(pb) || !uniform_geometry(pb)) return;
     const int64_t npe = pb->npe, m = pb->n_elem * npe;
     if (pb->uhat.n < 2 * npe * npe) pb->uhat.alloc(2 * npe * npe);

@@ -65,8 +65,10 @@ void gather_reserve(rsd_problem* pb, bool with_mass);
 // what assembling these forms (with the mass vector) will allocate, reserved
 // outside any stream capture
 void assembly_reserve(rsd_problem* pb, int32_t nf, const int32_t* kinds, bool with_mass);
-// the uniform-geometry arithmetic's buffers, when the mesh and tables allow it
-void uniform_reserve(rsd_problem* pb);
+// the buffers of the problem's element arithmetic (uniform geometry:
+// tables and element vectors; factored: element vectors), when the mesh and
+// tables allow that arithmetic
+void arith_reserve(rsd_problem* pb);
 // the Gauss point table (shape values and weights), computed on the device
 // by the element kernels' own functions; part of problem set-up
 void build_shape_table(rsd_problem* pb);

@@ -178,7 +178,7 @@ struct rsd_problem {
     // tables are not built
     int32_t gat_m = 0;            // element slots per row
     bool gather_on = true;        // matrices assemble by row gather when the tables exist
-    int32_t arith = 0;            // RSD_ARITH_REFERENCE, _CONTRACTED or _UNIFORM (the BE form pair)
+    int32_t arith = 0;            // RSD_ARITH_REFERENCE, _CONTRACTED, _UNIFORM or _FACTORED (the BE form pair)
     int32_t uniform_geom = -1;    // every element a translate of element 0: -1 not yet checked, 0 no, 1 yes
     rsd::DBuf<double> uhat;       // [2 * npe * npe] Kh then Mh of element 0 (uniform geometry)
     rsd::DBuf<double> vbuf;       // [n_elem * npe] element vectors of the uniform pair, reserved on first use

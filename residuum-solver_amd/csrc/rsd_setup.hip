@@ -984,8 +984,8 @@ int rsd_problem_set_assembly(rsd_problem* pb, int32_t mode, int32_t* in_use) {
         if (!pb->gather_on) {  // the element buffers are of no further use
             pb->ebuf.reset();
             pb->fbuf.reset();
-        } else if (pb->arith == RSD_ARITH_UNIFORM) {
-            uniform_reserve(pb);  // so that no later call allocates
+        } else if (pb->arith == RSD_ARITH_UNIFORM || pb->arith == RSD_ARITH_FACTORED) {
+            arith_reserve(pb);  // so that no later call allocates
         }
         if (in_use) *in_use = pb->gather_on && pb->gat_m > 0 ? 1 : 0;
     });
@@ -994,10 +994,11 @@ int rsd_problem_set_assembly(rsd_problem* pb, int32_t mode, int32_t* in_use) {
 int rsd_problem_set_arithmetic(rsd_problem* pb, int32_t mode) {
     return guard([&] {
         RSD_CHECK(pb, "rsd_problem_set_arithmetic: null problem");
-        RSD_CHECK(mode == RSD_ARITH_REFERENCE || mode == RSD_ARITH_CONTRACTED || mode == RSD_ARITH_UNIFORM,
-                  "rsd_problem_set_arithmetic: mode must be RSD_ARITH_REFERENCE, _CONTRACTED or _UNIFORM");
+        RSD_CHECK(mode == RSD_ARITH_REFERENCE || mode == RSD_ARITH_CONTRACTED || mode == RSD_ARITH_UNIFORM ||
+                      mode == RSD_ARITH_FACTORED,
+                  "rsd_problem_set_arithmetic: mode must be RSD_ARITH_REFERENCE, _CONTRACTED, _UNIFORM or _FACTORED");
         pb->arith = mode;
-        if (mode == RSD_ARITH_UNIFORM) uniform_reserve(pb);
+        if (mode == RSD_ARITH_UNIFORM || mode == RSD_ARITH_FACTORED) arith_reserve(pb);
     });
 }
 

@@ -24,7 +24,7 @@ SCALED = 0x100
 SEL_VALUE, SEL_RATE, SEL_OPERAND = 0, 1, 2
 PC_IDENTITY, PC_JACOBI = 0, 1
 TOL_RELATIVE, TOL_ABSOLUTE = 0, 1
-ARITH_REFERENCE, ARITH_CONTRACTED, ARITH_UNIFORM = 0, 1, 2
+ARITH_REFERENCE, ARITH_CONTRACTED, ARITH_UNIFORM, ARITH_FACTORED = 0, 1, 2, 3
 
 _d = C.POINTER(C.c_double)
 _f = C.POINTER(C.c_float)
@@ -421,8 +421,10 @@ class Problem:
 
     def set_arithmetic(self, mode):
         """Element arithmetic: ARITH_REFERENCE (default, the reference's
-        operation order, bitwise) or ARITH_CONTRACTED (the Backward Euler
-        form pair in fused multiply-adds)."""
+        operation order, bitwise), ARITH_CONTRACTED (the Backward Euler
+        form pair in fused multiply-adds), ARITH_UNIFORM (on a block mesh,
+        every element matrix from element 0's) or ARITH_FACTORED (the pair's
+        operand product on Q1 elements without element matrices)."""
         check(lib().rsd_problem_set_arithmetic(self.h, C.c_int32(mode)))
 
     def padding_f32(self):

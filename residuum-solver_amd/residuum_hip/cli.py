@@ -41,9 +41,10 @@ csr operator.
 
 `discretization: arithmetic:` (a device extension; default `reference`, the
 reference's operation order, bitwise) selects the element arithmetic of the
-Backward Euler form pair: `contracted` (fused multiply-adds) or `uniform`
-(on a block mesh, every element matrix from element 0's: no element pass);
-both within a few 1e-15 of the reference order.
+Backward Euler form pair: `contracted` (fused multiply-adds), `uniform`
+(on a block mesh, every element matrix from element 0's: no element pass)
+or `factored` (on Q1 meshes of any shape, the fem operator's product
+without element matrices); all within a few 1e-15 of the reference order.
 """
 import argparse
 import os
@@ -52,11 +53,12 @@ import time as _time
 
 import numpy as np
 
-from . import (ARITH_CONTRACTED, ARITH_REFERENCE, ARITH_UNIFORM, DIFFUSION, MASS, SCALED, PC_IDENTITY, PC_JACOBI,
+from . import (ARITH_CONTRACTED, ARITH_FACTORED, ARITH_REFERENCE, ARITH_UNIFORM, DIFFUSION, MASS, SCALED, PC_IDENTITY, PC_JACOBI,
                SEL_VALUE, PMSH_HEX, PMSH_LAGRANGE, PMSH_QUAD, Ctx, Problem, RsdError, block_mesh, read_pmsh,
                write_pmsh)
 
-ARITHMETIC = {"reference": ARITH_REFERENCE, "contracted": ARITH_CONTRACTED, "uniform": ARITH_UNIFORM}
+ARITHMETIC = {"reference": ARITH_REFERENCE, "contracted": ARITH_CONTRACTED, "uniform": ARITH_UNIFORM,
+              "factored": ARITH_FACTORED}
 
 
 class ConfigError(RuntimeError):

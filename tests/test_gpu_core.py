@@ -398,6 +398,91 @@ def test_uniform_geometry_arithmetic(ctx, case):
     assert np.array_equal(got[0][0], got[1][0]) and np.array_equal(got[0][1], got[1][1])
 
 
+FACTORED_CASES = [(2, (9, 7), (1, 1), 2, [0, 1, 2, 3], ["x*y"] * 4, None),
+                  (3, (7, 6, 5), (1, 1, 1), 2, [0, 1], ["1", "0"], None),
+                  (2, (6, 5), (1, 1), 3, [0, 2], ["1", "x"], None)]
+
+
+def distorted(ctx, case, seed=3, amp=0.25):
+    """A block mesh with its interior nodes moved by up to amp of an element
+    and the whole mesh sheared (x += 0.3 y), so that no element's Jacobian is
+    symmetric or constant: the GPU problem and the oracle on it."""
+    import residuum_hip as rh
+    dim, n, p, nq, tags, exprs, ext = case
+    oc = O.Case(dim, n, p, nq, tags, exprs, ext=ext, order=O.ORDER_GPU)
+    rng = np.random.default_rng(seed)
+    xyz = oc.xyz.reshape(-1, dim)
+    inner = np.all((xyz > 1e-12) & (xyz < 1 - 1e-12), axis=1)
+    xyz[inner] += rng.uniform(-amp, amp, (int(inner.sum()), dim)) / max(n)
+    xyz[:, 0] += 0.3 * xyz[:, 1]
+    oc.kappa = np.exp(0.5 * rng.standard_normal(oc.n_elem))
+    oc.set_order(O.ORDER_GPU)
+    pb = rh.Problem.from_mesh(ctx, dim, p, nq, oc.xyz, oc.ien, oc.rng, tags, exprs)
+    pb.set_conductivity(1.0, kappa=oc.kappa)
+    return pb, oc
+
+
+@pytest.mark.parametrize("case", FACTORED_CASES)
+def test_factored_arithmetic(ctx, case):
+    """RSD_ARITH_FACTORED on distorted Q1 meshes: the Backward Euler pair's
+    operand product from the elements' multilinear coefficients (no element
+    matrix), against the oracle's reference order on the same mesh:
+    - the product within 1e-12 of sum_j |A_ij x_j| (a CPU restatement of the
+      same arithmetic measured within 1.3e-15), for the operand, value and
+      rate selectors' operand;
+    - the matrix-free CG (to 1e-13) within 1e-10 of the oracle's solution;
+    - the matrix and diagonal are the contracted arithmetic's, bit for bit;
+    - on Q2 quads the product falls back to the contracted arithmetic, bit
+      for bit."""
+    import residuum_hip as rh
+    dt = 2e-3
+    forms = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 1.0 / dt)]
+    say("case", case)
+    pb, oc = distorted(ctx, case)
+    ref = oc.assemble_matrix(forms)
+    rows = np.repeat(np.arange(oc.n_free), np.diff(oc.rp))
+    x = np.random.default_rng(6).standard_normal(oc.n_free)
+    yo = oc.assemble_vector(forms, operand=x, selector=O.SEL_OPERAND)
+    scale = np.zeros(oc.n_free)
+    np.add.at(scale, rows, np.abs(ref * x[oc.ci]))
+    pb.set_arithmetic(rh.ARITH_FACTORED)
+    assert pb.set_assembly(1) == 1
+    y = pb.vector()
+    pb.fem_apply(y, forms, ctx.from_numpy(x))
+    err = np.max(np.abs(y.numpy() - yo) / scale)
+    say(f"factored operand product: max error {err:.2e} of sum |A_ij x_j|")
+    assert err <= 1e-12
+    b = np.random.default_rng(7).standard_normal(oc.n_free)
+    dx = pb.vector(0.0)
+    rep = pb.cg_fem(ctx.from_numpy(b), dx, forms, tol=1e-13, max_iters=4000)
+    xo, ro = oc.cg_fem(forms, b, tol=1e-13, max_iters=4000)
+    ex = np.max(np.abs(dx.numpy() - xo)) / np.max(np.abs(xo))
+    say(f"factored matrix-free CG: iters {rep.iterations} / {ro.iterations}, max rel diff {ex:.2e}")
+    assert rep.converged and ex <= 1e-10
+    K = pb.matrix()
+    pb.assemble_matrix(K, forms)
+    kf = K.values()
+    pb.set_arithmetic(rh.ARITH_CONTRACTED)
+    pb.assemble_matrix(K, forms)
+    assert np.array_equal(bits(kf), bits(K.values()))
+    K.close()
+    pb.close()
+    # Q2 quads: the contracted product
+    dim, n, p, nq, tags, exprs, ext = case
+    if dim != 2:
+        return
+    pq, oq = distorted(ctx, (dim, n, (2, 2), 3, tags, exprs, ext), seed=5, amp=0.1)
+    xq = np.random.default_rng(8).standard_normal(oq.n_free)
+    got = []
+    for arith in (rh.ARITH_FACTORED, rh.ARITH_CONTRACTED):
+        pq.set_arithmetic(arith)
+        yq = pq.vector()
+        pq.fem_apply(yq, forms, ctx.from_numpy(xq))
+        got.append(bits(yq.numpy()))
+    assert np.array_equal(got[0], got[1])
+    pq.close()
+
+
 @pytest.mark.parametrize("case", CASES[:2])
 def test_assemble_jacobian_tdep_bitexact(ctx, case):
     import residuum_hip as rh
