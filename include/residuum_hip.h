@@ -324,6 +324,21 @@ int rsd_matrix_store_csr(rsd_ctx* ctx, rsd_matrix* m, double* d_csr_values);
 /* y = A x: operations::matvec (linalg/operations/backend/cpu/MatrixOps.tpp:5-23),
  * row sums in the reference's left-to-right order (bitwise equal) */
 int rsd_spmv_f64(rsd_ctx* ctx, rsd_matrix* A, const double* x, double* y);
+/* The FP64 stencil copy (no reference counterpart; the product is
+ * operations::matvec, MatrixOps.tpp:5-23, bitwise).  When every row's column
+ * offsets lie in one 27-point (hex Q1) or 9-point (quad Q1) stencil of runs
+ * of three consecutive offsets (block meshes), a second copy of the values
+ * holds all stencil positions of each 64-row slice, zeros where a row lacks
+ * one, in 16-byte pairs; its product loads one x entry per run and lane and
+ * passes the neighbours between lanes.  rsd_cg_solve_f64, rsd_be_run and the
+ * other FP64 CG solves refresh and use it by themselves.
+ * rsd_problem_layout_stencil: out[2] = stencil points (0: no stencil copy),
+ * slots of the copy.  rsd_matrix_to_stencil refreshes the copy from A's
+ * values (RSD_ERR_ARG without a stencil); rsd_spmv_f64_stencil is y = A x
+ * over it, bitwise rsd_spmv_f64's result. */
+int rsd_problem_layout_stencil(rsd_problem* pb, int64_t* out);
+int rsd_matrix_to_stencil(rsd_ctx* ctx, rsd_matrix* A);
+int rsd_spmv_f64_stencil(rsd_ctx* ctx, rsd_matrix* A, const double* x, double* y);
 /* Jacobi::update (linalg/solver/preconditioner/Jacobi.hpp:23-36) via
  * CSROperator::diagonal (CSROperator.hpp:40-45): 1/d, or 1 if |d| <= 1e-300 */
 int rsd_jacobi_inv_diag(rsd_ctx* ctx, rsd_matrix* A, double* inv_diag);

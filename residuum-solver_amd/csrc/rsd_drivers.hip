@@ -133,6 +133,8 @@ int64_t be_run(rsd_problem* pb, double t0, double tf, double dt_policy, const ch
         const int32_t kinds[2] = {RSD_FORM_DIFFUSION, RSD_FORM_MASS | RSD_FORM_SCALED};
         if (gather_forms_product(pb) && matrix_mass_fusable(pb, 2, kinds)) ax0 = (double*)ctx->ws("be.Ax0", vb);
     }
+    // the FP64 CG reads the stencil copy where the layout has one (before capture)
+    if (!mixed) reserve_stencil(ctx, &K);
     update_gval(pb, t0);
     launch_copy(ctx, n, U, b.Uprev);
     Pinned pin;

@@ -108,6 +108,46 @@ __global__ __launch_bounds__(kBlock) void k_spmv_f64(int64_t n, int64_t nb, Cols
     if (r < n) y[r] = row_dot_f64<MODE>(r, C, val, x);
 }
 
+// The stencil copy from the sliced values: row r (one thread) writes all
+// its NP positions, its entries in CSR order at the positions its mask
+// holds, zeros elsewhere, as 16-byte pairs.  The copy's slots of the rows
+// past n stay as reserved (zero).
+template<int NP>
+__global__ __launch_bounds__(kBlock) void k_to_st(int64_t n, const int64_t* __restrict__ slice_ptr,
+                                                  const uint32_t* __restrict__ mask, const double* __restrict__ val,
+                                                  double* __restrict__ vst) {
+    const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t m = mask[r];
+    const int last = __builtin_popcount(m) - 1;
+    const double* src = val + slice_ptr[r >> 6] + (r & 63);
+    double v[NP];
+    int j = 0;
+#pragma unroll
+    for (int b = 0; b < NP; ++b) {
+        const bool has = (m >> b) & 1u;
+        // entry min(j, last) is one of the row's own (an empty row loads none)
+        const double t = last >= 0 ? src[(int64_t)min(j, last) * kSlice] : 0.0;
+        v[b] = has ? t : 0.0;
+        j += has;
+    }
+    double* dst = vst + (r >> 6) * (NP * 64);
+    const int64_t l = r & 63;
+#pragma unroll
+    for (int b = 0; b + 1 < NP; b += 2) *(rsd_double2*)(dst + 128 * (b >> 1) + 2 * l) = rsd_double2{v[b], v[b + 1]};
+    dst[128 * ((NP - 1) >> 1) + l] = v[NP - 1];
+}
+
+// y = A x over the stencil copy.  Every lane runs the row product (the
+// lanes past n feed their neighbours, row_dot_st); only rows < n store.
+template<int NR>
+__global__ __launch_bounds__(kBlock) void k_spmv_st(int64_t n, int64_t nb, StCols S, const double* __restrict__ val,
+                                                    const double* __restrict__ x, double* __restrict__ y) {
+    const int64_t r = xcd_block(blockIdx.x, nb) * kBlock + threadIdx.x;
+    const double s = row_dot_st<NR>(r, n, S, val, x);
+    if (r < n) y[r] = s;
+}
+
 __global__ void k_inv_diag(int64_t n, const int32_t* __restrict__ diag, const double* __restrict__ val,
                            double* __restrict__ inv) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -178,6 +218,23 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_pap(int64_t n, int64_t nb, C
     double v[1] = {0.0};
     if (r < n) {
         const double y = row_dot_f64<MODE>(r, C, val, p);
+        Ap[r] = y;
+        v[0] = p[r] * y;
+    }
+    block_partials<1>(v, lb, nb, partials);
+}
+
+// k_cg_spmv_pap over the stencil copy (all lanes run the row product)
+template<int NR>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_pap_st(int64_t n, int64_t nb, StCols S, const double* __restrict__ val,
+                                                           const double* __restrict__ p, double* __restrict__ Ap,
+                                                           const CGState* st, double* partials) {
+    if (st->stop) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t r = lb * kBlock + threadIdx.x;
+    const double y = row_dot_st<NR>(r, n, S, val, p);
+    double v[1] = {0.0};
+    if (r < n) {
         Ap[r] = y;
         v[0] = p[r] * y;
     }
@@ -275,6 +332,38 @@ void launch_spmv_f64(rsd_ctx* ctx, const rsd_matrix* A, const double* x, double*
                     x, y);
 }
 
+void reserve_stencil(rsd_ctx* ctx, const rsd_matrix* A) {
+    const Sell& S = A->pb->sell;
+    if (!S.has_st || A->val_st.n == S.st_slots()) return;
+    A->val_st.alloc(S.st_slots());
+    RSD_HIP(hipMemsetAsync(A->val_st.p, 0, sizeof(double) * (size_t)A->val_st.n, ctx->stream));
+}
+
+bool stencil_ready(const rsd_matrix* A) {
+    const Sell& S = A->pb->sell;
+    return S.has_st && S.n_rows > 0 && A->val_st.n == S.st_slots();
+}
+
+void launch_to_stencil(rsd_ctx* ctx, const rsd_matrix* A) {
+    const Sell& S = A->pb->sell;
+    const int64_t n = S.n_rows;
+    const dim3 grid((unsigned)nblocks(n)), block(kBlock);
+    if (S.st_np == 27)
+        hipLaunchKernelGGL(k_to_st<27>, grid, block, 0, ctx->stream, n, S.slice_ptr.p, S.st_mask.p, A->val.p, A->val_st.p);
+    else
+        hipLaunchKernelGGL(k_to_st<9>, grid, block, 0, ctx->stream, n, S.slice_ptr.p, S.st_mask.p, A->val.p, A->val_st.p);
+}
+
+void launch_spmv_stencil(rsd_ctx* ctx, const rsd_matrix* A, const double* x, double* y) {
+    const Sell& S = A->pb->sell;
+    const int64_t n = S.n_rows, nb = nblocks(n);
+    const dim3 grid((unsigned)nb), block(kBlock);
+    if (S.st_np == 27)
+        hipLaunchKernelGGL(k_spmv_st<9>, grid, block, 0, ctx->stream, n, nb, S.st_cols(), A->val_st.p, x, y);
+    else
+        hipLaunchKernelGGL(k_spmv_st<3>, grid, block, 0, ctx->stream, n, nb, S.st_cols(), A->val_st.p, x, y);
+}
+
 void launch_inv_diag(rsd_ctx* ctx, const rsd_matrix* A, double* inv) {
     const Sell& S = A->pb->sell;
     const int64_t n = S.n_rows;
@@ -325,7 +414,13 @@ void cg_launch_init(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* 
     const int64_t n = A->pb->sell.n_rows, nb = std::max<int64_t>(1, nblocks(n));
     double* part = ctx->partials(3 * nb);
     if (precond == RSD_PC_JACOBI) launch_inv_diag(ctx, A, w.invd);
-    launch_spmv_f64(ctx, A, x, w.Ap);
+    if (stencil_ready(A)) {
+        // the iterations read the stencil copy: refresh it from the values
+        launch_to_stencil(ctx, A);
+        launch_spmv_stencil(ctx, A, x, w.Ap);
+    } else {
+        launch_spmv_f64(ctx, A, x, w.Ap);
+    }
     hipLaunchKernelGGL(k_cg_init, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, n, nb, b, w.Ap, w.r, w.p,
                        precond == RSD_PC_JACOBI ? w.invd : nullptr, part);
     hipLaunchKernelGGL(k_cg_init_fold, dim3(1), dim3(kFold), 0, ctx->stream, nb, part, w.st, tol, w.abs_tol, max_iters,
@@ -339,8 +434,17 @@ void cg_launch_iter(rsd_ctx* ctx, const rsd_matrix* A, double* x, double tol, in
     double* part = ctx->partials(3 * nb);
     const double* invd = precond == RSD_PC_JACOBI ? w.invd : nullptr;
     hipStream_t s = ctx->stream;
-    RSD_LAUNCH_COLS(S.mode(), k_cg_spmv_pap, dim3((unsigned)nb), dim3(kBlock), s, n, nb, S.cols(), A->val.p, w.p, w.Ap,
-                    w.st, part);
+    if (stencil_ready(A)) {
+        if (S.st_np == 27)
+            hipLaunchKernelGGL(k_cg_spmv_pap_st<9>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.st_cols(),
+                               A->val_st.p, w.p, w.Ap, w.st, part);
+        else
+            hipLaunchKernelGGL(k_cg_spmv_pap_st<3>, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, S.st_cols(),
+                               A->val_st.p, w.p, w.Ap, w.st, part);
+    } else {
+        RSD_LAUNCH_COLS(S.mode(), k_cg_spmv_pap, dim3((unsigned)nb), dim3(kBlock), s, n, nb, S.cols(), A->val.p, w.p,
+                        w.Ap, w.st, part);
+    }
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kFold), 0, s, nb, part, w.st);
     hipLaunchKernelGGL(k_cg_update_r, dim3((unsigned)nb), dim3(kBlock), 0, s, n, nb, x, w.p, w.Ap, w.r, invd, w.st,
                        part + nb);
@@ -384,6 +488,7 @@ void cg_report(const CGState& s, int64_t max_iters, rsd_report* rep) {
 // Run a CG solve to completion with chunked launches; returns the final state.
 CGState cg_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
                int32_t precond, const CGWork& w) {
+    reserve_stencil(ctx, A);
     cg_launch_init(ctx, A, b, x, tol, max_iters, precond, w);
     CGState h{};
     CGState* pinned = nullptr;
@@ -579,6 +684,36 @@ int rsd_spmv_f64(rsd_ctx* ctx, rsd_matrix* A, const double* x, double* y) {
         RSD_CHECK(ctx && A && x && y, "rsd_spmv_f64: null argument");
         RSD_LIVE_MATRIX(A);
         launch_spmv_f64(ctx, A, x, y);
+        RSD_HIP(hipGetLastError());
+    });
+}
+
+int rsd_problem_layout_stencil(rsd_problem* pb, int64_t* out) {
+    return guard([&] {
+        RSD_CHECK(pb && out, "null argument");
+        const Sell& S = pb->sell;
+        out[0] = S.has_st ? S.st_np : 0;
+        out[1] = S.has_st ? S.st_slots() : 0;
+    });
+}
+
+int rsd_matrix_to_stencil(rsd_ctx* ctx, rsd_matrix* A) {
+    return guard([&] {
+        RSD_CHECK(ctx && A, "null argument");
+        RSD_LIVE_MATRIX(A);
+        RSD_CHECK(A->pb->sell.has_st, "rsd_matrix_to_stencil: the matrix's layout has no stencil copy");
+        reserve_stencil(ctx, A);
+        if (A->pb->sell.n_rows) launch_to_stencil(ctx, A);
+        RSD_HIP(hipGetLastError());
+    });
+}
+
+int rsd_spmv_f64_stencil(rsd_ctx* ctx, rsd_matrix* A, const double* x, double* y) {
+    return guard([&] {
+        RSD_CHECK(ctx && A && x && y, "rsd_spmv_f64_stencil: null argument");
+        RSD_LIVE_MATRIX(A);
+        RSD_CHECK(stencil_ready(A), "rsd_spmv_f64_stencil: call rsd_matrix_to_stencil first");
+        launch_spmv_stencil(ctx, A, x, y);
         RSD_HIP(hipGetLastError());
     });
 }

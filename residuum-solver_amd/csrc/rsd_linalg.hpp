@@ -21,6 +21,14 @@ struct CGWork {
 
 void launch_spmv_f64(rsd_ctx* ctx, const rsd_matrix* A, const double* x, double* y);
 void launch_inv_diag(rsd_ctx* ctx, const rsd_matrix* A, double* inv);
+// The FP64 stencil copy (Sell::has_st layouts): reserve_stencil allocates
+// and zeroes it once per matrix (outside stream capture); once reserved,
+// every CG solve on the matrix refreshes it in cg_launch_init and its
+// iterations read it.  launch_to_stencil / launch_spmv_stencil need it reserved.
+void reserve_stencil(rsd_ctx* ctx, const rsd_matrix* A);
+bool stencil_ready(const rsd_matrix* A);
+void launch_to_stencil(rsd_ctx* ctx, const rsd_matrix* A);
+void launch_spmv_stencil(rsd_ctx* ctx, const rsd_matrix* A, const double* x, double* y);
 void launch_dot(rsd_ctx* ctx, int64_t n, const double* x, const double* y, double* d_out, bool take_sqrt);
 void launch_axpy(rsd_ctx* ctx, int64_t n, double a, const double* x, double* y);
 void launch_axpby(rsd_ctx* ctx, int64_t n, double a, double b, const double* x, double* y);

@@ -63,6 +63,23 @@ struct Sell {
     int32_t uni_vidx[3 * kUniMaxRuns] = {};
     int64_t uni_stride = 0, uni_front = 0;
     int64_t uni_lo = 0, uni_hi = 0;  // padding an x vector needs before and after its n entries
+    // stencil layout of the FP64 copy the CG reads (COL_ST): when the union
+    // of every row's column offsets is st_np = 27 (hex Q1) or 9 (quad Q1)
+    // offsets in runs of three consecutive ones, each 64-row slice holds all
+    // st_np stencil positions of its rows, zeros where a row lacks one, in
+    // 16-byte pairs; st_mask[r] has bit b set when row r holds position b
+    bool has_st = false;
+    int32_t st_np = 0;
+    int32_t st_off[kStMaxPos] = {};  // column - row of each position, ascending
+    DBuf<uint32_t> st_mask;          // [n_rows]
+    int64_t st_slots() const { return n_slices * st_np * kSlice; }
+    StCols st_cols() const {
+        StCols c;
+        c.mask = st_mask.p;
+        c.np = st_np;
+        for (int k = 0; k < st_np / 3; ++k) c.coff[k] = st_off[3 * k + 1];
+        return c;
+    }
 
     Cols cols() const {
         Cols c;
@@ -218,6 +235,7 @@ struct rsd_matrix {
     rsd::DBuf<double> val;    // [slots] SELL order
     rsd::DBuf<float> val32;   // [sell.slots32()] FP32 copy for the mixed-precision path
     rsd::DBuf<float> val32f;  // [sell.slots] full FP32 copy in the FP64 layout (mixed GMRES, any matrix)
+    mutable rsd::DBuf<double> val_st;  // [sell.st_slots()] FP64 copy in the stencil layout, a cache the FP64 CG refreshes
     rsd::DBuf<int32_t> asym;  // [1] set by the FP32 conversion when the half layout's symmetry check fails
 };
 

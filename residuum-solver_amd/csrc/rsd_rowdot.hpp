@@ -44,6 +44,132 @@ struct Cols {
     int32_t u_vidx[3 * kUniMaxRuns] = {};    // value index - row of each entry
 };
 
+// Stencil layout of an FP64 copy (COL_ST).  Every row's column offsets lie
+// in one stencil of np = 3 nr positions, ascending, in nr runs of three
+// consecutive offsets (c - 1, c, c + 1), c = coff[k]: the 27-point hex and
+// 9-point quad Q1 stencils.  Slice s (rows 64 s .. 64 s + 63) holds np * 64
+// slots from s * np * 64: positions 2p and 2p + 1 of lane l form one 16-byte
+// pair at 128 p + 2 l, the last (even) position sits alone at 128 p + l.  A
+// row lacking a position holds a zero there, and bit b of mask[r] tells
+// which positions the row has.
+constexpr int kStMaxPos = 27;
+
+struct StCols {
+    const uint32_t* mask = nullptr;  // [rows]
+    int32_t np = 0;                  // 27 or 9
+    int32_t coff[kStMaxPos / 3] = {};
+};
+
+// slot of position b of row r (within the copy)
+__host__ __device__ inline int64_t st_slot(int64_t r, int b, int np) {
+    const int64_t l = r & 63;
+    const int64_t base = (r >> 6) * np * 64 + 128 * (b >> 1);
+    return (b == np - 1) ? base + l : base + 2 * l + (b & 1);
+}
+
+// x entry of a run's left / right neighbour from lane l -/+ 1 (DPP wave
+// shifts, 32 bits at a time); lane 0 (shr) and lane 63 (shl) keep `edge`
+__device__ __forceinline__ double wave_shr1(double edge, double v) {
+    const int2 e = __builtin_bit_cast(int2, edge), s = __builtin_bit_cast(int2, v);
+    int2 d;
+    d.x = __builtin_amdgcn_update_dpp(e.x, s.x, 0x138, 0xf, 0xf, false);
+    d.y = __builtin_amdgcn_update_dpp(e.y, s.y, 0x138, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, d);
+}
+__device__ __forceinline__ double wave_shl1(double edge, double v) {
+    const int2 e = __builtin_bit_cast(int2, edge), s = __builtin_bit_cast(int2, v);
+    int2 d;
+    d.x = __builtin_amdgcn_update_dpp(e.x, s.x, 0x130, 0xf, 0xf, false);
+    d.y = __builtin_amdgcn_update_dpp(e.y, s.y, 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, d);
+}
+
+typedef double rsd_double2 __attribute__((ext_vector_type(2)));
+
+// COL_ST row product over the stencil copy `val`.  The row's entries are
+// added in column order, each product and sum rounded on its own; a position
+// the row lacks leaves the sum unchanged (selected away by the mask, so a
+// non-finite x entry outside the row's stencil cannot reach it): bitwise
+// row_dot_f64's result.  Per run, one x load per lane at the centre column;
+// lane l takes its left and right neighbours from lanes l - 1 and l + 1 (rows
+// r - 1 and r + 1 read x[r - 1 + c] and x[r + 1 + c] as their centres), and
+// the two values past the wavefront's ends are wave-uniform scalar loads.
+// Must run in all 64 lanes of the wavefront, rows 64 s .. 64 s + 63 in lane
+// order, also the lanes past the last row (r >= n: mask 0, clamped loads,
+// result 0): their centres feed the last live row's right neighbours.
+// Values are loaded in two batches (runs 0..3, then 4..NR-1).
+template<int NR>
+__device__ __forceinline__ double row_dot_st(int64_t r, int64_t n, const StCols& S, const double* __restrict__ val,
+                                             const double* __restrict__ x) {
+    constexpr int NP = 3 * NR;
+    const bool live = r < n;
+    const int lane = (int)(r & 63);
+    const uint32_t m = live ? __builtin_nontemporal_load(S.mask + r) : 0u;
+    const double* sb = val + (live ? (r >> 6) : 0) * (NP * 64);
+    const int ri = (int)r, nm1 = (int)n - 1;
+    const int r0 = __builtin_amdgcn_readfirstlane(ri - lane);
+    double v[NP], xc[NR], xl[NR], xh[NR];
+    double sum = 0.0;
+    auto load_vals = [&](int b0, int b1) {
+#pragma unroll
+        for (int b = b0; b < b1; b += 2) {
+            if (b == NP - 1) {
+                v[b] = __builtin_nontemporal_load(sb + 128 * (b >> 1) + lane);
+            } else {
+                const rsd_double2 t = __builtin_nontemporal_load((const rsd_double2*)(sb + 128 * (b >> 1)) + lane);
+                v[b] = t.x;
+                v[b + 1] = t.y;
+            }
+        }
+    };
+    auto load_runs = [&](int k0, int k1) {
+#pragma unroll
+        for (int k = k0; k < k1; ++k) {
+            const int c = S.coff[k];
+            xc[k] = x[min(max(ri + c, 0), nm1)];
+            xl[k] = x[min(max(r0 + c - 1, 0), nm1)];
+            xh[k] = x[min(max(r0 + 64 + c, 0), nm1)];
+        }
+    };
+    auto runs = [&](int k0, int k1) {
+#pragma unroll
+        for (int k = k0; k < k1; ++k) {
+            const double xm = wave_shr1(xl[k], xc[k]);
+            const double xp = wave_shl1(xh[k], xc[k]);
+            const int b = 3 * k;
+            double t = sum + v[b] * xm;
+            sum = ((m >> b) & 1u) ? t : sum;
+            t = sum + v[b + 1] * xc[k];
+            sum = ((m >> (b + 1)) & 1u) ? t : sum;
+            t = sum + v[b + 2] * xp;
+            sum = ((m >> (b + 2)) & 1u) ? t : sum;
+        }
+    };
+    // each batch issues its value loads, then its x loads, then adds (the
+    // scheduling barriers keep the compiler from hoisting the x loads and
+    // lane shifts of later runs ahead of the value stream)
+    constexpr int K1 = NR > 4 ? 4 : NR;  // first batch: runs 0..K1-1, positions 0..3 K1 - 1 (+1 to pair up)
+    constexpr int B1 = (3 * K1 + 1) & ~1;
+    load_vals(0, B1 < NP ? B1 : NP);
+    __builtin_amdgcn_sched_barrier(0);
+    load_runs(0, K1);
+    __builtin_amdgcn_sched_barrier(0);
+    runs(0, K1);
+    if constexpr (K1 < NR) {
+        __builtin_amdgcn_sched_barrier(0);
+        load_vals(B1, NP);
+        __builtin_amdgcn_sched_barrier(0);
+        load_runs(K1, NR);
+        __builtin_amdgcn_sched_barrier(0);
+        runs(K1, NR);
+    }
+    // the sum is formed in every lane here: callers store it for rows < n
+    // only, and without this the compiler sinks the value loads into that
+    // branch, behind the x loads and lane shifts
+    asm volatile("" : "+v"(sum));
+    return sum;
+}
+
 // Matrix values and column indices are read once per product: they are
 // loaded non-temporally so the stream does not evict the x entries the
 // neighbouring rows gather from L2.

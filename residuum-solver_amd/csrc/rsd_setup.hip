@@ -382,6 +382,27 @@ void rsd::build_sell(rsd_problem* pb, const std::vector<int32_t>& rowlen) {
             S.pat_id.upload(pid.data(), n, st);
             S.pat_off.upload(off.data(), (int64_t)off.size(), st);
             S.pat_len.upload(len.data(), (int64_t)len.size(), st);
+            // stencil layout: the union of the patterns' offsets is 9 or 27
+            // offsets in runs of three consecutive ones
+            std::vector<int32_t> uni;
+            for (const auto& q : pats) uni.insert(uni.end(), q.begin(), q.end());
+            std::sort(uni.begin(), uni.end());
+            uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
+            bool st_ok = uni.size() == 9 || uni.size() == 27;
+            for (size_t k = 0; st_ok && k < uni.size(); k += 3)
+                st_ok = uni[k + 1] == uni[k] + 1 && uni[k + 2] == uni[k] + 2;
+            S.has_st = st_ok;
+            if (st_ok) {
+                S.st_np = (int32_t)uni.size();
+                std::copy(uni.begin(), uni.end(), S.st_off);
+                std::vector<uint32_t> pmask(pats.size(), 0u), rmask(n);
+                for (size_t q = 0; q < pats.size(); ++q)
+                    for (int32_t o : pats[q])
+                        pmask[q] |= 1u << (std::lower_bound(uni.begin(), uni.end(), o) - uni.begin());
+                for (int64_t r = 0; r < n; ++r) rmask[r] = pmask[pid[r]];
+                S.st_mask.upload(rmask.data(), n, st);
+                RSD_HIP(hipStreamSynchronize(st));  // rmask is a local
+            }
         }
     }
     S.slice_ptr.upload(S.h_slice_ptr.data(), S.n_slices + 1, st);

@@ -412,6 +412,14 @@ class Problem:
         check(lib().rsd_problem_layout(self.h, out))
         return tuple(int(v) for v in out)
 
+    def layout_stencil(self):
+        """(stencil points, slots) of the FP64 stencil copy the CG reads:
+        27 (hex Q1) or 9 (quad Q1) when every row's offsets lie in one such
+        stencil, else (0, 0)."""
+        out = (C.c_int64 * 2)()
+        check(lib().rsd_problem_layout_stencil(self.h, out))
+        return tuple(int(v) for v in out)
+
     def set_assembly(self, mode):
         """Matrix assembly scheme: 1 row gather (default), 0 colour-by-colour
         element scatter; returns 1 when the row gather is in use."""
@@ -652,6 +660,12 @@ class Matrix:
 
     def spmv(self, x, y):
         check(lib().rsd_spmv_f64(self.pb.ctx.h, self.h, dptr(x), dptr(y)))
+
+    def to_stencil(self):
+        check(lib().rsd_matrix_to_stencil(self.pb.ctx.h, self.h))
+
+    def spmv_stencil(self, x, y):
+        check(lib().rsd_spmv_f64_stencil(self.pb.ctx.h, self.h, dptr(x), dptr(y)))
 
     def inv_diag(self, out):
         check(lib().rsd_jacobi_inv_diag(self.pb.ctx.h, self.h, dptr(out)))

@@ -77,6 +77,70 @@ def test_spmv_bitexact(ctx, case):
     assert same(dy.numpy(), oc.spmv(val, x))
 
 
+STENCIL_CASES = CASES + [
+    # one element across in y and z: every free row sits on the boundary
+    (3, (9, 1, 1), (1, 1, 1), 2, [0], ["1"], None),
+    # 70 x 3 quads, all sides Dirichlet: rows whose stencils reach past 64 lanes
+    (2, (70, 3), (1, 1), 2, [0, 1, 2, 3], ["0"] * 4, None),
+    # 40 x 40 x 3 hexes, x faces Dirichlet: slices mixing line ends, a partial last slice
+    (3, (40, 40, 3), (1, 1, 1), 2, [0, 1], ["1", "0"], None),
+]
+
+
+@pytest.mark.parametrize("case", STENCIL_CASES)
+def test_spmv_stencil_bitexact(ctx, case):
+    """The stencil copy's product (one x load per run and lane, neighbours by
+    lane shifts) equals the sliced-ELL product bit for bit, also where x holds
+    infinities and NaNs: positions a row lacks never reach its sum.  (A NaN
+    result's sign is not fixed by IEEE 754 and follows the operand order the
+    compiler picks for each add, so NaNs are compared by position.)"""
+    import residuum_hip as rh
+    pb, oc = build(ctx, case)
+    npts, slots = pb.layout_stencil()
+    K = pb.matrix()
+    rng = np.random.default_rng(7)
+    K.set_values(rng.standard_normal(oc.nnz))
+    # the copy exists when the union of the rows' column offsets (oracle
+    # pattern) is 9 or 27 offsets in runs of three consecutive ones
+    rows = np.repeat(np.arange(oc.n_free), np.diff(oc.rp))
+    uni = np.unique(oc.ci - rows)
+    want = len(uni) in (9, 27) and all(np.array_equal(uni[k:k + 3], uni[k] + np.arange(3))
+                                       for k in range(0, len(uni), 3))
+    if not want:
+        assert (npts, slots) == (0, 0)
+        with pytest.raises(rh.RsdError):
+            K.to_stencil()
+        return
+    assert npts == len(uni) == (27 if case[0] == 3 else 9)
+    assert slots == -(-oc.n_free // 64) * 64 * npts
+    K.to_stencil()
+    for trial in range(3):
+        x = rng.standard_normal(oc.n_free)
+        if trial == 2:
+            bad = rng.choice(oc.n_free, size=min(5, oc.n_free), replace=False)
+            x[bad[::2]] = np.inf
+            x[bad[1::2]] = np.nan
+        dx, y0, y1 = ctx.from_numpy(x), pb.vector(), pb.vector()
+        K.spmv(dx, y0)
+        K.spmv_stencil(dx, y1)
+        a, b = y0.numpy(), y1.numpy()
+        nan = np.isnan(a)
+        assert np.array_equal(nan, np.isnan(b)), trial
+        assert np.array_equal(a[~nan].view(np.uint64), b[~nan].view(np.uint64)), trial
+        if trial == 2:
+            assert nan.any() and np.isinf(a).any()
+        if trial < 2:
+            assert same(b, oc.spmv(K.values(), x))
+    # a refresh after new values
+    K.set_values(rng.standard_normal(oc.nnz))
+    K.to_stencil()
+    x = rng.standard_normal(oc.n_free)
+    dx, y0, y1 = ctx.from_numpy(x), pb.vector(), pb.vector()
+    K.spmv(dx, y0)
+    K.spmv_stencil(dx, y1)
+    assert same(y0.numpy(), y1.numpy())
+
+
 @pytest.mark.parametrize("n", [0, 1, 63, 256, 257, 5000, 300001])
 def test_dot_norm_bitexact(ctx, n):
     say("dot n", n)
