@@ -621,10 +621,13 @@ def main():
     # fp64 and mixed in the reference's element arithmetic (the FP64 loop is
     # bitwise the oracle's); then both once more with the uniform-geometry
     # arithmetic (the block mesh's element matrices from element 0's, within
-    # a few ulps of the per-element ones), which removes the element pass
+    # a few ulps of the per-element ones), which removes the element pass;
+    # and in the contracted arithmetic (per-point factors and fused
+    # multiply-adds in every element, the fast assembly of a mesh of any shape)
     arith = {"fp64": rh.ARITH_REFERENCE, "mixed": rh.ARITH_REFERENCE,
-             "fp64_uniform": rh.ARITH_UNIFORM, "mixed_uniform": rh.ARITH_UNIFORM}
-    for mode in ("fp64", "mixed", "fp64_uniform", "mixed_uniform"):
+             "fp64_uniform": rh.ARITH_UNIFORM, "mixed_uniform": rh.ARITH_UNIFORM,
+             "fp64_contracted": rh.ARITH_CONTRACTED, "mixed_contracted": rh.ARITH_CONTRACTED}
+    for mode in ("fp64", "mixed", "fp64_uniform", "mixed_uniform", "fp64_contracted", "mixed_contracted"):
         mixed = mode.startswith("mixed")
         pb.set_arithmetic(arith[mode])
         U = pb.vector(0.0)
@@ -829,6 +832,7 @@ def main():
 
     mx, fp = results["mixed"], results["fp64"]
     fu, mu = results["fp64_uniform"], results["mixed_uniform"]
+    fc, mc = results["fp64_contracted"], results["mixed_contracted"]
     steps = args.steps
     line = {
         "metric": "CG DOF-iters/s + SpMV HBM GB/s, mixed vs FP64 GPU at rel resid 1e-8",
@@ -857,6 +861,16 @@ def main():
             "step_time_fp64_over_mixed": fu["total_ms"] / mu["total_ms"],
             "elements_per_s": pb.n_elem * steps / (mu["assembly_ms"] * 1e-3),
             "fp64_vs_reference_fp64_max_rel_diff": float(np.max(np.abs(fu["U"].numpy() - Ud)) /
+                                                         max(np.max(np.abs(Ud)), 1e-300))},
+        "contracted_arithmetic": {
+            "fp64": {"iters": fc["iters"], "solve_ms": fc["solve_ms"], "assembly_ms": fc["assembly_ms"],
+                     "ms_per_step": fc["total_ms"] / steps},
+            "mixed": {"iters": mc["iters"], "solve_ms": mc["solve_ms"], "assembly_ms": mc["assembly_ms"],
+                      "ms_per_step": mc["total_ms"] / steps, "dof_iters_per_s": mc["dof_iters_per_s"]},
+            "step_time_fp64_over_mixed": fc["total_ms"] / mc["total_ms"],
+            "assembly_ms_per_step": mc["assembly_ms"] / steps,
+            "elements_per_s": pb.n_elem * steps / (mc["assembly_ms"] * 1e-3),
+            "fp64_vs_reference_fp64_max_rel_diff": float(np.max(np.abs(fc["U"].numpy() - Ud)) /
                                                          max(np.max(np.abs(Ud)), 1e-300))},
         "mixed_over_fp64": {"dof_iters_per_s": mx["dof_iters_per_s"] / fp["dof_iters_per_s"],
                             "solve_time": fp["solve_ms"] / mx["solve_ms"],
