@@ -75,6 +75,17 @@ int rsd_ctx_destroy(rsd_ctx* ctx);
 int rsd_ctx_sync(rsd_ctx* ctx);
 /* the context's stream, as a hipStream_t, for callers that overlap work */
 int rsd_ctx_stream(rsd_ctx* ctx, void** stream);
+/* GMRES orthogonalisation for the context's later FP64 GMRES solves
+ * (rsd_gmres_solve_f64, Newton's GMRES inner solves; no reference
+ * counterpart).  RSD_ORTHO_MGS (the default): the reference's modified
+ * Gram-Schmidt (gmres/Solver.tpp:71-78), bitwise, one fold of block sums
+ * per basis vector.  RSD_ORTHO_CGS2: classical Gram-Schmidt with one
+ * reorthogonalisation -- h = Q^T w, w -= Q h, c = Q^T w, w -= Q c,
+ * H[:, j] = h + c -- three passes over the basis and three folds per Arnoldi
+ * step; iterates differ from the reference's by rounding. */
+#define RSD_ORTHO_MGS 0
+#define RSD_ORTHO_CGS2 1
+int rsd_ctx_set_gmres_orthogonalization(rsd_ctx* ctx, int32_t mode);
 const char* rsd_last_error_string(void);
 const char* rsd_version(void);
 /* build stamp (no reference counterpart): the first 16 hex digits of the

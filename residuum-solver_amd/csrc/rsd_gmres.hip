@@ -4,7 +4,9 @@
 // Modified Gram-Schmidt runs as one fused kernel per basis vector: it
 // finishes the previous projection w -= h_{i-1,j} q_{i-1} element by element
 // and accumulates q_i . w for the next coefficient, which is exactly the
-// reference's dot-then-axpy sequence.  The Hessenberg column, the Givens
+// reference's dot-then-axpy sequence.  A context may choose classical
+// Gram-Schmidt with one reorthogonalisation instead (RSD_ORTHO_CGS2: three
+// passes and three folds per step; not bitwise the reference).  The Hessenberg column, the Givens
 // rotations and the back-substitution are scalar work done by one device
 // thread.  A whole cycle is launched at once: after the stop flag is set
 // the remaining kernels of the cycle return immediately, and the host reads
@@ -70,6 +72,197 @@ __global__ __launch_bounds__(kFold) void k_gm_mgs_fold(int64_t nb, const double*
     if (threadIdx.x != 0) return;
     if (norm) st->hnext = sqrt(out[0]);
     else H[(int64_t)j * (m + 1) + i] = out[0];
+}
+
+// ------------------------------------------------------------------------
+// Classical Gram-Schmidt with one reorthogonalisation (CGS2, the context's
+// RSD_ORTHO_CGS2; not the reference's modified Gram-Schmidt, so results
+// differ from it by rounding).  Per Arnoldi step, after w = A M q_j:
+//   h  = Q^T w                 (k_gm_dots: one pass over w and the basis)
+//   w -= Q h, c = Q^T w        (k_gm_upd_dots: one pass)
+//   w -= Q c, ||w||^2          (k_gm_upd_norm: one pass)
+//   H[:, j] = h + c, h_{j+1,j} = ||w||
+// -- three block-sum folds per step instead of j + 2.  Each dot's block sums
+// use the fixed tree of every other reduction here.
+// ------------------------------------------------------------------------
+constexpr int kCgsMax = 32;  // basis vectors per pass of the multi-vector kernels
+
+// DPP lane moves of a double (bound_ctrl: a lane with no source reads 0;
+// lanes outside the row mask read 0 as well)
+template<int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int2 s = __builtin_bit_cast(int2, v);
+    int2 d;
+    d.x = __builtin_amdgcn_update_dpp(0, s.x, CTRL, ROWS, 0xf, true);
+    d.y = __builtin_amdgcn_update_dpp(0, s.y, CTRL, ROWS, 0xf, true);
+    return __builtin_bit_cast(double, d);
+}
+
+// Wavefront sum in a fixed tree of VALU lane moves: row_shr 1, 2, 4, 8
+// within each 16-lane row (lane 15 of a row then holds the row's sum),
+// row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2 and 3; lane 63
+// ends with the sum.  A many-value block sum through __shfl_down (two
+// ds_bpermute per step and value) cost the multi-dot kernel a third of its
+// time.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v = v + dpp_f64<0x111, 0xf>(v);
+    v = v + dpp_f64<0x112, 0xf>(v);
+    v = v + dpp_f64<0x114, 0xf>(v);
+    v = v + dpp_f64<0x118, 0xf>(v);
+    v = v + dpp_f64<0x142, 0xa>(v);
+    v = v + dpp_f64<0x143, 0xc>(v);
+    return v;
+}
+
+// block sums of cnt (<= kCgsMax, which the launches keep) values per thread, value c's sum of block lb
+// stored at partials[c * nb + lb]: each value's four wavefront sums
+// (wave_sum_dpp), then thread c adds them left to right
+__device__ __forceinline__ void block_partials_many(double (&v)[kCgsMax], int cnt, int64_t lb, int64_t nb,
+                                                    double* partials) {
+    __shared__ double sm[kCgsMax * 4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < kCgsMax; ++c) {
+        if (c < cnt) {  // cnt is uniform: whole wavefronts take the lane moves
+            const double t = wave_sum_dpp(v[c]);
+            if (lane == 63) sm[c * 4 + wave] = t;
+        }
+    }
+    __syncthreads();
+    const int c = threadIdx.x;
+    if (c < cnt) {
+        double r = sm[c * 4 + 0];
+        r += sm[c * 4 + 1];
+        r += sm[c * 4 + 2];
+        r += sm[c * 4 + 3];
+        partials[(int64_t)c * nb + lb] = r;
+    }
+}
+
+// The multi-dot kernels give each thread kCgsTiles rows, kBlock apart (its
+// products added in row order), so a block's cnt-value reduction is paid once
+// per kCgsTiles * kBlock rows: with one row per thread it took more time than
+// the block's loads.
+#ifndef RSD_CGS_TILES
+#define RSD_CGS_TILES 4
+#endif
+constexpr int kCgsTiles = RSD_CGS_TILES;
+inline int64_t cgs_blocks(int64_t n) { return std::max<int64_t>(1, (n + kCgsTiles * kBlock - 1) / (kCgsTiles * kBlock)); }
+
+// q[c] = Q[i0 + c][k] for c < cnt, 0 past it: every load is issued before
+// any of them is used (a load and its use under one guard would wait for
+// each load in turn)
+__device__ __forceinline__ void cgs_load(double (&q)[kCgsMax], const double* const* Q, int32_t i0, int32_t cnt,
+                                         int64_t k) {
+#pragma unroll
+    for (int c = 0; c < kCgsMax; ++c) {
+        q[c] = 0.0;
+        if (c < cnt) q[c] = Q[i0 + c][k];
+    }
+}
+
+// block sums of q_{i0+c} . w, c < cnt
+__global__ __launch_bounds__(kBlock) void k_gm_dots(int64_t n, int64_t nb, const double* __restrict__ w,
+                                                    const double* const* __restrict__ Q, int32_t i0, int32_t cnt,
+                                                    const GMState* st, double* partials) {
+    if (st->stop) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    double v[kCgsMax];
+#pragma unroll
+    for (int c = 0; c < kCgsMax; ++c) v[c] = 0.0;
+#pragma unroll
+    for (int t = 0; t < kCgsTiles; ++t) {
+        const int64_t k = (lb * kCgsTiles + t) * kBlock + threadIdx.x;
+        if (k < n) {
+            const double wk = w[k];
+            double q[kCgsMax];
+            cgs_load(q, Q, i0, cnt, k);
+#pragma unroll
+            for (int c = 0; c < kCgsMax; ++c)
+                if (c < cnt) v[c] += q[c] * wk;
+        }
+    }
+    block_partials_many(v, cnt, lb, nb, partials + (int64_t)i0 * nb);
+}
+
+// w -= sum_i h_i q_i (i = 0..cnt-1 in order), then, with dots, the block sums
+// of q_i . w for the new w, or, without, of w . w
+__device__ __forceinline__ double cgs_subtract(int64_t k, const double* w, const double* const* Q, const double* h,
+                                               int32_t cnt) {
+    double wk = w[k];
+#pragma unroll 8
+    for (int i = 0; i < cnt; ++i) wk = wk + -h[i] * Q[i][k];
+    return wk;
+}
+
+// (the dots of q_0 .. q_{cd-1}, cd <= kCgsMax; k_gm_dots forms the rest)
+__global__ __launch_bounds__(kBlock) void k_gm_upd_dots(int64_t n, int64_t nb, double* __restrict__ w,
+                                                        const double* const* __restrict__ Q, const double* h,
+                                                        int32_t cnt, int32_t cd, const GMState* st,
+                                                        double* partials) {
+    if (st->stop) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    double v[kCgsMax];
+#pragma unroll
+    for (int c = 0; c < kCgsMax; ++c) v[c] = 0.0;
+#pragma unroll
+    for (int t = 0; t < kCgsTiles; ++t) {
+        const int64_t k = (lb * kCgsTiles + t) * kBlock + threadIdx.x;
+        if (k < n) {
+            double q[kCgsMax];
+            double wk = w[k];
+            // w -= sum_i h_i q_i in order, kCgsMax basis entries at a time; the
+            // dots then use the first chunk (loaded again if a later one replaced it)
+            for (int32_t i0 = 0; i0 < cnt; i0 += kCgsMax) {
+                const int32_t cc = cnt - i0 < kCgsMax ? cnt - i0 : kCgsMax;
+                cgs_load(q, Q, i0, cc, k);
+#pragma unroll
+                for (int c = 0; c < kCgsMax; ++c)
+                    if (c < cc) wk = wk + -h[i0 + c] * q[c];
+            }
+            w[k] = wk;
+            if (cnt > kCgsMax) cgs_load(q, Q, 0, cd, k);
+#pragma unroll
+            for (int c = 0; c < kCgsMax; ++c)
+                if (c < cd) v[c] += q[c] * wk;
+        }
+    }
+    block_partials_many(v, cd, lb, nb, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gm_upd_norm(int64_t n, int64_t nb, double* __restrict__ w,
+                                                        const double* const* __restrict__ Q, const double* h,
+                                                        int32_t cnt, const GMState* st, double* partials) {
+    if (st->stop) return;
+    const int64_t lb = xcd_block(blockIdx.x, nb);
+    const int64_t k = lb * kBlock + threadIdx.x;
+    double v[1] = {0.0};
+    if (k < n) {
+        const double wk = cgs_subtract(k, w, Q, h, cnt);
+        w[k] = wk;
+        v[0] = wk * wk;
+    }
+    block_partials<1>(v, lb, nb, partials);
+}
+
+// one block per array: out[c] = the fold of partials[c * nb ..]
+__global__ __launch_bounds__(kFold) void k_gm_fold_many(int64_t nb, const double* __restrict__ partials,
+                                                        double* out, const GMState* st) {
+    if (st->stop) return;
+    double o[1];
+    fold_partials<1>(nb, partials + (int64_t)blockIdx.x * nb, o);
+    if (threadIdx.x == 0) out[blockIdx.x] = o[0];
+}
+
+// h_{j+1,j} = ||w|| and H[:, j] = h + c (one block)
+__global__ __launch_bounds__(kFold) void k_gm_norm_fold(int64_t nb, const double* __restrict__ partials, int32_t j,
+                                                        int32_t m, const double* __restrict__ c, double* H,
+                                                        GMState* st) {
+    if (st->stop) return;
+    double o[1];
+    fold_partials<1>(nb, partials, o);
+    if (threadIdx.x == 0) st->hnext = sqrt(o[0]);
+    if (threadIdx.x <= j) H[(int64_t)j * (m + 1) + threadIdx.x] += c[threadIdx.x];
 }
 
 // q_{j+1} = w * (1 / hnext) unless the column broke down
@@ -194,7 +387,11 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
     double* y = (double*)ctx->ws("gm.y", sizeof(double) * (size_t)(m + 1));
     double** dQ = (double**)ctx->ws("gm.Qptr", sizeof(double*) * (size_t)(m + 1));
     GMState* dst = (GMState*)ctx->ws("gm.state", sizeof(GMState));
-    double* part = ctx->partials(nb);
+    const bool cgs2 = ctx->gmres_orth == RSD_ORTHO_CGS2;
+    double* hc = cgs2 ? (double*)ctx->ws("gm.hc", sizeof(double) * (size_t)(m + 1)) : nullptr;
+    // MGS: one dot at a time; CGS2: up to m dots at a time
+    const int64_t nbc = cgs_blocks(n);  // blocks of the multi-dot kernels
+    double* part = ctx->partials(cgs2 ? std::max<int64_t>((int64_t)m * nbc, nb) : nb);
     RSD_HIP(hipMemcpyAsync(dQ, hq.data(), sizeof(double*) * (size_t)(m + 1), hipMemcpyHostToDevice, st));
     RSD_HIP(hipMemsetAsync(dst, 0, sizeof(GMState), st));
     if (invd) launch_inv_diag(ctx, A, invd);
@@ -229,12 +426,35 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
                 for (int j = 0; j < m; ++j) {
                     RSD_LAUNCH_COLS(S.mode(), k_gm_apply, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(),
                                     A->val.p, hq[j], invd, w, dst);
-                    for (int i = 0; i <= j + 1; ++i) {
-                        const bool norm = i == j + 1;
-                        hipLaunchKernelGGL(k_gm_mgs, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
-                                           i > 0 ? hq[i - 1] : nullptr, norm ? nullptr : hq[i], i, j, m, H, dst, part);
-                        hipLaunchKernelGGL(k_gm_mgs_fold, dim3(1), dim3(kFold), 0, st, nb, part, i, j, m,
-                                           (int32_t)norm, H, dst);
+                    if (cgs2) {
+                        // h = Q^T w (passes of kCgsMax vectors), w -= Q h with c = Q^T w,
+                        // w -= Q c with ||w||^2, H[:, j] = h + c
+                        const int32_t cnt = j + 1;
+                        double* hcol = H + (int64_t)j * (m + 1);
+                        for (int32_t i0 = 0; i0 < cnt; i0 += kCgsMax)
+                            hipLaunchKernelGGL(k_gm_dots, dim3((unsigned)nbc), dim3(kBlock), 0, st, n, nbc, w,
+                                               (const double* const*)dQ, i0, std::min(kCgsMax, cnt - i0), dst, part);
+                        hipLaunchKernelGGL(k_gm_fold_many, dim3((unsigned)cnt), dim3(kFold), 0, st, nbc, part, hcol,
+                                           dst);
+                        hipLaunchKernelGGL(k_gm_upd_dots, dim3((unsigned)nbc), dim3(kBlock), 0, st, n, nbc, w,
+                                           (const double* const*)dQ, hcol, cnt, std::min(kCgsMax, cnt), dst, part);
+                        for (int32_t i0 = kCgsMax; i0 < cnt; i0 += kCgsMax)
+                            hipLaunchKernelGGL(k_gm_dots, dim3((unsigned)nbc), dim3(kBlock), 0, st, n, nbc, w,
+                                               (const double* const*)dQ, i0, std::min(kCgsMax, cnt - i0), dst, part);
+                        hipLaunchKernelGGL(k_gm_fold_many, dim3((unsigned)cnt), dim3(kFold), 0, st, nbc, part, hc,
+                                           dst);
+                        hipLaunchKernelGGL(k_gm_upd_norm, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
+                                           (const double* const*)dQ, hc, cnt, dst, part);
+                        hipLaunchKernelGGL(k_gm_norm_fold, dim3(1), dim3(kFold), 0, st, nb, part, j, m, hc, H, dst);
+                    } else {
+                        for (int i = 0; i <= j + 1; ++i) {
+                            const bool norm = i == j + 1;
+                            hipLaunchKernelGGL(k_gm_mgs, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
+                                               i > 0 ? hq[i - 1] : nullptr, norm ? nullptr : hq[i], i, j, m, H, dst,
+                                               part);
+                            hipLaunchKernelGGL(k_gm_mgs_fold, dim3(1), dim3(kFold), 0, st, nb, part, i, j, m,
+                                               (int32_t)norm, H, dst);
+                        }
                     }
                     hipLaunchKernelGGL(k_gm_newq, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, w, hq[j + 1], dst);
                     hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, st, j, m, H, cs, sn, g, dst, tol,

@@ -211,6 +211,7 @@ struct rsd_ctx {
     hipStream_t stream = nullptr;
     rsd::Scratch scratch;
     double* d_scalars = nullptr;  // small device scalar area for BLAS-1 results
+    int32_t gmres_orth = RSD_ORTHO_MGS;  // GMRES orthogonalisation (rsd_ctx_set_gmres_orthogonalization)
     // make sure the partials area can hold `need` doubles
     double* partials(int64_t need);
     // named device workspaces, grown on demand and kept for the context's life

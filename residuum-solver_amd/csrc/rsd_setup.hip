@@ -666,6 +666,14 @@ int rsd_ctx_sync(rsd_ctx* ctx) {
     return guard([&] { RSD_CHECK(ctx, "null ctx"); RSD_HIP(hipStreamSynchronize(ctx->stream)); });
 }
 
+int rsd_ctx_set_gmres_orthogonalization(rsd_ctx* ctx, int32_t mode) {
+    return guard([&] {
+        RSD_CHECK(ctx, "null ctx");
+        RSD_CHECK(mode == RSD_ORTHO_MGS || mode == RSD_ORTHO_CGS2, "unknown GMRES orthogonalisation");
+        ctx->gmres_orth = mode;
+    });
+}
+
 int rsd_ctx_stream(rsd_ctx* ctx, void** stream) {
     return guard([&] { RSD_CHECK(ctx && stream, "null argument"); *stream = (void*)ctx->stream; });
 }

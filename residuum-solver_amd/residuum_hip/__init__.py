@@ -25,6 +25,7 @@ SEL_VALUE, SEL_RATE, SEL_OPERAND = 0, 1, 2
 PC_IDENTITY, PC_JACOBI = 0, 1
 TOL_RELATIVE, TOL_ABSOLUTE = 0, 1
 ARITH_REFERENCE, ARITH_CONTRACTED, ARITH_UNIFORM, ARITH_FACTORED = 0, 1, 2, 3
+ORTHO_MGS, ORTHO_CGS2 = 0, 1
 
 _d = C.POINTER(C.c_double)
 _f = C.POINTER(C.c_float)
@@ -109,6 +110,12 @@ class Ctx:
 
     def sync(self):
         check(lib().rsd_ctx_sync(self.h))
+
+    def set_gmres_orthogonalization(self, mode):
+        """ORTHO_MGS (default, the reference's modified Gram-Schmidt, bitwise)
+        or ORTHO_CGS2 (classical Gram-Schmidt, reorthogonalised once: three
+        folds per Arnoldi step) for this context's later FP64 GMRES solves."""
+        check(lib().rsd_ctx_set_gmres_orthogonalization(self.h, C.c_int32(mode)))
 
     def close(self):
         """Release every object created on this context, then the context."""

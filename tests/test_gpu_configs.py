@@ -211,12 +211,14 @@ MIXED_CG_STALLS = ("mixed CG on the full FP32 copy of this anisotropic, non-symm
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,lin_solver,lin_tol", [
-    (512, 1, 1e-3), (512, 2, 1e-2), (128, 1, 1e-12), (128, 2, 1e-10),
-    pytest.param(512, 3, 1e-10, marks=pytest.mark.skip(reason=MIXED_CG_STALLS)),
-    pytest.param(128, 3, 1e-10, marks=pytest.mark.skip(reason=MIXED_CG_STALLS))],
-    ids=["512-gmres30", "512-gmres30-mixed", "128-gmres30", "128-gmres30-mixed", "512-cg-mixed", "128-cg-mixed"])
-def test_newton_q2(ctx, n, lin_solver, lin_tol):
+@pytest.mark.parametrize("n,lin_solver,lin_tol,ortho", [
+    (512, 1, 1e-3, 0), (512, 1, 1e-3, 1), (512, 2, 1e-2, 0), (128, 1, 1e-12, 0), (128, 1, 1e-12, 1),
+    (128, 2, 1e-10, 0),
+    pytest.param(512, 3, 1e-10, 0, marks=pytest.mark.skip(reason=MIXED_CG_STALLS)),
+    pytest.param(128, 3, 1e-10, 0, marks=pytest.mark.skip(reason=MIXED_CG_STALLS))],
+    ids=["512-gmres30", "512-gmres30-cgs2", "512-gmres30-mixed", "128-gmres30", "128-gmres30-cgs2",
+         "128-gmres30-mixed", "512-cg-mixed", "128-cg-mixed"])
+def test_newton_q2(ctx, n, lin_solver, lin_tol, ortho):
     """Q2 quads on the unit square (3x3 Gauss), k(T) = (1 + 0.5 T)
     diag(1, 0.1), f = 10 sin(pi x) sin(pi y), T = 0 on every side; Newton
     (relative and absolute 1e-10) with GMRES(30) + Jacobi inner solves,
@@ -232,7 +234,8 @@ def test_newton_q2(ctx, n, lin_solver, lin_tol):
     half the outer tolerance, one refinement cycle per Newton step).  At
     128^2 FP64 GMRES(30) runs to 1e-12 and the mixed solve to 1e-10: the
     refinement's attainable relative residual on these Jacobians is about
-    1e-12."""
+    1e-12.  The cgs2 cases run the FP64 GMRES with classical Gram-Schmidt
+    reorthogonalised once (RSD_ORTHO_CGS2) under the same checks."""
     import residuum_hip as rh
     t0 = time.perf_counter()
     pb = rh.Problem(ctx, 2, (n, n), (2, 2), 3, [0, 1, 2, 3], ["0"] * 4)
@@ -242,8 +245,12 @@ def test_newton_q2(ctx, n, lin_solver, lin_tol):
     U = pb.vector(0.0)
     ctx.sync()
     t1 = time.perf_counter()
-    its, res, lin = pb.newton(U, src=src, abs_tol=1e-10, rel_tol=1e-10, lin_solver=lin_solver, lin_tol=lin_tol,
-                              lin_max=100000, krylov=30, max_newton=30)
+    ctx.set_gmres_orthogonalization(ortho)
+    try:
+        its, res, lin = pb.newton(U, src=src, abs_tol=1e-10, rel_tol=1e-10, lin_solver=lin_solver, lin_tol=lin_tol,
+                                  lin_max=100000, krylov=30, max_newton=30)
+    finally:
+        ctx.set_gmres_orthogonalization(rh.ORTHO_MGS)
     t1 = time.perf_counter() - t1
     say(f"newton: {its} iterations in {t1*1e3:.0f} ms, inner iterations {list(lin)}, residuals {list(res)}")
     assert its > 0 and (res[-1] < 1e-10 or res[-1] < 1e-10 * res[0])

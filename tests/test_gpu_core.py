@@ -923,6 +923,50 @@ def test_gmres_bitexact(ctx, case, precond):
 
 @pytest.mark.parametrize("case", CASES[:3])
 @pytest.mark.parametrize("precond", [0, 1])
+def test_gmres_cgs2(ctx, case, precond):
+    """RSD_ORTHO_CGS2 (classical Gram-Schmidt, reorthogonalised once) on the
+    non-symmetric Jacobians of test_gmres_bitexact: not bitwise the
+    reference's modified Gram-Schmidt, so against a dense solve -- converged
+    at 1e-12 with the recomputed residual below 1e-11 of ||b||, the solution
+    within 1e-9 of the dense one, and iteration counts within 5 % (at least
+    2) of the oracle's MGS.  Krylov dimensions past 32 take the multi-vector
+    passes in two chunks."""
+    dim, n, p, nq, tags, exprs, ext = case
+    import residuum_hip as rh
+    tensor = (1.0, 0.2, 0.2, 0.1) if dim == 2 else (1, 0.2, 0, 0.2, 0.1, 0, 0, 0, 1)
+    pb = rh.Problem(ctx, dim, n, p, nq, tags, exprs, ext=ext)
+    pb.set_conductivity(1.0, k_expr="1.0 + 0.5*T", dk_expr="0.5", tensor=tensor)
+    oc = O.Case(dim, n, p, nq, tags, exprs, ext=ext, order=O.ORDER_GPU, k_expr="1.0 + 0.5*T", dk_expr="0.5",
+                tensor=tensor)
+    U = np.random.default_rng(4).random(oc.n_free)
+    forms = [(O.DIFFUSION, 1.0), (O.TDIFFUSION, 1.0)]
+    J = pb.matrix()
+    pb.assemble_matrix(J, forms, U=ctx.from_numpy(U))
+    val = oc.assemble_matrix(forms, U=U)
+    dense = np.zeros((oc.n_free, oc.n_free))
+    rows = np.repeat(np.arange(oc.n_free), np.diff(oc.rp))
+    dense[rows, oc.ci] = val
+    b = np.random.default_rng(5).standard_normal(oc.n_free)
+    xs = np.linalg.solve(dense, b)
+    ctx.set_gmres_orthogonalization(rh.ORTHO_CGS2)
+    try:
+        for krylov in (5, 30, 40):
+            dx = pb.vector(0.0)
+            rep = J.gmres(ctx.from_numpy(b), dx, tol=1e-12, max_iters=4000, krylov=krylov, precond=precond)
+            _, ro = oc.gmres(val, b, tol=1e-12, max_iters=4000, krylov=krylov, precond=precond)
+            x = dx.numpy()
+            err = np.max(np.abs(x - xs)) / np.max(np.abs(xs))
+            res = np.linalg.norm(b - dense @ x) / np.linalg.norm(b)
+            say("gmres cgs2", krylov, rep.iterations, ro.iterations, f"err {err:.2e} res {res:.2e}")
+            assert rep.converged == 1
+            assert abs(rep.iterations - ro.iterations) <= max(2, 0.05 * ro.iterations)
+            assert res <= 1e-11 and err <= 1e-9
+    finally:
+        ctx.set_gmres_orthogonalization(rh.ORTHO_MGS)
+
+
+@pytest.mark.parametrize("case", CASES[:3])
+@pytest.mark.parametrize("precond", [0, 1])
 def test_gmres_mixed(ctx, case, precond):
     """Mixed-precision GMRES on the non-symmetric tangent Jacobian: its
     solution's FP64 relative residual, recomputed here from an FP64 product,
