@@ -39,6 +39,12 @@ steady solve, the mixed Backward Euler loop, and Newton's mixed inner
 solves (GMRES, or CG on a full FP32 copy of the Jacobian).  It needs the
 csr operator.
 
+A gmres node may carry `orthogonalization: cgs2` (a device extension; the
+default `mgs` is the reference's modified Gram-Schmidt, bitwise): classical
+Gram-Schmidt reorthogonalised once, three folds per Arnoldi step instead of
+one per basis vector, for the FP64 GMRES solves (steady and Newton's inner
+solves).
+
 `discretization: arithmetic:` (a device extension; default `reference`, the
 reference's operation order, bitwise) selects the element arithmetic of the
 Backward Euler form pair: `contracted` (fused multiply-adds), `uniform`
@@ -47,6 +53,7 @@ or `factored` (on Q1 meshes of any shape, the fem operator's product
 without element matrices); all within a few 1e-15 of the reference order.
 """
 import argparse
+import contextlib
 import os
 import sys
 import time as _time
@@ -54,9 +61,11 @@ import time as _time
 import numpy as np
 
 from . import (ARITH_CONTRACTED, ARITH_FACTORED, ARITH_REFERENCE, ARITH_UNIFORM, DIFFUSION, MASS, SCALED, PC_IDENTITY, PC_JACOBI,
+               ORTHO_CGS2, ORTHO_MGS,
                SEL_VALUE, PMSH_HEX, PMSH_LAGRANGE, PMSH_QUAD, Ctx, Problem, RsdError, block_mesh, read_pmsh,
                write_pmsh)
 
+ORTHO = {"mgs": ORTHO_MGS, "cgs2": ORTHO_CGS2}
 ARITHMETIC = {"reference": ARITH_REFERENCE, "contracted": ARITH_CONTRACTED, "uniform": ARITH_UNIFORM,
               "factored": ARITH_FACTORED}
 
@@ -103,11 +112,16 @@ def _linear(node, where):
                krylov_dim=int(_opt(node, "krylov_dim", 50)),
                operator=_req(op, "type", where + ".operator") if op else "csr",
                preconditioner=_req(pc, "type", where + ".preconditioner") if pc else "identity",
-               precision=str(_opt(node, "precision", "fp64")))
+               precision=str(_opt(node, "precision", "fp64")),
+               orthogonalization=str(_opt(node, "orthogonalization", "mgs")))
     if cfg["operator"] not in ("csr", "fem"):
         raise ConfigError(f"Unknown linear solver operator type: {cfg['operator']}. Valid options: csr, fem")
     if cfg["precision"] not in ("fp64", "mixed"):
         raise ConfigError(f"{where}: unknown precision '{cfg['precision']}'. Valid options: fp64, mixed")
+    if cfg["orthogonalization"] not in ORTHO:
+        raise ConfigError(f"{where}: unknown orthogonalization '{cfg['orthogonalization']}'. Valid options: mgs, cgs2")
+    if cfg["orthogonalization"] != "mgs" and typ != "gmres":
+        raise ConfigError(f"{where}: orthogonalization applies to gmres")
     if cfg["precision"] == "mixed" and cfg["operator"] != "csr":
         raise ConfigError(f"{where}: mixed precision needs the csr operator")
     if cfg["preconditioner"] not in ("identity", "jacobi"):
@@ -445,6 +459,16 @@ def _precond(lin):
     return PC_JACOBI if lin["preconditioner"] == "jacobi" else PC_IDENTITY
 
 
+@contextlib.contextmanager
+def _orthogonalization(ctx, lin):
+    """The linear node's GMRES orthogonalisation on the context for one call."""
+    ctx.set_gmres_orthogonalization(ORTHO[lin["orthogonalization"]])
+    try:
+        yield
+    finally:
+        ctx.set_gmres_orthogonalization(ORTHO_MGS)
+
+
 def _solve(pb, K, forms, lin, F, U, state=None, time=0.0):
     """One linear solve with the configured operator and solver."""
     if lin["operator"] == "fem":
@@ -460,8 +484,9 @@ def _solve(pb, K, forms, lin, F, U, state=None, time=0.0):
     if mixed:
         return K.gmres_mixed(F, U, tol=lin["tolerance"], max_iters=lin["max_iterations"], krylov=lin["krylov_dim"],
                              precond=_precond(lin))[0]
-    return K.gmres(F, U, tol=lin["tolerance"], max_iters=lin["max_iterations"], krylov=lin["krylov_dim"],
-                   precond=_precond(lin))
+    with _orthogonalization(pb.ctx, lin):
+        return K.gmres(F, U, tol=lin["tolerance"], max_iters=lin["max_iterations"], krylov=lin["krylov_dim"],
+                       precond=_precond(lin))
 
 
 def _write(pb, cfg, mesh, U, step, t, quiet):
@@ -491,12 +516,13 @@ def _steady(pb, cfg, U, src_text, src_nodal, quiet, mon=None):
         if lin["operator"] != "csr":
             raise ConfigError("Newton runs with the CSR operator on the device")
         t0 = _time.perf_counter()
-        its, res, lin_its = pb.newton(U, src=src_text, abs_tol=nl["abs_tol"], rel_tol=nl["rel_tol"],
-                                      max_newton=nl["max_iterations"],
-                                      lin_solver={("cg", "fp64"): 0, ("gmres", "fp64"): 1, ("gmres", "mixed"): 2,
-                                                  ("cg", "mixed"): 3}[(lin["type"], lin["precision"])],
-                                      lin_tol=lin["tolerance"], lin_max=lin["max_iterations"],
-                                      krylov=lin["krylov_dim"], precond=_precond(lin))
+        with _orthogonalization(pb.ctx, lin):
+            its, res, lin_its = pb.newton(U, src=src_text, abs_tol=nl["abs_tol"], rel_tol=nl["rel_tol"],
+                                          max_newton=nl["max_iterations"],
+                                          lin_solver={("cg", "fp64"): 0, ("gmres", "fp64"): 1, ("gmres", "mixed"): 2,
+                                                      ("cg", "mixed"): 3}[(lin["type"], lin["precision"])],
+                                          lin_tol=lin["tolerance"], lin_max=lin["max_iterations"],
+                                          krylov=lin["krylov_dim"], precond=_precond(lin))
         conv = its >= 0
         _log(f"newton: {its} iterations, residuals {list(res)} ({_time.perf_counter() - t0:.3f} s)", quiet)
         _write(pb, cfg, mesh, U, 0, 0.0, quiet)

@@ -66,6 +66,26 @@ def test_steady_mixed_precision(ctx, tmp_path, solver):
     assert np.max(np.abs(s["U"] - (1 - xy[:, 0]) * (1 - xy[:, 1]))) <= 1e-10
 
 
+@pytest.mark.parametrize("ortho", ["mgs", "cgs2"])
+def test_steady_gmres_orthogonalization(ctx, tmp_path, ortho):
+    """A steady GMRES(30) + Jacobi solve with either orthogonalisation
+    (`orthogonalization: cgs2` is classical Gram-Schmidt reorthogonalised
+    once): the bilinear solution to 1e-10."""
+    import residuum_hip as rh
+    from residuum_hip import cli
+    d = bench_config(tmp_path)
+    d["solver"]["linear"].update(type="gmres", krylov_dim=30, orthogonalization=ortho)
+    d["solver"]["linear"]["operator"]["type"] = "csr"
+    d["solver"]["linear"]["preconditioner"]["type"] = "jacobi"
+    s = cli.run_heat(write_cfg(tmp_path, d), quiet=True, ctx=ctx)
+    say("steady gmres", ortho, s["iterations"], s["final_residual"])
+    assert s["converged"]
+    xy = nodes_xy(ctx, str(tmp_path / "square_n16.pmsh"))
+    assert np.max(np.abs(s["U"] - (1 - xy[:, 0]) * (1 - xy[:, 1]))) <= 1e-10
+    with pytest.raises(rh.RsdError, match="orthogonalisation"):
+        ctx.set_gmres_orthogonalization(7)
+
+
 def test_transient_mixed_precision(ctx, tmp_path):
     """The Backward Euler loop on the device with `precision: mixed` agrees
     with the FP64 loop to the solver tolerance."""

@@ -95,7 +95,7 @@ def test_heat_config_parse():
     assert cfg["source"] == ("expression", "0.0") and cfg["initial"] == ("expression", "0.0")
     assert cfg["driver"] == "steady"
     assert cfg["linear"] == dict(type="cg", tolerance=1e-12, max_iterations=10000, krylov_dim=50, operator="fem",
-                                 preconditioner="identity", precision="fp64")
+                                 preconditioner="identity", precision="fp64", orthogonalization="mgs")
 
 
 def test_heat_config_arithmetic(tmp_path):
@@ -125,6 +125,19 @@ def test_heat_config_precision(tmp_path):
     assert cli.parse_heat_config(str(p))["linear"]["precision"] == "mixed"
 
 
+def test_heat_config_orthogonalization(tmp_path):
+    """`orthogonalization: cgs2` on a gmres node selects classical
+    Gram-Schmidt with reorthogonalisation; the reference's mgs is the default."""
+    import yaml
+    from residuum_hip import cli
+    d = yaml.safe_load(open(os.path.join(GOLDEN, "heateq_square_n16.yaml")))
+    d["solver"]["linear"].update(type="gmres", orthogonalization="cgs2")
+    d["solver"]["linear"]["operator"]["type"] = "csr"
+    p = tmp_path / "c.yaml"
+    p.write_text(yaml.safe_dump(d))
+    assert cli.parse_heat_config(str(p))["linear"]["orthogonalization"] == "cgs2"
+
+
 @pytest.mark.parametrize("edit,msg", [
     (lambda d: d.update(backend="cpu"), "backend"),
     (lambda d: d["solver"]["linear"].update(type="bicgstab"), "bicgstab"),
@@ -138,6 +151,8 @@ def test_heat_config_precision(tmp_path):
     (lambda d: d["solver"]["linear"].update(precision="fp16"), "precision"),
     (lambda d: d["solver"]["linear"].update(precision="mixed"), "csr operator"),
     (lambda d: d["discretization"].update(arithmetic="fast"), "arithmetic"),
+    (lambda d: d["solver"]["linear"].update(orthogonalization="householder"), "orthogonalization"),
+    (lambda d: d["solver"]["linear"].update(orthogonalization="cgs2"), "applies to gmres"),
 ])
 def test_heat_config_refusals(edit, msg, tmp_path):
     """What the device path does not run is refused with a message."""
