@@ -341,6 +341,52 @@ def run_batched(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
     print(json.dumps(line), flush=True)
 
 
+def run_newton(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
+    """The nonlinear workload (not the default line): 512x512 Q2 quads on the
+    unit square (3x3 Gauss), k(T) = (1 + 0.5 T) diag(1, 0.1), source
+    10 sin(pi x) sin(pi y), T = 0 on the boundary; Newton to relative and
+    absolute 1e-10 with FP64 GMRES(30) + Jacobi inner solves to 1e-3
+    (inexact Newton steps: GMRES(30) stagnates near 1e-4 at this size), once
+    with the reference's modified Gram-Schmidt (bitwise) and once with
+    classical Gram-Schmidt reorthogonalised once."""
+    n = args.newton_n
+    src = "10*sin(pi*x)*sin(pi*y)"
+    pb = rh.Problem(ctx, 2, (n, n), (2, 2), 3, [0, 1, 2, 3], ["0"] * 4)
+    pb.set_conductivity(1.0, k_expr="1.0 + 0.5*T", dk_expr="0.5", tensor=(1.0, 0.0, 0.0, 0.1))
+    runs = {}
+    for name, ortho in (("mgs", rh.ORTHO_MGS), ("cgs2", rh.ORTHO_CGS2)):
+        U = pb.vector(0.0)
+        ctx.set_gmres_orthogonalization(ortho)
+        ctx.sync()
+        e0, e1 = new_ev(), new_ev()
+        rec_ev(e0, stream)
+        its, res, lin = pb.newton(U, src=src, abs_tol=1e-10, rel_tol=1e-10, lin_solver=1, lin_tol=1e-3,
+                                  lin_max=100000, krylov=30, max_newton=30)
+        rec_ev(e1, stream)
+        ms = ev_ms(e0, e1)
+        ctx.set_gmres_orthogonalization(rh.ORTHO_MGS)
+        steps = int(np.sum(lin))
+        F, KU = pb.vector(), pb.vector()
+        pb.assemble_load(F, src=src)
+        pb.assemble_vector(KU, [(rh.DIFFUSION, 1.0)], U=U)
+        rr = float(np.linalg.norm(F.numpy() - KU.numpy()))
+        runs[name] = dict(ms=ms, newton_iterations=int(its), arnoldi_steps=steps, us_per_arnoldi_step=ms * 1e3 / steps,
+                          residuals=[float(r) for r in res], recomputed_residual=rr, U=U.numpy())
+        print(f"# newton {name}: {its} iterations, {steps} Arnoldi steps in {ms:.0f} ms "
+              f"({ms * 1e3 / steps:.1f} us per step), recomputed ||F - K(U)U|| {rr:.2e}", file=sys.stderr, flush=True)
+    a, b = runs["mgs"].pop("U"), runs["cgs2"].pop("U")
+    line = {
+        "metric": "Newton + GMRES(30) wall time (512^2 Q2 nonlinear steady heat)", "value": runs["cgs2"]["ms"],
+        "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "f64",
+        "data": "synthetic (the nonlinear configuration's closed-form coefficients)",
+        "config": {"workload": f"nonlinear steady heat, {n}x{n} Q2 quads, k(T) anisotropic, Newton 1e-10, "
+                               "GMRES(30)+Jacobi inner 1e-3", "dofs": pb.n_free, "nnz": pb.nnz},
+        "mgs": runs["mgs"], "cgs2": runs["cgs2"], "mgs_over_cgs2_time": runs["mgs"]["ms"] / runs["cgs2"]["ms"],
+        "cgs2_vs_mgs_solution_max_rel_diff": float(np.max(np.abs(a - b)) / max(np.max(np.abs(a)), 1e-300)),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def renumbered_workload(rh, ctx, stream, n, kappa_elem, dt, capture, new_ev, rec_ev, ev_ms, reps, bytes_of):
     """The benchmark's operator on the same 128^3 mesh with its nodes
     renumbered in 8^3-node tiles (tile-major, lexicographic inside a tile):
@@ -564,8 +610,10 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=320, help="oracle CG iterations timed for the CPU baseline (about 10 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-matrix-free", action="store_true", help="skip the matrix-free operator solve")
-    ap.add_argument("--workload", choices=["be128", "batched"], default="be128",
-                    help="be128: the Backward Euler line (default); batched: the batched-solve workload")
+    ap.add_argument("--workload", choices=["be128", "batched", "newton"], default="be128",
+                    help="be128: the Backward Euler line (default); batched: the batched-solve workload; "
+                         "newton: the nonlinear Q2 workload with both GMRES orthogonalisations")
+    ap.add_argument("--newton-n", type=int, default=512, help="elements per edge of the newton workload")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--tune-cache", default=None, help="JSON schedule cache for the batched workload")
     ap.add_argument("--no-renumbered", action="store_true",
@@ -600,6 +648,9 @@ def main():
     rh.check(rh.lib().rsd_ctx_stream(ctx.h, C.byref(stream)))
     if args.workload == "batched":
         run_batched(args, rh, ctx, stream, new_ev, rec_ev, ev_ms)
+        return
+    if args.workload == "newton":
+        run_newton(args, rh, ctx, stream, new_ev, rec_ev, ev_ms)
         return
     n = args.n
     dt = 1e-3
