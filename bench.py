@@ -345,22 +345,25 @@ def run_newton(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
     """The nonlinear workload (not the default line): 512x512 Q2 quads on the
     unit square (3x3 Gauss), k(T) = (1 + 0.5 T) diag(1, 0.1), source
     10 sin(pi x) sin(pi y), T = 0 on the boundary; Newton to relative and
-    absolute 1e-10 with FP64 GMRES(30) + Jacobi inner solves to 1e-3
-    (inexact Newton steps: GMRES(30) stagnates near 1e-4 at this size), once
-    with the reference's modified Gram-Schmidt (bitwise) and once with
-    classical Gram-Schmidt reorthogonalised once."""
+    absolute 1e-10 with GMRES(30) + Jacobi inner solves: FP64 to 1e-3
+    (inexact Newton steps: GMRES(30) stagnates near 1e-4 at this size) and
+    mixed precision (FP32 inner GMRES, FP64 refinement) to 1e-2, each with the
+    reference's modified Gram-Schmidt and with classical Gram-Schmidt
+    reorthogonalised once."""
     n = args.newton_n
     src = "10*sin(pi*x)*sin(pi*y)"
     pb = rh.Problem(ctx, 2, (n, n), (2, 2), 3, [0, 1, 2, 3], ["0"] * 4)
     pb.set_conductivity(1.0, k_expr="1.0 + 0.5*T", dk_expr="0.5", tensor=(1.0, 0.0, 0.0, 0.1))
     runs = {}
-    for name, ortho in (("mgs", rh.ORTHO_MGS), ("cgs2", rh.ORTHO_CGS2)):
+    for name, ortho, lin_solver, lin_tol in (("mgs", rh.ORTHO_MGS, 1, 1e-3), ("cgs2", rh.ORTHO_CGS2, 1, 1e-3),
+                                             ("mixed_mgs", rh.ORTHO_MGS, 2, 1e-2),
+                                             ("mixed_cgs2", rh.ORTHO_CGS2, 2, 1e-2)):
         U = pb.vector(0.0)
         ctx.set_gmres_orthogonalization(ortho)
         ctx.sync()
         e0, e1 = new_ev(), new_ev()
         rec_ev(e0, stream)
-        its, res, lin = pb.newton(U, src=src, abs_tol=1e-10, rel_tol=1e-10, lin_solver=1, lin_tol=1e-3,
+        its, res, lin = pb.newton(U, src=src, abs_tol=1e-10, rel_tol=1e-10, lin_solver=lin_solver, lin_tol=lin_tol,
                                   lin_max=100000, krylov=30, max_newton=30)
         rec_ev(e1, stream)
         ms = ev_ms(e0, e1)
@@ -375,6 +378,7 @@ def run_newton(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
         print(f"# newton {name}: {its} iterations, {steps} Arnoldi steps in {ms:.0f} ms "
               f"({ms * 1e3 / steps:.1f} us per step), recomputed ||F - K(U)U|| {rr:.2e}", file=sys.stderr, flush=True)
     a, b = runs["mgs"].pop("U"), runs["cgs2"].pop("U")
+    am, bm = runs["mixed_mgs"].pop("U"), runs["mixed_cgs2"].pop("U")
     line = {
         "metric": "Newton + GMRES(30) wall time (512^2 Q2 nonlinear steady heat)", "value": runs["cgs2"]["ms"],
         "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "f64",
@@ -383,6 +387,8 @@ def run_newton(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
                                "GMRES(30)+Jacobi inner 1e-3", "dofs": pb.n_free, "nnz": pb.nnz},
         "mgs": runs["mgs"], "cgs2": runs["cgs2"], "mgs_over_cgs2_time": runs["mgs"]["ms"] / runs["cgs2"]["ms"],
         "cgs2_vs_mgs_solution_max_rel_diff": float(np.max(np.abs(a - b)) / max(np.max(np.abs(a)), 1e-300)),
+        "mixed_mgs": runs["mixed_mgs"], "mixed_cgs2": runs["mixed_cgs2"],
+        "mixed_cgs2_vs_fp64_mgs_solution_max_rel_diff": float(np.max(np.abs(bm - a)) / max(np.max(np.abs(a)), 1e-300)),
     }
     print(json.dumps(line), flush=True)
 

@@ -75,9 +75,9 @@ int rsd_ctx_destroy(rsd_ctx* ctx);
 int rsd_ctx_sync(rsd_ctx* ctx);
 /* the context's stream, as a hipStream_t, for callers that overlap work */
 int rsd_ctx_stream(rsd_ctx* ctx, void** stream);
-/* GMRES orthogonalisation for the context's later FP64 GMRES solves
- * (rsd_gmres_solve_f64, Newton's GMRES inner solves; no reference
- * counterpart).  RSD_ORTHO_MGS (the default): the reference's modified
+/* GMRES orthogonalisation for the context's later GMRES solves
+ * (rsd_gmres_solve_f64, rsd_gmres_solve_mixed's FP32 inner GMRES, Newton's
+ * GMRES inner solves; no reference counterpart).  RSD_ORTHO_MGS (the default): the reference's modified
  * Gram-Schmidt (gmres/Solver.tpp:71-78), bitwise, one fold of block sums
  * per basis vector.  RSD_ORTHO_CGS2: classical Gram-Schmidt with one
  * reorthogonalisation -- h = Q^T w, w -= Q h, c = Q^T w, w -= Q c,

@@ -149,37 +149,48 @@ __device__ __forceinline__ void block_partials_many(double (&v)[kCgsMax], int cn
 constexpr int kCgsTiles = RSD_CGS_TILES;
 inline int64_t cgs_blocks(int64_t n) { return std::max<int64_t>(1, (n + kCgsTiles * kBlock - 1) / (kCgsTiles * kBlock)); }
 
+// The basis type's arithmetic: FP64 basis vectors (the FP64 solver) update
+// w with separately rounded products and sums; FP32 ones (the mixed solver's
+// inner GMRES) with FP32 fused multiply-adds, as its MGS kernel does.  Dots
+// sum the exact FP64 products of both.
+__device__ __forceinline__ double cgs_sub(double wk, double h, double q) { return wk + -h * q; }
+__device__ __forceinline__ float cgs_sub(float wk, double h, float q) { return __builtin_fmaf(-(float)h, q, wk); }
+
 // q[c] = Q[i0 + c][k] for c < cnt, 0 past it: every load is issued before
 // any of them is used (a load and its use under one guard would wait for
 // each load in turn)
-__device__ __forceinline__ void cgs_load(double (&q)[kCgsMax], const double* const* Q, int32_t i0, int32_t cnt,
-                                         int64_t k) {
+template<typename T>
+__device__ __forceinline__ void cgs_load(T (&q)[kCgsMax], const T* const* Q, int32_t i0, int32_t cnt, int64_t k) {
 #pragma unroll
     for (int c = 0; c < kCgsMax; ++c) {
-        q[c] = 0.0;
+        q[c] = T(0);
         if (c < cnt) q[c] = Q[i0 + c][k];
     }
 }
 
 // block sums of q_{i0+c} . w, c < cnt
-__global__ __launch_bounds__(kBlock) void k_gm_dots(int64_t n, int64_t nb, const double* __restrict__ w,
-                                                    const double* const* __restrict__ Q, int32_t i0, int32_t cnt,
+template<typename T>
+__global__ __launch_bounds__(kBlock) void k_gm_dots(int64_t n, int64_t nb, const T* __restrict__ w,
+                                                    const T* const* __restrict__ Q, int32_t i0, int32_t cnt,
                                                     const GMState* st, double* partials) {
     if (st->stop) return;
     const int64_t lb = xcd_block(blockIdx.x, nb);
     double v[kCgsMax];
 #pragma unroll
     for (int c = 0; c < kCgsMax; ++c) v[c] = 0.0;
-#pragma unroll
+    // FP64: tiles unrolled; FP32: one tile at a time (unrolled, the compiler
+    // hoisted every tile's loads and spilled 1.8 KB per thread)
+    constexpr int kTileUnroll = sizeof(T) == 8 ? kCgsTiles : 1;
+#pragma unroll kTileUnroll
     for (int t = 0; t < kCgsTiles; ++t) {
         const int64_t k = (lb * kCgsTiles + t) * kBlock + threadIdx.x;
         if (k < n) {
-            const double wk = w[k];
-            double q[kCgsMax];
+            const double wk = (double)w[k];
+            T q[kCgsMax];
             cgs_load(q, Q, i0, cnt, k);
 #pragma unroll
             for (int c = 0; c < kCgsMax; ++c)
-                if (c < cnt) v[c] += q[c] * wk;
+                if (c < cnt) v[c] += (double)q[c] * wk;
         }
     }
     block_partials_many(v, cnt, lb, nb, partials + (int64_t)i0 * nb);
@@ -187,17 +198,18 @@ __global__ __launch_bounds__(kBlock) void k_gm_dots(int64_t n, int64_t nb, const
 
 // w -= sum_i h_i q_i (i = 0..cnt-1 in order), then, with dots, the block sums
 // of q_i . w for the new w, or, without, of w . w
-__device__ __forceinline__ double cgs_subtract(int64_t k, const double* w, const double* const* Q, const double* h,
-                                               int32_t cnt) {
-    double wk = w[k];
+template<typename T>
+__device__ __forceinline__ T cgs_subtract(int64_t k, const T* w, const T* const* Q, const double* h, int32_t cnt) {
+    T wk = w[k];
 #pragma unroll 8
-    for (int i = 0; i < cnt; ++i) wk = wk + -h[i] * Q[i][k];
+    for (int i = 0; i < cnt; ++i) wk = cgs_sub(wk, h[i], Q[i][k]);
     return wk;
 }
 
 // (the dots of q_0 .. q_{cd-1}, cd <= kCgsMax; k_gm_dots forms the rest)
-__global__ __launch_bounds__(kBlock) void k_gm_upd_dots(int64_t n, int64_t nb, double* __restrict__ w,
-                                                        const double* const* __restrict__ Q, const double* h,
+template<typename T>
+__global__ __launch_bounds__(kBlock) void k_gm_upd_dots(int64_t n, int64_t nb, T* __restrict__ w,
+                                                        const T* const* __restrict__ Q, const double* h,
                                                         int32_t cnt, int32_t cd, const GMState* st,
                                                         double* partials) {
     if (st->stop) return;
@@ -209,8 +221,8 @@ __global__ __launch_bounds__(kBlock) void k_gm_upd_dots(int64_t n, int64_t nb, d
     for (int t = 0; t < kCgsTiles; ++t) {
         const int64_t k = (lb * kCgsTiles + t) * kBlock + threadIdx.x;
         if (k < n) {
-            double q[kCgsMax];
-            double wk = w[k];
+            T q[kCgsMax];
+            T wk = w[k];
             // w -= sum_i h_i q_i in order, kCgsMax basis entries at a time; the
             // dots then use the first chunk (loaded again if a later one replaced it)
             for (int32_t i0 = 0; i0 < cnt; i0 += kCgsMax) {
@@ -218,29 +230,30 @@ __global__ __launch_bounds__(kBlock) void k_gm_upd_dots(int64_t n, int64_t nb, d
                 cgs_load(q, Q, i0, cc, k);
 #pragma unroll
                 for (int c = 0; c < kCgsMax; ++c)
-                    if (c < cc) wk = wk + -h[i0 + c] * q[c];
+                    if (c < cc) wk = cgs_sub(wk, h[i0 + c], q[c]);
             }
             w[k] = wk;
             if (cnt > kCgsMax) cgs_load(q, Q, 0, cd, k);
 #pragma unroll
             for (int c = 0; c < kCgsMax; ++c)
-                if (c < cd) v[c] += q[c] * wk;
+                if (c < cd) v[c] += (double)q[c] * (double)wk;
         }
     }
     block_partials_many(v, cd, lb, nb, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_gm_upd_norm(int64_t n, int64_t nb, double* __restrict__ w,
-                                                        const double* const* __restrict__ Q, const double* h,
+template<typename T>
+__global__ __launch_bounds__(kBlock) void k_gm_upd_norm(int64_t n, int64_t nb, T* __restrict__ w,
+                                                        const T* const* __restrict__ Q, const double* h,
                                                         int32_t cnt, const GMState* st, double* partials) {
     if (st->stop) return;
     const int64_t lb = xcd_block(blockIdx.x, nb);
     const int64_t k = lb * kBlock + threadIdx.x;
     double v[1] = {0.0};
     if (k < n) {
-        const double wk = cgs_subtract(k, w, Q, h, cnt);
+        const T wk = cgs_subtract(k, w, Q, h, cnt);
         w[k] = wk;
-        v[0] = wk * wk;
+        v[0] = (double)wk * (double)wk;
     }
     block_partials<1>(v, lb, nb, partials);
 }
@@ -432,18 +445,18 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
                         const int32_t cnt = j + 1;
                         double* hcol = H + (int64_t)j * (m + 1);
                         for (int32_t i0 = 0; i0 < cnt; i0 += kCgsMax)
-                            hipLaunchKernelGGL(k_gm_dots, dim3((unsigned)nbc), dim3(kBlock), 0, st, n, nbc, w,
+                            hipLaunchKernelGGL(k_gm_dots<double>, dim3((unsigned)nbc), dim3(kBlock), 0, st, n, nbc, w,
                                                (const double* const*)dQ, i0, std::min(kCgsMax, cnt - i0), dst, part);
                         hipLaunchKernelGGL(k_gm_fold_many, dim3((unsigned)cnt), dim3(kFold), 0, st, nbc, part, hcol,
                                            dst);
-                        hipLaunchKernelGGL(k_gm_upd_dots, dim3((unsigned)nbc), dim3(kBlock), 0, st, n, nbc, w,
+                        hipLaunchKernelGGL(k_gm_upd_dots<double>, dim3((unsigned)nbc), dim3(kBlock), 0, st, n, nbc, w,
                                            (const double* const*)dQ, hcol, cnt, std::min(kCgsMax, cnt), dst, part);
                         for (int32_t i0 = kCgsMax; i0 < cnt; i0 += kCgsMax)
-                            hipLaunchKernelGGL(k_gm_dots, dim3((unsigned)nbc), dim3(kBlock), 0, st, n, nbc, w,
+                            hipLaunchKernelGGL(k_gm_dots<double>, dim3((unsigned)nbc), dim3(kBlock), 0, st, n, nbc, w,
                                                (const double* const*)dQ, i0, std::min(kCgsMax, cnt - i0), dst, part);
                         hipLaunchKernelGGL(k_gm_fold_many, dim3((unsigned)cnt), dim3(kFold), 0, st, nbc, part, hc,
                                            dst);
-                        hipLaunchKernelGGL(k_gm_upd_norm, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
+                        hipLaunchKernelGGL(k_gm_upd_norm<double>, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
                                            (const double* const*)dQ, hc, cnt, dst, part);
                         hipLaunchKernelGGL(k_gm_norm_fold, dim3(1), dim3(kFold), 0, st, nb, part, j, m, hc, H, dst);
                     } else {
@@ -634,7 +647,10 @@ MixedGMResult gmres_mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, doub
     float** dQ = (float**)ctx->ws("gm32.Qptr", sizeof(float*) * (size_t)(m + 1));
     GMState* so = (GMState*)ctx->ws("gm32.outer", sizeof(GMState));
     GMState* si = (GMState*)ctx->ws("gm32.inner", sizeof(GMState));
-    double* part = ctx->partials(nb);
+    const bool cgs2 = ctx->gmres_orth == RSD_ORTHO_CGS2;
+    double* hc = cgs2 ? (double*)ctx->ws("gm.hc", sizeof(double) * (size_t)(m + 1)) : nullptr;
+    const int64_t nbc = cgs_blocks(n);
+    double* part = ctx->partials(cgs2 ? std::max<int64_t>((int64_t)m * nbc, nb) : nb);
     if (A->val32f.n != S.slots) A->val32f.alloc(S.slots);
     if (S.slots)
         hipLaunchKernelGGL(k_gm32_convert, dim3((unsigned)nblocks(S.slots)), dim3(kBlock), 0, st, S.slots, A->val.p,
@@ -680,12 +696,33 @@ MixedGMResult gmres_mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, doub
                 for (int j = 0; j < m; ++j) {
                     RSD_LAUNCH_COLS(S.mode(), k_gm32_apply, dim3((unsigned)nb), dim3(kBlock), st, n, nb, S.cols(),
                                     A->val32f.p, hq[j], invd, w, si);
-                    for (int i = 0; i <= j + 1; ++i) {
-                        const bool norm = i == j + 1;
-                        hipLaunchKernelGGL(k_gm32_mgs, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
-                                           i > 0 ? hq[i - 1] : nullptr, norm ? nullptr : hq[i], i, j, m, H, si, part);
-                        hipLaunchKernelGGL(k_gm_mgs_fold, dim3(1), dim3(kFold), 0, st, nb, part, i, j, m,
-                                           (int32_t)norm, H, si);
+                    if (cgs2) {  // the FP64 solver's three passes on the FP32 basis
+                        const int32_t cnt = j + 1;
+                        const float* const* Qf = (const float* const*)dQ;
+                        double* hcol = H + (int64_t)j * (m + 1);
+                        for (int32_t i0 = 0; i0 < cnt; i0 += kCgsMax)
+                            hipLaunchKernelGGL(k_gm_dots<float>, dim3((unsigned)nbc), dim3(kBlock), 0, st, n, nbc, w, Qf,
+                                               i0, std::min(kCgsMax, cnt - i0), si, part);
+                        hipLaunchKernelGGL(k_gm_fold_many, dim3((unsigned)cnt), dim3(kFold), 0, st, nbc, part, hcol,
+                                           si);
+                        hipLaunchKernelGGL(k_gm_upd_dots<float>, dim3((unsigned)nbc), dim3(kBlock), 0, st, n, nbc, w,
+                                           Qf, hcol, cnt, std::min(kCgsMax, cnt), si, part);
+                        for (int32_t i0 = kCgsMax; i0 < cnt; i0 += kCgsMax)
+                            hipLaunchKernelGGL(k_gm_dots<float>, dim3((unsigned)nbc), dim3(kBlock), 0, st, n, nbc, w, Qf,
+                                               i0, std::min(kCgsMax, cnt - i0), si, part);
+                        hipLaunchKernelGGL(k_gm_fold_many, dim3((unsigned)cnt), dim3(kFold), 0, st, nbc, part, hc, si);
+                        hipLaunchKernelGGL(k_gm_upd_norm<float>, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w, Qf,
+                                           hc, cnt, si, part);
+                        hipLaunchKernelGGL(k_gm_norm_fold, dim3(1), dim3(kFold), 0, st, nb, part, j, m, hc, H, si);
+                    } else {
+                        for (int i = 0; i <= j + 1; ++i) {
+                            const bool norm = i == j + 1;
+                            hipLaunchKernelGGL(k_gm32_mgs, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
+                                               i > 0 ? hq[i - 1] : nullptr, norm ? nullptr : hq[i], i, j, m, H, si,
+                                               part);
+                            hipLaunchKernelGGL(k_gm_mgs_fold, dim3(1), dim3(kFold), 0, st, nb, part, i, j, m,
+                                               (int32_t)norm, H, si);
+                        }
                     }
                     hipLaunchKernelGGL(k_gm32_newq, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, w, hq[j + 1], si);
                     hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, st, j, m, H, cs, sn, g, si, in_tol, 0, cap);

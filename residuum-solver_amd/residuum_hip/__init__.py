@@ -114,7 +114,8 @@ class Ctx:
     def set_gmres_orthogonalization(self, mode):
         """ORTHO_MGS (default, the reference's modified Gram-Schmidt, bitwise)
         or ORTHO_CGS2 (classical Gram-Schmidt, reorthogonalised once: three
-        folds per Arnoldi step) for this context's later FP64 GMRES solves."""
+        folds per Arnoldi step) for this context's later GMRES solves (FP64,
+        and the mixed solver's FP32 inner GMRES)."""
         check(lib().rsd_ctx_set_gmres_orthogonalization(self.h, C.c_int32(mode)))
 
     def close(self):

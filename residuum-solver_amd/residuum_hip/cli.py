@@ -42,8 +42,8 @@ csr operator.
 A gmres node may carry `orthogonalization: cgs2` (a device extension; the
 default `mgs` is the reference's modified Gram-Schmidt, bitwise): classical
 Gram-Schmidt reorthogonalised once, three folds per Arnoldi step instead of
-one per basis vector, for the FP64 GMRES solves (steady and Newton's inner
-solves).
+one per basis vector, for the GMRES solves (steady and Newton's inner
+solves, FP64 or the mixed solver's FP32 inner GMRES).
 
 `discretization: arithmetic:` (a device extension; default `reference`, the
 reference's operation order, bitwise) selects the element arithmetic of the

@@ -212,12 +212,12 @@ MIXED_CG_STALLS = ("mixed CG on the full FP32 copy of this anisotropic, non-symm
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,lin_solver,lin_tol,ortho", [
-    (512, 1, 1e-3, 0), (512, 1, 1e-3, 1), (512, 2, 1e-2, 0), (128, 1, 1e-12, 0), (128, 1, 1e-12, 1),
-    (128, 2, 1e-10, 0),
+    (512, 1, 1e-3, 0), (512, 1, 1e-3, 1), (512, 2, 1e-2, 0), (512, 2, 1e-2, 1), (128, 1, 1e-12, 0),
+    (128, 1, 1e-12, 1), (128, 2, 1e-10, 0), (128, 2, 1e-10, 1),
     pytest.param(512, 3, 1e-10, 0, marks=pytest.mark.skip(reason=MIXED_CG_STALLS)),
     pytest.param(128, 3, 1e-10, 0, marks=pytest.mark.skip(reason=MIXED_CG_STALLS))],
-    ids=["512-gmres30", "512-gmres30-cgs2", "512-gmres30-mixed", "128-gmres30", "128-gmres30-cgs2",
-         "128-gmres30-mixed", "512-cg-mixed", "128-cg-mixed"])
+    ids=["512-gmres30", "512-gmres30-cgs2", "512-gmres30-mixed", "512-gmres30-mixed-cgs2", "128-gmres30",
+         "128-gmres30-cgs2", "128-gmres30-mixed", "128-gmres30-mixed-cgs2", "512-cg-mixed", "128-cg-mixed"])
 def test_newton_q2(ctx, n, lin_solver, lin_tol, ortho):
     """Q2 quads on the unit square (3x3 Gauss), k(T) = (1 + 0.5 T)
     diag(1, 0.1), f = 10 sin(pi x) sin(pi y), T = 0 on every side; Newton
@@ -234,8 +234,9 @@ def test_newton_q2(ctx, n, lin_solver, lin_tol, ortho):
     half the outer tolerance, one refinement cycle per Newton step).  At
     128^2 FP64 GMRES(30) runs to 1e-12 and the mixed solve to 1e-10: the
     refinement's attainable relative residual on these Jacobians is about
-    1e-12.  The cgs2 cases run the FP64 GMRES with classical Gram-Schmidt
-    reorthogonalised once (RSD_ORTHO_CGS2) under the same checks."""
+    1e-12.  The cgs2 cases run the GMRES (FP64, or the mixed solver's FP32
+    inner one) with classical Gram-Schmidt reorthogonalised once
+    (RSD_ORTHO_CGS2) under the same checks."""
     import residuum_hip as rh
     t0 = time.perf_counter()
     pb = rh.Problem(ctx, 2, (n, n), (2, 2), 3, [0, 1, 2, 3], ["0"] * 4)

@@ -967,11 +967,14 @@ def test_gmres_cgs2(ctx, case, precond):
 
 @pytest.mark.parametrize("case", CASES[:3])
 @pytest.mark.parametrize("precond", [0, 1])
-def test_gmres_mixed(ctx, case, precond):
+@pytest.mark.parametrize("ortho", [0, 1], ids=["mgs", "cgs2"])
+def test_gmres_mixed(ctx, case, precond, ortho):
     """Mixed-precision GMRES on the non-symmetric tangent Jacobian: its
     solution's FP64 relative residual, recomputed here from an FP64 product,
     is below the tolerance, which one FP32 inner solve cannot reach (so
-    several refinements ran), and it agrees with the FP64 GMRES solution."""
+    several refinements ran), and it agrees with the FP64 GMRES solution;
+    with the inner GMRES orthogonalised by modified or by classical
+    Gram-Schmidt reorthogonalised once (RSD_ORTHO_CGS2)."""
     import residuum_hip as rh
     dim, n, p, nq, tags, exprs, ext = case
     tensor = (1.0, 0.2, 0.2, 0.1) if dim == 2 else (1, 0.2, 0, 0.2, 0.1, 0, 0, 0, 1)
@@ -985,9 +988,13 @@ def test_gmres_mixed(ctx, case, precond):
     db = ctx.from_numpy(b)
     ref = pb.vector(0.0)
     J.gmres(db, ref, tol=1e-13, max_iters=4000, krylov=30, precond=precond)
-    for krylov in (5, 30):
+    for krylov in (5, 30, 40):
         dx = pb.vector(0.0)
-        rep, cyc = J.gmres_mixed(db, dx, tol=1e-10, max_iters=4000, krylov=krylov, precond=precond)
+        ctx.set_gmres_orthogonalization(ortho)
+        try:
+            rep, cyc = J.gmres_mixed(db, dx, tol=1e-10, max_iters=4000, krylov=krylov, precond=precond)
+        finally:
+            ctx.set_gmres_orthogonalization(rh.ORTHO_MGS)
         y = pb.vector()
         J.spmv(dx, y)
         rel = np.linalg.norm(b - y.numpy()) / np.linalg.norm(b)
