@@ -6,9 +6,10 @@
 // and accumulates q_i . w for the next coefficient, which is exactly the
 // reference's dot-then-axpy sequence.  A context may choose classical
 // Gram-Schmidt with one reorthogonalisation instead (RSD_ORTHO_CGS2: three
-// passes and three folds per step; not bitwise the reference).  The Hessenberg column, the Givens
-// rotations and the back-substitution are scalar work done by one device
-// thread.  A whole cycle is launched at once: after the stop flag is set
+// passes and three folds per step; not bitwise the reference).  The Givens
+// rotations run in the block that folds the step's norm, from an LDS copy
+// of the column, and the back-substitution in one device thread: scalar
+// work in the reference's order.  A whole cycle is launched at once: after the stop flag is set
 // the remaining kernels of the cycle return immediately, and the host reads
 // the state once per cycle.
 #include "rsd_gmres.hpp"
@@ -64,14 +65,75 @@ __global__ __launch_bounds__(kBlock) void k_gm_mgs(int64_t n, int64_t nb, double
     block_partials<1>(v, lb, nb, partials);
 }
 
+constexpr int kGivensMax = 512;  // the largest Krylov dimension the entry points accept
+
+// Rotations of column j by the previous j rotations, the new rotation and
+// the residual estimate (Solver.tpp:78-122), run by the whole block of a
+// norm fold once hnext is known (in thread 0): the threads stage column j
+// (plus cadd when given: CGS2's second coefficients), cs and sn in LDS
+// together, then thread 0 runs the sequential recurrence on the staged
+// copy.  A one-thread kernel made one dependent global round trip per
+// rotation; here the loads are one parallel batch.  Same operations in
+// the same order.
+__device__ __forceinline__ void givens_column(int32_t j, int32_t m, double* H, const double* cadd, double* cs,
+                                              double* sn, double* g, GMState* st, double hnext, double tol,
+                                              int32_t abs_tol, int64_t max_iters) {
+    __shared__ double col[kGivensMax], c[kGivensMax], s[kGivensMax];
+    double* hj = H + (int64_t)j * (m + 1);
+    for (int i = threadIdx.x; i <= j; i += blockDim.x) col[i] = cadd ? hj[i] + cadd[i] : hj[i];
+    for (int i = threadIdx.x; i < j; i += blockDim.x) {
+        c[i] = cs[i];
+        s[i] = sn[i];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    st->total += 1;
+    const int32_t breakdown = hnext <= 1e-14;
+    for (int i = 0; i < j; ++i) {
+        const double h1 = col[i], h2 = col[i + 1];
+        col[i] = c[i] * h1 + s[i] * h2;
+        col[i + 1] = -s[i] * h1 + c[i] * h2;
+    }
+    const double hjj = col[j];
+    const double den = sqrt(hjj * hjj + hnext * hnext);
+    double cj = 1.0, sj = 0.0;
+    if (!(den <= 1e-14)) {
+        cj = hjj / den;
+        sj = hnext / den;
+    }
+    cs[j] = cj;
+    sn[j] = sj;
+    col[j] = cj * hjj + sj * hnext;
+    for (int i = 0; i <= j; ++i) hj[i] = col[i];
+    const double gj = g[j];
+    g[j] = cj * gj;
+    const double gn = -sj * gj;
+    g[j + 1] = gn;
+    const double est = fabs(gn);
+    st->est = est;
+    st->mloc = j + 1;
+    st->breakdown = breakdown;
+    if ((abs_tol ? est : est / (st->res0 + 1e-50)) < tol || breakdown || st->total >= max_iters) st->stop = 1;
+}
+
 __global__ __launch_bounds__(kFold) void k_gm_mgs_fold(int64_t nb, const double* __restrict__ partials, int32_t i,
-                                                        int32_t j, int32_t m, int32_t norm, double* H, GMState* st) {
+                                                        int32_t j, int32_t m, double* H, GMState* st) {
     if (st->stop) return;
     double out[1];
     fold_partials<1>(nb, partials, out);
-    if (threadIdx.x != 0) return;
-    if (norm) st->hnext = sqrt(out[0]);
-    else H[(int64_t)j * (m + 1) + i] = out[0];
+    if (threadIdx.x == 0) H[(int64_t)j * (m + 1) + i] = out[0];
+}
+
+// the norm step's fold: h_{j+1,j} = ||w||, then column j's rotations
+__global__ __launch_bounds__(kFold) void k_gm_mgs_norm(int64_t nb, const double* __restrict__ partials, int32_t j,
+                                                        int32_t m, double* H, double* cs, double* sn, double* g,
+                                                        GMState* st, double tol, int32_t abs_tol, int64_t max_iters) {
+    if (st->stop) return;
+    double out[1];
+    fold_partials<1>(nb, partials, out);
+    const double hnext = sqrt(out[0]);
+    if (threadIdx.x == 0) st->hnext = hnext;
+    givens_column(j, m, H, nullptr, cs, sn, g, st, hnext, tol, abs_tol, max_iters);
 }
 
 // ------------------------------------------------------------------------
@@ -267,15 +329,17 @@ __global__ __launch_bounds__(kFold) void k_gm_fold_many(int64_t nb, const double
     if (threadIdx.x == 0) out[blockIdx.x] = o[0];
 }
 
-// h_{j+1,j} = ||w|| and H[:, j] = h + c (one block)
+// h_{j+1,j} = ||w|| and H[:, j] = h + c, then column j's rotations (one block)
 __global__ __launch_bounds__(kFold) void k_gm_norm_fold(int64_t nb, const double* __restrict__ partials, int32_t j,
-                                                        int32_t m, const double* __restrict__ c, double* H,
-                                                        GMState* st) {
+                                                        int32_t m, const double* __restrict__ c, double* H, double* cs,
+                                                        double* sn, double* g, GMState* st, double tol,
+                                                        int32_t abs_tol, int64_t max_iters) {
     if (st->stop) return;
     double o[1];
     fold_partials<1>(nb, partials, o);
-    if (threadIdx.x == 0) st->hnext = sqrt(o[0]);
-    if (threadIdx.x <= j) H[(int64_t)j * (m + 1) + threadIdx.x] += c[threadIdx.x];
+    const double hnext = sqrt(o[0]);
+    if (threadIdx.x == 0) st->hnext = hnext;
+    givens_column(j, m, H, c, cs, sn, g, st, hnext, tol, abs_tol, max_iters);
 }
 
 // q_{j+1} = w * (1 / hnext) unless the column broke down
@@ -286,34 +350,6 @@ __global__ void k_gm_newq(int64_t n, const double* __restrict__ w, double* __res
         const double s = 1.0 / st->hnext;
         q[k] = w[k] * s;
     }
-}
-
-// Givens rotations for column j (Solver.tpp:78-122)
-__global__ void k_gm_givens(int32_t j, int32_t m, double* H, double* cs, double* sn, double* g, GMState* st,
-                            double tol, int32_t abs_tol, int64_t max_iters) {
-    if (st->stop) return;
-    auto hat = [&](int i, int jj) -> double& { return H[(int64_t)jj * (m + 1) + i]; };
-    const double hnext = st->hnext;
-    st->total += 1;
-    const int32_t breakdown = hnext <= 1e-14;
-    for (int i = 0; i < j; ++i) {
-        const double h1 = hat(i, j), h2 = hat(i + 1, j);
-        hat(i, j) = cs[i] * h1 + sn[i] * h2;
-        hat(i + 1, j) = -sn[i] * h1 + cs[i] * h2;
-    }
-    const double hjj = hat(j, j);
-    const double den = sqrt(hjj * hjj + hnext * hnext);
-    if (den <= 1e-14) { cs[j] = 1.0; sn[j] = 0.0; }
-    else { cs[j] = hjj / den; sn[j] = hnext / den; }
-    hat(j, j) = cs[j] * hjj + sn[j] * hnext;
-    const double gj = g[j];
-    g[j] = cs[j] * gj;
-    g[j + 1] = -sn[j] * gj;
-    const double est = fabs(g[j + 1]);
-    st->est = est;
-    st->mloc = j + 1;
-    st->breakdown = breakdown;
-    if ((abs_tol ? est : est / (st->res0 + 1e-50)) < tol || breakdown || st->total >= max_iters) st->stop = 1;
 }
 
 // back-substitution H y = g (Solver.tpp:126-133)
@@ -458,20 +494,24 @@ GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double
                                            dst);
                         hipLaunchKernelGGL(k_gm_upd_norm<double>, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
                                            (const double* const*)dQ, hc, cnt, dst, part);
-                        hipLaunchKernelGGL(k_gm_norm_fold, dim3(1), dim3(kFold), 0, st, nb, part, j, m, hc, H, dst);
+                        hipLaunchKernelGGL(k_gm_norm_fold, dim3(1), dim3(kFold), 0, st, nb, part, j, m, hc, H, cs, sn,
+                                           g, dst, tol, (int32_t)abs_tol, max_iters);
                     } else {
                         for (int i = 0; i <= j + 1; ++i) {
                             const bool norm = i == j + 1;
                             hipLaunchKernelGGL(k_gm_mgs, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
                                                i > 0 ? hq[i - 1] : nullptr, norm ? nullptr : hq[i], i, j, m, H, dst,
                                                part);
-                            hipLaunchKernelGGL(k_gm_mgs_fold, dim3(1), dim3(kFold), 0, st, nb, part, i, j, m,
-                                               (int32_t)norm, H, dst);
+                            if (norm)
+                                hipLaunchKernelGGL(k_gm_mgs_norm, dim3(1), dim3(kFold), 0, st, nb, part, j, m, H, cs,
+                                                   sn, g, dst, tol, (int32_t)abs_tol, max_iters);
+                            else
+                                hipLaunchKernelGGL(k_gm_mgs_fold, dim3(1), dim3(kFold), 0, st, nb, part, i, j, m, H,
+                                                   dst);
                         }
                     }
+                    // the norm fold ran the rotations: q_{j+1} only when the cycle goes on
                     hipLaunchKernelGGL(k_gm_newq, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, w, hq[j + 1], dst);
-                    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, st, j, m, H, cs, sn, g, dst, tol,
-                                       (int32_t)abs_tol, max_iters);
                     RSD_HIP(hipGetLastError());
                 }
                 hipLaunchKernelGGL(k_gm_backsub, dim3(1), dim3(1), 0, st, m, H, g, y, dst);
@@ -713,19 +753,23 @@ MixedGMResult gmres_mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, doub
                         hipLaunchKernelGGL(k_gm_fold_many, dim3((unsigned)cnt), dim3(kFold), 0, st, nbc, part, hc, si);
                         hipLaunchKernelGGL(k_gm_upd_norm<float>, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w, Qf,
                                            hc, cnt, si, part);
-                        hipLaunchKernelGGL(k_gm_norm_fold, dim3(1), dim3(kFold), 0, st, nb, part, j, m, hc, H, si);
+                        hipLaunchKernelGGL(k_gm_norm_fold, dim3(1), dim3(kFold), 0, st, nb, part, j, m, hc, H, cs, sn,
+                                           g, si, in_tol, 0, cap);
                     } else {
                         for (int i = 0; i <= j + 1; ++i) {
                             const bool norm = i == j + 1;
                             hipLaunchKernelGGL(k_gm32_mgs, dim3((unsigned)nb), dim3(kBlock), 0, st, n, nb, w,
                                                i > 0 ? hq[i - 1] : nullptr, norm ? nullptr : hq[i], i, j, m, H, si,
                                                part);
-                            hipLaunchKernelGGL(k_gm_mgs_fold, dim3(1), dim3(kFold), 0, st, nb, part, i, j, m,
-                                               (int32_t)norm, H, si);
+                            if (norm)
+                                hipLaunchKernelGGL(k_gm_mgs_norm, dim3(1), dim3(kFold), 0, st, nb, part, j, m, H, cs,
+                                                   sn, g, si, in_tol, 0, cap);
+                            else
+                                hipLaunchKernelGGL(k_gm_mgs_fold, dim3(1), dim3(kFold), 0, st, nb, part, i, j, m, H,
+                                                   si);
                         }
                     }
                     hipLaunchKernelGGL(k_gm32_newq, dim3((unsigned)nbe), dim3(kBlock), 0, st, n, w, hq[j + 1], si);
-                    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, st, j, m, H, cs, sn, g, si, in_tol, 0, cap);
                     RSD_HIP(hipGetLastError());
                 }
                 hipLaunchKernelGGL(k_gm_backsub, dim3(1), dim3(1), 0, st, m, H, g, y, si);
