@@ -393,6 +393,77 @@ def run_newton(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
     print(json.dumps(line), flush=True)
 
 
+def run_adjoint(args, rh, ctx, stream, new_ev, rec_ev, ev_ms):
+    """The adjoint-gradient workload (not the default line): 96^3 Q1 hex on
+    the unit cube, kappa_e = exp(theta_e) with theta a Gaussian random field
+    (std 0.5, correlation length 0.1), rho c = 1, Backward Euler dt = 5e-3
+    for 20 steps from T = 0 with T = 1 on z = 0, loss 1/2 ||T_N[obs] -
+    T_obs||^2 over 4096 seeded free nodes, T_obs from the forward run with a
+    perturbed field; solves to 1e-13 relative (the gradient checks need
+    them converged).  Times the forward loss and the forward plus adjoint
+    sweep with HIP events, and checks the gradient against central
+    differences (h = 1e-6) along a few of the seeded directions the
+    full-size test uses."""
+    from residuum_hip.fields import gaussian_random_field, sample_without_replacement
+    n = args.adjoint_n
+    dt, steps, tol = 5e-3, 20, 1e-13
+    pb = rh.Problem(ctx, 3, (n, n, n), (1, 1, 1), 2, [4], ["1"])
+    pb.set_mass(1.0, 1.0)
+    theta = gaussian_random_field(20261203, (n, n, n), 0.1, 0.5)
+    xi = gaussian_random_field(20261205, (n, n, n), 0.1, 0.5)
+    obs = sample_without_replacement(20261204, pb.n_free, 4096)
+    U0 = pb.vector(0.0)
+    dUN = ctx.empty(pb.n_free)
+    dk = ctx.from_numpy(np.exp(theta + 0.1 * xi))
+    pb.adjoint(dk, U0, 0.0, dt, steps, np.zeros(0, np.int64), np.zeros(0), tol=tol, want_grad=False, d_UN=dUN)
+    t_obs = dUN.numpy()[obs]
+    dg = ctx.empty(pb.n_elem)
+
+    def timed(th, grad):
+        dk.upload(np.exp(th))
+        ctx.sync()
+        e0, e1 = new_ev(), new_ev()
+        rec_ev(e0, stream)
+        L = pb.adjoint(dk, U0, 0.0, dt, steps, obs, t_obs, tol=tol, want_grad=grad, d_grad=dg if grad else None)
+        rec_ev(e1, stream)
+        return L, ev_ms(e0, e1)
+
+    timed(theta, True)  # warm-up
+    reps = max(1, args.adjoint_reps)
+    grad_ms, fwd_ms = [], []
+    for _ in range(reps):
+        L, ms = timed(theta, True)
+        grad_ms.append(ms)
+    g = dg.numpy()
+    for _ in range(reps):
+        L2, ms = timed(theta, False)
+        fwd_ms.append(ms)
+    print(f"# adjoint {n}^3: loss {L:.6e}; forward {np.median(fwd_ms):.1f} ms, forward + adjoint sweep "
+          f"{np.median(grad_ms):.1f} ms", file=sys.stderr, flush=True)
+    h = 1e-6
+    checks = []
+    for j in range(args.adjoint_directions):
+        d = rh.philox_normal(20261206, j, pb.n_elem)
+        fd = (timed(theta + h * d, False)[0] - timed(theta - h * d, False)[0]) / (2 * h)
+        gd = float(g @ d)
+        checks.append(dict(direction=j, adjoint=gd, central_fd=fd, rel_diff=abs(fd - gd) / abs(gd)))
+        print(f"# direction {j}: adjoint {gd:+.10e}  central FD {fd:+.10e}  rel diff {checks[-1]['rel_diff']:.2e}",
+              file=sys.stderr, flush=True)
+    gms = float(np.median(grad_ms))
+    line = {
+        "metric": "adjoint gradient wall time (forward with checkpoints + adjoint sweep)", "value": gms,
+        "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "f64",
+        "data": "synthetic (seeded Gaussian random log-conductivity, seeded observations)",
+        "config": {"workload": f"adjoint gradient, {n}^3 Q1 hex, Backward Euler dt=5e-3 x {steps}, "
+                               "4096 observations, solves to 1e-13", "dofs": pb.n_free, "elements": pb.n_elem},
+        "loss": L, "forward_loss_repeats_bitwise": bool(L2 == L), "forward_ms": float(np.median(fwd_ms)),
+        "gradient_ms": gms, "gradient_over_forward": gms / float(np.median(fwd_ms)),
+        "element_steps_per_s": pb.n_elem * steps * 2 / (gms * 1e-3),
+        "fd_check": checks,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def renumbered_workload(rh, ctx, stream, n, kappa_elem, dt, capture, new_ev, rec_ev, ev_ms, reps, bytes_of):
     """The benchmark's operator on the same 128^3 mesh with its nodes
     renumbered in 8^3-node tiles (tile-major, lexicographic inside a tile):
@@ -616,10 +687,15 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=320, help="oracle CG iterations timed for the CPU baseline (about 10 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-matrix-free", action="store_true", help="skip the matrix-free operator solve")
-    ap.add_argument("--workload", choices=["be128", "batched", "newton"], default="be128",
+    ap.add_argument("--workload", choices=["be128", "batched", "newton", "adjoint"], default="be128",
                     help="be128: the Backward Euler line (default); batched: the batched-solve workload; "
-                         "newton: the nonlinear Q2 workload with both GMRES orthogonalisations")
+                         "newton: the nonlinear Q2 workload with both GMRES orthogonalisations; "
+                         "adjoint: the adjoint-gradient workload")
     ap.add_argument("--newton-n", type=int, default=512, help="elements per edge of the newton workload")
+    ap.add_argument("--adjoint-n", type=int, default=96, help="elements per edge of the adjoint workload")
+    ap.add_argument("--adjoint-reps", type=int, default=3, help="timed gradients (and forward runs)")
+    ap.add_argument("--adjoint-directions", type=int, default=4,
+                    help="central-difference directions checked against the gradient")
     ap.add_argument("--batch-problems", type=int, default=256)
     ap.add_argument("--tune-cache", default=None, help="JSON schedule cache for the batched workload")
     ap.add_argument("--no-renumbered", action="store_true",
@@ -657,6 +733,9 @@ def main():
         return
     if args.workload == "newton":
         run_newton(args, rh, ctx, stream, new_ev, rec_ev, ev_ms)
+        return
+    if args.workload == "adjoint":
+        run_adjoint(args, rh, ctx, stream, new_ev, rec_ev, ev_ms)
         return
     n = args.n
     dt = 1e-3
