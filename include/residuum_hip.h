@@ -350,6 +350,17 @@ int rsd_spmv_f64(rsd_ctx* ctx, rsd_matrix* A, const double* x, double* y);
 int rsd_problem_layout_stencil(rsd_problem* pb, int64_t* out);
 int rsd_matrix_to_stencil(rsd_ctx* ctx, rsd_matrix* A);
 int rsd_spmv_f64_stencil(rsd_ctx* ctx, rsd_matrix* A, const double* x, double* y);
+/* The slice-union copy the GMRES products read (no reference counterpart;
+ * the product is operations::matvec, MatrixOps.tpp:5-23, bitwise in FP64 and
+ * the mixed solver's FP32 row sums in FP32).  When the rows share offset
+ * patterns and every 64-row slice's union of its rows' column offsets has at
+ * most 32 positions (Q1 and Q2 quads, Q1 hexes), a copy holds each slice's
+ * union positions, zeros where a row lacks one, in 16-byte pairs, so the
+ * rows of a slice share their offsets even where their own patterns differ.
+ * rsd_gmres_solve_f64 and rsd_gmres_solve_mixed refresh and use it by
+ * themselves.  out[2] = 1 when the problem has the layout (else 0), slots
+ * of the copy. */
+int rsd_problem_layout_union(rsd_problem* pb, int64_t* out);
 /* Jacobi::update (linalg/solver/preconditioner/Jacobi.hpp:23-36) via
  * CSROperator::diagonal (CSROperator.hpp:40-45): 1/d, or 1 if |d| <= 1e-300 */
 int rsd_jacobi_inv_diag(rsd_ctx* ctx, rsd_matrix* A, double* inv_diag);

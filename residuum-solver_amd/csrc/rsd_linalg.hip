@@ -697,6 +697,15 @@ int rsd_problem_layout_stencil(rsd_problem* pb, int64_t* out) {
     });
 }
 
+int rsd_problem_layout_union(rsd_problem* pb, int64_t* out) {
+    return guard([&] {
+        RSD_CHECK(pb && out, "null argument");
+        const Sell& S = pb->sell;
+        out[0] = S.has_su ? 1 : 0;
+        out[1] = S.has_su ? S.su_slots : 0;
+    });
+}
+
 int rsd_matrix_to_stencil(rsd_ctx* ctx, rsd_matrix* A) {
     return guard([&] {
         RSD_CHECK(ctx && A, "null argument");

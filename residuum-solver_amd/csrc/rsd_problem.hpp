@@ -81,6 +81,25 @@ struct Sell {
         return c;
     }
 
+    // slice-union layout of the copies the GMRES products read (COL_SU):
+    // when every slice's union of column offsets has at most kSuMax
+    // positions and the copy has at most 5/4 the sliced layout's slots
+    bool has_su = false;
+    int64_t su_slots = 0;
+    DBuf<int64_t> su_ptr;   // [n_slices + 1]
+    DBuf<int32_t> su_off;   // [n_slices * kSuMax]
+    DBuf<int32_t> su_nu;    // [n_slices]
+    DBuf<uint32_t> su_mask; // [n_rows]
+    SuCols su_cols() const {
+        SuCols c;
+        c.ptr = su_ptr.p;
+        c.off = su_off.p;
+        c.nu = su_nu.p;
+        c.mask = su_mask.p;
+        c.n_slices = n_slices;
+        return c;
+    }
+
     Cols cols() const {
         Cols c;
         c.slice_ptr = slice_ptr.p;
@@ -236,6 +255,8 @@ struct rsd_matrix {
     rsd::DBuf<float> val32;   // [sell.slots32()] FP32 copy for the mixed-precision path
     rsd::DBuf<float> val32f;  // [sell.slots] full FP32 copy in the FP64 layout (mixed GMRES, any matrix)
     mutable rsd::DBuf<double> val_st;  // [sell.st_slots()] FP64 copy in the stencil layout, a cache the FP64 CG refreshes
+    mutable rsd::DBuf<double> val_su;  // [sell.su_slots] FP64 slice-union copy, a cache the FP64 GMRES refreshes
+    mutable rsd::DBuf<float> val_su32; // [sell.su_slots] FP32 slice-union copy, a cache the mixed GMRES refreshes
     rsd::DBuf<int32_t> asym;  // [1] set by the FP32 conversion when the half layout's symmetry check fails
 };
 

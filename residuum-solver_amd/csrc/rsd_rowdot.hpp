@@ -60,6 +60,28 @@ struct StCols {
     int32_t coff[kStMaxPos / 3] = {};
 };
 
+// Slice-union layout of a copy the GMRES products read (COL_SU): each
+// 64-row slice holds the nu positions of the union of its rows' column
+// offsets (ascending, at most kSuMax), zeros where a row lacks one, in
+// 16-byte pairs of positions (su_slot); mask[r] has bit k set when row r
+// holds position k.  Rows of one slice then share their column offsets
+// (uniform loads, coalesced x gathers) even where the rows' own patterns
+// alternate, as on Q2 meshes (rows of 9, 15 and 25 entries side by side).
+constexpr int kSuMax = 32;
+
+struct SuCols {
+    const int64_t* ptr = nullptr;    // [slices + 1] first slot of each slice
+    const int32_t* off = nullptr;    // [slices * kSuMax] column - row of each position
+    const int32_t* nu = nullptr;     // [slices] positions
+    const uint32_t* mask = nullptr;  // [rows]
+    int64_t n_slices = 0;
+};
+
+// slot of position k of lane l in a slice of nu positions starting at base
+__host__ __device__ inline int64_t su_slot(int64_t base, int k, int nu, int l) {
+    return (k == nu - 1 && (nu & 1)) ? base + 128 * (nu >> 1) + l : base + 128 * (k >> 1) + 2 * l + (k & 1);
+}
+
 // slot of position b of row r (within the copy)
 __host__ __device__ inline int64_t st_slot(int64_t r, int b, int np) {
     const int64_t l = r & 63;
@@ -375,6 +397,60 @@ __device__ __forceinline__ float row_dot_f32(int64_t r, const Cols& C, const flo
         const RowCols<MODE> rc(r, C);
         return run([&](int64_t s, int j) { return rc.col(C, r, s, j); }, rc.len);
     }
+}
+
+// Row r's product over the slice-union copy, its entries added in CSR
+// order (ascending positions, the mask skipping the ones the row lacks):
+// FP64, each product and sum rounded separately (row_dot_f64's bits), or
+// FP32 fused multiply-adds (row_dot_f32's).  Every lane of the wavefront
+// must belong to the same slice, s < n_slices (the caller returns for
+// wavefronts past the last slice); lanes past the last row return 0.  U
+// positions (even) per batch of loads: values in pairs, then the x entries.
+template<typename T>
+__device__ __forceinline__ T su_madd(T sum, T v, T x) {
+    if constexpr (sizeof(T) == 8) return sum + v * x;
+    else return __builtin_fmaf(v, x, sum);
+}
+
+template<typename T, int U = 6>
+__device__ __forceinline__ T row_dot_su(int64_t r, int64_t n, const SuCols& S, const T* __restrict__ val,
+                                        const T* __restrict__ x) {
+    static_assert(U % 2 == 0, "positions come in pairs");
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    const int64_t s = __builtin_amdgcn_readfirstlane((int)(r >> 6));
+    const int lane = (int)(r & 63);
+    const bool live = r < n;
+    const int nu = S.nu[s];
+    const int32_t* off = S.off + s * kSuMax;
+    const T* base = val + S.ptr[s];
+    const uint32_t m = live ? S.mask[r] : 0u;
+    const int ri = (int)r, nm1 = (int)n - 1;
+    T sum = T(0);
+    const int nu2 = nu & ~1;  // paired positions; an odd last one is single
+    int k = 0;
+    for (; k + U <= nu2; k += U) {
+        T v[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u += 2) {
+            const T2 t = __builtin_nontemporal_load((const T2*)(base + 128 * ((k + u) >> 1)) + lane);
+            v[u] = t.x;
+            v[u + 1] = t.y;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = x[min(max(ri + off[k + u], 0), nm1)];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const T t = su_madd(sum, v[u], xv[u]);
+            sum = ((m >> (k + u)) & 1u) ? t : sum;
+        }
+    }
+    for (; k < nu; ++k) {
+        const T v = __builtin_nontemporal_load(base + (su_slot(0, k, nu, lane)));
+        const T xk = x[min(max(ri + off[k], 0), nm1)];
+        const T t = su_madd(sum, v, xk);
+        sum = ((m >> k) & 1u) ? t : sum;
+    }
+    return sum;
 }
 
 }  // namespace rsd

@@ -403,6 +403,38 @@ void rsd::build_sell(rsd_problem* pb, const std::vector<int32_t>& rowlen) {
                 S.st_mask.upload(rmask.data(), n, st);
                 RSD_HIP(hipStreamSynchronize(st));  // rmask is a local
             }
+            // slice-union layout: per slice the union of its rows' offsets
+            const int64_t ns = S.n_slices;
+            std::vector<int64_t> sptr(ns + 1, 0);
+            std::vector<int32_t> soff((size_t)ns * kSuMax, 0), snu(ns, 0);
+            std::vector<uint32_t> smask(n, 0u);
+            bool su_ok = true;
+            std::vector<int32_t> u;
+            for (int64_t s = 0; s < ns && su_ok; ++s) {
+                u.clear();
+                for (int64_t r = s * kSlice; r < std::min(n, (s + 1) * kSlice); ++r)
+                    u.insert(u.end(), pats[pid[r]].begin(), pats[pid[r]].end());
+                std::sort(u.begin(), u.end());
+                u.erase(std::unique(u.begin(), u.end()), u.end());
+                su_ok = u.size() <= (size_t)kSuMax;
+                if (!su_ok) break;
+                snu[s] = (int32_t)u.size();
+                std::copy(u.begin(), u.end(), soff.begin() + s * kSuMax);
+                sptr[s + 1] = sptr[s] + (int64_t)u.size() * kSlice;
+                for (int64_t r = s * kSlice; r < std::min(n, (s + 1) * kSlice); ++r)
+                    for (int32_t o : pats[pid[r]])
+                        smask[r] |= 1u << (std::lower_bound(u.begin(), u.end(), o) - u.begin());
+            }
+            su_ok = su_ok && sptr[ns] * 4 <= S.slots * 5 && sptr[ns] < (int64_t)INT32_MAX;
+            S.has_su = su_ok;
+            if (su_ok) {
+                S.su_slots = sptr[ns];
+                S.su_ptr.upload(sptr.data(), ns + 1, st);
+                S.su_off.upload(soff.data(), (int64_t)soff.size(), st);
+                S.su_nu.upload(snu.data(), ns, st);
+                S.su_mask.upload(smask.data(), n, st);
+                RSD_HIP(hipStreamSynchronize(st));  // the tables are locals
+            }
         }
     }
     S.slice_ptr.upload(S.h_slice_ptr.data(), S.n_slices + 1, st);

@@ -428,6 +428,14 @@ class Problem:
         check(lib().rsd_problem_layout_stencil(self.h, out))
         return tuple(int(v) for v in out)
 
+    def layout_union(self):
+        """(1, slots) when the GMRES products read a slice-union copy (each
+        64-row slice's union of column offsets in at most 32 positions), else
+        (0, 0)."""
+        out = (C.c_int64 * 2)()
+        check(lib().rsd_problem_layout_union(self.h, out))
+        return tuple(int(v) for v in out)
+
     def set_assembly(self, mode):
         """Matrix assembly scheme: 1 row gather (default), 0 colour-by-colour
         element scatter; returns 1 when the row gather is in use."""

@@ -911,6 +911,9 @@ def test_gmres_bitexact(ctx, case, precond):
     pb.assemble_matrix(J, forms, U=ctx.from_numpy(U))
     val = oc.assemble_matrix(forms, U=U)
     b = np.random.default_rng(5).standard_normal(oc.n_free)
+    # the products read the slice-union copy on all three meshes
+    has_su, su_slots = pb.layout_union()
+    assert has_su == 1 and su_slots >= pb.nnz
     for krylov in (5, 30):
         dx = pb.vector(0.0)
         rep = J.gmres(ctx.from_numpy(b), dx, tol=1e-10, max_iters=2000, krylov=krylov, precond=precond)
