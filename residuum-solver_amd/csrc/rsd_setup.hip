@@ -410,10 +410,22 @@ void rsd::build_sell(rsd_problem* pb, const std::vector<int32_t>& rowlen) {
             std::vector<uint32_t> smask(n, 0u);
             bool su_ok = true;
             std::vector<int32_t> u;
+            std::vector<uint8_t> seen(pats.size(), 0), ids;
+            std::vector<uint32_t> pm(pats.size(), 0u);
             for (int64_t s = 0; s < ns && su_ok; ++s) {
+                // the slice's distinct patterns (a few), then their offsets
+                const int64_t r1 = std::min(n, (s + 1) * kSlice);
+                ids.clear();
+                for (int64_t r = s * kSlice; r < r1; ++r)
+                    if (!seen[pid[r]]) {
+                        seen[pid[r]] = 1;
+                        ids.push_back(pid[r]);
+                    }
                 u.clear();
-                for (int64_t r = s * kSlice; r < std::min(n, (s + 1) * kSlice); ++r)
-                    u.insert(u.end(), pats[pid[r]].begin(), pats[pid[r]].end());
+                for (uint8_t q : ids) {
+                    seen[q] = 0;
+                    u.insert(u.end(), pats[q].begin(), pats[q].end());
+                }
                 std::sort(u.begin(), u.end());
                 u.erase(std::unique(u.begin(), u.end()), u.end());
                 su_ok = u.size() <= (size_t)kSuMax;
@@ -421,9 +433,11 @@ void rsd::build_sell(rsd_problem* pb, const std::vector<int32_t>& rowlen) {
                 snu[s] = (int32_t)u.size();
                 std::copy(u.begin(), u.end(), soff.begin() + s * kSuMax);
                 sptr[s + 1] = sptr[s] + (int64_t)u.size() * kSlice;
-                for (int64_t r = s * kSlice; r < std::min(n, (s + 1) * kSlice); ++r)
-                    for (int32_t o : pats[pid[r]])
-                        smask[r] |= 1u << (std::lower_bound(u.begin(), u.end(), o) - u.begin());
+                for (uint8_t q : ids) {
+                    pm[q] = 0u;
+                    for (int32_t o : pats[q]) pm[q] |= 1u << (std::lower_bound(u.begin(), u.end(), o) - u.begin());
+                }
+                for (int64_t r = s * kSlice; r < r1; ++r) smask[r] = pm[pid[r]];
             }
             su_ok = su_ok && sptr[ns] * 4 <= S.slots * 5 && sptr[ns] < (int64_t)INT32_MAX;
             S.has_su = su_ok;
