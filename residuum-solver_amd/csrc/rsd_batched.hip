@@ -247,6 +247,7 @@ struct BMixDev {
     const int32_t* t2local;
     const int64_t* first_t;      // [np + 1]
     const int64_t* pbase;        // [np] partial range: nb block sums, then 2 nbu
+    const int64_t* off32;        // [np] FP32 vector segment starts (zeros around each, see bm_setup)
 };
 
 __device__ __forceinline__ void locate_t(const BatchDev& B, const BMixDev& M, int64_t total_t, int& pid, int64_t& lb,
@@ -257,11 +258,11 @@ __device__ __forceinline__ void locate_t(const BatchDev& B, const BMixDev& M, in
     nbu = M.first_t[pid + 1] - M.first_t[pid];
 }
 
-__global__ void k_bm_inv_diag(BatchDev B, int64_t total_blocks, float* __restrict__ invd) {
+__global__ void k_bm_inv_diag(BatchDev B, BMixDev M, int64_t total_blocks, float* __restrict__ invd) {
     int pid; int64_t lb, nb;
     locate(B, total_blocks, pid, lb, nb);
     const int64_t r = lb * kBlock + threadIdx.x;
-    const int64_t g = B.voff[pid] + r;
+    const int64_t g = M.off32[pid] + r;
     if (r >= B.nrows[pid]) { invd[g] = 1.0f; return; }
     const int32_t s = B.diag[pid][r];
     const double d = s >= 0 ? B.val[pid][s] : 0.0;
@@ -332,15 +333,15 @@ __global__ __launch_bounds__(kBlock) void k_bm_init(BatchDev B, BMixDev M, int64
     locate_t(B, M, total_t, pid, lb, nbu);
     const BMState* bs = st + pid;
     if (bs->done) return;
-    const int64_t n = B.nrows[pid], off = B.voff[pid];
+    const int64_t n = B.nrows[pid], off = B.voff[pid], off32 = M.off32[pid];
     const double sc = bs->m.res > 0.0 ? 1.0 / bs->m.res : 0.0;
     double v[2] = {0.0, 0.0};
 #pragma unroll
     for (int t = 0; t < kMxTiles; ++t) {
         const int64_t rl = (lb * kMxTiles + t) * kBlock + threadIdx.x;
         if (rl < n) {
-            const int64_t i = off + rl;
-            const float ri = (float)(r64[i] * sc);
+            const int64_t i = off32 + rl;
+            const float ri = (float)(r64[off + rl] * sc);
             const float ui = invd ? ri * invd[i] : ri;
             u[i] = ui;
             p[i] = 0.0f;
@@ -363,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void k_bm_spmv_wu(BatchDev B, BMixDev M, in
     locate(B, total_blocks, pid, lb, nb);
     if (st[pid].m.stop) return;
     const int64_t rl = lb * kBlock + threadIdx.x;
-    const int64_t off = B.voff[pid];
+    const int64_t off = M.off32[pid];
     double v[1] = {0.0};
     if (rl < B.nrows[pid]) {
         const float wi = row_dot_f32<MODE>(rl, M.c32[pid], M.val32[pid], u + off);
@@ -423,7 +424,7 @@ __global__ __launch_bounds__(kBlock) void k_bm_update(BatchDev B, BMixDev M, int
     locate_t(B, M, total_t, pid, lb, nbu);
     const MixedState* ms = &st[pid].m;
     if (ms->stop) return;
-    const int64_t n = B.nrows[pid], off = B.voff[pid];
+    const int64_t n = B.nrows[pid], off = M.off32[pid];
     const float a = (float)ms->alpha, bt = (float)ms->beta;
     constexpr int T = kMxTiles;
     int64_t idx[T];
@@ -467,14 +468,13 @@ __global__ __launch_bounds__(kBlock) void k_bm_update(BatchDev B, BMixDev M, int
 }
 
 // x += res d for every problem whose inner solve ran this cycle
-__global__ void k_bm_update_x(BatchDev B, int64_t total_blocks, double* __restrict__ x, const float* __restrict__ d,
-                              const BMState* st) {
+__global__ void k_bm_update_x(BatchDev B, BMixDev M, int64_t total_blocks, double* __restrict__ x,
+                              const float* __restrict__ d, const BMState* st) {
     int pid; int64_t lb, nb;
     locate(B, total_blocks, pid, lb, nb);
     if (st[pid].done) return;
     const int64_t rl = lb * kBlock + threadIdx.x;
-    const int64_t i = B.voff[pid] + rl;
-    if (rl < B.nrows[pid]) x[i] += st[pid].m.res * (double)d[i];
+    if (rl < B.nrows[pid]) x[B.voff[pid] + rl] += st[pid].m.res * (double)d[M.off32[pid] + rl];
 }
 
 }  // namespace rsd
@@ -515,7 +515,8 @@ struct rsd_batch {
     DBuf<Cols> d_c32;
     DBuf<const float*> d_val32;
     DBuf<int32_t> t2prob[2], t2local[2];
-    DBuf<int64_t> first_t, pbase;
+    DBuf<int64_t> first_t, pbase, off32;
+    int64_t total32 = 0;  // FP32 vector length: the problems' segments and the zeros around them
     DBuf<float> mu, mw, mp, ms, md, minvd;
     DBuf<double> r64, mpart;
     DBuf<BMState> mst;
@@ -527,6 +528,7 @@ struct rsd_batch {
         M.t2local = t2local[sched.order].p;
         M.first_t = first_t.p;
         M.pbase = pbase.p;
+        M.off32 = off32.p;
         return M;
     }
     BatchDev dev() const {
@@ -654,17 +656,38 @@ static void bm_setup(rsd_batch* bt) {
     const int np = bt->np;
     if (bt->mode32 < 0) {
         std::vector<Cols> c32(np);
-        std::vector<int64_t> first{0}, pbase(np);
+        std::vector<int64_t> first{0}, pbase(np), off32(np);
         int64_t pb_acc = 0;
+        // every problem on the uniform stencil: its FP32 product (bitwise the
+        // per-pattern walk of the same storage); else symmetric half storage;
+        // else the FP32 copy in the sliced ELL layout walked by absolute
+        // columns (bitwise any column mode)
+        bool all_uni = true, all_half = true, any_half = false;
+        for (int i = 0; i < np; ++i) {
+            alloc_f32(ctx, bt->mats[i]);  // builds the half layout when the pattern allows it
+            const Sell& S = bt->mats[i]->pb->sell;
+            all_uni = all_uni && S.has_uni;
+            all_half = all_half && S.has_half;
+            any_half = any_half || S.has_half;
+        }
+        RSD_CHECK(all_half || !any_half, "batched mixed CG: the problems' FP32 layouts differ");
+        bt->mode32 = all_uni ? COL_UNI : all_half ? COL_HALF : COL_ABS;
+        // FP32 vectors: problem i's rows from off32[i], its 256-row blocks,
+        // and the zeros its uniform-stencil product reads before and after
+        // them (none for the other layouts), segment starts on 64 floats
+        int64_t acc = 0;
+        auto up64 = [](int64_t v) { return (v + 63) / 64 * 64; };
+        for (int i = 0; i < np; ++i) {
+            const Sell& S = bt->mats[i]->pb->sell;
+            const bool uni = bt->mode32 == COL_UNI;
+            acc += up64(uni ? S.uni_lo : 0);
+            off32[i] = acc;
+            acc += std::max<int64_t>(1, nblocks(S.n_rows)) * kBlock + up64(uni ? S.uni_hi : 0);
+        }
+        bt->total32 = acc;
         for (int i = 0; i < np; ++i) {
             rsd_matrix* A = bt->mats[i];
-            alloc_f32(ctx, A);  // builds the half layout when the pattern allows it
             const Sell& S = A->pb->sell;
-            // symmetric half storage, else the FP32 copy in the sliced ELL
-            // layout walked by absolute columns (bitwise any column mode)
-            const int m = S.has_half ? COL_HALF : COL_ABS;
-            RSD_CHECK(i == 0 || m == bt->mode32, "batched mixed CG: the problems' FP32 layouts differ");
-            bt->mode32 = m;
             c32[i] = S.cols32();
             const int64_t nb = std::max<int64_t>(1, nblocks(S.n_rows)), nbu = mixed_blocks(S.n_rows);
             first.push_back(first.back() + nbu);
@@ -691,12 +714,14 @@ static void bm_setup(rsd_batch* bt) {
         }
         bt->first_t.upload(first.data(), np + 1, st);
         bt->pbase.upload(pbase.data(), np, st);
-        const int64_t n = bt->total_rows;
+        bt->off32.upload(off32.data(), np, st);
+        RSD_HIP(hipStreamSynchronize(st));  // the tables are locals
+        const int64_t n = bt->total32;
         for (DBuf<float>* v : {&bt->mu, &bt->mw, &bt->mp, &bt->ms, &bt->md, &bt->minvd}) {
             v->alloc(n);
             launch_fill_f32(ctx, n, v->p, 0.0f);
         }
-        bt->r64.alloc(n);
+        bt->r64.alloc(bt->total_rows);
         bt->mpart.alloc(pb_acc);
         bt->mst.alloc(np);
     }
@@ -722,7 +747,8 @@ static void bm_setup(rsd_batch* bt) {
 #define RSD_BM_SPMV32(bt, grid, ...)                                                                               \
     do {                                                                                                           \
         hipStream_t s_ = (bt)->ctx->stream;                                                                        \
-        if ((bt)->mode32 == COL_HALF) hipLaunchKernelGGL((k_bm_spmv_wu<COL_HALF>), grid, dim3(kBlock), 0, s_, __VA_ARGS__); \
+        if ((bt)->mode32 == COL_UNI) hipLaunchKernelGGL((k_bm_spmv_wu<COL_UNI>), grid, dim3(kBlock), 0, s_, __VA_ARGS__); \
+        else if ((bt)->mode32 == COL_HALF) hipLaunchKernelGGL((k_bm_spmv_wu<COL_HALF>), grid, dim3(kBlock), 0, s_, __VA_ARGS__); \
         else hipLaunchKernelGGL((k_bm_spmv_wu<COL_ABS>), grid, dim3(kBlock), 0, s_, __VA_ARGS__);                \
     } while (0)
 
@@ -747,7 +773,7 @@ void batched_mixed(rsd_batch* bt, const double* b, double* x, double tol, int64_
     const int np = bt->np;
     const int64_t nb = bt->total_blocks;
     if (precond == RSD_PC_JACOBI)
-        hipLaunchKernelGGL(k_bm_inv_diag, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, bt->minvd.p);
+        hipLaunchKernelGGL(k_bm_inv_diag, dim3((unsigned)nb), dim3(kBlock), 0, st, B, M, nb, bt->minvd.p);
     const float* invd = precond == RSD_PC_JACOBI ? bt->minvd.p : nullptr;
     BMState* h = nullptr;
     RSD_HIP(hipHostMalloc((void**)&h, sizeof(BMState) * (size_t)np, hipHostMallocDefault));
@@ -796,7 +822,7 @@ void batched_mixed(rsd_batch* bt, const double* b, double* x, double tol, int64_
                 if (stopped) break;
                 RSD_CHECK(launched <= max_iters + 2 * bt->sched.chunk, "batched mixed CG: inner solve did not stop");
             }
-            hipLaunchKernelGGL(k_bm_update_x, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, x, bt->md.p, bt->mst.p);
+            hipLaunchKernelGGL(k_bm_update_x, dim3((unsigned)nb), dim3(kBlock), 0, st, B, M, nb, x, bt->md.p, bt->mst.p);
             RSD_HIP(hipGetLastError());
         }
         out.assign(h, h + np);
