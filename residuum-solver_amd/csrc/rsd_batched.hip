@@ -55,6 +55,9 @@ struct BatchDev {
     const int64_t* first_block;       // [np + 1]
     const int64_t* nrows;             // [np]
     const int64_t* voff;              // [np + 1] vector segment offsets
+    const double* const* val_st;      // per problem, its FP64 stencil copy (when every problem has one)
+    const uint32_t* const* st_mask;
+    const int32_t* st_coff;           // [np * kStRuns] run centre offsets
     int32_t xcd;                      // remap launched blocks XCD by XCD
     __device__ __forceinline__ Cols cols(int p) const {
         Cols c;
@@ -134,6 +137,33 @@ __global__ __launch_bounds__(kFold) void k_b_init_fold(BatchDev B, const double*
     s->stop = 0;
     if (res0 / (res0 + 1e-50) < tol) { s->converged = 1; s->stop = 1; }
     else if (max_iters <= 1) s->stop = 1;
+}
+
+constexpr int kStRuns = kStMaxPos / 3;
+
+// k_b_spmv_pap over the problems' stencil copies (k_cg_spmv_pap_st per
+// problem; every lane of a block runs the row product)
+template<int NR>
+__global__ __launch_bounds__(kBlock) void k_b_spmv_pap_st(BatchDev B, int64_t total_blocks,
+                                                          const double* __restrict__ p, double* __restrict__ Ap,
+                                                          const BState* st, double* partials) {
+    int pid; int64_t lb, nb;
+    locate(B, total_blocks, pid, lb, nb);
+    if (st[pid].stop) return;
+    const int64_t rl = lb * kBlock + threadIdx.x;
+    const int64_t off = B.voff[pid], n = B.nrows[pid];
+    StCols c;
+    c.mask = B.st_mask[pid];
+    c.np = 3 * NR;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) c.coff[k] = B.st_coff[pid * kStRuns + k];
+    const double y = row_dot_st<NR>(rl, n, c, B.val_st[pid], p + off);
+    double v[1] = {0.0};
+    if (rl < n) {
+        Ap[off + rl] = y;
+        v[0] = p[off + rl] * y;
+    }
+    block_partials<1>(v, lb, nb, partials + 2 * B.first_block[pid]);
 }
 
 template<int MODE, int U>
@@ -503,6 +533,13 @@ struct rsd_batch {
     DBuf<const uint8_t*> d_pid;
     DBuf<int32_t> d_stride;
     DBuf<const double*> d_val;
+    // the problems' FP64 stencil copies, when every problem has one of the
+    // same size (st_nr runs of three: 9 hex, 3 quad; 0: the sliced products)
+    int32_t st_nr = 0;
+    std::vector<const double*> h_val_st;
+    DBuf<const double*> d_val_st;
+    DBuf<const uint32_t*> d_st_mask;
+    DBuf<int32_t> d_st_coff;
     DBuf<int32_t> blk2prob[2], blk2local[2];  // per block order
     DBuf<int64_t> first_block, nrows, voff;
     DBuf<double> r, p, Ap, invd;
@@ -547,6 +584,9 @@ struct rsd_batch {
         B.first_block = first_block.p;
         B.nrows = nrows.p;
         B.voff = voff.p;
+        B.val_st = d_val_st.p;
+        B.st_mask = d_st_mask.p;
+        B.st_coff = d_st_coff.p;
         B.xcd = sched.xcd;
         return B;
     }
@@ -574,7 +614,14 @@ static void b_launch_iter(rsd_batch* bt, double* x, double tol, int64_t max_iter
     const int64_t nb = bt->total_blocks;
     const double* invd = precond == RSD_PC_JACOBI ? bt->invd.p : nullptr;
     const unsigned np = (unsigned)bt->np;
-    RSD_B_SPMV(k_b_spmv_pap, bt, B, nb, bt->p.p, bt->Ap.p, bt->st.p, bt->partials.p);
+    if (bt->st_nr == 9)
+        hipLaunchKernelGGL(k_b_spmv_pap_st<9>, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, bt->p.p, bt->Ap.p,
+                           bt->st.p, bt->partials.p);
+    else if (bt->st_nr == 3)
+        hipLaunchKernelGGL(k_b_spmv_pap_st<3>, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, bt->p.p, bt->Ap.p,
+                           bt->st.p, bt->partials.p);
+    else
+        RSD_B_SPMV(k_b_spmv_pap, bt, B, nb, bt->p.p, bt->Ap.p, bt->st.p, bt->partials.p);
     hipLaunchKernelGGL(k_b_alpha, dim3(np), dim3(kFold), 0, st, B, bt->partials.p, bt->st.p);
     hipLaunchKernelGGL(k_b_update_r, dim3((unsigned)nb), dim3(kBlock), 0, st, B, nb, x, bt->p.p, bt->Ap.p, bt->r.p,
                        invd, bt->st.p, bt->partials.p);
@@ -586,6 +633,18 @@ static void b_launch_iter(rsd_batch* bt, double* x, double tol, int64_t max_iter
 static void b_launch_init(rsd_batch* bt, const double* b, const double* x, double tol, int64_t max_iters,
                           int32_t precond) {
     hipStream_t st = bt->ctx->stream;
+    if (bt->st_nr) {
+        // refresh every problem's stencil copy from its values; the copies
+        // are allocated once, so their addresses change only the first time
+        bool moved = false;
+        for (int i = 0; i < bt->np; ++i) {
+            reserve_stencil(bt->ctx, bt->mats[i]);
+            launch_to_stencil(bt->ctx, bt->mats[i]);
+            moved = moved || bt->h_val_st[i] != bt->mats[i]->val_st.p;
+            bt->h_val_st[i] = bt->mats[i]->val_st.p;
+        }
+        if (moved) bt->d_val_st.upload(bt->h_val_st.data(), bt->np, st);
+    }
     const BatchDev B = bt->dev();
     const int64_t nb = bt->total_blocks;
     if (precond == RSD_PC_JACOBI)
@@ -975,6 +1034,29 @@ int rsd_batch_create(rsd_ctx* ctx, int32_t n_problems, rsd_matrix* const* mats, 
         bt->total_blocks = bt->h_first[n_problems];
         bt->total_rows = bt->h_voff[n_problems];
         bt->mode = all_pat ? COL_PAT : COL_ABS;
+        // every problem with a stencil copy of the same size: the CG products
+        // read those (bitwise the sliced products), refreshed by each solve
+        bool all_st = true;
+        for (int i = 0; i < n_problems; ++i) {
+            const Sell& S = mats[i]->pb->sell;
+            all_st = all_st && S.has_st && S.n_rows > 0 && S.st_np == mats[0]->pb->sell.st_np;
+        }
+        if (all_st) {
+            hipStream_t st = ctx->stream;
+            bt->st_nr = mats[0]->pb->sell.st_np / 3;
+            std::vector<const uint32_t*> smask(n_problems);
+            std::vector<int32_t> scoff((size_t)n_problems * kStRuns, 0);
+            for (int i = 0; i < n_problems; ++i) {
+                const StCols c = mats[i]->pb->sell.st_cols();
+                smask[i] = c.mask;
+                for (int k = 0; k < bt->st_nr; ++k) scoff[(size_t)i * kStRuns + k] = c.coff[k];
+            }
+            bt->d_st_mask.upload(smask.data(), n_problems, st);
+            bt->d_st_coff.upload(scoff.data(), (int64_t)scoff.size(), st);
+            bt->h_val_st.assign(n_problems, nullptr);
+            bt->d_val_st.upload(bt->h_val_st.data(), n_problems, st);
+            RSD_HIP(hipStreamSynchronize(st));  // the tables are locals
+        }
         // block orders: problem by problem, and interleaved (block k of every
         // problem that has one, then block k + 1, ...)
         std::vector<int32_t> b2p[2], b2l[2];
