@@ -474,6 +474,7 @@ __global__ void k_gm_reset_flags(GMState* st) {
 GMRESResult gmres_run(rsd_ctx* ctx, const rsd_matrix* A, const double* b, double* x, double tol, int64_t max_iters,
                       int32_t m, int32_t precond, bool abs_tol) {
     auto crit = [&](double res, double res0) { return abs_tol ? res : res / (res0 + 1e-50); };
+    RSD_CHECK(m >= 1 && m <= kGivensMax, "GMRES: Krylov dimension outside [1, 512]");  // the rotations' LDS copy
     const Sell& S = A->pb->sell;
     const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n)), nbe = nblocks(std::max<int64_t>(n, 1));
     hipStream_t st = ctx->stream;
@@ -733,6 +734,7 @@ __global__ void k_gm32_update_x(int64_t n, double* __restrict__ x, const float* 
 
 MixedGMResult gmres_mixed_run(rsd_ctx* ctx, rsd_matrix* A, const double* b, double* x, double tol,
                               int64_t max_iters, int32_t m, int32_t precond, double inner_floor) {
+    RSD_CHECK(m >= 1 && m <= kGivensMax, "GMRES: Krylov dimension outside [1, 512]");  // the rotations' LDS copy
     const Sell& S = A->pb->sell;
     const int64_t n = S.n_rows, nb = std::max<int64_t>(1, nblocks(n)), nbe = nblocks(std::max<int64_t>(n, 1));
     hipStream_t st = ctx->stream;
