@@ -1109,6 +1109,35 @@ def permuted_problem(ctx, seed):
     return pb
 
 
+@pytest.mark.parametrize("precond", [0, 1])
+def test_gmres_sliced_fallback_bitexact(ctx, precond):
+    """GMRES on a renumbered mesh, whose rows share no offset patterns, so the
+    products have no slice-union copy and walk the sliced layout's explicit
+    columns: iterations, final residual and solution bitwise the oracle's
+    GMRES in the GPU order on the same CSR matrix."""
+    import ctypes as C
+    pb = permuted_problem(ctx, 7)
+    assert pb.layout_union() == (0, 0)
+    forms = [(O.DIFFUSION, 1.0), (O.MASS | O.SCALED, 1.0 / 1e-3)]
+    K = pb.matrix()
+    pb.assemble_matrix(K, forms)
+    rp, ci = pb.pattern()
+    val = K.values()
+    n = pb.n_free
+    b = np.random.default_rng(8).standard_normal(n)
+    for krylov in (5, 30):
+        dx = pb.vector(0.0)
+        rep = K.gmres(ctx.from_numpy(b), dx, tol=1e-10, max_iters=2000, krylov=krylov, precond=precond)
+        xo = np.zeros(n)
+        ro = O.OrcReport()
+        O.lib().orc_gmres(C.c_int64(n), O.lp(rp), O.lp(ci), O.dp(val), O.dp(b), O.dp(xo), C.c_double(1e-10),
+                          C.c_int64(2000), C.c_int64(krylov), precond, O.ORDER_GPU, C.byref(ro))
+        say("gmres fallback", krylov, rep.iterations, ro.iterations)
+        assert rep.converged == ro.converged == 1 and rep.iterations == ro.iterations
+        assert rep.final_residual == ro.final_residual
+        assert same(dx.numpy(), xo)
+
+
 def symmetric_upper(A):
     """The operator of the FP32 half storage: every entry below the diagonal
     replaced by its mirror above it."""
