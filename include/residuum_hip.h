@@ -597,7 +597,10 @@ int rsd_adjoint_grad_logk(rsd_ctx* ctx, rsd_problem* pb, const double* d_kappa, 
  * rsd_batch_layout (each segment padded to a multiple of 256 rows; padding
  * entries are ignored).  Each problem follows cg::Solver::solve
  * (cg/Solver.tpp:15-119) and its result is bitwise identical to
- * rsd_cg_solve_f64 on that problem alone.  reps: host [n_problems]. */
+ * rsd_cg_solve_f64 on that problem alone.  When every problem has an FP64
+ * stencil copy of the same size (rsd_problem_layout_stencil), the products
+ * read those copies, which each call refreshes from the values.  reps: host
+ * [n_problems]. */
 int rsd_batch_create(rsd_ctx* ctx, int32_t n_problems, rsd_matrix* const* mats, rsd_batch** out);
 int rsd_batch_destroy(rsd_batch* b);
 int rsd_batch_layout(rsd_batch* b, int64_t* total_rows, int64_t* offsets);
@@ -608,9 +611,12 @@ int rsd_cg_solve_batched(rsd_ctx* ctx, rsd_batch* b, const double* rhs, double* 
  * from each matrix's values by every call), FP64 outer residual -- in one
  * set of launches; each problem's result, report and outer cycle count
  * (outer_cycles: host [n_problems], nullable) are bitwise those of
- * rsd_cg_solve_mixed on that problem alone.  The problems' FP32 layouts
- * (rsd_problem_layout_f32) must agree; RSD_ERR_ARG names a problem whose
- * matrix fails the FP32 copy's symmetry check.  The batch's schedule
+ * rsd_cg_solve_mixed on that problem alone.  Either every problem's FP32
+ * copy has the symmetric half storage (rsd_problem_layout_f32), or none has
+ * (RSD_ERR_ARG otherwise); when every problem is on a uniform stencil, the
+ * products are the uniform-stencil kernel of the standalone solve.
+ * RSD_ERR_ARG names a problem whose matrix fails the FP32 copy's symmetry
+ * check.  The batch's schedule
  * applies (chunk: inner iterations per graph launch). */
 int rsd_cg_solve_batched_mixed(rsd_ctx* ctx, rsd_batch* b, const double* rhs, double* x, double tol,
                                int64_t max_iters, int32_t precond, double inner_floor, rsd_report* reps,
